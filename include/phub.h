@@ -1,0 +1,223 @@
+/*
+ * phub.h -- C ABI of the MI355X exact-diagonalization engine (libphub.so).
+ *
+ * Replaces the process + HDF5-file boundary of PyHub's hot path: the
+ * reference spawns Fortran executables with os.system (pyhub/core/core_solver.py:107-111,
+ * pyhub/core/basis.py:167-171) which read and write solver.h5 / basis.h5.  Here the
+ * same stages are in-process calls on device buffers.
+ *
+ * Conventions (identical to the reference):
+ *   - site J (1-based) is bit n-J of a mask (pyhub/libs/src/funcmod.f90:22-23);
+ *   - per-species basis order is HILBERT's (pyhub/libs/src/compute_basis.f90:67-96),
+ *     rank(x) = C(n,k) - 1 - colex(rev_n(x));
+ *   - Hubbard state index d*dim_up + u, u fastest (pyhub/libs/src/hprod_hubbard.f90:31);
+ *   - T is row-major T[(J-1)*n + (K-1)] = T(J,K): hopping c^dag_K c_J has amplitude
+ *     T(J,K), the diagonal T(J,J) is the on-site energy eps_J.
+ *
+ * Memory: every pointer documented "device" is a device pointer on the
+ * context's device; "host" pointers are read/written synchronously.  Calls are
+ * ordered on the context's stream.  No call allocates device memory except the
+ * *_create entry points and phub_lanczos (which allocates its Krylov basis for
+ * the duration of the call).
+ *
+ * Errors: every entry returns PHUB_OK (0) or a PHUB_ERR_* code; the message is
+ * available from phub_last_error(ctx).
+ */
+#ifndef PHUB_H
+#define PHUB_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PHUB_OK = 0,
+    PHUB_ERR_ARG = 1,          /* invalid argument / shape */
+    PHUB_ERR_UNSUPPORTED = 2,  /* size or feature outside what is built */
+    PHUB_ERR_OOM = 3,          /* device allocation failed */
+    PHUB_ERR_NOCONV = 4,       /* Lanczos hit max steps (reference: "MAX LCZ STEP REACH") */
+    PHUB_ERR_BREAKDOWN = 5,    /* invariant subspace found before convergence was checked */
+    PHUB_ERR_HIP = 6,          /* HIP runtime error */
+    PHUB_ERR_NORM_LOST = 7     /* reconstructed ground vector norm drifted (reference: "NORM LOST") */
+};
+
+typedef struct phub_ctx phub_ctx;
+typedef struct phub_op phub_op;
+
+/* ---- context --------------------------------------------------------------
+ * Replaces the per-call executable spawn (core_solver.py:107-111).
+ * stream: a hipStream_t (NULL = default stream). */
+int phub_ctx_create(int device, void *stream, phub_ctx **out);
+void phub_ctx_destroy(phub_ctx *ctx);
+const char *phub_last_error(const phub_ctx *ctx);
+int phub_sync(phub_ctx *ctx);
+/* timing of the most recent H*v launch on the context (HIP events), ms */
+double phub_last_kernel_ms(const phub_ctx *ctx);
+
+/* ---- basis ----------------------------------------------------------------
+ * Replaces basis.x / HILBERT (compute_basis.f90:67-96) and basis.h5
+ * (basis.py:65-83,214-228).  Masks in the reference order, built on the device. */
+uint64_t phub_binom(int n, int k);
+int phub_basis_build(phub_ctx *ctx, int n, int k, uint64_t *d_masks /* device, C(n,k) */);
+/* rank of each mask (reference order); masks need popcount k */
+int phub_basis_rank(phub_ctx *ctx, int n, int k, const uint64_t *d_masks, int64_t count,
+                    int64_t *d_ranks /* device */);
+
+/* ---- operators --------------------------------------------------------------
+ * Hubbard: the operator of HPROD (hprod_hubbard.f90:17-118) with per-site U
+ * (is_Uloc = 2, core_solver.py:133-141).  t: host n*n row-major; u: host n. */
+typedef struct {
+    int n_sites, n_up, n_dn;
+    const double *t;
+    const double *u;
+} phub_hubbard_desc;
+
+/* Heisenberg S_z = 0 sector (hprod_heis.f90:1-61): H = sum over ORDERED pairs
+ * J(J,K) S_J.S_K.  sign_convention 0 reproduces the reference's spin-flip
+ * element -(-1)^(J+K) J(J,K) exactly; 1 uses +(-1)^(J+K) J(J,K), the element of
+ * the physical spin model in the same fermion basis (the sign the reference's
+ * J_SPIN measures, static_rq_heis.f90:94-102).  The two agree up to a basis gauge
+ * on bipartite bond graphs only. */
+typedef struct {
+    int n_sites;
+    const double *j;
+    int sign_convention;
+} phub_heis_desc;
+
+typedef struct {
+    int kind;            /* 0 = Hubbard, 1 = Heisenberg */
+    int n_sites, n_up, n_dn;
+    int64_t dim_up, dim_dn, dim;
+    int max_hops_up, max_hops_dn;
+    double table_bytes;  /* device bytes of the hop / rank tables */
+} phub_op_info;
+
+int phub_op_hubbard(phub_ctx *ctx, const phub_hubbard_desc *desc, phub_op **out);
+int phub_op_heisenberg(phub_ctx *ctx, const phub_heis_desc *desc, phub_op **out);
+void phub_op_destroy(phub_op *op);
+int phub_op_get_info(const phub_op *op, phub_op_info *info);
+/* masks of the per-species bases held by the operator (device, dim_up / dim_dn) */
+int phub_op_masks(phub_ctx *ctx, const phub_op *op, uint64_t *d_up, uint64_t *d_dn);
+
+/* ---- H*v ---------------------------------------------------------------------
+ * w = H v, replaces HPROD (hprod_hubbard.f90:17-118, hprod_heis.f90:1-61).
+ * v, w: device, op dim; w must not alias v. */
+int phub_hv_f64(phub_ctx *ctx, const phub_op *op, const double *v, double *w);
+int phub_hv_f32(phub_ctx *ctx, const phub_op *op, const float *v, float *w);
+
+/* ---- Lanczos ground state ----------------------------------------------------
+ * Replaces LANCZOS (lanczos.f90:1-163) + SOLVE (solve.f90:1-23) +
+ * READ_PSI_HUBBARD (compute_solver.f90:183-290). */
+enum {
+    PHUB_LCZ_PLAIN64 = 0,   /* FP64 vectors, 3-term recurrence, no reorthogonalisation
+                               (the reference algorithm), psi by a second pass */
+    PHUB_LCZ_FULL64 = 1,    /* FP64 vectors, full reorthogonalisation, stored basis */
+    PHUB_LCZ_MIXED = 2      /* FP32 vectors, FP64 scalars, full reorthogonalisation,
+                               FP64 psi reconstruction + FP64 refinement steps */
+};
+typedef struct {
+    int mode;               /* PHUB_LCZ_* */
+    int max_steps;          /* reference max_lcz */
+    double tol;             /* convergence of the lowest Ritz value (reference acc_lcz) */
+    int check_every;        /* reference NADD = 5 */
+    int fixed_steps;        /* >0: run exactly this many steps, no convergence test */
+    int refine_steps;       /* FP64 residual-correction steps after MIXED (0..4) */
+    uint64_t seed;          /* start vector when d_v0 is NULL */
+    int want_psi;           /* reconstruct the ground vector into d_psi */
+} phub_lanczos_opts;
+
+typedef struct {
+    double e0;              /* final ground energy (after refinement when MIXED) */
+    double e0_ritz;         /* lowest Ritz value of the tridiagonal */
+    double residual;        /* ||H psi - e0 psi|| (FP64), when psi was built */
+    int steps;              /* Lanczos steps taken */
+    int converged;
+    int ndeg;               /* reference NDEG counter */
+    double seconds;         /* device wall time of the solve */
+    double seconds_lanczos; /* of which: Krylov steps */
+    double hv_ms_avg;       /* average H*v kernel time */
+} phub_lanczos_result;
+
+int phub_lanczos(phub_ctx *ctx, const phub_op *op, const phub_lanczos_opts *opts,
+                 const double *d_v0 /* device start vector or NULL */,
+                 double *alpha /* host, max_steps */, double *beta /* host, max_steps+1 */,
+                 phub_lanczos_result *res, double *d_psi /* device, dim, or NULL */);
+
+/* ---- reduced quantities ------------------------------------------------------
+ * Hubbard: ONE_RDM, TWO_RDM (NI), J_SPIN (static_rq.f90:153-395) for <bra| . |ket>.
+ * Outputs host, n*n row-major (rdm_up[(J-1)*n+(K-1)] = <c^dag_J c_K>), ni n,
+ * sisj n*n.  Any output may be NULL. */
+int phub_rdm_hubbard(phub_ctx *ctx, const phub_op *op, const double *d_bra, const double *d_ket,
+                     double *rdm_up, double *rdm_dn, double *ni, double *sisj);
+/* Heisenberg J_SPIN (static_rq_heis.f90:58-116) */
+int phub_rdm_heisenberg(phub_ctx *ctx, const phub_op *op, const double *d_bra,
+                        const double *d_ket, double *sisj);
+
+/* ---- Green's functions -------------------------------------------------------
+ * Excitation c^dag_{site,spin} psi (create=1) or c_{site,spin} psi (create=0) from
+ * src's sector into dst's sector (GF_E_* / GF_H_*, spgf_ed.f90:304-429; sign
+ * ANTICOM(site, x_spin) times (-1)^N_dn for up operators, :321,:386).
+ * site is 1-based; spin 0 = up, 1 = down. */
+int phub_excite(phub_ctx *ctx, const phub_op *src, const phub_op *dst, int site, int spin,
+                int create, const double *d_psi, double *d_out);
+/* Continued-fraction chain from d_start (single-vector case of GF_BANDLANCZOS,
+ * spgf_bandlanczos.f90:309-448): alpha[nsteps], beta[nsteps], norm2 = <start|start>.
+ * precision 64 or 32 (FP32 vectors, FP64 recurrence scalars). */
+int phub_cf_chain(phub_ctx *ctx, const phub_op *op, const double *d_start, int nsteps,
+                  int precision, double *alpha, double *beta, double *norm2);
+
+/* ---- batched clusters --------------------------------------------------------
+ * Many independent Hubbard clusters in one persistent launch: each workgroup
+ * runs whole Lanczos chains of one work item (ground state, then G_ii chains)
+ * with FP32 vectors and FP64 recurrence scalars.  See phub_batch.h-style
+ * descriptors below. */
+typedef struct {
+    int n_sites, n_up, n_dn;
+    const double *t;   /* host n*n */
+    const double *u;   /* host n */
+} phub_cluster;
+
+typedef struct {
+    int gs_steps;        /* Lanczos steps for the ground state */
+    int gf_steps;        /* continued-fraction steps per G_ii chain */
+    uint64_t seed;
+} phub_batch_opts;
+
+/* e0[n_clusters]; for cluster c, site i (0-based), spin s, kind k (0 particle
+ * c^dag psi0, 1 hole c psi0): chain slot q = ((c*32 + i)*2 + s)*2 + k holds
+ * gf_alpha[q*gf_steps..], gf_beta[q*gf_steps..], gf_norm2[q].  Chains that do
+ * not exist (sector empty) have norm2 = 0.  All outputs host. */
+int phub_batch_hubbard(phub_ctx *ctx, const phub_cluster *clusters, int n_clusters,
+                       const phub_batch_opts *opts, double *e0, double *gf_alpha,
+                       double *gf_beta, double *gf_norm2);
+
+/* Evaluate G(omega) = norm2 / (z - a0 - b0^2/(z - a1 - ...)) on a grid, on the
+ * device, for n_chains chains: z = sign*(omega + i eta) + shift[c].
+ * out: host complex128 interleaved, n_chains * n_omega. */
+int phub_cf_eval(phub_ctx *ctx, int n_chains, int nsteps, const double *alpha,
+                 const double *beta, const double *norm2, const double *shift,
+                 const double *sign, int n_omega, const double *omega, double eta,
+                 double *out);
+
+/* ---- gradients ---------------------------------------------------------------
+ * Hellmann-Feynman: dE0/dT(J,K), dE0/dU_J for the Hubbard operator in one pass
+ * over psi (no reference counterpart; pinned by static_rq.f90 contractions).
+ * d_t: host n*n, d_u: host n. */
+int phub_hf_grad_hubbard(phub_ctx *ctx, const phub_op *op, const double *d_psi,
+                         double *grad_t, double *grad_u);
+/* dE0/dJ(J,K) for the Heisenberg operator (ordered pairs). */
+int phub_hf_grad_heisenberg(phub_ctx *ctx, const phub_op *op, const double *d_psi,
+                            double *grad_j);
+/* Adjoint solve (H - e0) x = b with b orthogonal to psi, by MINRES restricted to
+ * the complement of psi (FP64).  Returns iterations in *iters. */
+int phub_adjoint_minres(phub_ctx *ctx, const phub_op *op, const double *d_psi, double e0,
+                        const double *d_b, double *d_x, int max_iter, double tol,
+                        int *iters, double *rel_res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHUB_H */

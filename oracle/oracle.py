@@ -1,0 +1,395 @@
+"""CPU oracle for the exact-diagonalization hot path (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this module, and only as the checker or the timed CPU baseline; the product
+package (pyhub_amd) never imports it.
+
+Contents
+- ctypes bindings to liboracle.so (ed_oracle.c: a C restatement of the
+  reference's Fortran, every function citing the file:line it follows);
+- ctypes bindings to the reference's own HPROD compiled from its Fortran
+  sources into oracle/_ref (when that build exists);
+- numpy dense ED (the DIAGMAT/EXACT_DIAG pattern, exact_diag.f90:2-33),
+  closed forms and a finite-ring Lieb-Wu Bethe-ansatz solver.
+
+Parity status of each function is listed in ed_oracle.c's header; the numpy
+helpers here are pinned by the closed forms in tests/test_oracle.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+_ref_hub = None
+_ref_heis = None
+
+u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
+        L = C.CDLL(path)
+        L.orc_binom.restype = C.c_uint64
+        L.orc_binom.argtypes = [C.c_int, C.c_int]
+        L.orc_hilbert.argtypes = [C.c_int, C.c_int, C.c_longlong, u64p]
+        L.orc_rank_ref.restype = C.c_longlong
+        L.orc_rank_ref.argtypes = [C.c_int, C.c_int, C.c_uint64]
+        hub_args = [C.c_int, C.c_int, C.c_int, C.c_longlong, C.c_longlong, u64p, u64p,
+                    f64p, f64p, f64p, f64p]
+        L.orc_hprod_hubbard.argtypes = hub_args
+        L.orc_hprod_hubbard_fast.argtypes = hub_args + [C.c_int]
+        L.orc_hprod_heis.argtypes = [C.c_int, C.c_longlong, u64p, f64p, f64p, f64p]
+        L.orc_tridiag_eig.restype = C.c_int
+        L.orc_tridiag_eig.argtypes = [C.c_int, f64p, f64p, C.c_void_p]
+        L.orc_lanczos.restype = C.c_int
+        L.orc_lanczos.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_longlong, C.c_longlong,
+                                  u64p, u64p, f64p, f64p, f64p, C.c_int, C.c_int,
+                                  C.c_int, C.c_double, f64p, f64p, f64p, f64p, f64p,
+                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_one_rdm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_longlong, C.c_longlong,
+                                  u64p, u64p, f64p, f64p, f64p]
+        L.orc_ni.argtypes = [C.c_int, C.c_longlong, C.c_longlong, u64p, u64p, f64p, f64p, f64p]
+        L.orc_j_spin.argtypes = [C.c_int, C.c_longlong, C.c_longlong, u64p, u64p, f64p, f64p, f64p]
+        L.orc_j_spin_heis.argtypes = [C.c_int, C.c_longlong, u64p, f64p, f64p, f64p]
+        L.orc_excite.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.c_longlong, C.c_longlong, u64p, u64p,
+                                 C.c_longlong, C.c_longlong, u64p, u64p, f64p, f64p]
+        L.orc_cf_chain.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_longlong, C.c_longlong,
+                                   u64p, u64p, f64p, f64p, f64p, C.c_int, C.c_int, f64p, C.c_int,
+                                   f64p, f64p, C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return (os.path.exists(os.path.join(_HERE, "_ref", "libref_hub.so"))
+            and os.path.exists(os.path.join(_HERE, "_ref", "libref_heis.so")))
+
+
+def _ref_libs():
+    global _ref_hub, _ref_heis
+    if _ref_hub is None:
+        _ref_hub = C.CDLL(os.path.join(_HERE, "_ref", "libref_hub.so"))
+        _ref_hub.phref_hub_setup.argtypes = [C.c_int] * 5 + [i32p, i32p, f64p, f64p]
+        _ref_hub.phref_hub_hprod.argtypes = [f64p, f64p]
+        _ref_heis = C.CDLL(os.path.join(_HERE, "_ref", "libref_heis.so"))
+        _ref_heis.phref_heis_setup.argtypes = [C.c_int, C.c_int, i32p, f64p]
+        _ref_heis.phref_heis_hprod.argtypes = [f64p, f64p]
+    return _ref_hub, _ref_heis
+
+
+# ---------------------------------------------------------------------------
+# basis
+# ---------------------------------------------------------------------------
+def binom(n: int, k: int) -> int:
+    return math.comb(n, k) if 0 <= k <= n else 0
+
+
+def hilbert(n: int, k: int) -> np.ndarray:
+    """Reference basis order (restates HILBERT, compute_basis.f90:67-96)."""
+    ns = binom(n, k)
+    out = np.zeros(ns, dtype=np.uint64)
+    lib().orc_hilbert(n, k, ns, out)
+    return out
+
+
+def rank_ref(n: int, k: int, x: int) -> int:
+    return int(lib().orc_rank_ref(n, k, int(x)))
+
+
+def brute_basis(n: int, k: int) -> np.ndarray:
+    """All n-bit masks with k set bits, sorted lexicographically by their
+    increasing list of set-bit positions (bit 0 first).  This is the order
+    HILBERT produces (compute_basis.f90:67-96), written as a brute-force sort."""
+    masks = [x for x in range(1 << n) if bin(x).count("1") == k]
+    masks.sort(key=lambda x: tuple(b for b in range(n) if x >> b & 1))
+    return np.array(masks, dtype=np.uint64)
+
+
+# ---------------------------------------------------------------------------
+# models
+# ---------------------------------------------------------------------------
+class HubbardModel:
+    """Hubbard cluster in the reference conventions: T[J-1][K-1] = T(J,K)
+    (hopping -t on bonds, eps_i on the diagonal), U per site."""
+
+    def __init__(self, n, nup, ndn, T, U):
+        self.n, self.nup, self.ndn = int(n), int(nup), int(ndn)
+        self.T = np.ascontiguousarray(T, dtype=np.float64)
+        U = np.asarray(U, dtype=np.float64)
+        self.U = np.ascontiguousarray(np.full(n, float(U)) if U.ndim == 0 else U)
+        self.bup = hilbert(n, nup)
+        self.bdn = hilbert(n, ndn)
+        self.nsup, self.nsdn = len(self.bup), len(self.bdn)
+        self.dim = self.nsup * self.nsdn
+
+    def hv(self, v, fast=False, nthreads=1):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        w = np.empty_like(v)
+        if fast:
+            lib().orc_hprod_hubbard_fast(self.n, self.nup, self.ndn, self.nsup, self.nsdn,
+                                         self.bup, self.bdn, self.T, self.U, v, w, nthreads)
+        else:
+            lib().orc_hprod_hubbard(self.n, self.nup, self.ndn, self.nsup, self.nsdn,
+                                    self.bup, self.bdn, self.T, self.U, v, w)
+        return w
+
+    def hv_ref(self, v):
+        """The reference's own HPROD (Fortran, compiled from its sources)."""
+        hub, _ = _ref_libs()
+        hub.phref_hub_setup(self.n, self.nup, self.ndn, self.nsup, self.nsdn,
+                            self.bup.astype(np.int32), self.bdn.astype(np.int32),
+                            self.T, self.U)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        w = np.empty_like(v)
+        hub.phref_hub_hprod(v, w)
+        return w
+
+    def dense(self, fast=True):
+        H = np.zeros((self.dim, self.dim))
+        e = np.zeros(self.dim)
+        for i in range(self.dim):
+            e[i] = 1.0
+            H[:, i] = self.hv(e, fast=fast)
+            e[i] = 0.0
+        return H
+
+    def _args(self, fast, nthreads):
+        return (0, self.n, self.nup, self.ndn, self.nsup, self.nsdn, self.bup, self.bdn,
+                self.T, self.U, np.zeros(1), int(fast), int(nthreads))
+
+    def lanczos(self, v0, maxlcz=600, acc=1e-8, fast=True, nthreads=1):
+        return _lanczos(self._args(fast, nthreads), v0, maxlcz, acc, self.dim)
+
+    def one_rdm(self, bra, ket=None):
+        ket = bra if ket is None else ket
+        out = np.zeros(2 * self.n * self.n)
+        lib().orc_one_rdm(self.n, self.nup, self.ndn, self.nsup, self.nsdn, self.bup, self.bdn,
+                          np.ascontiguousarray(bra, dtype=np.float64),
+                          np.ascontiguousarray(ket, dtype=np.float64), out)
+        return out.reshape(2, self.n, self.n)
+
+    def ni(self, bra, ket=None):
+        ket = bra if ket is None else ket
+        out = np.zeros(self.n)
+        lib().orc_ni(self.n, self.nsup, self.nsdn, self.bup, self.bdn,
+                     np.ascontiguousarray(bra, dtype=np.float64),
+                     np.ascontiguousarray(ket, dtype=np.float64), out)
+        return out
+
+    def sisj(self, bra, ket=None):
+        ket = bra if ket is None else ket
+        out = np.zeros(self.n * self.n)
+        lib().orc_j_spin(self.n, self.nsup, self.nsdn, self.bup, self.bdn,
+                         np.ascontiguousarray(bra, dtype=np.float64),
+                         np.ascontiguousarray(ket, dtype=np.float64), out)
+        return out.reshape(self.n, self.n)
+
+    def excite(self, psi, site, spin, create):
+        """c^dag_{site,spin} psi (create=1) or c_{site,spin} psi (create=0),
+        site 1-based; returns (target model, vector)."""
+        dn = (1 if create else -1)
+        tgt = HubbardModel(self.n, self.nup + (dn if spin == 0 else 0),
+                           self.ndn + (dn if spin == 1 else 0), self.T, self.U)
+        out = np.zeros(tgt.dim)
+        lib().orc_excite(self.n, site, spin, int(create), self.ndn,
+                         self.nsup, self.nsdn, self.bup, self.bdn,
+                         tgt.nsup, tgt.nsdn, tgt.bup, tgt.bdn,
+                         np.ascontiguousarray(psi, dtype=np.float64), out)
+        return tgt, out
+
+    def cf_chain(self, start, nsteps, fast=True, nthreads=1):
+        a = np.zeros(nsteps)
+        b = np.zeros(nsteps)
+        n2 = C.c_double(0)
+        lib().orc_cf_chain(*self._args(fast, nthreads), np.ascontiguousarray(start, dtype=np.float64),
+                           nsteps, a, b, C.byref(n2))
+        return a, b, n2.value
+
+
+class HeisenbergModel:
+    """Heisenberg cluster in the reference convention (hprod_heis.f90:1-61):
+    H = sum over ORDERED pairs J(J,K) S_J.S_K, S_z = 0 sector."""
+
+    def __init__(self, n, Jm):
+        self.n = int(n)
+        self.Jm = np.ascontiguousarray(Jm, dtype=np.float64)
+        self.bup = hilbert(n, n // 2)
+        self.nsup = len(self.bup)
+        self.dim = self.nsup
+
+    def hv(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        w = np.empty_like(v)
+        lib().orc_hprod_heis(self.n, self.nsup, self.bup, self.Jm, v, w)
+        return w
+
+    def hv_ref(self, v):
+        _, heis = _ref_libs()
+        heis.phref_heis_setup(self.n, self.nsup, self.bup.astype(np.int32), self.Jm)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        w = np.empty_like(v)
+        heis.phref_heis_hprod(v, w)
+        return w
+
+    def dense(self):
+        H = np.zeros((self.dim, self.dim))
+        e = np.zeros(self.dim)
+        for i in range(self.dim):
+            e[i] = 1.0
+            H[:, i] = self.hv(e)
+            e[i] = 0.0
+        return H
+
+    def lanczos(self, v0, maxlcz=600, acc=1e-8):
+        args = (1, self.n, self.n // 2, 0, self.nsup, 1, self.bup, self.bup,
+                np.zeros(1), np.zeros(1), self.Jm, 0, 1)
+        return _lanczos(args, v0, maxlcz, acc, self.dim)
+
+    def sisj(self, bra, ket=None):
+        ket = bra if ket is None else ket
+        out = np.zeros(self.n * self.n)
+        lib().orc_j_spin_heis(self.n, self.nsup, self.bup,
+                              np.ascontiguousarray(bra, dtype=np.float64),
+                              np.ascontiguousarray(ket, dtype=np.float64), out)
+        return out.reshape(self.n, self.n)
+
+
+def _lanczos(args, v0, maxlcz, acc, dim):
+    a = np.zeros(maxlcz + 1)
+    b = np.zeros(maxlcz + 2)
+    ev = np.zeros(maxlcz + 1)
+    psi = np.zeros(dim)
+    ndeg = C.c_int(0)
+    lost = C.c_int(0)
+    nb = lib().orc_lanczos(*args, maxlcz, acc, np.ascontiguousarray(v0, dtype=np.float64),
+                           a, b, ev, psi, C.byref(ndeg), C.byref(lost))
+    return {"e0": ev[0], "eig": ev[:nb].copy(), "alpha": a[:nb].copy(), "beta": b[:nb].copy(),
+            "nblcz": nb, "psi": psi, "ndeg": ndeg.value, "norm_lost": bool(lost.value)}
+
+
+def tridiag_eig(alpha, beta_off):
+    """Eigen-decomposition of the symmetric tridiagonal (alpha, beta_off)."""
+    m = len(alpha)
+    d = np.ascontiguousarray(alpha, dtype=np.float64).copy()
+    e = np.ascontiguousarray(np.append(beta_off, 0.0)[: max(m - 1, 1)], dtype=np.float64)
+    z = np.zeros(m * m)
+    rc = lib().orc_tridiag_eig(m, d, e, z.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise RuntimeError("tridiagonal QL did not converge")
+    return d, z.reshape(m, m).T  # columns are eigenvectors
+
+
+# ---------------------------------------------------------------------------
+# lattices and closed forms
+# ---------------------------------------------------------------------------
+def ring_T(L, t=1.0, eps=None, periodic=True):
+    T = np.zeros((L, L))
+    for i in range(L - (0 if periodic else 1)):
+        j = (i + 1) % L
+        T[i, j] = T[j, i] = -t
+    if eps is not None:
+        T[np.arange(L), np.arange(L)] = eps
+    return T
+
+
+def square_T(Lx, Ly, t=1.0, tp=0.0, periodic=True):
+    """Site s = x + Lx*y; T = -t on NN bonds, -tp on NNN bonds."""
+    n = Lx * Ly
+    T = np.zeros((n, n))
+
+    def s(x, y):
+        return (x % Lx) + Lx * (y % Ly)
+
+    for y in range(Ly):
+        for x in range(Lx):
+            i = s(x, y)
+            for dx, dy, amp in ((1, 0, -t), (0, 1, -t), (1, 1, -tp), (1, -1, -tp)):
+                xx, yy = x + dx, y + dy
+                if not periodic and not (0 <= xx < Lx and 0 <= yy < Ly):
+                    continue
+                j = s(xx, yy)
+                if i == j or amp == 0.0:
+                    continue
+                T[i, j] += amp if T[i, j] == 0.0 else 0.0
+                T[j, i] = T[i, j]
+    return T
+
+
+def free_fermion_e0(T, nup, ndn):
+    ek = np.linalg.eigvalsh(T)
+    return float(np.sum(ek[:nup]) + np.sum(ek[:ndn]))
+
+
+def dimer_e0(t, U):
+    return U / 2 - math.sqrt(U * U / 4 + 4 * t * t)
+
+
+def heisenberg_standard_dense(n, Jstd):
+    """Standard H = sum_{i<j} J_ij S_i.S_j on n spins-1/2, dense (S_z=0 block
+    is not extracted: full 2^n space).  For small n only."""
+    sx = np.array([[0, 0.5], [0.5, 0]])
+    sy = np.array([[0, -0.5j], [0.5j, 0]])
+    sz = np.array([[0.5, 0], [0, -0.5]])
+    I = np.eye(2)
+
+    def op(o, i):
+        m = np.array([[1.0]])
+        for k in range(n):
+            m = np.kron(m, o if k == i else I)
+        return m
+
+    H = np.zeros((2 ** n, 2 ** n), dtype=complex)
+    for i in range(n):
+        for j in range(i + 1, n):
+            if Jstd[i, j] != 0:
+                for o in (sx, sy, sz):
+                    H += Jstd[i, j] * op(o, i) @ op(o, j)
+    return H.real
+
+
+def lieb_wu_ring(L, nup, ndn, U, t=1.0, I=None, Jq=None, tol=1e-14):
+    """Finite periodic ring, Lieb-Wu Bethe equations for H = -t sum c^dag c + U sum n n,
+    N = nup + ndn electrons, M = ndn down spins (M <= N/2):
+        L k_j = 2 pi I_j - sum_b 2 atan((sin k_j - Lam_b)/u),           u = U/(4t)
+        sum_j 2 atan((Lam_a - sin k_j)/u) = 2 pi J_a + sum_b 2 atan((Lam_a - Lam_b)/(2u))
+        E = -2 t sum_j cos k_j.
+    Quantum numbers default to the ground-state choice (I_j integer for M even,
+    half-odd for M odd, J_a integer for N-M odd); when N is even and I_j are
+    integers, the set is shifted to {-(N/2-1)..N/2}."""
+    from scipy.optimize import fsolve
+    N, M = nup + ndn, min(nup, ndn)
+    u = U / (4.0 * t)
+    if I is None:
+        off = 0.0 if M % 2 == 0 else 0.5
+        I = np.arange(N) - (N - 1) / 2.0
+        if (I[0] - off) % 1.0 != 0.0:
+            I = I + 0.5
+    if Jq is None:
+        Jq = np.arange(M) - (M - 1) / 2.0
+    I = np.asarray(I, float)
+    Jq = np.asarray(Jq, float)
+
+    def eqs(x):
+        k, lam = x[:N], x[N:]
+        r1 = L * k - 2 * np.pi * I + np.sum(2 * np.arctan((np.sin(k)[:, None] - lam[None, :]) / u), axis=1)
+        r2 = (np.sum(2 * np.arctan((lam[:, None] - np.sin(k)[None, :]) / u), axis=1)
+              - 2 * np.pi * Jq - np.sum(2 * np.arctan((lam[:, None] - lam[None, :]) / (2 * u)), axis=1))
+        return np.concatenate([r1, r2])
+
+    k0 = 2 * np.pi * I / L
+    lam0 = np.sinh(np.pi * Jq / max(M, 1)) * 0.5 if M else np.zeros(0)
+    x, info, ier, msg = fsolve(eqs, np.concatenate([k0, lam0]), xtol=tol, full_output=True)
+    res = np.max(np.abs(eqs(x))) if len(x) else 0.0
+    return float(-2 * t * np.sum(np.cos(x[:N]))), float(res)

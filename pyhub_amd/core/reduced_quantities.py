@@ -1,0 +1,195 @@
+"""Reduced quantities and Green's functions with PyHub's API (mirror of
+pyhub/core/reduced_quantities.py).  The reference contracts HDF5 datasets
+written by the Fortran; here the per-state arrays come from libphub.so
+(phub_rdm_hubbard / phub_rdm_heisenberg / phub_excite / phub_cf_chain) and are
+Boltzmann-weighted the same way (reduced_quantities.py:12-51,84-110).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from pyhub_amd.tools.pole import Pole
+
+
+class StaticQuantities:
+    def __init__(self):
+        pass
+
+    def _w(self):
+        return self.wgt / np.sum(self.wgt)
+
+    @property
+    def one_rdm(self):
+        w = self._w()
+        rq = self._rq
+        return {"up": np.einsum("ijk,i->jk", rq["density_matrix_up"][: len(w)], w),
+                "down": np.einsum("ijk,i->jk", rq["density_matrix_down"][: len(w)], w)}
+
+    @property
+    def ni(self):
+        w = self._w()
+        return np.einsum("ij,i->j", self._rq["ni"][: len(w)], w)
+
+    @property
+    def two_rdm(self):
+        raise NotImplementedError("the full 4-index DBL_OCC tensor (is_Uloc = 3) is not built yet")
+
+    @property
+    def density(self):
+        return {spin: np.diag(self.one_rdm[spin]) for spin in ["up", "down"]}
+
+    @property
+    def density_matrix(self):
+        return self.one_rdm
+
+    @property
+    def one_rdm0(self):
+        nup, ndn = self.nup[0], self.ndown[0]
+        return {"up": self.Vk @ np.diag([1.0 if i < nup else 0.0 for i in range(self.nb_sites)]) @ self.Vk.T,
+                "down": self.Vk @ np.diag([1.0 if i < ndn else 0.0 for i in range(self.nb_sites)]) @ self.Vk.T}
+
+    @property
+    def density_matrix0(self):
+        return self.one_rdm0
+
+    @property
+    def wi(self):
+        return self.ni
+
+    @property
+    def spin_cor_matrix(self):
+        w = self._w()
+        return np.einsum("ijk,i->jk", self._rq["sisj"][: len(w)], w)
+
+    @property
+    def s2(self):
+        return np.sum(self.spin_cor_matrix)
+
+    @property
+    def eta_k(self):
+        return {"up": np.diag(self.Vk.T @ self.density_matrix["up"] @ self.Vk),
+                "down": np.diag(self.Vk.T @ self.density_matrix["down"] @ self.Vk)}
+
+    @property
+    def eta_k0(self):
+        nup, ndn = self.nup[0], self.ndown[0]
+        return {"up": np.array([1.0 if i < nup else 0.0 for i in range(self.nb_sites)]),
+                "down": np.array([1.0 if i < ndn else 0.0 for i in range(self.nb_sites)])}
+
+    @property
+    def e_one_body(self):
+        return (np.einsum("ij,ij->", self.one_rdm["up"], self.t_matrix)
+                + np.einsum("ij,ij->", self.one_rdm["down"], self.t_matrix))
+
+    @property
+    def e_two_body(self):
+        # is_Uloc is 2 for scalar / per-site U (core_solver.py:133-141), truthy
+        w = self._w()
+        return (np.sum(self.ni * self.U_matrix)
+                + np.einsum("bij,ij,b", self._rq["sisj"][: len(w)], self.J_matrix, w))
+
+
+class GreenFunction:
+    def __init__(self):
+        pass
+
+    def compute_gf(self):
+        d = self._gfdata
+        ncomp = self.nb_sites_comp
+        w = self.wgt
+        if "chains" in d:
+            # diagonal G_ii only (continued-fraction chains); off-diagonal left empty
+            gf = {}
+            e0 = self.e[0]
+            for spin in ("up", "down"):
+                arr = np.empty((ncomp, ncomp), dtype=object)
+                for i in range(ncomp):
+                    for j in range(ncomp):
+                        arr[i, j] = Pole.empty()
+                    thg, qg = d["chains"][f"greater_{spin}"][i][:2] if d["chains"][f"greater_{spin}"] else (np.zeros(0), np.zeros(0))
+                    thl, ql = d["chains"][f"lesser_{spin}"][i][:2] if d["chains"][f"lesser_{spin}"] else (np.zeros(0), np.zeros(0))
+                    arr[i, i] = (Pole({"positions": -e0 + thg, "weights": qg * qg}).clean(wtol=1.0e-8)
+                                 + Pole({"positions": e0 - thl, "weights": ql * ql}).clean(wtol=1.0e-8))
+                gf[spin] = arr
+            return gf
+        nb = self.nb_poles
+        gf = {}
+        for spin in ("up", "down"):
+            s = 0 if spin == "up" else 1
+            Qg, Ql = d[f"Q_greater_{spin}"], d[f"Q_lesser_{spin}"]
+            pg, pl = d[f"positions_greater_{spin}"], d[f"positions_lesser_{spin}"]
+            gf[spin] = np.array([[Pole.sum([
+                Pole({"positions": -self.e[b] + pg,
+                      "weights": Qg[i, : nb[1, s], b] * Qg[j, : nb[1, s], b] * wb / np.sum(w)}).clean(wtol=1.0e-8)
+                + Pole({"positions": self.e[b] - pl,
+                        "weights": Ql[i, : nb[0, s], b] * Ql[j, : nb[0, s], b] * wb / np.sum(w)}).clean(wtol=1.0e-8)
+                for b, wb in enumerate(w)]) for i in range(ncomp)] for j in range(ncomp)], dtype=Pole)
+        return gf
+
+    @property
+    def nb_poles(self):
+        return self._gfdata.get("nb_poles")
+
+    def _first_pole(self, kind, spin):
+        d = self._gfdata
+        if "chains" in d:
+            ch = d["chains"][f"{kind}_{spin}"]
+            cands = [c[0][np.argmax(np.abs(c[1]) > 1e-8)] for c in ch if len(c[0])]
+            if not cands:
+                return np.nan
+            return min(cands)
+        return d[f"positions_{kind}_{spin}"][0]
+
+    @property
+    def ip(self):
+        return {spin: (self._first_pole("lesser", spin) - self.e0) for spin in ["up", "down"]}
+
+    @property
+    def ae(self):
+        return {spin: (self.e0 - self._first_pole("greater", spin)) for spin in ["up", "down"]}
+
+    @property
+    def mu(self):
+        return {spin: (-self.ip[spin] - self.ae[spin]) / 2 for spin in ["up", "down"]}
+
+    @property
+    def density_matrix_from_gf(self):
+        return self.one_rdm_from_gf
+
+    @property
+    def one_rdm_from_gf(self):
+        d = self._gfdata
+        if "chains" in d:
+            raise NotImplementedError("needs the full Q matrix (band Lanczos / dense path)")
+        w = self.wgt / np.sum(self.wgt)
+        A = np.einsum("ikb,jkb,b->ij", d["Q_lesser_up"], d["Q_lesser_up"], w)
+        if self.nup == self.ndown:
+            return {"up": A, "down": A}
+        return {"up": A, "down": np.einsum("ikb,jkb,b->ij", d["Q_lesser_down"], d["Q_lesser_down"], w)}
+
+    @property
+    def gf0(self):
+        return {spin: np.array([[Pole({"positions": self.ek, "weights": self.Vk[i, :] * self.Vk[j, :]})
+                                 for i in range(self.nb_sites)] for j in range(self.nb_sites)])
+                for spin in ("up", "down")}
+
+    def gf_diag(self, omega, eta=0.05):
+        """G_ii(omega) for every computed site and spin, evaluated on the device from
+        the continued-fraction coefficients (chains path) -- complex (2, ncomp, nw)."""
+        from pyhub_amd.engine import Context, cf_eval
+        d = self._gfdata
+        if "chains" not in d:
+            gf = self.gf
+            return np.array([[gf[s][i, i].retarded(np.asarray(omega), eta) for i in range(self.nb_sites_comp)]
+                             for s in ("up", "down")])
+        ctx = Context.default()
+        e0 = self.e[0]
+        out = np.zeros((2, self.nb_sites_comp, len(omega)), dtype=complex)
+        for si, spin in enumerate(("up", "down")):
+            for kind, sign in (("greater", 1.0), ("lesser", -1.0)):
+                for i, ch in enumerate(d["chains"][f"{kind}_{spin}"]):
+                    a, b, n2 = ch[2], ch[3], ch[4]
+                    if n2 <= 0:
+                        continue
+                    out[si, i] += cf_eval(ctx, a, b, n2, e0, sign, omega, eta)[0]
+        return out

@@ -1,0 +1,442 @@
+// basis.hip -- context, device-built bases and ranking / hop tables.
+//
+// Basis order (reference: HILBERT, pyhub/libs/src/compute_basis.f90:67-96) is the
+// lexicographic order of the increasing set-bit positions; its closed form is
+// rank(x) = C(n,k) - 1 - colex(rev_n(x)).  Bases are produced here by unranking
+// that closed form on the device, one thread per state.
+#include "phub_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+int phub_fail(phub_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int phub_hip_fail(phub_ctx *ctx, hipError_t e, const char *what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    if (ctx) ctx->err = m;
+    return e == hipErrorOutOfMemory ? PHUB_ERR_OOM : PHUB_ERR_HIP;
+}
+
+static void host_binom(uint64_t *tab) {
+    for (int n = 0; n <= PHUB_MAXN; ++n)
+        for (int k = 0; k <= PHUB_MAXN; ++k) {
+            uint64_t v = 0;
+            if (k <= n) {
+                v = 1;
+                for (int i = 1; i <= k; ++i) v = v * (uint64_t)(n - k + i) / (uint64_t)i;
+            }
+            tab[n * (PHUB_MAXN + 1) + k] = v;
+        }
+}
+
+extern "C" uint64_t phub_binom(int n, int k) {
+    if (n < 0 || n > PHUB_MAXN || k < 0 || k > n) return 0;
+    uint64_t v = 1;
+    for (int i = 1; i <= k; ++i) v = v * (uint64_t)(n - k + i) / (uint64_t)i;
+    return v;
+}
+
+extern "C" int phub_ctx_create(int device, void *stream, phub_ctx **out) {
+    if (!out) return PHUB_ERR_ARG;
+    phub_ctx *c = new phub_ctx();
+    c->device = device;
+    c->stream = (hipStream_t)stream;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) { int r = phub_hip_fail(c, e, "hipSetDevice"); *out = c; return r; }
+    std::vector<uint64_t> tab((PHUB_MAXN + 1) * (PHUB_MAXN + 1));
+    host_binom(tab.data());
+    e = hipMalloc(&c->d_binom, tab.size() * sizeof(uint64_t));
+    if (e == hipSuccess)
+        e = hipMemcpy(c->d_binom, tab.data(), tab.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    *out = c;
+    if (e != hipSuccess) return phub_hip_fail(c, e, "phub_ctx_create");
+    return phub_ensure_red(c, 1 << 16);
+}
+
+extern "C" void phub_ctx_destroy(phub_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->d_binom) hipFree(c->d_binom);
+    if (c->d_red) hipFree(c->d_red);
+    if (c->h_red) hipHostFree(c->h_red);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    delete c;
+}
+
+extern "C" const char *phub_last_error(const phub_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int phub_sync(phub_ctx *c) {
+    if (!c) return PHUB_ERR_ARG;
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    return PHUB_OK;
+}
+
+extern "C" double phub_last_kernel_ms(const phub_ctx *c) { return c ? c->last_ms : 0.0; }
+
+int phub_ensure_red(phub_ctx *c, size_t nd) {
+    if (c->red_cap < nd) {
+        if (c->d_red) hipFree(c->d_red);
+        c->d_red = nullptr;
+        c->red_cap = 0;
+        PHUB_HIP(c, hipMalloc(&c->d_red, nd * sizeof(double)));
+        c->red_cap = nd;
+    }
+    if (c->h_red_cap < 4096) {
+        PHUB_HIP(c, hipHostMalloc(&c->h_red, 4096 * sizeof(double), hipHostMallocDefault));
+        c->h_red_cap = 4096;
+    }
+    return PHUB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+// partial layout: [block][comp]; one workgroup per component sums all blocks.
+__global__ void k_reduce_partials(const double *__restrict__ part, int nblocks, int ncomp,
+                                  double *__restrict__ out) {
+    __shared__ double sh[256];
+    int c = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += part[(size_t)b * ncomp + c];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = sh[0];
+}
+
+int phub_reduce_partials(phub_ctx *c, int nblocks, int ncomp, double *host_out) {
+    if ((size_t)nblocks * ncomp + ncomp > c->red_cap || (size_t)ncomp > c->h_red_cap)
+        return phub_fail(c, PHUB_ERR_ARG, "reduction buffer too small");
+    double *dst = c->d_red + (size_t)nblocks * ncomp;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(ncomp), dim3(256), 0, c->stream, c->d_red,
+                       nblocks, ncomp, dst);
+    PHUB_HIP(c, hipGetLastError());
+    PHUB_HIP(c, hipMemcpyAsync(c->h_red, dst, ncomp * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    std::memcpy(host_out, c->h_red, ncomp * sizeof(double));
+    return PHUB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// basis: unrank (reference order) and rank
+// ---------------------------------------------------------------------------
+template <typename M>
+__global__ void k_unrank(const uint64_t *__restrict__ binom, int n, int k, int64_t count,
+                         M *__restrict__ out) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= count) return;
+    uint64_t c = binom_lookup(binom, n, k) - 1 - (uint64_t)s;   // colex rank of rev_n(x)
+    uint64_t x = 0;
+    int p = n - 1;
+    for (int i = k; i >= 1; --i) {
+        while (binom_lookup(binom, p, i) > c) --p;             // largest p with C(p,i) <= c
+        x |= 1ull << (n - 1 - p);
+        c -= binom_lookup(binom, p, i);
+        --p;
+    }
+    out[s] = (M)x;
+}
+
+__global__ void k_rank(const uint64_t *__restrict__ binom, int n, int k,
+                       const uint64_t *__restrict__ masks, int64_t count,
+                       int64_t *__restrict__ ranks) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= count) return;
+    uint64_t x = masks[s];
+    ranks[s] = (__popcll(x) == k) ? rank_ref_dev(binom, x, n, k) : -1;
+}
+
+static inline unsigned nblk(int64_t count, int bs) { return (unsigned)((count + bs - 1) / bs); }
+
+extern "C" int phub_basis_build(phub_ctx *c, int n, int k, uint64_t *d_masks) {
+    if (!c || !d_masks) return phub_fail(c, PHUB_ERR_ARG, "null argument");
+    if (n < 0 || n > PHUB_MAXN || k < 0 || k > n) return phub_fail(c, PHUB_ERR_ARG, "bad (n,k)");
+    int64_t cnt = (int64_t)phub_binom(n, k);
+    if (cnt <= 0) return PHUB_OK;
+    hipLaunchKernelGGL(k_unrank<uint64_t>, dim3(nblk(cnt, 256)), dim3(256), 0, c->stream,
+                       c->d_binom, n, k, cnt, d_masks);
+    PHUB_HIP(c, hipGetLastError());
+    return PHUB_OK;
+}
+
+extern "C" int phub_basis_rank(phub_ctx *c, int n, int k, const uint64_t *d_masks,
+                               int64_t count, int64_t *d_ranks) {
+    if (!c || (!d_masks && count > 0) || (!d_ranks && count > 0))
+        return phub_fail(c, PHUB_ERR_ARG, "null argument");
+    if (n < 0 || n > PHUB_MAXN || k < 0 || k > n) return phub_fail(c, PHUB_ERR_ARG, "bad (n,k)");
+    if (count <= 0) return PHUB_OK;
+    hipLaunchKernelGGL(k_rank, dim3(nblk(count, 256)), dim3(256), 0, c->stream, c->d_binom, n,
+                       k, d_masks, count, d_ranks);
+    PHUB_HIP(c, hipGetLastError());
+    return PHUB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// hop tables (Hubbard)
+// ---------------------------------------------------------------------------
+struct PairList {
+    int np;
+    const int *bj;        // bit of the annihilated site J  (n-J)
+    const int *bk;        // bit of the created site K      (n-K)
+    const double *t;      // T(J,K)
+    const int16_t *pid;   // (J-1)*n+(K-1)
+};
+
+__global__ void k_hop_count(const uint32_t *__restrict__ masks, int64_t dim, PairList pl,
+                            int32_t *__restrict__ cnt, int *__restrict__ maxh) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int c = 0;
+    if (s < dim) {
+        uint32_t x = masks[s];
+        for (int p = 0; p < pl.np; ++p)
+            c += (((x >> pl.bj[p]) & 1u) && !((x >> pl.bk[p]) & 1u)) ? 1 : 0;
+        cnt[s] = c;
+    }
+    // wave max then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) c = max(c, __shfl_xor(c, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxh, c);
+}
+
+__global__ void k_hop_fill(const uint64_t *__restrict__ binom, int n, int k,
+                           const uint32_t *__restrict__ masks, int64_t dim, PairList pl, int maxh,
+                           bool kmajor, int32_t *__restrict__ idx, double *__restrict__ c64,
+                           float *__restrict__ c32, int16_t *__restrict__ pair,
+                           double *__restrict__ eloc, const double *__restrict__ tdiag) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= dim) return;
+    uint32_t x = masks[s];
+    int c = 0;
+    for (int p = 0; p < pl.np; ++p) {
+        int bj = pl.bj[p], bk = pl.bk[p];
+        if (((x >> bj) & 1u) && !((x >> bk) & 1u)) {
+            uint32_t xn = x ^ (1u << bj) ^ (1u << bk);
+            // JW sign: occupied sites strictly between J and K (ANTICOM(J,x)*ANTICOM(K,x'))
+            int lo = min(bj, bk), hi = max(bj, bk);
+            uint32_t between = (hi - lo > 1) ? (((1u << (hi - lo - 1)) - 1u) << (lo + 1)) : 0u;
+            double sg = (__popc(x & between) & 1) ? -1.0 : 1.0;
+            int64_t pos = rank_ref_dev(binom, xn, n, k);
+            size_t o = kmajor ? ((size_t)c * dim + s) : ((size_t)s * maxh + c);
+            idx[o] = (int32_t)pos;
+            double cc = sg * pl.t[p];
+            c64[o] = cc;
+            c32[o] = (float)cc;
+            if (pair) pair[o] = pl.pid[p];
+            ++c;
+        }
+    }
+    // pad the tail of row-major tables (uniform loops read cnt only)
+    double e = 0.0;
+    for (int b = 0; b < n; ++b)
+        if ((x >> b) & 1u) e += tdiag[n - 1 - b];
+    eloc[s] = e;
+}
+
+void phub_free_hop_table(HopTable *t) {
+    if (t->masks) hipFree(t->masks);
+    if (t->eloc) hipFree(t->eloc);
+    if (t->cnt) hipFree(t->cnt);
+    if (t->idx) hipFree(t->idx);
+    if (t->c64) hipFree(t->c64);
+    if (t->c32) hipFree(t->c32);
+    if (t->pair) hipFree(t->pair);
+    *t = HopTable();
+}
+
+int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor,
+                         bool all_pairs, HopTable *out) {
+    phub_free_hop_table(out);
+    out->n = n;
+    out->k = k;
+    out->kmajor = kmajor;
+    int64_t dim = (int64_t)phub_binom(n, k);
+    out->dim = dim;
+    if (dim > (int64_t)INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "species dimension exceeds int32");
+    // directed pairs J != K in the reference loop order (hprod_hubbard.f90:26,36-37)
+    std::vector<int> bj, bk;
+    std::vector<double> tv, tdiag(n);
+    std::vector<int16_t> pid;
+    for (int J = 1; J <= n; ++J) {
+        tdiag[J - 1] = (std::fabs(T[(J - 1) * n + (J - 1)]) >= 1e-14) ? T[(J - 1) * n + (J - 1)] : 0.0;
+        for (int K = 1; K <= n; ++K) {
+            if (J == K) continue;
+            double t = T[(J - 1) * n + (K - 1)];
+            if (!all_pairs && !(std::fabs(t) >= 1e-14)) continue;
+            bj.push_back(n - J);
+            bk.push_back(n - K);
+            tv.push_back(all_pairs ? 1.0 : t);
+            pid.push_back((int16_t)((J - 1) * n + (K - 1)));
+        }
+    }
+    int np = (int)bj.size();
+    int *d_bj = nullptr, *d_bk = nullptr, *d_max = nullptr;
+    double *d_t = nullptr, *d_td = nullptr;
+    int16_t *d_pid = nullptr;
+    size_t npa = std::max(np, 1);
+    PHUB_HIP(c, hipMalloc(&d_bj, npa * sizeof(int)));
+    PHUB_HIP(c, hipMalloc(&d_bk, npa * sizeof(int)));
+    PHUB_HIP(c, hipMalloc(&d_t, npa * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&d_pid, npa * sizeof(int16_t)));
+    PHUB_HIP(c, hipMalloc(&d_td, std::max(n, 1) * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&d_max, sizeof(int)));
+    if (np) {
+        PHUB_HIP(c, hipMemcpy(d_bj, bj.data(), np * sizeof(int), hipMemcpyHostToDevice));
+        PHUB_HIP(c, hipMemcpy(d_bk, bk.data(), np * sizeof(int), hipMemcpyHostToDevice));
+        PHUB_HIP(c, hipMemcpy(d_t, tv.data(), np * sizeof(double), hipMemcpyHostToDevice));
+        PHUB_HIP(c, hipMemcpy(d_pid, pid.data(), np * sizeof(int16_t), hipMemcpyHostToDevice));
+    }
+    if (n) PHUB_HIP(c, hipMemcpy(d_td, tdiag.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    PHUB_HIP(c, hipMemset(d_max, 0, sizeof(int)));
+    PHUB_HIP(c, hipMalloc(&out->masks, dim * sizeof(uint32_t)));
+    PHUB_HIP(c, hipMalloc(&out->eloc, dim * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&out->cnt, dim * sizeof(int32_t)));
+    hipLaunchKernelGGL(k_unrank<uint32_t>, dim3(nblk(dim, 256)), dim3(256), 0, c->stream,
+                       c->d_binom, n, k, dim, out->masks);
+    PairList pl{np, d_bj, d_bk, d_t, d_pid};
+    hipLaunchKernelGGL(k_hop_count, dim3(nblk(dim, 256)), dim3(256), 0, c->stream, out->masks,
+                       dim, pl, out->cnt, d_max);
+    PHUB_HIP(c, hipGetLastError());
+    int mh = 0;
+    PHUB_HIP(c, hipMemcpyAsync(&mh, d_max, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    out->maxh = mh;
+    size_t ne = (size_t)dim * std::max(mh, 1);
+    PHUB_HIP(c, hipMalloc(&out->idx, ne * sizeof(int32_t)));
+    PHUB_HIP(c, hipMalloc(&out->c64, ne * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&out->c32, ne * sizeof(float)));
+    PHUB_HIP(c, hipMemsetAsync(out->idx, 0, ne * sizeof(int32_t), c->stream));
+    PHUB_HIP(c, hipMemsetAsync(out->c64, 0, ne * sizeof(double), c->stream));
+    PHUB_HIP(c, hipMemsetAsync(out->c32, 0, ne * sizeof(float), c->stream));
+    if (all_pairs) PHUB_HIP(c, hipMalloc(&out->pair, ne * sizeof(int16_t)));
+    hipLaunchKernelGGL(k_hop_fill, dim3(nblk(dim, 256)), dim3(256), 0, c->stream, c->d_binom, n,
+                       k, out->masks, dim, pl, mh, kmajor, out->idx, out->c64, out->c32,
+                       out->pair, out->eloc, d_td);
+    PHUB_HIP(c, hipGetLastError());
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    hipFree(d_bj); hipFree(d_bk); hipFree(d_t); hipFree(d_pid); hipFree(d_td); hipFree(d_max);
+    return PHUB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Heisenberg tables: masks, two-half colex rank tables (built on the device)
+// ---------------------------------------------------------------------------
+// A[xh] for the n-L high bits of x; B[m][xl] for the L low bits given m = popc(xh).
+// colex(rev_n(x)) = A[x >> L] + B[popc(x >> L)][x & (2^L - 1)].
+__global__ void k_rank_tab_A(const uint64_t *__restrict__ binom, int n, int L,
+                             uint64_t *__restrict__ A) {
+    int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int nh = n - L;
+    if (h >= (1ll << nh)) return;
+    uint64_t c = 0;
+    int i = 0;
+    for (int b = nh - 1; b >= 0; --b)
+        if ((h >> b) & 1) { ++i; c += binom_lookup(binom, n - 1 - (b + L), i); }
+    A[h] = c;
+}
+
+__global__ void k_rank_tab_B(const uint64_t *__restrict__ binom, int n, int L,
+                             uint64_t *__restrict__ B) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int nh = n - L;
+    int64_t per = 1ll << L;
+    if (t >= per * (nh + 1)) return;
+    int m = (int)(t / per);
+    int64_t xl = t % per;
+    uint64_t c = 0;
+    int i = m;
+    for (int b = L - 1; b >= 0; --b)
+        if ((xl >> b) & 1) { ++i; c += binom_lookup(binom, n - 1 - b, i); }
+    B[t] = c;
+}
+
+void phub_free_heis(HeisTables *t) {
+    if (t->masks) hipFree(t->masks);
+    if (t->tabA) hipFree(t->tabA);
+    if (t->tabB) hipFree(t->tabB);
+    if (t->bj) hipFree(t->bj);
+    if (t->bk) hipFree(t->bk);
+    if (t->bJ) hipFree(t->bJ);
+    if (t->bJr) hipFree(t->bJr);
+    if (t->bsign) hipFree(t->bsign);
+    if (t->bond_ising) hipFree(t->bond_ising);
+    *t = HeisTables();
+}
+
+int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *out) {
+    phub_free_heis(out);
+    if (n < 2 || n > 40) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "Heisenberg needs 2 <= n <= 40");
+    int64_t dim = (int64_t)phub_binom(n, n / 2);
+    out->n = n;
+    out->dim = dim;
+    int L = n / 2;
+    out->L = L;
+    int nh = n - L;
+    PHUB_HIP(c, hipMalloc(&out->tabA, ((size_t)1 << nh) * sizeof(uint64_t)));
+    PHUB_HIP(c, hipMalloc(&out->tabB, ((size_t)1 << L) * (nh + 1) * sizeof(uint64_t)));
+    hipLaunchKernelGGL(k_rank_tab_A, dim3(nblk(1ll << nh, 256)), dim3(256), 0, c->stream,
+                       c->d_binom, n, L, out->tabA);
+    hipLaunchKernelGGL(k_rank_tab_B, dim3(nblk((1ll << L) * (nh + 1), 256)), dim3(256), 0,
+                       c->stream, c->d_binom, n, L, out->tabB);
+    PHUB_HIP(c, hipGetLastError());
+    PHUB_HIP(c, hipMalloc(&out->masks, dim * sizeof(uint64_t)));
+    hipLaunchKernelGGL(k_unrank<uint64_t>, dim3(nblk(dim, 256)), dim3(256), 0, c->stream,
+                       c->d_binom, n, n / 2, dim, out->masks);
+    PHUB_HIP(c, hipGetLastError());
+    // bonds: unordered {J<K} with either J(J,K) or J(K,J) non-zero
+    std::vector<int> bj, bk;
+    std::vector<double> bJ, bJr, bs, bis;
+    double dconst = 0.0;
+    for (int J = 1; J <= n; ++J) {
+        double jj = Jm[(J - 1) * n + (J - 1)];
+        if (std::fabs(jj) > 1e-14) dconst += 0.5 * jj;      // hprod_heis.f90:11-21
+        dconst += 0.25 * jj;                               // Ising J=K term (:54-58)
+        for (int K = J + 1; K <= n; ++K) {
+            double jk = Jm[(J - 1) * n + (K - 1)], kj = Jm[(K - 1) * n + (J - 1)];
+            if (jk == 0.0 && kj == 0.0) continue;
+            bj.push_back(n - J);
+            bk.push_back(n - K);
+            // flip element for the source with the spin-up at the first site of the
+            // ordered pair: -(-1)^{J+K} J(J,K) (hprod_heis.f90:39-47)
+            bJ.push_back(std::fabs(jk) > 1e-14 ? jk : 0.0);
+            bJr.push_back(std::fabs(kj) > 1e-14 ? kj : 0.0);
+            // reference HPROD: -(-1)^{J+K}; proper spin model in this fermion basis
+            // (the sign J_SPIN measures, static_rq_heis.f90:94-102): +(-1)^{J+K}
+            {
+                const double pr = ((J + K) & 1) ? -1.0 : 1.0;
+                bs.push_back(conv == 0 ? -pr : pr);
+            }
+            bis.push_back(jk + kj);
+            dconst += 0.25 * (jk + kj);
+        }
+    }
+    int nb = (int)bj.size();
+    out->nb = nb;
+    out->diag_const = dconst;
+    size_t nba = std::max(nb, 1);
+    PHUB_HIP(c, hipMalloc(&out->bj, nba * sizeof(int)));
+    PHUB_HIP(c, hipMalloc(&out->bk, nba * sizeof(int)));
+    PHUB_HIP(c, hipMalloc(&out->bJ, nba * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&out->bJr, nba * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&out->bsign, nba * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&out->bond_ising, nba * sizeof(double)));
+    if (nb) {
+        PHUB_HIP(c, hipMemcpy(out->bj, bj.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+        PHUB_HIP(c, hipMemcpy(out->bk, bk.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+        PHUB_HIP(c, hipMemcpy(out->bJ, bJ.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+        PHUB_HIP(c, hipMemcpy(out->bJr, bJr.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+        PHUB_HIP(c, hipMemcpy(out->bsign, bs.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+        PHUB_HIP(c, hipMemcpy(out->bond_ising, bis.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+    }
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    return PHUB_OK;
+}

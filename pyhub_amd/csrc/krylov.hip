@@ -1,0 +1,892 @@
+// krylov.hip -- Lanczos ground state, continued-fraction chains and the adjoint
+// MINRES solve, driven from the host over device-resident vectors.
+//
+// Reference: LANCZOS (pyhub/libs/src/lanczos.f90:1-163) -- a plain three-term
+// recurrence, Ritz values every NADD = 5 steps, the NDEG degeneracy logic
+// (:74-89), ground vector psi = V y / |V y| (compute_solver.f90:282-284).
+//
+// Modes
+//   PLAIN64: the reference algorithm on FP64 vectors (3 resident), psi rebuilt by
+//            a second pass of the same recurrence (the reference stores every
+//            vector in HDF5 instead, :49).
+//   FULL64 : FP64 vectors, full reorthogonalisation against the stored basis.
+//   MIXED  : FP32 vectors, FP64 alpha/beta/reorth coefficients and tridiagonal,
+//            FP64 reconstruction of psi, then FP64 Rayleigh-Ritz refinement steps.
+// Every Krylov vector x_k is kept unnormalised with a scale sig_k (x_k = sig_k * xt_k),
+// so the normalisation is folded into the next fused H*v instead of a pass.
+#include "phub_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+// ---------------------------------------------------------------------------
+// host tridiagonal eigensolver (implicit QL, Wilkinson shift)
+// ---------------------------------------------------------------------------
+// d[m] diagonal, e[m-1] off-diagonal.  Returns eigenvalues ascending in d; if z
+// is non-null it receives the eigenvectors, column j = z[j*m .. j*m+m-1].
+static int tridiag_ql(int m, double *d, const double *e_in, double *z) {
+    std::vector<double> e(m + 1, 0.0);
+    for (int i = 0; i + 1 < m; ++i) e[i] = e_in[i];
+    if (z) {
+        std::fill(z, z + (size_t)m * m, 0.0);
+        for (int i = 0; i < m; ++i) z[(size_t)i * m + i] = 1.0;
+    }
+    for (int l = 0; l < m; ++l) {
+        int it = 0;
+        while (true) {
+            int mm = l;
+            for (; mm < m - 1; ++mm) {
+                double dd = std::fabs(d[mm]) + std::fabs(d[mm + 1]);
+                if (std::fabs(e[mm]) <= 2.2e-16 * dd + 1e-300) break;
+            }
+            if (mm == l) break;
+            if (++it > 300) return 1;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = d[mm] - d[l] + e[l] / (g + std::copysign(r, g));
+            double s = 1.0, c = 1.0, p = 0.0;
+            int i = mm - 1;
+            bool early = false;
+            for (; i >= l; --i) {
+                double f = s * e[i], b = c * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] -= p;
+                    e[mm] = 0.0;
+                    early = true;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - b;
+                if (z) {
+                    double *zi = z + (size_t)i * m, *zi1 = z + (size_t)(i + 1) * m;
+                    for (int k = 0; k < m; ++k) {
+                        double t = zi1[k];
+                        zi1[k] = s * zi[k] + c * t;
+                        zi[k] = c * zi[k] - s * t;
+                    }
+                }
+            }
+            if (early) continue;
+            d[l] -= p;
+            e[l] = g;
+            e[mm] = 0.0;
+        }
+    }
+    // selection sort ascending
+    for (int i = 0; i + 1 < m; ++i) {
+        int k = i;
+        for (int j = i + 1; j < m; ++j)
+            if (d[j] < d[k]) k = j;
+        if (k != i) {
+            std::swap(d[i], d[k]);
+            if (z)
+                for (int q = 0; q < m; ++q) std::swap(z[(size_t)i * m + q], z[(size_t)k * m + q]);
+        }
+    }
+    return 0;
+}
+
+// lowest Ritz pair of the m x m Lanczos tridiagonal (alpha[0..m-1], beta[1..m-1])
+static int lowest_ritz(int m, const double *alpha, const double *beta, double *e0, double *e1,
+                       std::vector<double> *y) {
+    std::vector<double> d(alpha, alpha + m), e(std::max(m - 1, 1), 0.0);
+    for (int i = 0; i + 1 < m; ++i) e[i] = beta[i + 1];
+    std::vector<double> z;
+    if (y) z.resize((size_t)m * m);
+    if (tridiag_ql(m, d.data(), e.data(), y ? z.data() : nullptr)) return 1;
+    *e0 = d[0];
+    if (e1) *e1 = (m > 1) ? d[1] : d[0];
+    if (y) y->assign(z.begin(), z.begin() + m);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ void k_randvec(T *__restrict__ x, int64_t n, uint64_t seed) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        uint64_t r = splitmix64(seed * 0x632BE59BD9B4E019ull + (uint64_t)i);
+        x[i] = (T)((double)(r >> 11) * (1.0 / 9007199254740992.0) - 0.5);
+    }
+}
+
+template <typename T>
+__global__ void k_convert(const double *__restrict__ a, T *__restrict__ b, int64_t n, double s) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (T)(s * a[i]);
+}
+
+__device__ __forceinline__ double wave_sum(double a) {
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+    return a;
+}
+
+// per-block partial sums of a.b over a grid-stride range
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_dot(const TA *__restrict__ a, const TB *__restrict__ b,
+                                             int64_t n, double *__restrict__ part) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        s += (double)a[i] * (double)b[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// y = a*x + b*y ; partial |y|^2
+template <typename TX, typename TY>
+__global__ __launch_bounds__(256) void k_axpby(double a, const TX *__restrict__ x, double b,
+                                               TY *__restrict__ y, int64_t n,
+                                               double *__restrict__ part) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double v = (x ? a * (double)x[i] : 0.0) + b * (double)y[i];
+        TY t = (TY)v;
+        y[i] = t;
+        s += (double)t * (double)t;
+    }
+    if (part) {
+        s = wave_sum(s);
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    }
+}
+
+// second pass of the plain recurrence: out = (out - a*x) * binv ; psi += y*out
+template <typename T>
+__global__ __launch_bounds__(256) void k_second_pass(T *__restrict__ out, const T *__restrict__ x,
+                                                     double a, double binv,
+                                                     double *__restrict__ psi, double y,
+                                                     int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double v = ((double)out[i] - a * (double)x[i]) * binv;
+        T t = (T)v;
+        out[i] = t;
+        psi[i] += y * (double)t;
+    }
+}
+
+// multi-dot against a stored basis: part[block*k + j] = sum_slice xt_j . w
+// The block stages its slice of w in LDS; each wave takes basis vectors
+// j = wave, wave+4, ... and reduces its dot with DPP shuffles.
+constexpr int MD_SLICE = 2048;
+template <typename T, typename TW>
+__global__ __launch_bounds__(256) void k_multidot(const T *__restrict__ basis, int64_t ld, int k,
+                                                  const TW *__restrict__ w, int64_t n,
+                                                  double *__restrict__ part) {
+    __shared__ double ws[MD_SLICE];
+    extern __shared__ double acc[];   // k doubles
+    for (int j = threadIdx.x; j < k; j += blockDim.x) acc[j] = 0.0;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int64_t base = (int64_t)blockIdx.x * MD_SLICE; base < n;
+         base += (int64_t)gridDim.x * MD_SLICE) {
+        const int len = (int)std::min<int64_t>(MD_SLICE, n - base);
+        __syncthreads();
+        for (int i = threadIdx.x; i < MD_SLICE; i += blockDim.x)
+            ws[i] = (i < len) ? (double)w[base + i] : 0.0;
+        __syncthreads();
+        for (int j = wave; j < k; j += 4) {
+            const T *vj = basis + (size_t)j * ld + base;
+            double s = 0.0;
+            for (int i = lane; i < len; i += 64) s += (double)vj[i] * ws[i];
+            s = wave_sum(s);
+            if (lane == 0) acc[j] += s;
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < k; j += blockDim.x) part[(size_t)blockIdx.x * k + j] = acc[j];
+}
+
+// w -= sum_j coef_j * xt_j  (coef in FP64); partial |w|^2.  When psi_mode,
+// w = sum_j coef_j xt_j (w need not be initialised).
+template <typename T, typename TW>
+__global__ __launch_bounds__(256) void k_combine(const T *__restrict__ basis, int64_t ld, int k,
+                                                 const double *__restrict__ coef,
+                                                 TW *__restrict__ w, int64_t n, int psi_mode,
+                                                 double *__restrict__ part) {
+    extern __shared__ double cs[];
+    __shared__ double sh[4];
+    for (int j = threadIdx.x; j < k; j += blockDim.x) cs[j] = coef[j];
+    __syncthreads();
+    double s2 = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double v = psi_mode ? 0.0 : (double)w[i];
+        double sum = 0.0;
+        for (int j = 0; j < k; ++j) sum += cs[j] * (double)basis[(size_t)j * ld + i];
+        v = psi_mode ? sum : v - sum;
+        TW t = (TW)v;
+        w[i] = t;
+        s2 += (double)t * (double)t;
+    }
+    if (part) {
+        s2 = wave_sum(s2);
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s2;
+        __syncthreads();
+        if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// small host helpers
+// ---------------------------------------------------------------------------
+static const int VEC_GRID = 4096;
+
+static inline unsigned grid_for(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, VEC_GRID));
+}
+
+template <typename TA, typename TB>
+static int dev_dot(phub_ctx *c, const TA *a, const TB *b, int64_t n, double *out) {
+    unsigned g = grid_for(n);
+    int rc = phub_ensure_red(c, g + 64);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_dot<TA, TB>), dim3(g), dim3(256), 0, c->stream, a, b, n, c->d_red);
+    PHUB_HIP(c, hipGetLastError());
+    return phub_reduce_partials(c, (int)g, 1, out);
+}
+
+template <typename TX, typename TY>
+static int dev_axpby(phub_ctx *c, double a, const TX *x, double b, TY *y, int64_t n,
+                     double *norm2) {
+    unsigned g = grid_for(n);
+    int rc = phub_ensure_red(c, g + 64);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_axpby<TX, TY>), dim3(g), dim3(256), 0, c->stream, a, x, b, y, n,
+                       norm2 ? c->d_red : nullptr);
+    PHUB_HIP(c, hipGetLastError());
+    if (norm2) return phub_reduce_partials(c, (int)g, 1, norm2);
+    return PHUB_OK;
+}
+
+template <typename T, typename TW>
+static int dev_multidot(phub_ctx *c, const T *basis, int64_t ld, int k, const TW *w, int64_t n,
+                        double *out) {
+    int64_t slices = (n + MD_SLICE - 1) / MD_SLICE;
+    unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(slices, 1024));
+    int rc = phub_ensure_red(c, (size_t)g * k + k + 64);
+    if (rc) return rc;
+    if (k > 4096) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "too many basis vectors");
+    hipLaunchKernelGGL((k_multidot<T, TW>), dim3(g), dim3(256), k * sizeof(double), c->stream,
+                       basis, ld, k, w, n, c->d_red);
+    PHUB_HIP(c, hipGetLastError());
+    // reduce k components; host buffer may need to grow
+    if ((size_t)k > c->h_red_cap) {
+        if (c->h_red) hipHostFree(c->h_red);
+        PHUB_HIP(c, hipHostMalloc(&c->h_red, (size_t)k * sizeof(double), hipHostMallocDefault));
+        c->h_red_cap = k;
+    }
+    return phub_reduce_partials(c, (int)g, k, out);
+}
+
+template <typename T, typename TW>
+static int dev_combine(phub_ctx *c, const T *basis, int64_t ld, int k, const double *coef_host,
+                       TW *w, int64_t n, bool psi_mode, double *norm2, double *d_coef) {
+    PHUB_HIP(c, hipMemcpyAsync(d_coef, coef_host, k * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+    unsigned g = grid_for(n);
+    int rc = phub_ensure_red(c, g + 64);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_combine<T, TW>), dim3(g), dim3(256), k * sizeof(double), c->stream,
+                       basis, ld, k, d_coef, w, n, psi_mode ? 1 : 0,
+                       norm2 ? c->d_red : nullptr);
+    PHUB_HIP(c, hipGetLastError());
+    if (norm2) return phub_reduce_partials(c, (int)g, 1, norm2);
+    // keep d_coef alive until the kernel has read it
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    return PHUB_OK;
+}
+
+struct EvTimer {
+    phub_ctx *c;
+    hipEvent_t a, b;
+    double total_ms = 0.0;
+    int n = 0;
+    explicit EvTimer(phub_ctx *cc) : c(cc) {
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+    }
+    ~EvTimer() {
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+    }
+    void start() { hipEventRecord(a, c->stream); }
+    void stop() {
+        hipEventRecord(b, c->stream);
+        hipEventSynchronize(b);
+        float ms = 0;
+        hipEventElapsedTime(&ms, a, b);
+        total_ms += ms;
+        ++n;
+    }
+};
+
+// fused step: out = sig_k H xt_k - beta_eff xt_{k-1}; returns <x_k,out>, |out|^2
+template <typename T>
+static int fused_hv(phub_ctx *c, const phub_op *op, const T *xk, double sig, const T *xp,
+                    double beta_eff, T *out, double *dots, EvTimer *tm) {
+    int nb = 0;
+    if (tm) tm->start();
+    int rc = phub_hv_launch<T>(c, op, xk, out, sig, xp, beta_eff, true, &nb);
+    if (rc) return rc;
+    if (tm) tm->stop();
+    return phub_reduce_partials(c, nb, 2, dots);
+}
+
+// ---------------------------------------------------------------------------
+// Lanczos
+// ---------------------------------------------------------------------------
+struct LczState {
+    std::vector<double> alpha, beta;   // beta[0] = start norm, beta[k] couples k-1,k
+    int steps = 0;
+    int ndeg = 0;
+    bool converged = false;
+    bool breakdown = false;
+    bool norm_lost = false;
+    double ev = 0.0, e0 = 0.0;
+};
+
+// convergence test in the reference's form (lanczos.f90:74-89) on the Ritz values
+static void ref_convergence(LczState &s, double tol, int max_steps, const double *ritz) {
+    if (std::fabs(ritz[s.ndeg] - s.ev) < tol) {
+        if (std::fabs(ritz[s.ndeg] - ritz[0]) > tol && (s.ndeg + 1) > 1)
+            s.converged = true;
+        else
+            s.ndeg += 1;
+    } else {
+        if (s.steps <= max_steps - 10) s.ev = ritz[s.ndeg];
+    }
+}
+
+static int all_ritz(int m, const double *alpha, const double *beta, std::vector<double> &out) {
+    out.assign(alpha, alpha + m);
+    std::vector<double> e(std::max(m - 1, 1), 0.0);
+    for (int i = 0; i + 1 < m; ++i) e[i] = beta[i + 1];
+    return tridiag_ql(m, out.data(), e.data(), nullptr);
+}
+
+// plain three-term recurrence (no reorthogonalisation), 3 resident vectors
+template <typename T>
+static int lanczos_plain(phub_ctx *c, const phub_op *op, const phub_lanczos_opts *o,
+                         T *v0t, T *b1, T *b2, LczState &s, EvTimer *tm, bool fixed) {
+    const int64_t n = op->dim;
+    const int maxs = o->max_steps;
+    const int every = o->check_every > 0 ? o->check_every : 5;
+    s.alpha.assign(maxs + 2, 0.0);
+    s.beta.assign(maxs + 2, 0.0);
+    double n2 = 0.0;
+    int rc = dev_dot<T, T>(c, v0t, v0t, n, &n2);
+    if (rc) return rc;
+    if (!(n2 > 0.0)) return phub_fail(c, PHUB_ERR_ARG, "start vector is zero");
+    s.beta[0] = std::sqrt(n2);
+    T *xp = b1, *xk = v0t, *out = b2;   // xp content irrelevant while beta_eff = 0
+    double sig = 1.0 / s.beta[0], sigp = 0.0;
+    std::vector<double> ritz;
+    while (!s.converged && !s.breakdown && s.steps < maxs) {
+        double dots[2];
+        rc = fused_hv<T>(c, op, xk, sig, s.steps ? xp : nullptr, s.beta[s.steps] * sigp, out,
+                         dots, tm);
+        if (rc) return rc;
+        const double a = dots[0];
+        s.alpha[s.steps] = a;
+        double nn = 0.0;
+        rc = dev_axpby<T, T>(c, -a * sig, xk, 1.0, out, n, &nn);
+        if (rc) return rc;
+        const double b = std::sqrt(std::max(nn, 0.0));
+        s.steps++;
+        s.beta[s.steps] = b;
+        std::swap(xp, xk);
+        std::swap(xk, out);
+        sigp = sig;
+        sig = (b > 0.0) ? 1.0 / b : 0.0;
+        if (!(b > 1e-13 * std::max(1.0, std::fabs(a)))) s.breakdown = true;
+        if (!fixed && (s.steps % every == 0 || s.breakdown || s.steps == maxs)) {
+            if (all_ritz(s.steps, s.alpha.data(), s.beta.data(), ritz))
+                return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
+            s.e0 = ritz[0];
+            if (s.breakdown) { s.converged = true; break; }
+            ref_convergence(s, o->tol, maxs, ritz.data());
+        }
+    }
+    if (fixed || s.steps == 0 || !s.converged) {
+        if (all_ritz(s.steps, s.alpha.data(), s.beta.data(), ritz) == 0 && s.steps) s.e0 = ritz[0];
+    }
+    return PHUB_OK;
+}
+
+// second pass of the plain recurrence to rebuild psi = sum_k y_k x_k
+template <typename T>
+static int plain_second_pass(phub_ctx *c, const phub_op *op, const T *v0t, T *b1, T *b2, T *b3,
+                             const LczState &s, const std::vector<double> &y, double *psi) {
+    const int64_t n = op->dim;
+    const unsigned g = grid_for(n);
+    PHUB_HIP(c, hipMemsetAsync(psi, 0, n * sizeof(double), c->stream));
+    // x_0 = v0 / beta0 in b1
+    hipLaunchKernelGGL((k_axpby<T, T>), dim3(g), dim3(256), 0, c->stream, 1.0 / s.beta[0], v0t,
+                       0.0, b1, n, (double *)nullptr);
+    hipLaunchKernelGGL((k_axpby<T, double>), dim3(g), dim3(256), 0, c->stream, y[0], b1, 1.0,
+                       psi, n, (double *)nullptr);
+    T *xp = b2, *xk = b1, *out = b3;
+    const int m = (int)y.size();
+    for (int k = 0; k + 1 < m; ++k) {
+        int rc = phub_hv_launch<T>(c, op, xk, out, 1.0, k ? xp : nullptr, s.beta[k], false, nullptr);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_second_pass<T>), dim3(g), dim3(256), 0, c->stream, out, xk,
+                           s.alpha[k], 1.0 / s.beta[k + 1], psi, y[k + 1], n);
+        PHUB_HIP(c, hipGetLastError());
+        std::swap(xp, xk);
+        std::swap(xk, out);
+    }
+    return PHUB_OK;
+}
+
+// Lanczos with a stored basis and full reorthogonalisation (classical Gram-Schmidt,
+// repeated once when the norm drops by more than 1/sqrt(2)).
+template <typename T>
+static int lanczos_full(phub_ctx *c, const phub_op *op, const phub_lanczos_opts *o, T *basis,
+                        int64_t ld, LczState &s, std::vector<double> &sig, double *d_coef,
+                        EvTimer *tm, bool fixed) {
+    const int64_t n = op->dim;
+    const int maxs = o->max_steps;
+    const int every = o->check_every > 0 ? o->check_every : 5;
+    s.alpha.assign(maxs + 2, 0.0);
+    s.beta.assign(maxs + 2, 0.0);
+    sig.assign(maxs + 2, 0.0);
+    double n2 = 0.0;
+    int rc = dev_dot<T, T>(c, basis, basis, n, &n2);
+    if (rc) return rc;
+    if (!(n2 > 0.0)) return phub_fail(c, PHUB_ERR_ARG, "start vector is zero");
+    s.beta[0] = std::sqrt(n2);
+    sig[0] = 1.0 / s.beta[0];
+    std::vector<double> cf(maxs + 2), ritz;
+    while (!s.converged && !s.breakdown && s.steps < maxs) {
+        const int k = s.steps;
+        T *xk = basis + (size_t)k * ld;
+        T *out = basis + (size_t)(k + 1) * ld;
+        const T *xp = k ? basis + (size_t)(k - 1) * ld : nullptr;
+        double dots[2];
+        rc = fused_hv<T>(c, op, xk, sig[k], xp, k ? s.beta[k] * sig[k - 1] : 0.0, out, dots, tm);
+        if (rc) return rc;
+        // CGS against x_0..x_k (alpha_k = <x_k, out> comes out of the same pass)
+        double nbefore = dots[1], nafter = 0.0;
+        rc = dev_multidot<T, T>(c, basis, ld, k + 1, out, n, cf.data());
+        if (rc) return rc;
+        for (int j = 0; j <= k; ++j) cf[j] *= sig[j];   // <x_j, out>
+        s.alpha[k] = cf[k];
+        for (int j = 0; j <= k; ++j) cf[j] *= sig[j];   // coefficients on xt_j
+        rc = dev_combine<T, T>(c, basis, ld, k + 1, cf.data(), out, n, false, &nafter, d_coef);
+        if (rc) return rc;
+        if (nafter < 0.5 * nbefore) {
+            double nb2 = nafter;
+            rc = dev_multidot<T, T>(c, basis, ld, k + 1, out, n, cf.data());
+            if (rc) return rc;
+            for (int j = 0; j <= k; ++j) cf[j] *= sig[j];
+            s.alpha[k] += cf[k];
+            for (int j = 0; j <= k; ++j) cf[j] *= sig[j];
+            rc = dev_combine<T, T>(c, basis, ld, k + 1, cf.data(), out, n, false, &nafter, d_coef);
+            if (rc) return rc;
+            (void)nb2;
+        }
+        const double b = std::sqrt(std::max(nafter, 0.0));
+        s.steps++;
+        s.beta[s.steps] = b;
+        sig[s.steps] = (b > 0.0) ? 1.0 / b : 0.0;
+        if (!(b > 1e-13 * std::max(1.0, std::fabs(s.alpha[k])))) s.breakdown = true;
+        if (!fixed && (s.steps % every == 0 || s.breakdown || s.steps == maxs)) {
+            if (all_ritz(s.steps, s.alpha.data(), s.beta.data(), ritz))
+                return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
+            s.e0 = ritz[0];
+            if (s.breakdown) { s.converged = true; break; }
+            ref_convergence(s, o->tol, maxs, ritz.data());
+        }
+    }
+    if (s.steps && all_ritz(s.steps, s.alpha.data(), s.beta.data(), ritz) == 0) s.e0 = ritz[0];
+    return PHUB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Rayleigh-Ritz refinement of psi in FP64: a short Lanczos (m steps, full
+// reorthogonalisation) started from psi; psi <- lowest Ritz vector.
+// ---------------------------------------------------------------------------
+static int refine64(phub_ctx *c, const phub_op *op, double *psi, double *basis64, int m,
+                    double *d_coef, double *e_out) {
+    const int64_t n = op->dim;
+    PHUB_HIP(c, hipMemcpyAsync(basis64, psi, n * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream));
+    phub_lanczos_opts o{};
+    o.max_steps = m;
+    o.check_every = 1000000;
+    LczState s;
+    std::vector<double> sig;
+    int rc = lanczos_full<double>(c, op, &o, basis64, n, s, sig, d_coef, nullptr, true);
+    if (rc) return rc;
+    double e0 = 0, e1 = 0;
+    std::vector<double> y;
+    if (lowest_ritz(s.steps, s.alpha.data(), s.beta.data(), &e0, &e1, &y))
+        return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
+    std::vector<double> cf(s.steps);
+    for (int j = 0; j < s.steps; ++j) cf[j] = y[j] * sig[j];
+    double nn = 0.0;
+    rc = dev_combine<double, double>(c, basis64, n, s.steps, cf.data(), psi, n, true, &nn, d_coef);
+    if (rc) return rc;
+    rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / std::sqrt(nn), psi, n,
+                                   nullptr);
+    if (rc) return rc;
+    *e_out = e0;
+    return PHUB_OK;
+}
+
+// E = <psi|H|psi>, residual |H psi - E psi| (psi normalised), FP64; tmp: n doubles
+static int rq_residual(phub_ctx *c, const phub_op *op, const double *psi, double *tmp,
+                       double *e, double *res) {
+    const int64_t n = op->dim;
+    int rc = phub_hv_launch<double>(c, op, psi, tmp, 1.0, (const double *)nullptr, 0.0, false,
+                                    nullptr);
+    if (rc) return rc;
+    double pp = 0, ph = 0;
+    rc = dev_dot<double, double>(c, psi, psi, n, &pp);
+    if (!rc) rc = dev_dot<double, double>(c, psi, tmp, n, &ph);
+    if (rc) return rc;
+    const double E = ph / pp;
+    double r2 = 0;
+    rc = dev_axpby<double, double>(c, -E, psi, 1.0, tmp, n, &r2);
+    if (rc) return rc;
+    *e = E;
+    *res = std::sqrt(r2 / pp);
+    return PHUB_OK;
+}
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    ~DevBuf() {
+        if (p) hipFree(p);
+    }
+};
+
+extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_opts *o,
+                            const double *d_v0, double *alpha, double *beta,
+                            phub_lanczos_result *res, double *d_psi) {
+    if (!c || !op || !o || !res) return phub_fail(c, PHUB_ERR_ARG, "null argument");
+    if (o->max_steps < 1) return phub_fail(c, PHUB_ERR_ARG, "max_steps must be >= 1");
+    if (o->mode < 0 || o->mode > 2) return phub_fail(c, PHUB_ERR_ARG, "unknown Lanczos mode");
+    if (o->want_psi && !d_psi) return phub_fail(c, PHUB_ERR_ARG, "want_psi needs d_psi");
+    std::memset(res, 0, sizeof(*res));
+    PHUB_HIP(c, hipSetDevice(c->device));
+    const int64_t n = op->dim;
+    if (n == 0) return phub_fail(c, PHUB_ERR_ARG, "empty sector");
+    const bool fixed = o->fixed_steps > 0;
+    phub_lanczos_opts oo = *o;
+    if (fixed) oo.max_steps = o->fixed_steps;
+    oo.max_steps = (int)std::min<int64_t>(oo.max_steps, std::max<int64_t>(n, 1));
+    const unsigned g = (unsigned)((n + 255) / 256);
+    auto t0 = std::chrono::steady_clock::now();
+    EvTimer tm(c);
+    LczState s;
+    DevBuf<double> coef;
+    PHUB_HIP(c, hipMalloc(&coef.p, (size_t)(oo.max_steps + 8) * sizeof(double)));
+    int rc = PHUB_OK;
+    double t_lcz = 0.0;
+
+    if (o->mode == PHUB_LCZ_PLAIN64) {
+        DevBuf<double> v0, b1, b2, b3;
+        PHUB_HIP(c, hipMalloc(&v0.p, n * sizeof(double)));
+        PHUB_HIP(c, hipMalloc(&b1.p, n * sizeof(double)));
+        PHUB_HIP(c, hipMalloc(&b2.p, n * sizeof(double)));
+        PHUB_HIP(c, hipMalloc(&b3.p, n * sizeof(double)));
+        if (d_v0)
+            PHUB_HIP(c, hipMemcpyAsync(v0.p, d_v0, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        else
+            hipLaunchKernelGGL(k_randvec<double>, dim3(g), dim3(256), 0, c->stream, v0.p, n, o->seed);
+        // the recurrence rotates three buffers; keep v0 intact for the second pass
+        PHUB_HIP(c, hipMemcpyAsync(b3.p, v0.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        rc = lanczos_plain<double>(c, op, &oo, b3.p, b1.p, b2.p, s, &tm, fixed);
+        if (rc) return rc;
+        t_lcz = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (o->want_psi) {
+            double e0 = 0, e1 = 0;
+            std::vector<double> y;
+            if (lowest_ritz(s.steps, s.alpha.data(), s.beta.data(), &e0, &e1, &y))
+                return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
+            rc = plain_second_pass<double>(c, op, v0.p, b1.p, b2.p, b3.p, s, y, d_psi);
+            if (rc) return rc;
+            double nn = 0;
+            rc = dev_dot<double, double>(c, d_psi, d_psi, n, &nn);
+            if (rc) return rc;
+            if (std::fabs(1.0 - std::sqrt(nn)) > 1e-5) s.norm_lost = true;   // lanczos.f90:74-76
+            rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / std::sqrt(nn),
+                                           d_psi, n, nullptr);
+            if (rc) return rc;
+            PHUB_HIP(c, hipStreamSynchronize(c->stream));
+            res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            rc = rq_residual(c, op, d_psi, b1.p, &res->e0, &res->residual);
+            if (rc) return rc;
+        }
+        res->e0_ritz = s.e0;
+        if (!o->want_psi) res->e0 = s.e0;
+        else res->e0 = s.e0;
+    } else if (o->mode == PHUB_LCZ_FULL64) {
+        DevBuf<double> basis;
+        size_t bytes = (size_t)(oo.max_steps + 1) * n * sizeof(double);
+        hipError_t e = hipMalloc(&basis.p, bytes);
+        if (e != hipSuccess)
+            return phub_fail(c, PHUB_ERR_OOM, "FP64 Krylov basis does not fit: " +
+                                                  std::to_string(bytes / 1e9) + " GB requested");
+        if (d_v0)
+            PHUB_HIP(c, hipMemcpyAsync(basis.p, d_v0, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        else
+            hipLaunchKernelGGL(k_randvec<double>, dim3(g), dim3(256), 0, c->stream, basis.p, n, o->seed);
+        std::vector<double> sig;
+        rc = lanczos_full<double>(c, op, &oo, basis.p, n, s, sig, coef.p, &tm, fixed);
+        if (rc) return rc;
+        t_lcz = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        res->e0_ritz = res->e0 = s.e0;
+        if (o->want_psi) {
+            double e0 = 0, e1 = 0;
+            std::vector<double> y;
+            if (lowest_ritz(s.steps, s.alpha.data(), s.beta.data(), &e0, &e1, &y))
+                return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
+            std::vector<double> cf(s.steps);
+            for (int j = 0; j < s.steps; ++j) cf[j] = y[j] * sig[j];
+            double nn = 0;
+            rc = dev_combine<double, double>(c, basis.p, n, s.steps, cf.data(), d_psi, n, true, &nn, coef.p);
+            if (rc) return rc;
+            if (std::fabs(1.0 - std::sqrt(nn)) > 1e-5) s.norm_lost = true;
+            rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / std::sqrt(nn),
+                                           d_psi, n, nullptr);
+            if (rc) return rc;
+            PHUB_HIP(c, hipStreamSynchronize(c->stream));
+            res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            double E = 0;
+            rc = rq_residual(c, op, d_psi, basis.p, &E, &res->residual);
+            if (rc) return rc;
+        }
+    } else {   // MIXED
+        DevBuf<float> basis;
+        size_t bytes = (size_t)(oo.max_steps + 1) * n * sizeof(float);
+        hipError_t e = hipMalloc(&basis.p, bytes);
+        if (e != hipSuccess)
+            return phub_fail(c, PHUB_ERR_OOM, "FP32 Krylov basis does not fit: " +
+                                                  std::to_string(bytes / 1e9) + " GB requested");
+        if (d_v0)
+            hipLaunchKernelGGL(k_convert<float>, dim3(g), dim3(256), 0, c->stream, d_v0, basis.p, n, 1.0);
+        else
+            hipLaunchKernelGGL(k_randvec<float>, dim3(g), dim3(256), 0, c->stream, basis.p, n, o->seed);
+        std::vector<double> sig;
+        rc = lanczos_full<float>(c, op, &oo, basis.p, n, s, sig, coef.p, &tm, fixed);
+        if (rc) return rc;
+        t_lcz = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        res->e0_ritz = res->e0 = s.e0;
+        if (o->want_psi || o->refine_steps > 0) {
+            if (!d_psi) return phub_fail(c, PHUB_ERR_ARG, "refinement needs d_psi");
+            double e0 = 0, e1 = 0;
+            std::vector<double> y;
+            if (lowest_ritz(s.steps, s.alpha.data(), s.beta.data(), &e0, &e1, &y))
+                return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
+            std::vector<double> cf(s.steps);
+            for (int j = 0; j < s.steps; ++j) cf[j] = y[j] * sig[j];
+            double nn = 0;
+            rc = dev_combine<float, double>(c, basis.p, n, s.steps, cf.data(), d_psi, n, true, &nn, coef.p);
+            if (rc) return rc;
+            rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / std::sqrt(nn),
+                                           d_psi, n, nullptr);
+            if (rc) return rc;
+            // the FP32 basis is no longer needed: free it before the FP64 work
+            hipFree(basis.p);
+            basis.p = nullptr;
+            const int rs = std::max(0, std::min(o->refine_steps, 4));
+            DevBuf<double> b64;
+            const int m = 3;
+            PHUB_HIP(c, hipMalloc(&b64.p, (size_t)(m + 1) * n * sizeof(double)));
+            for (int it = 0; it < rs; ++it) {
+                double er = 0;
+                rc = refine64(c, op, d_psi, b64.p, m, coef.p, &er);
+                if (rc) return rc;
+                res->e0 = er;
+            }
+            PHUB_HIP(c, hipStreamSynchronize(c->stream));
+            res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            double E = 0;
+            rc = rq_residual(c, op, d_psi, b64.p, &E, &res->residual);
+            if (rc) return rc;
+            if (rs == 0) res->e0 = E;
+        }
+    }
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    if (!o->want_psi && !(o->mode == PHUB_LCZ_MIXED && o->refine_steps > 0))
+        res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    res->seconds_lanczos = t_lcz;
+    res->steps = s.steps;
+    res->converged = (s.converged || fixed) ? 1 : 0;
+    res->ndeg = s.ndeg;
+    res->hv_ms_avg = tm.n ? tm.total_ms / tm.n : 0.0;
+    if (alpha)
+        for (int i = 0; i < s.steps; ++i) alpha[i] = s.alpha[i];
+    if (beta)
+        for (int i = 0; i <= s.steps; ++i) beta[i] = s.beta[i];
+    if (s.norm_lost) return phub_fail(c, PHUB_ERR_NORM_LOST, "FORTRAN LANCZOS ERROR: NORM LOST equivalent: |psi| drifted by > 1e-5");
+    if (!res->converged)
+        return phub_fail(c, PHUB_ERR_NOCONV, "Lanczos reached max_steps without convergence");
+    return PHUB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// continued-fraction chain (single-vector GF_BANDLANCZOS, spgf_bandlanczos.f90:309-448)
+// ---------------------------------------------------------------------------
+template <typename T>
+static int cf_chain_t(phub_ctx *c, const phub_op *op, const double *start, int nsteps,
+                      double *alpha, double *beta, double *norm2) {
+    const int64_t n = op->dim;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    DevBuf<T> v0, b1, b2;
+    PHUB_HIP(c, hipMalloc(&v0.p, n * sizeof(T)));
+    PHUB_HIP(c, hipMalloc(&b1.p, n * sizeof(T)));
+    PHUB_HIP(c, hipMalloc(&b2.p, n * sizeof(T)));
+    double n2 = 0;
+    int rc = dev_dot<double, double>(c, start, start, n, &n2);
+    if (rc) return rc;
+    *norm2 = n2;
+    for (int i = 0; i < nsteps; ++i) alpha[i] = beta[i] = 0.0;
+    if (!(n2 > 0.0)) return PHUB_OK;
+    hipLaunchKernelGGL(k_convert<T>, dim3(g), dim3(256), 0, c->stream, start, v0.p, n, 1.0);
+    phub_lanczos_opts o{};
+    o.max_steps = nsteps;
+    LczState s;
+    rc = lanczos_plain<T>(c, op, &o, v0.p, b1.p, b2.p, s, nullptr, true);
+    if (rc) return rc;
+    for (int i = 0; i < s.steps && i < nsteps; ++i) {
+        alpha[i] = s.alpha[i];
+        beta[i] = s.beta[i + 1];
+    }
+    return PHUB_OK;
+}
+
+extern "C" int phub_cf_chain(phub_ctx *c, const phub_op *op, const double *d_start, int nsteps,
+                             int precision, double *alpha, double *beta, double *norm2) {
+    if (!c || !op || !d_start || !alpha || !beta || !norm2 || nsteps < 1)
+        return phub_fail(c, PHUB_ERR_ARG, "bad argument");
+    if (op->dim == 0) {
+        *norm2 = 0;
+        for (int i = 0; i < nsteps; ++i) alpha[i] = beta[i] = 0.0;
+        return PHUB_OK;
+    }
+    if (precision == 64) return cf_chain_t<double>(c, op, d_start, nsteps, alpha, beta, norm2);
+    if (precision == 32) return cf_chain_t<float>(c, op, d_start, nsteps, alpha, beta, norm2);
+    return phub_fail(c, PHUB_ERR_ARG, "precision must be 32 or 64");
+}
+
+// ---------------------------------------------------------------------------
+// adjoint MINRES: (H - e0) x = b on the complement of psi (Paige-Saunders MINRES)
+// ---------------------------------------------------------------------------
+static int project_out(phub_ctx *c, const double *psi, double *y, int64_t n) {
+    double p = 0;
+    int rc = dev_dot<double, double>(c, psi, y, n, &p);
+    if (rc) return rc;
+    return dev_axpby<double, double>(c, -p, psi, 1.0, y, n, nullptr);
+}
+
+extern "C" int phub_adjoint_minres(phub_ctx *c, const phub_op *op, const double *psi, double e0,
+                                   const double *b, double *x, int max_iter, double tol,
+                                   int *iters, double *rel_res) {
+    if (!c || !op || !psi || !b || !x) return phub_fail(c, PHUB_ERR_ARG, "null argument");
+    const int64_t n = op->dim;
+    DevBuf<double> r1, r2, y, v, w, w1, w2;
+    for (auto *bf : {&r1, &r2, &y, &v, &w, &w1, &w2})
+        PHUB_HIP(c, hipMalloc(&bf->p, n * sizeof(double)));
+    PHUB_HIP(c, hipMemsetAsync(x, 0, n * sizeof(double), c->stream));
+    PHUB_HIP(c, hipMemsetAsync(w.p, 0, n * sizeof(double), c->stream));
+    PHUB_HIP(c, hipMemsetAsync(w2.p, 0, n * sizeof(double), c->stream));
+    PHUB_HIP(c, hipMemcpyAsync(r1.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    int rc = project_out(c, psi, r1.p, n);
+    if (rc) return rc;
+    PHUB_HIP(c, hipMemcpyAsync(r2.p, r1.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    PHUB_HIP(c, hipMemcpyAsync(y.p, r1.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    double b1sq = 0;
+    rc = dev_dot<double, double>(c, r1.p, r1.p, n, &b1sq);
+    if (rc) return rc;
+    const double beta1 = std::sqrt(b1sq);
+    if (iters) *iters = 0;
+    if (rel_res) *rel_res = 0.0;
+    if (!(beta1 > 0.0)) return PHUB_OK;
+    double oldb = 0, beta = beta1, dbar = 0, epsln = 0, phibar = beta1, cs = -1, sn = 0;
+    int it = 0;
+    double rnorm = beta1;
+    for (it = 1; it <= max_iter; ++it) {
+        // v = y / beta
+        PHUB_HIP(c, hipMemcpyAsync(v.p, y.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / beta, v.p, n, nullptr);
+        if (rc) return rc;
+        // y = P (H - e0) v
+        rc = phub_hv_launch<double>(c, op, v.p, y.p, 1.0, (const double *)nullptr, 0.0, false, nullptr);
+        if (rc) return rc;
+        rc = dev_axpby<double, double>(c, -e0, v.p, 1.0, y.p, n, nullptr);
+        if (!rc) rc = project_out(c, psi, y.p, n);
+        if (rc) return rc;
+        if (it >= 2) {
+            rc = dev_axpby<double, double>(c, -beta / oldb, r1.p, 1.0, y.p, n, nullptr);
+            if (rc) return rc;
+        }
+        double alfa = 0;
+        rc = dev_dot<double, double>(c, v.p, y.p, n, &alfa);
+        if (rc) return rc;
+        double yy = 0;
+        rc = dev_axpby<double, double>(c, -alfa / beta, r2.p, 1.0, y.p, n, &yy);
+        if (rc) return rc;
+        std::swap(r1.p, r2.p);   // r1 <- r2
+        PHUB_HIP(c, hipMemcpyAsync(r2.p, y.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        oldb = beta;
+        beta = std::sqrt(std::max(yy, 0.0));
+        const double oldeps = epsln;
+        const double delta = cs * dbar + sn * alfa;
+        const double gbar = sn * dbar - cs * alfa;
+        epsln = sn * beta;
+        dbar = -cs * beta;
+        double gamma = std::max(std::hypot(gbar, beta), 1e-300);
+        cs = gbar / gamma;
+        sn = beta / gamma;
+        const double phi = cs * phibar;
+        phibar = sn * phibar;
+        // w1 <- w2, w2 <- w, w = (v - oldeps*w1 - delta*w2)/gamma
+        std::swap(w1.p, w2.p);
+        std::swap(w2.p, w.p);
+        PHUB_HIP(c, hipMemcpyAsync(w.p, v.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        rc = dev_axpby<double, double>(c, -oldeps, w1.p, 1.0, w.p, n, nullptr);
+        if (!rc) rc = dev_axpby<double, double>(c, -delta, w2.p, 1.0, w.p, n, nullptr);
+        if (!rc) rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / gamma, w.p, n, nullptr);
+        if (!rc) rc = dev_axpby<double, double>(c, phi, w.p, 1.0, x, n, nullptr);
+        if (rc) return rc;
+        rnorm = phibar;
+        if (rnorm / beta1 < tol || !(beta > 0.0)) break;
+    }
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    if (iters) *iters = std::min(it, max_iter);
+    if (rel_res) *rel_res = rnorm / beta1;
+    return PHUB_OK;
+}

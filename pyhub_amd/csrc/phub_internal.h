@@ -1,0 +1,123 @@
+// phub_internal.h -- internal types shared by the HIP sources of libphub.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/phub.h"
+
+#define PHUB_MAXN 64
+
+struct phub_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    uint64_t *d_binom = nullptr;     // (PHUB_MAXN+1)^2 binomials on the device
+    double *d_red = nullptr;         // reduction partials
+    size_t red_cap = 0;              // doubles
+    double *h_red = nullptr;         // pinned host mirror of reduction results
+    size_t h_red_cap = 0;
+};
+
+// Per-species hop table: for every state s the list of (target index, coefficient)
+// produced by c^dag_K c_J with J != K and |T(J,K)| >= 1e-14 (hprod_hubbard.f90:37,39-50).
+// Layout KMAJOR: entry k of state s at [k*dim + s] (lane-varying consumers);
+// ROWMAJOR: at [s*maxh + k] (wave-uniform consumers).
+struct HopTable {
+    int n = 0, k = 0;
+    int64_t dim = 0;
+    int maxh = 0;
+    bool kmajor = true;
+    uint32_t *masks = nullptr;    // device, dim
+    double *eloc = nullptr;       // device, dim: sum of T(J,J) over occupied J
+    int32_t *cnt = nullptr;       // device, dim
+    int32_t *idx = nullptr;       // device, dim*maxh
+    double *c64 = nullptr;        // device, dim*maxh
+    float *c32 = nullptr;         // device, dim*maxh
+    int16_t *pair = nullptr;      // device, dim*maxh, (J-1)*n+(K-1) (all-pairs tables only)
+};
+
+struct HeisTables {
+    int n = 0;
+    int64_t dim = 0;
+    uint64_t *masks = nullptr;     // device, dim
+    // two-half rank tables: rank(x) = C - 1 - (A[x >> L] + B[popc(x>>L)][x & (2^L-1)])
+    int L = 0;
+    uint64_t *tabA = nullptr;      // device, 2^(n-L)
+    uint64_t *tabB = nullptr;      // device, (n-L+1) * 2^L
+    int nb = 0;                    // number of undirected bonds
+    int *bj = nullptr, *bk = nullptr;   // device, bit positions of the two sites
+    double *bJ = nullptr;          // device, J(J,K) used when site J is up in the source
+    double *bJr = nullptr;         // device, J(K,J)
+    double *bsign = nullptr;       // device, flip sign factor per bond
+    double diag_const = 0.0;       // sum of 0.75 J(J,J) + 0.5 * (on-site) terms
+    double *bond_ising = nullptr;  // device, Ising weight per bond
+};
+
+struct phub_op {
+    int kind = 0;                  // 0 Hubbard, 1 Heisenberg
+    int n = 0, nup = 0, ndn = 0;
+    int64_t dim_up = 0, dim_dn = 0, dim = 0;
+    // Hubbard
+    HopTable up, dn;               // up KMAJOR, dn ROWMAJOR (hop tables of T)
+    HopTable rup, rdn;             // all-pairs tables for reduced quantities (lazy)
+    double U[PHUB_MAXN];
+    int u_uniform = 1;
+    double u0 = 0.0;
+    std::vector<double> T;         // host copy, row-major
+    // Heisenberg
+    HeisTables h;
+    std::vector<double> J;         // host copy
+    int sign_convention = 0;
+    double table_bytes = 0.0;
+    phub_ctx *ctx = nullptr;
+};
+
+// error helpers ------------------------------------------------------------------
+int phub_fail(phub_ctx *ctx, int code, const std::string &msg);
+int phub_hip_fail(phub_ctx *ctx, hipError_t e, const char *what);
+
+#define PHUB_HIP(ctx, call)                                                   \
+    do {                                                                      \
+        hipError_t _e = (call);                                               \
+        if (_e != hipSuccess) return phub_hip_fail((ctx), _e, #call);         \
+    } while (0)
+
+// shared device helpers ----------------------------------------------------------
+__host__ __device__ inline uint64_t binom_lookup(const uint64_t *tab, int n, int k) {
+    return (k < 0 || k > n) ? 0 : tab[n * (PHUB_MAXN + 1) + k];
+}
+
+// reference-order rank: C(n,k) - 1 - colex(rev_n(x)), closed form of HILBERT's order
+__device__ inline int64_t rank_ref_dev(const uint64_t *binom, uint64_t x, int n, int k) {
+    uint64_t c = 0;
+    int i = 0;
+    for (int b = n - 1; b >= 0; --b) {
+        if ((x >> b) & 1ull) {
+            ++i;
+            c += binom_lookup(binom, n - 1 - b, i);
+        }
+    }
+    return (int64_t)(binom_lookup(binom, n, k) - 1 - c);
+}
+
+// internal launches (implemented in the .hip files) ------------------------------
+int phub_build_hop_table(phub_ctx *ctx, int n, int k, const double *T_host, bool kmajor,
+                         bool all_pairs, HopTable *out);
+void phub_free_hop_table(HopTable *t);
+int phub_build_heis(phub_ctx *ctx, int n, const double *J_host, int sign_convention,
+                    HeisTables *out);
+void phub_free_heis(HeisTables *t);
+
+// reductions: sums of `ncomp` interleaved per-block partials -> host (synchronous)
+int phub_reduce_partials(phub_ctx *ctx, int nblocks, int ncomp, double *host_out);
+int phub_ensure_red(phub_ctx *ctx, size_t ndoubles);
+
+// H*v launchers with a fused Lanczos epilogue:
+//   out = scale * (H x) - beta * prev   (prev may be null)
+//   partials: <x_scaled, out>, <out, out> per block (when want_dots)
+template <typename T>
+int phub_hv_launch(phub_ctx *ctx, const phub_op *op, const T *x, T *out, double scale,
+                   const T *prev, double beta, bool want_dots, int *nblocks_out);

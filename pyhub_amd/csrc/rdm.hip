@@ -1,0 +1,498 @@
+// rdm.hip -- reduced quantities <bra| O |ket> and Hellmann-Feynman gradients.
+//
+// Reference: ONE_RDM, TWO_RDM (double occupancy NI) and J_SPIN
+// (pyhub/libs/src/static_rq.f90:153-395) and the Heisenberg J_SPIN
+// (static_rq_heis.f90:58-116).  The reference loops over site pairs with a linear
+// search per hop; here every quantity is one or two passes over the state:
+//   * hop expectations <bra| c^dag_K c_J |ket> for all J != K:
+//       up spin   -- lanes own a column u, read their all-pairs entries once and
+//                    accumulate over a chunk of rows in registers;
+//       down spin -- entries are uniform over a row, block-reduced per entry;
+//   * diagonal quantities (densities, double occupancy, <n n> correlations) from
+//     row sums p(d), column sums q(u) and R[d][J] = sum_u w[d][u] n_J(u);
+//   * spin-flip part of <S_J.S_K>: per element, up hop J->K and down hop K->J.
+#include "phub_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+__device__ __forceinline__ double wsum(double a) {
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+    return a;
+}
+
+// block-wide sum of one value (blockDim = 256) -> returned to every thread
+__device__ __forceinline__ double block_sum(double a, double *sh) {
+    a = wsum(a);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+    __syncthreads();
+    return sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+constexpr int RDM_ROWS = 64;   // rows per workgroup in the up-hop pass
+constexpr int RDM_CHUNK = 8;   // entries accumulated in registers at once
+
+// up-spin hop expectations: out[p] += sign * sum_d bra[d][u] * ket[d][u'] over entries
+__global__ __launch_bounds__(256) void k_rdm_up(const int64_t nu, const int64_t nd,
+                                                const int32_t *__restrict__ cnt,
+                                                const int32_t *__restrict__ idx,
+                                                const double *__restrict__ sgn,
+                                                const int16_t *__restrict__ pair,
+                                                const double *__restrict__ bra,
+                                                const double *__restrict__ ket, int np,
+                                                double *__restrict__ out, int ncb) {
+    extern __shared__ double acc[];   // np doubles
+    for (int i = threadIdx.x; i < np; i += blockDim.x) acc[i] = 0.0;
+    __syncthreads();
+    const int cb = blockIdx.x % ncb;
+    const int64_t d0 = (int64_t)(blockIdx.x / ncb) * RDM_ROWS;
+    const int64_t d1 = min(d0 + RDM_ROWS, nd);
+    const int64_t u = (int64_t)cb * 256 + threadIdx.x;
+    if (u < nu) {
+        const int c = cnt[u];
+        for (int k0 = 0; k0 < c; k0 += RDM_CHUNK) {
+            double a[RDM_CHUNK];
+            int32_t t[RDM_CHUNK];
+#pragma unroll
+            for (int e = 0; e < RDM_CHUNK; ++e) {
+                a[e] = 0.0;
+                t[e] = (k0 + e < c) ? idx[(size_t)(k0 + e) * nu + u] : (int32_t)u;
+            }
+            for (int64_t d = d0; d < d1; ++d) {
+                const double b = bra[d * nu + u];
+#pragma unroll
+                for (int e = 0; e < RDM_CHUNK; ++e) a[e] += b * ket[d * nu + t[e]];
+            }
+#pragma unroll
+            for (int e = 0; e < RDM_CHUNK; ++e)
+                if (k0 + e < c) {
+                    const size_t o = (size_t)(k0 + e) * nu + u;
+                    atomicAdd(&acc[pair[o]], sgn[o] * a[e]);
+                }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < np; i += blockDim.x)
+        if (acc[i] != 0.0) atomicAdd(&out[i], acc[i]);
+}
+
+// down-spin hop expectations: one workgroup per row d, entries uniform
+__global__ __launch_bounds__(256) void k_rdm_dn(const int64_t nu, const int64_t nd,
+                                                const int32_t *__restrict__ cnt,
+                                                const int32_t *__restrict__ idx,
+                                                const double *__restrict__ sgn,
+                                                const int16_t *__restrict__ pair, int maxh,
+                                                const double *__restrict__ bra,
+                                                const double *__restrict__ ket,
+                                                double *__restrict__ out) {
+    __shared__ double sh[4];
+    for (int64_t d = blockIdx.x; d < nd; d += gridDim.x) {
+        const int c = cnt[d];
+        for (int k = 0; k < c; ++k) {
+            const size_t o = (size_t)d * maxh + k;
+            const int64_t dp = idx[o];
+            double s = 0.0;
+            for (int64_t u = threadIdx.x; u < nu; u += blockDim.x) s += bra[d * nu + u] * ket[dp * nu + u];
+            s = block_sum(s, sh);
+            if (threadIdx.x == 0) atomicAdd(&out[pair[o]], sgn[o] * s);
+        }
+    }
+}
+
+// diagonal pass: R[d][J] (bit position b), p[d] via block reduction per row chunk;
+// q[u] accumulated per lane over the rows of the workgroup.
+__global__ __launch_bounds__(256) void k_rdm_diag(const int64_t nu, const int64_t nd, int n,
+                                                  const uint32_t *__restrict__ mu,
+                                                  const double *__restrict__ bra,
+                                                  const double *__restrict__ ket,
+                                                  double *__restrict__ R, double *__restrict__ p,
+                                                  double *__restrict__ q, int ncb) {
+    __shared__ double sh[4];
+    const int cb = blockIdx.x % ncb;
+    const int64_t d0 = (int64_t)(blockIdx.x / ncb) * RDM_ROWS;
+    const int64_t d1 = min(d0 + RDM_ROWS, nd);
+    const int64_t u = (int64_t)cb * 256 + threadIdx.x;
+    const uint32_t x = (u < nu) ? mu[u] : 0u;
+    double qs = 0.0;
+    for (int64_t d = d0; d < d1; ++d) {
+        const double w = (u < nu) ? bra[d * nu + u] * ket[d * nu + u] : 0.0;
+        qs += w;
+        const double ps = block_sum(w, sh);
+        if (threadIdx.x == 0) atomicAdd(&p[d], ps);
+        for (int b = 0; b < n; ++b) {
+            const double r = block_sum(((x >> b) & 1u) ? w : 0.0, sh);
+            if (threadIdx.x == 0) atomicAdd(&R[d * n + b], r);
+        }
+    }
+    if (u < nu) atomicAdd(&q[u], qs);
+}
+
+// spin-flip part of J_SPIN (static_rq.f90:345-383): for J != K,
+//   F[J][K] = sum ac*ac2 * bra[d][u] * ket[d'][u'], u' = c^dag_K c_J u (up),
+//   d' = c^dag_J c_K d (down).  Accumulated in LDS per ordered pair.
+__global__ __launch_bounds__(256) void k_rdm_flip(const uint64_t *__restrict__ binom,
+                                                  const int64_t nu, const int64_t nd, int n,
+                                                  int ndn, const uint32_t *__restrict__ mu,
+                                                  const uint32_t *__restrict__ md,
+                                                  const int32_t *__restrict__ cnt,
+                                                  const int32_t *__restrict__ idx,
+                                                  const double *__restrict__ sgn,
+                                                  const int16_t *__restrict__ pair,
+                                                  const double *__restrict__ bra,
+                                                  const double *__restrict__ ket,
+                                                  double *__restrict__ out, int ncb) {
+    extern __shared__ double acc[];   // n*n
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) acc[i] = 0.0;
+    __syncthreads();
+    const int cb = blockIdx.x % ncb;
+    const int64_t d0 = (int64_t)(blockIdx.x / ncb) * RDM_ROWS;
+    const int64_t d1 = min(d0 + RDM_ROWS, nd);
+    const int64_t u = (int64_t)cb * 256 + threadIdx.x;
+    if (u < nu) {
+        const int c = cnt[u];
+        for (int k = 0; k < c; ++k) {
+            const size_t o = (size_t)k * nu + u;
+            const int p = pair[o];
+            const int J = p / n + 1, K = p % n + 1;
+            const int bJ = n - J, bK = n - K;
+            const int64_t up2 = idx[o];
+            const double ac = sgn[o];
+            const int lo = min(bJ, bK), hi = max(bJ, bK);
+            const uint32_t between = (hi - lo > 1) ? (((1u << (hi - lo - 1)) - 1u) << (lo + 1)) : 0u;
+            double s = 0.0;
+            for (int64_t d = d0; d < d1; ++d) {
+                const uint32_t xd = md[d];
+                if (((xd >> bK) & 1u) && !((xd >> bJ) & 1u)) {
+                    const uint32_t xn = xd ^ (1u << bK) ^ (1u << bJ);
+                    const double ac2 = (__popc(xd & between) & 1) ? -1.0 : 1.0;
+                    const int64_t dp = rank_ref_dev(binom, xn, n, ndn);
+                    s += ac2 * bra[d * nu + u] * ket[dp * nu + up2];
+                }
+            }
+            if (s != 0.0) atomicAdd(&acc[p], ac * s);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x)
+        if (acc[i] != 0.0) atomicAdd(&out[i], acc[i]);
+}
+
+// ---------------------------------------------------------------------------
+// host assembly
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    ~DBuf() {
+        if (p) hipFree(p);
+    }
+};
+
+static int ensure_rdm_tables(phub_ctx *c, phub_op *op) {
+    if (!op->rup.idx) {
+        std::vector<double> ones(op->n * op->n, 1.0);
+        int rc = phub_build_hop_table(c, op->n, op->nup, ones.data(), true, true, &op->rup);
+        if (rc) return rc;
+    }
+    if (!op->rdn.idx) {
+        std::vector<double> ones(op->n * op->n, 1.0);
+        int rc = phub_build_hop_table(c, op->n, op->ndn, ones.data(), false, true, &op->rdn);
+        if (rc) return rc;
+    }
+    return PHUB_OK;
+}
+
+// Raw expectation values for the Hubbard operator:
+//  hop_up[J][K] = <bra| c^dag_K c_J (up) |ket>, J != K (raw ordered pair, not symmetrised)
+//  hop_dn[J][K] likewise for down; nu_[J], nd_[J] densities; ni[J] double occupancy;
+//  nn_uu, nn_ud, nn_dd [J][K] = <n_Ja n_Kb>; flip[J][K] as in J_SPIN.
+struct HubRaw {
+    std::vector<double> hop_up, hop_dn, dens_up, dens_dn, ni, nn_uu, nn_ud, nn_dd, flip;
+};
+
+static int hubbard_raw(phub_ctx *c, phub_op *op, const double *bra, const double *ket,
+                       HubRaw &r, bool want_flip) {
+    const int n = op->n;
+    const int64_t nu = op->dim_up, nd = op->dim_dn;
+    int rc = ensure_rdm_tables(c, op);
+    if (rc) return rc;
+    DBuf<double> dout, dR, dp, dq;
+    const int nn = n * n;
+    PHUB_HIP(c, hipMalloc(&dout.p, 3 * nn * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&dR.p, nd * n * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&dp.p, nd * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&dq.p, nu * sizeof(double)));
+    PHUB_HIP(c, hipMemsetAsync(dout.p, 0, 3 * nn * sizeof(double), c->stream));
+    PHUB_HIP(c, hipMemsetAsync(dR.p, 0, nd * n * sizeof(double), c->stream));
+    PHUB_HIP(c, hipMemsetAsync(dp.p, 0, nd * sizeof(double), c->stream));
+    PHUB_HIP(c, hipMemsetAsync(dq.p, 0, nu * sizeof(double), c->stream));
+    const int ncb = (int)((nu + 255) / 256);
+    const int64_t nrg = (nd + RDM_ROWS - 1) / RDM_ROWS;
+    const unsigned grid = (unsigned)(ncb * nrg);
+    if (op->rup.maxh > 0)
+        hipLaunchKernelGGL(k_rdm_up, dim3(grid), dim3(256), nn * sizeof(double), c->stream, nu, nd,
+                           op->rup.cnt, op->rup.idx, op->rup.c64, op->rup.pair, bra, ket, nn,
+                           dout.p, ncb);
+    if (op->rdn.maxh > 0)
+        hipLaunchKernelGGL(k_rdm_dn, dim3((unsigned)std::min<int64_t>(nd, 8192)), dim3(256), 0,
+                           c->stream, nu, nd, op->rdn.cnt, op->rdn.idx, op->rdn.c64,
+                           op->rdn.pair, op->rdn.maxh, bra, ket, dout.p + nn);
+    hipLaunchKernelGGL(k_rdm_diag, dim3(grid), dim3(256), 0, c->stream, nu, nd, n, op->up.masks,
+                       bra, ket, dR.p, dp.p, dq.p, ncb);
+    if (want_flip && op->rup.maxh > 0)
+        hipLaunchKernelGGL(k_rdm_flip, dim3(grid), dim3(256), nn * sizeof(double), c->stream,
+                           c->d_binom, nu, nd, n, op->ndn, op->up.masks, op->dn.masks,
+                           op->rup.cnt, op->rup.idx, op->rup.c64, op->rup.pair, bra, ket,
+                           dout.p + 2 * nn, ncb);
+    PHUB_HIP(c, hipGetLastError());
+    std::vector<double> hout(3 * nn), hR(nd * n), hp(nd), hq(nu);
+    std::vector<uint64_t> mup(nu), mdn(nd);
+    std::vector<uint32_t> m32u(nu), m32d(nd);
+    PHUB_HIP(c, hipMemcpyAsync(hout.data(), dout.p, 3 * nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PHUB_HIP(c, hipMemcpyAsync(hR.data(), dR.p, nd * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PHUB_HIP(c, hipMemcpyAsync(hp.data(), dp.p, nd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PHUB_HIP(c, hipMemcpyAsync(hq.data(), dq.p, nu * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PHUB_HIP(c, hipMemcpyAsync(m32u.data(), op->up.masks, nu * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    PHUB_HIP(c, hipMemcpyAsync(m32d.data(), op->dn.masks, nd * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    r.hop_up.assign(hout.begin(), hout.begin() + nn);
+    r.hop_dn.assign(hout.begin() + nn, hout.begin() + 2 * nn);
+    r.flip.assign(hout.begin() + 2 * nn, hout.end());
+    r.dens_up.assign(n, 0.0);
+    r.dens_dn.assign(n, 0.0);
+    r.ni.assign(n, 0.0);
+    r.nn_uu.assign(nn, 0.0);
+    r.nn_ud.assign(nn, 0.0);
+    r.nn_dd.assign(nn, 0.0);
+    // site J <-> bit n-J
+    for (int64_t u = 0; u < nu; ++u) {
+        const uint32_t x = m32u[u];
+        for (int J = 1; J <= n; ++J) {
+            if (!((x >> (n - J)) & 1u)) continue;
+            r.dens_up[J - 1] += hq[u];
+            for (int K = 1; K <= n; ++K)
+                if ((x >> (n - K)) & 1u) r.nn_uu[(J - 1) * n + (K - 1)] += hq[u];
+        }
+    }
+    for (int64_t d = 0; d < nd; ++d) {
+        const uint32_t x = m32d[d];
+        for (int J = 1; J <= n; ++J) {
+            if (!((x >> (n - J)) & 1u)) continue;
+            r.dens_dn[J - 1] += hp[d];
+            r.ni[J - 1] += hR[d * n + (n - J)];
+            for (int K = 1; K <= n; ++K) {
+                if ((x >> (n - K)) & 1u) r.nn_dd[(J - 1) * n + (K - 1)] += hp[d];
+                // <n_K,up n_J,dn>
+                r.nn_ud[(K - 1) * n + (J - 1)] += hR[d * n + (n - K)];
+            }
+        }
+    }
+    return PHUB_OK;
+}
+
+extern "C" int phub_rdm_hubbard(phub_ctx *c, const phub_op *op_, const double *bra,
+                                const double *ket, double *rdm_up, double *rdm_dn, double *ni,
+                                double *sisj) {
+    if (!c || !op_ || !bra || !ket) return phub_fail(c, PHUB_ERR_ARG, "null argument");
+    if (op_->kind != 0) return phub_fail(c, PHUB_ERR_ARG, "not a Hubbard operator");
+    phub_op *op = const_cast<phub_op *>(op_);
+    HubRaw r;
+    int rc = hubbard_raw(c, op, bra, ket, r, sisj != nullptr);
+    if (rc) return rc;
+    const int n = op->n;
+    // ONE_RDM (static_rq.f90:161-243): element (J,K), K <= J, mirrored to (K,J)
+    for (int J = 1; J <= n; ++J)
+        for (int K = 1; K <= J; ++K) {
+            double vu = (J == K) ? r.dens_up[J - 1] : r.hop_up[(J - 1) * n + (K - 1)];
+            double vd = (J == K) ? r.dens_dn[J - 1] : r.hop_dn[(J - 1) * n + (K - 1)];
+            if (rdm_up) rdm_up[(J - 1) * n + (K - 1)] = rdm_up[(K - 1) * n + (J - 1)] = vu;
+            if (rdm_dn) rdm_dn[(J - 1) * n + (K - 1)] = rdm_dn[(K - 1) * n + (J - 1)] = vd;
+        }
+    if (ni)
+        for (int J = 0; J < n; ++J) ni[J] = r.ni[J];
+    // J_SPIN (static_rq.f90:323-395): K <= J, mirrored
+    if (sisj)
+        for (int J = 1; J <= n; ++J)
+            for (int K = 1; K <= J; ++K) {
+                const int jk = (J - 1) * n + (K - 1), kj = (K - 1) * n + (J - 1);
+                double szsz = 0.25 * (r.nn_uu[jk] - r.nn_ud[jk] - r.nn_ud[kj] + r.nn_dd[jk]);
+                double v;
+                if (J == K) {
+                    const double single = r.dens_up[J - 1] + r.dens_dn[J - 1] - 2.0 * r.ni[J - 1];
+                    v = 0.5 * single + szsz;
+                } else {
+                    v = -r.flip[jk] + szsz;
+                }
+                sisj[jk] = sisj[kj] = v;
+            }
+    return PHUB_OK;
+}
+
+// Hellmann-Feynman: dE/dT(J,K) = sum_s <c^dag_K c_J>, dE/dT(J,J) = <n_J>, dE/dU_J = NI(J)
+extern "C" int phub_hf_grad_hubbard(phub_ctx *c, const phub_op *op_, const double *psi,
+                                    double *grad_t, double *grad_u) {
+    if (!c || !op_ || !psi) return phub_fail(c, PHUB_ERR_ARG, "null argument");
+    if (op_->kind != 0) return phub_fail(c, PHUB_ERR_ARG, "not a Hubbard operator");
+    phub_op *op = const_cast<phub_op *>(op_);
+    HubRaw r;
+    int rc = hubbard_raw(c, op, psi, psi, r, false);
+    if (rc) return rc;
+    const int n = op->n;
+    if (grad_t)
+        for (int J = 1; J <= n; ++J)
+            for (int K = 1; K <= n; ++K) {
+                const int jk = (J - 1) * n + (K - 1);
+                grad_t[jk] = (J == K) ? r.dens_up[J - 1] + r.dens_dn[J - 1] : r.hop_up[jk] + r.hop_dn[jk];
+            }
+    if (grad_u)
+        for (int J = 0; J < n; ++J) grad_u[J] = r.ni[J];
+    return PHUB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Heisenberg J_SPIN (static_rq_heis.f90:58-116) and dE/dJ
+// ---------------------------------------------------------------------------
+// flip[b] = sum over states i with site J up, K down of bra[pos] * ket[i]
+// (pos = flipped state), for bond b = {J<K}; and the reverse (K up, J down).
+// zz[b] = sum_i bra[i]*ket[i] * (same spin ? +1 : -1); norm = sum bra*ket.
+__global__ __launch_bounds__(256) void k_heis_bonds(int64_t dim, int L, uint64_t total_m1,
+                                                    const uint64_t *__restrict__ masks,
+                                                    const uint64_t *__restrict__ tabA,
+                                                    const uint64_t *__restrict__ tabB, int nb,
+                                                    const int *__restrict__ bj,
+                                                    const int *__restrict__ bk,
+                                                    const double *__restrict__ bra,
+                                                    const double *__restrict__ ket,
+                                                    double *__restrict__ out) {
+    extern __shared__ double acc[];   // 3*nb + 1
+    __shared__ double sh[4];
+    for (int i = threadIdx.x; i < 3 * nb + 1; i += blockDim.x) acc[i] = 0.0;
+    __syncthreads();
+    const uint64_t lmask = (1ull << L) - 1ull;
+    double nrm = 0.0;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < dim;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t x = masks[s];
+        const double k = ket[s];
+        const double w = bra[s] * k;
+        nrm += w;
+        for (int b = 0; b < nb; ++b) {
+            const int j = bj[b], kk = bk[b];
+            const bool diff = ((x >> j) ^ (x >> kk)) & 1ull;
+            atomicAdd(&acc[3 * b + 2], diff ? -w : w);
+            if (diff) {
+                const uint64_t x2 = x ^ ((1ull << j) | (1ull << kk));
+                const uint64_t xh = x2 >> L;
+                const uint64_t colex = tabA[xh] + tabB[((uint64_t)__popcll(xh) << L) + (x2 & lmask)];
+                const int64_t pos = (int64_t)(total_m1 - colex);
+                // source i = s has J up (bit j set): contributes to ordered (J,K)
+                atomicAdd(&acc[3 * b + (((x >> j) & 1ull) ? 0 : 1)], bra[pos] * k);
+            }
+        }
+    }
+    nrm = wsum(nrm);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) acc[3 * nb] += sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * nb + 1; i += blockDim.x) atomicAdd(&out[i], acc[i]);
+}
+
+static int heis_bonds(phub_ctx *c, const phub_op *op, const double *bra, const double *ket,
+                      std::vector<double> &h) {
+    const HeisTables &t = op->h;
+    const int nb = t.nb;
+    DBuf<double> d;
+    PHUB_HIP(c, hipMalloc(&d.p, (3 * nb + 1) * sizeof(double)));
+    PHUB_HIP(c, hipMemsetAsync(d.p, 0, (3 * nb + 1) * sizeof(double), c->stream));
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((t.dim + 255) / 256, 2048));
+    hipLaunchKernelGGL(k_heis_bonds, dim3(grid), dim3(256), (3 * nb + 1) * sizeof(double),
+                       c->stream, t.dim, t.L, (uint64_t)t.dim - 1, t.masks, t.tabA, t.tabB, nb,
+                       t.bj, t.bk, bra, ket, d.p);
+    PHUB_HIP(c, hipGetLastError());
+    h.resize(3 * nb + 1);
+    PHUB_HIP(c, hipMemcpyAsync(h.data(), d.p, (3 * nb + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    return PHUB_OK;
+}
+
+// host copy of the bond list in the same order phub_build_heis uses
+static void bond_list(const phub_op *op, std::vector<int> &J1, std::vector<int> &K1) {
+    const int n = op->n;
+    for (int J = 1; J <= n; ++J)
+        for (int K = J + 1; K <= n; ++K) {
+            const double jk = op->J[(J - 1) * n + (K - 1)], kj = op->J[(K - 1) * n + (J - 1)];
+            if (jk == 0.0 && kj == 0.0) continue;
+            J1.push_back(J);
+            K1.push_back(K);
+        }
+}
+
+extern "C" int phub_rdm_heisenberg(phub_ctx *c, const phub_op *op, const double *bra,
+                                   const double *ket, double *sisj) {
+    if (!c || !op || !bra || !ket || !sisj) return phub_fail(c, PHUB_ERR_ARG, "null argument");
+    if (op->kind != 1) return phub_fail(c, PHUB_ERR_ARG, "not a Heisenberg operator");
+    // J_SPIN runs over every ordered pair; pairs without a bond still need their
+    // <S_J.S_K>: compute with an all-pairs operator view.
+    const int n = op->n;
+    std::vector<double> ones(n * n, 1.0);
+    for (int i = 0; i < n; ++i) ones[i * n + i] = 0.0;
+    phub_op *all = nullptr;
+    phub_heis_desc hd{n, ones.data(), op->sign_convention};
+    int rc = phub_op_heisenberg(c, &hd, &all);
+    if (rc) return rc;
+    std::vector<double> h;
+    rc = heis_bonds(c, all, bra, ket, h);
+    phub_op_destroy(all);
+    if (rc) return rc;
+    std::vector<int> J1, K1;
+    for (int J = 1; J <= n; ++J)
+        for (int K = J + 1; K <= n; ++K) { J1.push_back(J); K1.push_back(K); }
+    const double nrm = h[3 * (int)J1.size()];
+    for (int J = 0; J < n; ++J) sisj[J * n + J] = 0.75 * nrm;   // 0.5*P(single) + 0.25
+    for (size_t b = 0; b < J1.size(); ++b) {
+        const int J = J1[b], K = K1[b];
+        // flip term: -AC * bra(pos) * ket(i), AC = -(-1)^{J+K} (four ANTICOMs)
+        const double sgn = ((J + K) & 1) ? -1.0 : 1.0;   // (-1)^{J+K}
+        const double zz = 0.25 * h[3 * b + 2];
+        sisj[(J - 1) * n + (K - 1)] = sgn * h[3 * b + 0] + zz;   // J up in the ket state
+        sisj[(K - 1) * n + (J - 1)] = sgn * h[3 * b + 1] + zz;
+    }
+    return PHUB_OK;
+}
+
+// dE/dJ(J,K) for the operator as built (both sign conventions), ordered pairs
+extern "C" int phub_hf_grad_heisenberg(phub_ctx *c, const phub_op *op, const double *psi,
+                                       double *grad_j) {
+    if (!c || !op || !psi || !grad_j) return phub_fail(c, PHUB_ERR_ARG, "null argument");
+    if (op->kind != 1) return phub_fail(c, PHUB_ERR_ARG, "not a Heisenberg operator");
+    const int n = op->n;
+    std::vector<double> ones(n * n, 1.0);
+    for (int i = 0; i < n; ++i) ones[i * n + i] = 0.0;
+    phub_op *all = nullptr;
+    phub_heis_desc hd{n, ones.data(), op->sign_convention};
+    int rc = phub_op_heisenberg(c, &hd, &all);
+    if (rc) return rc;
+    std::vector<double> h;
+    rc = heis_bonds(c, all, psi, psi, h);
+    phub_op_destroy(all);
+    if (rc) return rc;
+    const double nrm = h[h.size() - 1];
+    int b = 0;
+    for (int J = 1; J <= n; ++J) grad_j[(J - 1) * n + (J - 1)] = 0.75 * nrm;
+    for (int J = 1; J <= n; ++J)
+        for (int K = J + 1; K <= n; ++K, ++b) {
+            const double pr = ((J + K) & 1) ? -1.0 : 1.0;
+            const double fs = (op->sign_convention == 0) ? -pr : pr;   // flip element sign
+            // H_JK = J(J,K) [sz sz + flip(source J up)] ; derivative = <.>
+            // in pull form element(source J up -> target) = fs*J(J,K): <psi|.|psi> =
+            // fs * sum_{i: J up} psi[pos] psi[i]
+            const double zz = 0.25 * h[3 * b + 2];
+            grad_j[(J - 1) * n + (K - 1)] = fs * h[3 * b + 0] + zz;
+            grad_j[(K - 1) * n + (J - 1)] = fs * h[3 * b + 1] + zz;
+        }
+    return PHUB_OK;
+}

@@ -1,0 +1,1 @@
+"""Subpackage of pyhub_amd mirroring pyhub.solver."""

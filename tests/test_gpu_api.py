@@ -1,0 +1,112 @@
+"""GPU tests of the PyHub-compatible API (FermiHubbard / Heisenberg / CoreSolver),
+modelled on the reference's examples (examples/04,05,07,08,09)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CFG1_E0 = -4.603526299989137
+
+
+def test_fermi_hubbard_config1_lanczos_path(ctx):
+    from pyhub_amd.solver.fermi_hubbard import FermiHubbard
+    T = O.ring_T(8)
+    fh = FermiHubbard(8, 4, 4, T, U=4.0)
+    fh.start_vector = np.random.default_rng(1).uniform(-0.5, 0.5, 4900)
+    fh.kernel(max_lcz=600, acc_lcz=1e-12)
+    assert fh.is_lanczos
+    assert abs(fh.e0 - CFG1_E0) < 1e-10 * abs(CFG1_E0)
+    m = O.HubbardModel(8, 4, 4, T, 4.0)
+    ev, V = np.linalg.eigh(m.dense())
+    psi = V[:, 0]
+    ro = m.one_rdm(psi)
+    assert np.max(np.abs(fh.one_rdm["up"] - ro[0])) < 1e-8
+    assert np.max(np.abs(fh.one_rdm["down"] - ro[1])) < 1e-8
+    assert np.max(np.abs(fh.ni - m.ni(psi))) < 1e-8
+    assert np.max(np.abs(fh.spin_cor_matrix - m.sisj(psi))) < 1e-8
+    assert abs(fh.s2) < 1e-8     # singlet
+    # Hellmann-Feynman identity E0 = <T> + <U n n>
+    assert abs(fh.e_one_body + np.sum(fh.ni * 4.0) - fh.e0) < 1e-9
+    assert fh.lcz_results["step"] == fh.nb_lcz
+
+
+def test_fermi_hubbard_dense_path_and_green_functions(ctx):
+    from pyhub_amd.solver.fermi_hubbard import FermiHubbard
+    T = O.ring_T(6)
+    T[0, 3] = T[3, 0] = -0.3
+    fh = FermiHubbard(6, 3, 2, T, U=3.0)
+    fh.kernel(max_lcz=600, compute_spgf=True)      # dim 300 < 600 -> EXACT_DIAG path
+    assert not fh.is_lanczos
+    m = O.HubbardModel(6, 3, 2, T, 3.0)
+    ev, V = np.linalg.eigh(m.dense())
+    assert abs(fh.e0 - ev[0]) < 1e-12
+    assert np.max(np.abs(fh.e - ev)) < 1e-10
+    # G_ii sum rule: total weight of the diagonal = 1
+    for spin in ("up", "down"):
+        for i in range(6):
+            assert abs(np.sum(fh.gf[spin][i, i].pole["weights"]) - 1.0) < 1e-6
+    # lesser weights reproduce the 1-RDM (static_rq vs spgf consistency)
+    A = fh.one_rdm_from_gf["up"]
+    assert np.max(np.abs(A - fh.one_rdm["up"])) < 1e-10
+    # poles vs oracle: lowest N+1 (up) eigenvalue
+    mt, _ = m.excite(V[:, 0], 1, 0, 1)
+    e_np1 = np.linalg.eigvalsh(mt.dense())[0]
+    assert abs(fh._gfdata["positions_greater_up"][0] - e_np1) < 1e-10
+    assert abs(fh.ae["up"] - (fh.e0 - e_np1)) < 1e-10
+
+
+def test_lanczos_path_chains_sum_rule(ctx):
+    from pyhub_amd.solver.fermi_hubbard import FermiHubbard
+    T = O.ring_T(8)
+    fh = FermiHubbard(8, 4, 4, T, U=4.0)
+    fh.kernel(max_lcz=600, acc_lcz=1e-12, compute_spgf=True, nb_sites_comp=2)
+    for spin in ("up", "down"):
+        for i in range(2):
+            w = fh.gf[spin][i, i].pole["weights"]
+            assert abs(np.sum(w) - 1.0) < 1e-6
+    omega = np.linspace(-8, 8, 64)
+    G = fh.gf_diag(omega, eta=0.2)
+    Gp = np.array([fh.gf["up"][0, 0].retarded(omega, 0.2)])
+    assert np.max(np.abs(G[0, 0] - Gp[0])) < 1e-6
+
+
+def test_heisenberg_ring(ctx):
+    from pyhub_amd.solver.heisenberg import Heisenberg
+    n = 8
+    J = np.zeros((n, n))
+    for i in range(n):
+        J[i, (i + 1) % n] = J[(i + 1) % n, i] = 1.0
+    h = Heisenberg(n, J)
+    h.kernel(max_lcz=20, acc_lcz=1e-12)              # dim 70 >= 20 -> Lanczos path
+    m = O.HeisenbergModel(n, J)
+    ev, V = np.linalg.eigh(m.dense())
+    assert abs(h.e0 - ev[0]) < 1e-9
+    assert np.max(np.abs(h.spin_cor_matrix - m.sisj(V[:, 0]))) < 1e-7
+    # reference convention on a bipartite ring: 2x the standard energy
+    es = np.linalg.eigvalsh(O.heisenberg_standard_dense(n, np.triu(J, 1)))[0]
+    assert abs(h.e0 - 2 * es) < 1e-9
+
+
+def test_dimer_closed_form(ctx):
+    from pyhub_amd.solver.fermi_hubbard import FermiHubbard
+    for U in (0.0, 2.0, 8.0):
+        fh = FermiHubbard(2, 1, 1, np.array([[0.0, -1.0], [-1.0, 0.0]]), U=U)
+        fh.kernel()
+        assert abs(fh.e0 - O.dimer_e0(1.0, U)) < 1e-12
+
+
+def test_errors(ctx):
+    from pyhub_amd.solver.fermi_hubbard import FermiHubbard
+    with pytest.raises(ValueError):
+        FermiHubbard(3, 1, 1, np.array([[0, 1, 0], [0, 0, 1], [0, 0, 0.0]]), U=1.0)
+    fh = FermiHubbard(8, 4, 4, O.ring_T(8), U=4.0)
+    fh.kernel(max_lcz=12, acc_lcz=1e-14, compute_rq=False)   # step cap reached
+    with pytest.raises(StopIteration):
+        fh.lcz_results
+    T = O.ring_T(4)
+    T[0, 0] = np.nan
+    with pytest.raises(ValueError):
+        FermiHubbard(4, 2, 2, T, U=1.0).kernel()

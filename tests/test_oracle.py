@@ -1,0 +1,154 @@
+"""CPU tests: the oracle pinned against the reference's own HPROD (compiled
+from its Fortran sources), brute force, closed forms and the Bethe ansatz; the
+C-ABI library exports everything include/phub.h declares."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_basis_order_brute_force_and_rank():
+    for n in range(1, 13):
+        for k in range(n + 1):
+            b = O.hilbert(n, k)
+            assert np.array_equal(b, O.brute_basis(n, k)), (n, k)
+            assert [O.rank_ref(n, k, int(x)) for x in b] == list(range(len(b)))
+    assert list(O.hilbert(4, 2)) == [3, 5, 9, 6, 10, 12]
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference HPROD build (oracle/_ref) absent")
+@pytest.mark.parametrize("n,nu,nd,seed", [(4, 2, 2, 0), (5, 2, 3, 1), (6, 3, 2, 2), (6, 1, 4, 3),
+                                          (7, 3, 3, 4), (8, 4, 4, 5), (6, 0, 3, 6), (5, 5, 2, 7)])
+def test_hubbard_restatement_bit_exact_vs_reference(n, nu, nd, seed):
+    rng = np.random.default_rng(seed)
+    T = rng.normal(size=(n, n))
+    T = (T + T.T) / 2
+    U = rng.uniform(1, 5, n)
+    m = O.HubbardModel(n, nu, nd, T, U)
+    v = rng.normal(size=m.dim)
+    assert np.array_equal(m.hv(v), m.hv_ref(v))
+    assert np.allclose(m.hv(v, fast=True), m.hv_ref(v), rtol=1e-13, atol=1e-12)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference HPROD build (oracle/_ref) absent")
+@pytest.mark.parametrize("n,seed", [(4, 0), (6, 1), (8, 2), (10, 3)])
+def test_heisenberg_restatement_bit_exact_vs_reference(n, seed):
+    rng = np.random.default_rng(seed)
+    J = rng.normal(size=(n, n))
+    J = (J + J.T) / 2
+    m = O.HeisenbergModel(n, J)
+    v = rng.normal(size=m.dim)
+    assert np.array_equal(m.hv(v), m.hv_ref(v))
+
+
+def test_closed_forms():
+    T = O.ring_T(8)
+    m0 = O.HubbardModel(8, 4, 4, T, 0.0)
+    assert abs(np.linalg.eigvalsh(m0.dense())[0] - O.free_fermion_e0(T, 4, 4)) < 1e-12
+    md = O.HubbardModel(2, 1, 1, np.array([[0, -1.], [-1, 0]]), 4.0)
+    assert abs(np.linalg.eigvalsh(md.dense())[0] - O.dimer_e0(1, 4)) < 1e-13
+    # atomic limit t = 0: diagonal, E0 = U * max(0, N - L) double occupancies
+    ma = O.HubbardModel(4, 3, 3, np.zeros((4, 4)), 2.0)
+    H = ma.dense()
+    assert np.allclose(H, np.diag(np.diag(H)))
+    assert abs(np.diag(H).min() - 2.0 * 2) < 1e-14
+
+
+def test_config1_dense_vs_bethe_ansatz():
+    """Config 1 (8-site ring, U=4, 4+4): dense FP64 ED vs the finite-ring Lieb-Wu solution."""
+    m = O.HubbardModel(8, 4, 4, O.ring_T(8), 4.0)
+    ev = np.linalg.eigvalsh(m.dense())
+    e_ba, res = O.lieb_wu_ring(8, 4, 4, 4.0)
+    assert res < 1e-12
+    assert abs(ev[0] - e_ba) / abs(ev[0]) < 1e-12
+    assert abs(ev[0] - (-4.603526299989137)) < 1e-11
+
+
+def test_oracle_lanczos_matches_dense():
+    m = O.HubbardModel(8, 4, 4, O.ring_T(8), 4.0)
+    rng = np.random.default_rng(1)
+    r = m.lanczos(rng.uniform(-0.5, 0.5, m.dim), maxlcz=600, acc=1e-12)
+    assert abs(r["e0"] - (-4.603526299989137)) < 1e-10
+    Hpsi = m.hv(r["psi"])
+    assert np.linalg.norm(Hpsi - r["e0"] * r["psi"]) < 1e-6
+
+
+def test_oracle_rdm_sum_rules_and_hellmann_feynman():
+    rng = np.random.default_rng(3)
+    n = 6
+    T = O.ring_T(n, eps=rng.normal(0, 0.3, n))
+    T[0, 3] = T[3, 0] = -0.4
+    m = O.HubbardModel(n, 3, 2, T, rng.uniform(2, 6, n))
+    H = m.dense()
+    ev, V = np.linalg.eigh(H)
+    psi = V[:, 0]
+    rho = m.one_rdm(psi)
+    assert abs(np.trace(rho[0]) - 3) < 1e-12 and abs(np.trace(rho[1]) - 2) < 1e-12
+    ni = m.ni(psi)
+    # E0 = sum_JK T(J,K) (rho_up + rho_dn)(J,K) + sum U ni  (Hellmann-Feynman / virial identity)
+    e = np.sum(T * (rho[0] + rho[1])) + np.sum(m.U * ni)
+    assert abs(e - ev[0]) < 1e-11
+    # dE/dU_3 by central finite differences equals ni[3]
+    h = 1e-5
+    Up, Um = m.U.copy(), m.U.copy()
+    Up[3] += h
+    Um[3] -= h
+    ep = np.linalg.eigvalsh(O.HubbardModel(n, 3, 2, T, Up).dense())[0]
+    em = np.linalg.eigvalsh(O.HubbardModel(n, 3, 2, T, Um).dense())[0]
+    assert abs((ep - em) / (2 * h) - ni[3]) < 1e-7
+    # total spin: S^2 = sum_JK <S_J.S_K> = S(S+1) for the Sz = 1/2 ground state when it is a doublet
+    s2 = np.sum(m.sisj(psi))
+    S = (-1 + math.sqrt(1 + 4 * s2)) / 2
+    assert abs(S - round(S * 2) / 2) < 1e-8
+
+
+def test_oracle_excitation_sum_rule():
+    """sum_k w_k of G_ii: <c c^dag> + <c^dag c> = 1 per site and spin."""
+    m = O.HubbardModel(6, 3, 3, O.ring_T(6), 3.0)
+    ev, V = np.linalg.eigh(m.dense())
+    psi = V[:, 0]
+    for site in (1, 4):
+        for spin in (0, 1):
+            _, p = m.excite(psi, site, spin, 1)
+            _, h = m.excite(psi, site, spin, 0)
+            assert abs(p @ p + h @ h - 1.0) < 1e-12
+
+
+def test_heisenberg_reference_convention():
+    """The reference Heisenberg HPROD is 2x the standard model on bipartite bond
+    graphs; with frustrating bonds it is not (its flip sign is -(-1)^{J+K})."""
+    for n, frustrate in ((6, False), (6, True), (8, False)):
+        J = np.zeros((n, n))
+        for i in range(n):
+            J[i, (i + 1) % n] = J[(i + 1) % n, i] = 1.0
+            if frustrate:
+                J[i, (i + 2) % n] = J[(i + 2) % n, i] = 0.5
+        e_ref = np.linalg.eigvalsh(O.HeisenbergModel(n, J).dense())[0]
+        e_std = np.linalg.eigvalsh(O.heisenberg_standard_dense(n, np.triu(J)))[0]
+        if frustrate:
+            assert abs(e_ref - 2 * e_std) > 1e-3
+        else:
+            assert abs(e_ref - 2 * e_std) < 1e-10
+
+
+def test_library_exports_header_symbols():
+    """libphub.so loads and exports every function include/phub.h declares."""
+    so = os.path.join(ROOT, "pyhub_amd", "libphub.so")
+    if not os.path.exists(so):
+        pytest.skip("libphub.so not built")
+    hdr = open(os.path.join(ROOT, "include", "phub.h")).read()
+    names = set(re.findall(r"^\s*(?:int|void|double|uint64_t|const char \*)\s*\*?\s*(phub_\w+)\s*\(",
+                           hdr, flags=re.M))
+    assert len(names) >= 20
+    lib = ctypes.CDLL(so)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    from pyhub_amd import _lib
+    assert names <= set(_lib.SIGNATURES), names - set(_lib.SIGNATURES)
