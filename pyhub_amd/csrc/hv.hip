@@ -230,6 +230,8 @@ template <typename T>
 int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double scale,
                    const T *prev, double beta, bool want_dots, int *nblocks_out) {
     int nblocks = 0;
+    // the epilogue kernel applies scale / prev and writes dot partials
+    const bool epi = want_dots || prev != nullptr || scale != 1.0;
     if (op->kind == 0) {
         if (op->n > 32) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "Hubbard H*v needs n <= 32");
         const int ncb = (int)((op->dim_up + 255) / 256);
@@ -237,12 +239,12 @@ int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double sc
         const int64_t nb = (int64_t)ncb * nrg;
         if (nb > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
         nblocks = (int)nb;
-        if (want_dots) {
+        if (epi) {
             int rc = phub_ensure_red(c, (size_t)nblocks * 2 + 64);
             if (rc) return rc;
         }
         HubArgs<T> a = hub_args<T>(op);
-        if (want_dots)
+        if (epi)
             hipLaunchKernelGGL((k_hv_hub<T, HUB_R, true>), dim3(nblocks), dim3(256), 0, c->stream,
                                a, x, out, scale, prev, beta, c->d_red, ncb);
         else
@@ -252,12 +254,12 @@ int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double sc
         const int64_t nb = (op->dim + 255) / 256;
         if (nb > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
         nblocks = (int)nb;
-        if (want_dots) {
+        if (epi) {
             int rc = phub_ensure_red(c, (size_t)nblocks * 2 + 64);
             if (rc) return rc;
         }
         HeisArgs a = heis_args(op);
-        if (want_dots)
+        if (epi)
             hipLaunchKernelGGL((k_hv_heis<T, true>), dim3(nblocks), dim3(256), 0, c->stream, a,
                                x, out, scale, prev, beta, c->d_red);
         else

@@ -500,10 +500,12 @@ static int lanczos_full(phub_ctx *c, const phub_op *op, const phub_lanczos_opts 
         if (rc) return rc;
         for (int j = 0; j <= k; ++j) cf[j] *= sig[j];   // <x_j, out>
         s.alpha[k] = cf[k];
+        // norm left after the local three-term part: the DGKS test compares against it
+        const double nlocal = std::max(nbefore - cf[k] * cf[k], 0.0);
         for (int j = 0; j <= k; ++j) cf[j] *= sig[j];   // coefficients on xt_j
         rc = dev_combine<T, T>(c, basis, ld, k + 1, cf.data(), out, n, false, &nafter, d_coef);
         if (rc) return rc;
-        if (nafter < 0.5 * nbefore) {
+        if (nafter < 0.5 * nlocal) {
             double nb2 = nafter;
             rc = dev_multidot<T, T>(c, basis, ld, k + 1, out, n, cf.data());
             if (rc) return rc;

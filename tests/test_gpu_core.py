@@ -121,7 +121,8 @@ def test_config1_lanczos_modes(ctx, mode):
     assert np.max(np.abs(up - ro[0])) < 1e-12
     ev, V = np.linalg.eigh(m.dense())
     rd = m.one_rdm(V[:, 0])
-    assert np.max(np.abs(up - rd[0])) < 1e-8
+    # psi error ~ residual / gap (gap 0.3035 above E0)
+    assert np.max(np.abs(up - rd[0])) < 1e-10 + 20 * r.residual / 0.3
 
 
 @pytest.mark.parametrize("n,nu,nd,seed", [(6, 3, 2, 20), (7, 3, 3, 21), (8, 4, 3, 22), (6, 3, 3, 23)])
