@@ -196,10 +196,15 @@ __global__ __launch_bounds__(256) void k_second_pass(T *__restrict__ out, const 
 // The block stages its slice of w in LDS; each wave takes basis vectors
 // j = wave, wave+4, ... and reduces its dot with DPP shuffles.
 constexpr int MD_SLICE = 2048;
+// The staged vector is w - a_eff * xk (formed in FP64): projecting the dominant
+// local component out before the global coefficients are taken keeps the
+// reorthogonalisation stable with one pass (otherwise the overlap with older
+// vectors is multiplied by alpha/beta every step).
 template <typename T, typename TW>
 __global__ __launch_bounds__(256) void k_multidot(const T *__restrict__ basis, int64_t ld, int k,
-                                                  const TW *__restrict__ w, int64_t n,
-                                                  double *__restrict__ part) {
+                                                  const TW *__restrict__ w,
+                                                  const T *__restrict__ xk, double a_eff,
+                                                  int64_t n, double *__restrict__ part) {
     __shared__ double ws[MD_SLICE];
     extern __shared__ double acc[];   // k doubles
     for (int j = threadIdx.x; j < k; j += blockDim.x) acc[j] = 0.0;
@@ -209,7 +214,7 @@ __global__ __launch_bounds__(256) void k_multidot(const T *__restrict__ basis, i
         const int len = (int)std::min<int64_t>(MD_SLICE, n - base);
         __syncthreads();
         for (int i = threadIdx.x; i < MD_SLICE; i += blockDim.x)
-            ws[i] = (i < len) ? (double)w[base + i] : 0.0;
+            ws[i] = (i < len) ? (double)w[base + i] - (xk ? a_eff * (double)xk[base + i] : 0.0) : 0.0;
         __syncthreads();
         for (int j = wave; j < k; j += 4) {
             const T *vj = basis + (size_t)j * ld + base;
@@ -288,14 +293,14 @@ static int dev_axpby(phub_ctx *c, double a, const TX *x, double b, TY *y, int64_
 
 template <typename T, typename TW>
 static int dev_multidot(phub_ctx *c, const T *basis, int64_t ld, int k, const TW *w, int64_t n,
-                        double *out) {
+                        double *out, const T *xk = nullptr, double a_eff = 0.0) {
     int64_t slices = (n + MD_SLICE - 1) / MD_SLICE;
     unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(slices, 1024));
     int rc = phub_ensure_red(c, (size_t)g * k + k + 64);
     if (rc) return rc;
     if (k > 4096) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "too many basis vectors");
     hipLaunchKernelGGL((k_multidot<T, TW>), dim3(g), dim3(256), k * sizeof(double), c->stream,
-                       basis, ld, k, w, n, c->d_red);
+                       basis, ld, k, w, xk, a_eff, n, c->d_red);
     PHUB_HIP(c, hipGetLastError());
     // reduce k components; host buffer may need to grow
     if ((size_t)k > c->h_red_cap) {
@@ -494,18 +499,20 @@ static int lanczos_full(phub_ctx *c, const phub_op *op, const phub_lanczos_opts 
         double dots[2];
         rc = fused_hv<T>(c, op, xk, sig[k], xp, k ? s.beta[k] * sig[k - 1] : 0.0, out, dots, tm);
         if (rc) return rc;
-        // CGS against x_0..x_k (alpha_k = <x_k, out> comes out of the same pass)
+        // alpha from the fused pass; global coefficients on (out - alpha x_k)
+        const double a0 = dots[0];
         double nbefore = dots[1], nafter = 0.0;
-        rc = dev_multidot<T, T>(c, basis, ld, k + 1, out, n, cf.data());
+        rc = dev_multidot<T, T>(c, basis, ld, k + 1, out, n, cf.data(), xk, a0 * sig[k]);
         if (rc) return rc;
-        for (int j = 0; j <= k; ++j) cf[j] *= sig[j];   // <x_j, out>
-        s.alpha[k] = cf[k];
+        for (int j = 0; j <= k; ++j) cf[j] *= sig[j];   // <x_j, out - a0 x_k>
+        s.alpha[k] = a0 + cf[k];
         // norm left after the local three-term part: the DGKS test compares against it
-        const double nlocal = std::max(nbefore - cf[k] * cf[k], 0.0);
+        const double nlocal = std::max(nbefore - a0 * a0, 0.0);
         for (int j = 0; j <= k; ++j) cf[j] *= sig[j];   // coefficients on xt_j
+        cf[k] += a0 * sig[k];
         rc = dev_combine<T, T>(c, basis, ld, k + 1, cf.data(), out, n, false, &nafter, d_coef);
         if (rc) return rc;
-        if (nafter < 0.5 * nlocal) {
+        if (nafter < 0.25 * nlocal) {
             double nb2 = nafter;
             rc = dev_multidot<T, T>(c, basis, ld, k + 1, out, n, cf.data());
             if (rc) return rc;

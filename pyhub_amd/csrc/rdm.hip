@@ -303,11 +303,14 @@ extern "C" int phub_rdm_hubbard(phub_ctx *c, const phub_op *op_, const double *b
     int rc = hubbard_raw(c, op, bra, ket, r, sisj != nullptr);
     if (rc) return rc;
     const int n = op->n;
-    // ONE_RDM (static_rq.f90:161-243): element (J,K), K <= J, mirrored to (K,J)
+    // ONE_RDM (static_rq.f90:161-243): element (J,K), K <= J, mirrored to (K,J);
+    // when N_up = N_dn the reference copies the up matrix into the down one (:212-241)
+    const bool copy_up = (op->nup == op->ndn);
     for (int J = 1; J <= n; ++J)
         for (int K = 1; K <= J; ++K) {
             double vu = (J == K) ? r.dens_up[J - 1] : r.hop_up[(J - 1) * n + (K - 1)];
             double vd = (J == K) ? r.dens_dn[J - 1] : r.hop_dn[(J - 1) * n + (K - 1)];
+            if (copy_up) vd = vu;
             if (rdm_up) rdm_up[(J - 1) * n + (K - 1)] = rdm_up[(K - 1) * n + (J - 1)] = vu;
             if (rdm_dn) rdm_dn[(J - 1) * n + (K - 1)] = rdm_dn[(K - 1) * n + (J - 1)] = vd;
         }
