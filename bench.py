@@ -143,7 +143,7 @@ def main():
                "mean_hops_per_species": hbar},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(gbs / 8000.0, 4), "traffic": None,
-                     "kernel": "k_hv_hub<float,4,false>",
+                     "kernel": "k_hv_up<float,2> + k_hv_dn<float,false> (one H*v)",
                      "valu_fp32_tflops": round(flops / (ms_avg * 1e-3) / 1e12, 2)},
     }
 

@@ -250,7 +250,30 @@ void phub_free_hop_table(HopTable *t) {
     if (t->c64) hipFree(t->c64);
     if (t->c32) hipFree(t->c32);
     if (t->pair) hipFree(t->pair);
+    if (t->packed) hipFree(t->packed);
+    if (t->cls64) hipFree(t->cls64);
+    if (t->cls32) hipFree(t->cls32);
     *t = HopTable();
+}
+
+// compressed entries: (target index << 8) | class of the signed coefficient
+__global__ void k_hop_pack(const int32_t *__restrict__ idx, const double *__restrict__ c64,
+                           const int32_t *__restrict__ cnt, int64_t dim, int maxh,
+                           const double *__restrict__ cls, int ncls,
+                           uint32_t *__restrict__ packed) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= dim) return;
+    const int c = cnt[s];
+    for (int k = 0; k < maxh; ++k) {
+        const size_t o = (size_t)k * dim + s;
+        uint32_t e = 0;
+        if (k < c) {
+            int q = 0;
+            while (q < ncls - 1 && cls[q] != c64[o]) ++q;
+            e = ((uint32_t)idx[o] << 8) | (uint32_t)q;
+        }
+        packed[o] = e;
+    }
 }
 
 int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor,
@@ -324,6 +347,26 @@ int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor
     PHUB_HIP(c, hipGetLastError());
     PHUB_HIP(c, hipStreamSynchronize(c->stream));
     hipFree(d_bj); hipFree(d_bk); hipFree(d_t); hipFree(d_pid); hipFree(d_td); hipFree(d_max);
+    // distinct signed coefficients (+-T(J,K)); compressed table when they fit a byte
+    if (kmajor && !all_pairs && mh > 0 && dim < (1ll << 24)) {
+        std::vector<double> cls;
+        for (double t : tv)
+            for (double sg : {1.0, -1.0})
+                if (std::find(cls.begin(), cls.end(), sg * t) == cls.end()) cls.push_back(sg * t);
+        if (cls.size() <= 255) {
+            out->ncls = (int)cls.size();
+            std::vector<float> cls32(cls.begin(), cls.end());
+            PHUB_HIP(c, hipMalloc(&out->cls64, cls.size() * sizeof(double)));
+            PHUB_HIP(c, hipMalloc(&out->cls32, cls.size() * sizeof(float)));
+            PHUB_HIP(c, hipMemcpy(out->cls64, cls.data(), cls.size() * sizeof(double), hipMemcpyHostToDevice));
+            PHUB_HIP(c, hipMemcpy(out->cls32, cls32.data(), cls.size() * sizeof(float), hipMemcpyHostToDevice));
+            PHUB_HIP(c, hipMalloc(&out->packed, (size_t)dim * mh * sizeof(uint32_t)));
+            hipLaunchKernelGGL(k_hop_pack, dim3(nblk(dim, 256)), dim3(256), 0, c->stream, out->idx,
+                               out->c64, out->cnt, dim, mh, out->cls64, out->ncls, out->packed);
+            PHUB_HIP(c, hipGetLastError());
+            PHUB_HIP(c, hipStreamSynchronize(c->stream));
+        }
+    }
     return PHUB_OK;
 }
 

@@ -126,6 +126,126 @@ __global__ __launch_bounds__(256) void k_hv_hub(HubArgs<T> a, const T *__restric
     if (EPI) block_sum2<T>(s_xo, s_oo, part);
 }
 
+// ---------------------------------------------------------------------------
+// Two-kernel Hubbard H*v (used when R rows of the state fit in LDS):
+//  k_hv_up : up hops + diagonal.  A workgroup stages R rows d0..d0+R-1 in LDS;
+//            each lane walks its columns u, reads each compressed hop entry
+//            (idx << 8 | class) once and gathers from the R staged rows.
+//            out[d][u] = diag * x[d][u] + sum_up.
+//  k_hv_dn : down hops + fused epilogue.  Each XCD owns 64-column slabs of the
+//            state (workgroup b runs on XCD b % 8; slabs are dealt so that the
+//            XCD's concurrent workgroups share one slab, which then stays in
+//            its L2).  A wave takes one row d: its hop list is wave-uniform
+//            (scalar loads) and every gather is a coalesced 64-lane row segment.
+//            out = scale * (out + sum_dn) - beta * prev, dot partials.
+// ---------------------------------------------------------------------------
+constexpr int UP_THREADS = 512;
+constexpr int DN_ROWS = 64;     // rows per workgroup in k_hv_dn
+constexpr int DN_COLS = 64;     // slab width (one wave)
+
+template <typename T, int R>
+__global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
+                                                      const uint32_t *__restrict__ packed,
+                                                      const T *__restrict__ cls, int ncls,
+                                                      const T *__restrict__ x,
+                                                      T *__restrict__ out) {
+    extern __shared__ double smem_d[];
+    T *rows = reinterpret_cast<T *>(smem_d);
+    const int64_t nu = a.dim_up;
+    T *cl = rows + R * nu;
+    const int64_t d0 = (int64_t)blockIdx.x * R;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const bool ok = d0 + r < a.dim_dn;
+        const T *src = x + (d0 + r) * nu;
+        for (int64_t u = threadIdx.x; u < nu; u += UP_THREADS) rows[r * nu + u] = ok ? src[u] : (T)0;
+    }
+    for (int i = threadIdx.x; i < ncls; i += UP_THREADS) cl[i] = cls[i];
+    __syncthreads();
+    uint32_t xd[R];
+    double ed[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const bool ok = d0 + r < a.dim_dn;
+        xd[r] = ok ? a.md[d0 + r] : 0u;
+        ed[r] = ok ? a.ed[d0 + r] : 0.0;
+    }
+    for (int64_t u = threadIdx.x; u < nu; u += UP_THREADS) {
+        const uint32_t xu = a.mu[u];
+        const double eu = a.eu[u];
+        T acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t both = xu & xd[r];
+            double dg = eu + ed[r];
+            if (a.u_uniform) {
+                dg += a.u0 * (double)__popc(both);
+            } else {
+                uint32_t bb = both;
+                while (bb) {
+                    const int b = __ffs(bb) - 1;
+                    dg += a.U[a.n - 1 - b];
+                    bb &= bb - 1;
+                }
+            }
+            acc[r] = (T)dg * rows[r * nu + u];
+        }
+        const int cnt = a.cu[u];
+        for (int k = 0; k < cnt; ++k) {
+            const uint32_t e = packed[(size_t)k * nu + u];
+            const uint32_t j = e >> 8;
+            const T c = cl[e & 255u];
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] += c * rows[r * nu + j];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (d0 + r < a.dim_dn) out[(d0 + r) * nu + u] = acc[r];
+    }
+}
+
+template <typename T, bool EPI>
+__global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict__ x,
+                                               T *__restrict__ out, double scale,
+                                               const T *__restrict__ prev, double beta,
+                                               double *__restrict__ part, int nchunks, int nrb) {
+    const int b = blockIdx.x;
+    const int xcd = b & 7, i = b >> 3;
+    const int chunk = (i / nrb) * 8 + xcd;
+    const int rb = i % nrb;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t nu = a.dim_up;
+    const int64_t col = (int64_t)chunk * DN_COLS + lane;
+    const bool colok = (chunk < nchunks) && (col < nu);
+    double s_xo = 0.0, s_oo = 0.0;
+    if (chunk < nchunks) {
+        for (int rr = wave; rr < DN_ROWS; rr += 4) {
+            const int64_t d = (int64_t)rb * DN_ROWS + rr;
+            if (d >= a.dim_dn) break;
+            const int cd = a.cd[d];
+            const int32_t *idr = a.id + (size_t)d * a.maxh_dn;
+            const T *cdr = a.cfd + (size_t)d * a.maxh_dn;
+            T s = (T)0;
+            if (colok)
+                for (int k = 0; k < cd; ++k) s += cdr[k] * x[(int64_t)idr[k] * nu + col];
+            if (colok) {
+                const size_t o = (size_t)d * nu + col;
+                if (EPI) {
+                    double val = scale * ((double)out[o] + (double)s);
+                    if (prev) val -= beta * (double)prev[o];
+                    const T vt = (T)val;
+                    out[o] = vt;
+                    s_xo += scale * (double)x[o] * (double)vt;
+                    s_oo += (double)vt * (double)vt;
+                } else {
+                    out[o] = out[o] + s;
+                }
+            }
+        }
+    }
+    if (EPI) block_sum2<T>(s_xo, s_oo, part);
+}
+
 struct HeisArgs {
     int64_t dim;
     int n, L, nb;
@@ -232,6 +352,43 @@ int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double sc
     int nblocks = 0;
     // the epilogue kernel applies scale / prev and writes dot partials
     const bool epi = want_dots || prev != nullptr || scale != 1.0;
+    if (op->kind == 0 && op->up.packed) {
+        // two-kernel path: R rows of the state in LDS for the up hops
+        constexpr int R = sizeof(T) == 4 ? 2 : 1;
+        const size_t lds = ((size_t)R * op->dim_up + op->up.ncls) * sizeof(T);
+        if (lds <= 150 * 1024) {
+            HubArgs<T> a = hub_args<T>(op);
+            static bool attr_set[2] = {false, false};
+            if (!attr_set[sizeof(T) == 8]) {
+                PHUB_HIP(c, hipFuncSetAttribute((const void *)k_hv_up<T, R>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                160 * 1024));
+                attr_set[sizeof(T) == 8] = true;
+            }
+            const T *cls = (sizeof(T) == 8) ? (const T *)op->up.cls64 : (const T *)op->up.cls32;
+            const unsigned g1 = (unsigned)((op->dim_dn + R - 1) / R);
+            hipLaunchKernelGGL((k_hv_up<T, R>), dim3(g1), dim3(UP_THREADS), lds, c->stream, a,
+                               op->up.packed, cls, op->up.ncls, x, out);
+            PHUB_HIP(c, hipGetLastError());
+            const int nchunks = (int)((op->dim_up + DN_COLS - 1) / DN_COLS);
+            const int nrb = (int)((op->dim_dn + DN_ROWS - 1) / DN_ROWS);
+            const int64_t g2 = (int64_t)8 * ((nchunks + 7) / 8) * nrb;
+            if (g2 > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
+            nblocks = (int)g2;
+            if (epi) {
+                int rc = phub_ensure_red(c, (size_t)nblocks * 2 + 64);
+                if (rc) return rc;
+                hipLaunchKernelGGL((k_hv_dn<T, true>), dim3(nblocks), dim3(256), 0, c->stream, a,
+                                   x, out, scale, prev, beta, c->d_red, nchunks, nrb);
+            } else {
+                hipLaunchKernelGGL((k_hv_dn<T, false>), dim3(nblocks), dim3(256), 0, c->stream,
+                                   a, x, out, scale, prev, beta, c->d_red, nchunks, nrb);
+            }
+            PHUB_HIP(c, hipGetLastError());
+            if (nblocks_out) *nblocks_out = nblocks;
+            return PHUB_OK;
+        }
+    }
     if (op->kind == 0) {
         if (op->n > 32) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "Hubbard H*v needs n <= 32");
         const int ncb = (int)((op->dim_up + 255) / 256);

@@ -37,6 +37,12 @@ struct HopTable {
     double *c64 = nullptr;        // device, dim*maxh
     float *c32 = nullptr;         // device, dim*maxh
     int16_t *pair = nullptr;      // device, dim*maxh, (J-1)*n+(K-1) (all-pairs tables only)
+    // compressed k-major entries (idx << 8 | coefficient class) when the signed
+    // hopping values take at most 255 distinct values
+    uint32_t *packed = nullptr;   // device, dim*maxh
+    int ncls = 0;
+    double *cls64 = nullptr;      // device, ncls signed coefficient values
+    float *cls32 = nullptr;
 };
 
 struct HeisTables {
