@@ -139,9 +139,10 @@ __global__ __launch_bounds__(256) void k_hv_hub(HubArgs<T> a, const T *__restric
 //            (scalar loads) and every gather is a coalesced 64-lane row segment.
 //            out = scale * (out + sum_dn) - beta * prev, dot partials.
 // ---------------------------------------------------------------------------
-constexpr int UP_THREADS = 512;
-constexpr int DN_ROWS = 64;     // rows per workgroup in k_hv_dn
-constexpr int DN_COLS = 64;     // slab width (one wave)
+constexpr int UP_THREADS = 1024;
+constexpr int DN_ROWS = 16;     // rows per workgroup in k_hv_dn: many workgroups per slab
+constexpr int UP_UNROLL = 8;    // independent hop-entry loads in flight per lane
+constexpr int DN_UNROLL = 4;    // independent row gathers in flight per lane
 
 template <typename T, int R>
 __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
@@ -191,8 +192,22 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
             acc[r] = (T)dg * rows[r * nu + u];
         }
         const int cnt = a.cu[u];
-        for (int k = 0; k < cnt; ++k) {
-            const uint32_t e = packed[(size_t)k * nu + u];
+        const uint32_t *pe = packed + u;
+        int k = 0;
+        for (; k + UP_UNROLL <= cnt; k += UP_UNROLL) {
+            uint32_t e[UP_UNROLL];
+#pragma unroll
+            for (int q = 0; q < UP_UNROLL; ++q) e[q] = pe[(size_t)(k + q) * nu];
+#pragma unroll
+            for (int q = 0; q < UP_UNROLL; ++q) {
+                const uint32_t j = e[q] >> 8;
+                const T c = cl[e[q] & 255u];
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] += c * rows[r * nu + j];
+            }
+        }
+        for (; k < cnt; ++k) {
+            const uint32_t e = pe[(size_t)k * nu];
             const uint32_t j = e >> 8;
             const T c = cl[e & 255u];
 #pragma unroll
@@ -204,42 +219,52 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
     }
 }
 
-template <typename T, bool EPI>
+// COLS columns per row segment (64 FP32 = 256 B, 32 FP64 = 256 B): a wave covers
+// 64/COLS rows at once.
+template <typename T, bool EPI, int COLS>
 __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict__ x,
                                                T *__restrict__ out, double scale,
                                                const T *__restrict__ prev, double beta,
                                                double *__restrict__ part, int nchunks, int nrb) {
+    constexpr int RPW = 64 / COLS;
     const int b = blockIdx.x;
     const int xcd = b & 7, i = b >> 3;
     const int chunk = (i / nrb) * 8 + xcd;
     const int rb = i % nrb;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int sub = lane / COLS, cidx = lane % COLS;
     const int64_t nu = a.dim_up;
-    const int64_t col = (int64_t)chunk * DN_COLS + lane;
+    const int64_t col = (int64_t)chunk * COLS + cidx;
     const bool colok = (chunk < nchunks) && (col < nu);
     double s_xo = 0.0, s_oo = 0.0;
-    if (chunk < nchunks) {
-        for (int rr = wave; rr < DN_ROWS; rr += 4) {
+    if (colok) {
+        for (int rr = wave * RPW + sub; rr < DN_ROWS; rr += 4 * RPW) {
             const int64_t d = (int64_t)rb * DN_ROWS + rr;
             if (d >= a.dim_dn) break;
             const int cd = a.cd[d];
             const int32_t *idr = a.id + (size_t)d * a.maxh_dn;
             const T *cdr = a.cfd + (size_t)d * a.maxh_dn;
+            const T *xc = x + col;
             T s = (T)0;
-            if (colok)
-                for (int k = 0; k < cd; ++k) s += cdr[k] * x[(int64_t)idr[k] * nu + col];
-            if (colok) {
-                const size_t o = (size_t)d * nu + col;
-                if (EPI) {
-                    double val = scale * ((double)out[o] + (double)s);
-                    if (prev) val -= beta * (double)prev[o];
-                    const T vt = (T)val;
-                    out[o] = vt;
-                    s_xo += scale * (double)x[o] * (double)vt;
-                    s_oo += (double)vt * (double)vt;
-                } else {
-                    out[o] = out[o] + s;
-                }
+            int k = 0;
+            for (; k + DN_UNROLL <= cd; k += DN_UNROLL) {
+                T v[DN_UNROLL];
+#pragma unroll
+                for (int q = 0; q < DN_UNROLL; ++q) v[q] = xc[(int64_t)idr[k + q] * nu];
+#pragma unroll
+                for (int q = 0; q < DN_UNROLL; ++q) s += cdr[k + q] * v[q];
+            }
+            for (; k < cd; ++k) s += cdr[k] * xc[(int64_t)idr[k] * nu];
+            const size_t o = (size_t)d * nu + col;
+            if (EPI) {
+                double val = scale * ((double)out[o] + (double)s);
+                if (prev) val -= beta * (double)prev[o];
+                const T vt = (T)val;
+                out[o] = vt;
+                s_xo += scale * (double)x[o] * (double)vt;
+                s_oo += (double)vt * (double)vt;
+            } else {
+                out[o] = out[o] + s;
             }
         }
     }
@@ -370,7 +395,8 @@ int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double sc
             hipLaunchKernelGGL((k_hv_up<T, R>), dim3(g1), dim3(UP_THREADS), lds, c->stream, a,
                                op->up.packed, cls, op->up.ncls, x, out);
             PHUB_HIP(c, hipGetLastError());
-            const int nchunks = (int)((op->dim_up + DN_COLS - 1) / DN_COLS);
+            constexpr int COLS = sizeof(T) == 4 ? 64 : 32;
+            const int nchunks = (int)((op->dim_up + COLS - 1) / COLS);
             const int nrb = (int)((op->dim_dn + DN_ROWS - 1) / DN_ROWS);
             const int64_t g2 = (int64_t)8 * ((nchunks + 7) / 8) * nrb;
             if (g2 > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
@@ -378,10 +404,10 @@ int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double sc
             if (epi) {
                 int rc = phub_ensure_red(c, (size_t)nblocks * 2 + 64);
                 if (rc) return rc;
-                hipLaunchKernelGGL((k_hv_dn<T, true>), dim3(nblocks), dim3(256), 0, c->stream, a,
+                hipLaunchKernelGGL((k_hv_dn<T, true, COLS>), dim3(nblocks), dim3(256), 0, c->stream, a,
                                    x, out, scale, prev, beta, c->d_red, nchunks, nrb);
             } else {
-                hipLaunchKernelGGL((k_hv_dn<T, false>), dim3(nblocks), dim3(256), 0, c->stream,
+                hipLaunchKernelGGL((k_hv_dn<T, false, COLS>), dim3(nblocks), dim3(256), 0, c->stream,
                                    a, x, out, scale, prev, beta, c->d_red, nchunks, nrb);
             }
             PHUB_HIP(c, hipGetLastError());
