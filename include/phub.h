@@ -73,6 +73,10 @@ typedef struct {
     int n_sites, n_up, n_dn;
     const double *t;
     const double *u;
+    /* optional down-spin hopping matrix (NULL = t).  No reference counterpart:
+     * the reference uses one T for both spins; a spin-dependent one-body
+     * operator is what the adjoint gradient applies to psi. */
+    const double *t_dn;
 } phub_hubbard_desc;
 
 /* Heisenberg S_z = 0 sector (hprod_heis.f90:1-61): H = sum over ORDERED pairs
@@ -152,6 +156,11 @@ int phub_lanczos(phub_ctx *ctx, const phub_op *op, const phub_lanczos_opts *opts
  * sisj n*n.  Any output may be NULL. */
 int phub_rdm_hubbard(phub_ctx *ctx, const phub_op *op, const double *d_bra, const double *d_ket,
                      double *rdm_up, double *rdm_dn, double *ni, double *sisj);
+/* Spin-resolved transition 1-RDMs g_s[(J-1)*n+(K-1)] = <bra| c^dag_Js c_Ks |ket>
+ * for every ordered (J,K), without ONE_RDM's mirroring and up-copy rule (used by
+ * the adjoint gradient; no reference counterpart).  Host outputs, n*n each. */
+int phub_one_rdm_spin(phub_ctx *ctx, const phub_op *op, const double *d_bra, const double *d_ket,
+                      double *g_up, double *g_dn);
 /* Heisenberg J_SPIN (static_rq_heis.f90:58-116) */
 int phub_rdm_heisenberg(phub_ctx *ctx, const phub_op *op, const double *d_bra,
                         const double *d_ket, double *sisj);
@@ -208,6 +217,11 @@ int phub_cf_eval(phub_ctx *ctx, int n_chains, int nsteps, const double *alpha,
  * d_t: host n*n, d_u: host n. */
 int phub_hf_grad_hubbard(phub_ctx *ctx, const phub_op *op, const double *d_psi,
                          double *grad_t, double *grad_u);
+/* Transition derivatives <bra| dH/dtheta |ket> of the Hubbard operator for every
+ * T(J,K) (grad_t, host n*n) and U_J (grad_u, host n): the contraction the adjoint
+ * gradient of a psi-dependent loss needs (no reference counterpart). */
+int phub_grad_hubbard(phub_ctx *ctx, const phub_op *op, const double *d_bra, const double *d_ket,
+                      double *grad_t, double *grad_u);
 /* dE0/dJ(J,K) for the Heisenberg operator (ordered pairs). */
 int phub_hf_grad_heisenberg(phub_ctx *ctx, const phub_op *op, const double *d_psi,
                             double *grad_j);

@@ -20,7 +20,8 @@ LCZ_PLAIN64, LCZ_FULL64, LCZ_MIXED = 0, 1, 2
 
 class HubbardDesc(C.Structure):
     _fields_ = [("n_sites", C.c_int), ("n_up", C.c_int), ("n_dn", C.c_int),
-                ("t", C.POINTER(C.c_double)), ("u", C.POINTER(C.c_double))]
+                ("t", C.POINTER(C.c_double)), ("u", C.POINTER(C.c_double)),
+                ("t_dn", C.POINTER(C.c_double))]
 
 
 class HeisDesc(C.Structure):
@@ -78,6 +79,7 @@ SIGNATURES = {
                                C.POINTER(LanczosResult), _vp]),
     "phub_rdm_hubbard": (C.c_int, [_vp, _vp, _vp, _vp, _dp, _dp, _dp, _dp]),
     "phub_rdm_heisenberg": (C.c_int, [_vp, _vp, _vp, _vp, _dp]),
+    "phub_one_rdm_spin": (C.c_int, [_vp, _vp, _vp, _vp, _dp, _dp]),
     "phub_excite": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
     "phub_cf_chain": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _dp, _dp, _dp]),
     "phub_batch_hubbard": (C.c_int, [_vp, C.POINTER(Cluster), C.c_int, C.POINTER(BatchOpts),
@@ -85,6 +87,7 @@ SIGNATURES = {
     "phub_cf_eval": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp,
                                C.c_double, _dp]),
     "phub_hf_grad_hubbard": (C.c_int, [_vp, _vp, _vp, _dp, _dp]),
+    "phub_grad_hubbard": (C.c_int, [_vp, _vp, _vp, _vp, _dp, _dp]),
     "phub_hf_grad_heisenberg": (C.c_int, [_vp, _vp, _vp, _dp]),
     "phub_adjoint_minres": (C.c_int, [_vp, _vp, _vp, C.c_double, _vp, _vp, C.c_int, C.c_double,
                                       C.POINTER(C.c_int), _dp]),

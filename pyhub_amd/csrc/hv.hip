@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_hv_hub(HubArgs<T> a, const T *__restric
 constexpr int UP_THREADS = 1024;
 constexpr int DN_ROWS = 16;     // rows per workgroup in k_hv_dn: many workgroups per slab
 constexpr int UP_UNROLL = 8;    // independent hop-entry loads in flight per lane
-constexpr int DN_UNROLL = 4;    // independent row gathers in flight per lane
+constexpr int DN_UNROLL = 8;    // independent row gathers in flight per lane
 
 template <typename T, int R>
 __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
@@ -159,7 +159,8 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
     for (int r = 0; r < R; ++r) {
         const bool ok = d0 + r < a.dim_dn;
         const T *src = x + (d0 + r) * nu;
-        for (int64_t u = threadIdx.x; u < nu; u += UP_THREADS) rows[r * nu + u] = ok ? src[u] : (T)0;
+        for (int64_t u = threadIdx.x; u < nu; u += UP_THREADS)
+            rows[r * nu + u] = ok ? __builtin_nontemporal_load(src + u) : (T)0;
     }
     for (int i = threadIdx.x; i < ncls; i += UP_THREADS) cl[i] = cls[i];
     __syncthreads();
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
         }
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            if (d0 + r < a.dim_dn) out[(d0 + r) * nu + u] = acc[r];
+            if (d0 + r < a.dim_dn) __builtin_nontemporal_store(acc[r], out + (d0 + r) * nu + u);
     }
 }
 
@@ -257,14 +258,15 @@ __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict
             for (; k < cd; ++k) s += cdr[k] * xc[(int64_t)idr[k] * nu];
             const size_t o = (size_t)d * nu + col;
             if (EPI) {
-                double val = scale * ((double)out[o] + (double)s);
-                if (prev) val -= beta * (double)prev[o];
+                // streamed once: non-temporal, so the slab keeps the L2
+                double val = scale * ((double)__builtin_nontemporal_load(out + o) + (double)s);
+                if (prev) val -= beta * (double)__builtin_nontemporal_load(prev + o);
                 const T vt = (T)val;
-                out[o] = vt;
+                __builtin_nontemporal_store(vt, out + o);
                 s_xo += scale * (double)x[o] * (double)vt;
                 s_oo += (double)vt * (double)vt;
             } else {
-                out[o] = out[o] + s;
+                __builtin_nontemporal_store(__builtin_nontemporal_load(out + o) + s, out + o);
             }
         }
     }

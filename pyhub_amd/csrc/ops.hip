@@ -26,6 +26,10 @@ extern "C" int phub_op_hubbard(phub_ctx *c, const phub_hubbard_desc *d, phub_op 
         if (!std::isfinite(d->u[i])) return phub_fail(c, PHUB_ERR_ARG, "U has non-finite entries");
     // fermi_hubbard.py:26-27
     if (!symmetric(d->t, n)) return phub_fail(c, PHUB_ERR_ARG, "t_matrix must be symmetric");
+    const double *tdn = d->t_dn ? d->t_dn : d->t;
+    for (int i = 0; i < n * n; ++i)
+        if (!std::isfinite(tdn[i])) return phub_fail(c, PHUB_ERR_ARG, "t_dn has non-finite entries");
+    if (!symmetric(tdn, n)) return phub_fail(c, PHUB_ERR_ARG, "t_dn must be symmetric");
     PHUB_HIP(c, hipSetDevice(c->device));
     phub_op *op = new phub_op();
     op->kind = 0;
@@ -40,7 +44,7 @@ extern "C" int phub_op_hubbard(phub_ctx *c, const phub_hubbard_desc *d, phub_op 
     for (int i = 1; i < n; ++i)
         if (d->u[i] != d->u[0]) op->u_uniform = 0;
     int rc = phub_build_hop_table(c, n, d->n_up, d->t, true, false, &op->up);
-    if (!rc) rc = phub_build_hop_table(c, n, d->n_dn, d->t, false, false, &op->dn);
+    if (!rc) rc = phub_build_hop_table(c, n, d->n_dn, tdn, false, false, &op->dn);
     if (rc) { phub_op_destroy(op); return rc; }
     op->dim_up = op->up.dim;
     op->dim_dn = op->dn.dim;

@@ -334,14 +334,34 @@ extern "C" int phub_rdm_hubbard(phub_ctx *c, const phub_op *op_, const double *b
     return PHUB_OK;
 }
 
-// Hellmann-Feynman: dE/dT(J,K) = sum_s <c^dag_K c_J>, dE/dT(J,J) = <n_J>, dE/dU_J = NI(J)
-extern "C" int phub_hf_grad_hubbard(phub_ctx *c, const phub_op *op_, const double *psi,
-                                    double *grad_t, double *grad_u) {
-    if (!c || !op_ || !psi) return phub_fail(c, PHUB_ERR_ARG, "null argument");
+extern "C" int phub_one_rdm_spin(phub_ctx *c, const phub_op *op_, const double *bra,
+                                 const double *ket, double *g_up, double *g_dn) {
+    if (!c || !op_ || !bra || !ket) return phub_fail(c, PHUB_ERR_ARG, "null argument");
     if (op_->kind != 0) return phub_fail(c, PHUB_ERR_ARG, "not a Hubbard operator");
     phub_op *op = const_cast<phub_op *>(op_);
     HubRaw r;
-    int rc = hubbard_raw(c, op, psi, psi, r, false);
+    // hop(b = ket, k = bra)[K][J] = <bra| c^dag_J c_K |ket>
+    int rc = hubbard_raw(c, op, ket, bra, r, false);
+    if (rc) return rc;
+    const int n = op->n;
+    for (int J = 1; J <= n; ++J)
+        for (int K = 1; K <= n; ++K) {
+            const int jk = (J - 1) * n + (K - 1), kj = (K - 1) * n + (J - 1);
+            if (g_up) g_up[jk] = (J == K) ? r.dens_up[J - 1] : r.hop_up[kj];
+            if (g_dn) g_dn[jk] = (J == K) ? r.dens_dn[J - 1] : r.hop_dn[kj];
+        }
+    return PHUB_OK;
+}
+
+// <bra| dH/dT(J,K) |ket> = sum_s <bra| c^dag_K c_J |ket>, <bra| dH/dT(J,J) |ket> = sum_s <n_J>,
+// <bra| dH/dU_J |ket> = <n_Ju n_Jd>.  hubbard_raw(b, k) contracts <k| O |b>.
+extern "C" int phub_grad_hubbard(phub_ctx *c, const phub_op *op_, const double *bra,
+                                 const double *ket, double *grad_t, double *grad_u) {
+    if (!c || !op_ || !bra || !ket) return phub_fail(c, PHUB_ERR_ARG, "null argument");
+    if (op_->kind != 0) return phub_fail(c, PHUB_ERR_ARG, "not a Hubbard operator");
+    phub_op *op = const_cast<phub_op *>(op_);
+    HubRaw r;
+    int rc = hubbard_raw(c, op, ket, bra, r, false);
     if (rc) return rc;
     const int n = op->n;
     if (grad_t)
@@ -353,6 +373,12 @@ extern "C" int phub_hf_grad_hubbard(phub_ctx *c, const phub_op *op_, const doubl
     if (grad_u)
         for (int J = 0; J < n; ++J) grad_u[J] = r.ni[J];
     return PHUB_OK;
+}
+
+// Hellmann-Feynman: dE/dT(J,K) = sum_s <c^dag_K c_J>, dE/dT(J,J) = <n_J>, dE/dU_J = NI(J)
+extern "C" int phub_hf_grad_hubbard(phub_ctx *c, const phub_op *op, const double *psi,
+                                    double *grad_t, double *grad_u) {
+    return phub_grad_hubbard(c, op, psi, psi, grad_t, grad_u);
 }
 
 // ---------------------------------------------------------------------------

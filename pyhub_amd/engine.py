@@ -185,13 +185,16 @@ class _Op:
 class HubbardOp(_Op):
     """Hubbard operator of HPROD (hprod_hubbard.f90:17-118) on one (N_up, N_dn) sector."""
 
-    def __init__(self, ctx: Context, n: int, nup: int, ndn: int, T, U):
+    def __init__(self, ctx: Context, n: int, nup: int, ndn: int, T, U, T_dn=None):
         self.ctx = ctx
         self.n, self.nup, self.ndn = int(n), int(nup), int(ndn)
         self.T = np.ascontiguousarray(np.asarray(T, dtype=np.float64).reshape(n, n))
         U = np.asarray(U, dtype=np.float64)
         self.U = np.ascontiguousarray(np.full(n, float(U)) if U.ndim == 0 else U.reshape(n))
-        d = L.HubbardDesc(self.n, self.nup, self.ndn, _dptr(self.T), _dptr(self.U))
+        self.T_dn = None if T_dn is None else np.ascontiguousarray(
+            np.asarray(T_dn, dtype=np.float64).reshape(n, n))
+        d = L.HubbardDesc(self.n, self.nup, self.ndn, _dptr(self.T), _dptr(self.U),
+                          _dptr(self.T_dn) if self.T_dn is not None else None)
         h = C.c_void_p()
         rc = ctx.lib.phub_op_hubbard(ctx.h, C.byref(d), C.byref(h))
         self.h = h
@@ -213,6 +216,25 @@ class HubbardOp(_Op):
         self.ctx.check(self.ctx.lib.phub_hf_grad_hubbard(self.ctx.h, self.h,
                                                          C.c_void_p(psi.data_ptr()), _dptr(gt),
                                                          _dptr(gu)))
+        return gt.reshape(n, n), gu
+
+    def one_rdm_spin(self, bra: torch.Tensor, ket: torch.Tensor | None = None):
+        """Proper spin-resolved gamma_s[J][K] = <bra| c^dag_J c_K |ket> (0-based J, K)."""
+        ket = bra if ket is None else ket
+        n = self.n
+        gu, gd = np.zeros(n * n), np.zeros(n * n)
+        self.ctx.check(self.ctx.lib.phub_one_rdm_spin(self.ctx.h, self.h, C.c_void_p(bra.data_ptr()),
+                                                      C.c_void_p(ket.data_ptr()), _dptr(gu),
+                                                      _dptr(gd)))
+        return gu.reshape(n, n), gd.reshape(n, n)
+
+    def grad(self, bra: torch.Tensor, ket: torch.Tensor):
+        """<bra| dH/dT(J,K) |ket> (n x n) and <bra| dH/dU_J |ket> (n)."""
+        n = self.n
+        gt, gu = np.zeros(n * n), np.zeros(n)
+        self.ctx.check(self.ctx.lib.phub_grad_hubbard(self.ctx.h, self.h, C.c_void_p(bra.data_ptr()),
+                                                      C.c_void_p(ket.data_ptr()), _dptr(gt),
+                                                      _dptr(gu)))
         return gt.reshape(n, n), gu
 
     def excite(self, psi: torch.Tensor, site: int, spin: int, create: bool, target=None):

@@ -1,0 +1,159 @@
+"""Differentiable ground-state quantities of Hubbard clusters.
+
+No counterpart in the reference (it has no autograd); the derivatives are pinned
+by the reference's own contractions (ONE_RDM / TWO_RDM, static_rq.f90) and checked
+against central finite differences of the engine's energies and RDMs.
+
+* Energy: Hellmann-Feynman, one pass of libphub.so over psi (phub_hf_grad_hubbard)
+      dE0/dT(J,K) = sum_s <c^dag_Ks c_Js>,   dE0/dU_J = <n_Ju n_Jd>,
+  mapped to bond amplitudes t_b (T = -t_b on both ordered pairs of bond b) and
+  on-site energies eps_J = T(J,J).
+* 1-RDM loss L = sum_s ||gamma_s - target_s||_F^2, gamma_s[J][K] = <c^dag_Js c_Ks>:
+  adjoint method.  With G_s = dL/dgamma_s = 2 (gamma_s - target_s) (symmetric),
+      g = dL/dpsi = 2 sum_s sum_JK G_s[J][K] c^dag_Js c_Ks psi
+  is one H*v of an operator with T_up = G_up, T_dn = G_dn, U = 0 (same kernels);
+      (H - E0) lam = P_perp g      (MINRES on the complement of psi, 5 vectors),
+      dL/dtheta = -<lam| dH/dtheta |psi>      (phub_grad_hubbard).
+  No Krylov basis is stored for the adjoint.
+
+`E0Function` / `RDMLossFunction` wrap both as torch.autograd.Functions over the
+parameter tensors (t, eps, U).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from pyhub_amd import _lib as L
+from pyhub_amd.engine import Context, HubbardOp
+
+
+@dataclass
+class ClusterSpec:
+    """Cluster geometry: n sites, bonds (i, j) 0-based, particle numbers."""
+    n: int
+    bonds: list
+    n_up: int
+    n_dn: int
+    lanczos_tol: float = 1e-13
+    max_steps: int = 400
+    minres_tol: float = 1e-12
+
+    def T(self, t, eps):
+        T = np.zeros((self.n, self.n))
+        for b, (i, j) in enumerate(self.bonds):
+            T[i, j] = T[j, i] = -float(t[b])
+        T[np.arange(self.n), np.arange(self.n)] = np.asarray(eps, float)
+        return T
+
+    def U_vec(self, U):
+        U = np.asarray(U, float)
+        return np.full(self.n, float(U)) if U.ndim == 0 else U
+
+
+def solve(spec: ClusterSpec, t, eps, U, ctx: Context | None = None):
+    """FP64 Lanczos with full reorthogonalisation; returns (op, E0, psi) with E0 the
+    Rayleigh quotient of the reconstructed psi."""
+    ctx = ctx or Context.default()
+    op = HubbardOp(ctx, spec.n, spec.n_up, spec.n_dn, spec.T(t, eps), spec.U_vec(U))
+    r = op.lanczos(mode=L.LCZ_FULL64, max_steps=min(spec.max_steps, op.dim), tol=spec.lanczos_tol,
+                   check_every=5, seed=12345)
+    w = op.hv(r.psi)
+    e0 = float(torch.dot(r.psi, w))
+    return op, e0, r.psi
+
+
+def _param_grads(spec: ClusterSpec, gT: np.ndarray, gU: np.ndarray, scalar_u: bool):
+    """Map d/dT(J,K) and d/dU_J to the parameters (t_b, eps_J, U)."""
+    gt = np.array([-(gT[i, j] + gT[j, i]) for (i, j) in spec.bonds])
+    ge = np.diag(gT).copy()
+    gu = float(gU.sum()) if scalar_u else gU.copy()
+    return gt, ge, gu
+
+
+def hellmann_feynman(spec: ClusterSpec, t, eps, U, ctx=None):
+    """E0 and (dE/dt_b, dE/deps_J, dE/dU) from one pass over psi."""
+    op, e0, psi = solve(spec, t, eps, U, ctx)
+    gT, gU = op.hf_grad(psi)
+    gt, ge, gu = _param_grads(spec, gT, gU, np.ndim(U) == 0)
+    return e0, gt, ge, gu
+
+
+def rdm_loss_and_grad(spec: ClusterSpec, t, eps, U, target_up, target_dn, ctx=None):
+    """L = sum_s ||gamma_s - target_s||_F^2 and dL/d(t, eps, U) by the adjoint solve."""
+    ctx = ctx or Context.default()
+    op, e0, psi = solve(spec, t, eps, U, ctx)
+    n = spec.n
+    gam_up, gam_dn = op.one_rdm_spin(psi)
+    tu, td = np.asarray(target_up, float), np.asarray(target_dn, float)
+    L_val = float(np.sum((gam_up - tu) ** 2) + np.sum((gam_dn - td) ** 2))
+    Gu = 2.0 * (gam_up - tu)
+    Gd = 2.0 * (gam_dn - td)
+    Gu = 0.5 * (Gu + Gu.T)
+    Gd = 0.5 * (Gd + Gd.T)
+    gop = HubbardOp(ctx, n, spec.n_up, spec.n_dn, Gu, np.zeros(n), T_dn=Gd)
+    g = 2.0 * gop.hv(psi)
+    lam, iters, rr = op.minres(psi, e0, g, max_iter=3000, tol=spec.minres_tol)
+    gT, gU = op.grad(lam, psi)
+    gt, ge, gu = _param_grads(spec, -gT, -gU, np.ndim(U) == 0)
+    info = {"minres_iters": iters, "minres_rel_res": rr, "e0": e0,
+            "gamma_up": gam_up, "gamma_dn": gam_dn}
+    return L_val, gt, ge, gu, info
+
+
+class E0Function(torch.autograd.Function):
+    """E0(t, eps, U) with the Hellmann-Feynman backward."""
+
+    @staticmethod
+    def forward(ctx, t, eps, U, spec):
+        e0, gt, ge, gu = hellmann_feynman(spec, t.detach().cpu().numpy(),
+                                          eps.detach().cpu().numpy(), U.detach().cpu().numpy())
+        ctx.save_for_backward(torch.as_tensor(gt), torch.as_tensor(ge),
+                              torch.as_tensor(np.asarray(gu, float)))
+        return torch.tensor(e0, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, go):
+        gt, ge, gu = ctx.saved_tensors
+        return go * gt, go * ge, go * gu, None
+
+
+class RDMLossFunction(torch.autograd.Function):
+    """L(t, eps, U) = sum_s ||gamma_s - target_s||^2 with the adjoint backward."""
+
+    @staticmethod
+    def forward(ctx, t, eps, U, spec, target_up, target_dn):
+        L_val, gt, ge, gu, _ = rdm_loss_and_grad(spec, t.detach().cpu().numpy(),
+                                                 eps.detach().cpu().numpy(),
+                                                 U.detach().cpu().numpy(), target_up, target_dn)
+        ctx.save_for_backward(torch.as_tensor(gt), torch.as_tensor(ge),
+                              torch.as_tensor(np.asarray(gu, float)))
+        return torch.tensor(L_val, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, go):
+        gt, ge, gu = ctx.saved_tensors
+        return go * gt, go * ge, go * gu, None, None, None
+
+
+def ground_energy(t, eps, U, spec: ClusterSpec):
+    return E0Function.apply(t, eps, U, spec)
+
+
+def rdm_loss(t, eps, U, spec: ClusterSpec, target_up, target_dn):
+    return RDMLossFunction.apply(t, eps, U, spec, target_up, target_dn)
+
+
+def lattice_bonds(Lx, Ly, periodic=False):
+    """Nearest-neighbour bonds of an Lx x Ly lattice, site s = x + Lx*y."""
+    b = []
+    for y in range(Ly):
+        for x in range(Lx):
+            s = x + Lx * y
+            if x + 1 < Lx or periodic:
+                b.append((s, (x + 1) % Lx + Lx * y))
+            if y + 1 < Ly or periodic:
+                b.append((s, x + Lx * ((y + 1) % Ly)))
+    return b

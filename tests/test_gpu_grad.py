@@ -1,0 +1,138 @@
+"""GPU tests of the gradients: Hellmann-Feynman energy gradients and the adjoint
+1-RDM-loss gradient against central finite differences (h = 1e-5, FP64) and
+against the oracle's reduced quantities."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def small_spec(nup=3, ndn=3):
+    from pyhub_amd.grad import ClusterSpec, lattice_bonds
+    return ClusterSpec(n=6, bonds=lattice_bonds(3, 2), n_up=nup, n_dn=ndn)
+
+
+def params(spec, seed):
+    rng = np.random.default_rng(seed)
+    t = rng.uniform(0.8, 1.2, len(spec.bonds))
+    eps = rng.normal(0, 0.2, spec.n)
+    U = rng.uniform(4, 8)
+    return t, eps, U
+
+
+@pytest.mark.parametrize("nup,ndn", [(3, 3), (3, 2)])
+def test_hellmann_feynman_vs_fd_and_oracle(ctx, nup, ndn):
+    from pyhub_amd.grad import hellmann_feynman, solve
+    spec = small_spec(nup, ndn)
+    t, eps, U = params(spec, 1)
+    e0, gt, ge, gu = hellmann_feynman(spec, t, eps, U)
+    # oracle: dense ED energy and its RDM contraction (same formulas, CPU)
+    m = O.HubbardModel(spec.n, nup, ndn, spec.T(t, eps), U)
+    ev, V = np.linalg.eigh(m.dense())
+    assert abs(e0 - ev[0]) < 1e-10
+    rho = m.one_rdm(V[:, 0])
+    ni = m.ni(V[:, 0])
+    assert abs(gu - ni.sum()) < 1e-8
+    assert np.max(np.abs(ge - (np.diag(rho[0]) + np.diag(rho[1])))) < 1e-8
+    h = 1e-5
+    for b in range(len(spec.bonds)):
+        tp, tm = t.copy(), t.copy()
+        tp[b] += h
+        tm[b] -= h
+        fd = (solve(spec, tp, eps, U)[1] - solve(spec, tm, eps, U)[1]) / (2 * h)
+        assert abs(fd - gt[b]) < 1e-6, (b, fd, gt[b])
+    for i in (0, 4):
+        ep, em = eps.copy(), eps.copy()
+        ep[i] += h
+        em[i] -= h
+        fd = (solve(spec, t, ep, U)[1] - solve(spec, t, em, U)[1]) / (2 * h)
+        assert abs(fd - ge[i]) < 1e-6
+    fd = (solve(spec, t, eps, U + h)[1] - solve(spec, t, eps, U - h)[1]) / (2 * h)
+    assert abs(fd - gu) < 1e-6
+
+
+@pytest.mark.parametrize("nup,ndn", [(3, 3), (3, 2)])
+def test_rdm_loss_adjoint_vs_fd(ctx, nup, ndn):
+    from pyhub_amd.grad import rdm_loss_and_grad, solve
+    spec = small_spec(nup, ndn)
+    t, eps, U = params(spec, 2)
+    rng = np.random.default_rng(7)
+    tt, et, Ut = t + rng.normal(0, 0.05, len(t)), eps + rng.normal(0, 0.05, spec.n), U + rng.normal(0, 0.05)
+    op_t, _, psi_t = solve(spec, tt, et, Ut)
+    tg_up, tg_dn = op_t.one_rdm_spin(psi_t)
+    L0, gt, ge, gu, info = rdm_loss_and_grad(spec, t, eps, U, tg_up, tg_dn)
+    assert info["minres_rel_res"] < 1e-9
+    # spin-resolved RDMs equal the oracle's ONE_RDM of the same psi
+    m = O.HubbardModel(spec.n, nup, ndn, spec.T(t, eps), U)
+    ev, V = np.linalg.eigh(m.dense())
+    rho = m.one_rdm(V[:, 0])
+    assert np.max(np.abs(info["gamma_up"] - rho[0])) < 1e-8
+
+    def loss(tq, eq, Uq):
+        op, _, psi = solve(spec, tq, eq, Uq)
+        gu_, gd_ = op.one_rdm_spin(psi)
+        return np.sum((gu_ - tg_up) ** 2) + np.sum((gd_ - tg_dn) ** 2)
+
+    h = 1e-5
+    scale = max(1.0, np.max(np.abs(gt)))
+    for b in (0, 3, len(spec.bonds) - 1):
+        tp, tm = t.copy(), t.copy()
+        tp[b] += h
+        tm[b] -= h
+        fd = (loss(tp, eps, U) - loss(tm, eps, U)) / (2 * h)
+        assert abs(fd - gt[b]) < 1e-5 * scale, (b, fd, gt[b])
+    ep, em = eps.copy(), eps.copy()
+    ep[2] += h
+    em[2] -= h
+    fd = (loss(t, ep, U) - loss(t, em, U)) / (2 * h)
+    assert abs(fd - ge[2]) < 1e-5 * scale
+    fd = (loss(t, eps, U + h) - loss(t, eps, U - h)) / (2 * h)
+    assert abs(fd - gu) < 1e-5 * scale
+
+
+def test_autograd_functions(ctx):
+    from pyhub_amd.grad import ground_energy, hellmann_feynman
+    spec = small_spec()
+    t, eps, U = params(spec, 3)
+    tt = torch.tensor(t, requires_grad=True)
+    et = torch.tensor(eps, requires_grad=True)
+    Ut = torch.tensor(U, requires_grad=True)
+    E = ground_energy(tt, et, Ut, spec)
+    E.backward()
+    e0, gt, ge, gu = hellmann_feynman(spec, t, eps, U)
+    assert abs(E.item() - e0) < 1e-12
+    assert np.allclose(tt.grad.numpy(), gt) and np.allclose(et.grad.numpy(), ge)
+    assert abs(Ut.grad.item() - gu) < 1e-12
+
+
+def test_config4_geometry_single_set(ctx):
+    """3x4 open-boundary cluster at half filling (dimension 924^2 = 853,776):
+    adjoint RDM-loss gradient against central differences for two bonds."""
+    from pyhub_amd.grad import ClusterSpec, lattice_bonds, rdm_loss_and_grad, solve
+    spec = ClusterSpec(n=12, bonds=lattice_bonds(4, 3), n_up=6, n_dn=6, lanczos_tol=1e-12)
+    assert len(spec.bonds) == 17
+    rng = np.random.default_rng([4, 0])
+    t = rng.uniform(0.8, 1.2, 17)
+    U = rng.uniform(4, 8)
+    eps = rng.normal(0, 0.2, 12)
+    r1 = np.random.default_rng([4, 0, 1])
+    op_t, _, psi_t = solve(spec, t + r1.normal(0, 0.05, 17), eps + r1.normal(0, 0.05, 12),
+                           U + r1.normal(0, 0.05))
+    tg_up, tg_dn = op_t.one_rdm_spin(psi_t)
+    L0, gt, ge, gu, info = rdm_loss_and_grad(spec, t, eps, U, tg_up, tg_dn)
+
+    def loss(tq):
+        op, _, psi = solve(spec, tq, eps, U)
+        a, b = op.one_rdm_spin(psi)
+        return np.sum((a - tg_up) ** 2) + np.sum((b - tg_dn) ** 2)
+
+    h = 1e-5
+    for b in (0, 9):
+        tp, tm = t.copy(), t.copy()
+        tp[b] += h
+        tm[b] -= h
+        fd = (loss(tp) - loss(tm)) / (2 * h)
+        assert abs(fd - gt[b]) < 1e-4 * max(1.0, abs(gt[b])), (b, fd, gt[b])
