@@ -251,6 +251,8 @@ void phub_free_hop_table(HopTable *t) {
     if (t->c32) hipFree(t->c32);
     if (t->pair) hipFree(t->pair);
     if (t->packed) hipFree(t->packed);
+    if (t->rowptr) hipFree(t->rowptr);
+    if (t->csr) hipFree(t->csr);
     if (t->cls64) hipFree(t->cls64);
     if (t->cls32) hipFree(t->cls32);
     *t = HopTable();
@@ -273,6 +275,23 @@ __global__ void k_hop_pack(const int32_t *__restrict__ idx, const double *__rest
             e = ((uint32_t)idx[o] << 8) | (uint32_t)q;
         }
         packed[o] = e;
+    }
+}
+
+// row-major table -> CSR of compressed entries
+__global__ void k_hop_pack_csr(const int32_t *__restrict__ idx, const double *__restrict__ c64,
+                               const int32_t *__restrict__ cnt, const int32_t *__restrict__ rowptr,
+                               int64_t dim, int maxh, const double *__restrict__ cls, int ncls,
+                               uint32_t *__restrict__ csr) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= dim) return;
+    const int c = cnt[s];
+    const int32_t base = rowptr[s];
+    for (int k = 0; k < c; ++k) {
+        const size_t o = (size_t)s * maxh + k;
+        int q = 0;
+        while (q < ncls - 1 && cls[q] != c64[o]) ++q;
+        csr[base + k] = ((uint32_t)idx[o] << 8) | (uint32_t)q;
     }
 }
 
@@ -348,7 +367,7 @@ int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor
     PHUB_HIP(c, hipStreamSynchronize(c->stream));
     hipFree(d_bj); hipFree(d_bk); hipFree(d_t); hipFree(d_pid); hipFree(d_td); hipFree(d_max);
     // distinct signed coefficients (+-T(J,K)); compressed table when they fit a byte
-    if (kmajor && !all_pairs && mh > 0 && dim < (1ll << 24)) {
+    if (!all_pairs && mh > 0 && dim < (1ll << 24)) {
         std::vector<double> cls;
         for (double t : tv)
             for (double sg : {1.0, -1.0})
@@ -360,9 +379,22 @@ int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor
             PHUB_HIP(c, hipMalloc(&out->cls32, cls.size() * sizeof(float)));
             PHUB_HIP(c, hipMemcpy(out->cls64, cls.data(), cls.size() * sizeof(double), hipMemcpyHostToDevice));
             PHUB_HIP(c, hipMemcpy(out->cls32, cls32.data(), cls.size() * sizeof(float), hipMemcpyHostToDevice));
-            PHUB_HIP(c, hipMalloc(&out->packed, (size_t)dim * mh * sizeof(uint32_t)));
-            hipLaunchKernelGGL(k_hop_pack, dim3(nblk(dim, 256)), dim3(256), 0, c->stream, out->idx,
-                               out->c64, out->cnt, dim, mh, out->cls64, out->ncls, out->packed);
+            if (kmajor) {
+                PHUB_HIP(c, hipMalloc(&out->packed, (size_t)dim * mh * sizeof(uint32_t)));
+                hipLaunchKernelGGL(k_hop_pack, dim3(nblk(dim, 256)), dim3(256), 0, c->stream,
+                                   out->idx, out->c64, out->cnt, dim, mh, out->cls64, out->ncls,
+                                   out->packed);
+            } else {
+                std::vector<int32_t> hcnt(dim), rp(dim + 1, 0);
+                PHUB_HIP(c, hipMemcpy(hcnt.data(), out->cnt, dim * sizeof(int32_t), hipMemcpyDeviceToHost));
+                for (int64_t s = 0; s < dim; ++s) rp[s + 1] = rp[s] + hcnt[s];
+                PHUB_HIP(c, hipMalloc(&out->rowptr, (dim + 1) * sizeof(int32_t)));
+                PHUB_HIP(c, hipMalloc(&out->csr, std::max<int64_t>(rp[dim], 1) * sizeof(uint32_t)));
+                PHUB_HIP(c, hipMemcpy(out->rowptr, rp.data(), (dim + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+                hipLaunchKernelGGL(k_hop_pack_csr, dim3(nblk(dim, 256)), dim3(256), 0, c->stream,
+                                   out->idx, out->c64, out->cnt, out->rowptr, dim, mh, out->cls64,
+                                   out->ncls, out->csr);
+            }
             PHUB_HIP(c, hipGetLastError());
             PHUB_HIP(c, hipStreamSynchronize(c->stream));
         }

@@ -1,7 +1,511 @@
-// batch.hip -- placeholder translation unit for the batched cluster solver.
+// batch.hip -- many small Hubbard clusters in one persistent launch.
+//
+// Work: for every cluster a ground state (Lanczos, FP32 vectors, FP64 recurrence
+// scalars, psi0 rebuilt by a second pass), then for every site i and spin s the
+// particle chain c^dag_is psi0 and the hole chain c_is psi0 (continued-fraction
+// Lanczos, the single-vector case of GF_BANDLANCZOS, spgf_bandlanczos.f90:309-448;
+// excitation signs as GF_E_* / GF_H_*, spgf_ed.f90:304-429).
+//
+// One launch, a grid no larger than the device can hold resident.  Workgroups
+// pull tasks from an atomic counter: all ground-state tasks first (largest
+// cluster first), then all chains.  A chain waits on its cluster's ground-state
+// flag; the flag is set by a workgroup that is already running (ground-state
+// tasks never wait), and the wait is bounded: after the spin limit the workgroup
+// records an error and leaves, so every wave reaches the exit.
 #include "phub_internal.h"
 
-extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *, int, const phub_batch_opts *,
-                                  double *, double *, double *, double *) {
-    return phub_fail(c, PHUB_ERR_UNSUPPORTED, "batched solver not built yet");
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace {
+
+constexpr int BT = 1024;          // threads per workgroup
+constexpr int MAXN = 16;          // sites per cluster in the batched path
+constexpr int MAXSTEPS = 256;     // Lanczos steps per task
+
+struct SectorDev {
+    int64_t nu, nd;
+    int nup, ndn;
+    const uint32_t *mu, *md;
+    const double *eu, *ed;
+    const int32_t *cu, *iu;
+    const float *cfu;
+    const int32_t *cd, *id;
+    const float *cfd;
+    int maxh_dn;
+};
+
+struct ClusterDev {
+    int n;
+    int u_uniform;
+    double u0;
+    double U[MAXN];
+    SectorDev sec[5];      // 0 ground, 1 up-1, 2 up+1, 3 dn-1, 4 dn+1
+    float *psi0;           // ground-state vector (sec[0] dimension)
+    int64_t psi_off;
+};
+
+struct Task {
+    int cluster;
+    int kind;              // 0 ground state, 1 chain
+    int site, spin, create;
+    int slot;              // output slot of the chain
+};
+
+struct BatchDev {
+    const ClusterDev *cl;
+    const Task *tasks;
+    int ntasks;
+    int gs_steps, gf_steps;
+    uint64_t seed;
+    unsigned int *counter;
+    int *gs_done;
+    int *error;
+    float *scratch;        // per workgroup: 3 vectors of dmax floats
+    int64_t dmax;
+    double *e0;            // per cluster
+    double *ga, *gb, *gn;  // chain outputs [slot * gf_steps], norm2 [slot]
+    const uint64_t *binom;
+    double *tri;           // per workgroup FP64 scratch: alpha/beta of up to MAXSTEPS
+    long long spin_limit;
+};
+
+__device__ inline uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// workgroup-wide sum of two doubles (BT threads, 16 waves)
+__device__ inline void wg_sum2(double &a, double &b, double *sh) {
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+    }
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) { sh[2 * w] = a; sh[2 * w + 1] = b; }
+    __syncthreads();
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = 0; i < BT / 64; ++i) { s0 += sh[2 * i]; s1 += sh[2 * i + 1]; }
+    a = s0;
+    b = s1;
+}
+
+// out = scale * (H x) - bprev * xp ; returns (<scale x, out>, |out|^2) workgroup-wide
+__device__ void wg_hv(const ClusterDev &c, const SectorDev &s, const float *__restrict__ x,
+                      double scale, const float *__restrict__ xp, double bprev,
+                      float *__restrict__ out, double &d_xo, double &d_oo, double *sh) {
+    const int64_t D = s.nu * s.nd;
+    double a = 0.0, b = 0.0;
+    for (int64_t e = threadIdx.x; e < D; e += BT) {
+        const int64_t d = e / s.nu, u = e - d * s.nu;
+        const uint32_t xu = s.mu[u], xd = s.md[d];
+        double dg = s.eu[u] + s.ed[d];
+        const uint32_t both = xu & xd;
+        if (c.u_uniform) {
+            dg += c.u0 * (double)__popc(both);
+        } else {
+            uint32_t bb = both;
+            while (bb) {
+                const int bt = __ffs(bb) - 1;
+                dg += c.U[c.n - 1 - bt];
+                bb &= bb - 1;
+            }
+        }
+        float acc = (float)dg * x[e];
+        const int cu = s.cu[u];
+        for (int k = 0; k < cu; ++k)
+            acc += s.cfu[(size_t)k * s.nu + u] * x[d * s.nu + s.iu[(size_t)k * s.nu + u]];
+        const int cd = s.cd[d];
+        for (int k = 0; k < cd; ++k)
+            acc += s.cfd[(size_t)d * s.maxh_dn + k] * x[(int64_t)s.id[(size_t)d * s.maxh_dn + k] * s.nu + u];
+        double v = scale * (double)acc;
+        if (xp) v -= bprev * (double)xp[e];
+        const float vt = (float)v;
+        out[e] = vt;
+        a += scale * (double)x[e] * (double)vt;
+        b += (double)vt * (double)vt;
+    }
+    wg_sum2(a, b, sh);
+    d_xo = a;
+    d_oo = b;
+}
+
+// y = y - a * x (x scaled by sx) ; returns |y|^2 workgroup-wide
+__device__ double wg_update(float *__restrict__ y, const float *__restrict__ x, double a, int64_t D,
+                            double *sh) {
+    double s = 0.0, dummy = 0.0;
+    for (int64_t e = threadIdx.x; e < D; e += BT) {
+        const float t = (float)((double)y[e] - a * (double)x[e]);
+        y[e] = t;
+        s += (double)t * (double)t;
+    }
+    wg_sum2(s, dummy, sh);
+    return s;
+}
+
+// Lanczos chain of `steps` steps from v (overwritten).  alpha[k], beta[k] (beta[k]
+// couples k and k+1).  Three buffers rotate; vectors are kept unnormalised with
+// scale sig.  If psi is non-null, the pass accumulates psi += y[k] * x_k.
+__device__ int wg_lanczos(const ClusterDev &c, const SectorDev &s, float *v0, float *b1, float *b2,
+                          int steps, double *alpha, double *beta, double norm0, const double *y,
+                          float *psi, double *sh) {
+    const int64_t D = s.nu * s.nd;
+    float *xp = b1, *xk = v0, *out = b2;
+    double sig = 1.0 / norm0, sigp = 0.0, bk = 0.0;
+    int m = 0;
+    for (int k = 0; k < steps; ++k) {
+        if (psi) {
+            for (int64_t e = threadIdx.x; e < D; e += BT) psi[e] += (float)(y[k] * sig * (double)xk[e]);
+        }
+        if (k == steps - 1 && psi) { m = steps; break; }
+        double dxo, doo;
+        wg_hv(c, s, xk, sig, k ? xp : nullptr, bk * sigp, out, dxo, doo, sh);
+        const double a = dxo;
+        const double nn = wg_update(out, xk, a * sig, D, sh);
+        const double b = sqrt(fmax(nn, 0.0));
+        if (!psi && threadIdx.x == 0) { alpha[k] = a; beta[k] = b; }
+        m = k + 1;
+        // rotate
+        float *t = xp;
+        xp = xk;
+        xk = out;
+        out = t;
+        sigp = sig;
+        bk = psi ? beta[k] : b;
+        sig = (bk > 0.0) ? 1.0 / bk : 0.0;
+        __syncthreads();
+        if (!(b > 1e-10 * fmax(1.0, fabs(a)))) break;   // invariant subspace
+    }
+    return m;
+}
+
+// lowest eigenvector of the tridiagonal (alpha[0..m-1], beta[0..m-2]) by
+// bisection + inverse iteration, single thread (m <= MAXSTEPS)
+__device__ double tri_lowest(int m, const double *al, const double *be, double *y, double *wk) {
+    double lo = al[0], hi = al[0];
+    for (int i = 0; i < m; ++i) {
+        const double r = (i > 0 ? fabs(be[i - 1]) : 0.0) + (i < m - 1 ? fabs(be[i]) : 0.0);
+        lo = fmin(lo, al[i] - r);
+        hi = fmax(hi, al[i] + r);
+    }
+    for (int it = 0; it < 200; ++it) {          // Sturm count: eigenvalues below mid
+        const double mid = 0.5 * (lo + hi);
+        int cnt = 0;
+        double q = al[0] - mid;
+        if (q < 0) ++cnt;
+        for (int i = 1; i < m; ++i) {
+            q = al[i] - mid - be[i - 1] * be[i - 1] / (q != 0.0 ? q : 1e-300);
+            if (q < 0) ++cnt;
+        }
+        if (cnt >= 1) hi = mid; else lo = mid;
+        if (hi - lo <= 1e-15 * fmax(1.0, fabs(mid))) break;
+    }
+    const double lam = 0.5 * (lo + hi) - 1e-13 * fmax(1.0, fabs(lo));
+    // inverse iteration (T - lam) y = y, Thomas algorithm, 3 sweeps
+    for (int i = 0; i < m; ++i) y[i] = 1.0;
+    double *cp = wk, *dp = wk + m;
+    for (int sweep = 0; sweep < 3; ++sweep) {
+        cp[0] = (m > 1 ? be[0] : 0.0) / (al[0] - lam);
+        dp[0] = y[0] / (al[0] - lam);
+        for (int i = 1; i < m; ++i) {
+            const double den = (al[i] - lam) - be[i - 1] * cp[i - 1];
+            cp[i] = (i < m - 1 ? be[i] : 0.0) / den;
+            dp[i] = (y[i] - be[i - 1] * dp[i - 1]) / den;
+        }
+        y[m - 1] = dp[m - 1];
+        for (int i = m - 2; i >= 0; --i) y[i] = dp[i] - cp[i] * y[i + 1];
+        double nrm = 0.0;
+        for (int i = 0; i < m; ++i) nrm += y[i] * y[i];
+        nrm = sqrt(nrm);
+        for (int i = 0; i < m; ++i) y[i] /= nrm;
+    }
+    return 0.5 * (lo + hi);
+}
+
+// excitation c^dag / c of psi0 into sector t (pull form), returns |out|^2
+__device__ double wg_excite(const BatchDev &B, const ClusterDev &c, const SectorDev &g,
+                            const SectorDev &t, int site, int spin, int create, float *out,
+                            double *sh) {
+    const int n = c.n, bit = n - site;
+    const uint32_t bm = 1u << bit;
+    const uint32_t above = (bit + 1 < 32) ? ~((bm << 1) - 1u) : 0u;
+    const double parity = (spin == 0 && (g.ndn & 1)) ? -1.0 : 1.0;
+    const int64_t D = t.nu * t.nd;
+    double s = 0.0, dummy = 0.0;
+    for (int64_t e = threadIdx.x; e < D; e += BT) {
+        const int64_t d = e / t.nu, u = e - d * t.nu;
+        float v = 0.f;
+        if (spin == 0) {
+            const uint32_t xt = t.mu[u];
+            const bool occ = (xt & bm) != 0;
+            if (create ? occ : !occ) {
+                const uint32_t xs = xt ^ bm;
+                const int64_t us = rank_ref_dev(B.binom, xs, n, g.nup);
+                const double ac = (__popc(xs & above) & 1) ? -1.0 : 1.0;
+                v = (float)(parity * ac * (double)c.psi0[d * g.nu + us]);
+            }
+        } else {
+            const uint32_t xt = t.md[d];
+            const bool occ = (xt & bm) != 0;
+            if (create ? occ : !occ) {
+                const uint32_t xs = xt ^ bm;
+                const int64_t ds = rank_ref_dev(B.binom, xs, n, g.ndn);
+                const double ac = (__popc(xs & above) & 1) ? -1.0 : 1.0;
+                v = (float)(ac * (double)c.psi0[ds * g.nu + u]);
+            }
+        }
+        out[e] = v;
+        s += (double)v * (double)v;
+    }
+    wg_sum2(s, dummy, sh);
+    return s;
+}
+
+__global__ __launch_bounds__(BT) void k_batch(BatchDev B) {
+    __shared__ double sh[2 * BT / 64];
+    __shared__ int task_sh;
+    __shared__ int ok_sh;
+    float *w0 = B.scratch + (size_t)blockIdx.x * 3 * B.dmax;
+    float *w1 = w0 + B.dmax, *w2 = w1 + B.dmax;
+    double *tri = B.tri + (size_t)blockIdx.x * 6 * MAXSTEPS;
+    while (true) {
+        if (threadIdx.x == 0) task_sh = (int)atomicAdd(B.counter, 1u);
+        __syncthreads();
+        const int ti = task_sh;
+        __syncthreads();
+        if (ti >= B.ntasks) break;
+        const Task tk = B.tasks[ti];
+        const ClusterDev &c = B.cl[tk.cluster];
+        const SectorDev &g = c.sec[0];
+        const int64_t Dg = g.nu * g.nd;
+        if (tk.kind == 0) {
+            // ground state: random start, plain recurrence, second pass for psi0
+            double n2 = 0.0, dummy = 0.0;
+            for (int64_t e = threadIdx.x; e < Dg; e += BT) {
+                const uint64_t r = mix64(B.seed * 0x9E3779B97F4A7C15ull + (uint64_t)tk.cluster * 0x100000001B3ull + (uint64_t)e);
+                const float v = (float)((double)(r >> 11) * (1.0 / 9007199254740992.0) - 0.5);
+                w0[e] = v;
+                c.psi0[e] = v;   // keep the start vector for the second pass
+                n2 += (double)v * (double)v;
+            }
+            wg_sum2(n2, dummy, sh);
+            const double nrm = sqrt(n2);
+            double *al = tri, *be = tri + MAXSTEPS, *y = tri + 2 * MAXSTEPS, *wk = tri + 3 * MAXSTEPS;
+            const int steps = (int)min<int64_t>(B.gs_steps, Dg);
+            const int m = wg_lanczos(c, g, w0, w1, w2, steps, al, be, nrm, nullptr, nullptr, sh);
+            __syncthreads();
+            if (threadIdx.x == 0) B.e0[tk.cluster] = tri_lowest(m, al, be, y, wk);
+            __syncthreads();
+            // second pass: w0 <- start, psi0 accumulated in w1's place (psi0 holds start)
+            for (int64_t e = threadIdx.x; e < Dg; e += BT) {
+                w0[e] = c.psi0[e];
+                c.psi0[e] = 0.f;
+            }
+            __syncthreads();
+            wg_lanczos(c, g, w0, w1, w2, m, al, be, nrm, y, c.psi0, sh);
+            __syncthreads();
+            // normalise psi0
+            double p2 = 0.0, d2 = 0.0;
+            for (int64_t e = threadIdx.x; e < Dg; e += BT) p2 += (double)c.psi0[e] * (double)c.psi0[e];
+            wg_sum2(p2, d2, sh);
+            const float inv = (float)(1.0 / sqrt(p2));
+            for (int64_t e = threadIdx.x; e < Dg; e += BT) c.psi0[e] *= inv;
+            __threadfence();
+            __syncthreads();
+            if (threadIdx.x == 0) atomicExch(&B.gs_done[tk.cluster], 1);
+        } else {
+            // wait (bounded) for the cluster's ground state
+            if (threadIdx.x == 0) {
+                long long spins = 0;
+                int ok = 1;
+                while (atomicAdd(&B.gs_done[tk.cluster], 0) == 0) {
+                    __builtin_amdgcn_s_sleep(8);
+                    if (++spins > B.spin_limit) { ok = 0; atomicExch(B.error, 1); break; }
+                }
+                ok_sh = ok;
+            }
+            __syncthreads();
+            if (!ok_sh) break;
+            __threadfence();
+            const int sidx = (tk.spin == 0) ? (tk.create ? 2 : 1) : (tk.create ? 4 : 3);
+            const SectorDev &t = c.sec[sidx];
+            const int64_t Dt = t.nu * t.nd;
+            double *ga = B.ga + (size_t)tk.slot * B.gf_steps;
+            double *gb = B.gb + (size_t)tk.slot * B.gf_steps;
+            if (Dt == 0) {
+                if (threadIdx.x == 0) B.gn[tk.slot] = 0.0;
+                for (int k = threadIdx.x; k < B.gf_steps; k += BT) { ga[k] = 0.0; gb[k] = 0.0; }
+                continue;
+            }
+            const double n2 = wg_excite(B, c, g, t, tk.site, tk.spin, tk.create, w0, sh);
+            if (threadIdx.x == 0) B.gn[tk.slot] = n2;
+            for (int k = threadIdx.x; k < B.gf_steps; k += BT) { ga[k] = 0.0; gb[k] = 0.0; }
+            __syncthreads();
+            if (n2 > 0.0) {
+                const int steps = (int)min<int64_t>(B.gf_steps, Dt);
+                wg_lanczos(c, t, w0, w1, w2, steps, ga, gb, sqrt(n2), nullptr, nullptr, sh);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <typename T>
+struct DB {
+    T *p = nullptr;
+    ~DB() {
+        if (p) hipFree(p);
+    }
+};
+
+}  // namespace
+
+extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
+                                  const phub_batch_opts *o, double *e0, double *gf_alpha,
+                                  double *gf_beta, double *gf_norm2) {
+    if (!c || !cls || ncl < 1 || !o || !e0) return phub_fail(c, PHUB_ERR_ARG, "bad argument");
+    if (o->gs_steps < 2 || o->gs_steps > MAXSTEPS || o->gf_steps < 1 || o->gf_steps > MAXSTEPS)
+        return phub_fail(c, PHUB_ERR_ARG, "steps must be in [2, 256]");
+    PHUB_HIP(c, hipSetDevice(c->device));
+    std::vector<phub_op *> ops;
+    auto cleanup = [&]() {
+        for (auto *p : ops) phub_op_destroy(p);
+    };
+    std::vector<ClusterDev> hc(ncl);
+    int64_t dmax = 1, psi_total = 0;
+    for (int i = 0; i < ncl; ++i) {
+        const phub_cluster &q = cls[i];
+        if (q.n_sites < 1 || q.n_sites > MAXN) { cleanup(); return phub_fail(c, PHUB_ERR_UNSUPPORTED, "batched clusters need n <= 16"); }
+        ClusterDev &h = hc[i];
+        std::memset(&h, 0, sizeof(h));
+        h.n = q.n_sites;
+        h.u_uniform = 1;
+        h.u0 = q.u[0];
+        for (int j = 0; j < h.n; ++j) {
+            h.U[j] = q.u[j];
+            if (q.u[j] != q.u[0]) h.u_uniform = 0;
+        }
+        const int secs[5][2] = {{q.n_up, q.n_dn}, {q.n_up - 1, q.n_dn}, {q.n_up + 1, q.n_dn},
+                                {q.n_up, q.n_dn - 1}, {q.n_up, q.n_dn + 1}};
+        for (int s = 0; s < 5; ++s) {
+            SectorDev &sd = h.sec[s];
+            const int nu = secs[s][0], nd = secs[s][1];
+            sd.nup = nu;
+            sd.ndn = nd;
+            if (nu < 0 || nu > q.n_sites || nd < 0 || nd > q.n_sites) continue;   // empty sector
+            phub_hubbard_desc desc{q.n_sites, nu, nd, q.t, q.u, nullptr};
+            phub_op *op = nullptr;
+            int rc = phub_op_hubbard(c, &desc, &op);
+            if (rc) { cleanup(); return rc; }
+            ops.push_back(op);
+            sd.nu = op->dim_up;
+            sd.nd = op->dim_dn;
+            sd.mu = op->up.masks;
+            sd.md = op->dn.masks;
+            sd.eu = op->up.eloc;
+            sd.ed = op->dn.eloc;
+            sd.cu = op->up.cnt;
+            sd.iu = op->up.idx;
+            sd.cfu = op->up.c32;
+            sd.cd = op->dn.cnt;
+            sd.id = op->dn.idx;
+            sd.cfd = op->dn.c32;
+            sd.maxh_dn = std::max(op->dn.maxh, 1);
+            dmax = std::max<int64_t>(dmax, op->dim);
+        }
+        h.psi_off = psi_total;
+        psi_total += std::max<int64_t>(h.sec[0].nu * h.sec[0].nd, 1);
+    }
+    // tasks: ground states (largest first), then chains (largest first)
+    std::vector<Task> tasks;
+    std::vector<int> order(ncl);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+        return hc[a].sec[0].nu * hc[a].sec[0].nd > hc[b].sec[0].nu * hc[b].sec[0].nd;
+    });
+    for (int i : order) tasks.push_back(Task{i, 0, 0, 0, 0, 0});
+    std::vector<Task> chains;
+    for (int i = 0; i < ncl; ++i)
+        for (int site = 1; site <= hc[i].n; ++site)
+            for (int spin = 0; spin < 2; ++spin)
+                for (int kind = 0; kind < 2; ++kind) {
+                    const int slot = ((i * 32 + (site - 1)) * 2 + spin) * 2 + kind;
+                    chains.push_back(Task{i, 1, site, spin, kind == 0 ? 1 : 0, slot});
+                }
+    auto tdim = [&](const Task &t) {
+        const int s = (t.spin == 0) ? (t.create ? 2 : 1) : (t.create ? 4 : 3);
+        return hc[t.cluster].sec[s].nu * hc[t.cluster].sec[s].nd;
+    };
+    std::stable_sort(chains.begin(), chains.end(),
+                     [&](const Task &a, const Task &b) { return tdim(a) > tdim(b); });
+    tasks.insert(tasks.end(), chains.begin(), chains.end());
+    const int nslots = ncl * 32 * 2 * 2;
+    // device buffers
+    int dev_cu = 0;
+    hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, c->device);
+    const int grid = std::max(1, std::min<int>(dev_cu > 0 ? dev_cu : 256, (int)tasks.size()));
+    DB<float> psi, scratch;
+    DB<ClusterDev> dcl;
+    DB<Task> dtasks;
+    DB<unsigned int> counter;
+    DB<int> done, err;
+    DB<double> de0, ga, gb, gn, tri;
+    PHUB_HIP(c, hipMalloc(&psi.p, psi_total * sizeof(float)));
+    for (int i = 0; i < ncl; ++i) hc[i].psi0 = psi.p + hc[i].psi_off;
+    PHUB_HIP(c, hipMalloc(&scratch.p, (size_t)grid * 3 * dmax * sizeof(float)));
+    PHUB_HIP(c, hipMalloc(&dcl.p, ncl * sizeof(ClusterDev)));
+    PHUB_HIP(c, hipMalloc(&dtasks.p, tasks.size() * sizeof(Task)));
+    PHUB_HIP(c, hipMalloc(&counter.p, sizeof(unsigned int)));
+    PHUB_HIP(c, hipMalloc(&done.p, ncl * sizeof(int)));
+    PHUB_HIP(c, hipMalloc(&err.p, sizeof(int)));
+    PHUB_HIP(c, hipMalloc(&de0.p, ncl * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&ga.p, (size_t)nslots * o->gf_steps * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&gb.p, (size_t)nslots * o->gf_steps * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&gn.p, nslots * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&tri.p, (size_t)grid * 6 * MAXSTEPS * sizeof(double)));
+    PHUB_HIP(c, hipMemcpy(dcl.p, hc.data(), ncl * sizeof(ClusterDev), hipMemcpyHostToDevice));
+    PHUB_HIP(c, hipMemcpy(dtasks.p, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice));
+    PHUB_HIP(c, hipMemset(counter.p, 0, sizeof(unsigned int)));
+    PHUB_HIP(c, hipMemset(done.p, 0, ncl * sizeof(int)));
+    PHUB_HIP(c, hipMemset(err.p, 0, sizeof(int)));
+    PHUB_HIP(c, hipMemset(gn.p, 0, nslots * sizeof(double)));
+    PHUB_HIP(c, hipMemset(ga.p, 0, (size_t)nslots * o->gf_steps * sizeof(double)));
+    PHUB_HIP(c, hipMemset(gb.p, 0, (size_t)nslots * o->gf_steps * sizeof(double)));
+    BatchDev B{};
+    B.cl = dcl.p;
+    B.tasks = dtasks.p;
+    B.ntasks = (int)tasks.size();
+    B.gs_steps = o->gs_steps;
+    B.gf_steps = o->gf_steps;
+    B.seed = o->seed;
+    B.counter = counter.p;
+    B.gs_done = done.p;
+    B.error = err.p;
+    B.scratch = scratch.p;
+    B.dmax = dmax;
+    B.e0 = de0.p;
+    B.ga = ga.p;
+    B.gb = gb.p;
+    B.gn = gn.p;
+    B.binom = c->d_binom;
+    B.tri = tri.p;
+    B.spin_limit = 1ll << 28;
+    hipLaunchKernelGGL(k_batch, dim3(grid), dim3(BT), 0, c->stream, B);
+    PHUB_HIP(c, hipGetLastError());
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    int herr = 0;
+    PHUB_HIP(c, hipMemcpy(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost));
+    PHUB_HIP(c, hipMemcpy(e0, de0.p, ncl * sizeof(double), hipMemcpyDeviceToHost));
+    if (gf_alpha) PHUB_HIP(c, hipMemcpy(gf_alpha, ga.p, (size_t)nslots * o->gf_steps * sizeof(double), hipMemcpyDeviceToHost));
+    if (gf_beta) PHUB_HIP(c, hipMemcpy(gf_beta, gb.p, (size_t)nslots * o->gf_steps * sizeof(double), hipMemcpyDeviceToHost));
+    if (gf_norm2) PHUB_HIP(c, hipMemcpy(gf_norm2, gn.p, nslots * sizeof(double), hipMemcpyDeviceToHost));
+    cleanup();
+    if (herr) return phub_fail(c, PHUB_ERR_HIP, "batched launch: a chain waited too long for its ground state");
+    return PHUB_OK;
 }

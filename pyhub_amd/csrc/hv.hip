@@ -31,6 +31,10 @@ struct HubArgs {
     const int32_t *cd, *id;
     const T *cfd;
     int maxh_dn;
+    const int32_t *rp_dn;     // CSR row pointers of the down table (compressed entries)
+    const uint32_t *csr_dn;
+    const T *cls_dn;
+    int ncls_dn;
     int n;
     int u_uniform;
     double u0;
@@ -237,25 +241,32 @@ __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict
     const int64_t nu = a.dim_up;
     const int64_t col = (int64_t)chunk * COLS + cidx;
     const bool colok = (chunk < nchunks) && (col < nu);
+    __shared__ T cls[256];
+    for (int q = threadIdx.x; q < a.ncls_dn; q += 256) cls[q] = a.cls_dn[q];
+    __syncthreads();
     double s_xo = 0.0, s_oo = 0.0;
     if (colok) {
         for (int rr = wave * RPW + sub; rr < DN_ROWS; rr += 4 * RPW) {
             const int64_t d = (int64_t)rb * DN_ROWS + rr;
             if (d >= a.dim_dn) break;
-            const int cd = a.cd[d];
-            const int32_t *idr = a.id + (size_t)d * a.maxh_dn;
-            const T *cdr = a.cfd + (size_t)d * a.maxh_dn;
+            const int32_t r0 = a.rp_dn[d], r1 = a.rp_dn[d + 1];
             const T *xc = x + col;
             T s = (T)0;
-            int k = 0;
-            for (; k + DN_UNROLL <= cd; k += DN_UNROLL) {
+            int32_t k = r0;
+            for (; k + DN_UNROLL <= r1; k += DN_UNROLL) {
+                uint32_t e[DN_UNROLL];
                 T v[DN_UNROLL];
 #pragma unroll
-                for (int q = 0; q < DN_UNROLL; ++q) v[q] = xc[(int64_t)idr[k + q] * nu];
+                for (int q = 0; q < DN_UNROLL; ++q) e[q] = a.csr_dn[k + q];
 #pragma unroll
-                for (int q = 0; q < DN_UNROLL; ++q) s += cdr[k + q] * v[q];
+                for (int q = 0; q < DN_UNROLL; ++q) v[q] = xc[(int64_t)(e[q] >> 8) * nu];
+#pragma unroll
+                for (int q = 0; q < DN_UNROLL; ++q) s += cls[e[q] & 255u] * v[q];
             }
-            for (; k < cd; ++k) s += cdr[k] * xc[(int64_t)idr[k] * nu];
+            for (; k < r1; ++k) {
+                const uint32_t e = a.csr_dn[k];
+                s += cls[e & 255u] * xc[(int64_t)(e >> 8) * nu];
+            }
             const size_t o = (size_t)d * nu + col;
             if (EPI) {
                 // streamed once: non-temporal, so the slab keeps the L2
@@ -343,6 +354,10 @@ static HubArgs<T> hub_args(const phub_op *op) {
         a.cfd = (const T *)op->dn.c32;
     }
     a.maxh_dn = std::max(op->dn.maxh, 1);
+    a.rp_dn = op->dn.rowptr;
+    a.csr_dn = op->dn.csr;
+    a.cls_dn = (sizeof(T) == 8) ? (const T *)op->dn.cls64 : (const T *)op->dn.cls32;
+    a.ncls_dn = op->dn.ncls;
     a.n = op->n;
     a.u_uniform = op->u_uniform;
     a.u0 = op->u0;
@@ -379,7 +394,7 @@ int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double sc
     int nblocks = 0;
     // the epilogue kernel applies scale / prev and writes dot partials
     const bool epi = want_dots || prev != nullptr || scale != 1.0;
-    if (op->kind == 0 && op->up.packed) {
+    if (op->kind == 0 && op->up.packed && op->dn.csr) {
         // two-kernel path: R rows of the state in LDS for the up hops
         constexpr int R = sizeof(T) == 4 ? 2 : 1;
         const size_t lds = ((size_t)R * op->dim_up + op->up.ncls) * sizeof(T);
@@ -397,7 +412,7 @@ int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double sc
             hipLaunchKernelGGL((k_hv_up<T, R>), dim3(g1), dim3(UP_THREADS), lds, c->stream, a,
                                op->up.packed, cls, op->up.ncls, x, out);
             PHUB_HIP(c, hipGetLastError());
-            constexpr int COLS = sizeof(T) == 4 ? 64 : 32;
+            constexpr int COLS = sizeof(T) == 4 ? 32 : 16;
             const int nchunks = (int)((op->dim_up + COLS - 1) / COLS);
             const int nrb = (int)((op->dim_dn + DN_ROWS - 1) / DN_ROWS);
             const int64_t g2 = (int64_t)8 * ((nchunks + 7) / 8) * nrb;

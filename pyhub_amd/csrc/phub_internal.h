@@ -39,7 +39,9 @@ struct HopTable {
     int16_t *pair = nullptr;      // device, dim*maxh, (J-1)*n+(K-1) (all-pairs tables only)
     // compressed k-major entries (idx << 8 | coefficient class) when the signed
     // hopping values take at most 255 distinct values
-    uint32_t *packed = nullptr;   // device, dim*maxh
+    uint32_t *packed = nullptr;   // device, dim*maxh (k-major tables)
+    int32_t *rowptr = nullptr;    // device, dim+1 (row-major tables: CSR of packed entries)
+    uint32_t *csr = nullptr;      // device, rowptr[dim]
     int ncls = 0;
     double *cls64 = nullptr;      // device, ncls signed coefficient values
     float *cls32 = nullptr;

@@ -1,0 +1,38 @@
+"""GPU tests of the batched persistent launch (many clusters, ground states and
+diagonal Green's functions) against the single-cluster FP64 path."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batched_clusters_vs_single_path(ctx):
+    from pyhub_amd import _lib as L
+    from pyhub_amd.batch import gii_omega, random_cluster, run_batch
+    from pyhub_amd.engine import HubbardOp, cf_eval
+    clusters = [random_cluster(c) for c in range(12)]
+    e0, ga, gb, gn = run_batch(clusters, gs_steps=150, gf_steps=80, seed=3)
+    omega = np.linspace(-15, 15, 256)
+    G = gii_omega(e0, ga, gb, gn, clusters, omega, eta=0.3)
+    for ci, cl in enumerate(clusters):
+        op = HubbardOp(ctx, cl.n, cl.n_up, cl.n_dn, cl.T, cl.U)
+        r = op.lanczos(mode=L.LCZ_FULL64, max_steps=min(300, op.dim), tol=1e-12)
+        assert abs(e0[ci] - r.e0) <= 2e-5 * max(1.0, abs(r.e0)), (ci, e0[ci], r.e0)
+        for site in (1, cl.n):
+            for spin in (0, 1):
+                # sum rule <c c^dag> + <c^dag c> = 1
+                assert abs(gn[ci, site - 1, spin].sum() - 1.0) < 1e-4
+                # G_ii(omega) against FP64 chains from the FP64 ground state
+                Gref = np.zeros(len(omega), dtype=complex)
+                for kind, create, sign in ((0, True, 1.0), (1, False, -1.0)):
+                    nu = cl.n_up + ((1 if create else -1) if spin == 0 else 0)
+                    nd = cl.n_dn + ((1 if create else -1) if spin == 1 else 0)
+                    if not (0 <= nu <= cl.n and 0 <= nd <= cl.n):
+                        continue
+                    tgt, x = op.excite(r.psi, site, spin, create)
+                    a, b, n2 = tgt.cf_chain(x, min(80, tgt.dim), precision=64)
+                    if n2 > 0:
+                        Gref += cf_eval(ctx, a, b, n2, r.e0, sign, omega, 0.3)[0]
+                err = np.max(np.abs(G[ci, site - 1, spin] - Gref)) / max(1e-12, np.max(np.abs(Gref)))
+                assert err < 2e-3, (ci, site, spin, err)
