@@ -119,8 +119,10 @@ enum {
     PHUB_LCZ_PLAIN64 = 0,   /* FP64 vectors, 3-term recurrence, no reorthogonalisation
                                (the reference algorithm), psi by a second pass */
     PHUB_LCZ_FULL64 = 1,    /* FP64 vectors, full reorthogonalisation, stored basis */
-    PHUB_LCZ_MIXED = 2      /* FP32 vectors, FP64 scalars, full reorthogonalisation,
+    PHUB_LCZ_MIXED = 2,     /* FP32 vectors, FP64 scalars, full reorthogonalisation,
                                FP64 psi reconstruction + FP64 refinement steps */
+    PHUB_LCZ_PLAIN32 = 3    /* FP32 vectors (3 resident), FP64 scalars, 3-term
+                               recurrence; for sectors whose Krylov basis cannot be stored */
 };
 typedef struct {
     int mode;               /* PHUB_LCZ_* */

@@ -22,6 +22,7 @@
  *                      Lieb-Wu finite-ring Bethe ansatz.
  *   orc_hprod_heis     bit-exact against the reference Heisenberg HPROD
  *                      (oracle/_ref); 2x standard spectrum on bipartite rings.
+ *   orc_hprod_heis_fast bit-identical to orc_hprod_heis (tests/test_oracle.py).
  *   orc_lanczos        converged E0 against dense FP64 eigh.
  *   orc_one_rdm/...    sum rules (tr rho = N), Hellmann-Feynman vs finite
  *                      differences of E0, brute-force operator matrices.
@@ -261,6 +262,51 @@ static long long colex_rank_ref(int n, int k, u64 x) {
 }
 
 long long orc_rank_ref(int n, int k, u64 x) { return colex_rank_ref(n, k, x); }
+
+/* orc_hprod_heis with the linear search replaced by the closed-form rank (the
+ * basis is complete and in reference order, so both give the same position);
+ * identical arithmetic in identical order, hence bit-identical output. */
+void orc_hprod_heis_fast(int n, long long nsup, const u64 *bup, const double *Jm,
+                         const double *v, double *w) {
+    u64 full = (n >= 64) ? ~(u64)0 : (((u64)1 << n) - 1);
+    for (long long i = 0; i < nsup; ++i) w[i] = 0.0;
+    for (int J = 1; J <= n; ++J) {
+        double jjj = Jm[(J - 1) * n + (J - 1)];
+        if (fabs(jjj) > 1.e-14) {
+            for (long long i = 0; i < nsup; ++i) {
+                u64 bd = full - bup[i];
+                if (is_part(n, J, bup[i]) && !is_part(n, J, bd))
+                    w[i] = w[i] + v[i] * 0.5 * jjj;
+                else if (!is_part(n, J, bup[i]) && is_part(n, J, bd))
+                    w[i] = w[i] + v[i] * 0.5 * jjj;
+            }
+        }
+        for (int K = 1; K <= n; ++K) {
+            double jjk = Jm[(J - 1) * n + (K - 1)];
+            if (fabs(jjk) > 1.e-14 && J != K) {
+                for (long long i = 0; i < nsup; ++i) {
+                    long long nw = kinop(n, bup[i], J, K);
+                    if (nw > 0) {
+                        long long pos = colex_rank_ref(n, n / 2, (u64)nw);
+                        u64 bd = full - bup[i];
+                        double ac = anticom(n, J, bup[i]);
+                        ac = ac * anticom(n, K, (u64)nw);
+                        long long nw2 = kinop(n, bd, K, J);
+                        if ((u64)nw2 == full - bup[pos]) {
+                            ac = ac * anticom(n, J, bd);
+                            ac = ac * anticom(n, K, (u64)nw2);
+                            w[pos] = w[pos] - ac * v[i] * jjk;
+                        }
+                    }
+                }
+            }
+            for (long long i = 0; i < nsup; ++i) {
+                u64 bd = full - bup[i];
+                w[i] = w[i] + v[i] * szf(n, J, bup[i], bd) * szf(n, K, bup[i], bd) * jjk;
+            }
+        }
+    }
+}
 
 typedef struct { long long pos; double c; } hop_t;
 

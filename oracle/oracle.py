@@ -50,6 +50,7 @@ def lib():
         L.orc_hprod_hubbard.argtypes = hub_args
         L.orc_hprod_hubbard_fast.argtypes = hub_args + [C.c_int]
         L.orc_hprod_heis.argtypes = [C.c_int, C.c_longlong, u64p, f64p, f64p, f64p]
+        L.orc_hprod_heis_fast.argtypes = [C.c_int, C.c_longlong, u64p, f64p, f64p, f64p]
         L.orc_tridiag_eig.restype = C.c_int
         L.orc_tridiag_eig.argtypes = [C.c_int, f64p, f64p, C.c_void_p]
         L.orc_lanczos.restype = C.c_int
@@ -229,10 +230,11 @@ class HeisenbergModel:
         self.nsup = len(self.bup)
         self.dim = self.nsup
 
-    def hv(self, v):
+    def hv(self, v, fast=False):
         v = np.ascontiguousarray(v, dtype=np.float64)
         w = np.empty_like(v)
-        lib().orc_hprod_heis(self.n, self.nsup, self.bup, self.Jm, v, w)
+        f = lib().orc_hprod_heis_fast if fast else lib().orc_hprod_heis
+        f(self.n, self.nsup, self.bup, self.Jm, v, w)
         return w
 
     def hv_ref(self, v):
@@ -357,6 +359,59 @@ def heisenberg_standard_dense(n, Jstd):
                 for o in (sx, sy, sz):
                     H += Jstd[i, j] * op(o, i) @ op(o, j)
     return H.real
+
+
+def heisenberg_standard_sz0(n, Jstd):
+    """Standard H = sum_{i<j} J_ij S_i.S_j restricted to S_z = 0 (n even), as a
+    scipy sparse matrix over the masks with n/2 set bits (set bit = spin up,
+    numerical order).  S_i.S_j = S^z_i S^z_j + (S^+_i S^-_j + S^-_i S^+_j)/2:
+    +-1/4 on the diagonal, 1/2 between states that differ by swapping i and j.
+    Independent of any Jordan-Wigner gauge; checked against the full-space
+    Kronecker construction above."""
+    import itertools
+    import scipy.sparse as sp
+    k = n // 2
+    states = np.array(sorted(sum(1 << b for b in c) for c in itertools.combinations(range(n), k)),
+                      dtype=np.int64)
+    D = len(states)
+    diag = np.zeros(D)
+    rows, cols, vals = [], [], []
+    idx = np.arange(D)
+    for i in range(n):
+        for j in range(i + 1, n):
+            Jij = Jstd[i, j]
+            if Jij == 0:
+                continue
+            bi = (states >> i) & 1
+            bj = (states >> j) & 1
+            diag += np.where(bi == bj, 0.25, -0.25) * Jij
+            sel = bi != bj
+            tgt = states[sel] ^ ((1 << i) | (1 << j))
+            rows.append(np.searchsorted(states, tgt))
+            cols.append(idx[sel])
+            vals.append(np.full(int(sel.sum()), 0.5 * Jij))
+    rows.append(idx)
+    cols.append(idx)
+    vals.append(diag)
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(D, D))
+
+
+def j1j2_square(Lx, Ly, J1, J2):
+    """Standard-convention couplings Jstd[i, j] (i<j) of the J1-J2 model on an
+    Lx x Ly torus, site s = x + Lx*y; duplicate bonds of small tori add up."""
+    n = Lx * Ly
+    Jstd = np.zeros((n, n))
+    for y in range(Ly):
+        for x in range(Lx):
+            s = x + Lx * y
+            for dx, dy, Jb in ((1, 0, J1), (0, 1, J1), (1, 1, J2), (1, -1, J2)):
+                t = (x + dx) % Lx + Lx * ((y + dy) % Ly)
+                if t == s:
+                    continue
+                i, j = min(s, t), max(s, t)
+                Jstd[i, j] += Jb
+    return Jstd
 
 
 def lieb_wu_ring(L, nup, ndn, U, t=1.0, I=None, Jq=None, tol=1e-14):

@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libphub.so")
 
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_OOM, ERR_NOCONV, ERR_BREAKDOWN, ERR_HIP, ERR_NORM_LOST = range(8)
-LCZ_PLAIN64, LCZ_FULL64, LCZ_MIXED = 0, 1, 2
+LCZ_PLAIN64, LCZ_FULL64, LCZ_MIXED, LCZ_PLAIN32 = 0, 1, 2, 3
 
 
 class HubbardDesc(C.Structure):

@@ -21,7 +21,8 @@ from pyhub_amd.core.basis import Basis
 from pyhub_amd.core.reduced_quantities import GreenFunction, StaticQuantities
 from pyhub_amd.engine import Context, HeisenbergOp, HubbardOp
 
-_MODES = {"plain64": L.LCZ_PLAIN64, "full64": L.LCZ_FULL64, "mixed": L.LCZ_MIXED}
+_MODES = {"plain64": L.LCZ_PLAIN64, "full64": L.LCZ_FULL64, "mixed": L.LCZ_MIXED,
+          "plain32": L.LCZ_PLAIN32}
 
 
 def boltzmann_weights(val, T, nb_comp_states):

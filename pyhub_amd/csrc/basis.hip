@@ -434,6 +434,19 @@ __global__ void k_rank_tab_B(const uint64_t *__restrict__ binom, int n, int L,
     B[t] = c;
 }
 
+__global__ void k_narrow32(const uint64_t *__restrict__ a, int64_t n, uint32_t *__restrict__ b) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (uint32_t)a[i];
+}
+
+// inverse of A per popcount: headpat[hoff[popc(xh)] + A[xh]] = xh
+__global__ void k_headpat(const uint64_t *__restrict__ A, int nh, const int64_t *__restrict__ hoff,
+                          uint32_t *__restrict__ headpat) {
+    int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= (1ll << nh)) return;
+    headpat[hoff[__popcll((uint64_t)h)] + (int64_t)A[h]] = (uint32_t)h;
+}
+
 void phub_free_heis(HeisTables *t) {
     if (t->masks) hipFree(t->masks);
     if (t->tabA) hipFree(t->tabA);
@@ -444,11 +457,17 @@ void phub_free_heis(HeisTables *t) {
     if (t->bJr) hipFree(t->bJr);
     if (t->bsign) hipFree(t->bsign);
     if (t->bond_ising) hipFree(t->bond_ising);
+    if (t->headpat) hipFree(t->headpat);
+    if (t->hoff) hipFree(t->hoff);
+    if (t->tabA32) hipFree(t->tabA32);
+    if (t->sbonds) hipFree(t->sbonds);
+    if (t->items) hipFree(t->items);
     *t = HeisTables();
 }
 
 int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *out) {
     phub_free_heis(out);
+    // the block H*v packs the low pattern in 20 bits and a chunk number in 12
     if (n < 2 || n > 40) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "Heisenberg needs 2 <= n <= 40");
     int64_t dim = (int64_t)phub_binom(n, n / 2);
     out->n = n;
@@ -463,10 +482,50 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
     hipLaunchKernelGGL(k_rank_tab_B, dim3(nblk((1ll << L) * (nh + 1), 256)), dim3(256), 0,
                        c->stream, c->d_binom, n, L, out->tabB);
     PHUB_HIP(c, hipGetLastError());
-    PHUB_HIP(c, hipMalloc(&out->masks, dim * sizeof(uint64_t)));
-    hipLaunchKernelGGL(k_unrank<uint64_t>, dim3(nblk(dim, 256)), dim3(256), 0, c->stream,
-                       c->d_binom, n, n / 2, dim, out->masks);
-    PHUB_HIP(c, hipGetLastError());
+    // the mask table is only kept when small (reduced quantities use it); the H*v
+    // enumerates blocks of equal low-half pattern instead
+    if (dim <= (1ll << 28)) {
+        PHUB_HIP(c, hipMalloc(&out->masks, dim * sizeof(uint64_t)));
+        hipLaunchKernelGGL(k_unrank<uint64_t>, dim3(nblk(dim, 256)), dim3(256), 0, c->stream,
+                           c->d_binom, n, n / 2, dim, out->masks);
+        PHUB_HIP(c, hipGetLastError());
+    }
+    {
+        std::vector<int64_t> hoff(nh + 2, 0);
+        for (int m = 0; m <= nh; ++m) hoff[m + 1] = hoff[m] + (int64_t)phub_binom(nh, m);
+        PHUB_HIP(c, hipMalloc(&out->hoff, (nh + 2) * sizeof(int64_t)));
+        PHUB_HIP(c, hipMemcpy(out->hoff, hoff.data(), (nh + 2) * sizeof(int64_t), hipMemcpyHostToDevice));
+        PHUB_HIP(c, hipMalloc(&out->headpat, ((size_t)1 << nh) * sizeof(uint32_t)));
+        hipLaunchKernelGGL(k_headpat, dim3(nblk(1ll << nh, 256)), dim3(256), 0, c->stream,
+                           out->tabA, nh, out->hoff, out->headpat);
+        PHUB_HIP(c, hipGetLastError());
+        PHUB_HIP(c, hipMalloc(&out->tabA32, ((size_t)1 << nh) * sizeof(uint32_t)));
+        hipLaunchKernelGGL(k_narrow32, dim3(nblk(1ll << nh, 256)), dim3(256), 0, c->stream,
+                           out->tabA, 1ll << nh, out->tabA32);
+        PHUB_HIP(c, hipGetLastError());
+        // work items: every low pattern xl with a valid high popcount m = n/2 - popc(xl)
+        // owns the contiguous block of C(nh, m) states that ends at C-1-B[m][xl]
+        std::vector<uint64_t> Bh(((size_t)1 << L) * (nh + 1));
+        PHUB_HIP(c, hipMemcpy(Bh.data(), out->tabB, Bh.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        const int k = n / 2;
+        std::vector<std::pair<uint64_t, uint32_t>> blocks;   // (first index, xl)
+        for (uint32_t xl = 0; xl < (1u << L); ++xl) {
+            const int m = k - __builtin_popcount(xl);
+            if (m < 0 || m > nh) continue;
+            const uint64_t len = phub_binom(nh, m);
+            blocks.push_back({(uint64_t)dim - 1 - Bh[((size_t)m << L) + xl] - (len - 1), xl});
+        }
+        std::sort(blocks.begin(), blocks.end());
+        std::vector<uint32_t> items;
+        for (auto &bl : blocks) {
+            const int m = k - __builtin_popcount(bl.second);
+            const uint64_t nch = (phub_binom(nh, m) + HEIS_CHUNK - 1) / HEIS_CHUNK;
+            for (uint64_t ch = 0; ch < nch; ++ch) items.push_back((bl.second << 12) | (uint32_t)ch);
+        }
+        out->nitems = (int64_t)items.size();
+        PHUB_HIP(c, hipMalloc(&out->items, std::max<size_t>(items.size(), 1) * sizeof(uint32_t)));
+        PHUB_HIP(c, hipMemcpy(out->items, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
     // bonds: unordered {J<K} with either J(J,K) or J(K,J) non-zero
     std::vector<int> bj, bk;
     std::vector<double> bJ, bJr, bs, bis;
@@ -496,6 +555,22 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
     }
     int nb = (int)bj.size();
     out->nb = nb;
+    {   // bonds grouped by type for the block H*v (the arrays above keep pair order)
+        std::vector<HeisBond> g[3];
+        for (int b = 0; b < nb; ++b) {
+            const bool jl = bj[b] < L, kl = bk[b] < L;
+            const int ty = (jl && kl) ? 0 : ((!jl && !kl) ? 1 : 2);
+            g[ty].push_back(HeisBond{bj[b], bk[b], bs[b] * bJ[b], bs[b] * bJr[b], 0.5 * bis[b]});
+        }
+        out->ntt = (int)g[0].size();
+        out->nhh = (int)g[1].size();
+        out->nx = (int)g[2].size();
+        std::vector<HeisBond> all;
+        for (auto &v : g) all.insert(all.end(), v.begin(), v.end());
+        PHUB_HIP(c, hipMalloc(&out->sbonds, std::max<size_t>(all.size(), 1) * sizeof(HeisBond)));
+        if (!all.empty())
+            PHUB_HIP(c, hipMemcpy(out->sbonds, all.data(), all.size() * sizeof(HeisBond), hipMemcpyHostToDevice));
+    }
     out->diag_const = dconst;
     size_t nba = std::max(nb, 1);
     PHUB_HIP(c, hipMalloc(&out->bj, nba * sizeof(int)));

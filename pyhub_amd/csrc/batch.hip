@@ -6,12 +6,11 @@
 // Lanczos, the single-vector case of GF_BANDLANCZOS, spgf_bandlanczos.f90:309-448;
 // excitation signs as GF_E_* / GF_H_*, spgf_ed.f90:304-429).
 //
-// One launch, a grid no larger than the device can hold resident.  Workgroups
-// pull tasks from an atomic counter: all ground-state tasks first (largest
-// cluster first), then all chains.  A chain waits on its cluster's ground-state
-// flag; the flag is set by a workgroup that is already running (ground-state
-// tasks never wait), and the wait is bounded: after the spin limit the workgroup
-// records an error and leaves, so every wave reaches the exit.
+// Two launches on one stream, each a persistent grid of one workgroup per CU
+// pulling tasks from an atomic counter: the first solves every ground state
+// (largest cluster first), the second runs every chain (largest sector first).
+// Stream order makes the ground states visible to the chains, so no workgroup
+// ever waits on another.
 #include "phub_internal.h"
 
 #include <algorithm>
@@ -62,15 +61,14 @@ struct BatchDev {
     int gs_steps, gf_steps;
     uint64_t seed;
     unsigned int *counter;
-    int *gs_done;
-    int *error;
     float *scratch;        // per workgroup: 3 vectors of dmax floats
     int64_t dmax;
     double *e0;            // per cluster
     double *ga, *gb, *gn;  // chain outputs [slot * gf_steps], norm2 [slot]
     const uint64_t *binom;
     double *tri;           // per workgroup FP64 scratch: alpha/beta of up to MAXSTEPS
-    long long spin_limit;
+    int first;             // task range of this launch: [first, last)
+    int last;
 };
 
 __device__ inline uint64_t mix64(uint64_t z) {
@@ -270,16 +268,15 @@ __device__ double wg_excite(const BatchDev &B, const ClusterDev &c, const Sector
 __global__ __launch_bounds__(BT) void k_batch(BatchDev B) {
     __shared__ double sh[2 * BT / 64];
     __shared__ int task_sh;
-    __shared__ int ok_sh;
     float *w0 = B.scratch + (size_t)blockIdx.x * 3 * B.dmax;
     float *w1 = w0 + B.dmax, *w2 = w1 + B.dmax;
     double *tri = B.tri + (size_t)blockIdx.x * 6 * MAXSTEPS;
     while (true) {
-        if (threadIdx.x == 0) task_sh = (int)atomicAdd(B.counter, 1u);
+        if (threadIdx.x == 0) task_sh = B.first + (int)atomicAdd(B.counter, 1u);
         __syncthreads();
         const int ti = task_sh;
         __syncthreads();
-        if (ti >= B.ntasks) break;
+        if (ti >= B.last) break;
         const Task tk = B.tasks[ti];
         const ClusterDev &c = B.cl[tk.cluster];
         const SectorDev &g = c.sec[0];
@@ -316,23 +313,8 @@ __global__ __launch_bounds__(BT) void k_batch(BatchDev B) {
             wg_sum2(p2, d2, sh);
             const float inv = (float)(1.0 / sqrt(p2));
             for (int64_t e = threadIdx.x; e < Dg; e += BT) c.psi0[e] *= inv;
-            __threadfence();
             __syncthreads();
-            if (threadIdx.x == 0) atomicExch(&B.gs_done[tk.cluster], 1);
         } else {
-            // wait (bounded) for the cluster's ground state
-            if (threadIdx.x == 0) {
-                long long spins = 0;
-                int ok = 1;
-                while (atomicAdd(&B.gs_done[tk.cluster], 0) == 0) {
-                    __builtin_amdgcn_s_sleep(8);
-                    if (++spins > B.spin_limit) { ok = 0; atomicExch(B.error, 1); break; }
-                }
-                ok_sh = ok;
-            }
-            __syncthreads();
-            if (!ok_sh) break;
-            __threadfence();
             const int sidx = (tk.spin == 0) ? (tk.create ? 2 : 1) : (tk.create ? 4 : 3);
             const SectorDev &t = c.sec[sidx];
             const int64_t Dt = t.nu * t.nd;
@@ -454,16 +436,13 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
     DB<ClusterDev> dcl;
     DB<Task> dtasks;
     DB<unsigned int> counter;
-    DB<int> done, err;
     DB<double> de0, ga, gb, gn, tri;
     PHUB_HIP(c, hipMalloc(&psi.p, psi_total * sizeof(float)));
     for (int i = 0; i < ncl; ++i) hc[i].psi0 = psi.p + hc[i].psi_off;
     PHUB_HIP(c, hipMalloc(&scratch.p, (size_t)grid * 3 * dmax * sizeof(float)));
     PHUB_HIP(c, hipMalloc(&dcl.p, ncl * sizeof(ClusterDev)));
     PHUB_HIP(c, hipMalloc(&dtasks.p, tasks.size() * sizeof(Task)));
-    PHUB_HIP(c, hipMalloc(&counter.p, sizeof(unsigned int)));
-    PHUB_HIP(c, hipMalloc(&done.p, ncl * sizeof(int)));
-    PHUB_HIP(c, hipMalloc(&err.p, sizeof(int)));
+    PHUB_HIP(c, hipMalloc(&counter.p, 2 * sizeof(unsigned int)));
     PHUB_HIP(c, hipMalloc(&de0.p, ncl * sizeof(double)));
     PHUB_HIP(c, hipMalloc(&ga.p, (size_t)nslots * o->gf_steps * sizeof(double)));
     PHUB_HIP(c, hipMalloc(&gb.p, (size_t)nslots * o->gf_steps * sizeof(double)));
@@ -471,9 +450,7 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
     PHUB_HIP(c, hipMalloc(&tri.p, (size_t)grid * 6 * MAXSTEPS * sizeof(double)));
     PHUB_HIP(c, hipMemcpy(dcl.p, hc.data(), ncl * sizeof(ClusterDev), hipMemcpyHostToDevice));
     PHUB_HIP(c, hipMemcpy(dtasks.p, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice));
-    PHUB_HIP(c, hipMemset(counter.p, 0, sizeof(unsigned int)));
-    PHUB_HIP(c, hipMemset(done.p, 0, ncl * sizeof(int)));
-    PHUB_HIP(c, hipMemset(err.p, 0, sizeof(int)));
+    PHUB_HIP(c, hipMemset(counter.p, 0, 2 * sizeof(unsigned int)));
     PHUB_HIP(c, hipMemset(gn.p, 0, nslots * sizeof(double)));
     PHUB_HIP(c, hipMemset(ga.p, 0, (size_t)nslots * o->gf_steps * sizeof(double)));
     PHUB_HIP(c, hipMemset(gb.p, 0, (size_t)nslots * o->gf_steps * sizeof(double)));
@@ -484,9 +461,6 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
     B.gs_steps = o->gs_steps;
     B.gf_steps = o->gf_steps;
     B.seed = o->seed;
-    B.counter = counter.p;
-    B.gs_done = done.p;
-    B.error = err.p;
     B.scratch = scratch.p;
     B.dmax = dmax;
     B.e0 = de0.p;
@@ -495,17 +469,20 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
     B.gn = gn.p;
     B.binom = c->d_binom;
     B.tri = tri.p;
-    B.spin_limit = 1ll << 28;
-    hipLaunchKernelGGL(k_batch, dim3(grid), dim3(BT), 0, c->stream, B);
-    PHUB_HIP(c, hipGetLastError());
+    // launch 1: ground states; launch 2: chains (stream order publishes psi0)
+    for (int phase = 0; phase < 2; ++phase) {
+        B.counter = counter.p + phase;
+        B.first = phase == 0 ? 0 : ncl;
+        B.last = phase == 0 ? ncl : (int)tasks.size();
+        const int g = std::max(1, std::min(grid, B.last - B.first));
+        if (B.last > B.first) hipLaunchKernelGGL(k_batch, dim3(g), dim3(BT), 0, c->stream, B);
+        PHUB_HIP(c, hipGetLastError());
+    }
     PHUB_HIP(c, hipStreamSynchronize(c->stream));
-    int herr = 0;
-    PHUB_HIP(c, hipMemcpy(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost));
     PHUB_HIP(c, hipMemcpy(e0, de0.p, ncl * sizeof(double), hipMemcpyDeviceToHost));
     if (gf_alpha) PHUB_HIP(c, hipMemcpy(gf_alpha, ga.p, (size_t)nslots * o->gf_steps * sizeof(double), hipMemcpyDeviceToHost));
     if (gf_beta) PHUB_HIP(c, hipMemcpy(gf_beta, gb.p, (size_t)nslots * o->gf_steps * sizeof(double), hipMemcpyDeviceToHost));
     if (gf_norm2) PHUB_HIP(c, hipMemcpy(gf_norm2, gn.p, nslots * sizeof(double), hipMemcpyDeviceToHost));
     cleanup();
-    if (herr) return phub_fail(c, PHUB_ERR_HIP, "batched launch: a chain waited too long for its ground state");
     return PHUB_OK;
 }

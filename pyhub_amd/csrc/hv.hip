@@ -11,8 +11,8 @@
 //   they are scalar table loads and fully coalesced vector loads of other rows.
 //   A workgroup owns R rows x 256 columns: each lane reads its up-hop list once
 //   and applies it to R rows.
-// Heisenberg (reference HPROD, pyhub/libs/src/hprod_heis.f90:1-61): one lane per
-//   state, bonds uniform, the flipped state's index from two-half colex tables.
+// Heisenberg (reference HPROD, pyhub/libs/src/hprod_heis.f90:1-61): blocks of
+//   states sharing their low-half pattern, no mask table (k_hv_heis_blk below).
 //
 // Fused Lanczos epilogue (optional): out = scale*(H x) - beta*prev and per-block
 // partial sums <scale*x, out>, <out, out> (FP64) written to the context's
@@ -284,50 +284,129 @@ __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict
     if (EPI) block_sum2<T>(s_xo, s_oo, part);
 }
 
-struct HeisArgs {
-    int64_t dim;
-    int n, L, nb;
+// Heisenberg H*v without a mask table (replaces HPROD, hprod_heis.f90:1-61).
+// A state x splits into its high bits xh = x >> L and low bits xl; its reference
+// index is C-1-B[m][xl]-A[xh] with m = popc(xh).  For a fixed xl the C(n-L, m)
+// states with m = n/2 - popc(xl) are contiguous (inner index A[xh] counted down
+// from the block's last index), and headpat lists their xh in that order.  One
+// workgroup takes HEIS_CHUNK consecutive inner indices of one block:
+//   low-low bonds  flip xl only: the source is the same inner index of another
+//                  block, a coalesced read, and whether the flip exists is
+//                  uniform over the workgroup;
+//   high-high bonds flip xh only: a gather inside the block through A;
+//   cross bonds    flip one bit of each half: the low bit is uniform, so the
+//                  source block and the coefficient are uniform, the inner
+//                  index is A[xh ^ bit].
+// Matrix elements as the mask kernel: pull coefficient bsign*J(K,J) when the
+// target has site J up, bsign*J(J,K) otherwise; diagonal dconst minus half the
+// Ising weight of every antiparallel bond.
+struct HeisBlkArgs {
+    int L, nh, k;
+    int ntt, nhh, nx;
     uint64_t total_m1;   // C(n, n/2) - 1
-    const uint64_t *masks, *tabA, *tabB;
-    const int *bj, *bk;
-    const double *bJ, *bJr, *bsign, *ising;
+    const uint64_t *tabB;
+    const uint32_t *tabA32, *headpat, *items;
+    const int64_t *hoff;
+    const HeisBond *bonds;
     double dconst;
 };
 
 template <typename T, bool EPI>
-__global__ __launch_bounds__(256) void k_hv_heis(HeisArgs a, const T *__restrict__ x,
-                                                 T *__restrict__ out, double scale,
-                                                 const T *__restrict__ prev, double beta,
-                                                 double *__restrict__ part) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double s_xo = 0.0, s_oo = 0.0;
-    if (s < a.dim) {
-        const uint64_t xs = a.masks[s];
-        const uint64_t lmask = (1ull << a.L) - 1ull;
-        double dg = a.dconst;
-        T acc = (T)0;
-        for (int b = 0; b < a.nb; ++b) {
-            const int bj = a.bj[b], bk = a.bk[b];
-            if (((xs >> bj) ^ (xs >> bk)) & 1ull) {
-                dg -= 0.5 * a.ising[b];
-                const uint64_t x2 = xs ^ ((1ull << bj) | (1ull << bk));
-                const uint64_t xh = x2 >> a.L;
-                const uint64_t colex = a.tabA[xh] + a.tabB[((uint64_t)__popcll(xh) << a.L) + (x2 & lmask)];
-                const int64_t r = (int64_t)(a.total_m1 - colex);
-                const double cf = a.bsign[b] * (((xs >> bj) & 1ull) ? a.bJr[b] : a.bJ[b]);
-                acc += (T)cf * x[r];
+__global__ __launch_bounds__(256) void k_hv_heis_blk(HeisBlkArgs a, const T *__restrict__ x,
+                                                     T *__restrict__ out, double scale,
+                                                     const T *__restrict__ prev, double beta,
+                                                     double *__restrict__ part) {
+    constexpr int R = HEIS_CHUNK / 256;
+    const uint32_t item = a.items[blockIdx.x];
+    const uint32_t xl = item >> 12;
+    const int ch = (int)(item & 4095u);
+    const int m = a.k - __popc(xl);
+    const int64_t h0 = a.hoff[m];
+    const int len = (int)(a.hoff[m + 1] - h0);
+    const int L = a.L;
+    const int64_t last = (int64_t)(a.total_m1 - a.tabB[((uint64_t)m << L) + xl]);
+    const T *__restrict__ xb = x + last;   // inner index i lives at xb[-i]
+    int ai[R];
+    bool ok[R];
+    uint32_t xh[R];
+    double dg[R];
+    T acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        ai[r] = ch * HEIS_CHUNK + r * 256 + (int)threadIdx.x;
+        ok[r] = ai[r] < len;
+        xh[r] = ok[r] ? a.headpat[h0 + ai[r]] : 0u;
+        dg[r] = 0.0;
+        acc[r] = (T)0;
+    }
+    double dgu = a.dconst;
+    // low-low bonds
+    for (int b = 0; b < a.ntt; ++b) {
+        const HeisBond bd = a.bonds[b];
+        const uint32_t tj = (xl >> bd.j) & 1u, tk = (xl >> bd.k) & 1u;
+        if (tj != tk) {
+            dgu -= bd.hi;
+            const uint32_t xl2 = xl ^ ((1u << bd.j) | (1u << bd.k));
+            const int64_t last2 = (int64_t)(a.total_m1 - a.tabB[((uint64_t)m << L) + xl2]);
+            const T cf = (T)(tj ? bd.cr : bd.cj);
+            const T *__restrict__ xs = x + last2;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (ok[r]) acc[r] += cf * xs[-ai[r]];
+        }
+    }
+    // high-high bonds
+    for (int b = a.ntt; b < a.ntt + a.nhh; ++b) {
+        const HeisBond bd = a.bonds[b];
+        const int hj = bd.j - L, hk = bd.k - L;
+        const uint32_t f = (1u << hj) | (1u << hk);
+        const T cj = (T)bd.cj, cr = (T)bd.cr;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t vj = (xh[r] >> hj) & 1u, vk = (xh[r] >> hk) & 1u;
+            if (ok[r] && vj != vk) {
+                dg[r] += bd.hi;
+                const uint32_t a2 = a.tabA32[xh[r] ^ f];
+                acc[r] += (vj ? cr : cj) * xb[-(int64_t)a2];
             }
         }
-        acc += (T)dg * x[s];
+    }
+    // cross bonds: site J in the high half, site K in the low half
+    for (int b = a.ntt + a.nhh; b < a.ntt + a.nhh + a.nx; ++b) {
+        const HeisBond bd = a.bonds[b];
+        const uint32_t tb = (xl >> bd.k) & 1u;    // the target's low bit; its high bit must differ
+        const int m2 = tb ? m + 1 : m - 1;         // popcount of the source's high half
+        if (m2 < 0 || m2 > a.nh) continue;         // no lane of this block can flip
+        const int hj = bd.j - L;
+        const int64_t last2 = (int64_t)(a.total_m1 - a.tabB[((uint64_t)m2 << L) + (xl ^ (1u << bd.k))]);
+        const T cf = (T)(tb ? bd.cj : bd.cr);      // target site J up iff tb == 0
+        const T *__restrict__ xs = x + last2;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t hb = (xh[r] >> hj) & 1u;
+            if (ok[r] && hb != tb) {
+                dg[r] += bd.hi;
+                const uint32_t a2 = a.tabA32[xh[r] ^ (1u << hj)];
+                acc[r] += cf * xs[-(int64_t)a2];
+            }
+        }
+    }
+    double s_xo = 0.0, s_oo = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (!ok[r]) continue;
+        const int64_t s = last - ai[r];
+        const T xv = x[s];
+        acc[r] += (T)(dgu - dg[r]) * xv;
         if (EPI) {
-            double val = scale * (double)acc;
+            double val = scale * (double)acc[r];
             if (prev) val -= beta * (double)prev[s];
             const T vt = (T)val;
             out[s] = vt;
-            s_xo += scale * (double)x[s] * (double)vt;
+            s_xo += scale * (double)xv * (double)vt;
             s_oo += (double)vt * (double)vt;
         } else {
-            out[s] = acc;
+            out[s] = acc[r];
         }
     }
     if (EPI) block_sum2<T>(s_xo, s_oo, part);
@@ -365,23 +444,22 @@ static HubArgs<T> hub_args(const phub_op *op) {
     return a;
 }
 
-static HeisArgs heis_args(const phub_op *op) {
+static HeisBlkArgs heis_blk_args(const phub_op *op) {
     const HeisTables &h = op->h;
-    HeisArgs a{};
-    a.dim = h.dim;
-    a.n = h.n;
+    HeisBlkArgs a{};
     a.L = h.L;
-    a.nb = h.nb;
+    a.nh = h.n - h.L;
+    a.k = h.n / 2;
+    a.ntt = h.ntt;
+    a.nhh = h.nhh;
+    a.nx = h.nx;
     a.total_m1 = (uint64_t)h.dim - 1;
-    a.masks = h.masks;
-    a.tabA = h.tabA;
     a.tabB = h.tabB;
-    a.bj = h.bj;
-    a.bk = h.bk;
-    a.bJ = h.bJ;
-    a.bJr = h.bJr;
-    a.bsign = h.bsign;
-    a.ising = h.bond_ising;
+    a.tabA32 = h.tabA32;
+    a.headpat = h.headpat;
+    a.items = h.items;
+    a.hoff = h.hoff;
+    a.bonds = h.sbonds;
     a.dconst = h.diag_const;
     return a;
 }
@@ -451,20 +529,20 @@ int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double sc
             hipLaunchKernelGGL((k_hv_hub<T, HUB_R, false>), dim3(nblocks), dim3(256), 0,
                                c->stream, a, x, out, scale, prev, beta, c->d_red, ncb);
     } else {
-        const int64_t nb = (op->dim + 255) / 256;
+        const int64_t nb = op->h.nitems;
         if (nb > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
         nblocks = (int)nb;
         if (epi) {
             int rc = phub_ensure_red(c, (size_t)nblocks * 2 + 64);
             if (rc) return rc;
         }
-        HeisArgs a = heis_args(op);
+        HeisBlkArgs a = heis_blk_args(op);
         if (epi)
-            hipLaunchKernelGGL((k_hv_heis<T, true>), dim3(nblocks), dim3(256), 0, c->stream, a,
-                               x, out, scale, prev, beta, c->d_red);
+            hipLaunchKernelGGL((k_hv_heis_blk<T, true>), dim3(nblocks), dim3(256), 0, c->stream,
+                               a, x, out, scale, prev, beta, c->d_red);
         else
-            hipLaunchKernelGGL((k_hv_heis<T, false>), dim3(nblocks), dim3(256), 0, c->stream, a,
-                               x, out, scale, prev, beta, c->d_red);
+            hipLaunchKernelGGL((k_hv_heis_blk<T, false>), dim3(nblocks), dim3(256), 0, c->stream,
+                               a, x, out, scale, prev, beta, c->d_red);
     }
     PHUB_HIP(c, hipGetLastError());
     if (nblocks_out) *nblocks_out = nblocks;

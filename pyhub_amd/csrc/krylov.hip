@@ -600,12 +600,68 @@ struct DevBuf {
     }
 };
 
+// plain recurrence with 3 resident vectors of type T (FP64 scalars); psi by a
+// second pass from a kept copy of the start vector
+template <typename T>
+static int plain_mode(phub_ctx *c, const phub_op *op, const phub_lanczos_opts *o,
+                      const phub_lanczos_opts *oo, const double *d_v0, double *d_psi,
+                      LczState &s, EvTimer *tm, bool fixed, phub_lanczos_result *res,
+                      std::chrono::steady_clock::time_point t0, double *t_lcz) {
+    const int64_t n = op->dim;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    DevBuf<T> v0, b1, b2, b3;
+    PHUB_HIP(c, hipMalloc(&b1.p, n * sizeof(T)));
+    PHUB_HIP(c, hipMalloc(&b2.p, n * sizeof(T)));
+    PHUB_HIP(c, hipMalloc(&b3.p, n * sizeof(T)));
+    if (d_v0)
+        hipLaunchKernelGGL(k_convert<T>, dim3(g), dim3(256), 0, c->stream, d_v0, b3.p, n, 1.0);
+    else
+        hipLaunchKernelGGL(k_randvec<T>, dim3(g), dim3(256), 0, c->stream, b3.p, n, o->seed);
+    PHUB_HIP(c, hipGetLastError());
+    if (o->want_psi) {
+        // the recurrence rotates three buffers; keep the start vector for the second pass
+        PHUB_HIP(c, hipMalloc(&v0.p, n * sizeof(T)));
+        PHUB_HIP(c, hipMemcpyAsync(v0.p, b3.p, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    }
+    int rc = lanczos_plain<T>(c, op, oo, b3.p, b1.p, b2.p, s, tm, fixed);
+    if (rc) return rc;
+    *t_lcz = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    res->e0_ritz = res->e0 = s.e0;
+    if (o->want_psi) {
+        double e0 = 0, e1 = 0;
+        std::vector<double> y;
+        if (lowest_ritz(s.steps, s.alpha.data(), s.beta.data(), &e0, &e1, &y))
+            return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
+        rc = plain_second_pass<T>(c, op, v0.p, b1.p, b2.p, b3.p, s, y, d_psi);
+        if (rc) return rc;
+        hipFree(b2.p);
+        b2.p = nullptr;
+        hipFree(b3.p);
+        b3.p = nullptr;
+        double nn = 0;
+        rc = dev_dot<double, double>(c, d_psi, d_psi, n, &nn);
+        if (rc) return rc;
+        if (std::fabs(1.0 - std::sqrt(nn)) > 1e-5) s.norm_lost = true;   // lanczos.f90:74-76
+        rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / std::sqrt(nn),
+                                       d_psi, n, nullptr);
+        if (rc) return rc;
+        PHUB_HIP(c, hipStreamSynchronize(c->stream));
+        res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        DevBuf<double> tmp;
+        PHUB_HIP(c, hipMalloc(&tmp.p, n * sizeof(double)));
+        double erq = 0;   // e0 stays the Ritz value, as the reference reports it
+        rc = rq_residual(c, op, d_psi, tmp.p, &erq, &res->residual);
+        if (rc) return rc;
+    }
+    return PHUB_OK;
+}
+
 extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_opts *o,
                             const double *d_v0, double *alpha, double *beta,
                             phub_lanczos_result *res, double *d_psi) {
     if (!c || !op || !o || !res) return phub_fail(c, PHUB_ERR_ARG, "null argument");
     if (o->max_steps < 1) return phub_fail(c, PHUB_ERR_ARG, "max_steps must be >= 1");
-    if (o->mode < 0 || o->mode > 2) return phub_fail(c, PHUB_ERR_ARG, "unknown Lanczos mode");
+    if (o->mode < 0 || o->mode > 3) return phub_fail(c, PHUB_ERR_ARG, "unknown Lanczos mode");
     if (o->want_psi && !d_psi) return phub_fail(c, PHUB_ERR_ARG, "want_psi needs d_psi");
     std::memset(res, 0, sizeof(*res));
     PHUB_HIP(c, hipSetDevice(c->device));
@@ -625,42 +681,8 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
     double t_lcz = 0.0;
 
     if (o->mode == PHUB_LCZ_PLAIN64) {
-        DevBuf<double> v0, b1, b2, b3;
-        PHUB_HIP(c, hipMalloc(&v0.p, n * sizeof(double)));
-        PHUB_HIP(c, hipMalloc(&b1.p, n * sizeof(double)));
-        PHUB_HIP(c, hipMalloc(&b2.p, n * sizeof(double)));
-        PHUB_HIP(c, hipMalloc(&b3.p, n * sizeof(double)));
-        if (d_v0)
-            PHUB_HIP(c, hipMemcpyAsync(v0.p, d_v0, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        else
-            hipLaunchKernelGGL(k_randvec<double>, dim3(g), dim3(256), 0, c->stream, v0.p, n, o->seed);
-        // the recurrence rotates three buffers; keep v0 intact for the second pass
-        PHUB_HIP(c, hipMemcpyAsync(b3.p, v0.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        rc = lanczos_plain<double>(c, op, &oo, b3.p, b1.p, b2.p, s, &tm, fixed);
+        rc = plain_mode<double>(c, op, o, &oo, d_v0, d_psi, s, &tm, fixed, res, t0, &t_lcz);
         if (rc) return rc;
-        t_lcz = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        if (o->want_psi) {
-            double e0 = 0, e1 = 0;
-            std::vector<double> y;
-            if (lowest_ritz(s.steps, s.alpha.data(), s.beta.data(), &e0, &e1, &y))
-                return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
-            rc = plain_second_pass<double>(c, op, v0.p, b1.p, b2.p, b3.p, s, y, d_psi);
-            if (rc) return rc;
-            double nn = 0;
-            rc = dev_dot<double, double>(c, d_psi, d_psi, n, &nn);
-            if (rc) return rc;
-            if (std::fabs(1.0 - std::sqrt(nn)) > 1e-5) s.norm_lost = true;   // lanczos.f90:74-76
-            rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / std::sqrt(nn),
-                                           d_psi, n, nullptr);
-            if (rc) return rc;
-            PHUB_HIP(c, hipStreamSynchronize(c->stream));
-            res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            rc = rq_residual(c, op, d_psi, b1.p, &res->e0, &res->residual);
-            if (rc) return rc;
-        }
-        res->e0_ritz = s.e0;
-        if (!o->want_psi) res->e0 = s.e0;
-        else res->e0 = s.e0;
     } else if (o->mode == PHUB_LCZ_FULL64) {
         DevBuf<double> basis;
         size_t bytes = (size_t)(oo.max_steps + 1) * n * sizeof(double);
@@ -697,6 +719,9 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
             rc = rq_residual(c, op, d_psi, basis.p, &E, &res->residual);
             if (rc) return rc;
         }
+    } else if (o->mode == PHUB_LCZ_PLAIN32) {
+        rc = plain_mode<float>(c, op, o, &oo, d_v0, d_psi, s, &tm, fixed, res, t0, &t_lcz);
+        if (rc) return rc;
     } else {   // MIXED
         DevBuf<float> basis;
         size_t bytes = (size_t)(oo.max_steps + 1) * n * sizeof(float);

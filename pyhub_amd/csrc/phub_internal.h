@@ -47,6 +47,16 @@ struct HopTable {
     float *cls32 = nullptr;
 };
 
+// one bond {J<K} for the block H*v: bit positions j > k of sites J, K; the pull
+// coefficient for a target whose site J is down (cj) or up (cr); half the Ising weight
+struct HeisBond {
+    int j, k;
+    double cj, cr, hi;
+};
+
+// elements per work item of the block Heisenberg H*v
+constexpr int HEIS_CHUNK = 1024;
+
 struct HeisTables {
     int n = 0;
     int64_t dim = 0;
@@ -62,6 +72,19 @@ struct HeisTables {
     double *bsign = nullptr;       // device, flip sign factor per bond
     double diag_const = 0.0;       // sum of 0.75 J(J,J) + 0.5 * (on-site) terms
     double *bond_ising = nullptr;  // device, Ising weight per bond
+    // block enumeration (no mask table needed): states with the same low-half
+    // pattern xl are contiguous; headpat lists the high-half patterns of each
+    // popcount m in the order of A (headpat[hoff[m] + A[xh]] = xh)
+    uint32_t *headpat = nullptr;   // device, 2^(n-L)
+    int64_t *hoff = nullptr;       // device, n-L+2 offsets
+    uint32_t *tabA32 = nullptr;    // device, A as 32-bit (A < C(n-L, m))
+    // bonds grouped by where their sites lie: [0,ntt) both low half, [ntt,ntt+nhh)
+    // both high half, then cross bonds (site J in the high half, K in the low half)
+    HeisBond *sbonds = nullptr;    // device
+    int ntt = 0, nhh = 0, nx = 0;
+    // work items of the block H*v: (low pattern << 12) | chunk, in memory order
+    uint32_t *items = nullptr;     // device
+    int64_t nitems = 0;
 };
 
 struct phub_op {

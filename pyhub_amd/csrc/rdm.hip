@@ -433,6 +433,8 @@ __global__ __launch_bounds__(256) void k_heis_bonds(int64_t dim, int L, uint64_t
 static int heis_bonds(phub_ctx *c, const phub_op *op, const double *bra, const double *ket,
                       std::vector<double> &h) {
     const HeisTables &t = op->h;
+    if (!t.masks)
+        return phub_fail(c, PHUB_ERR_UNSUPPORTED, "spin correlations need dim <= 2^28 (mask table)");
     const int nb = t.nb;
     DBuf<double> d;
     PHUB_HIP(c, hipMalloc(&d.p, (3 * nb + 1) * sizeof(double)));

@@ -11,13 +11,18 @@ def test_batched_clusters_vs_single_path(ctx):
     from pyhub_amd import _lib as L
     from pyhub_amd.batch import gii_omega, random_cluster, run_batch
     from pyhub_amd.engine import HubbardOp, cf_eval
+    import time
     clusters = [random_cluster(c) for c in range(12)]
+    t0 = time.perf_counter()
     e0, ga, gb, gn = run_batch(clusters, gs_steps=150, gf_steps=80, seed=3)
+    print(f"batched launch: {len(clusters)} clusters in {time.perf_counter() - t0:.2f} s", flush=True)
     omega = np.linspace(-15, 15, 256)
     G = gii_omega(e0, ga, gb, gn, clusters, omega, eta=0.3)
     for ci, cl in enumerate(clusters):
         op = HubbardOp(ctx, cl.n, cl.n_up, cl.n_dn, cl.T, cl.U)
         r = op.lanczos(mode=L.LCZ_FULL64, max_steps=min(300, op.dim), tol=1e-12)
+        print(f"cluster {ci}: n={cl.n} ({cl.n_up},{cl.n_dn}) dim={op.dim} e0 {e0[ci]:.8f} ref {r.e0:.8f}",
+              flush=True)
         assert abs(e0[ci] - r.e0) <= 2e-5 * max(1.0, abs(r.e0)), (ci, e0[ci], r.e0)
         for site in (1, cl.n):
             for spin in (0, 1):

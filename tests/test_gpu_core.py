@@ -99,6 +99,71 @@ def test_heisenberg_hv_vs_oracle(ctx, n, conv, seed):
         assert abs(e[0] - (2 * es[0] + 0.75 * np.trace(J))) < 1e-10
 
 
+@pytest.mark.parametrize("n,seed,dtype", [(5, 40, torch.float64), (12, 41, torch.float64),
+                                          (14, 42, torch.float64), (16, 43, torch.float64),
+                                          (20, 44, torch.float64), (16, 45, torch.float32)])
+def test_heisenberg_block_hv_vs_oracle(ctx, n, seed, dtype):
+    """The block H*v at sizes where blocks of every popcount and all three bond
+    kinds (low-low, high-high, cross) occur, against the oracle's HPROD."""
+    from pyhub_amd.engine import HeisenbergOp
+    rng = np.random.default_rng(seed)
+    J = rng.normal(size=(n, n))
+    J[rng.random((n, n)) < 0.5] = 0.0
+    J = (J + J.T) / 2
+    op = HeisenbergOp(ctx, n, J)
+    m = O.HeisenbergModel(n, J)
+    v = rng.normal(size=m.dim)
+    ref = m.hv(v, fast=True)
+    w = op.hv(dev(v, dtype)).double().cpu().numpy()
+    tol = 1e-12 if dtype == torch.float64 else 2e-5
+    print(f"n={n} dim={m.dim} max|w-ref|={np.max(np.abs(w - ref)):.3e}", flush=True)
+    assert np.max(np.abs(w - ref)) <= tol * max(1.0, np.max(np.abs(ref)))
+
+
+def test_j1j2_4x4_standard_convention_e0(ctx):
+    """J1-J2 on the 4x4 torus (J2/J1 = 0.5) in the standard convention: the
+    ordered-pair matrix carries J/2 per pair (sign_convention=1); E0 against the
+    sparse S_z=0 matrix of sum_{i<j} J_ij S_i.S_j (scipy eigsh), FP64 plain
+    recurrence and FP32 vectors with FP64 scalars."""
+    import scipy.sparse.linalg as sla
+    from pyhub_amd import _lib as L
+    from pyhub_amd.engine import HeisenbergOp
+    Jstd = O.j1j2_square(4, 4, 1.0, 0.5)
+    e_ref = sla.eigsh(O.heisenberg_standard_sz0(16, Jstd), k=1, which="SA")[0][0]
+    Jm = (Jstd + Jstd.T) / 2.0
+    op = HeisenbergOp(ctx, 16, Jm, sign_convention=1)
+    r64 = op.lanczos(mode=L.LCZ_PLAIN64, max_steps=300, tol=1e-12, want_psi=True)
+    r32 = op.lanczos(mode=L.LCZ_PLAIN32, fixed_steps=150, max_steps=150, want_psi=False)
+    print(f"E0 ref {e_ref:.12f} plain64 {r64.e0:.12f} plain32 {r32.e0:.12f}", flush=True)
+    assert abs(r64.e0 - e_ref) < 1e-10 * abs(e_ref)
+    assert r64.residual < 1e-5
+    assert abs(r32.e0 - e_ref) < 1e-5 * abs(e_ref)
+
+
+@pytest.mark.parametrize("Lx,Ly", [(6, 4), (8, 4)])
+def test_heisenberg_large_sector_properties(ctx, Lx, Ly):
+    """J1-J2 tori of 24 and 32 sites (dim 2,704,156 and 601,080,390, where the
+    state index passes 2^29): symmetry <u, H v> = <H u, v>, FP32 vs FP64 H*v,
+    and the reduced-quantity kernels refuse a sector without a mask table."""
+    from pyhub_amd.engine import HeisenbergOp
+    Jstd = O.j1j2_square(Lx, Ly, 1.0, 0.5)
+    n = Lx * Ly
+    print(f"n={n}", flush=True)
+    op = HeisenbergOp(ctx, n, (Jstd + Jstd.T) / 2.0, sign_convention=1)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    u = torch.randn(op.dim, dtype=torch.float64, device="cuda", generator=g)
+    v = torch.randn(op.dim, dtype=torch.float64, device="cuda", generator=g)
+    hu, hv = op.hv(u), op.hv(v)
+    a, b = float(torch.dot(u, hv)), float(torch.dot(hu, v))
+    assert abs(a - b) < 1e-10 * max(1.0, abs(a))
+    h32 = op.hv(v.float()).double()
+    assert float(torch.max(torch.abs(h32 - hv))) < 2e-5 * float(torch.max(torch.abs(hv)))
+    if op.dim > (1 << 28):
+        del hu, h32
+        with pytest.raises(Exception):
+            op.sisj(u, v)
+
+
 CFG1_E0 = -4.603526299989137
 
 

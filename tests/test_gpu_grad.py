@@ -97,9 +97,11 @@ def test_autograd_functions(ctx):
     from pyhub_amd.grad import ground_energy, hellmann_feynman
     spec = small_spec()
     t, eps, U = params(spec, 3)
-    tt = torch.tensor(t, requires_grad=True)
-    et = torch.tensor(eps, requires_grad=True)
-    Ut = torch.tensor(U, requires_grad=True)
+    # float64 leaves: torch.tensor of a Python float defaults to float32, which
+    # would move U by its FP32 rounding
+    tt = torch.tensor(t, dtype=torch.float64, requires_grad=True)
+    et = torch.tensor(eps, dtype=torch.float64, requires_grad=True)
+    Ut = torch.tensor(U, dtype=torch.float64, requires_grad=True)
     E = ground_energy(tt, et, Ut, spec)
     E.backward()
     e0, gt, ge, gu = hellmann_feynman(spec, t, eps, U)
