@@ -154,20 +154,37 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
                                                       const T *__restrict__ cls, int ncls,
                                                       const T *__restrict__ x,
                                                       T *__restrict__ out) {
+    // rows interleaved in LDS: element u of row r at rows[u * R + r], so one gather
+    // of column j fetches all R rows (a single 64-bit LDS read for two FP32 rows)
     extern __shared__ double smem_d[];
     T *rows = reinterpret_cast<T *>(smem_d);
     const int64_t nu = a.dim_up;
     T *cl = rows + R * nu;
     const int64_t d0 = (int64_t)blockIdx.x * R;
+    for (int64_t u = threadIdx.x; u < nu; u += UP_THREADS) {
+        T v[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const bool ok = d0 + r < a.dim_dn;
-        const T *src = x + (d0 + r) * nu;
-        for (int64_t u = threadIdx.x; u < nu; u += UP_THREADS)
-            rows[r * nu + u] = ok ? __builtin_nontemporal_load(src + u) : (T)0;
+        for (int r = 0; r < R; ++r)
+            v[r] = (d0 + r < a.dim_dn) ? __builtin_nontemporal_load(x + (d0 + r) * nu + u) : (T)0;
+        if constexpr (R == 2 && sizeof(T) == 4) {
+            reinterpret_cast<float2 *>(rows)[u] = make_float2(v[0], v[1]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) rows[u * R + r] = v[r];
+        }
     }
     for (int i = threadIdx.x; i < ncls; i += UP_THREADS) cl[i] = cls[i];
     __syncthreads();
+    auto gather = [&](uint32_t j, T *v) {
+        if constexpr (R == 2 && sizeof(T) == 4) {
+            const float2 p = reinterpret_cast<const float2 *>(rows)[j];
+            v[0] = p.x;
+            v[1] = p.y;
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = rows[j * R + r];
+        }
+    };
     uint32_t xd[R];
     double ed[R];
 #pragma unroll
@@ -179,7 +196,8 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
     for (int64_t u = threadIdx.x; u < nu; u += UP_THREADS) {
         const uint32_t xu = a.mu[u];
         const double eu = a.eu[u];
-        T acc[R];
+        T acc[R], self[R];
+        gather((uint32_t)u, self);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const uint32_t both = xu & xd[r];
@@ -194,7 +212,7 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
                     bb &= bb - 1;
                 }
             }
-            acc[r] = (T)dg * rows[r * nu + u];
+            acc[r] = (T)dg * self[r];
         }
         const int cnt = a.cu[u];
         const uint32_t *pe = packed + u;
@@ -205,18 +223,20 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
             for (int q = 0; q < UP_UNROLL; ++q) e[q] = pe[(size_t)(k + q) * nu];
 #pragma unroll
             for (int q = 0; q < UP_UNROLL; ++q) {
-                const uint32_t j = e[q] >> 8;
                 const T c = cl[e[q] & 255u];
+                T v[R];
+                gather(e[q] >> 8, v);
 #pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] += c * rows[r * nu + j];
+                for (int r = 0; r < R; ++r) acc[r] += c * v[r];
             }
         }
         for (; k < cnt; ++k) {
             const uint32_t e = pe[(size_t)k * nu];
-            const uint32_t j = e >> 8;
             const T c = cl[e & 255u];
+            T v[R];
+            gather(e >> 8, v);
 #pragma unroll
-            for (int r = 0; r < R; ++r) acc[r] += c * rows[r * nu + j];
+            for (int r = 0; r < R; ++r) acc[r] += c * v[r];
         }
 #pragma unroll
         for (int r = 0; r < R; ++r)
