@@ -13,6 +13,10 @@ the mixed form (FP32 Krylov vectors, FP64 scalars, full reorthogonalisation,
 FP64 refinement) and the plain FP64 form (the reference's three-term
 recurrence), and times the CPU oracle on the host cores as the reported baseline.
 
+With --workload batch256 it times the batched configuration: 256 random Hubbard
+clusters (6..12 sites), ground state + particle/hole G_ii chains of 120 steps for
+every site and spin in one call, G_ii(omega) on 2048 points (eta = 0.05).
+
 With --workload heis6x6 the script measures instead the largest configuration:
 the spin-1/2 J1-J2 Heisenberg model on the 6x6 torus (J1 = 1, J2 = 0.5, standard
 convention, S_z = 0, dimension C(36,18) = 9,075,135,300): FP32 H*v throughput and
@@ -58,7 +62,7 @@ def main():
     ap.add_argument("--lanczos-tol", type=float, default=1e-9)
     ap.add_argument("--lanczos-max", type=int, default=300)
     ap.add_argument("--calib", action="store_true", help="add a 1 GiB device copy (counter calibration)")
-    ap.add_argument("--workload", default="hubbard4x4", choices=["hubbard4x4", "heis6x6"])
+    ap.add_argument("--workload", default="hubbard4x4", choices=["hubbard4x4", "heis6x6", "batch256"])
     ap.add_argument("--lx", type=int, default=6, help="Heisenberg torus width")
     ap.add_argument("--ly", type=int, default=6, help="Heisenberg torus height")
     ap.add_argument("--lanczos-steps", type=int, default=150, help="Heisenberg Lanczos steps")
@@ -76,6 +80,9 @@ def main():
     dev = ctx.device
     if args.workload == "heis6x6":
         _heis_bench(args, ctx, torch, L)
+        return
+    if args.workload == "batch256":
+        _batch_bench(args, ctx, torch)
         return
     Lx = Ly = 4
     n = Lx * Ly
@@ -161,19 +168,24 @@ def main():
     }
 
     if args.calib:
-        # plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE calibrate the
-        # HBM-byte counters read for the H*v kernel in the same profiled run
-        src = torch.empty(256 * 2 ** 20, dtype=torch.float32, device=dev).fill_(1.0)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        out["calib_copy_bytes"] = src.numel() * 4
+        _calib(out, torch, dev)
     if not args.no_lanczos:
         out["lanczos"] = _lanczos_bench(op, args, L, torch)
     if not args.no_cpu:
         out["cpu_baseline"] = _cpu_baseline(T, U, D, b_alg)
     print(json.dumps(out), flush=True)
+
+
+def _calib(out, torch, dev):
+    # plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE calibrate the
+    # HBM-byte counters read for the H*v kernel in the same profiled run
+    src = torch.empty(256 * 2 ** 20, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    out["calib_copy_bytes"] = src.numel() * 4
+    del src, dst
 
 
 def j1j2_square(Lx, Ly, J1, J2):
@@ -251,8 +263,11 @@ def _heis_bench(args, ctx, torch, L):
                      "kernel": "k_hv_heis_blk<float,false>",
                      "valu_fp32_tflops": round(flops / (ms_avg * 1e-3) / 1e12, 2)},
     }
+    if args.calib:
+        _calib(out, torch, dev)
     if not args.no_lanczos:
         steps = args.lanczos_steps
+        os.environ["PHUB_PROGRESS"] = "10"   # a progress line every 10 steps
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         r = op.lanczos(mode=L.LCZ_PLAIN32, max_steps=steps, fixed_steps=steps, seed=5,
@@ -265,6 +280,49 @@ def _heis_bench(args, ctx, torch, L):
         print(f"# lanczos plain32: {r.steps} steps, E0={r.e0:.10f}, E0/site={r.e0 / n:.10f}, "
               f"{wall:.1f}s", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
+
+
+def _batch_bench(args, ctx, torch):
+    from pyhub_amd.batch import gii_omega, random_cluster, run_batch
+    clusters = [random_cluster(c) for c in range(256)]
+    dims = [_comb(c.n, c.n_up) * _comb(c.n, c.n_dn) for c in clusters]
+    nchains = sum(4 * c.n for c in clusters)
+    omega = np.linspace(-15.0, 15.0, 2048)
+    for _ in range(max(0, args.warmup)):
+        run_batch(clusters[:8], gs_steps=256, gf_steps=120, seed=3)
+    times = []
+    for _ in range(max(1, args.steps)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0, ga, gb, gn = run_batch(clusters, gs_steps=256, gf_steps=120, seed=3, gs_tol=1e-9)
+        t1 = time.perf_counter()
+        G = gii_omega(e0, ga, gb, gn, clusters, omega, eta=0.05)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        times.append((t1 - t0, t2 - t1))
+        print(f"# batch256: solve {t1 - t0:.3f} s, G(omega) {t2 - t1:.3f} s", file=sys.stderr,
+              flush=True)
+    ts = float(np.mean([a for a, _ in times]))
+    tg = float(np.mean([b for _, b in times]))
+    out = {
+        "metric": "batched clusters per second (ground state + all G_ii chains)",
+        "value": round(256 / (ts + tg), 2), "unit": "clusters/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round((ts + tg) * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 vectors / fp64 scalars",
+        "data": "synthetic random clusters, generator default_rng([3, c])",
+        "config": {"workload": "256 Hubbard clusters, L in 6..12, 120-step chains, 2048 omega",
+                   "parallelism": "single GPU"},
+        "batch": {"solve_s": ts, "gf_eval_s": tg, "chains": nchains, "max_dim": max(dims),
+                  "sum_dim": int(sum(dims)), "e0_first": float(e0[0]),
+                  "g_finite": bool(np.all(np.isfinite(G)))},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _comb(n, k):
+    from math import comb
+    return comb(n, k) if 0 <= k <= n else 0
 
 
 def _mean_hops(T, k):

@@ -192,9 +192,11 @@ typedef struct {
 } phub_cluster;
 
 typedef struct {
-    int gs_steps;        /* Lanczos steps for the ground state */
+    int gs_steps;        /* maximum Lanczos steps for the ground state (<= 256) */
     int gf_steps;        /* continued-fraction steps per G_ii chain */
     uint64_t seed;
+    double gs_tol;       /* stop the ground state once |beta_m y_m| < gs_tol*max(1,|E|)
+                            (checked every 10 steps); 0 runs all gs_steps */
 } phub_batch_opts;
 
 /* e0[n_clusters]; for cluster c, site i (0-based), spin s, kind k (0 particle

@@ -53,7 +53,8 @@ class Cluster(C.Structure):
 
 
 class BatchOpts(C.Structure):
-    _fields_ = [("gs_steps", C.c_int), ("gf_steps", C.c_int), ("seed", C.c_uint64)]
+    _fields_ = [("gs_steps", C.c_int), ("gf_steps", C.c_int), ("seed", C.c_uint64),
+                ("gs_tol", C.c_double)]
 
 
 # every symbol declared in include/phub.h, with its ctypes signature

@@ -1,5 +1,6 @@
 """Batched clusters: ground states and diagonal Green's functions of many small
-Hubbard clusters in one persistent launch (phub_batch_hubbard), and the
+Hubbard clusters in one call (phub_batch_hubbard: two persistent launches, ground
+states in FP64 vectors then all chains in FP32 vectors with FP64 scalars), and the
 continued-fraction evaluation of G_ii(omega) on a grid (phub_cf_eval).
 
 The random-cluster workload follows the batched configuration: L sites uniform in
@@ -54,7 +55,8 @@ def random_cluster(c: int) -> Cluster:
     return Cluster(L_, nup, ndn, T, U)
 
 
-def run_batch(clusters, gs_steps=120, gf_steps=120, seed=7, ctx: Context | None = None):
+def run_batch(clusters, gs_steps=256, gf_steps=120, seed=7, gs_tol=1e-9,
+              ctx: Context | None = None):
     """One persistent launch; returns e0 (ncl,), alpha/beta (ncl, 32, 2, 2, steps) and
     norm2 (ncl, 32, 2, 2): index [c, site-1, spin, kind] with kind 0 = particle
     (c^dag psi0), 1 = hole (c psi0)."""
@@ -68,7 +70,7 @@ def run_batch(clusters, gs_steps=120, gf_steps=120, seed=7, ctx: Context | None 
         keep += [T, U]
         arr[i] = L.Cluster(cl.n, cl.n_up, cl.n_dn, T.ctypes.data_as(C.POINTER(C.c_double)),
                            U.ctypes.data_as(C.POINTER(C.c_double)))
-    opts = L.BatchOpts(gs_steps, gf_steps, seed)
+    opts = L.BatchOpts(gs_steps, gf_steps, seed, gs_tol)
     e0 = np.zeros(ncl)
     nslot = ncl * 32 * 2 * 2
     ga = np.zeros(nslot * gf_steps)

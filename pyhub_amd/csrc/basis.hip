@@ -460,6 +460,8 @@ void phub_free_heis(HeisTables *t) {
     if (t->headpat) hipFree(t->headpat);
     if (t->hoff) hipFree(t->hoff);
     if (t->tabA32) hipFree(t->tabA32);
+    if (t->tabA1) hipFree(t->tabA1);
+    if (t->tabA2) hipFree(t->tabA2);
     if (t->sbonds) hipFree(t->sbonds);
     if (t->items) hipFree(t->items);
     *t = HeisTables();
@@ -503,6 +505,30 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
         hipLaunchKernelGGL(k_narrow32, dim3(nblk(1ll << nh, 256)), dim3(256), 0, c->stream,
                            out->tabA, 1ll << nh, out->tabA32);
         PHUB_HIP(c, hipGetLastError());
+        {   // two-half split of A for LDS (same construction as A/B one level up)
+            const int LA = nh / 2, nh2 = nh - LA;
+            out->LA = LA;
+            std::vector<uint32_t> a1((size_t)1 << nh2), a2((size_t)(nh2 + 1) << LA);
+            for (uint32_t h = 0; h < (1u << nh2); ++h) {
+                uint64_t cc = 0;
+                int i = 0;
+                for (int b = nh2 - 1; b >= 0; --b)
+                    if ((h >> b) & 1u) { ++i; cc += phub_binom(n - 1 - (b + LA + L), i); }
+                a1[h] = (uint32_t)cc;
+            }
+            for (int p = 0; p <= nh2; ++p)
+                for (uint32_t lo = 0; lo < (1u << LA); ++lo) {
+                    uint64_t cc = 0;
+                    int i = p;
+                    for (int b = LA - 1; b >= 0; --b)
+                        if ((lo >> b) & 1u) { ++i; cc += phub_binom(n - 1 - (b + L), i); }
+                    a2[((size_t)p << LA) + lo] = (uint32_t)cc;
+                }
+            PHUB_HIP(c, hipMalloc(&out->tabA1, a1.size() * sizeof(uint32_t)));
+            PHUB_HIP(c, hipMalloc(&out->tabA2, a2.size() * sizeof(uint32_t)));
+            PHUB_HIP(c, hipMemcpy(out->tabA1, a1.data(), a1.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+            PHUB_HIP(c, hipMemcpy(out->tabA2, a2.data(), a2.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        }
         // work items: every low pattern xl with a valid high popcount m = n/2 - popc(xl)
         // owns the contiguous block of C(nh, m) states that ends at C-1-B[m][xl]
         std::vector<uint64_t> Bh(((size_t)1 << L) * (nh + 1));
@@ -560,8 +586,11 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
         for (int b = 0; b < nb; ++b) {
             const bool jl = bj[b] < L, kl = bk[b] < L;
             const int ty = (jl && kl) ? 0 : ((!jl && !kl) ? 1 : 2);
-            g[ty].push_back(HeisBond{bj[b], bk[b], bs[b] * bJ[b], bs[b] * bJr[b], 0.5 * bis[b]});
+            const double cj = bs[b] * bJ[b], cr = bs[b] * bJr[b], hi = 0.5 * bis[b];
+            g[ty].push_back(HeisBond{bj[b], bk[b], cj, cr, hi, (float)cj, (float)cr, (float)hi, 0});
         }
+        std::stable_sort(g[2].begin(), g[2].end(),
+                         [](const HeisBond &p, const HeisBond &q) { return p.j > q.j; });
         out->ntt = (int)g[0].size();
         out->nhh = (int)g[1].size();
         out->nx = (int)g[2].size();

@@ -1,10 +1,12 @@
-// batch.hip -- many small Hubbard clusters in one persistent launch.
+// batch.hip -- many small Hubbard clusters in one call.
 //
-// Work: for every cluster a ground state (Lanczos, FP32 vectors, FP64 recurrence
-// scalars, psi0 rebuilt by a second pass), then for every site i and spin s the
+// Work: for every cluster a ground state (Lanczos in FP64 vectors, stopped on the
+// Ritz residual, psi0 rebuilt by a second pass; the chains' accuracy is set by
+// psi0, so it is not computed in FP32), then for every site i and spin s the
 // particle chain c^dag_is psi0 and the hole chain c_is psi0 (continued-fraction
 // Lanczos, the single-vector case of GF_BANDLANCZOS, spgf_bandlanczos.f90:309-448;
-// excitation signs as GF_E_* / GF_H_*, spgf_ed.f90:304-429).
+// excitation signs as GF_E_* / GF_H_*, spgf_ed.f90:304-429) in FP32 vectors
+// with FP64 recurrence scalars.
 //
 // Two launches on one stream, each a persistent grid of one workgroup per CU
 // pulling tasks from an atomic counter: the first solves every ground state
@@ -32,10 +34,23 @@ struct SectorDev {
     const double *eu, *ed;
     const int32_t *cu, *iu;
     const float *cfu;
+    const double *cfu64;
     const int32_t *cd, *id;
     const float *cfd;
+    const double *cfd64;
     int maxh_dn;
 };
+
+template <typename V>
+__device__ __forceinline__ V cf_up(const SectorDev &s, size_t i) {
+    if constexpr (sizeof(V) == 8) return s.cfu64[i];
+    else return s.cfu[i];
+}
+template <typename V>
+__device__ __forceinline__ V cf_dn(const SectorDev &s, size_t i) {
+    if constexpr (sizeof(V) == 8) return s.cfd64[i];
+    else return s.cfd[i];
+}
 
 struct ClusterDev {
     int n;
@@ -43,7 +58,7 @@ struct ClusterDev {
     double u0;
     double U[MAXN];
     SectorDev sec[5];      // 0 ground, 1 up-1, 2 up+1, 3 dn-1, 4 dn+1
-    float *psi0;           // ground-state vector (sec[0] dimension)
+    double *psi0;          // ground-state vector (sec[0] dimension), FP64
     int64_t psi_off;
 };
 
@@ -59,9 +74,10 @@ struct BatchDev {
     const Task *tasks;
     int ntasks;
     int gs_steps, gf_steps;
+    double gs_tol;
     uint64_t seed;
     unsigned int *counter;
-    float *scratch;        // per workgroup: 3 vectors of dmax floats
+    double *scratch;       // per workgroup: 3 vectors of dmax doubles (FP32 chains use the front)
     int64_t dmax;
     double *e0;            // per cluster
     double *ga, *gb, *gn;  // chain outputs [slot * gf_steps], norm2 [slot]
@@ -95,9 +111,10 @@ __device__ inline void wg_sum2(double &a, double &b, double *sh) {
 }
 
 // out = scale * (H x) - bprev * xp ; returns (<scale x, out>, |out|^2) workgroup-wide
-__device__ void wg_hv(const ClusterDev &c, const SectorDev &s, const float *__restrict__ x,
-                      double scale, const float *__restrict__ xp, double bprev,
-                      float *__restrict__ out, double &d_xo, double &d_oo, double *sh) {
+template <typename V>
+__device__ void wg_hv(const ClusterDev &c, const SectorDev &s, const V *__restrict__ x,
+                      double scale, const V *__restrict__ xp, double bprev,
+                      V *__restrict__ out, double &d_xo, double &d_oo, double *sh) {
     const int64_t D = s.nu * s.nd;
     double a = 0.0, b = 0.0;
     for (int64_t e = threadIdx.x; e < D; e += BT) {
@@ -115,16 +132,16 @@ __device__ void wg_hv(const ClusterDev &c, const SectorDev &s, const float *__re
                 bb &= bb - 1;
             }
         }
-        float acc = (float)dg * x[e];
+        V acc = (V)dg * x[e];
         const int cu = s.cu[u];
         for (int k = 0; k < cu; ++k)
-            acc += s.cfu[(size_t)k * s.nu + u] * x[d * s.nu + s.iu[(size_t)k * s.nu + u]];
+            acc += cf_up<V>(s, (size_t)k * s.nu + u) * x[d * s.nu + s.iu[(size_t)k * s.nu + u]];
         const int cd = s.cd[d];
         for (int k = 0; k < cd; ++k)
-            acc += s.cfd[(size_t)d * s.maxh_dn + k] * x[(int64_t)s.id[(size_t)d * s.maxh_dn + k] * s.nu + u];
+            acc += cf_dn<V>(s, (size_t)d * s.maxh_dn + k) * x[(int64_t)s.id[(size_t)d * s.maxh_dn + k] * s.nu + u];
         double v = scale * (double)acc;
         if (xp) v -= bprev * (double)xp[e];
-        const float vt = (float)v;
+        const V vt = (V)v;
         out[e] = vt;
         a += scale * (double)x[e] * (double)vt;
         b += (double)vt * (double)vt;
@@ -135,11 +152,12 @@ __device__ void wg_hv(const ClusterDev &c, const SectorDev &s, const float *__re
 }
 
 // y = y - a * x (x scaled by sx) ; returns |y|^2 workgroup-wide
-__device__ double wg_update(float *__restrict__ y, const float *__restrict__ x, double a, int64_t D,
+template <typename V>
+__device__ double wg_update(V *__restrict__ y, const V *__restrict__ x, double a, int64_t D,
                             double *sh) {
     double s = 0.0, dummy = 0.0;
     for (int64_t e = threadIdx.x; e < D; e += BT) {
-        const float t = (float)((double)y[e] - a * (double)x[e]);
+        const V t = (V)((double)y[e] - a * (double)x[e]);
         y[e] = t;
         s += (double)t * (double)t;
     }
@@ -150,16 +168,24 @@ __device__ double wg_update(float *__restrict__ y, const float *__restrict__ x, 
 // Lanczos chain of `steps` steps from v (overwritten).  alpha[k], beta[k] (beta[k]
 // couples k and k+1).  Three buffers rotate; vectors are kept unnormalised with
 // scale sig.  If psi is non-null, the pass accumulates psi += y[k] * x_k.
-__device__ int wg_lanczos(const ClusterDev &c, const SectorDev &s, float *v0, float *b1, float *b2,
+__device__ double tri_lowest(int m, const double *al, const double *be, double *y, double *wk);
+
+// With tol > 0 (ground states) the recurrence stops once the Ritz residual
+// |beta_m * y_m| of the lowest pair drops below tol * max(1, |E|), tested every
+// 10 steps from step 20 (y and wk: FP64 scratch of MAXSTEPS each).
+template <typename V>
+__device__ int wg_lanczos(const ClusterDev &c, const SectorDev &s, V *v0, V *b1, V *b2,
                           int steps, double *alpha, double *beta, double norm0, const double *y,
-                          float *psi, double *sh) {
+                          V *psi, double *sh, double tol = 0.0, double *ty = nullptr,
+                          double *twk = nullptr) {
+    __shared__ int stop_sh;
     const int64_t D = s.nu * s.nd;
-    float *xp = b1, *xk = v0, *out = b2;
+    V *xp = b1, *xk = v0, *out = b2;
     double sig = 1.0 / norm0, sigp = 0.0, bk = 0.0;
     int m = 0;
     for (int k = 0; k < steps; ++k) {
         if (psi) {
-            for (int64_t e = threadIdx.x; e < D; e += BT) psi[e] += (float)(y[k] * sig * (double)xk[e]);
+            for (int64_t e = threadIdx.x; e < D; e += BT) psi[e] += (V)(y[k] * sig * (double)xk[e]);
         }
         if (k == steps - 1 && psi) { m = steps; break; }
         double dxo, doo;
@@ -170,7 +196,7 @@ __device__ int wg_lanczos(const ClusterDev &c, const SectorDev &s, float *v0, fl
         if (!psi && threadIdx.x == 0) { alpha[k] = a; beta[k] = b; }
         m = k + 1;
         // rotate
-        float *t = xp;
+        V *t = xp;
         xp = xk;
         xk = out;
         out = t;
@@ -179,6 +205,16 @@ __device__ int wg_lanczos(const ClusterDev &c, const SectorDev &s, float *v0, fl
         sig = (bk > 0.0) ? 1.0 / bk : 0.0;
         __syncthreads();
         if (!(b > 1e-10 * fmax(1.0, fabs(a)))) break;   // invariant subspace
+        if (tol > 0.0 && !psi && m >= 20 && m % 10 == 0 && m < steps) {
+            if (threadIdx.x == 0) {
+                const double e = tri_lowest(m, alpha, beta, ty, twk);
+                stop_sh = fabs(b * ty[m - 1]) < tol * fmax(1.0, fabs(e));
+            }
+            __syncthreads();
+            const int stop = stop_sh;
+            __syncthreads();
+            if (stop) break;
+        }
     }
     return m;
 }
@@ -268,7 +304,10 @@ __device__ double wg_excite(const BatchDev &B, const ClusterDev &c, const Sector
 __global__ __launch_bounds__(BT) void k_batch(BatchDev B) {
     __shared__ double sh[2 * BT / 64];
     __shared__ int task_sh;
-    float *w0 = B.scratch + (size_t)blockIdx.x * 3 * B.dmax;
+    // FP64 view for the ground state, FP32 view (front half) for the chains
+    double *d0 = B.scratch + (size_t)blockIdx.x * 3 * B.dmax;
+    double *d1 = d0 + B.dmax, *d2 = d1 + B.dmax;
+    float *w0 = reinterpret_cast<float *>(d0);
     float *w1 = w0 + B.dmax, *w2 = w1 + B.dmax;
     double *tri = B.tri + (size_t)blockIdx.x * 6 * MAXSTEPS;
     while (true) {
@@ -282,36 +321,37 @@ __global__ __launch_bounds__(BT) void k_batch(BatchDev B) {
         const SectorDev &g = c.sec[0];
         const int64_t Dg = g.nu * g.nd;
         if (tk.kind == 0) {
-            // ground state: random start, plain recurrence, second pass for psi0
+            // ground state in FP64 vectors: random start, plain recurrence, second pass for psi0
             double n2 = 0.0, dummy = 0.0;
             for (int64_t e = threadIdx.x; e < Dg; e += BT) {
                 const uint64_t r = mix64(B.seed * 0x9E3779B97F4A7C15ull + (uint64_t)tk.cluster * 0x100000001B3ull + (uint64_t)e);
-                const float v = (float)((double)(r >> 11) * (1.0 / 9007199254740992.0) - 0.5);
-                w0[e] = v;
+                const double v = (double)(r >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+                d0[e] = v;
                 c.psi0[e] = v;   // keep the start vector for the second pass
-                n2 += (double)v * (double)v;
+                n2 += v * v;
             }
             wg_sum2(n2, dummy, sh);
             const double nrm = sqrt(n2);
             double *al = tri, *be = tri + MAXSTEPS, *y = tri + 2 * MAXSTEPS, *wk = tri + 3 * MAXSTEPS;
             const int steps = (int)min<int64_t>(B.gs_steps, Dg);
-            const int m = wg_lanczos(c, g, w0, w1, w2, steps, al, be, nrm, nullptr, nullptr, sh);
+            const int m = wg_lanczos<double>(c, g, d0, d1, d2, steps, al, be, nrm, nullptr, nullptr, sh,
+                                             B.gs_tol, y, wk);
             __syncthreads();
             if (threadIdx.x == 0) B.e0[tk.cluster] = tri_lowest(m, al, be, y, wk);
             __syncthreads();
-            // second pass: w0 <- start, psi0 accumulated in w1's place (psi0 holds start)
+            // second pass: d0 <- start, psi0 accumulated in place (psi0 holds the start)
             for (int64_t e = threadIdx.x; e < Dg; e += BT) {
-                w0[e] = c.psi0[e];
-                c.psi0[e] = 0.f;
+                d0[e] = c.psi0[e];
+                c.psi0[e] = 0.0;
             }
             __syncthreads();
-            wg_lanczos(c, g, w0, w1, w2, m, al, be, nrm, y, c.psi0, sh);
+            wg_lanczos<double>(c, g, d0, d1, d2, m, al, be, nrm, y, c.psi0, sh);
             __syncthreads();
             // normalise psi0
-            double p2 = 0.0, d2 = 0.0;
-            for (int64_t e = threadIdx.x; e < Dg; e += BT) p2 += (double)c.psi0[e] * (double)c.psi0[e];
-            wg_sum2(p2, d2, sh);
-            const float inv = (float)(1.0 / sqrt(p2));
+            double p2 = 0.0, d2s = 0.0;
+            for (int64_t e = threadIdx.x; e < Dg; e += BT) p2 += c.psi0[e] * c.psi0[e];
+            wg_sum2(p2, d2s, sh);
+            const double inv = 1.0 / sqrt(p2);
             for (int64_t e = threadIdx.x; e < Dg; e += BT) c.psi0[e] *= inv;
             __syncthreads();
         } else {
@@ -331,7 +371,7 @@ __global__ __launch_bounds__(BT) void k_batch(BatchDev B) {
             __syncthreads();
             if (n2 > 0.0) {
                 const int steps = (int)min<int64_t>(B.gf_steps, Dt);
-                wg_lanczos(c, t, w0, w1, w2, steps, ga, gb, sqrt(n2), nullptr, nullptr, sh);
+                wg_lanczos<float>(c, t, w0, w1, w2, steps, ga, gb, sqrt(n2), nullptr, nullptr, sh);
             }
             __syncthreads();
         }
@@ -395,9 +435,11 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
             sd.cu = op->up.cnt;
             sd.iu = op->up.idx;
             sd.cfu = op->up.c32;
+            sd.cfu64 = op->up.c64;
             sd.cd = op->dn.cnt;
             sd.id = op->dn.idx;
             sd.cfd = op->dn.c32;
+            sd.cfd64 = op->dn.c64;
             sd.maxh_dn = std::max(op->dn.maxh, 1);
             dmax = std::max<int64_t>(dmax, op->dim);
         }
@@ -432,14 +474,14 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
     int dev_cu = 0;
     hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, c->device);
     const int grid = std::max(1, std::min<int>(dev_cu > 0 ? dev_cu : 256, (int)tasks.size()));
-    DB<float> psi, scratch;
+    DB<double> psi, scratch;
     DB<ClusterDev> dcl;
     DB<Task> dtasks;
     DB<unsigned int> counter;
     DB<double> de0, ga, gb, gn, tri;
-    PHUB_HIP(c, hipMalloc(&psi.p, psi_total * sizeof(float)));
+    PHUB_HIP(c, hipMalloc(&psi.p, psi_total * sizeof(double)));
     for (int i = 0; i < ncl; ++i) hc[i].psi0 = psi.p + hc[i].psi_off;
-    PHUB_HIP(c, hipMalloc(&scratch.p, (size_t)grid * 3 * dmax * sizeof(float)));
+    PHUB_HIP(c, hipMalloc(&scratch.p, (size_t)grid * 3 * dmax * sizeof(double)));
     PHUB_HIP(c, hipMalloc(&dcl.p, ncl * sizeof(ClusterDev)));
     PHUB_HIP(c, hipMalloc(&dtasks.p, tasks.size() * sizeof(Task)));
     PHUB_HIP(c, hipMalloc(&counter.p, 2 * sizeof(unsigned int)));
@@ -459,6 +501,7 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
     B.tasks = dtasks.p;
     B.ntasks = (int)tasks.size();
     B.gs_steps = o->gs_steps;
+    B.gs_tol = o->gs_tol;
     B.gf_steps = o->gf_steps;
     B.seed = o->seed;
     B.scratch = scratch.p;

@@ -323,110 +323,150 @@ __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict
 struct HeisBlkArgs {
     int L, nh, k;
     int ntt, nhh, nx;
+    int LA, na1, na2;    // split A tables: sizes of A1 and A2
+    int64_t nitems;
     uint64_t total_m1;   // C(n, n/2) - 1
     const uint64_t *tabB;
-    const uint32_t *tabA32, *headpat, *items;
+    const uint32_t *tabA1, *tabA2, *headpat, *items;
     const int64_t *hoff;
     const HeisBond *bonds;
     double dconst;
 };
 
+template <typename T>
+__device__ __forceinline__ T bond_c(const HeisBond &b, bool up) {
+    if constexpr (sizeof(T) == 4) return up ? b.fr : b.fj;
+    else return up ? b.cr : b.cj;
+}
+template <typename T>
+__device__ __forceinline__ T bond_h(const HeisBond &b) {
+    if constexpr (sizeof(T) == 4) return b.fh;
+    else return b.hi;
+}
+
+// Work item i of the launch runs on XCD (i % 8); the item list is dealt so that
+// each XCD takes one contiguous eighth of it (whole blocks stay in one L2).
+// Every load below is issued for all lanes (inactive lanes read an in-range
+// element and contribute zero), so gathers of different bonds overlap.
 template <typename T, bool EPI>
 __global__ __launch_bounds__(256) void k_hv_heis_blk(HeisBlkArgs a, const T *__restrict__ x,
                                                      T *__restrict__ out, double scale,
                                                      const T *__restrict__ prev, double beta,
                                                      double *__restrict__ part) {
     constexpr int R = HEIS_CHUNK / 256;
-    const uint32_t item = a.items[blockIdx.x];
-    const uint32_t xl = item >> 12;
-    const int ch = (int)(item & 4095u);
-    const int m = a.k - __popc(xl);
-    const int64_t h0 = a.hoff[m];
-    const int len = (int)(a.hoff[m + 1] - h0);
-    const int L = a.L;
-    const int64_t last = (int64_t)(a.total_m1 - a.tabB[((uint64_t)m << L) + xl]);
-    const T *__restrict__ xb = x + last;   // inner index i lives at xb[-i]
-    int ai[R];
-    bool ok[R];
-    uint32_t xh[R];
-    double dg[R];
-    T acc[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        ai[r] = ch * HEIS_CHUNK + r * 256 + (int)threadIdx.x;
-        ok[r] = ai[r] < len;
-        xh[r] = ok[r] ? a.headpat[h0 + ai[r]] : 0u;
-        dg[r] = 0.0;
-        acc[r] = (T)0;
-    }
-    double dgu = a.dconst;
-    // low-low bonds
-    for (int b = 0; b < a.ntt; ++b) {
-        const HeisBond bd = a.bonds[b];
-        const uint32_t tj = (xl >> bd.j) & 1u, tk = (xl >> bd.k) & 1u;
-        if (tj != tk) {
-            dgu -= bd.hi;
-            const uint32_t xl2 = xl ^ ((1u << bd.j) | (1u << bd.k));
-            const int64_t last2 = (int64_t)(a.total_m1 - a.tabB[((uint64_t)m << L) + xl2]);
-            const T cf = (T)(tj ? bd.cr : bd.cj);
-            const T *__restrict__ xs = x + last2;
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-                if (ok[r]) acc[r] += cf * xs[-ai[r]];
-        }
-    }
-    // high-high bonds
-    for (int b = a.ntt; b < a.ntt + a.nhh; ++b) {
-        const HeisBond bd = a.bonds[b];
-        const int hj = bd.j - L, hk = bd.k - L;
-        const uint32_t f = (1u << hj) | (1u << hk);
-        const T cj = (T)bd.cj, cr = (T)bd.cr;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const uint32_t vj = (xh[r] >> hj) & 1u, vk = (xh[r] >> hk) & 1u;
-            if (ok[r] && vj != vk) {
-                dg[r] += bd.hi;
-                const uint32_t a2 = a.tabA32[xh[r] ^ f];
-                acc[r] += (vj ? cr : cj) * xb[-(int64_t)a2];
-            }
-        }
-    }
-    // cross bonds: site J in the high half, site K in the low half
-    for (int b = a.ntt + a.nhh; b < a.ntt + a.nhh + a.nx; ++b) {
-        const HeisBond bd = a.bonds[b];
-        const uint32_t tb = (xl >> bd.k) & 1u;    // the target's low bit; its high bit must differ
-        const int m2 = tb ? m + 1 : m - 1;         // popcount of the source's high half
-        if (m2 < 0 || m2 > a.nh) continue;         // no lane of this block can flip
-        const int hj = bd.j - L;
-        const int64_t last2 = (int64_t)(a.total_m1 - a.tabB[((uint64_t)m2 << L) + (xl ^ (1u << bd.k))]);
-        const T cf = (T)(tb ? bd.cj : bd.cr);      // target site J up iff tb == 0
-        const T *__restrict__ xs = x + last2;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const uint32_t hb = (xh[r] >> hj) & 1u;
-            if (ok[r] && hb != tb) {
-                dg[r] += bd.hi;
-                const uint32_t a2 = a.tabA32[xh[r] ^ (1u << hj)];
-                acc[r] += cf * xs[-(int64_t)a2];
-            }
-        }
-    }
+    extern __shared__ uint32_t sA[];   // A1 then A2
+    uint32_t *sA1 = sA, *sA2 = sA + a.na1;
+    for (int i = threadIdx.x; i < a.na1; i += 256) sA1[i] = a.tabA1[i];
+    for (int i = threadIdx.x; i < a.na2; i += 256) sA2[i] = a.tabA2[i];
+    const int64_t per = (a.nitems + 7) / 8;
+    const int64_t it = (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
     double s_xo = 0.0, s_oo = 0.0;
+    __syncthreads();
+    if (it < a.nitems) {
+        const uint32_t item = a.items[it];
+        const uint32_t xl = item >> 12;
+        const int ch = (int)(item & 4095u);
+        const int m = a.k - __popc(xl);
+        const int64_t h0 = a.hoff[m];
+        const int len = (int)(a.hoff[m + 1] - h0);
+        const int L = a.L, LA = a.LA;
+        const uint32_t lomask = (1u << LA) - 1u;
+        auto Alook = [&](uint32_t h) -> int {
+            const uint32_t hi = h >> LA;
+            return (int)(sA1[hi] + sA2[((uint32_t)__popc(hi) << LA) + (h & lomask)]);
+        };
+        const int64_t last = (int64_t)(a.total_m1 - a.tabB[((uint64_t)m << L) + xl]);
+        const T *__restrict__ xb = x + last;   // inner index i lives at xb[-i]
+        int ai[R];
+        bool ok[R];
+        uint32_t xh[R];
+        T dg[R], acc[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        if (!ok[r]) continue;
-        const int64_t s = last - ai[r];
-        const T xv = x[s];
-        acc[r] += (T)(dgu - dg[r]) * xv;
-        if (EPI) {
-            double val = scale * (double)acc[r];
-            if (prev) val -= beta * (double)prev[s];
-            const T vt = (T)val;
-            out[s] = vt;
-            s_xo += scale * (double)xv * (double)vt;
-            s_oo += (double)vt * (double)vt;
-        } else {
-            out[s] = acc[r];
+        for (int r = 0; r < R; ++r) {
+            const int i = ch * HEIS_CHUNK + r * 256 + (int)threadIdx.x;
+            ok[r] = i < len;
+            ai[r] = ok[r] ? i : 0;   // inactive lanes read element 0 of the block
+            xh[r] = ok[r] ? a.headpat[h0 + i] : 0u;
+            dg[r] = (T)0;
+            acc[r] = (T)0;
+        }
+        T dgu = (T)a.dconst;
+        // low-low bonds: existence and source block uniform
+        for (int b = 0; b < a.ntt; ++b) {
+            const HeisBond bd = a.bonds[b];
+            const uint32_t tj = (xl >> bd.j) & 1u, tk = (xl >> bd.k) & 1u;
+            if (tj != tk) {
+                dgu -= bond_h<T>(bd);
+                const uint32_t xl2 = xl ^ ((1u << bd.j) | (1u << bd.k));
+                const int64_t last2 = (int64_t)(a.total_m1 - a.tabB[((uint64_t)m << L) + xl2]);
+                const T cf = bond_c<T>(bd, tj != 0);
+                const T *__restrict__ xs = x + last2;
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] += cf * xs[-ai[r]];
+            }
+        }
+        // high-high bonds: per-lane existence, gather inside the block
+        for (int b = a.ntt; b < a.ntt + a.nhh; ++b) {
+            const HeisBond bd = a.bonds[b];
+            const int hj = bd.j - L, hk = bd.k - L;
+            const uint32_t f = (1u << hj) | (1u << hk);
+            const T cj = bond_c<T>(bd, false), cr = bond_c<T>(bd, true), hh = bond_h<T>(bd);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t vj = (xh[r] >> hj) & 1u, vk = (xh[r] >> hk) & 1u;
+                const bool on = ok[r] && (vj != vk);
+                const int a2 = on ? Alook(xh[r] ^ f) : ai[r];
+                const T c = on ? (vj ? cr : cj) : (T)0;
+                dg[r] += on ? hh : (T)0;
+                acc[r] += c * xb[-a2];
+            }
+        }
+        // cross bonds, grouped by their high site: one A lookup per group
+        const int bx0 = a.ntt + a.nhh, bx1 = bx0 + a.nx;
+        for (int b = bx0; b < bx1;) {
+            const int j = a.bonds[b].j;
+            const int hj = j - L;
+            int af[R];
+            uint32_t hb[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                hb[r] = (xh[r] >> hj) & 1u;
+                af[r] = Alook(xh[r] ^ (1u << hj));
+            }
+            for (; b < bx1 && a.bonds[b].j == j; ++b) {
+                const HeisBond bd = a.bonds[b];
+                const uint32_t tb = (xl >> bd.k) & 1u;   // target's low bit; its high bit must differ
+                const int m2 = tb ? m + 1 : m - 1;        // popcount of the source's high half
+                if (m2 < 0 || m2 > a.nh) continue;        // no lane of this block can flip
+                const int64_t last2 = (int64_t)(a.total_m1 - a.tabB[((uint64_t)m2 << L) + (xl ^ (1u << bd.k))]);
+                const T cf = bond_c<T>(bd, tb == 0);      // target site J up iff tb == 0
+                const T hh = bond_h<T>(bd);
+                const T *__restrict__ xs = x + last2;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const bool on = ok[r] && (hb[r] != tb);
+                    const int a2 = on ? af[r] : 0;
+                    dg[r] += on ? hh : (T)0;
+                    acc[r] += (on ? cf : (T)0) * xs[-a2];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!ok[r]) continue;
+            const int64_t s = last - ai[r];
+            const T xv = x[s];
+            acc[r] += (dgu - dg[r]) * xv;
+            if (EPI) {
+                double val = scale * (double)acc[r];
+                if (prev) val -= beta * (double)prev[s];
+                const T vt = (T)val;
+                out[s] = vt;
+                s_xo += scale * (double)xv * (double)vt;
+                s_oo += (double)vt * (double)vt;
+            } else {
+                out[s] = acc[r];
+            }
         }
     }
     if (EPI) block_sum2<T>(s_xo, s_oo, part);
@@ -473,9 +513,14 @@ static HeisBlkArgs heis_blk_args(const phub_op *op) {
     a.ntt = h.ntt;
     a.nhh = h.nhh;
     a.nx = h.nx;
+    a.LA = h.LA;
+    a.na1 = 1 << (a.nh - h.LA);
+    a.na2 = (a.nh - h.LA + 1) << h.LA;
+    a.nitems = h.nitems;
     a.total_m1 = (uint64_t)h.dim - 1;
     a.tabB = h.tabB;
-    a.tabA32 = h.tabA32;
+    a.tabA1 = h.tabA1;
+    a.tabA2 = h.tabA2;
     a.headpat = h.headpat;
     a.items = h.items;
     a.hoff = h.hoff;
@@ -549,7 +594,7 @@ int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double sc
             hipLaunchKernelGGL((k_hv_hub<T, HUB_R, false>), dim3(nblocks), dim3(256), 0,
                                c->stream, a, x, out, scale, prev, beta, c->d_red, ncb);
     } else {
-        const int64_t nb = op->h.nitems;
+        const int64_t nb = 8 * ((op->h.nitems + 7) / 8);
         if (nb > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
         nblocks = (int)nb;
         if (epi) {
@@ -557,11 +602,12 @@ int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double sc
             if (rc) return rc;
         }
         HeisBlkArgs a = heis_blk_args(op);
+        const size_t lds = (size_t)(a.na1 + a.na2) * sizeof(uint32_t);
         if (epi)
-            hipLaunchKernelGGL((k_hv_heis_blk<T, true>), dim3(nblocks), dim3(256), 0, c->stream,
+            hipLaunchKernelGGL((k_hv_heis_blk<T, true>), dim3(nblocks), dim3(256), lds, c->stream,
                                a, x, out, scale, prev, beta, c->d_red);
         else
-            hipLaunchKernelGGL((k_hv_heis_blk<T, false>), dim3(nblocks), dim3(256), 0, c->stream,
+            hipLaunchKernelGGL((k_hv_heis_blk<T, false>), dim3(nblocks), dim3(256), lds, c->stream,
                                a, x, out, scale, prev, beta, c->d_red);
     }
     PHUB_HIP(c, hipGetLastError());

@@ -18,6 +18,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -414,6 +416,10 @@ static int lanczos_plain(phub_ctx *c, const phub_op *op, const phub_lanczos_opts
     T *xp = b1, *xk = v0t, *out = b2;   // xp content irrelevant while beta_eff = 0
     double sig = 1.0 / s.beta[0], sigp = 0.0;
     std::vector<double> ritz;
+    // PHUB_PROGRESS=k: one line on stderr every k steps (long solves)
+    const char *pe = std::getenv("PHUB_PROGRESS");
+    const int progress = pe ? std::atoi(pe) : 0;
+    const auto tp0 = std::chrono::steady_clock::now();
     while (!s.converged && !s.breakdown && s.steps < maxs) {
         double dots[2];
         rc = fused_hv<T>(c, op, xk, sig, s.steps ? xp : nullptr, s.beta[s.steps] * sigp, out,
@@ -432,6 +438,13 @@ static int lanczos_plain(phub_ctx *c, const phub_op *op, const phub_lanczos_opts
         sigp = sig;
         sig = (b > 0.0) ? 1.0 / b : 0.0;
         if (!(b > 1e-13 * std::max(1.0, std::fabs(a)))) s.breakdown = true;
+        if (progress > 0 && s.steps % progress == 0) {
+            double e = 0.0;
+            if (all_ritz(s.steps, s.alpha.data(), s.beta.data(), ritz) == 0) e = ritz[0];
+            std::fprintf(stderr, "phub lanczos: step %d  lowest Ritz %.12f  %.1f s\n", s.steps, e,
+                         std::chrono::duration<double>(std::chrono::steady_clock::now() - tp0).count());
+            std::fflush(stderr);
+        }
         if (!fixed && (s.steps % every == 0 || s.breakdown || s.steps == maxs)) {
             if (all_ritz(s.steps, s.alpha.data(), s.beta.data(), ritz))
                 return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");

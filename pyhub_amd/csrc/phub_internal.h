@@ -52,6 +52,8 @@ struct HopTable {
 struct HeisBond {
     int j, k;
     double cj, cr, hi;
+    float fj, fr, fh;     // the same three in FP32
+    int pad;
 };
 
 // elements per work item of the block Heisenberg H*v
@@ -78,8 +80,14 @@ struct HeisTables {
     uint32_t *headpat = nullptr;   // device, 2^(n-L)
     int64_t *hoff = nullptr;       // device, n-L+2 offsets
     uint32_t *tabA32 = nullptr;    // device, A as 32-bit (A < C(n-L, m))
+    // A itself split in two halves (small enough for LDS):
+    // A[h] = A1[h >> LA] + A2[popc(h >> LA)][h & (2^LA - 1)]
+    int LA = 0;
+    uint32_t *tabA1 = nullptr;     // device, 2^(n-L-LA)
+    uint32_t *tabA2 = nullptr;     // device, (n-L-LA+1) * 2^LA
     // bonds grouped by where their sites lie: [0,ntt) both low half, [ntt,ntt+nhh)
-    // both high half, then cross bonds (site J in the high half, K in the low half)
+    // both high half, then cross bonds (site J in the high half, K in the low
+    // half) ordered by J so that bonds sharing their high site are adjacent
     HeisBond *sbonds = nullptr;    // device
     int ntt = 0, nhh = 0, nx = 0;
     // work items of the block H*v: (low pattern << 12) | chunk, in memory order

@@ -14,21 +14,23 @@ def test_batched_clusters_vs_single_path(ctx):
     import time
     clusters = [random_cluster(c) for c in range(12)]
     t0 = time.perf_counter()
-    e0, ga, gb, gn = run_batch(clusters, gs_steps=150, gf_steps=80, seed=3)
+    e0, ga, gb, gn = run_batch(clusters, gs_steps=256, gf_steps=80, seed=3, gs_tol=1e-9)
     print(f"batched launch: {len(clusters)} clusters in {time.perf_counter() - t0:.2f} s", flush=True)
     omega = np.linspace(-15, 15, 256)
-    G = gii_omega(e0, ga, gb, gn, clusters, omega, eta=0.3)
+    eta = 1.0
+    G = gii_omega(e0, ga, gb, gn, clusters, omega, eta=eta)
+    errs, cerrs = [], []
     for ci, cl in enumerate(clusters):
         op = HubbardOp(ctx, cl.n, cl.n_up, cl.n_dn, cl.T, cl.U)
         r = op.lanczos(mode=L.LCZ_FULL64, max_steps=min(300, op.dim), tol=1e-12)
-        print(f"cluster {ci}: n={cl.n} ({cl.n_up},{cl.n_dn}) dim={op.dim} e0 {e0[ci]:.8f} ref {r.e0:.8f}",
+        print(f"cluster {ci}: n={cl.n} ({cl.n_up},{cl.n_dn}) dim={op.dim} e0 {e0[ci]:.12f} ref {r.e0:.12f}",
               flush=True)
-        assert abs(e0[ci] - r.e0) <= 2e-5 * max(1.0, abs(r.e0)), (ci, e0[ci], r.e0)
+        # ground state in FP64 vectors: E0 to the FP64 solver's accuracy
+        assert abs(e0[ci] - r.e0) <= 1e-8 * max(1.0, abs(r.e0)), (ci, e0[ci], r.e0)
         for site in (1, cl.n):
             for spin in (0, 1):
                 # sum rule <c c^dag> + <c^dag c> = 1
-                assert abs(gn[ci, site - 1, spin].sum() - 1.0) < 1e-4
-                # G_ii(omega) against FP64 chains from the FP64 ground state
+                assert abs(gn[ci, site - 1, spin].sum() - 1.0) < 1e-6
                 Gref = np.zeros(len(omega), dtype=complex)
                 for kind, create, sign in ((0, True, 1.0), (1, False, -1.0)):
                     nu = cl.n_up + ((1 if create else -1) if spin == 0 else 0)
@@ -38,6 +40,21 @@ def test_batched_clusters_vs_single_path(ctx):
                     tgt, x = op.excite(r.psi, site, spin, create)
                     a, b, n2 = tgt.cf_chain(x, min(80, tgt.dim), precision=64)
                     if n2 > 0:
-                        Gref += cf_eval(ctx, a, b, n2, r.e0, sign, omega, 0.3)[0]
+                        Gref += cf_eval(ctx, a, b, n2, r.e0, sign, omega, eta)[0]
+                        # the leading recurrence coefficients fix the low moments of
+                        # the spectral function: FP32 chain vectors against FP64
+                        k = min(10, len(a))
+                        sc = max(1.0, np.max(np.abs(a[:k])))
+                        ca = np.max(np.abs(ga[ci, site - 1, spin, kind, :k] - a[:k])) / sc
+                        cb = np.max(np.abs(gb[ci, site - 1, spin, kind, :k - 1] - b[:k - 1])) / sc
+                        cerrs.append((max(ca, cb), ci, site, spin, kind))
+                        assert abs(gn[ci, site - 1, spin, kind] - n2) < 1e-9
                 err = np.max(np.abs(G[ci, site - 1, spin] - Gref)) / max(1e-12, np.max(np.abs(Gref)))
-                assert err < 2e-3, (ci, site, spin, err)
+                errs.append((err, ci, site, spin))
+    errs.sort(reverse=True)
+    cerrs.sort(reverse=True)
+    print("largest G_ii errors (rel. to max|G|, eta=1):", [(f"{e:.2e}", c, i, s) for e, c, i, s in errs[:5]],
+          flush=True)
+    print("largest leading-coefficient errors:", [(f"{e:.2e}",) + t for e, *t in cerrs[:5]], flush=True)
+    assert cerrs[0][0] < 1e-5, cerrs[0]
+    assert errs[0][0] < 5e-3, errs[0]
