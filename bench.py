@@ -139,8 +139,10 @@ def main():
         torch.cuda.synchronize()
         e64.append(a.elapsed_time(b))
     ms64 = float(np.mean(e64))
-    print(f"# H*v FP32 {ms_avg:.3f} ms ({gbs:.0f} GB/s alg), FP64 {ms64:.3f} ms", file=sys.stderr,
-          flush=True)
+    op._info()
+    tiles = [int(t) for t in op.info.dn_tile]
+    print(f"# H*v FP32 {ms_avg:.3f} ms ({gbs:.0f} GB/s alg), FP64 {ms64:.3f} ms; down tiles "
+          f"(segment bytes*1000 + rows) fp32 {tiles[0]} fp64 {tiles[1]}", file=sys.stderr, flush=True)
 
     out = {
         "metric": "H*v HBM GB/s (%peak); Lanczos s/solve, mixed vs FP64, at 1e-9 rel. energy",
@@ -160,7 +162,7 @@ def main():
         "hv": {"fp32_ms": round(ms_avg, 4), "fp64_ms": round(ms64, 4),
                "states_per_s": D / (ms_avg * 1e-3), "alg_bytes": b_alg, "alg_flops": flops,
                "fp32_gbs": round(gbs, 1), "fp64_gbs": round((2 * 8 * D + 2 * 12870 * hbar * 12) / (ms64 * 1e-3) / 1e9, 1),
-               "mean_hops_per_species": hbar},
+               "mean_hops_per_species": hbar, "dn_tile_fp32": tiles[0], "dn_tile_fp64": tiles[1]},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(gbs / 8000.0, 4), "traffic": None,
                      "kernel": "k_hv_up<float,2> + k_hv_dn<float,false> (one H*v)",

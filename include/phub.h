@@ -97,6 +97,8 @@ typedef struct {
     int64_t dim_up, dim_dn, dim;
     int max_hops_up, max_hops_dn;
     double table_bytes;  /* device bytes of the hop / rank tables */
+    int dn_tile[2];      /* Hubbard down-hop tile the H*v tuner chose, FP32 / FP64:
+                            (bytes per row segment) * 1000 + rows per workgroup; -1 = not yet tuned */
 } phub_op_info;
 
 int phub_op_hubbard(phub_ctx *ctx, const phub_hubbard_desc *desc, phub_op **out);

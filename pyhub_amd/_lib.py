@@ -31,7 +31,8 @@ class HeisDesc(C.Structure):
 class OpInfo(C.Structure):
     _fields_ = [("kind", C.c_int), ("n_sites", C.c_int), ("n_up", C.c_int), ("n_dn", C.c_int),
                 ("dim_up", C.c_int64), ("dim_dn", C.c_int64), ("dim", C.c_int64),
-                ("max_hops_up", C.c_int), ("max_hops_dn", C.c_int), ("table_bytes", C.c_double)]
+                ("max_hops_up", C.c_int), ("max_hops_dn", C.c_int), ("table_bytes", C.c_double),
+                ("dn_tile", C.c_int * 2)]
 
 
 class LanczosOpts(C.Structure):

@@ -20,6 +20,8 @@
 #include "phub_internal.h"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 
 template <typename T>
 struct HubArgs {
@@ -144,7 +146,6 @@ __global__ __launch_bounds__(256) void k_hv_hub(HubArgs<T> a, const T *__restric
 //            out = scale * (out + sum_dn) - beta * prev, dot partials.
 // ---------------------------------------------------------------------------
 constexpr int UP_THREADS = 1024;
-constexpr int DN_ROWS = 16;     // rows per workgroup in k_hv_dn: many workgroups per slab
 constexpr int UP_UNROLL = 8;    // independent hop-entry loads in flight per lane
 constexpr int DN_UNROLL = 8;    // independent row gathers in flight per lane
 
@@ -244,9 +245,9 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
     }
 }
 
-// COLS columns per row segment (64 FP32 = 256 B, 32 FP64 = 256 B): a wave covers
-// 64/COLS rows at once.
-template <typename T, bool EPI, int COLS>
+// COLS columns per row segment (a wave covers 64/COLS rows at once), ROWS rows
+// per workgroup; both are picked per operator by the tuner in phub_hv_launch.
+template <typename T, bool EPI, int COLS, int ROWS>
 __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict__ x,
                                                T *__restrict__ out, double scale,
                                                const T *__restrict__ prev, double beta,
@@ -266,8 +267,8 @@ __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict
     __syncthreads();
     double s_xo = 0.0, s_oo = 0.0;
     if (colok) {
-        for (int rr = wave * RPW + sub; rr < DN_ROWS; rr += 4 * RPW) {
-            const int64_t d = (int64_t)rb * DN_ROWS + rr;
+        for (int rr = wave * RPW + sub; rr < ROWS; rr += 4 * RPW) {
+            const int64_t d = (int64_t)rb * ROWS + rr;
             if (d >= a.dim_dn) break;
             const int32_t r0 = a.rp_dn[d], r1 = a.rp_dn[d + 1];
             const T *xc = x + col;
@@ -531,6 +532,116 @@ static HeisBlkArgs heis_blk_args(const phub_op *op) {
 
 constexpr int HUB_R = 4;
 
+// Candidate tiles of the down-hop kernel: (columns per row segment, rows per
+// workgroup).  FP32 and FP64 use the same byte widths (64, 128, 256 B segments).
+constexpr int DN_NCFG = 9;
+template <typename T>
+struct DnCfg {
+    static constexpr int cols[3] = {sizeof(T) == 4 ? 16 : 8, sizeof(T) == 4 ? 32 : 16,
+                                    sizeof(T) == 4 ? 64 : 32};
+    static constexpr int rows[3] = {8, 16, 32};
+};
+constexpr int DN_DEFAULT = 4;   // (128 B segments, 16 rows)
+
+int phub_dn_tile_code(int cfg, bool fp64) {
+    if (cfg < 0) return -1;
+    const int bytes = fp64 ? DnCfg<double>::cols[cfg / 3] * 8 : DnCfg<float>::cols[cfg / 3] * 4;
+    return bytes * 1000 + DnCfg<float>::rows[cfg % 3];
+}
+
+template <typename T, bool EPI, int COLS, int ROWS>
+static void dn_go(phub_ctx *c, const phub_op *op, const HubArgs<T> &a, const T *x, T *out,
+                  double scale, const T *prev, double beta, int *nblocks) {
+    const int nchunks = (int)((op->dim_up + COLS - 1) / COLS);
+    const int nrb = (int)((op->dim_dn + ROWS - 1) / ROWS);
+    *nblocks = (int)std::min<int64_t>((int64_t)8 * ((nchunks + 7) / 8) * nrb, INT32_MAX);
+    hipLaunchKernelGGL((k_hv_dn<T, EPI, COLS, ROWS>), dim3(*nblocks), dim3(256), 0, c->stream, a,
+                       x, out, scale, prev, beta, c->d_red, nchunks, nrb);
+}
+
+template <typename T, bool EPI>
+static void dn_dispatch(phub_ctx *c, const phub_op *op, const HubArgs<T> &a, const T *x, T *out,
+                        double scale, const T *prev, double beta, int cfg, int *nb) {
+    constexpr int C0 = DnCfg<T>::cols[0], C1 = DnCfg<T>::cols[1], C2 = DnCfg<T>::cols[2];
+    switch (cfg) {
+    case 0: dn_go<T, EPI, C0, 8>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 1: dn_go<T, EPI, C0, 16>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 2: dn_go<T, EPI, C0, 32>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 3: dn_go<T, EPI, C1, 8>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 4: dn_go<T, EPI, C1, 16>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 5: dn_go<T, EPI, C1, 32>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 6: dn_go<T, EPI, C2, 8>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 7: dn_go<T, EPI, C2, 16>(c, op, a, x, out, scale, prev, beta, nb); break;
+    default: dn_go<T, EPI, C2, 32>(c, op, a, x, out, scale, prev, beta, nb); break;
+    }
+}
+
+template <typename T>
+static int launch_dn(phub_ctx *c, const phub_op *op, const HubArgs<T> &a, const T *x, T *out,
+                     double scale, const T *prev, double beta, bool epi, int cfg, int *nblocks) {
+    const int cols = DnCfg<T>::cols[cfg / 3], rows = DnCfg<T>::rows[cfg % 3];
+    const int64_t nchunks = (op->dim_up + cols - 1) / cols, nrb = (op->dim_dn + rows - 1) / rows;
+    const int64_t g2 = 8 * ((nchunks + 7) / 8) * nrb;
+    if (g2 > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
+    if (epi) {
+        int rc = phub_ensure_red(c, (size_t)g2 * 2 + 64);
+        if (rc) return rc;
+        dn_dispatch<T, true>(c, op, a, x, out, scale, prev, beta, cfg, nblocks);
+    } else {
+        dn_dispatch<T, false>(c, op, a, x, out, scale, prev, beta, cfg, nblocks);
+    }
+    PHUB_HIP(c, hipGetLastError());
+    return PHUB_OK;
+}
+
+// Tuner: on an operator's first H*v in a precision, time every down-hop tile on
+// the caller's own vectors (up kernel, then the candidate; the best of two runs
+// each) and keep the fastest.  PHUB_TUNE=0 keeps the default tile, PHUB_DN_TILE=k
+// forces tile k.  Every run
+// rewrites out completely, so the caller's result is recomputed afterwards.
+template <typename T>
+static int tune_dn(phub_ctx *c, phub_op *op, const HubArgs<T> &a, const T *x, T *out,
+                   double scale, const T *prev, double beta, bool epi, unsigned g1, size_t lds,
+                   const T *cls) {
+    constexpr int R = sizeof(T) == 4 ? 2 : 1;
+    const int ti = sizeof(T) == 8;
+    const char *env = std::getenv("PHUB_TUNE");
+    if (env && env[0] == '0') {
+        op->tune_dn[ti] = DN_DEFAULT;
+        return PHUB_OK;
+    }
+    const char *force = std::getenv("PHUB_DN_TILE");   // a fixed tile index 0..8
+    if (force && force[0] >= '0' && force[0] < '0' + DN_NCFG) {
+        op->tune_dn[ti] = force[0] - '0';
+        return PHUB_OK;
+    }
+    float best = 1e30f;
+    int bestc = DN_DEFAULT;
+    for (int cfg = 0; cfg < DN_NCFG; ++cfg) {
+        float tmin = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+            hipLaunchKernelGGL((k_hv_up<T, R>), dim3(g1), dim3(UP_THREADS), lds, c->stream, a,
+                               op->up.packed, cls, op->up.ncls, x, out);
+            PHUB_HIP(c, hipEventRecord(c->ev0, c->stream));
+            int nb = 0;
+            int rc = launch_dn<T>(c, op, a, x, out, scale, prev, beta, epi, cfg, &nb);
+            if (rc) return rc;
+            PHUB_HIP(c, hipEventRecord(c->ev1, c->stream));
+            PHUB_HIP(c, hipEventSynchronize(c->ev1));
+            float ms = 0.f;
+            PHUB_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            tmin = std::min(tmin, ms);
+        }
+        if (std::getenv("PHUB_TUNE_VERBOSE"))
+            std::fprintf(stderr, "phub tune: %s down tile %d cols x %d rows: %.3f ms\n",
+                         sizeof(T) == 4 ? "fp32" : "fp64", DnCfg<T>::cols[cfg / 3],
+                         DnCfg<T>::rows[cfg % 3], tmin);
+        if (tmin < best) { best = tmin; bestc = cfg; }
+    }
+    op->tune_dn[ti] = bestc;
+    return PHUB_OK;
+}
+
 template <typename T>
 int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double scale,
                    const T *prev, double beta, bool want_dots, int *nblocks_out) {
@@ -555,21 +666,17 @@ int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double sc
             hipLaunchKernelGGL((k_hv_up<T, R>), dim3(g1), dim3(UP_THREADS), lds, c->stream, a,
                                op->up.packed, cls, op->up.ncls, x, out);
             PHUB_HIP(c, hipGetLastError());
-            constexpr int COLS = sizeof(T) == 4 ? 32 : 16;
-            const int nchunks = (int)((op->dim_up + COLS - 1) / COLS);
-            const int nrb = (int)((op->dim_dn + DN_ROWS - 1) / DN_ROWS);
-            const int64_t g2 = (int64_t)8 * ((nchunks + 7) / 8) * nrb;
-            if (g2 > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
-            nblocks = (int)g2;
-            if (epi) {
-                int rc = phub_ensure_red(c, (size_t)nblocks * 2 + 64);
+            const int ti = sizeof(T) == 8;
+            phub_op *mop = const_cast<phub_op *>(op);
+            if (mop->tune_dn[ti] < 0) {
+                int rc = tune_dn<T>(c, mop, a, x, out, scale, prev, beta, epi, g1, lds, cls);
                 if (rc) return rc;
-                hipLaunchKernelGGL((k_hv_dn<T, true, COLS>), dim3(nblocks), dim3(256), 0, c->stream, a,
-                                   x, out, scale, prev, beta, c->d_red, nchunks, nrb);
-            } else {
-                hipLaunchKernelGGL((k_hv_dn<T, false, COLS>), dim3(nblocks), dim3(256), 0, c->stream,
-                                   a, x, out, scale, prev, beta, c->d_red, nchunks, nrb);
+                // the tuner's runs overwrote out: recompute the up part
+                hipLaunchKernelGGL((k_hv_up<T, R>), dim3(g1), dim3(UP_THREADS), lds, c->stream, a,
+                                   op->up.packed, cls, op->up.ncls, x, out);
             }
+            int rc = launch_dn<T>(c, op, a, x, out, scale, prev, beta, epi, mop->tune_dn[ti], &nblocks);
+            if (rc) return rc;
             PHUB_HIP(c, hipGetLastError());
             if (nblocks_out) *nblocks_out = nblocks;
             return PHUB_OK;

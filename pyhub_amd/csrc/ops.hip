@@ -111,6 +111,7 @@ extern "C" int phub_op_get_info(const phub_op *op, phub_op_info *info) {
     info->max_hops_up = op->up.maxh;
     info->max_hops_dn = op->dn.maxh;
     info->table_bytes = op->table_bytes;
+    for (int t = 0; t < 2; ++t) info->dn_tile[t] = phub_dn_tile_code(op->tune_dn[t], t == 1);
     return PHUB_OK;
 }
 

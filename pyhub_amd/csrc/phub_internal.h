@@ -112,6 +112,8 @@ struct phub_op {
     int sign_convention = 0;
     double table_bytes = 0.0;
     phub_ctx *ctx = nullptr;
+    // down-hop tile chosen by the H*v tuner, per precision (FP32, FP64); -1 = not yet
+    int tune_dn[2] = {-1, -1};
 };
 
 // error helpers ------------------------------------------------------------------
@@ -157,6 +159,9 @@ int phub_ensure_red(phub_ctx *ctx, size_t ndoubles);
 // H*v launchers with a fused Lanczos epilogue:
 //   out = scale * (H x) - beta * prev   (prev may be null)
 //   partials: <x_scaled, out>, <out, out> per block (when want_dots)
+// tuner tile index -> (segment bytes * 1000 + rows), -1 if untuned (hv.hip)
+int phub_dn_tile_code(int cfg, bool fp64);
+
 template <typename T>
 int phub_hv_launch(phub_ctx *ctx, const phub_op *op, const T *x, T *out, double scale,
                    const T *prev, double beta, bool want_dots, int *nblocks_out);

@@ -48,7 +48,8 @@ def test_batched_clusters_vs_single_path(ctx):
                         ca = np.max(np.abs(ga[ci, site - 1, spin, kind, :k] - a[:k])) / sc
                         cb = np.max(np.abs(gb[ci, site - 1, spin, kind, :k - 1] - b[:k - 1])) / sc
                         cerrs.append((max(ca, cb), ci, site, spin, kind))
-                        assert abs(gn[ci, site - 1, spin, kind] - n2) < 1e-9
+                        # norm of the FP32-rounded excitation against FP64
+                        assert abs(gn[ci, site - 1, spin, kind] - n2) < 1e-6 * n2
                 err = np.max(np.abs(G[ci, site - 1, spin] - Gref)) / max(1e-12, np.max(np.abs(Gref)))
                 errs.append((err, ci, site, spin))
     errs.sort(reverse=True)
