@@ -176,14 +176,17 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
     }
     for (int i = threadIdx.x; i < ncls; i += UP_THREADS) cl[i] = cls[i];
     __syncthreads();
-    auto gather = [&](uint32_t j, T *v) {
-        if constexpr (R == 2 && sizeof(T) == 4) {
-            const float2 p = reinterpret_cast<const float2 *>(rows)[j];
+    // one column of the R staged rows is 8 bytes (two FP32 or one FP64); the packed
+    // entries carry that byte offset directly: e = (j * 8) << 8 | class
+    static_assert(R * sizeof(T) == 8, "8-byte LDS columns");
+    const char *rowb = reinterpret_cast<const char *>(rows);
+    auto gather = [&](uint32_t off, T *v) {
+        if constexpr (R == 2) {
+            const float2 p = *reinterpret_cast<const float2 *>(rowb + off);
             v[0] = p.x;
             v[1] = p.y;
         } else {
-#pragma unroll
-            for (int r = 0; r < R; ++r) v[r] = rows[j * R + r];
+            v[0] = *reinterpret_cast<const T *>(rowb + off);
         }
     };
     uint32_t xd[R];
@@ -198,7 +201,7 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
         const uint32_t xu = a.mu[u];
         const double eu = a.eu[u];
         T acc[R], self[R];
-        gather((uint32_t)u, self);
+        gather((uint32_t)u * 8u, self);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const uint32_t both = xu & xd[r];
@@ -216,12 +219,12 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
             acc[r] = (T)dg * self[r];
         }
         const int cnt = a.cu[u];
-        const uint32_t *pe = packed + u;
         int k = 0;
         for (; k + UP_UNROLL <= cnt; k += UP_UNROLL) {
             uint32_t e[UP_UNROLL];
+            // the entry row (k-major) is a wave-uniform base; u the per-lane offset
 #pragma unroll
-            for (int q = 0; q < UP_UNROLL; ++q) e[q] = pe[(size_t)(k + q) * nu];
+            for (int q = 0; q < UP_UNROLL; ++q) e[q] = (packed + (size_t)(k + q) * nu)[u];
 #pragma unroll
             for (int q = 0; q < UP_UNROLL; ++q) {
                 const T c = cl[e[q] & 255u];
@@ -232,7 +235,7 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
             }
         }
         for (; k < cnt; ++k) {
-            const uint32_t e = pe[(size_t)k * nu];
+            const uint32_t e = (packed + (size_t)k * nu)[u];
             const T c = cl[e & 255u];
             T v[R];
             gather(e >> 8, v);
@@ -245,60 +248,91 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
     }
 }
 
-// COLS columns per row segment (a wave covers 64/COLS rows at once), ROWS rows
-// per workgroup; both are picked per operator by the tuner in phub_hv_launch.
-template <typename T, bool EPI, int COLS, int ROWS>
+// COLS columns per row segment, ROWS rows per workgroup, VW columns per lane
+// (spaced COLS/VW apart, so every load instruction is one coalesced segment of
+// COLS/VW elements); a wave covers 64*VW/COLS rows at once.  The tile is picked
+// per operator by the tuner in phub_hv_launch.
+template <typename T, bool EPI, int COLS, int ROWS, int VW>
 __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict__ x,
                                                T *__restrict__ out, double scale,
                                                const T *__restrict__ prev, double beta,
                                                double *__restrict__ part, int nchunks, int nrb) {
-    constexpr int RPW = 64 / COLS;
+    constexpr int LPR = COLS / VW;          // lanes per row
+    constexpr int RPW = 64 / LPR;           // rows per wave
+    static_assert(LPR * VW == COLS && RPW * LPR == 64, "tile");
+    constexpr int UN = VW == 1 ? DN_UNROLL : (VW == 2 ? 4 : 2);
     const int b = blockIdx.x;
     const int xcd = b & 7, i = b >> 3;
     const int chunk = (i / nrb) * 8 + xcd;
     const int rb = i % nrb;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int sub = lane / COLS, cidx = lane % COLS;
+    const int sub = lane / LPR, cidx = lane % LPR;
     const int64_t nu = a.dim_up;
-    const int64_t col = (int64_t)chunk * COLS + cidx;
-    const bool colok = (chunk < nchunks) && (col < nu);
+    const int64_t col0 = (int64_t)chunk * COLS + cidx;
+    bool ok[VW];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < VW; ++j) {
+        ok[j] = (chunk < nchunks) && (col0 + j * LPR < nu);
+        any |= ok[j];
+    }
     __shared__ T cls[256];
     for (int q = threadIdx.x; q < a.ncls_dn; q += 256) cls[q] = a.cls_dn[q];
     __syncthreads();
     double s_xo = 0.0, s_oo = 0.0;
-    if (colok) {
+    if (any) {
+        // columns past the end read column col0 (in range) and are not stored
+        int64_t cj[VW];
+#pragma unroll
+        for (int j = 0; j < VW; ++j) cj[j] = ok[j] ? col0 + j * LPR : col0;
         for (int rr = wave * RPW + sub; rr < ROWS; rr += 4 * RPW) {
             const int64_t d = (int64_t)rb * ROWS + rr;
             if (d >= a.dim_dn) break;
             const int32_t r0 = a.rp_dn[d], r1 = a.rp_dn[d + 1];
-            const T *xc = x + col;
-            T s = (T)0;
+            T s[VW];
+#pragma unroll
+            for (int j = 0; j < VW; ++j) s[j] = (T)0;
             int32_t k = r0;
-            for (; k + DN_UNROLL <= r1; k += DN_UNROLL) {
-                uint32_t e[DN_UNROLL];
-                T v[DN_UNROLL];
+            for (; k + UN <= r1; k += UN) {
+                uint32_t e[UN];
+                T v[UN][VW];
 #pragma unroll
-                for (int q = 0; q < DN_UNROLL; ++q) e[q] = a.csr_dn[k + q];
+                for (int q = 0; q < UN; ++q) e[q] = a.csr_dn[k + q];
 #pragma unroll
-                for (int q = 0; q < DN_UNROLL; ++q) v[q] = xc[(int64_t)(e[q] >> 8) * nu];
+                for (int q = 0; q < UN; ++q) {
+                    const T *row = x + (int64_t)(e[q] >> 8) * nu;
 #pragma unroll
-                for (int q = 0; q < DN_UNROLL; ++q) s += cls[e[q] & 255u] * v[q];
+                    for (int j = 0; j < VW; ++j) v[q][j] = row[cj[j]];
+                }
+#pragma unroll
+                for (int q = 0; q < UN; ++q) {
+                    const T c = cls[e[q] & 255u];
+#pragma unroll
+                    for (int j = 0; j < VW; ++j) s[j] += c * v[q][j];
+                }
             }
             for (; k < r1; ++k) {
                 const uint32_t e = a.csr_dn[k];
-                s += cls[e & 255u] * xc[(int64_t)(e >> 8) * nu];
+                const T *row = x + (int64_t)(e >> 8) * nu;
+                const T c = cls[e & 255u];
+#pragma unroll
+                for (int j = 0; j < VW; ++j) s[j] += c * row[cj[j]];
             }
-            const size_t o = (size_t)d * nu + col;
-            if (EPI) {
-                // streamed once: non-temporal, so the slab keeps the L2
-                double val = scale * ((double)__builtin_nontemporal_load(out + o) + (double)s);
-                if (prev) val -= beta * (double)__builtin_nontemporal_load(prev + o);
-                const T vt = (T)val;
-                __builtin_nontemporal_store(vt, out + o);
-                s_xo += scale * (double)x[o] * (double)vt;
-                s_oo += (double)vt * (double)vt;
-            } else {
-                __builtin_nontemporal_store(__builtin_nontemporal_load(out + o) + s, out + o);
+#pragma unroll
+            for (int j = 0; j < VW; ++j) {
+                if (!ok[j]) continue;
+                const size_t o = (size_t)d * nu + cj[j];
+                if (EPI) {
+                    // streamed once: non-temporal, so the slab keeps the L2
+                    double val = scale * ((double)__builtin_nontemporal_load(out + o) + (double)s[j]);
+                    if (prev) val -= beta * (double)__builtin_nontemporal_load(prev + o);
+                    const T vt = (T)val;
+                    __builtin_nontemporal_store(vt, out + o);
+                    s_xo += scale * (double)x[o] * (double)vt;
+                    s_oo += (double)vt * (double)vt;
+                } else {
+                    __builtin_nontemporal_store(__builtin_nontemporal_load(out + o) + s[j], out + o);
+                }
             }
         }
     }
@@ -533,53 +567,64 @@ static HeisBlkArgs heis_blk_args(const phub_op *op) {
 constexpr int HUB_R = 4;
 
 // Candidate tiles of the down-hop kernel: (columns per row segment, rows per
-// workgroup).  FP32 and FP64 use the same byte widths (64, 128, 256 B segments).
-constexpr int DN_NCFG = 9;
-template <typename T>
-struct DnCfg {
-    static constexpr int cols[3] = {sizeof(T) == 4 ? 16 : 8, sizeof(T) == 4 ? 32 : 16,
-                                    sizeof(T) == 4 ? 64 : 32};
-    static constexpr int rows[3] = {8, 16, 32};
+// workgroup, columns per lane).  FP32 and FP64 list the same byte widths, from
+// 64 B to 1 KB per row segment.
+struct DnTile {
+    int cols, rows, vw;
 };
-constexpr int DN_DEFAULT = 4;   // (128 B segments, 16 rows)
+constexpr int DN_NCFG = 10;
+constexpr DnTile DN_F32[DN_NCFG] = {{16, 16, 1}, {32, 16, 1}, {32, 32, 1}, {64, 8, 1},
+                                    {64, 16, 1}, {64, 32, 1}, {128, 16, 2}, {128, 32, 2},
+                                    {256, 16, 4}, {256, 32, 4}};
+constexpr DnTile DN_F64[DN_NCFG] = {{8, 16, 1},  {16, 16, 1}, {16, 32, 1}, {32, 8, 1},
+                                    {32, 16, 1}, {32, 32, 1}, {64, 16, 2}, {64, 32, 2},
+                                    {128, 16, 4}, {128, 32, 4}};
+constexpr int DN_DEFAULT = 4;   // 256 B segments, 16 rows
+
+template <typename T>
+constexpr DnTile dn_tile(int cfg) {
+    return sizeof(T) == 4 ? DN_F32[cfg] : DN_F64[cfg];
+}
 
 int phub_dn_tile_code(int cfg, bool fp64) {
     if (cfg < 0) return -1;
-    const int bytes = fp64 ? DnCfg<double>::cols[cfg / 3] * 8 : DnCfg<float>::cols[cfg / 3] * 4;
-    return bytes * 1000 + DnCfg<float>::rows[cfg % 3];
+    const DnTile t = fp64 ? DN_F64[cfg] : DN_F32[cfg];
+    return t.cols * (fp64 ? 8 : 4) * 1000 + t.rows;
 }
 
-template <typename T, bool EPI, int COLS, int ROWS>
+template <typename T, bool EPI, int CFG>
 static void dn_go(phub_ctx *c, const phub_op *op, const HubArgs<T> &a, const T *x, T *out,
                   double scale, const T *prev, double beta, int *nblocks) {
-    const int nchunks = (int)((op->dim_up + COLS - 1) / COLS);
-    const int nrb = (int)((op->dim_dn + ROWS - 1) / ROWS);
+    constexpr DnTile t = dn_tile<T>(CFG);
+    const int nchunks = (int)((op->dim_up + t.cols - 1) / t.cols);
+    const int nrb = (int)((op->dim_dn + t.rows - 1) / t.rows);
     *nblocks = (int)std::min<int64_t>((int64_t)8 * ((nchunks + 7) / 8) * nrb, INT32_MAX);
-    hipLaunchKernelGGL((k_hv_dn<T, EPI, COLS, ROWS>), dim3(*nblocks), dim3(256), 0, c->stream, a,
-                       x, out, scale, prev, beta, c->d_red, nchunks, nrb);
+    hipLaunchKernelGGL((k_hv_dn<T, EPI, t.cols, t.rows, t.vw>), dim3(*nblocks), dim3(256), 0,
+                       c->stream, a, x, out, scale, prev, beta, c->d_red, nchunks, nrb);
 }
 
 template <typename T, bool EPI>
 static void dn_dispatch(phub_ctx *c, const phub_op *op, const HubArgs<T> &a, const T *x, T *out,
                         double scale, const T *prev, double beta, int cfg, int *nb) {
-    constexpr int C0 = DnCfg<T>::cols[0], C1 = DnCfg<T>::cols[1], C2 = DnCfg<T>::cols[2];
     switch (cfg) {
-    case 0: dn_go<T, EPI, C0, 8>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 1: dn_go<T, EPI, C0, 16>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 2: dn_go<T, EPI, C0, 32>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 3: dn_go<T, EPI, C1, 8>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 4: dn_go<T, EPI, C1, 16>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 5: dn_go<T, EPI, C1, 32>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 6: dn_go<T, EPI, C2, 8>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 7: dn_go<T, EPI, C2, 16>(c, op, a, x, out, scale, prev, beta, nb); break;
-    default: dn_go<T, EPI, C2, 32>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 0: dn_go<T, EPI, 0>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 1: dn_go<T, EPI, 1>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 2: dn_go<T, EPI, 2>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 3: dn_go<T, EPI, 3>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 4: dn_go<T, EPI, 4>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 5: dn_go<T, EPI, 5>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 6: dn_go<T, EPI, 6>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 7: dn_go<T, EPI, 7>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 8: dn_go<T, EPI, 8>(c, op, a, x, out, scale, prev, beta, nb); break;
+    default: dn_go<T, EPI, 9>(c, op, a, x, out, scale, prev, beta, nb); break;
     }
 }
 
 template <typename T>
 static int launch_dn(phub_ctx *c, const phub_op *op, const HubArgs<T> &a, const T *x, T *out,
                      double scale, const T *prev, double beta, bool epi, int cfg, int *nblocks) {
-    const int cols = DnCfg<T>::cols[cfg / 3], rows = DnCfg<T>::rows[cfg % 3];
+    const DnTile t = dn_tile<T>(cfg);
+    const int cols = t.cols, rows = t.rows;
     const int64_t nchunks = (op->dim_up + cols - 1) / cols, nrb = (op->dim_dn + rows - 1) / rows;
     const int64_t g2 = 8 * ((nchunks + 7) / 8) * nrb;
     if (g2 > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
@@ -610,9 +655,9 @@ static int tune_dn(phub_ctx *c, phub_op *op, const HubArgs<T> &a, const T *x, T 
         op->tune_dn[ti] = DN_DEFAULT;
         return PHUB_OK;
     }
-    const char *force = std::getenv("PHUB_DN_TILE");   // a fixed tile index 0..8
-    if (force && force[0] >= '0' && force[0] < '0' + DN_NCFG) {
-        op->tune_dn[ti] = force[0] - '0';
+    const char *force = std::getenv("PHUB_DN_TILE");   // a fixed tile index
+    if (force && std::atoi(force) >= 0 && std::atoi(force) < DN_NCFG) {
+        op->tune_dn[ti] = std::atoi(force);
         return PHUB_OK;
     }
     float best = 1e30f;
@@ -633,9 +678,9 @@ static int tune_dn(phub_ctx *c, phub_op *op, const HubArgs<T> &a, const T *x, T 
             tmin = std::min(tmin, ms);
         }
         if (std::getenv("PHUB_TUNE_VERBOSE"))
-            std::fprintf(stderr, "phub tune: %s down tile %d cols x %d rows: %.3f ms\n",
-                         sizeof(T) == 4 ? "fp32" : "fp64", DnCfg<T>::cols[cfg / 3],
-                         DnCfg<T>::rows[cfg % 3], tmin);
+            std::fprintf(stderr, "phub tune: %s down tile %d cols x %d rows (%d per lane): %.3f ms\n",
+                         sizeof(T) == 4 ? "fp32" : "fp64", dn_tile<T>(cfg).cols,
+                         dn_tile<T>(cfg).rows, dn_tile<T>(cfg).vw, tmin);
         if (tmin < best) { best = tmin; bestc = cfg; }
     }
     op->tune_dn[ti] = bestc;

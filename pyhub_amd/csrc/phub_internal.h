@@ -37,9 +37,10 @@ struct HopTable {
     double *c64 = nullptr;        // device, dim*maxh
     float *c32 = nullptr;         // device, dim*maxh
     int16_t *pair = nullptr;      // device, dim*maxh, (J-1)*n+(K-1) (all-pairs tables only)
-    // compressed k-major entries (idx << 8 | coefficient class) when the signed
-    // hopping values take at most 255 distinct values
-    uint32_t *packed = nullptr;   // device, dim*maxh (k-major tables)
+    // compressed entries when the signed hopping values take at most 255 distinct
+    // values: k-major ((idx * 8) << 8 | class, the target's byte offset in an LDS
+    // array of 8-byte columns), CSR (idx << 8 | class)
+    uint32_t *packed = nullptr;   // device, dim*maxh (k-major tables, dim < 2^21)
     int32_t *rowptr = nullptr;    // device, dim+1 (row-major tables: CSR of packed entries)
     uint32_t *csr = nullptr;      // device, rowptr[dim]
     int ncls = 0;

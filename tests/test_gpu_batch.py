@@ -43,10 +43,16 @@ def test_batched_clusters_vs_single_path(ctx):
                         Gref += cf_eval(ctx, a, b, n2, r.e0, sign, omega, eta)[0]
                         # the leading recurrence coefficients fix the low moments of
                         # the spectral function: FP32 chain vectors against FP64
+                        # compare up to the first beta that (nearly) terminates the
+                        # chain: past it the coefficients are rounding noise in both
+                        sc = max(1.0, np.max(np.abs(a[:10])))
                         k = min(10, len(a))
-                        sc = max(1.0, np.max(np.abs(a[:k])))
+                        small = np.nonzero(b[:k] < 1e-3 * sc)[0]
+                        if len(small):
+                            k = int(small[0]) + 1
                         ca = np.max(np.abs(ga[ci, site - 1, spin, kind, :k] - a[:k])) / sc
-                        cb = np.max(np.abs(gb[ci, site - 1, spin, kind, :k - 1] - b[:k - 1])) / sc
+                        cb = (np.max(np.abs(gb[ci, site - 1, spin, kind, :k - 1] - b[:k - 1])) / sc
+                              if k > 1 else 0.0)
                         cerrs.append((max(ca, cb), ci, site, spin, kind))
                         # norm of the FP32-rounded excitation against FP64
                         assert abs(gn[ci, site - 1, spin, kind] - n2) < 1e-6 * n2
@@ -56,6 +62,7 @@ def test_batched_clusters_vs_single_path(ctx):
     cerrs.sort(reverse=True)
     print("largest G_ii errors (rel. to max|G|, eta=1):", [(f"{e:.2e}", c, i, s) for e, c, i, s in errs[:5]],
           flush=True)
-    print("largest leading-coefficient errors:", [(f"{e:.2e}",) + t for e, *t in cerrs[:5]], flush=True)
+    print("largest leading-coefficient errors:", [(f"{e:.2e}",) + tuple(t) for e, *t in cerrs[:5]],
+          flush=True)
     assert cerrs[0][0] < 1e-5, cerrs[0]
     assert errs[0][0] < 5e-3, errs[0]
