@@ -464,6 +464,8 @@ void phub_free_heis(HeisTables *t) {
     if (t->tabA2) hipFree(t->tabA2);
     if (t->sbonds) hipFree(t->sbonds);
     if (t->items) hipFree(t->items);
+    if (t->ehh32) hipFree(t->ehh32);
+    if (t->ehh64) hipFree(t->ehh64);
     *t = HeisTables();
 }
 
@@ -591,6 +593,31 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
         }
         std::stable_sort(g[2].begin(), g[2].end(),
                          [](const HeisBond &p, const HeisBond &q) { return p.j > q.j; });
+        // high-high bonds by position relative to the A split at bit LA of the high half
+        const int LA = (n - L) / 2;
+        auto hcls = [&](const HeisBond &b) {
+            const bool jl = (b.j - L) < LA, kl = (b.k - L) < LA;
+            return (jl && kl) ? 0 : ((!jl && !kl) ? 1 : 2);
+        };
+        std::stable_sort(g[1].begin(), g[1].end(),
+                         [&](const HeisBond &p, const HeisBond &q) { return hcls(p) < hcls(q); });
+        out->nhh_lo = (int)std::count_if(g[1].begin(), g[1].end(), [&](const HeisBond &b) { return hcls(b) == 0; });
+        out->nhh_hi = (int)std::count_if(g[1].begin(), g[1].end(), [&](const HeisBond &b) { return hcls(b) == 1; });
+        {   // Ising weight of the high-high bonds per high pattern
+            const int nhp = n - L;
+            std::vector<double> e64((size_t)1 << nhp, 0.0);
+            for (uint32_t h = 0; h < (1u << nhp); ++h) {
+                double e = 0.0;
+                for (const HeisBond &b : g[1])
+                    if (((h >> (b.j - L)) ^ (h >> (b.k - L))) & 1u) e += b.hi;
+                e64[h] = e;
+            }
+            std::vector<float> e32(e64.begin(), e64.end());
+            PHUB_HIP(c, hipMalloc(&out->ehh64, e64.size() * sizeof(double)));
+            PHUB_HIP(c, hipMalloc(&out->ehh32, e32.size() * sizeof(float)));
+            PHUB_HIP(c, hipMemcpy(out->ehh64, e64.data(), e64.size() * sizeof(double), hipMemcpyHostToDevice));
+            PHUB_HIP(c, hipMemcpy(out->ehh32, e32.data(), e32.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
         out->ntt = (int)g[0].size();
         out->nhh = (int)g[1].size();
         out->nx = (int)g[2].size();

@@ -357,21 +357,25 @@ __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict
 // Ising weight of every antiparallel bond.
 struct HeisBlkArgs {
     int L, nh, k;
-    int ntt, nhh, nx;
+    int ntt, nhh, nx, nhh_lo, nhh_hi;
     int LA, na1, na2;    // split A tables: sizes of A1 and A2
+    int order;           // 0: each XCD takes one contiguous eighth of the items; 1: round robin
     int64_t nitems;
     uint64_t total_m1;   // C(n, n/2) - 1
     const uint64_t *tabB;
     const uint32_t *tabA1, *tabA2, *headpat, *items;
     const int64_t *hoff;
     const HeisBond *bonds;
+    const float *ehh32;
+    const double *ehh64;
     double dconst;
 };
 
 template <typename T>
-__device__ __forceinline__ T bond_c(const HeisBond &b, bool up) {
-    if constexpr (sizeof(T) == 4) return up ? b.fr : b.fj;
-    else return up ? b.cr : b.cj;
+__device__ __forceinline__ T bond_c(const HeisBond &b) {
+    // J is symmetric (checked when the operator is built), so cj == cr
+    if constexpr (sizeof(T) == 4) return b.fj;
+    else return b.cj;
 }
 template <typename T>
 __device__ __forceinline__ T bond_h(const HeisBond &b) {
@@ -379,10 +383,11 @@ __device__ __forceinline__ T bond_h(const HeisBond &b) {
     else return b.hi;
 }
 
-// Work item i of the launch runs on XCD (i % 8); the item list is dealt so that
-// each XCD takes one contiguous eighth of it (whole blocks stay in one L2).
 // Every load below is issued for all lanes (inactive lanes read an in-range
-// element and contribute zero), so gathers of different bonds overlap.
+// element and contribute zero), so gathers of different bonds overlap.  A of a
+// high pattern h = (hi, lo) split at bit LA is A1[hi] + A2[popc(hi)][lo]: a flip
+// inside lo keeps A1[hi] and the A2 row, a flip inside hi keeps A2[popc(hi)][lo],
+// so only straddling flips need both lookups.
 template <typename T, bool EPI>
 __global__ __launch_bounds__(256) void k_hv_heis_blk(HeisBlkArgs a, const T *__restrict__ x,
                                                      T *__restrict__ out, double scale,
@@ -394,7 +399,8 @@ __global__ __launch_bounds__(256) void k_hv_heis_blk(HeisBlkArgs a, const T *__r
     for (int i = threadIdx.x; i < a.na1; i += 256) sA1[i] = a.tabA1[i];
     for (int i = threadIdx.x; i < a.na2; i += 256) sA2[i] = a.tabA2[i];
     const int64_t per = (a.nitems + 7) / 8;
-    const int64_t it = (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int64_t it = a.order ? (int64_t)blockIdx.x
+                               : (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
     double s_xo = 0.0, s_oo = 0.0;
     __syncthreads();
     if (it < a.nitems) {
@@ -406,15 +412,12 @@ __global__ __launch_bounds__(256) void k_hv_heis_blk(HeisBlkArgs a, const T *__r
         const int len = (int)(a.hoff[m + 1] - h0);
         const int L = a.L, LA = a.LA;
         const uint32_t lomask = (1u << LA) - 1u;
-        auto Alook = [&](uint32_t h) -> int {
-            const uint32_t hi = h >> LA;
-            return (int)(sA1[hi] + sA2[((uint32_t)__popc(hi) << LA) + (h & lomask)]);
-        };
         const int64_t last = (int64_t)(a.total_m1 - a.tabB[((uint64_t)m << L) + xl]);
         const T *__restrict__ xb = x + last;   // inner index i lives at xb[-i]
         int ai[R];
         bool ok[R];
-        uint32_t xh[R];
+        uint32_t xh[R], lo[R], hi[R], row[R];
+        int A1h[R], A2l[R];
         T dg[R], acc[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -422,7 +425,13 @@ __global__ __launch_bounds__(256) void k_hv_heis_blk(HeisBlkArgs a, const T *__r
             ok[r] = i < len;
             ai[r] = ok[r] ? i : 0;   // inactive lanes read element 0 of the block
             xh[r] = ok[r] ? a.headpat[h0 + i] : 0u;
-            dg[r] = (T)0;
+            hi[r] = xh[r] >> LA;
+            lo[r] = xh[r] & lomask;
+            row[r] = (uint32_t)__popc(hi[r]) << LA;
+            A1h[r] = (int)sA1[hi[r]];
+            A2l[r] = (int)sA2[row[r] + lo[r]];
+            if constexpr (sizeof(T) == 4) dg[r] = a.ehh32[xh[r]];
+            else dg[r] = a.ehh64[xh[r]];
             acc[r] = (T)0;
         }
         T dgu = (T)a.dconst;
@@ -434,26 +443,50 @@ __global__ __launch_bounds__(256) void k_hv_heis_blk(HeisBlkArgs a, const T *__r
                 dgu -= bond_h<T>(bd);
                 const uint32_t xl2 = xl ^ ((1u << bd.j) | (1u << bd.k));
                 const int64_t last2 = (int64_t)(a.total_m1 - a.tabB[((uint64_t)m << L) + xl2]);
-                const T cf = bond_c<T>(bd, tj != 0);
+                const T cf = bond_c<T>(bd);
                 const T *__restrict__ xs = x + last2;
 #pragma unroll
                 for (int r = 0; r < R; ++r) acc[r] += cf * xs[-ai[r]];
             }
         }
-        // high-high bonds: per-lane existence, gather inside the block
-        for (int b = a.ntt; b < a.ntt + a.nhh; ++b) {
+        // high-high bonds (their Ising part is in dg already), by position in the A split
+        const int b0 = a.ntt, b1 = b0 + a.nhh_lo, b2 = b1 + a.nhh_hi, b3 = b0 + a.nhh;
+        for (int b = b0; b < b1; ++b) {   // both bits below LA
             const HeisBond bd = a.bonds[b];
             const int hj = bd.j - L, hk = bd.k - L;
             const uint32_t f = (1u << hj) | (1u << hk);
-            const T cj = bond_c<T>(bd, false), cr = bond_c<T>(bd, true), hh = bond_h<T>(bd);
+            const T c = bond_c<T>(bd);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const uint32_t vj = (xh[r] >> hj) & 1u, vk = (xh[r] >> hk) & 1u;
-                const bool on = ok[r] && (vj != vk);
-                const int a2 = on ? Alook(xh[r] ^ f) : ai[r];
-                const T c = on ? (vj ? cr : cj) : (T)0;
-                dg[r] += on ? hh : (T)0;
-                acc[r] += c * xb[-a2];
+                const bool on = ok[r] && (((xh[r] >> hj) ^ (xh[r] >> hk)) & 1u);
+                const int a2 = on ? A1h[r] + (int)sA2[row[r] + (lo[r] ^ f)] : ai[r];
+                acc[r] += (on ? c : (T)0) * xb[-a2];
+            }
+        }
+        for (int b = b1; b < b2; ++b) {   // both bits at or above LA
+            const HeisBond bd = a.bonds[b];
+            const int hj = bd.j - L, hk = bd.k - L;
+            const uint32_t f = ((1u << hj) | (1u << hk)) >> LA;
+            const T c = bond_c<T>(bd);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const bool on = ok[r] && (((xh[r] >> hj) ^ (xh[r] >> hk)) & 1u);
+                const int a2 = on ? (int)sA1[hi[r] ^ f] + A2l[r] : ai[r];
+                acc[r] += (on ? c : (T)0) * xb[-a2];
+            }
+        }
+        for (int b = b2; b < b3; ++b) {   // straddling the split
+            const HeisBond bd = a.bonds[b];
+            const int hj = bd.j - L, hk = bd.k - L;
+            const uint32_t f = (1u << hj) | (1u << hk);
+            const T c = bond_c<T>(bd);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const bool on = ok[r] && (((xh[r] >> hj) ^ (xh[r] >> hk)) & 1u);
+                const uint32_t h2 = xh[r] ^ f, hh2 = h2 >> LA;
+                const int av = (int)(sA1[hh2] + sA2[((uint32_t)__popc(hh2) << LA) + (h2 & lomask)]);
+                const int a2 = on ? av : ai[r];
+                acc[r] += (on ? c : (T)0) * xb[-a2];
             }
         }
         // cross bonds, grouped by their high site: one A lookup per group
@@ -466,25 +499,30 @@ __global__ __launch_bounds__(256) void k_hv_heis_blk(HeisBlkArgs a, const T *__r
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 hb[r] = (xh[r] >> hj) & 1u;
-                af[r] = Alook(xh[r] ^ (1u << hj));
+                const uint32_t h2 = xh[r] ^ (1u << hj), hh2 = h2 >> LA;
+                af[r] = (int)(sA1[hh2] + sA2[((uint32_t)__popc(hh2) << LA) + (h2 & lomask)]);
             }
+            T w0 = (T)0, w1 = (T)0;   // Ising weights of the group's bonds whose low bit is 0 / 1
             for (; b < bx1 && a.bonds[b].j == j; ++b) {
                 const HeisBond bd = a.bonds[b];
                 const uint32_t tb = (xl >> bd.k) & 1u;   // target's low bit; its high bit must differ
+                if (tb) w1 += bond_h<T>(bd);
+                else w0 += bond_h<T>(bd);
                 const int m2 = tb ? m + 1 : m - 1;        // popcount of the source's high half
                 if (m2 < 0 || m2 > a.nh) continue;        // no lane of this block can flip
                 const int64_t last2 = (int64_t)(a.total_m1 - a.tabB[((uint64_t)m2 << L) + (xl ^ (1u << bd.k))]);
-                const T cf = bond_c<T>(bd, tb == 0);      // target site J up iff tb == 0
-                const T hh = bond_h<T>(bd);
+                const T cf = bond_c<T>(bd);
                 const T *__restrict__ xs = x + last2;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const bool on = ok[r] && (hb[r] != tb);
                     const int a2 = on ? af[r] : 0;
-                    dg[r] += on ? hh : (T)0;
                     acc[r] += (on ? cf : (T)0) * xs[-a2];
                 }
             }
+            // antiparallel: high bit 1 with low bit 0, or high bit 0 with low bit 1
+#pragma unroll
+            for (int r = 0; r < R; ++r) dg[r] += hb[r] ? w0 : w1;
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -549,6 +587,10 @@ static HeisBlkArgs heis_blk_args(const phub_op *op) {
     a.nhh = h.nhh;
     a.nx = h.nx;
     a.LA = h.LA;
+    a.nhh_lo = h.nhh_lo;
+    a.nhh_hi = h.nhh_hi;
+    a.ehh32 = h.ehh32;
+    a.ehh64 = h.ehh64;
     a.na1 = 1 << (a.nh - h.LA);
     a.na2 = (a.nh - h.LA + 1) << h.LA;
     a.nitems = h.nitems;
@@ -755,12 +797,41 @@ int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double sc
         }
         HeisBlkArgs a = heis_blk_args(op);
         const size_t lds = (size_t)(a.na1 + a.na2) * sizeof(uint32_t);
-        if (epi)
-            hipLaunchKernelGGL((k_hv_heis_blk<T, true>), dim3(nblocks), dim3(256), lds, c->stream,
-                               a, x, out, scale, prev, beta, c->d_red);
-        else
-            hipLaunchKernelGGL((k_hv_heis_blk<T, false>), dim3(nblocks), dim3(256), lds, c->stream,
-                               a, x, out, scale, prev, beta, c->d_red);
+        auto go = [&](int order) {
+            a.order = order;
+            if (epi)
+                hipLaunchKernelGGL((k_hv_heis_blk<T, true>), dim3(nblocks), dim3(256), lds,
+                                   c->stream, a, x, out, scale, prev, beta, c->d_red);
+            else
+                hipLaunchKernelGGL((k_hv_heis_blk<T, false>), dim3(nblocks), dim3(256), lds,
+                                   c->stream, a, x, out, scale, prev, beta, c->d_red);
+        };
+        const int ti = sizeof(T) == 8;
+        phub_op *mop = const_cast<phub_op *>(op);
+        if (mop->tune_heis[ti] < 0) {
+            // tuner: time both item orders on the caller's vectors (the kernel
+            // rewrites out completely, so the final launch below gives the result)
+            const char *env = std::getenv("PHUB_TUNE");
+            int best = 0;
+            if (!(env && env[0] == '0')) {
+                float tb = 1e30f;
+                for (int order = 0; order < 2; ++order) {
+                    PHUB_HIP(c, hipEventRecord(c->ev0, c->stream));
+                    go(order);
+                    PHUB_HIP(c, hipGetLastError());
+                    PHUB_HIP(c, hipEventRecord(c->ev1, c->stream));
+                    PHUB_HIP(c, hipEventSynchronize(c->ev1));
+                    float ms = 0.f;
+                    PHUB_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+                    if (std::getenv("PHUB_TUNE_VERBOSE"))
+                        std::fprintf(stderr, "phub tune: %s heisenberg item order %d: %.3f ms\n",
+                                     sizeof(T) == 4 ? "fp32" : "fp64", order, ms);
+                    if (ms < tb) { tb = ms; best = order; }
+                }
+            }
+            mop->tune_heis[ti] = best;
+        }
+        go(mop->tune_heis[ti]);
     }
     PHUB_HIP(c, hipGetLastError());
     if (nblocks_out) *nblocks_out = nblocks;

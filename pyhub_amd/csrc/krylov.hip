@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <type_traits>
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
@@ -230,6 +231,156 @@ __global__ __launch_bounds__(256) void k_multidot(const T *__restrict__ basis, i
     for (int j = threadIdx.x; j < k; j += blockDim.x) part[(size_t)blockIdx.x * k + j] = acc[j];
 }
 
+// Vectorised forms of the two kernels above for 16-byte aligned vectors (the
+// Krylov basis is allocated with a leading dimension padded to 64 elements).
+// k_multidot_v: each thread keeps its 16 / sizeof(T) * NV elements of the
+// staged w - a_eff * xk in FP64 registers and streams every basis vector
+// through 16-byte loads (two vectors per iteration, so eight loads are in
+// flight); per-wave partial dots go to LDS, one slot per wave and vector.
+template <typename T, typename TW>
+__global__ __launch_bounds__(256) void k_multidot_v(const T *__restrict__ basis, int64_t ld, int k,
+                                                    const TW *__restrict__ w,
+                                                    const T *__restrict__ xk, double a_eff,
+                                                    int64_t n, double *__restrict__ part) {
+    constexpr int VEC = 16 / sizeof(T);
+    constexpr int NV = 4;
+    constexpr int E = VEC * NV;
+    constexpr int64_t S = 256 * E;
+    extern __shared__ double acc[];   // 4 * k
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int j = threadIdx.x; j < 4 * k; j += 256) acc[j] = 0.0;
+    __syncthreads();
+    for (int64_t base = (int64_t)blockIdx.x * S; base < n; base += (int64_t)gridDim.x * S) {
+        const bool full = base + S <= n;
+        double ws[E];
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                const int64_t i = base + ((int64_t)q * 256 + threadIdx.x) * VEC + e;
+                ws[q * VEC + e] = (full || i < n)
+                                      ? (double)w[i] - (xk ? a_eff * (double)xk[i] : 0.0)
+                                      : 0.0;
+            }
+        auto dot = [&](const T *vj) -> double {
+            double sacc = 0.0;
+            if (full) {
+                using V4 = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+                V4 v[NV];
+#pragma unroll
+                for (int q = 0; q < NV; ++q)
+                    v[q] = *reinterpret_cast<const V4 *>(vj + base + ((int64_t)q * 256 + threadIdx.x) * VEC);
+#pragma unroll
+                for (int q = 0; q < NV; ++q) {
+                    const T *pv = reinterpret_cast<const T *>(&v[q]);
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) sacc += ws[q * VEC + e] * (double)pv[e];
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < NV; ++q)
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) {
+                        const int64_t i = base + ((int64_t)q * 256 + threadIdx.x) * VEC + e;
+                        if (i < n) sacc += ws[q * VEC + e] * (double)vj[i];
+                    }
+            }
+            return sacc;
+        };
+        int j = 0;
+        for (; j + 1 < k; j += 2) {
+            double s0 = dot(basis + (size_t)j * ld);
+            double s1 = dot(basis + (size_t)(j + 1) * ld);
+            s0 = wave_sum(s0);
+            s1 = wave_sum(s1);
+            if (lane == 0) {
+                acc[wave * k + j] += s0;
+                acc[wave * k + j + 1] += s1;
+            }
+        }
+        if (j < k) {
+            const double s0 = wave_sum(dot(basis + (size_t)j * ld));
+            if (lane == 0) acc[wave * k + j] += s0;
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < k; j += 256)
+        part[(size_t)blockIdx.x * k + j] = acc[j] + acc[k + j] + acc[2 * k + j] + acc[3 * k + j];
+}
+
+// k_combine_v: w -= sum_j coef_j v_j over 16 / sizeof(T) * 2 elements per thread
+// with 16-byte loads (two basis vectors per iteration); psi_mode as k_combine.
+template <typename T, typename TW>
+__global__ __launch_bounds__(256) void k_combine_v(const T *__restrict__ basis, int64_t ld, int k,
+                                                   const double *__restrict__ coef,
+                                                   TW *__restrict__ w, int64_t n, int psi_mode,
+                                                   double *__restrict__ part) {
+    constexpr int VEC = 16 / sizeof(T);
+    constexpr int NV = 2;
+    constexpr int E = VEC * NV;
+    constexpr int64_t S = 256 * E;
+    using V4 = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    extern __shared__ double cs[];
+    __shared__ double sh[4];
+    for (int j = threadIdx.x; j < k; j += blockDim.x) cs[j] = coef[j];
+    __syncthreads();
+    double s2 = 0.0;
+    for (int64_t base = (int64_t)blockIdx.x * S; base < n; base += (int64_t)gridDim.x * S) {
+        const bool full = base + S <= n;
+        double sum[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) sum[e] = 0.0;
+        auto add = [&](int j) {
+            const T *vj = basis + (size_t)j * ld + base;
+            const double cj = cs[j];
+            if (full) {
+                V4 v[NV];
+#pragma unroll
+                for (int q = 0; q < NV; ++q)
+                    v[q] = *reinterpret_cast<const V4 *>(vj + ((int64_t)q * 256 + threadIdx.x) * VEC);
+#pragma unroll
+                for (int q = 0; q < NV; ++q) {
+                    const T *pv = reinterpret_cast<const T *>(&v[q]);
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) sum[q * VEC + e] += cj * (double)pv[e];
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < NV; ++q)
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) {
+                        const int64_t o = ((int64_t)q * 256 + threadIdx.x) * VEC + e;
+                        if (base + o < n) sum[q * VEC + e] += cj * (double)vj[o];
+                    }
+            }
+        };
+        int j = 0;
+        for (; j + 1 < k; j += 2) {
+            add(j);
+            add(j + 1);
+        }
+        if (j < k) add(j);
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                const int64_t i = base + ((int64_t)q * 256 + threadIdx.x) * VEC + e;
+                if (full || i < n) {
+                    const double v = psi_mode ? sum[q * VEC + e] : (double)w[i] - sum[q * VEC + e];
+                    const TW t = (TW)v;
+                    w[i] = t;
+                    s2 += (double)t * (double)t;
+                }
+            }
+    }
+    if (part) {
+        s2 = wave_sum(s2);
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s2;
+        __syncthreads();
+        if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    }
+}
+
 // w -= sum_j coef_j * xt_j  (coef in FP64); partial |w|^2.  When psi_mode,
 // w = sum_j coef_j xt_j (w need not be initialised).
 template <typename T, typename TW>
@@ -293,16 +444,34 @@ static int dev_axpby(phub_ctx *c, double a, const TX *x, double b, TY *y, int64_
     return PHUB_OK;
 }
 
+// 16-byte aligned pointer and a leading dimension that keeps every vector aligned
+static inline bool vec_ok(const void *p, int64_t ld) {
+    return ((uintptr_t)p % 16 == 0) && (ld % 4 == 0);
+}
+
+// leading dimension of a stored Krylov basis: padded to 64 elements (256 B in FP32)
+static inline int64_t basis_ld(int64_t n) { return (n + 63) / 64 * 64; }
+
 template <typename T, typename TW>
 static int dev_multidot(phub_ctx *c, const T *basis, int64_t ld, int k, const TW *w, int64_t n,
                         double *out, const T *xk = nullptr, double a_eff = 0.0) {
-    int64_t slices = (n + MD_SLICE - 1) / MD_SLICE;
-    unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(slices, 1024));
-    int rc = phub_ensure_red(c, (size_t)g * k + k + 64);
-    if (rc) return rc;
     if (k > 4096) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "too many basis vectors");
-    hipLaunchKernelGGL((k_multidot<T, TW>), dim3(g), dim3(256), k * sizeof(double), c->stream,
-                       basis, ld, k, w, xk, a_eff, n, c->d_red);
+    unsigned g;
+    if (vec_ok(basis, ld) && vec_ok(w, 0) && (!xk || vec_ok(xk, 0)) && k <= 1536) {
+        const int64_t slices = (n + 256 * 64 / (int64_t)sizeof(T) - 1) / (256 * 64 / (int64_t)sizeof(T));
+        g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(slices, 2048));
+        int rc = phub_ensure_red(c, (size_t)g * k + k + 64);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_multidot_v<T, TW>), dim3(g), dim3(256), 4 * k * sizeof(double),
+                           c->stream, basis, ld, k, w, xk, a_eff, n, c->d_red);
+    } else {
+        const int64_t slices = (n + MD_SLICE - 1) / MD_SLICE;
+        g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(slices, 1024));
+        int rc = phub_ensure_red(c, (size_t)g * k + k + 64);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_multidot<T, TW>), dim3(g), dim3(256), k * sizeof(double), c->stream,
+                           basis, ld, k, w, xk, a_eff, n, c->d_red);
+    }
     PHUB_HIP(c, hipGetLastError());
     // reduce k components; host buffer may need to grow
     if ((size_t)k > c->h_red_cap) {
@@ -321,9 +490,14 @@ static int dev_combine(phub_ctx *c, const T *basis, int64_t ld, int k, const dou
     unsigned g = grid_for(n);
     int rc = phub_ensure_red(c, g + 64);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_combine<T, TW>), dim3(g), dim3(256), k * sizeof(double), c->stream,
-                       basis, ld, k, d_coef, w, n, psi_mode ? 1 : 0,
-                       norm2 ? c->d_red : nullptr);
+    if (vec_ok(basis, ld) && vec_ok(w, 0) && sizeof(TW) == sizeof(T))
+        hipLaunchKernelGGL((k_combine_v<T, TW>), dim3(g), dim3(256), k * sizeof(double), c->stream,
+                           basis, ld, k, d_coef, w, n, psi_mode ? 1 : 0,
+                           norm2 ? c->d_red : nullptr);
+    else
+        hipLaunchKernelGGL((k_combine<T, TW>), dim3(g), dim3(256), k * sizeof(double), c->stream,
+                           basis, ld, k, d_coef, w, n, psi_mode ? 1 : 0,
+                           norm2 ? c->d_red : nullptr);
     PHUB_HIP(c, hipGetLastError());
     if (norm2) return phub_reduce_partials(c, (int)g, 1, norm2);
     // keep d_coef alive until the kernel has read it
@@ -567,7 +741,7 @@ static int refine64(phub_ctx *c, const phub_op *op, double *psi, double *basis64
     o.check_every = 1000000;
     LczState s;
     std::vector<double> sig;
-    int rc = lanczos_full<double>(c, op, &o, basis64, n, s, sig, d_coef, nullptr, true);
+    int rc = lanczos_full<double>(c, op, &o, basis64, basis_ld(n), s, sig, d_coef, nullptr, true);
     if (rc) return rc;
     double e0 = 0, e1 = 0;
     std::vector<double> y;
@@ -576,7 +750,7 @@ static int refine64(phub_ctx *c, const phub_op *op, double *psi, double *basis64
     std::vector<double> cf(s.steps);
     for (int j = 0; j < s.steps; ++j) cf[j] = y[j] * sig[j];
     double nn = 0.0;
-    rc = dev_combine<double, double>(c, basis64, n, s.steps, cf.data(), psi, n, true, &nn, d_coef);
+    rc = dev_combine<double, double>(c, basis64, basis_ld(n), s.steps, cf.data(), psi, n, true, &nn, d_coef);
     if (rc) return rc;
     rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / std::sqrt(nn), psi, n,
                                    nullptr);
@@ -698,7 +872,8 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
         if (rc) return rc;
     } else if (o->mode == PHUB_LCZ_FULL64) {
         DevBuf<double> basis;
-        size_t bytes = (size_t)(oo.max_steps + 1) * n * sizeof(double);
+        const int64_t ld = basis_ld(n);
+        size_t bytes = (size_t)(oo.max_steps + 1) * ld * sizeof(double);
         hipError_t e = hipMalloc(&basis.p, bytes);
         if (e != hipSuccess)
             return phub_fail(c, PHUB_ERR_OOM, "FP64 Krylov basis does not fit: " +
@@ -708,7 +883,7 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
         else
             hipLaunchKernelGGL(k_randvec<double>, dim3(g), dim3(256), 0, c->stream, basis.p, n, o->seed);
         std::vector<double> sig;
-        rc = lanczos_full<double>(c, op, &oo, basis.p, n, s, sig, coef.p, &tm, fixed);
+        rc = lanczos_full<double>(c, op, &oo, basis.p, ld, s, sig, coef.p, &tm, fixed);
         if (rc) return rc;
         t_lcz = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         res->e0_ritz = res->e0 = s.e0;
@@ -720,7 +895,7 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
             std::vector<double> cf(s.steps);
             for (int j = 0; j < s.steps; ++j) cf[j] = y[j] * sig[j];
             double nn = 0;
-            rc = dev_combine<double, double>(c, basis.p, n, s.steps, cf.data(), d_psi, n, true, &nn, coef.p);
+            rc = dev_combine<double, double>(c, basis.p, ld, s.steps, cf.data(), d_psi, n, true, &nn, coef.p);
             if (rc) return rc;
             if (std::fabs(1.0 - std::sqrt(nn)) > 1e-5) s.norm_lost = true;
             rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / std::sqrt(nn),
@@ -737,7 +912,8 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
         if (rc) return rc;
     } else {   // MIXED
         DevBuf<float> basis;
-        size_t bytes = (size_t)(oo.max_steps + 1) * n * sizeof(float);
+        const int64_t ld = basis_ld(n);
+        size_t bytes = (size_t)(oo.max_steps + 1) * ld * sizeof(float);
         hipError_t e = hipMalloc(&basis.p, bytes);
         if (e != hipSuccess)
             return phub_fail(c, PHUB_ERR_OOM, "FP32 Krylov basis does not fit: " +
@@ -747,7 +923,7 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
         else
             hipLaunchKernelGGL(k_randvec<float>, dim3(g), dim3(256), 0, c->stream, basis.p, n, o->seed);
         std::vector<double> sig;
-        rc = lanczos_full<float>(c, op, &oo, basis.p, n, s, sig, coef.p, &tm, fixed);
+        rc = lanczos_full<float>(c, op, &oo, basis.p, ld, s, sig, coef.p, &tm, fixed);
         if (rc) return rc;
         t_lcz = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         res->e0_ritz = res->e0 = s.e0;
@@ -760,7 +936,7 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
             std::vector<double> cf(s.steps);
             for (int j = 0; j < s.steps; ++j) cf[j] = y[j] * sig[j];
             double nn = 0;
-            rc = dev_combine<float, double>(c, basis.p, n, s.steps, cf.data(), d_psi, n, true, &nn, coef.p);
+            rc = dev_combine<float, double>(c, basis.p, ld, s.steps, cf.data(), d_psi, n, true, &nn, coef.p);
             if (rc) return rc;
             rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / std::sqrt(nn),
                                            d_psi, n, nullptr);
@@ -771,7 +947,7 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
             const int rs = std::max(0, std::min(o->refine_steps, 4));
             DevBuf<double> b64;
             const int m = 3;
-            PHUB_HIP(c, hipMalloc(&b64.p, (size_t)(m + 1) * n * sizeof(double)));
+            PHUB_HIP(c, hipMalloc(&b64.p, (size_t)(m + 1) * basis_ld(n) * sizeof(double)));
             for (int it = 0; it < rs; ++it) {
                 double er = 0;
                 rc = refine64(c, op, d_psi, b64.p, m, coef.p, &er);

@@ -91,6 +91,12 @@ struct HeisTables {
     // half) ordered by J so that bonds sharing their high site are adjacent
     HeisBond *sbonds = nullptr;    // device
     int ntt = 0, nhh = 0, nx = 0;
+    // high-high bonds are further ordered by where their two bits fall in the A split:
+    // [0,nhh_lo) both below LA, [nhh_lo,nhh_lo+nhh_hi) both at or above, then straddling
+    int nhh_lo = 0, nhh_hi = 0;
+    // half Ising weight of the antiparallel high-high bonds of each high pattern
+    float *ehh32 = nullptr;        // device, 2^(n-L)
+    double *ehh64 = nullptr;       // device, 2^(n-L)
     // work items of the block H*v: (low pattern << 12) | chunk, in memory order
     uint32_t *items = nullptr;     // device
     int64_t nitems = 0;
@@ -115,6 +121,8 @@ struct phub_op {
     phub_ctx *ctx = nullptr;
     // down-hop tile chosen by the H*v tuner, per precision (FP32, FP64); -1 = not yet
     int tune_dn[2] = {-1, -1};
+    // Heisenberg work-item order chosen by the tuner (0 XCD-contiguous, 1 round robin)
+    int tune_heis[2] = {-1, -1};
 };
 
 // error helpers ------------------------------------------------------------------
