@@ -308,15 +308,16 @@ __global__ __launch_bounds__(256) void k_multidot_v(const T *__restrict__ basis,
         part[(size_t)blockIdx.x * k + j] = acc[j] + acc[k + j] + acc[2 * k + j] + acc[3 * k + j];
 }
 
-// k_combine_v: w -= sum_j coef_j v_j over 16 / sizeof(T) * 2 elements per thread
-// with 16-byte loads (two basis vectors per iteration); psi_mode as k_combine.
+// k_combine_v: w -= sum_j coef_j v_j over 16 / sizeof(T) * 4 elements per thread
+// with 16-byte loads (two basis vectors per iteration, eight loads in flight);
+// psi_mode as k_combine.
 template <typename T, typename TW>
 __global__ __launch_bounds__(256) void k_combine_v(const T *__restrict__ basis, int64_t ld, int k,
                                                    const double *__restrict__ coef,
                                                    TW *__restrict__ w, int64_t n, int psi_mode,
                                                    double *__restrict__ part) {
     constexpr int VEC = 16 / sizeof(T);
-    constexpr int NV = 2;
+    constexpr int NV = 4;
     constexpr int E = VEC * NV;
     constexpr int64_t S = 256 * E;
     using V4 = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
