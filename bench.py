@@ -164,7 +164,8 @@ def main():
                "fp32_gbs": round(gbs, 1), "fp64_gbs": round((2 * 8 * D + 2 * 12870 * hbar * 12) / (ms64 * 1e-3) / 1e9, 1),
                "mean_hops_per_species": hbar, "dn_tile_fp32": tiles[0], "dn_tile_fp64": tiles[1]},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(gbs / 8000.0, 4), "traffic": None,
+                     "frac": round(gbs / 8000.0, 4),
+                     "traffic": _profile_traffic("k_hv_hubbard_4x4.md"),
                      "kernel": "k_hv_up<float,2> + k_hv_dn<float,false> (one H*v)",
                      "valu_fp32_tflops": round(flops / (ms_avg * 1e-3) / 1e12, 2)},
     }
@@ -178,6 +179,19 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = _cpu_baseline(T, U, D, b_alg)
     print(json.dumps(out), flush=True)
+
+
+def _profile_traffic(name):
+    """Fabric bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated against a
+    1 GiB copy) from the committed rocprofv3 summary, or None when it is absent."""
+    path = os.path.join(HERE, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_hv_fp32:"):
+                return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
 
 
 def _calib(out, torch, dev):
@@ -263,7 +277,8 @@ def _heis_bench(args, ctx, torch, L):
         "hv": {"fp32_ms": round(ms_avg, 3), "states_per_s": D / (ms_avg * 1e-3),
                "alg_bytes": b_alg, "alg_flops": flops, "mean_flips_per_state": hbar},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(gbs / 8000.0, 4), "traffic": None,
+                     "frac": round(gbs / 8000.0, 4),
+                     "traffic": _profile_traffic("k_hv_heis_blk_j1j2_6x6.md") if (Lx, Ly) == (6, 6) else None,
                      "kernel": "k_hv_heis_blk<float,false>",
                      "valu_fp32_tflops": round(flops / (ms_avg * 1e-3) / 1e12, 2)},
     }

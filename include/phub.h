@@ -182,6 +182,23 @@ int phub_excite(phub_ctx *ctx, const phub_op *src, const phub_op *dst, int site,
 int phub_cf_chain(phub_ctx *ctx, const phub_op *op, const double *d_start, int nsteps,
                   int precision, double *alpha, double *beta, double *norm2);
 
+/* Band Lanczos for the full G_ij (GF_BANDLANCZOS, spgf_bandlanczos.f90:309-448, and
+ * the weights Q of SPGF_BANDLANCZOS, :90-100).  d_V: p start vectors (device,
+ * FP64, vector i at d_V + i*dim, not modified).  They are orthonormalised
+ * (Gram-Schmidt, V = Q R^T); the band recurrence adds one vector per step with a
+ * deflation threshold of 1e-10 and checks the lowest eigenvalue of the band
+ * matrix every p steps against tol, up to max_steps - 10 steps (as the
+ * reference).  Outputs (host): positions[*npoles] ascending (eigenvalues of the
+ * band matrix after deflated rows are dropped) and
+ * Q[k*p + i] = sum_{j<=i} R(i,j) VEC(j,k) for pole k, site i.  positions and Q
+ * must hold max_steps + p and (max_steps + p) * p doubles. */
+typedef struct {
+    int max_steps;          /* reference max_lcz */
+    double tol;             /* reference acc_lcz */
+} phub_band_opts;
+int phub_band_lanczos(phub_ctx *ctx, const phub_op *op, const double *d_V, int p,
+                      const phub_band_opts *opts, double *positions, double *Q, int *npoles);
+
 /* ---- batched clusters --------------------------------------------------------
  * Many independent Hubbard clusters in one persistent launch: each workgroup
  * runs whole Lanczos chains of one work item (ground state, then G_ii chains)

@@ -241,7 +241,10 @@ template <typename T, typename TW>
 __global__ __launch_bounds__(256) void k_multidot_v(const T *__restrict__ basis, int64_t ld, int k,
                                                     const TW *__restrict__ w,
                                                     const T *__restrict__ xk, double a_eff,
-                                                    int64_t n, double *__restrict__ part) {
+                                                    int64_t n, double *__restrict__ part,
+                                                    const T *const *__restrict__ ptrs) {
+    // vector j: ptrs[j] when a pointer table is given, else basis + j * ld
+    auto vec = [&](int j) { return ptrs ? ptrs[j] : basis + (size_t)j * ld; };
     constexpr int VEC = 16 / sizeof(T);
     constexpr int NV = 4;
     constexpr int E = VEC * NV;
@@ -289,8 +292,8 @@ __global__ __launch_bounds__(256) void k_multidot_v(const T *__restrict__ basis,
         };
         int j = 0;
         for (; j + 1 < k; j += 2) {
-            double s0 = dot(basis + (size_t)j * ld);
-            double s1 = dot(basis + (size_t)(j + 1) * ld);
+            double s0 = dot(vec(j));
+            double s1 = dot(vec(j + 1));
             s0 = wave_sum(s0);
             s1 = wave_sum(s1);
             if (lane == 0) {
@@ -299,7 +302,7 @@ __global__ __launch_bounds__(256) void k_multidot_v(const T *__restrict__ basis,
             }
         }
         if (j < k) {
-            const double s0 = wave_sum(dot(basis + (size_t)j * ld));
+            const double s0 = wave_sum(dot(vec(j)));
             if (lane == 0) acc[wave * k + j] += s0;
         }
     }
@@ -315,7 +318,8 @@ template <typename T, typename TW>
 __global__ __launch_bounds__(256) void k_combine_v(const T *__restrict__ basis, int64_t ld, int k,
                                                    const double *__restrict__ coef,
                                                    TW *__restrict__ w, int64_t n, int psi_mode,
-                                                   double *__restrict__ part) {
+                                                   double *__restrict__ part,
+                                                   const T *const *__restrict__ ptrs) {
     constexpr int VEC = 16 / sizeof(T);
     constexpr int NV = 4;
     constexpr int E = VEC * NV;
@@ -332,7 +336,7 @@ __global__ __launch_bounds__(256) void k_combine_v(const T *__restrict__ basis, 
 #pragma unroll
         for (int e = 0; e < E; ++e) sum[e] = 0.0;
         auto add = [&](int j) {
-            const T *vj = basis + (size_t)j * ld + base;
+            const T *vj = (ptrs ? ptrs[j] : basis + (size_t)j * ld) + base;
             const double cj = cs[j];
             if (full) {
                 V4 v[NV];
@@ -464,7 +468,8 @@ static int dev_multidot(phub_ctx *c, const T *basis, int64_t ld, int k, const TW
         int rc = phub_ensure_red(c, (size_t)g * k + k + 64);
         if (rc) return rc;
         hipLaunchKernelGGL((k_multidot_v<T, TW>), dim3(g), dim3(256), 4 * k * sizeof(double),
-                           c->stream, basis, ld, k, w, xk, a_eff, n, c->d_red);
+                           c->stream, basis, ld, k, w, xk, a_eff, n, c->d_red,
+                           (const T *const *)nullptr);
     } else {
         const int64_t slices = (n + MD_SLICE - 1) / MD_SLICE;
         g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(slices, 1024));
@@ -494,7 +499,7 @@ static int dev_combine(phub_ctx *c, const T *basis, int64_t ld, int k, const dou
     if (vec_ok(basis, ld) && vec_ok(w, 0) && sizeof(TW) == sizeof(T))
         hipLaunchKernelGGL((k_combine_v<T, TW>), dim3(g), dim3(256), k * sizeof(double), c->stream,
                            basis, ld, k, d_coef, w, n, psi_mode ? 1 : 0,
-                           norm2 ? c->d_red : nullptr);
+                           norm2 ? c->d_red : nullptr, (const T *const *)nullptr);
     else
         hipLaunchKernelGGL((k_combine<T, TW>), dim3(g), dim3(256), k * sizeof(double), c->stream,
                            basis, ld, k, d_coef, w, n, psi_mode ? 1 : 0,
@@ -1009,6 +1014,287 @@ static int cf_chain_t(phub_ctx *c, const phub_op *op, const double *start, int n
         alpha[i] = s.alpha[i];
         beta[i] = s.beta[i + 1];
     }
+    return PHUB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// dense symmetric eigensolver for the band matrix (host): Householder reduction
+// to tridiagonal form, then the implicit QL above.  A is row-major m x m and
+// destroyed; w ascending; Z (optional) column j = eigenvector j (Z[j*m + i]).
+// ---------------------------------------------------------------------------
+static int sym_eig(int m, std::vector<double> &A, std::vector<double> &w, std::vector<double> *Z) {
+    w.assign(m, 0.0);
+    if (m == 0) return 0;
+    std::vector<double> Qm;
+    if (Z) {
+        Qm.assign((size_t)m * m, 0.0);
+        for (int i = 0; i < m; ++i) Qm[(size_t)i * m + i] = 1.0;
+    }
+    std::vector<double> v(m), pv(m), q(m);
+    for (int k = 0; k + 2 < m; ++k) {
+        double sn = 0.0;
+        for (int i = k + 1; i < m; ++i) sn += A[(size_t)i * m + k] * A[(size_t)i * m + k];
+        if (sn == 0.0) continue;
+        double al = std::sqrt(sn);
+        if (A[(size_t)(k + 1) * m + k] > 0.0) al = -al;
+        std::fill(v.begin(), v.end(), 0.0);
+        v[k + 1] = A[(size_t)(k + 1) * m + k] - al;
+        for (int i = k + 2; i < m; ++i) v[i] = A[(size_t)i * m + k];
+        double vn2 = 0.0;
+        for (int i = k + 1; i < m; ++i) vn2 += v[i] * v[i];
+        if (vn2 == 0.0) continue;
+        const double bt = 2.0 / vn2;
+        // A <- H A H with H = I - bt v v^T:  p = bt A v, q = p - (bt v.p / 2) v,
+        // A -= v q^T + q v^T
+        double vp = 0.0;
+        for (int i = k; i < m; ++i) {
+            double t = 0.0;
+            for (int j = k + 1; j < m; ++j) t += A[(size_t)i * m + j] * v[j];
+            pv[i] = bt * t;
+            vp += v[i] * pv[i];
+        }
+        const double K = 0.5 * bt * vp;
+        for (int i = k; i < m; ++i) q[i] = pv[i] - K * v[i];
+        for (int i = k; i < m; ++i)
+            for (int j = k; j < m; ++j) A[(size_t)i * m + j] -= v[i] * q[j] + q[i] * v[j];
+        if (Z)
+            for (int r = 0; r < m; ++r) {
+                double t = 0.0;
+                for (int j = k + 1; j < m; ++j) t += Qm[(size_t)r * m + j] * v[j];
+                t *= bt;
+                for (int j = k + 1; j < m; ++j) Qm[(size_t)r * m + j] -= t * v[j];
+            }
+    }
+    std::vector<double> d(m), e(std::max(m - 1, 1), 0.0), z;
+    for (int i = 0; i < m; ++i) d[i] = A[(size_t)i * m + i];
+    for (int i = 0; i + 1 < m; ++i) e[i] = A[(size_t)(i + 1) * m + i];
+    if (Z) z.resize((size_t)m * m);
+    if (tridiag_ql(m, d.data(), e.data(), Z ? z.data() : nullptr)) return 1;
+    w = d;
+    if (Z) {
+        Z->assign((size_t)m * m, 0.0);
+        for (int jv = 0; jv < m; ++jv)
+            for (int i = 0; i < m; ++i) {
+                double t = 0.0;
+                for (int r = 0; r < m; ++r) t += Qm[(size_t)i * m + r] * z[(size_t)jv * m + r];
+                (*Z)[(size_t)jv * m + i] = t;
+            }
+    }
+    return 0;
+}
+
+// lowest eigenvalue of a symmetric band matrix (bandwidth bw, row-major m x m) by
+// bisection on the inertia of A - sigma I (number of negative pivots of its
+// LDL^T, Sylvester's law), O(m bw^2) per shift
+static double band_lowest(int m, int bw, const std::vector<double> &A) {
+    double lo = 1e300, hi = -1e300;
+    for (int i = 0; i < m; ++i) {
+        double r = 0.0;
+        for (int j = std::max(0, i - bw); j <= std::min(m - 1, i + bw); ++j)
+            if (j != i) r += std::fabs(A[(size_t)i * m + j]);
+        lo = std::min(lo, A[(size_t)i * m + i] - r);
+        hi = std::max(hi, A[(size_t)i * m + i] + r);
+    }
+    const int w = bw + 1;
+    std::vector<double> L((size_t)m * w, 0.0), D(m);
+    auto below = [&](double sg) {
+        int neg = 0;
+        for (int i = 0; i < m; ++i) {
+            const int j0 = std::max(0, i - bw);
+            for (int j = j0; j < i; ++j) {
+                double t = A[(size_t)i * m + j];
+                for (int k = std::max(j0, j - bw); k < j; ++k)
+                    t -= L[(size_t)i * w + (i - k)] * L[(size_t)j * w + (j - k)] * D[k];
+                L[(size_t)i * w + (i - j)] = t / D[j];
+            }
+            double d = A[(size_t)i * m + i] - sg;
+            for (int k = j0; k < i; ++k) d -= L[(size_t)i * w + (i - k)] * L[(size_t)i * w + (i - k)] * D[k];
+            if (d == 0.0) d = -1e-300;
+            D[i] = d;
+            if (d < 0.0) ++neg;
+        }
+        return neg;
+    };
+    for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(1.0, std::fabs(lo)); ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (below(mid) >= 1) hi = mid;
+        else lo = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+// band matrix of the recurrence with the deflated indices dropped
+// (spgf_bandlanczos.f90:396-431): ht[(m x m) row-major], returns m
+static int band_matrix(int nb, int p, const std::vector<double> &alpha,
+                       const std::vector<double> &beta2, const std::vector<int> &def,
+                       std::vector<double> &ht) {
+    std::vector<int> keep;
+    for (int i = 1; i <= nb; ++i) {     // 1-based index as DEF
+        bool drop = false;
+        for (int k = 0; k < p; ++k)
+            if (def[k] != 0 && i >= def[k] && (i - def[k]) % p == 0) drop = true;
+        if (!drop) keep.push_back(i - 1);
+    }
+    const int m = (int)keep.size();
+    std::vector<int> pos(nb, -1);
+    for (int a = 0; a < m; ++a) pos[keep[a]] = a;
+    ht.assign((size_t)m * m, 0.0);
+    for (int i = 0; i < nb; ++i) {
+        if (pos[i] >= 0) ht[(size_t)pos[i] * m + pos[i]] = alpha[i];
+        for (int j = 1; j <= p && i + j < nb; ++j) {
+            if (pos[i] < 0 || pos[i + j] < 0) continue;
+            const double b = beta2[(size_t)i * p + (j - 1)];
+            ht[(size_t)pos[i + j] * m + pos[i]] = b;
+            ht[(size_t)pos[i] * m + pos[i + j]] = b;
+        }
+    }
+    return m;
+}
+
+extern "C" int phub_band_lanczos(phub_ctx *c, const phub_op *op, const double *d_V, int p,
+                                 const phub_band_opts *o, double *positions, double *Q,
+                                 int *npoles) {
+    if (!c || !op || !d_V || !o || !positions || !Q || !npoles)
+        return phub_fail(c, PHUB_ERR_ARG, "null argument");
+    if (p < 1 || p > 256) return phub_fail(c, PHUB_ERR_ARG, "p must be in [1, 256]");
+    if (o->max_steps < p + 11) return phub_fail(c, PHUB_ERR_ARG, "max_steps too small for p");
+    PHUB_HIP(c, hipSetDevice(c->device));
+    *npoles = 0;
+    const int64_t n = op->dim, ld = basis_ld(n);
+    const double DF = 1e-10;   // deflation threshold (spgf_bandlanczos.f90:312)
+    // 2p + 1 vector slots: the p previous vectors, the current one and its p - 1
+    // successors, and the work vector; rotated by pointer, never copied
+    DevBuf<double> slots, coefd;
+    DevBuf<const double *> dptr;
+    PHUB_HIP(c, hipMalloc(&slots.p, (size_t)(2 * p + 1) * ld * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&coefd.p, (size_t)(2 * p + 8) * sizeof(double)));
+    PHUB_HIP(c, hipMalloc(&dptr.p, (size_t)(2 * p + 1) * sizeof(double *)));
+    PHUB_HIP(c, hipMemsetAsync(slots.p, 0, (size_t)(2 * p + 1) * ld * sizeof(double), c->stream));
+    std::vector<double *> ring(2 * p);       // [0,p): previous block V0, [p,2p): V
+    for (int i = 0; i < 2 * p; ++i) ring[i] = slots.p + (size_t)i * ld;
+    double *wv = slots.p + (size_t)(2 * p) * ld;
+    for (int i = 0; i < p; ++i)
+        PHUB_HIP(c, hipMemcpyAsync(ring[p + i], d_V + (size_t)i * n, n * sizeof(double),
+                                   hipMemcpyDeviceToDevice, c->stream));
+    // multidot / combine over a list of vectors (pointer table on the device)
+    std::vector<const double *> hp(2 * p + 1);
+    auto mdot = [&](const std::vector<double *> &vs, int first, int cnt, const double *w,
+                    double *out) -> int {
+        for (int j = 0; j < cnt; ++j) hp[j] = vs[first + j];
+        PHUB_HIP(c, hipMemcpyAsync(dptr.p, hp.data(), cnt * sizeof(double *), hipMemcpyHostToDevice, c->stream));
+        const int64_t slices = (n + 2048 - 1) / 2048;
+        const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(slices, 2048));
+        int rc = phub_ensure_red(c, (size_t)g * cnt + cnt + 64);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_multidot_v<double, double>), dim3(g), dim3(256), 4 * cnt * sizeof(double),
+                           c->stream, (const double *)nullptr, (int64_t)0, cnt, w,
+                           (const double *)nullptr, 0.0, n, c->d_red, (const double *const *)dptr.p);
+        PHUB_HIP(c, hipGetLastError());
+        return phub_reduce_partials(c, (int)g, cnt, out);
+    };
+    auto msub = [&](const std::vector<double *> &vs, int first, int cnt, const double *coef,
+                    double *w, double *norm2) -> int {
+        for (int j = 0; j < cnt; ++j) hp[j] = vs[first + j];
+        PHUB_HIP(c, hipMemcpyAsync(dptr.p, hp.data(), cnt * sizeof(double *), hipMemcpyHostToDevice, c->stream));
+        PHUB_HIP(c, hipMemcpyAsync(coefd.p, coef, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        const unsigned g = grid_for(n);
+        int rc = phub_ensure_red(c, g + 64);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_combine_v<double, double>), dim3(g), dim3(256), cnt * sizeof(double),
+                           c->stream, (const double *)nullptr, (int64_t)0, cnt, coefd.p, w, n, 0,
+                           norm2 ? c->d_red : nullptr, (const double *const *)dptr.p);
+        PHUB_HIP(c, hipGetLastError());
+        if (norm2) return phub_reduce_partials(c, (int)g, 1, norm2);
+        PHUB_HIP(c, hipStreamSynchronize(c->stream));   // coef / pointer uploads stay valid
+        return PHUB_OK;
+    };
+    // Gram-Schmidt of the start block (spgf_bandlanczos.f90:331-346): V_i = sum_j R(i,j) q_j
+    std::vector<double> R((size_t)p * p, 0.0), cf(2 * p + 8);
+    for (int i = 0; i < p; ++i) {
+        double *vi = ring[p + i];
+        double nn = 0.0;
+        if (i > 0) {
+            int rc = mdot(ring, p, i, vi, cf.data());
+            if (rc) return rc;
+            for (int j = 0; j < i; ++j) R[(size_t)i * p + j] = cf[j];
+            rc = msub(ring, p, i, cf.data(), vi, &nn);
+            if (rc) return rc;
+        } else {
+            int rc = dev_dot<double, double>(c, vi, vi, n, &nn);
+            if (rc) return rc;
+        }
+        const double nr = std::sqrt(std::max(nn, 0.0));
+        R[(size_t)i * p + i] = nr;
+        if (!(nr > 0.0)) return phub_fail(c, PHUB_ERR_ARG, "start vectors are linearly dependent");
+        int rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / nr, vi, n, nullptr);
+        if (rc) return rc;
+    }
+    // band recurrence (spgf_bandlanczos.f90:352-390)
+    const int cap = o->max_steps + p;
+    std::vector<double> alpha(cap, 0.0), beta2((size_t)cap * p, 0.0), ht, val, vec;
+    std::vector<int> def(p, 0);
+    int nb = 0, m = 0;
+    bool cv = false;
+    double ev = 0.0;
+    while (!cv && nb < o->max_steps - 10) {
+        for (int i = 0; i < p; ++i) {
+            double *cur = ring[p];
+            // w = H v, alpha = <v, w>
+            int nblk = 0;
+            int rc = phub_hv_launch<double>(c, op, cur, wv, 1.0, (const double *)nullptr, 0.0, true, &nblk);
+            if (rc) return rc;
+            double dots[2];
+            rc = phub_reduce_partials(c, nblk, 2, dots);
+            if (rc) return rc;
+            const double a = dots[0];
+            alpha[nb] = a;
+            // against the previous block (coefficients not kept, :56-59)
+            rc = mdot(ring, 0, p, wv, cf.data());
+            if (rc) return rc;
+            rc = msub(ring, 0, p, cf.data(), wv, nullptr);
+            if (rc) return rc;
+            // against the successors: band entries (:60-63)
+            if (p > 1) {
+                rc = mdot(ring, p + 1, p - 1, wv, cf.data());
+                if (rc) return rc;
+                for (int j = 0; j < p - 1; ++j) beta2[(size_t)nb * p + j] = cf[j];
+                rc = msub(ring, p + 1, p - 1, cf.data(), wv, nullptr);
+                if (rc) return rc;
+            }
+            double nn = 0.0;
+            rc = dev_axpby<double, double>(c, -a, cur, 1.0, wv, n, &nn);
+            if (rc) return rc;
+            const double b = std::sqrt(std::max(nn, 0.0));
+            beta2[(size_t)nb * p + (p - 1)] = b;
+            ++nb;
+            if (b <= DF && def[i] == 0) def[i] = nb + p;   // deflation (:66-72), 1-based
+            // shift: the oldest vector's slot receives the new last vector w / b
+            double *freed = ring[0];
+            for (int j = 0; j + 1 < 2 * p; ++j) ring[j] = ring[j + 1];
+            ring[2 * p - 1] = wv;
+            rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, b > 0.0 ? 1.0 / b : 0.0,
+                                           wv, n, nullptr);
+            if (rc) return rc;
+            wv = freed;
+        }
+        m = band_matrix(nb, p, alpha, beta2, def, ht);
+        const double e1 = band_lowest(m, p, ht);
+        if (std::fabs(e1 - ev) < o->tol) cv = true;
+        else ev = e1;
+    }
+    // final poles and weights
+    m = band_matrix(nb, p, alpha, beta2, def, ht);
+    if (sym_eig(m, ht, val, &vec)) return phub_fail(c, PHUB_ERR_HIP, "band eigensolver failed");
+    for (int k = 0; k < m; ++k) {
+        positions[k] = val[k];
+        for (int i = 0; i < p; ++i) {
+            double t = 0.0;
+            for (int j = 0; j <= i; ++j) t += R[(size_t)i * p + j] * vec[(size_t)k * m + j];
+            Q[(size_t)k * p + i] = t;
+        }
+    }
+    *npoles = m;
+    if (!cv) return phub_fail(c, PHUB_ERR_NOCONV, "FORTRAN BANDLANCZOS ERROR: MAX LCZ STEP REACH equivalent");
     return PHUB_OK;
 }
 
