@@ -53,6 +53,10 @@ class Cluster(C.Structure):
                 ("t", C.POINTER(C.c_double)), ("u", C.POINTER(C.c_double))]
 
 
+class BandOpts(C.Structure):
+    _fields_ = [("max_steps", C.c_int), ("tol", C.c_double)]
+
+
 class BatchOpts(C.Structure):
     _fields_ = [("gs_steps", C.c_int), ("gf_steps", C.c_int), ("seed", C.c_uint64),
                 ("gs_tol", C.c_double)]
@@ -84,6 +88,8 @@ SIGNATURES = {
     "phub_one_rdm_spin": (C.c_int, [_vp, _vp, _vp, _vp, _dp, _dp]),
     "phub_excite": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
     "phub_cf_chain": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _dp, _dp, _dp]),
+    "phub_band_lanczos": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(BandOpts), _dp, _dp,
+                                    C.POINTER(C.c_int)]),
     "phub_batch_hubbard": (C.c_int, [_vp, C.POINTER(Cluster), C.c_int, C.POINTER(BatchOpts),
                                      _dp, _dp, _dp, _dp]),
     "phub_cf_eval": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp,

@@ -213,14 +213,22 @@ class CoreSolver(Basis, GreenFunction, StaticQuantities):
         self.is_reduced_quantities = True
         self.printv(f"end calculation | elapsed time {np.around(pc() - t0, 4)} s")
 
-    def spgf(self, nb_sites_comp=None):
+    def spgf(self, nb_sites_comp=None, method=None):
+        """Green's functions of the first nb_sites_comp sites (core_solver.py:81-101).
+        Lanczos path: method "band" (default, SPGF_BANDLANCZOS: full G_ij) or "chains"
+        (one continued fraction per site: diagonal G_ii only, cheaper)."""
         self.nb_sites_comp = self.nb_sites if nb_sites_comp is None else nb_sites_comp
         t0 = pc()
         self.printv("start spgf calculation")
         if self.exec_name == "heisenberg.x":
             raise NotImplementedError("Green's functions are defined for the Hubbard model")
         if self._res["is_lanczos"]:
-            self._spgf_chains()
+            if (method or "band") == "band":
+                self._spgf_band()
+            elif method == "chains":
+                self._spgf_chains()
+            else:
+                raise ValueError("method must be 'band' or 'chains'")
         else:
             self._spgf_ed()
         self.is_spgf = True
@@ -260,6 +268,44 @@ class CoreSolver(Basis, GreenFunction, StaticQuantities):
                 out[f"positions_{kind}_{spin}"] = val.cpu().numpy()
                 out[f"Q_{kind}_{spin}"] = Q
                 nb_poles[ki, si] = tgt.dim
+        out["nb_poles"] = nb_poles
+        self._gfdata = out
+
+    def _spgf_band(self):
+        """SPGF_BANDLANCZOS (spgf_bandlanczos.f90:1-300): for each spin and kind the
+        nb_sites_comp excitations c_i psi0 (lesser) / c^dag_i psi0 (greater) start a band
+        Lanczos in the N-1 / N+1 sector; positions are the band matrix eigenvalues and
+        Q[i, k] the weights of pole k for site i.  The down spin reuses the up results
+        when N_up = N_dn, as the reference does."""
+        if np.count_nonzero(self.wgt) > 1:
+            raise NotImplementedError("band Lanczos Green's functions are ground-state only "
+                                      "(spgf_bandlanczos.f90:63)")
+        op = self._op
+        psi = self._state_vectors()[0]
+        ncomp = self.nb_sites_comp
+        out = {}
+        nb_poles = np.zeros((2, 2), dtype=int)
+        for si, spin in enumerate(("up", "down")):
+            if spin == "down" and self.n_up[0] == self.n_down[0]:
+                for kind in ("lesser", "greater"):
+                    out[f"positions_{kind}_down"] = out[f"positions_{kind}_up"]
+                    out[f"Q_{kind}_down"] = out[f"Q_{kind}_up"]
+                nb_poles[:, 1] = nb_poles[:, 0]
+                break
+            for ki, (kind, create) in enumerate((("lesser", False), ("greater", True))):
+                nu = self.n_up[0] + ((1 if create else -1) if si == 0 else 0)
+                nd = self.n_down[0] + ((1 if create else -1) if si == 1 else 0)
+                if not (0 <= nu <= self.nb_sites and 0 <= nd <= self.nb_sites):
+                    out[f"positions_{kind}_{spin}"] = np.zeros(0)
+                    out[f"Q_{kind}_{spin}"] = np.zeros((ncomp, 0, 1))
+                    continue
+                tgt = self._model_op(nu, nd)
+                V = torch.stack([op.excite(psi, i + 1, si, create, target=tgt)[1]
+                                 for i in range(ncomp)])
+                pos, Q = tgt.band_lanczos(V, max_steps=self.max_lcz, tol=self.acc_lcz)
+                out[f"positions_{kind}_{spin}"] = pos
+                out[f"Q_{kind}_{spin}"] = Q.T[:, :, None]
+                nb_poles[ki, si] = len(pos)
         out["nb_poles"] = nb_poles
         self._gfdata = out
 

@@ -123,7 +123,8 @@ class GreenFunction:
                       "weights": Qg[i, : nb[1, s], b] * Qg[j, : nb[1, s], b] * wb / np.sum(w)}).clean(wtol=1.0e-8)
                 + Pole({"positions": self.e[b] - pl,
                         "weights": Ql[i, : nb[0, s], b] * Ql[j, : nb[0, s], b] * wb / np.sum(w)}).clean(wtol=1.0e-8)
-                for b, wb in enumerate(w)]) for i in range(ncomp)] for j in range(ncomp)], dtype=Pole)
+                for b, wb in enumerate(w[: Qg.shape[2]])]) for i in range(ncomp)] for j in range(ncomp)],
+                dtype=Pole)
         return gf
 
     @property

@@ -172,6 +172,26 @@ class _Op:
                                                   nsteps, precision, _dptr(a), _dptr(b), _dptr(n2)))
         return a, b, float(n2[0])
 
+    def band_lanczos(self, V: torch.Tensor, max_steps: int = 600, tol: float = 1e-8,
+                     raise_noconv: bool = False):
+        """Band Lanczos from the p start vectors V (p, dim) (GF_BANDLANCZOS): returns the
+        poles (ascending eigenvalues of the band matrix) and Q (npoles, p) with
+        Q[k, i] = sum_{j<=i} R[i, j] VEC[j, k]."""
+        V = V.to(self.ctx.device, torch.float64).contiguous()
+        p = V.shape[0]
+        if V.shape[1] != self.dim:
+            raise ValueError("start vectors must have the operator dimension")
+        o = L.BandOpts(int(max_steps), float(tol))
+        pos = np.zeros(max_steps + p)
+        Q = np.zeros((max_steps + p) * p)
+        npl = C.c_int(0)
+        rc = self.ctx.lib.phub_band_lanczos(self.ctx.h, self.h, C.c_void_p(V.data_ptr()), p,
+                                            C.byref(o), _dptr(pos), _dptr(Q), C.byref(npl))
+        if not (rc == L.ERR_NOCONV and not raise_noconv):
+            self.ctx.check(rc)
+        m = npl.value
+        return pos[:m].copy(), Q[: m * p].reshape(m, p).copy()
+
     def minres(self, psi: torch.Tensor, e0: float, b: torch.Tensor, max_iter=500, tol=1e-10):
         x = self.vec(torch.float64)
         it = C.c_int(0)

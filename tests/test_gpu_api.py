@@ -66,7 +66,8 @@ def test_lanczos_path_chains_sum_rule(ctx):
     from pyhub_amd.solver.fermi_hubbard import FermiHubbard
     T = O.ring_T(8)
     fh = FermiHubbard(8, 4, 4, T, U=4.0)
-    fh.kernel(max_lcz=600, acc_lcz=1e-12, compute_spgf=True, nb_sites_comp=2)
+    fh.kernel(max_lcz=600, acc_lcz=1e-12)
+    fh.spgf(nb_sites_comp=2, method="chains")
     for spin in ("up", "down"):
         for i in range(2):
             w = fh.gf[spin][i, i].pole["weights"]
@@ -81,6 +82,29 @@ def test_lanczos_path_chains_sum_rule(ctx):
     Gp = (np.sum(qg ** 2 / (z[:, None] - (thg - e0)), axis=1)
           + np.sum(ql ** 2 / (z[:, None] - (e0 - thl)), axis=1))
     assert np.max(np.abs(G[0, 0] - Gp)) < 1e-9
+
+
+def test_lanczos_path_band_green_function_vs_dense_path(ctx):
+    """Lanczos path (max_lcz below the dimension) with the band Lanczos G_ij against
+    the dense path (SPGF_ED) of the same model: every G_ij(omega), both spins."""
+    from pyhub_amd.solver.fermi_hubbard import FermiHubbard
+    T = O.ring_T(6)
+    T[0, 3] = T[3, 0] = -0.3
+    T[1, 1] = 0.2
+    fl = FermiHubbard(6, 3, 2, T, U=3.0)
+    fl.kernel(max_lcz=250, acc_lcz=1e-13, compute_spgf=True)   # dim 300 > 250: Lanczos
+    assert fl.is_lanczos
+    fd = FermiHubbard(6, 3, 2, T, U=3.0)
+    fd.kernel(max_lcz=600, compute_spgf=True)
+    omega = np.linspace(-10, 10, 101)
+    for spin in ("up", "down"):
+        for i in range(6):
+            for j in range(6):
+                gl = fl.gf[spin][i, j].retarded(omega, 0.3)
+                gd = fd.gf[spin][i, j].retarded(omega, 0.3)
+                assert np.max(np.abs(gl - gd)) < 1e-7, (spin, i, j)
+    assert abs(fl.ae["up"] - fd.ae["up"]) < 1e-9
+    assert np.max(np.abs(fl.one_rdm_from_gf["down"] - fd.one_rdm["down"])) < 1e-8
 
 
 def test_heisenberg_ring(ctx):

@@ -242,6 +242,48 @@ def test_heisenberg_sisj_vs_oracle(ctx, n, seed):
     assert np.max(np.abs(op.sisj(dev(a), dev(b)) - m.sisj(a, b))) < 1e-10
 
 
+def test_band_lanczos_exact_and_moments(ctx):
+    """Band Lanczos (GF_BANDLANCZOS) from the p excitations c_i psi0:
+    - a 225-state sector is exhausted, so every G_ij(omega) equals dense ED;
+    - on a 52,920-state sector the zeroth and first moments are exact at any step
+      count: sum_k Q_ik Q_jk = <v_i|v_j>, sum_k Q_ik Q_jk E_k = <v_i|H|v_j>."""
+    from pyhub_amd.engine import HubbardOp
+    T = O.ring_T(6)
+    T[0, 3] = T[3, 0] = -0.3
+    T[2, 2] = 0.1
+    op = HubbardOp(ctx, 6, 3, 2, T, 3.0)
+    m = O.HubbardModel(6, 3, 2, T, 3.0)
+    ev, V = np.linalg.eigh(m.dense())
+    psi = dev(V[:, 0])
+    xs, tgt = [], None
+    for site in range(1, 7):
+        tgt, x = op.excite(psi, site, 0, False)
+        xs.append(x)
+    Vs = torch.stack(xs)
+    pos, Q = tgt.band_lanczos(Vs, max_steps=400, tol=1e-13)
+    mt = O.HubbardModel(6, 2, 2, T, 3.0)
+    et, Ut = np.linalg.eigh(mt.dense())
+    Qd = Ut.T @ Vs.cpu().numpy().T          # (poles, sites)
+    z = np.linspace(-8, 8, 81) + 0.3j
+    Gb = np.einsum("ki,kj,wk->wij", Q, Q, 1.0 / (z[:, None] - pos[None, :]))
+    Gd = np.einsum("ki,kj,wk->wij", Qd, Qd, 1.0 / (z[:, None] - et[None, :]))
+    assert np.max(np.abs(Gb - Gd)) < 1e-9
+    # moments on a larger sector
+    op = HubbardOp(ctx, 10, 5, 5, O.ring_T(10), 4.0)
+    r = op.lanczos(mode=1, max_steps=300, tol=1e-12)
+    xs = []
+    for site in range(1, 11):
+        tgt, x = op.excite(r.psi, site, 1, False)
+        xs.append(x)
+    Vs = torch.stack(xs)
+    pos, Q = tgt.band_lanczos(Vs, max_steps=120, tol=1e-12)
+    S0 = (Vs @ Vs.T).cpu().numpy()
+    HV = torch.stack([tgt.hv(v.contiguous()) for v in Vs])
+    S1 = (Vs @ HV.T).cpu().numpy()
+    assert np.max(np.abs(Q.T @ Q - S0)) < 1e-10
+    assert np.max(np.abs((Q.T * pos) @ Q - S1)) < 1e-9 * max(1.0, np.max(np.abs(S1)))
+
+
 @pytest.mark.parametrize("site,spin,create", [(1, 0, 1), (3, 0, 0), (2, 1, 1), (6, 1, 0)])
 def test_excitation_and_cf_chain_vs_oracle(ctx, site, spin, create):
     from pyhub_amd.engine import HubbardOp
