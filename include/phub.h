@@ -77,6 +77,12 @@ typedef struct {
      * the reference uses one T for both spins; a spin-dependent one-body
      * operator is what the adjoint gradient applies to psi. */
     const double *t_dn;
+    /* optional 4-index interaction (NULL = per-site u), host n^4 row-major
+     * u4[((J-1)*n+(K-1))*n*n+(M-1)*n+(N-1)] = UL(J,K,M,N):
+     * sum_{JKMN} UL(J,K,M,N) c^dag_{K up} c_{J up} c^dag_{N dn} c_{M dn}
+     * (is_Uloc = 3, hprod_hubbard.f90:62-89); u is then ignored.  UL must satisfy
+     * UL(J,K,M,N) = UL(K,J,N,M) (core_solver.py:143). */
+    const double *u4;
 } phub_hubbard_desc;
 
 /* Heisenberg S_z = 0 sector (hprod_heis.f90:1-61): H = sum over ORDERED pairs
@@ -160,6 +166,12 @@ int phub_lanczos(phub_ctx *ctx, const phub_op *op, const phub_lanczos_opts *opts
  * sisj n*n.  Any output may be NULL. */
 int phub_rdm_hubbard(phub_ctx *ctx, const phub_op *op, const double *d_bra, const double *d_ket,
                      double *rdm_up, double *rdm_dn, double *ni, double *sisj);
+/* DBL_OCC of TWO_RDM (static_rq.f90:247-306): out[((J-1)*n+(K-1))*n*n+(M-1)*n+(N-1)]
+ * = sum over basis states of bra(src) ket(c^dag_{K up} c_{J up} c^dag_{N dn} c_{M dn} src)
+ * times the fermion signs, computed for K <= J, N <= M and copied to (J,K,N,M),
+ * (K,J,M,N), (K,J,N,M) as the reference does.  out: host n^4. */
+int phub_two_rdm_hubbard(phub_ctx *ctx, const phub_op *op, const double *d_bra,
+                         const double *d_ket, double *out);
 /* Spin-resolved transition 1-RDMs g_s[(J-1)*n+(K-1)] = <bra| c^dag_Js c_Ks |ket>
  * for every ordered (J,K), without ONE_RDM's mirroring and up-copy rule (used by
  * the adjoint gradient; no reference counterpart).  Host outputs, n*n each. */

@@ -23,6 +23,11 @@
  *   orc_hprod_heis     bit-exact against the reference Heisenberg HPROD
  *                      (oracle/_ref); 2x standard spectrum on bipartite rings.
  *   orc_hprod_heis_fast bit-identical to orc_hprod_heis (tests/test_oracle.py).
+ *   orc_hprod_hubbard_ul bit-exact against the reference HPROD with is_Uloc = 3
+ *                      (oracle/_ref); equal to the per-site U form when UL holds
+ *                      only UL(J,J,J,J) = U_J.
+ *   orc_two_rdm        its (J,J,J,J) entries equal the double occupancy NI;
+ *                      E_int = sum UL * DBL_OCC equals <psi|H_U|psi>.
  *   orc_lanczos        converged E0 against dense FP64 eigh.
  *   orc_one_rdm/...    sum rules (tr rho = N), Hellmann-Feynman vs finite
  *                      differences of E0, brute-force operator matrices.
@@ -197,6 +202,116 @@ void orc_hprod_hubbard(int n, int nup, int ndn, long long nsup, long long nsdn,
             }
         }
     }
+}
+
+/* ---------------------------------------------------------------------------
+ * Hubbard H*v with a 4-index interaction (is_Uloc = 3): HPROD
+ * (hprod_hubbard.f90:17-118) with
+ *   sum_{JKMN} UL(J,K,M,N) c^dag_{K up} c_{J up} c^dag_{N dn} c_{M dn}   (:62-89)
+ * and no per-site U.  UL row-major UL[(((J-1)*n+(K-1))*n+(M-1))*n+(N-1)];
+ * UP[(J-1)*n+(K-1)] = UPARTIAL(J,K) = sum_{MN} UL(J,K,M,N)^2 (core_solver.py:146).
+ * Pairs (J,K) enter when |T(J,K)| >= 1e-14 or UPARTIAL(J,K) >= 1e-14 (:37).
+ * ------------------------------------------------------------------------- */
+void orc_hprod_hubbard_ul(int n, int nup, int ndn, long long nsup, long long nsdn,
+                          const u64 *bup, const u64 *bdn, const double *T,
+                          const double *UL, const double *UP, const double *v, double *w) {
+    long long nst = nsup * nsdn;
+    for (long long i = 0; i < nst; ++i) w[i] = 0.0;
+    for (int J = 1; J <= n; ++J) {
+        for (int K = 1; K <= n; ++K) {
+            double t = T[(J - 1) * n + (K - 1)];
+            double upj = UP[(J - 1) * n + (K - 1)];
+            if (!((fabs(t) >= 1.e-14) || (upj >= 1.e-14))) continue;
+            for (long long u = 0; u < nsup; ++u) {
+                long long nw = kinop(n, bup[u], J, K);
+                if (nw > 0) {
+                    long long pos = lsearch(bup, nsup, (u64)nw);
+                    if (pos >= 0) {
+                        double ac = anticom(n, J, bup[u]);
+                        ac = ac * anticom(n, K, (u64)nw);
+                        if (nup == ndn) {
+                            for (long long d = 0; d < nsdn; ++d) {
+                                w[d * nsup + pos] = w[d * nsup + pos] + ac * v[d * nsup + u] * t;
+                                w[pos * nsup + d] = w[pos * nsup + d] + ac * v[u * nsup + d] * t;
+                            }
+                        } else {
+                            for (long long d = 0; d < nsdn; ++d)
+                                w[d * nsup + pos] = w[d * nsup + pos] + ac * v[d * nsup + u] * t;
+                        }
+                        if (upj >= 1.e-14) {
+                            for (int M = 1; M <= n; ++M)
+                                for (int N = 1; N <= n; ++N) {
+                                    double ul = UL[(((size_t)(J - 1) * n + (K - 1)) * n + (M - 1)) * n + (N - 1)];
+                                    if (!(fabs(ul) >= 1.e-14)) continue;
+                                    for (long long d = 0; d < nsdn; ++d) {
+                                        long long nw2 = kinop(n, bdn[d], M, N);
+                                        if (nw2 > 0) {
+                                            long long pos2 = lsearch(bdn, nsdn, (u64)nw2);
+                                            if (pos2 >= 0) {
+                                                double ac2 = anticom(n, M, bdn[d]);
+                                                ac2 = ac2 * anticom(n, N, (u64)nw2);
+                                                w[pos2 * nsup + pos] = w[pos2 * nsup + pos] +
+                                                                       ac * ac2 * v[d * nsup + u] * ul;
+                                            }
+                                        }
+                                    }
+                                }
+                        }
+                    }
+                }
+            }
+            if (nup != ndn) {
+                for (long long d = 0; d < nsdn; ++d) {
+                    long long nw = kinop(n, bdn[d], J, K);
+                    if (nw > 0) {
+                        long long pos = lsearch(bdn, nsdn, (u64)nw);
+                        if (pos >= 0) {
+                            double ac = anticom(n, J, bdn[d]);
+                            ac = ac * anticom(n, K, (u64)nw);
+                            for (long long u = 0; u < nsup; ++u)
+                                w[pos * nsup + u] = w[pos * nsup + u] + ac * v[d * nsup + u] * t;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* DBL_OCC of TWO_RDM (static_rq.f90:247-306) for <bra| . |ket>: computed for
+ * K <= J, N <= M and mirrored to (J,K,N,M), (K,J,M,N), (K,J,N,M); out row-major
+ * as UL above. */
+void orc_two_rdm(int n, long long nsup, long long nsdn, const u64 *bup, const u64 *bdn,
+                 const double *bra, const double *ket, double *out) {
+    for (int J = 1; J <= n; ++J)
+        for (int K = 1; K <= J; ++K)
+            for (int M = 1; M <= n; ++M)
+                for (int N = 1; N <= M; ++N) {
+                    double acc = 0.0;
+                    for (long long i = 0; i < nsup; ++i) {
+                        long long nw = kinop(n, bup[i], J, K);
+                        if (nw <= 0) continue;
+                        long long pos = lsearch(bup, nsup, (u64)nw);
+                        if (pos < 0) continue;
+                        double ac = anticom(n, J, bup[i]);
+                        ac = ac * anticom(n, K, (u64)nw);
+                        for (long long l1 = 0; l1 < nsdn; ++l1) {
+                            long long nw2 = kinop(n, bdn[l1], M, N);
+                            if (nw2 <= 0) continue;
+                            long long pos2 = lsearch(bdn, nsdn, (u64)nw2);
+                            if (pos2 < 0) continue;
+                            double ac2 = anticom(n, M, bdn[l1]);
+                            ac2 = ac2 * anticom(n, N, (u64)nw2);
+                            acc = acc + ac * ac2 * bra[l1 * nsup + i] * ket[pos2 * nsup + pos];
+                        }
+                    }
+#define IX(a, b, c, d) ((((size_t)(a - 1) * n + (b - 1)) * n + (c - 1)) * n + (d - 1))
+                    out[IX(J, K, M, N)] = acc;
+                    out[IX(J, K, N, M)] = acc;
+                    out[IX(K, J, M, N)] = acc;
+                    out[IX(K, J, N, M)] = acc;
+#undef IX
+                }
 }
 
 /* ---------------------------------------------------------------------------

@@ -49,6 +49,9 @@ def lib():
                     f64p, f64p, f64p, f64p]
         L.orc_hprod_hubbard.argtypes = hub_args
         L.orc_hprod_hubbard_fast.argtypes = hub_args + [C.c_int]
+        L.orc_hprod_hubbard_ul.argtypes = [C.c_int, C.c_int, C.c_int, C.c_longlong, C.c_longlong,
+                                           u64p, u64p, f64p, f64p, f64p, f64p, f64p]
+        L.orc_two_rdm.argtypes = [C.c_int, C.c_longlong, C.c_longlong, u64p, u64p, f64p, f64p, f64p]
         L.orc_hprod_heis.argtypes = [C.c_int, C.c_longlong, u64p, f64p, f64p, f64p]
         L.orc_hprod_heis_fast.argtypes = [C.c_int, C.c_longlong, u64p, f64p, f64p, f64p]
         L.orc_tridiag_eig.restype = C.c_int
@@ -84,6 +87,7 @@ def _ref_libs():
         _ref_hub = C.CDLL(os.path.join(_HERE, "_ref", "libref_hub.so"))
         _ref_hub.phref_hub_setup.argtypes = [C.c_int] * 5 + [i32p, i32p, f64p, f64p]
         _ref_hub.phref_hub_hprod.argtypes = [f64p, f64p]
+        _ref_hub.phref_hub_setup_ul.argtypes = [C.c_int] * 5 + [i32p, i32p, f64p, f64p, f64p]
         _ref_heis = C.CDLL(os.path.join(_HERE, "_ref", "libref_heis.so"))
         _ref_heis.phref_heis_setup.argtypes = [C.c_int, C.c_int, i32p, f64p]
         _ref_heis.phref_heis_hprod.argtypes = [f64p, f64p]
@@ -129,6 +133,12 @@ class HubbardModel:
         self.n, self.nup, self.ndn = int(n), int(nup), int(ndn)
         self.T = np.ascontiguousarray(T, dtype=np.float64)
         U = np.asarray(U, dtype=np.float64)
+        # a 4-index U (n, n, n, n) is the is_Uloc = 3 interaction (core_solver.py:142-148)
+        self.UL = None
+        if U.ndim == 4:
+            self.UL = np.ascontiguousarray(U)
+            self.UP = np.ascontiguousarray(np.einsum("ijkl,ijkl->ij", U, U))
+            U = np.zeros(n)
         self.U = np.ascontiguousarray(np.full(n, float(U)) if U.ndim == 0 else U)
         self.bup = hilbert(n, nup)
         self.bdn = hilbert(n, ndn)
@@ -138,7 +148,10 @@ class HubbardModel:
     def hv(self, v, fast=False, nthreads=1):
         v = np.ascontiguousarray(v, dtype=np.float64)
         w = np.empty_like(v)
-        if fast:
+        if self.UL is not None:
+            lib().orc_hprod_hubbard_ul(self.n, self.nup, self.ndn, self.nsup, self.nsdn,
+                                       self.bup, self.bdn, self.T, self.UL, self.UP, v, w)
+        elif fast:
             lib().orc_hprod_hubbard_fast(self.n, self.nup, self.ndn, self.nsup, self.nsdn,
                                          self.bup, self.bdn, self.T, self.U, v, w, nthreads)
         else:
@@ -149,9 +162,14 @@ class HubbardModel:
     def hv_ref(self, v):
         """The reference's own HPROD (Fortran, compiled from its sources)."""
         hub, _ = _ref_libs()
-        hub.phref_hub_setup(self.n, self.nup, self.ndn, self.nsup, self.nsdn,
-                            self.bup.astype(np.int32), self.bdn.astype(np.int32),
-                            self.T, self.U)
+        if self.UL is not None:
+            hub.phref_hub_setup_ul(self.n, self.nup, self.ndn, self.nsup, self.nsdn,
+                                   self.bup.astype(np.int32), self.bdn.astype(np.int32),
+                                   self.T, self.UL, self.UP)
+        else:
+            hub.phref_hub_setup(self.n, self.nup, self.ndn, self.nsup, self.nsdn,
+                                self.bup.astype(np.int32), self.bdn.astype(np.int32),
+                                self.T, self.U)
         v = np.ascontiguousarray(v, dtype=np.float64)
         w = np.empty_like(v)
         hub.phref_hub_hprod(v, w)
@@ -171,7 +189,18 @@ class HubbardModel:
                 self.T, self.U, np.zeros(1), int(fast), int(nthreads))
 
     def lanczos(self, v0, maxlcz=600, acc=1e-8, fast=True, nthreads=1):
+        if self.UL is not None:
+            raise NotImplementedError("the oracle Lanczos covers the per-site U only")
         return _lanczos(self._args(fast, nthreads), v0, maxlcz, acc, self.dim)
+
+    def two_rdm(self, bra, ket=None):
+        """DBL_OCC(J,K,M,N) of TWO_RDM (static_rq.f90:247-306), shape (n, n, n, n)."""
+        ket = bra if ket is None else ket
+        out = np.zeros(self.n ** 4)
+        lib().orc_two_rdm(self.n, self.nsup, self.nsdn, self.bup, self.bdn,
+                          np.ascontiguousarray(bra, dtype=np.float64),
+                          np.ascontiguousarray(ket, dtype=np.float64), out)
+        return out.reshape(self.n, self.n, self.n, self.n)
 
     def one_rdm(self, bra, ket=None):
         ket = bra if ket is None else ket

@@ -21,7 +21,7 @@ LCZ_PLAIN64, LCZ_FULL64, LCZ_MIXED, LCZ_PLAIN32 = 0, 1, 2, 3
 class HubbardDesc(C.Structure):
     _fields_ = [("n_sites", C.c_int), ("n_up", C.c_int), ("n_dn", C.c_int),
                 ("t", C.POINTER(C.c_double)), ("u", C.POINTER(C.c_double)),
-                ("t_dn", C.POINTER(C.c_double))]
+                ("t_dn", C.POINTER(C.c_double)), ("u4", C.POINTER(C.c_double))]
 
 
 class HeisDesc(C.Structure):
@@ -88,6 +88,7 @@ SIGNATURES = {
     "phub_one_rdm_spin": (C.c_int, [_vp, _vp, _vp, _vp, _dp, _dp]),
     "phub_excite": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
     "phub_cf_chain": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _dp, _dp, _dp]),
+    "phub_two_rdm_hubbard": (C.c_int, [_vp, _vp, _vp, _vp, _dp]),
     "phub_band_lanczos": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(BandOpts), _dp, _dp,
                                     C.POINTER(C.c_int)]),
     "phub_batch_hubbard": (C.c_int, [_vp, C.POINTER(Cluster), C.c_int, C.POINTER(BatchOpts),

@@ -73,10 +73,7 @@ class CoreSolver(Basis, GreenFunction, StaticQuantities):
         if self.exec_name == "heisenberg.x":
             return HeisenbergOp(ctx, self.nb_sites, self.J_matrix,
                                 sign_convention=getattr(self, "sign_convention", 0))
-        U = self.U_matrix
-        if U.ndim == 4:
-            raise NotImplementedError("4-index U (is_Uloc = 3) is not built in this engine yet")
-        return HubbardOp(ctx, self.nb_sites, nu, nd, self.t_matrix, U)
+        return HubbardOp(ctx, self.nb_sites, nu, nd, self.t_matrix, self.U_matrix)
 
     def _validate(self):
         if isinstance(self.U, (float, int, np.floating, np.integer)):
@@ -193,9 +190,14 @@ class CoreSolver(Basis, GreenFunction, StaticQuantities):
         vecs = self._state_vectors()
         nw = len(vecs)
         rq = {"sisj": np.zeros((nw, n, n))}
+        # the 4-index DBL_OCC is computed when U is a 4-index tensor (do_rq_two_body,
+        # core_solver.py:147)
+        two_body = self.exec_name != "heisenberg.x" and self.U_matrix.ndim == 4
         if self.exec_name != "heisenberg.x":
             rq.update(density_matrix_up=np.zeros((nw, n, n)), density_matrix_down=np.zeros((nw, n, n)),
                       ni=np.zeros((nw, n)))
+        if two_body:
+            rq["dbl_occ"] = np.zeros((nw, n, n, n, n))
         for b, v in enumerate(vecs):
             if v is None:
                 continue
@@ -209,6 +211,8 @@ class CoreSolver(Basis, GreenFunction, StaticQuantities):
                 rq["density_matrix_down"][b] = dn
                 rq["ni"][b] = ni
                 rq["sisj"][b] = ss
+                if two_body:
+                    rq["dbl_occ"][b] = op.two_rdm(v)
         self._rq = rq
         self.is_reduced_quantities = True
         self.printv(f"end calculation | elapsed time {np.around(pc() - t0, 4)} s")

@@ -32,7 +32,12 @@ class StaticQuantities:
 
     @property
     def two_rdm(self):
-        raise NotImplementedError("the full 4-index DBL_OCC tensor (is_Uloc = 3) is not built yet")
+        """two_rdm[J,K,M,N] = DBL_OCC(J,K,M,N) (reduced_quantities.py:25-27), only when U
+        is a 4-index tensor (static_rq.f90:256-306)."""
+        if "dbl_occ" not in self._rq:
+            raise KeyError("dbl_occ is computed only for a 4-index U (is_Uloc = 3)")
+        w = self._w()
+        return np.einsum("bjkmn,b->jkmn", self._rq["dbl_occ"][: len(w)], w)
 
     @property
     def density(self):

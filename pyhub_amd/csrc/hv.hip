@@ -545,6 +545,134 @@ __global__ __launch_bounds__(256) void k_hv_heis_blk(HeisBlkArgs a, const T *__r
     if (EPI) block_sum2<T>(s_xo, s_oo, part);
 }
 
+// 4-index interaction (is_Uloc = 3, hprod_hubbard.f90:62-89) in pull form: the
+// element (d', u') collects UL(J,K,M,N) s_up s_dn x[d][u] over the sources with
+// u' = c^dag_{K up} c_{J up} u and d' = c^dag_{N dn} c_{M dn} d.  The sources are
+// the all-pairs table entries of u' and d' read backwards (u = c^dag_J c_K u', same
+// sign) plus the occupied sites (J = K, the number operator).  Added to out;
+// with EPI the Lanczos epilogue is applied to the sum.
+struct UlArgs {
+    int n;
+    int64_t nu, nd;
+    const uint32_t *mu, *md;
+    const int32_t *cu, *iu;   // up all-pairs table, k-major
+    const double *su;
+    const int16_t *pu;
+    const int32_t *cd, *id;   // down all-pairs table, row-major
+    const double *sd;
+    const int16_t *pd;
+    int maxh_d;
+    const double *ul;         // n^4, thresholds applied
+};
+
+template <typename T, bool EPI>
+__global__ __launch_bounds__(256) void k_hv_ul(UlArgs a, const T *__restrict__ x,
+                                               T *__restrict__ out, double scale,
+                                               const T *__restrict__ prev, double beta,
+                                               double *__restrict__ part) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s_xo = 0.0, s_oo = 0.0;
+    if (e < a.nu * a.nd) {
+        const int n = a.n, n2 = n * n;
+        const int64_t dp = e / a.nu, upr = e - dp * a.nu;
+        const uint32_t xu = a.mu[upr], xd = a.md[dp];
+        const int cntu = a.cu[upr], cntd = a.cd[dp];
+        double acc = 0.0;
+        for (int ia = 0; ia < cntu + n; ++ia) {
+            int J, K;
+            double sg;
+            int64_t us;
+            if (ia < cntu) {
+                const size_t o = (size_t)ia * a.nu + upr;
+                const int pr = a.pu[o];
+                J = pr % n;
+                K = pr / n;
+                sg = a.su[o];
+                us = a.iu[o];
+            } else {
+                const int site = ia - cntu;
+                if (!((xu >> (n - 1 - site)) & 1u)) continue;
+                J = K = site;
+                sg = 1.0;
+                us = upr;
+            }
+            const double *ulJK = a.ul + ((size_t)J * n + K) * n2;
+            for (int ib = 0; ib < cntd + n; ++ib) {
+                int M, N;
+                double sg2;
+                int64_t ds;
+                if (ib < cntd) {
+                    const size_t o = (size_t)dp * a.maxh_d + ib;
+                    const int q = a.pd[o];
+                    M = q % n;
+                    N = q / n;
+                    sg2 = a.sd[o];
+                    ds = a.id[o];
+                } else {
+                    const int site = ib - cntd;
+                    if (!((xd >> (n - 1 - site)) & 1u)) continue;
+                    M = N = site;
+                    sg2 = 1.0;
+                    ds = dp;
+                }
+                const double w = ulJK[M * n + N];
+                if (w != 0.0) acc += w * sg * sg2 * (double)x[ds * a.nu + us];
+            }
+        }
+        if (EPI) {
+            double val = scale * ((double)out[e] + acc);
+            if (prev) val -= beta * (double)prev[e];
+            const T vt = (T)val;
+            out[e] = vt;
+            s_xo += scale * (double)x[e] * (double)vt;
+            s_oo += (double)vt * (double)vt;
+        } else {
+            out[e] = (T)((double)out[e] + acc);
+        }
+    }
+    if (EPI) block_sum2<T>(s_xo, s_oo, part);
+}
+
+template <typename T>
+int phub_hv_ul(phub_ctx *c, const phub_op *op, const T *x, T *out, double scale, const T *prev,
+               double beta, bool epi, int *nblocks_out) {
+    UlArgs a{};
+    a.n = op->n;
+    a.nu = op->dim_up;
+    a.nd = op->dim_dn;
+    a.mu = op->up.masks;
+    a.md = op->dn.masks;
+    a.cu = op->rup.cnt;
+    a.iu = op->rup.idx;
+    a.su = op->rup.c64;
+    a.pu = op->rup.pair;
+    a.cd = op->rdn.cnt;
+    a.id = op->rdn.idx;
+    a.sd = op->rdn.c64;
+    a.pd = op->rdn.pair;
+    a.maxh_d = std::max(op->rdn.maxh, 1);
+    a.ul = op->d_ul;
+    const int64_t nb = (op->dim + 255) / 256;
+    if (nb > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
+    if (epi) {
+        int rc = phub_ensure_red(c, (size_t)nb * 2 + 64);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_hv_ul<T, true>), dim3((unsigned)nb), dim3(256), 0, c->stream, a, x,
+                           out, scale, prev, beta, c->d_red);
+    } else {
+        hipLaunchKernelGGL((k_hv_ul<T, false>), dim3((unsigned)nb), dim3(256), 0, c->stream, a, x,
+                           out, scale, prev, beta, c->d_red);
+    }
+    PHUB_HIP(c, hipGetLastError());
+    if (nblocks_out) *nblocks_out = (int)nb;
+    return PHUB_OK;
+}
+
+template int phub_hv_ul<double>(phub_ctx *, const phub_op *, const double *, double *, double,
+                                const double *, double, bool, int *);
+template int phub_hv_ul<float>(phub_ctx *, const phub_op *, const float *, float *, double,
+                               const float *, double, bool, int *);
+
 template <typename T>
 static HubArgs<T> hub_args(const phub_op *op) {
     HubArgs<T> a{};
@@ -730,10 +858,28 @@ static int tune_dn(phub_ctx *c, phub_op *op, const HubArgs<T> &a, const T *x, T 
 }
 
 template <typename T>
+static int hv_launch_main(phub_ctx *c, const phub_op *op, const T *x, T *out, double scale,
+                          const T *prev, double beta, bool want_dots, int *nblocks_out);
+
+template <typename T>
 int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double scale,
                    const T *prev, double beta, bool want_dots, int *nblocks_out) {
-    int nblocks = 0;
     // the epilogue kernel applies scale / prev and writes dot partials
+    const bool epi = want_dots || prev != nullptr || scale != 1.0;
+    if (op->kind == 0 && op->d_ul) {
+        // one-body part, then the 4-index term with the epilogue on the sum
+        int rc = hv_launch_main<T>(c, op, x, out, 1.0, (const T *)nullptr, 0.0, false, nullptr);
+        if (rc) return rc;
+        return phub_hv_ul<T>(c, op, x, out, scale, prev, beta, epi, nblocks_out);
+    }
+    return hv_launch_main<T>(c, op, x, out, scale, prev, beta, want_dots, nblocks_out);
+}
+
+// the one-body (T, per-site U) Hubbard operator and the Heisenberg operator
+template <typename T>
+static int hv_launch_main(phub_ctx *c, const phub_op *op, const T *x, T *out, double scale,
+                          const T *prev, double beta, bool want_dots, int *nblocks_out) {
+    int nblocks = 0;
     const bool epi = want_dots || prev != nullptr || scale != 1.0;
     if (op->kind == 0 && op->up.packed && op->dn.csr) {
         // two-kernel path: R rows of the state in LDS for the up hops

@@ -15,15 +15,31 @@ static bool symmetric(const double *a, int n) {
 }
 
 extern "C" int phub_op_hubbard(phub_ctx *c, const phub_hubbard_desc *d, phub_op **out) {
-    if (!c || !d || !out || !d->t || !d->u) return phub_fail(c, PHUB_ERR_ARG, "null argument");
+    if (!c || !d || !out || !d->t || (!d->u && !d->u4)) return phub_fail(c, PHUB_ERR_ARG, "null argument");
     const int n = d->n_sites;
     if (n < 1 || n > 32) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "Hubbard operator needs 1 <= n_sites <= 32");
     if (d->n_up < 0 || d->n_up > n || d->n_dn < 0 || d->n_dn > n)
         return phub_fail(c, PHUB_ERR_ARG, "particle numbers out of range");
     for (int i = 0; i < n * n; ++i)
         if (!std::isfinite(d->t[i])) return phub_fail(c, PHUB_ERR_ARG, "t_matrix has non-finite entries");
-    for (int i = 0; i < n; ++i)
-        if (!std::isfinite(d->u[i])) return phub_fail(c, PHUB_ERR_ARG, "U has non-finite entries");
+    const int n4 = n * n * n * n;
+    if (d->u4) {
+        // core_solver.py:142-145: np.isclose(U, einsum('ijkl->jilk', U)) everywhere
+        for (int i = 0; i < n4; ++i)
+            if (!std::isfinite(d->u4[i])) return phub_fail(c, PHUB_ERR_ARG, "U has non-finite entries");
+        for (int J = 0; J < n; ++J)
+            for (int K = 0; K < n; ++K)
+                for (int M = 0; M < n; ++M)
+                    for (int N = 0; N < n; ++N) {
+                        const double a = d->u4[((J * n + K) * n + M) * n + N];
+                        const double b = d->u4[((K * n + J) * n + N) * n + M];
+                        if (!(std::fabs(a - b) <= 1e-8 + 1e-5 * std::fabs(b)))
+                            return phub_fail(c, PHUB_ERR_ARG, "U tensor must be symmetric");
+                    }
+    } else {
+        for (int i = 0; i < n; ++i)
+            if (!std::isfinite(d->u[i])) return phub_fail(c, PHUB_ERR_ARG, "U has non-finite entries");
+    }
     // fermi_hubbard.py:26-27
     if (!symmetric(d->t, n)) return phub_fail(c, PHUB_ERR_ARG, "t_matrix must be symmetric");
     const double *tdn = d->t_dn ? d->t_dn : d->t;
@@ -39,9 +55,9 @@ extern "C" int phub_op_hubbard(phub_ctx *c, const phub_hubbard_desc *d, phub_op 
     op->ndn = d->n_dn;
     op->T.assign(d->t, d->t + n * n);
     op->u_uniform = 1;
-    op->u0 = d->u[0];
-    for (int i = 0; i < PHUB_MAXN; ++i) op->U[i] = (i < n) ? d->u[i] : 0.0;
-    for (int i = 1; i < n; ++i)
+    op->u0 = d->u4 ? 0.0 : d->u[0];
+    for (int i = 0; i < PHUB_MAXN; ++i) op->U[i] = (i < n && !d->u4) ? d->u[i] : 0.0;
+    for (int i = 1; i < n && !d->u4; ++i)
         if (d->u[i] != d->u[0]) op->u_uniform = 0;
     int rc = phub_build_hop_table(c, n, d->n_up, d->t, true, false, &op->up);
     if (!rc) rc = phub_build_hop_table(c, n, d->n_dn, tdn, false, false, &op->dn);
@@ -53,6 +69,25 @@ extern "C" int phub_op_hubbard(phub_ctx *c, const phub_hubbard_desc *d, phub_op 
         return (double)t.dim * (4.0 + 8.0 + 4.0) + (double)t.dim * t.maxh * (4.0 + 8.0 + 4.0);
     };
     op->table_bytes = tb(op->up) + tb(op->dn);
+    if (d->u4) {
+        // UPARTIAL(J,K) = sum_{MN} UL(J,K,M,N)^2; a term enters when UPARTIAL(J,K) >= 1e-14
+        // and |UL(J,K,M,N)| >= 1e-14 (hprod_hubbard.f90:62-65): keep exactly those
+        std::vector<double> ul(d->u4, d->u4 + n4);
+        for (int jk = 0; jk < n * n; ++jk) {
+            double up = 0.0;
+            for (int mn = 0; mn < n * n; ++mn) up += ul[(size_t)jk * n * n + mn] * ul[(size_t)jk * n * n + mn];
+            for (int mn = 0; mn < n * n; ++mn) {
+                double &x = ul[(size_t)jk * n * n + mn];
+                if (!(up >= 1e-14) || !(std::fabs(x) >= 1e-14)) x = 0.0;
+            }
+        }
+        hipError_t e = hipMalloc(&op->d_ul, n4 * sizeof(double));
+        if (e == hipSuccess) e = hipMemcpy(op->d_ul, ul.data(), n4 * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { phub_op_destroy(op); return phub_hip_fail(c, e, "4-index U upload"); }
+        rc = phub_ensure_pair_tables(c, op);
+        if (rc) { phub_op_destroy(op); return rc; }
+        op->table_bytes += n4 * 8.0;
+    }
     *out = op;
     return PHUB_OK;
 }
@@ -96,6 +131,7 @@ extern "C" void phub_op_destroy(phub_op *op) {
     phub_free_hop_table(&op->rup);
     phub_free_hop_table(&op->rdn);
     phub_free_heis(&op->h);
+    if (op->d_ul) hipFree(op->d_ul);
     delete op;
 }
 

@@ -112,6 +112,9 @@ struct phub_op {
     double U[PHUB_MAXN];
     int u_uniform = 1;
     double u0 = 0.0;
+    // 4-index interaction (is_Uloc = 3): device n^4, entries below the reference's
+    // thresholds zeroed; the all-pairs tables rup / rdn are built with it
+    double *d_ul = nullptr;
     std::vector<double> T;         // host copy, row-major
     // Heisenberg
     HeisTables h;
@@ -168,6 +171,13 @@ int phub_ensure_red(phub_ctx *ctx, size_t ndoubles);
 // H*v launchers with a fused Lanczos epilogue:
 //   out = scale * (H x) - beta * prev   (prev may be null)
 //   partials: <x_scaled, out>, <out, out> per block (when want_dots)
+// all-pairs hop tables of an operator (reduced quantities, 4-index interaction)
+int phub_ensure_pair_tables(phub_ctx *ctx, phub_op *op);
+// the 4-index interaction term, added to out (hv.hip)
+template <typename T>
+int phub_hv_ul(phub_ctx *ctx, const phub_op *op, const T *x, T *out, double scale,
+               const T *prev, double beta, bool epi, int *nblocks_out);
+
 // tuner tile index -> (segment bytes * 1000 + rows), -1 if untuned (hv.hip)
 int phub_dn_tile_code(int cfg, bool fp64);
 

@@ -191,7 +191,7 @@ struct DBuf {
     }
 };
 
-static int ensure_rdm_tables(phub_ctx *c, phub_op *op) {
+int phub_ensure_pair_tables(phub_ctx *c, phub_op *op) {
     if (!op->rup.idx) {
         std::vector<double> ones(op->n * op->n, 1.0);
         int rc = phub_build_hop_table(c, op->n, op->nup, ones.data(), true, true, &op->rup);
@@ -217,7 +217,7 @@ static int hubbard_raw(phub_ctx *c, phub_op *op, const double *bra, const double
                        HubRaw &r, bool want_flip) {
     const int n = op->n;
     const int64_t nu = op->dim_up, nd = op->dim_dn;
-    int rc = ensure_rdm_tables(c, op);
+    int rc = phub_ensure_pair_tables(c, op);
     if (rc) return rc;
     DBuf<double> dout, dR, dp, dq;
     const int nn = n * n;
@@ -382,6 +382,110 @@ extern "C" int phub_hf_grad_hubbard(phub_ctx *c, const phub_op *op, const double
 }
 
 // ---------------------------------------------------------------------------
+// DBL_OCC of TWO_RDM (static_rq.f90:247-306), push form: the source (l1, i) with
+// amplitude bra, every up move c^dag_K c_J i (K <= J, J = K the occupation) and down
+// move c^dag_N c_M l1 (N <= M) adds s_up s_dn bra ket[target] to (J,K,M,N).
+__global__ __launch_bounds__(256) void k_two_rdm(int n, int64_t nu, int64_t nd,
+                                                 const uint32_t *__restrict__ mu,
+                                                 const uint32_t *__restrict__ md,
+                                                 const int32_t *__restrict__ cu,
+                                                 const int32_t *__restrict__ iu,
+                                                 const double *__restrict__ su,
+                                                 const int16_t *__restrict__ pu,
+                                                 const int32_t *__restrict__ cd,
+                                                 const int32_t *__restrict__ id,
+                                                 const double *__restrict__ sd,
+                                                 const int16_t *__restrict__ pd, int maxh_d,
+                                                 const double *__restrict__ bra,
+                                                 const double *__restrict__ ket,
+                                                 double *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nu * nd) return;
+    const double b = bra[e];
+    if (b == 0.0) return;
+    const int64_t l1 = e / nu, i = e - l1 * nu;
+    const uint32_t xu = mu[i], xd = md[l1];
+    const int cntu = cu[i], cntd = cd[l1];
+    for (int ia = 0; ia < cntu + n; ++ia) {
+        int J, K;
+        double s1;
+        int64_t pos;
+        if (ia < cntu) {
+            const size_t o = (size_t)ia * nu + i;
+            const int pr = pu[o];     // (J-1)*n + (K-1) of c^dag_K c_J
+            J = pr / n;
+            K = pr % n;
+            if (K > J) continue;
+            s1 = su[o];
+            pos = iu[o];
+        } else {
+            const int site = ia - cntu;
+            if (!((xu >> (n - 1 - site)) & 1u)) continue;
+            J = K = site;
+            s1 = 1.0;
+            pos = i;
+        }
+        for (int ib = 0; ib < cntd + n; ++ib) {
+            int M, N;
+            double s2;
+            int64_t pos2;
+            if (ib < cntd) {
+                const size_t o = (size_t)l1 * maxh_d + ib;
+                const int q = pd[o];
+                M = q / n;
+                N = q % n;
+                if (N > M) continue;
+                s2 = sd[o];
+                pos2 = id[o];
+            } else {
+                const int site = ib - cntd;
+                if (!((xd >> (n - 1 - site)) & 1u)) continue;
+                M = N = site;
+                s2 = 1.0;
+                pos2 = l1;
+            }
+            atomicAdd(&out[(((size_t)J * n + K) * n + M) * n + N], s1 * s2 * b * ket[pos2 * nu + pos]);
+        }
+    }
+}
+
+extern "C" int phub_two_rdm_hubbard(phub_ctx *c, const phub_op *op, const double *bra,
+                                    const double *ket, double *out) {
+    if (!c || !op || !bra || !ket || !out) return phub_fail(c, PHUB_ERR_ARG, "null argument");
+    if (op->kind != 0) return phub_fail(c, PHUB_ERR_ARG, "not a Hubbard operator");
+    phub_op *mop = const_cast<phub_op *>(op);
+    int rc = phub_ensure_pair_tables(c, mop);
+    if (rc) return rc;
+    const int n = op->n;
+    const size_t n4 = (size_t)n * n * n * n;
+    DBuf<double> d;
+    PHUB_HIP(c, hipMalloc(&d.p, n4 * sizeof(double)));
+    PHUB_HIP(c, hipMemsetAsync(d.p, 0, n4 * sizeof(double), c->stream));
+    const int64_t nb = (op->dim + 255) / 256;
+    if (nb > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
+    hipLaunchKernelGGL(k_two_rdm, dim3((unsigned)nb), dim3(256), 0, c->stream, n, op->dim_up,
+                       op->dim_dn, op->up.masks, op->dn.masks, op->rup.cnt, op->rup.idx,
+                       op->rup.c64, op->rup.pair, op->rdn.cnt, op->rdn.idx, op->rdn.c64,
+                       op->rdn.pair, std::max(op->rdn.maxh, 1), bra, ket, d.p);
+    PHUB_HIP(c, hipGetLastError());
+    std::vector<double> h(n4);
+    PHUB_HIP(c, hipMemcpyAsync(h.data(), d.p, n4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    // computed for K <= J, N <= M; the reference copies to the other orderings
+    auto ix = [n](int a, int b, int cc, int dd) { return (((size_t)a * n + b) * n + cc) * n + dd; };
+    for (int J = 0; J < n; ++J)
+        for (int K = 0; K <= J; ++K)
+            for (int M = 0; M < n; ++M)
+                for (int N = 0; N <= M; ++N) {
+                    const double v = h[ix(J, K, M, N)];
+                    out[ix(J, K, M, N)] = v;
+                    out[ix(J, K, N, M)] = v;
+                    out[ix(K, J, M, N)] = v;
+                    out[ix(K, J, N, M)] = v;
+                }
+    return PHUB_OK;
+}
+
 // Heisenberg J_SPIN (static_rq_heis.f90:58-116) and dE/dJ
 // ---------------------------------------------------------------------------
 // flip[b] = sum over states i with site J up, K down of bra[pos] * ket[i]

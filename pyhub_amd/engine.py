@@ -210,11 +210,16 @@ class HubbardOp(_Op):
         self.n, self.nup, self.ndn = int(n), int(nup), int(ndn)
         self.T = np.ascontiguousarray(np.asarray(T, dtype=np.float64).reshape(n, n))
         U = np.asarray(U, dtype=np.float64)
+        # a (n, n, n, n) U is the 4-index interaction (is_Uloc = 3); per-site U is then 0
+        self.U4 = np.ascontiguousarray(U) if U.ndim == 4 else None
+        if self.U4 is not None:
+            U = np.zeros(n)
         self.U = np.ascontiguousarray(np.full(n, float(U)) if U.ndim == 0 else U.reshape(n))
         self.T_dn = None if T_dn is None else np.ascontiguousarray(
             np.asarray(T_dn, dtype=np.float64).reshape(n, n))
         d = L.HubbardDesc(self.n, self.nup, self.ndn, _dptr(self.T), _dptr(self.U),
-                          _dptr(self.T_dn) if self.T_dn is not None else None)
+                          _dptr(self.T_dn) if self.T_dn is not None else None,
+                          _dptr(self.U4) if self.U4 is not None else None)
         h = C.c_void_p()
         rc = ctx.lib.phub_op_hubbard(ctx.h, C.byref(d), C.byref(h))
         self.h = h
@@ -229,6 +234,16 @@ class HubbardOp(_Op):
                                                      C.c_void_p(ket.data_ptr()), _dptr(up),
                                                      _dptr(dn), _dptr(ni), _dptr(ss)))
         return up.reshape(n, n), dn.reshape(n, n), ni, ss.reshape(n, n)
+
+    def two_rdm(self, bra: torch.Tensor, ket: torch.Tensor | None = None):
+        """DBL_OCC(J,K,M,N) (TWO_RDM, static_rq.f90:247-306), shape (n, n, n, n)."""
+        ket = bra if ket is None else ket
+        n = self.n
+        out = np.zeros(n ** 4)
+        self.ctx.check(self.ctx.lib.phub_two_rdm_hubbard(self.ctx.h, self.h,
+                                                         C.c_void_p(bra.data_ptr()),
+                                                         C.c_void_p(ket.data_ptr()), _dptr(out)))
+        return out.reshape(n, n, n, n)
 
     def hf_grad(self, psi: torch.Tensor):
         n = self.n
@@ -294,6 +309,16 @@ class HeisenbergOp(_Op):
                                                         C.c_void_p(bra.data_ptr()),
                                                         C.c_void_p(ket.data_ptr()), _dptr(ss)))
         return ss.reshape(n, n)
+
+    def two_rdm(self, bra: torch.Tensor, ket: torch.Tensor | None = None):
+        """DBL_OCC(J,K,M,N) (TWO_RDM, static_rq.f90:247-306), shape (n, n, n, n)."""
+        ket = bra if ket is None else ket
+        n = self.n
+        out = np.zeros(n ** 4)
+        self.ctx.check(self.ctx.lib.phub_two_rdm_hubbard(self.ctx.h, self.h,
+                                                         C.c_void_p(bra.data_ptr()),
+                                                         C.c_void_p(ket.data_ptr()), _dptr(out)))
+        return out.reshape(n, n, n, n)
 
     def hf_grad(self, psi: torch.Tensor):
         n = self.n

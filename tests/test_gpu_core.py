@@ -244,7 +244,8 @@ def test_heisenberg_sisj_vs_oracle(ctx, n, seed):
 
 def test_band_lanczos_exact_and_moments(ctx):
     """Band Lanczos (GF_BANDLANCZOS) from the p excitations c_i psi0:
-    - a 225-state sector is exhausted, so every G_ij(omega) equals dense ED;
+    - run past the size of a 225-state sector (the recurrence deflates), every
+      G_ij(omega) equals dense ED;
     - on a 52,920-state sector the zeroth and first moments are exact at any step
       count: sum_k Q_ik Q_jk = <v_i|v_j>, sum_k Q_ik Q_jk E_k = <v_i|H|v_j>."""
     from pyhub_amd.engine import HubbardOp
@@ -260,7 +261,8 @@ def test_band_lanczos_exact_and_moments(ctx):
         tgt, x = op.excite(psi, site, 0, False)
         xs.append(x)
     Vs = torch.stack(xs)
-    pos, Q = tgt.band_lanczos(Vs, max_steps=400, tol=1e-13)
+    # tol 0: run to max_steps - 10 = 390 steps, past the 225 states (deflated tail)
+    pos, Q = tgt.band_lanczos(Vs, max_steps=400, tol=0.0)
     mt = O.HubbardModel(6, 2, 2, T, 3.0)
     et, Ut = np.linalg.eigh(mt.dense())
     Qd = Ut.T @ Vs.cpu().numpy().T          # (poles, sites)
@@ -282,6 +284,40 @@ def test_band_lanczos_exact_and_moments(ctx):
     S1 = (Vs @ HV.T).cpu().numpy()
     assert np.max(np.abs(Q.T @ Q - S0)) < 1e-10
     assert np.max(np.abs((Q.T * pos) @ Q - S1)) < 1e-9 * max(1.0, np.max(np.abs(S1)))
+
+
+def _rand_u4(rng, n, density=0.15):
+    U = rng.normal(size=(n, n, n, n))
+    U[rng.random((n, n, n, n)) > density] = 0.0
+    U = 0.5 * (U + np.einsum("ijkl->jilk", U))
+    for J in range(n):
+        U[J, J, J, J] = rng.uniform(1, 4)
+    return U
+
+
+@pytest.mark.parametrize("n,nu,nd,seed", [(6, 3, 2, 0), (6, 3, 3, 1), (7, 2, 4, 2)])
+def test_hubbard_4index_u_vs_oracle(ctx, n, nu, nd, seed):
+    """is_Uloc = 3: H*v (FP64 and FP32) and DBL_OCC against the oracle's HPROD /
+    TWO_RDM restatements, and the Lanczos ground energy against dense ED."""
+    from pyhub_amd import _lib as L
+    from pyhub_amd.engine import HubbardOp
+    rng = np.random.default_rng(80 + seed)
+    T, _ = rand_cluster(rng, n)
+    U4 = _rand_u4(rng, n)
+    op = HubbardOp(ctx, n, nu, nd, T, U4)
+    m = O.HubbardModel(n, nu, nd, T, U4)
+    v = rng.normal(size=m.dim)
+    ref = m.hv(v)
+    w = op.hv(dev(v)).cpu().numpy()
+    assert np.max(np.abs(w - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+    w32 = op.hv(dev(v, torch.float32)).double().cpu().numpy()
+    assert np.max(np.abs(w32 - ref)) <= 2e-5 * max(1.0, np.max(np.abs(ref)))
+    a, b = rng.normal(size=m.dim), rng.normal(size=m.dim)
+    D = op.two_rdm(dev(a), dev(b))
+    assert np.max(np.abs(D - m.two_rdm(a, b))) < 1e-11
+    e_ref = np.linalg.eigvalsh(m.dense())[0]
+    r = op.lanczos(mode=L.LCZ_FULL64, max_steps=min(300, op.dim), tol=1e-13)
+    assert abs(r.e0 - e_ref) < 1e-10 * max(1.0, abs(e_ref))
 
 
 @pytest.mark.parametrize("site,spin,create", [(1, 0, 1), (3, 0, 0), (2, 1, 1), (6, 1, 0)])

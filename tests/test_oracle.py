@@ -170,6 +170,61 @@ def test_heisenberg_fast_oracle_and_standard_sz0():
     assert (O.j1j2_square(6, 6, 1.0, 0.5) > 0).sum() == 144
 
 
+def rand_u4(rng, n, density=0.15):
+    """Random 4-index U with the reference's required symmetry U[i,j,k,l] = U[j,i,l,k]
+    (core_solver.py:143) plus a local part on U[J,J,J,J]."""
+    U = rng.normal(size=(n, n, n, n))
+    U[rng.random((n, n, n, n)) > density] = 0.0
+    U = 0.5 * (U + np.einsum("ijkl->jilk", U))
+    for J in range(n):
+        U[J, J, J, J] = rng.uniform(1, 4)
+    return U
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference HPROD build (oracle/_ref) absent")
+@pytest.mark.parametrize("n,nu,nd,seed", [(5, 2, 3, 0), (6, 3, 3, 1), (6, 2, 3, 2)])
+def test_hubbard_4index_u_bit_exact_vs_reference(n, nu, nd, seed):
+    """is_Uloc = 3: the oracle's HPROD restatement against the reference's own HPROD
+    (compiled from its sources), bit for bit; and a purely local UL equals the
+    per-site U operator."""
+    rng = np.random.default_rng(seed)
+    T = rng.integers(-4, 5, size=(n, n)) / 4.0
+    T = np.triu(T, 1)
+    T = T + T.T
+    U4 = rand_u4(rng, n)
+    m = O.HubbardModel(n, nu, nd, T, U4)
+    for _ in range(2):
+        v = rng.normal(size=m.dim)
+        assert np.array_equal(m.hv(v), m.hv_ref(v))
+    Ud = rng.uniform(1, 4, n)
+    U4 = np.zeros((n, n, n, n))
+    for J in range(n):
+        U4[J, J, J, J] = Ud[J]
+    v = rng.normal(size=m.dim)
+    a = O.HubbardModel(n, nu, nd, T, U4).hv(v)
+    b = O.HubbardModel(n, nu, nd, T, Ud).hv(v)
+    assert np.max(np.abs(a - b)) < 1e-13
+
+
+def test_two_rdm_restatement():
+    """DBL_OCC (TWO_RDM) against two independent pins: its (J,J,J,J) entries are the
+    double occupancies NI, and for K <= J, N <= M each entry equals <bra|H_1|ket>
+    with H_1 the 4-index operator whose only non-zero UL is at (J,K,M,N)."""
+    rng = np.random.default_rng(7)
+    n, nu, nd = 5, 2, 3
+    m = O.HubbardModel(n, nu, nd, np.zeros((n, n)), np.zeros(n))
+    a = rng.normal(size=m.dim)
+    b = rng.normal(size=m.dim)
+    D = m.two_rdm(a, b)
+    ni = m.ni(a, b)
+    assert np.max(np.abs(np.array([D[j, j, j, j] for j in range(n)]) - ni)) < 1e-13
+    for (J, K, M, N) in [(0, 0, 1, 1), (2, 1, 3, 0), (4, 4, 2, 2), (3, 0, 4, 1), (1, 1, 4, 3)]:
+        U4 = np.zeros((n, n, n, n))
+        U4[J, K, M, N] = 1.0
+        h1 = O.HubbardModel(n, nu, nd, np.zeros((n, n)), U4)
+        assert abs(D[J, K, M, N] - b @ h1.hv(a)) < 1e-13, (J, K, M, N)
+
+
 def test_library_exports_header_symbols():
     """libphub.so loads and exports every function include/phub.h declares."""
     so = os.path.join(ROOT, "pyhub_amd", "libphub.so")
