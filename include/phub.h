@@ -141,7 +141,13 @@ typedef struct {
     int refine_steps;       /* FP64 residual-correction steps after MIXED (0..4) */
     uint64_t seed;          /* start vector when d_v0 is NULL */
     int want_psi;           /* reconstruct the ground vector into d_psi */
+    int n_states;           /* Ritz vectors to reconstruct (READ_PSI_HUBBARD for EXC_STATE
+                               1..n_states, compute_solver.f90:183-290); 0 or 1 = ground
+                               only, at most PHUB_MAX_STATES; d_psi then holds n_states
+                               vectors of dim, state k at d_psi + k*dim */
 } phub_lanczos_opts;
+
+#define PHUB_MAX_STATES 16
 
 typedef struct {
     double e0;              /* final ground energy (after refinement when MIXED) */
@@ -153,6 +159,7 @@ typedef struct {
     double seconds;         /* device wall time of the solve */
     double seconds_lanczos; /* of which: Krylov steps */
     double hv_ms_avg;       /* average H*v kernel time */
+    double e_states[PHUB_MAX_STATES];   /* lowest Ritz values (e_states[0] = e0) */
 } phub_lanczos_result;
 
 int phub_lanczos(phub_ctx *ctx, const phub_op *op, const phub_lanczos_opts *opts,

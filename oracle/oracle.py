@@ -335,6 +335,14 @@ def ring_T(L, t=1.0, eps=None, periodic=True):
     return T
 
 
+def disordered_ring_T(L, seed=11, scale=0.6):
+    """Ring hoppings plus a random symmetric matrix (all pairs, on-site terms
+    included): no lattice symmetry is left, so low spectra are non-degenerate."""
+    rng = np.random.default_rng(seed)
+    T = rng.normal(size=(L, L)) * scale
+    return (T + T.T) / 2 + ring_T(L)
+
+
 def square_T(Lx, Ly, t=1.0, tp=0.0, periodic=True):
     """Site s = x + Lx*y; T = -t on NN bonds, -tp on NNN bonds."""
     n = Lx * Ly

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libphub.so")
 
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_OOM, ERR_NOCONV, ERR_BREAKDOWN, ERR_HIP, ERR_NORM_LOST = range(8)
 LCZ_PLAIN64, LCZ_FULL64, LCZ_MIXED, LCZ_PLAIN32 = 0, 1, 2, 3
+MAX_STATES = 16                     # PHUB_MAX_STATES
 
 
 class HubbardDesc(C.Structure):
@@ -38,14 +39,14 @@ class OpInfo(C.Structure):
 class LanczosOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("max_steps", C.c_int), ("tol", C.c_double),
                 ("check_every", C.c_int), ("fixed_steps", C.c_int), ("refine_steps", C.c_int),
-                ("seed", C.c_uint64), ("want_psi", C.c_int)]
+                ("seed", C.c_uint64), ("want_psi", C.c_int), ("n_states", C.c_int)]
 
 
 class LanczosResult(C.Structure):
     _fields_ = [("e0", C.c_double), ("e0_ritz", C.c_double), ("residual", C.c_double),
                 ("steps", C.c_int), ("converged", C.c_int), ("ndeg", C.c_int),
                 ("seconds", C.c_double), ("seconds_lanczos", C.c_double),
-                ("hv_ms_avg", C.c_double)]
+                ("hv_ms_avg", C.c_double), ("e_states", C.c_double * MAX_STATES)]
 
 
 class Cluster(C.Structure):

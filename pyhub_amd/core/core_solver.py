@@ -146,21 +146,35 @@ class CoreSolver(Basis, GreenFunction, StaticQuantities):
             v0 = None
             if self.start_vector is not None:
                 v0 = torch.as_tensor(np.asarray(self.start_vector, float), device=op.ctx.device)
-            out = op.lanczos(mode=_MODES[self.lanczos_mode], max_steps=self.max_lcz,
-                             tol=self.acc_lcz, v0=v0, refine_steps=2, want_psi=True)
             from scipy.linalg import eigh_tridiagonal
-            m = len(out.alpha)
-            val, vec = eigh_tridiagonal(out.alpha, out.beta[1:m]) if m > 1 else (out.alpha.copy(), np.ones((1, 1)))
+            # The Ritz vectors of every state with a Boltzmann weight are rebuilt from
+            # the Lanczos vectors (READ_PSI_HUBBARD over NW states).  NW is known only
+            # after the run: when T > 0 lifts it above the states rebuilt, the
+            # recurrence is repeated (same start vector, same alpha/beta) for NW states.
+            nst = max(1, self.nb_comp_states_)
+            while True:
+                if nst > L.MAX_STATES:
+                    raise NotImplementedError(
+                        f"{nst} Lanczos states carry a Boltzmann weight; at most "
+                        f"{L.MAX_STATES} are rebuilt (lower T or use the dense path)")
+                out = op.lanczos(mode=_MODES[self.lanczos_mode], max_steps=self.max_lcz,
+                                 tol=self.acc_lcz, v0=v0, refine_steps=2, want_psi=True,
+                                 n_states=nst)
+                m = len(out.alpha)
+                val, vec = eigh_tridiagonal(out.alpha, out.beta[1:m]) if m > 1 else (out.alpha.copy(), np.ones((1, 1)))
+                nw = len(boltzmann_weights(val, self.T, self.nb_comp_states_))
+                if nw <= nst or nst >= m:
+                    break
+                nst = nw
             if self.lanczos_mode == "mixed":
                 val = val.copy()
                 val[0] = out.e0
             r.update(is_lanczos=True, e=val, tri_vectors=vec, psi0_dev=out.psi, ndeg=out.ndeg,
+                     psis_dev=out.psis,
                      alpha=out.alpha, beta=out.beta, steps=out.steps, converged=out.converged,
                      conv=out.e0 - out.e0_ritz, residual=out.residual, lanczos=out,
                      H=(np.diag(out.alpha) + np.diag(out.beta[1:m], 1) + np.diag(out.beta[1:m], -1))
                      if self.store_H else None)
-            if self.nb_comp_states_ > 1 and not np.isclose(self.T, 0.0):
-                raise NotImplementedError("excited Lanczos states (T > 0) are not reconstructed yet")
         r["boltzmann"] = boltzmann_weights(r["e"], self.T, self.nb_comp_states_)
         self._res = r
         self.is_solve = True
@@ -178,8 +192,10 @@ class CoreSolver(Basis, GreenFunction, StaticQuantities):
         nw = len(self.wgt)
         if not r["is_lanczos"]:
             return [r["vectors"][:, b].contiguous() for b in range(nw)]
-        if nw > 1 and np.count_nonzero(self.wgt[1:]) > 0:
-            raise NotImplementedError("excited Lanczos states are not reconstructed yet")
+        # Ritz vectors of the lowest states (READ_PSI_HUBBARD, compute_solver.f90:271-285)
+        if r.get("psis_dev") is not None:
+            ps = r["psis_dev"]
+            return [ps[b] if b < ps.shape[0] else None for b in range(nw)]
         return [r["psi0_dev"]] + [None] * (nw - 1)
 
     def reduced_quantities(self):
@@ -318,6 +334,9 @@ class CoreSolver(Basis, GreenFunction, StaticQuantities):
         case of GF_BANDLANCZOS, spgf_bandlanczos.f90:309-448).  Each chain gives its
         own poles (Ritz values) and weights Q_ik = |c psi| y_k[0]."""
         from scipy.linalg import eigh_tridiagonal
+        if np.count_nonzero(self.wgt) > 1:
+            raise NotImplementedError("Lanczos-chain Green's functions are ground-state only "
+                                      "(spgf_bandlanczos.f90:63); use the dense path at T > 0")
         op = self._op
         psi = self._state_vectors()[0]
         nsteps = nsteps or min(self.max_lcz, 200)
@@ -414,15 +433,20 @@ class CoreSolver(Basis, GreenFunction, StaticQuantities):
         if not r["is_lanczos"]:
             V = r["vectors"][:, : len(w)].cpu().numpy()
             return V @ w
+        ps = r.get("psis_dev")
+        if ps is not None and len(w) > 1:
+            return w @ ps[: len(w)].cpu().numpy()
         return r["psi0_dev"].cpu().numpy()
 
     def psi(self, vector: int):
         r = self._res
         if not r["is_lanczos"]:
             return r["vectors"][:, vector].cpu().numpy()
-        if vector != 0:
-            raise NotImplementedError("only the ground Lanczos vector is reconstructed")
-        return r["psi0_dev"].cpu().numpy()
+        ps = r.get("psis_dev")
+        if ps is None or not (0 <= vector < ps.shape[0]):
+            raise ValueError(f"Lanczos state {vector} was not reconstructed "
+                             f"(nb_comp_states = {self.nb_comp_states_})")
+        return ps[vector].cpu().numpy()
 
     def psi_as_dict(self, vector: int):
         psi = self.psi(vector)

@@ -558,6 +558,7 @@ struct LczState {
     bool breakdown = false;
     bool norm_lost = false;
     double ev = 0.0, e0 = 0.0;
+    std::vector<double> prev;          // lowest Ritz values at the previous check
 };
 
 // convergence test in the reference's form (lanczos.f90:74-89) on the Ritz values
@@ -572,11 +573,41 @@ static void ref_convergence(LczState &s, double tol, int max_steps, const double
     }
 }
 
+// With several requested states the run also continues until each of the nst lowest
+// Ritz values has moved by less than tol since the previous check (the reference's
+// test alone follows the two lowest distinct levels).  nst = 1 is the reference's test.
+static void convergence(LczState &s, double tol, int max_steps, const std::vector<double> &ritz,
+                        int nst) {
+    ref_convergence(s, tol, max_steps, ritz.data());
+    if (nst <= 1) return;
+    const int m = (int)ritz.size();
+    bool stable = m >= nst && (int)s.prev.size() >= nst;
+    for (int k = 0; stable && k < nst; ++k) stable = std::fabs(ritz[k] - s.prev[k]) < tol;
+    s.prev.assign(ritz.begin(), ritz.begin() + std::min(m, nst));
+    if (!stable) s.converged = false;
+}
+
 static int all_ritz(int m, const double *alpha, const double *beta, std::vector<double> &out) {
     out.assign(alpha, alpha + m);
     std::vector<double> e(std::max(m - 1, 1), 0.0);
     for (int i = 0; i + 1 < m; ++i) e[i] = beta[i + 1];
     return tridiag_ql(m, out.data(), e.data(), nullptr);
+}
+
+// the nst lowest Ritz values and their tridiagonal eigenvectors (ys[k][j], j < m)
+static int ritz_states(const LczState &s, int nst, std::vector<double> &vals,
+                       std::vector<std::vector<double>> &ys) {
+    const int m = s.steps;
+    std::vector<double> d(s.alpha.begin(), s.alpha.begin() + m), e(std::max(m - 1, 1), 0.0);
+    for (int i = 0; i + 1 < m; ++i) e[i] = s.beta[i + 1];
+    std::vector<double> z((size_t)m * m);
+    if (tridiag_ql(m, d.data(), e.data(), z.data())) return 1;
+    nst = std::max(1, std::min(nst, m));
+    vals.assign(d.begin(), d.begin() + nst);
+    ys.assign(nst, std::vector<double>(m));
+    for (int k = 0; k < nst; ++k)
+        for (int j = 0; j < m; ++j) ys[k][j] = z[(size_t)k * m + j];
+    return 0;
 }
 
 // plain three-term recurrence (no reorthogonalisation), 3 resident vectors
@@ -630,7 +661,7 @@ static int lanczos_plain(phub_ctx *c, const phub_op *op, const phub_lanczos_opts
                 return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
             s.e0 = ritz[0];
             if (s.breakdown) { s.converged = true; break; }
-            ref_convergence(s, o->tol, maxs, ritz.data());
+            convergence(s, o->tol, maxs, ritz, o->n_states);
         }
     }
     if (fixed || s.steps == 0 || !s.converged) {
@@ -642,25 +673,48 @@ static int lanczos_plain(phub_ctx *c, const phub_op *op, const phub_lanczos_opts
 // second pass of the plain recurrence to rebuild psi = sum_k y_k x_k
 template <typename T>
 static int plain_second_pass(phub_ctx *c, const phub_op *op, const T *v0t, T *b1, T *b2, T *b3,
-                             const LczState &s, const std::vector<double> &y, double *psi) {
+                             const LczState &s, const std::vector<std::vector<double>> &ys,
+                             double *psi) {
+    // psi_k = sum_j ys[k][j] x_j for every requested state k (state k at psi + k*n)
     const int64_t n = op->dim;
     const unsigned g = grid_for(n);
-    PHUB_HIP(c, hipMemsetAsync(psi, 0, n * sizeof(double), c->stream));
+    const int nst = (int)ys.size();
+    PHUB_HIP(c, hipMemsetAsync(psi, 0, (size_t)nst * n * sizeof(double), c->stream));
     // x_0 = v0 / beta0 in b1
     hipLaunchKernelGGL((k_axpby<T, T>), dim3(g), dim3(256), 0, c->stream, 1.0 / s.beta[0], v0t,
                        0.0, b1, n, (double *)nullptr);
-    hipLaunchKernelGGL((k_axpby<T, double>), dim3(g), dim3(256), 0, c->stream, y[0], b1, 1.0,
-                       psi, n, (double *)nullptr);
+    for (int k = 0; k < nst; ++k)
+        hipLaunchKernelGGL((k_axpby<T, double>), dim3(g), dim3(256), 0, c->stream, ys[k][0], b1,
+                           1.0, psi + (size_t)k * n, n, (double *)nullptr);
     T *xp = b2, *xk = b1, *out = b3;
-    const int m = (int)y.size();
+    const int m = (int)ys[0].size();
     for (int k = 0; k + 1 < m; ++k) {
         int rc = phub_hv_launch<T>(c, op, xk, out, 1.0, k ? xp : nullptr, s.beta[k], false, nullptr);
         if (rc) return rc;
         hipLaunchKernelGGL((k_second_pass<T>), dim3(g), dim3(256), 0, c->stream, out, xk,
-                           s.alpha[k], 1.0 / s.beta[k + 1], psi, y[k + 1], n);
+                           s.alpha[k], 1.0 / s.beta[k + 1], psi, ys[0][k + 1], n);
+        for (int q = 1; q < nst; ++q)
+            hipLaunchKernelGGL((k_axpby<T, double>), dim3(g), dim3(256), 0, c->stream,
+                               ys[q][k + 1], out, 1.0, psi + (size_t)q * n, n, (double *)nullptr);
         PHUB_HIP(c, hipGetLastError());
         std::swap(xp, xk);
         std::swap(xk, out);
+    }
+    return PHUB_OK;
+}
+
+// normalise the reconstructed states; the ground state's drift is the reference's
+// NORM LOST check (lanczos.f90:74-76)
+static int normalise_states(phub_ctx *c, double *psi, int nst, int64_t n, LczState &s) {
+    for (int k = 0; k < nst; ++k) {
+        double *pk = psi + (size_t)k * n;
+        double nn = 0;
+        int rc = dev_dot<double, double>(c, pk, pk, n, &nn);
+        if (rc) return rc;
+        if (k == 0 && std::fabs(1.0 - std::sqrt(nn)) > 1e-5) s.norm_lost = true;
+        rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / std::sqrt(nn), pk,
+                                       n, nullptr);
+        if (rc) return rc;
     }
     return PHUB_OK;
 }
@@ -726,7 +780,7 @@ static int lanczos_full(phub_ctx *c, const phub_op *op, const phub_lanczos_opts 
                 return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
             s.e0 = ritz[0];
             if (s.breakdown) { s.converged = true; break; }
-            ref_convergence(s, o->tol, maxs, ritz.data());
+            convergence(s, o->tol, maxs, ritz, o->n_states);
         }
     }
     if (s.steps && all_ritz(s.steps, s.alpha.data(), s.beta.data(), ritz) == 0) s.e0 = ritz[0];
@@ -821,22 +875,17 @@ static int plain_mode(phub_ctx *c, const phub_op *op, const phub_lanczos_opts *o
     *t_lcz = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     res->e0_ritz = res->e0 = s.e0;
     if (o->want_psi) {
-        double e0 = 0, e1 = 0;
-        std::vector<double> y;
-        if (lowest_ritz(s.steps, s.alpha.data(), s.beta.data(), &e0, &e1, &y))
+        std::vector<double> vals;
+        std::vector<std::vector<double>> ys;
+        if (ritz_states(s, std::max(1, o->n_states), vals, ys))
             return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
-        rc = plain_second_pass<T>(c, op, v0.p, b1.p, b2.p, b3.p, s, y, d_psi);
+        rc = plain_second_pass<T>(c, op, v0.p, b1.p, b2.p, b3.p, s, ys, d_psi);
         if (rc) return rc;
         hipFree(b2.p);
         b2.p = nullptr;
         hipFree(b3.p);
         b3.p = nullptr;
-        double nn = 0;
-        rc = dev_dot<double, double>(c, d_psi, d_psi, n, &nn);
-        if (rc) return rc;
-        if (std::fabs(1.0 - std::sqrt(nn)) > 1e-5) s.norm_lost = true;   // lanczos.f90:74-76
-        rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / std::sqrt(nn),
-                                       d_psi, n, nullptr);
+        rc = normalise_states(c, d_psi, (int)ys.size(), n, s);
         if (rc) return rc;
         PHUB_HIP(c, hipStreamSynchronize(c->stream));
         res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -856,6 +905,8 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
     if (o->max_steps < 1) return phub_fail(c, PHUB_ERR_ARG, "max_steps must be >= 1");
     if (o->mode < 0 || o->mode > 3) return phub_fail(c, PHUB_ERR_ARG, "unknown Lanczos mode");
     if (o->want_psi && !d_psi) return phub_fail(c, PHUB_ERR_ARG, "want_psi needs d_psi");
+    if (o->n_states < 0 || o->n_states > PHUB_MAX_STATES)
+        return phub_fail(c, PHUB_ERR_ARG, "n_states must be in [0, PHUB_MAX_STATES]");
     std::memset(res, 0, sizeof(*res));
     PHUB_HIP(c, hipSetDevice(c->device));
     const int64_t n = op->dim;
@@ -894,18 +945,19 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
         t_lcz = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         res->e0_ritz = res->e0 = s.e0;
         if (o->want_psi) {
-            double e0 = 0, e1 = 0;
-            std::vector<double> y;
-            if (lowest_ritz(s.steps, s.alpha.data(), s.beta.data(), &e0, &e1, &y))
+            std::vector<double> vals;
+            std::vector<std::vector<double>> ys;
+            if (ritz_states(s, std::max(1, o->n_states), vals, ys))
                 return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
-            std::vector<double> cf(s.steps);
-            for (int j = 0; j < s.steps; ++j) cf[j] = y[j] * sig[j];
-            double nn = 0;
-            rc = dev_combine<double, double>(c, basis.p, ld, s.steps, cf.data(), d_psi, n, true, &nn, coef.p);
-            if (rc) return rc;
-            if (std::fabs(1.0 - std::sqrt(nn)) > 1e-5) s.norm_lost = true;
-            rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / std::sqrt(nn),
-                                           d_psi, n, nullptr);
+            for (size_t k = 0; k < ys.size(); ++k) {
+                std::vector<double> cf(s.steps);
+                for (int j = 0; j < s.steps; ++j) cf[j] = ys[k][j] * sig[j];
+                double nn = 0;
+                rc = dev_combine<double, double>(c, basis.p, ld, s.steps, cf.data(), d_psi + k * n,
+                                                 n, true, &nn, coef.p);
+                if (rc) return rc;
+            }
+            rc = normalise_states(c, d_psi, (int)ys.size(), n, s);
             if (rc) return rc;
             PHUB_HIP(c, hipStreamSynchronize(c->stream));
             res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -935,18 +987,23 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
         res->e0_ritz = res->e0 = s.e0;
         if (o->want_psi || o->refine_steps > 0) {
             if (!d_psi) return phub_fail(c, PHUB_ERR_ARG, "refinement needs d_psi");
-            double e0 = 0, e1 = 0;
-            std::vector<double> y;
-            if (lowest_ritz(s.steps, s.alpha.data(), s.beta.data(), &e0, &e1, &y))
+            std::vector<double> vals;
+            std::vector<std::vector<double>> ys;
+            if (ritz_states(s, std::max(1, o->n_states), vals, ys))
                 return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
-            std::vector<double> cf(s.steps);
-            for (int j = 0; j < s.steps; ++j) cf[j] = y[j] * sig[j];
-            double nn = 0;
-            rc = dev_combine<float, double>(c, basis.p, ld, s.steps, cf.data(), d_psi, n, true, &nn, coef.p);
-            if (rc) return rc;
-            rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / std::sqrt(nn),
-                                           d_psi, n, nullptr);
-            if (rc) return rc;
+            for (size_t k = 0; k < ys.size(); ++k) {
+                std::vector<double> cf(s.steps);
+                for (int j = 0; j < s.steps; ++j) cf[j] = ys[k][j] * sig[j];
+                double nn = 0;
+                rc = dev_combine<float, double>(c, basis.p, ld, s.steps, cf.data(), d_psi + k * n,
+                                                n, true, &nn, coef.p);
+                if (rc) return rc;
+            }
+            {
+                LczState tmp = s;   // the FP32 basis drifts more: no norm check here
+                rc = normalise_states(c, d_psi, (int)ys.size(), n, tmp);
+                if (rc) return rc;
+            }
             // the FP32 basis is no longer needed: free it before the FP64 work
             hipFree(basis.p);
             basis.p = nullptr;
@@ -972,6 +1029,12 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
     if (!o->want_psi && !(o->mode == PHUB_LCZ_MIXED && o->refine_steps > 0))
         res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     res->seconds_lanczos = t_lcz;
+    {
+        std::vector<double> rv;
+        if (s.steps > 0 && all_ritz(s.steps, s.alpha.data(), s.beta.data(), rv) == 0)
+            for (int k = 0; k < PHUB_MAX_STATES && k < s.steps; ++k) res->e_states[k] = rv[k];
+        if (o->mode == PHUB_LCZ_MIXED) res->e_states[0] = res->e0;
+    }
     res->steps = s.steps;
     res->converged = (s.converged || fixed) ? 1 : 0;
     res->ndeg = s.ndeg;

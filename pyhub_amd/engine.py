@@ -91,6 +91,8 @@ class LanczosOutput:
     alpha: np.ndarray
     beta: np.ndarray
     psi: torch.Tensor | None
+    psis: torch.Tensor | None = None        # (n_states, dim) when n_states > 1
+    e_states: np.ndarray | None = None      # lowest Ritz values
 
 
 class _Op:
@@ -140,28 +142,38 @@ class _Op:
 
     def lanczos(self, mode=L.LCZ_PLAIN64, max_steps=600, tol=1e-8, check_every=5,
                 fixed_steps=0, refine_steps=0, seed=1, v0: torch.Tensor | None = None,
-                want_psi=True, raise_noconv=False) -> LanczosOutput:
+                want_psi=True, raise_noconv=False, n_states=1) -> LanczosOutput:
         o = L.LanczosOpts(mode, max_steps, tol, check_every, fixed_steps, refine_steps, seed,
-                          1 if want_psi else 0)
+                          1 if want_psi else 0, int(n_states))
         res = L.LanczosResult()
         nal = max(max_steps, fixed_steps) + 2
         alpha = np.zeros(nal)
         beta = np.zeros(nal + 1)
-        psi = self.vec(torch.float64) if (want_psi or mode == L.LCZ_MIXED) else None
+        psis = None
+        if n_states > 1 and want_psi:
+            psis = torch.zeros((int(n_states), self.dim), dtype=torch.float64, device=self.ctx.device)
+            psi = psis[0]
+        else:
+            psi = self.vec(torch.float64) if (want_psi or mode == L.LCZ_MIXED) else None
         v0p = None
         if v0 is not None:
             v0 = v0.to(self.ctx.device, torch.float64).contiguous()
             v0p = C.c_void_p(v0.data_ptr())
+        pbuf = psis if psis is not None else psi
         rc = self.ctx.lib.phub_lanczos(self.ctx.h, self.h, C.byref(o), v0p, _dptr(alpha),
                                        _dptr(beta), C.byref(res),
-                                       C.c_void_p(psi.data_ptr()) if psi is not None else None)
+                                       C.c_void_p(pbuf.data_ptr()) if pbuf is not None else None)
         if rc == L.ERR_NOCONV and not raise_noconv:
             pass
         else:
             self.ctx.check(rc)
+        nst = min(res.steps, L.MAX_STATES)
+        if psis is None and psi is not None and want_psi:
+            psis = psi.unsqueeze(0)
         return LanczosOutput(res.e0, res.e0_ritz, res.residual, res.steps, bool(res.converged),
                              res.ndeg, res.seconds, res.seconds_lanczos, res.hv_ms_avg,
-                             alpha[: res.steps].copy(), beta[: res.steps + 1].copy(), psi)
+                             alpha[: res.steps].copy(), beta[: res.steps + 1].copy(), psi,
+                             psis, np.array(res.e_states[:nst]))
 
     def cf_chain(self, start: torch.Tensor, nsteps: int, precision: int = 64):
         a = np.zeros(nsteps)
