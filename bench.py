@@ -142,7 +142,7 @@ def main():
     op._info()
     tiles = [int(t) for t in op.info.dn_tile]
     print(f"# H*v FP32 {ms_avg:.3f} ms ({gbs:.0f} GB/s alg), FP64 {ms64:.3f} ms; down tiles "
-          f"(segment bytes*1000 + rows) fp32 {tiles[0]} fp64 {tiles[1]}", file=sys.stderr, flush=True)
+          f"(segment bytes*100000 + rows*100 + in-flight) fp32 {tiles[0]} fp64 {tiles[1]}", file=sys.stderr, flush=True)
 
     out = {
         "metric": "H*v HBM GB/s (%peak); Lanczos s/solve, mixed vs FP64, at 1e-9 rel. energy",
@@ -358,16 +358,29 @@ def _lanczos_bench(op, args, L, torch):
     g = torch.Generator(device=op.ctx.device)
     g.manual_seed(2)
     v0 = (torch.rand(op.dim, generator=g, device=op.ctx.device, dtype=torch.float64) - 0.5)
-    # mixed: FP32 basis + FP64 scalars/refinement; fp64_full: FP64 basis, the same full
+    # mixed: FP32 basis + FP64 scalars/refinement; mixed_plain: FP32 three-term recurrence
+    # without reorthogonalisation + FP64 scalars, psi combined from the kept FP32 vectors,
+    # then the same FP64 refinement; fp64_full: FP64 basis, the same full
     # reorthogonalisation (a 300-step FP64 basis needs 397 GB, so it is capped at 180
-    # steps = 240 GB); fp64: the reference's three-term recurrence, no reorthogonalisation
-    for name, mode, cap in (("mixed", L.LCZ_MIXED, args.lanczos_max),
-                            ("fp64_full", L.LCZ_FULL64, min(args.lanczos_max, 180)),
-                            ("fp64", L.LCZ_PLAIN64, args.lanczos_max)):
+    # steps = 240 GB); fp64: the reference's three-term recurrence, no
+    # reorthogonalisation, psi combined from the kept vectors (as the reference)
+    runs = (("mixed", L.LCZ_MIXED, args.lanczos_max),
+            ("mixed_plain", L.LCZ_PLAIN32, args.lanczos_max),
+            ("fp64_full", L.LCZ_FULL64, min(args.lanczos_max, 180)),
+            ("fp64", L.LCZ_PLAIN64, args.lanczos_max))
+    # one workspace for every run, allocated before the timed solves: the Krylov
+    # vectors of each run live in it (the plain FP64 run keeps as many as fit and
+    # rebuilds psi by a second pass beyond that).  Allocation is outside the timing:
+    # a fresh allocation after large frees can cost seconds of driver-side clearing.
+    free, _ = torch.cuda.mem_get_info(op.ctx.device)
+    need = max(op.lanczos_workspace_bytes(m, cap, 0, 2, True) for _, m, cap in runs)
+    ws = torch.empty(min(need, int(0.9 * free)), dtype=torch.uint8, device=op.ctx.device)
+    res["workspace_gb"] = ws.numel() / 1e9
+    for name, mode, cap in runs:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         r = op.lanczos(mode=mode, max_steps=cap, tol=args.lanczos_tol, v0=v0,
-                       refine_steps=2, want_psi=True)
+                       refine_steps=2, want_psi=True, workspace=ws)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         res[name] = {"e0": r.e0, "e0_ritz": r.e0_ritz, "steps": r.steps,
@@ -377,12 +390,15 @@ def _lanczos_bench(op, args, L, torch):
         print(f"# lanczos {name}: E0={r.e0:.12f} steps={r.steps} {r.seconds:.2f}s "
               f"res={r.residual:.2e}", file=sys.stderr, flush=True)
         del r
-        torch.cuda.empty_cache()
+    del ws
+    torch.cuda.empty_cache()
     e64 = res["fp64_full"]["e0"]
-    for k in ("mixed", "fp64"):
+    for k in ("mixed", "mixed_plain", "fp64"):
         res[k]["rel_err_vs_fp64_full"] = abs(res[k]["e0"] - e64) / abs(e64)
     res["speedup_mixed_vs_fp64_full"] = res["fp64_full"]["seconds"] / res["mixed"]["seconds"]
     res["speedup_mixed_vs_fp64_plain"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
+    res["speedup_mixed_plain_vs_fp64_plain"] = res["fp64"]["seconds"] / res["mixed_plain"]["seconds"]
+    res["speedup_mixed_plain_vs_fp64_full"] = res["fp64_full"]["seconds"] / res["mixed_plain"]["seconds"]
     return res
 
 

@@ -104,7 +104,8 @@ typedef struct {
     int max_hops_up, max_hops_dn;
     double table_bytes;  /* device bytes of the hop / rank tables */
     int dn_tile[2];      /* Hubbard down-hop tile the H*v tuner chose, FP32 / FP64:
-                            (bytes per row segment) * 1000 + rows per workgroup; -1 = not yet tuned */
+                            (bytes per row segment) * 100000 + (rows per workgroup) * 100
+                            + (row gathers in flight per lane); -1 = not yet tuned */
 } phub_op_info;
 
 int phub_op_hubbard(phub_ctx *ctx, const phub_hubbard_desc *desc, phub_op **out);
@@ -125,12 +126,16 @@ int phub_hv_f32(phub_ctx *ctx, const phub_op *op, const float *v, float *w);
  * READ_PSI_HUBBARD (compute_solver.f90:183-290). */
 enum {
     PHUB_LCZ_PLAIN64 = 0,   /* FP64 vectors, 3-term recurrence, no reorthogonalisation
-                               (the reference algorithm), psi by a second pass */
+                               (the reference algorithm); with want_psi the vectors are
+                               kept while memory allows and psi is combined from them
+                               (as the reference), else psi comes from a second pass */
     PHUB_LCZ_FULL64 = 1,    /* FP64 vectors, full reorthogonalisation, stored basis */
     PHUB_LCZ_MIXED = 2,     /* FP32 vectors, FP64 scalars, full reorthogonalisation,
                                FP64 psi reconstruction + FP64 refinement steps */
-    PHUB_LCZ_PLAIN32 = 3    /* FP32 vectors (3 resident), FP64 scalars, 3-term
-                               recurrence; for sectors whose Krylov basis cannot be stored */
+    PHUB_LCZ_PLAIN32 = 3    /* FP32 vectors, FP64 scalars, 3-term recurrence (vectors kept
+                               as PLAIN64); with refine_steps, FP64 refinement as MIXED.
+                               Runs in 3 resident vectors when psi is not wanted (sectors
+                               whose Krylov basis cannot be stored) */
 };
 typedef struct {
     int mode;               /* PHUB_LCZ_* */
@@ -138,13 +143,19 @@ typedef struct {
     double tol;             /* convergence of the lowest Ritz value (reference acc_lcz) */
     int check_every;        /* reference NADD = 5 */
     int fixed_steps;        /* >0: run exactly this many steps, no convergence test */
-    int refine_steps;       /* FP64 residual-correction steps after MIXED (0..4) */
+    int refine_steps;       /* FP64 refinement steps after MIXED / PLAIN32 (0..4) */
     uint64_t seed;          /* start vector when d_v0 is NULL */
     int want_psi;           /* reconstruct the ground vector into d_psi */
     int n_states;           /* Ritz vectors to reconstruct (READ_PSI_HUBBARD for EXC_STATE
                                1..n_states, compute_solver.f90:183-290); 0 or 1 = ground
                                only, at most PHUB_MAX_STATES; d_psi then holds n_states
                                vectors of dim, state k at d_psi + k*dim */
+    void *workspace;        /* caller-owned device memory for the Krylov vectors, or NULL
+                               (then allocated per call); phub_lanczos_workspace_bytes
+                               gives the size that holds every vector.  With a smaller
+                               one the plain modes keep fewer vectors (psi by a second
+                               pass) and FULL64 / MIXED allocate their basis instead */
+    size_t workspace_bytes;
 } phub_lanczos_opts;
 
 #define PHUB_MAX_STATES 16
@@ -162,6 +173,11 @@ typedef struct {
     double e_states[PHUB_MAX_STATES];   /* lowest Ritz values (e_states[0] = e0) */
 } phub_lanczos_result;
 
+/* Device bytes phub_lanczos uses for its Krylov vectors under these options (the
+ * workspace size that lets it allocate none).  No counterpart in the reference
+ * (which writes every vector to solver.h5, lanczos.f90:49). */
+int phub_lanczos_workspace_bytes(const phub_op *op, const phub_lanczos_opts *opts,
+                                 size_t *bytes);
 int phub_lanczos(phub_ctx *ctx, const phub_op *op, const phub_lanczos_opts *opts,
                  const double *d_v0 /* device start vector or NULL */,
                  double *alpha /* host, max_steps */, double *beta /* host, max_steps+1 */,

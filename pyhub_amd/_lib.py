@@ -39,7 +39,8 @@ class OpInfo(C.Structure):
 class LanczosOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("max_steps", C.c_int), ("tol", C.c_double),
                 ("check_every", C.c_int), ("fixed_steps", C.c_int), ("refine_steps", C.c_int),
-                ("seed", C.c_uint64), ("want_psi", C.c_int), ("n_states", C.c_int)]
+                ("seed", C.c_uint64), ("want_psi", C.c_int), ("n_states", C.c_int),
+                ("workspace", C.c_void_p), ("workspace_bytes", C.c_size_t)]
 
 
 class LanczosResult(C.Structure):
@@ -82,6 +83,7 @@ SIGNATURES = {
     "phub_op_masks": (C.c_int, [_vp, _vp, _vp, _vp]),
     "phub_hv_f64": (C.c_int, [_vp, _vp, _vp, _vp]),
     "phub_hv_f32": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "phub_lanczos_workspace_bytes": (C.c_int, [_vp, C.POINTER(LanczosOpts), C.POINTER(C.c_size_t)]),
     "phub_lanczos": (C.c_int, [_vp, _vp, C.POINTER(LanczosOpts), _vp, _dp, _dp,
                                C.POINTER(LanczosResult), _vp]),
     "phub_rdm_hubbard": (C.c_int, [_vp, _vp, _vp, _vp, _dp, _dp, _dp, _dp]),

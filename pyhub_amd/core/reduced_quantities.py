@@ -179,6 +179,60 @@ class GreenFunction:
                                  for i in range(self.nb_sites)] for j in range(self.nb_sites)])
                 for spin in ("up", "down")}
 
+    def _g_matrix(self, w, eta, spin, weighted):
+        """Interacting G(w) of the computed sites from the pole data (the sum in
+        reduced_quantities.py:134-161).  weighted: each state b enters with
+        wgt_b / sum(wgt); otherwise the states are summed with weight 1, as the
+        reference does when N_up != N_dn."""
+        d = self._gfdata
+        if "chains" in d:
+            raise NotImplementedError("needs the full Q matrix (band Lanczos / dense path)")
+        s = 0 if spin == "up" else 1
+        nb = self.nb_poles
+        Qg, Ql = d[f"Q_greater_{spin}"], d[f"Q_lesser_{spin}"]
+        pg, pl = d[f"positions_greater_{spin}"], d[f"positions_lesser_{spin}"]
+        G = 0.0
+        nst = min(len(self.wgt) if weighted else self.nb_comp_states, Qg.shape[2])
+        for b in range(nst):
+            f = self.wgt[b] / np.sum(self.wgt) if weighted else 1.0
+            qg = Qg[:, : nb[1, s], b]
+            ql = Ql[:, : nb[0, s], b]
+            G = G + (np.einsum("ik,jk,k->ij", qg, qg, 1.0 / (w - (-self.e[b] + pg[: nb[1, s]]) + 1j * eta))
+                     + np.einsum("ik,jk,k->ij", ql, ql, 1.0 / (w - (self.e[b] - pl[: nb[0, s]]) + 1j * eta))) * f
+        return G
+
+    def self_energy(self, w, eta=0.05):
+        """Sigma(w) = G0(w)^+ - G(w)^+ (pseudo-inverses) for both spins
+        (reduced_quantities.py:134-161).  G0 is the non-interacting G of t_matrix
+        (ek, Vk).  A scalar w gives {spin: (n, n)}; an array gives
+        {spin: (n, n, len(w))} indexed [j, i, k] = Sigma(w_k)[i, j], as the
+        reference stacks it."""
+        if isinstance(w, (list, np.ndarray)):
+            A = [self.self_energy(w_, eta=eta) for w_ in w]
+            return {spin: np.transpose(np.array([a[spin] for a in A], dtype=np.complex128), (2, 1, 0))
+                    for spin in ("up", "down")}
+        G0 = np.einsum("ik,jk,k->ij", self.Vk, self.Vk, 1.0 / (w - self.ek + 1j * eta))
+        if self.nup == self.ndown:
+            A = np.linalg.pinv(G0) - np.linalg.pinv(self._g_matrix(w, eta, "up", True))
+            return {"up": A, "down": A}
+        return {spin: np.linalg.pinv(G0) - np.linalg.pinv(self._g_matrix(w, eta, spin, False))
+                for spin in ("up", "down")}
+
+    @property
+    def galitskii_migdal(self):
+        # an empty property in the reference (reduced_quantities.py:163-166)
+        return None
+
+    @property
+    def qpw(self):
+        """Quasiparticle weight (I - Re dSigma_up/dw at mu_up)^+ with Sigma at
+        eta = 1e-4 (reduced_quantities.py:167-169); the derivative is the
+        three-point central difference with dx = 1e-6."""
+        mu = self.mu["up"]
+        dx = 1.0e-6
+        d = (self.self_energy(mu + dx, eta=0.0001)["up"] - self.self_energy(mu - dx, eta=0.0001)["up"]) / (2 * dx)
+        return np.linalg.pinv(np.identity(self.nb_sites) - np.real(d))
+
     def gf_diag(self, omega, eta=0.05):
         """G_ii(omega) for every computed site and spin, evaluated on the device from
         the continued-fraction coefficients (chains path) -- complex (2, ncomp, nw)."""

@@ -147,7 +147,6 @@ __global__ __launch_bounds__(256) void k_hv_hub(HubArgs<T> a, const T *__restric
 // ---------------------------------------------------------------------------
 constexpr int UP_THREADS = 1024;
 constexpr int UP_UNROLL = 8;    // independent hop-entry loads in flight per lane
-constexpr int DN_UNROLL = 8;    // independent row gathers in flight per lane
 
 template <typename T, int R>
 __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
@@ -252,7 +251,7 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
 // (spaced COLS/VW apart, so every load instruction is one coalesced segment of
 // COLS/VW elements); a wave covers 64*VW/COLS rows at once.  The tile is picked
 // per operator by the tuner in phub_hv_launch.
-template <typename T, bool EPI, int COLS, int ROWS, int VW>
+template <typename T, bool EPI, int COLS, int ROWS, int VW, int UN>
 __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict__ x,
                                                T *__restrict__ out, double scale,
                                                const T *__restrict__ prev, double beta,
@@ -260,7 +259,6 @@ __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict
     constexpr int LPR = COLS / VW;          // lanes per row
     constexpr int RPW = 64 / LPR;           // rows per wave
     static_assert(LPR * VW == COLS && RPW * LPR == 64, "tile");
-    constexpr int UN = VW == 1 ? DN_UNROLL : (VW == 2 ? 4 : 2);
     const int b = blockIdx.x;
     const int xcd = b & 7, i = b >> 3;
     const int chunk = (i / nrb) * 8 + xcd;
@@ -739,16 +737,19 @@ constexpr int HUB_R = 4;
 // Candidate tiles of the down-hop kernel: (columns per row segment, rows per
 // workgroup, columns per lane).  FP32 and FP64 list the same byte widths, from
 // 64 B to 1 KB per row segment.
+// un: row gathers in flight per lane (each of vw elements)
 struct DnTile {
-    int cols, rows, vw;
+    int cols, rows, vw, un;
 };
-constexpr int DN_NCFG = 10;
-constexpr DnTile DN_F32[DN_NCFG] = {{16, 16, 1}, {32, 16, 1}, {32, 32, 1}, {64, 8, 1},
-                                    {64, 16, 1}, {64, 32, 1}, {128, 16, 2}, {128, 32, 2},
-                                    {256, 16, 4}, {256, 32, 4}};
-constexpr DnTile DN_F64[DN_NCFG] = {{8, 16, 1},  {16, 16, 1}, {16, 32, 1}, {32, 8, 1},
-                                    {32, 16, 1}, {32, 32, 1}, {64, 16, 2}, {64, 32, 2},
-                                    {128, 16, 4}, {128, 32, 4}};
+constexpr int DN_NCFG = 14;
+constexpr DnTile DN_F32[DN_NCFG] = {
+    {16, 16, 1, 8},  {32, 16, 1, 8},  {32, 32, 1, 8},  {64, 8, 1, 8},   {64, 16, 1, 8},
+    {64, 32, 1, 8},  {128, 16, 2, 4}, {128, 32, 2, 4}, {256, 16, 4, 2}, {256, 32, 4, 2},
+    {128, 16, 2, 8}, {128, 32, 2, 8}, {256, 16, 4, 4}, {256, 32, 4, 4}};
+constexpr DnTile DN_F64[DN_NCFG] = {
+    {8, 16, 1, 8},   {16, 16, 1, 8},  {16, 32, 1, 8},  {32, 8, 1, 8},   {32, 16, 1, 8},
+    {32, 32, 1, 8},  {64, 16, 2, 4},  {64, 32, 2, 4},  {128, 16, 4, 2}, {128, 32, 4, 2},
+    {64, 16, 2, 8},  {64, 32, 2, 8},  {128, 16, 4, 4}, {128, 32, 4, 4}};
 constexpr int DN_DEFAULT = 4;   // 256 B segments, 16 rows
 
 template <typename T>
@@ -759,7 +760,7 @@ constexpr DnTile dn_tile(int cfg) {
 int phub_dn_tile_code(int cfg, bool fp64) {
     if (cfg < 0) return -1;
     const DnTile t = fp64 ? DN_F64[cfg] : DN_F32[cfg];
-    return t.cols * (fp64 ? 8 : 4) * 1000 + t.rows;
+    return t.cols * (fp64 ? 8 : 4) * 100000 + t.rows * 100 + t.un;
 }
 
 template <typename T, bool EPI, int CFG>
@@ -769,7 +770,7 @@ static void dn_go(phub_ctx *c, const phub_op *op, const HubArgs<T> &a, const T *
     const int nchunks = (int)((op->dim_up + t.cols - 1) / t.cols);
     const int nrb = (int)((op->dim_dn + t.rows - 1) / t.rows);
     *nblocks = (int)std::min<int64_t>((int64_t)8 * ((nchunks + 7) / 8) * nrb, INT32_MAX);
-    hipLaunchKernelGGL((k_hv_dn<T, EPI, t.cols, t.rows, t.vw>), dim3(*nblocks), dim3(256), 0,
+    hipLaunchKernelGGL((k_hv_dn<T, EPI, t.cols, t.rows, t.vw, t.un>), dim3(*nblocks), dim3(256), 0,
                        c->stream, a, x, out, scale, prev, beta, c->d_red, nchunks, nrb);
 }
 
@@ -786,7 +787,11 @@ static void dn_dispatch(phub_ctx *c, const phub_op *op, const HubArgs<T> &a, con
     case 6: dn_go<T, EPI, 6>(c, op, a, x, out, scale, prev, beta, nb); break;
     case 7: dn_go<T, EPI, 7>(c, op, a, x, out, scale, prev, beta, nb); break;
     case 8: dn_go<T, EPI, 8>(c, op, a, x, out, scale, prev, beta, nb); break;
-    default: dn_go<T, EPI, 9>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 9: dn_go<T, EPI, 9>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 10: dn_go<T, EPI, 10>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 11: dn_go<T, EPI, 11>(c, op, a, x, out, scale, prev, beta, nb); break;
+    case 12: dn_go<T, EPI, 12>(c, op, a, x, out, scale, prev, beta, nb); break;
+    default: dn_go<T, EPI, 13>(c, op, a, x, out, scale, prev, beta, nb); break;
     }
 }
 
@@ -848,9 +853,9 @@ static int tune_dn(phub_ctx *c, phub_op *op, const HubArgs<T> &a, const T *x, T 
             tmin = std::min(tmin, ms);
         }
         if (std::getenv("PHUB_TUNE_VERBOSE"))
-            std::fprintf(stderr, "phub tune: %s down tile %d cols x %d rows (%d per lane): %.3f ms\n",
-                         sizeof(T) == 4 ? "fp32" : "fp64", dn_tile<T>(cfg).cols,
-                         dn_tile<T>(cfg).rows, dn_tile<T>(cfg).vw, tmin);
+            std::fprintf(stderr, "phub tune: %s down tile %d: %d cols x %d rows (%d per lane, %d in flight): %.3f ms\n",
+                         sizeof(T) == 4 ? "fp32" : "fp64", cfg, dn_tile<T>(cfg).cols,
+                         dn_tile<T>(cfg).rows, dn_tile<T>(cfg).vw, dn_tile<T>(cfg).un, tmin);
         if (tmin < best) { best = tmin; bestc = cfg; }
     }
     op->tune_dn[ti] = bestc;

@@ -511,6 +511,23 @@ static int dev_combine(phub_ctx *c, const T *basis, int64_t ld, int k, const dou
     return PHUB_OK;
 }
 
+// psi-mode combine over vectors at arbitrary (16-byte aligned) addresses:
+// w = sum_j coef_j vs[j]; d_ptrs holds at least k pointers
+template <typename T, typename TW>
+static int dev_combine_ptrs(phub_ctx *c, T *const *vs, int k, const double *coef_host, TW *w,
+                            int64_t n, double *d_coef, const T **d_ptrs) {
+    PHUB_HIP(c, hipMemcpyAsync(d_ptrs, vs, k * sizeof(T *), hipMemcpyHostToDevice, c->stream));
+    PHUB_HIP(c, hipMemcpyAsync(d_coef, coef_host, k * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+    hipLaunchKernelGGL((k_combine_v<T, TW>), dim3(grid_for(n)), dim3(256), k * sizeof(double),
+                       c->stream, (const T *)nullptr, (int64_t)0, k, d_coef, w, n, 1,
+                       (double *)nullptr, (const T *const *)d_ptrs);
+    PHUB_HIP(c, hipGetLastError());
+    // the host arrays must stay valid until the uploads have been consumed
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    return PHUB_OK;
+}
+
 struct EvTimer {
     phub_ctx *c;
     hipEvent_t a, b;
@@ -610,10 +627,91 @@ static int ritz_states(const LczState &s, int nst, std::vector<double> &vals,
     return 0;
 }
 
-// plain three-term recurrence (no reorthogonalisation), 3 resident vectors
+// Device memory for Krylov vectors: carved from a caller workspace while it lasts
+// (256-byte aligned pieces), then allocated here and freed with the arena.
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0, used = 0;
+    std::vector<void *> owned;
+    Arena(void *ws, size_t bytes) {
+        if (ws) {
+            const uintptr_t a = ((uintptr_t)ws + 255) & ~(uintptr_t)255;
+            const size_t skip = a - (uintptr_t)ws;
+            if (bytes > skip) {
+                base = (char *)a;
+                size = bytes - skip;
+            }
+        }
+    }
+    ~Arena() { release(); }
+    Arena(const Arena &) = delete;
+    Arena &operator=(const Arena &) = delete;
+    size_t room() const {
+        const size_t off = (used + 255) & ~(size_t)255;
+        return base && off < size ? size - off : 0;
+    }
+    void *take(size_t n) {
+        if (base && n <= room()) {
+            const size_t off = (used + 255) & ~(size_t)255;
+            used = off + n;
+            return base + off;
+        }
+        void *p = nullptr;
+        if (hipMalloc(&p, n) != hipSuccess) {
+            (void)hipGetLastError();   // a failed allocation is not a sticky error here
+            return nullptr;
+        }
+        owned.push_back(p);
+        return p;
+    }
+    // every piece handed out so far becomes invalid
+    void release() {
+        for (void *p : owned) hipFree(p);
+        owned.clear();
+        used = 0;
+    }
+};
+
+// Vectors of the plain recurrence, kept while device memory allows: the reference
+// keeps every Lanczos vector (lanczos.f90:49) and rebuilds psi from them
+// (compute_solver.f90:271-285).  slot[k] holds vector k while `growing`; chunks of
+// up to 8 vectors are added on demand.  When a chunk cannot be allocated the store
+// stops growing and the recurrence rotates three slots other than slot 0 (the
+// start vector), so psi can still be rebuilt by a second pass.  Vectors come from
+// an Arena: with a caller workspace the store never grows beyond it.
+template <typename T>
+struct KrylovStore {
+    Arena &ar;
+    std::vector<T *> slot;
+    int64_t ld = 0;
+    bool growing = false;
+    bool keep_first = false;   // slot 0 (the start vector) is never overwritten
+    int cap = 0;               // at most this many vectors (0: no limit)
+    KrylovStore(Arena &a, int64_t n) : ar(a), ld(basis_ld(n)) {}
+    bool add(int cnt) {
+        T *p = (T *)ar.take((size_t)cnt * ld * sizeof(T));
+        if (!p) return false;
+        for (int i = 0; i < cnt; ++i) slot.push_back(p + (size_t)i * ld);
+        return true;
+    }
+    bool grow() {
+        if (cap > 0 && (int)slot.size() >= cap) return false;
+        int cnt = cap > 0 ? std::min(8, cap - (int)slot.size()) : 8;
+        if (ar.base) {
+            // a caller workspace bounds the store: nothing is allocated for it
+            const size_t fit = ar.room() / ((size_t)ld * sizeof(T));
+            return fit >= 1 && add((int)std::min<size_t>(cnt, fit));
+        }
+        return add(cnt) || add(1);
+    }
+};
+
+// plain three-term recurrence (no reorthogonalisation): three rotating buffers
+// (v0t, b1, b2), or the vectors of a KrylovStore (st: v0t = st->slot[0])
 template <typename T>
 static int lanczos_plain(phub_ctx *c, const phub_op *op, const phub_lanczos_opts *o,
-                         T *v0t, T *b1, T *b2, LczState &s, EvTimer *tm, bool fixed) {
+                         T *v0t, T *b1, T *b2, LczState &s, EvTimer *tm, bool fixed,
+                         KrylovStore<T> *st = nullptr) {
     const int64_t n = op->dim;
     const int maxs = o->max_steps;
     const int every = o->check_every > 0 ? o->check_every : 5;
@@ -625,6 +723,7 @@ static int lanczos_plain(phub_ctx *c, const phub_op *op, const phub_lanczos_opts
     if (!(n2 > 0.0)) return phub_fail(c, PHUB_ERR_ARG, "start vector is zero");
     s.beta[0] = std::sqrt(n2);
     T *xp = b1, *xk = v0t, *out = b2;   // xp content irrelevant while beta_eff = 0
+    int ip = -1, ik = 0;                 // slot indices of xp, xk (with a store)
     double sig = 1.0 / s.beta[0], sigp = 0.0;
     std::vector<double> ritz;
     // PHUB_PROGRESS=k: one line on stderr every k steps (long solves)
@@ -632,6 +731,19 @@ static int lanczos_plain(phub_ctx *c, const phub_op *op, const phub_lanczos_opts
     const int progress = pe ? std::atoi(pe) : 0;
     const auto tp0 = std::chrono::steady_clock::now();
     while (!s.converged && !s.breakdown && s.steps < maxs) {
+        int io = -1;
+        if (st) {
+            if (st->growing && s.steps + 1 >= (int)st->slot.size() && !st->grow())
+                st->growing = false;
+            if (st->growing) {
+                io = s.steps + 1;
+            } else {
+                for (int q = st->keep_first ? 1 : 0; q < (int)st->slot.size() && io < 0; ++q)
+                    if (q != ip && q != ik) io = q;
+                if (io < 0) return phub_fail(c, PHUB_ERR_ARG, "Krylov store has too few slots");
+            }
+            out = st->slot[io];
+        }
         double dots[2];
         rc = fused_hv<T>(c, op, xk, sig, s.steps ? xp : nullptr, s.beta[s.steps] * sigp, out,
                          dots, tm);
@@ -646,6 +758,8 @@ static int lanczos_plain(phub_ctx *c, const phub_op *op, const phub_lanczos_opts
         s.beta[s.steps] = b;
         std::swap(xp, xk);
         std::swap(xk, out);
+        ip = ik;
+        ik = io;
         sigp = sig;
         sig = (b > 0.0) ? 1.0 / b : 0.0;
         if (!(b > 1e-13 * std::max(1.0, std::fabs(a)))) s.breakdown = true;
@@ -853,24 +967,25 @@ template <typename T>
 static int plain_mode(phub_ctx *c, const phub_op *op, const phub_lanczos_opts *o,
                       const phub_lanczos_opts *oo, const double *d_v0, double *d_psi,
                       LczState &s, EvTimer *tm, bool fixed, phub_lanczos_result *res,
-                      std::chrono::steady_clock::time_point t0, double *t_lcz) {
+                      std::chrono::steady_clock::time_point t0, double *t_lcz, double *coef,
+                      Arena &ar) {
     const int64_t n = op->dim;
     const unsigned g = (unsigned)((n + 255) / 256);
-    DevBuf<T> v0, b1, b2, b3;
-    PHUB_HIP(c, hipMalloc(&b1.p, n * sizeof(T)));
-    PHUB_HIP(c, hipMalloc(&b2.p, n * sizeof(T)));
-    PHUB_HIP(c, hipMalloc(&b3.p, n * sizeof(T)));
+    // with psi wanted the vectors are kept (slot 0 = start vector survives for a
+    // second pass if the store stops growing); otherwise three rotating slots
+    KrylovStore<T> st(ar, n);
+    if (!st.add(o->want_psi ? 4 : 3))
+        return phub_fail(c, PHUB_ERR_OOM, "Lanczos vectors do not fit in device memory");
+    st.growing = st.keep_first = o->want_psi != 0;
+    // PHUB_LANCZOS_KEEP=k caps the vectors kept (device memory); psi then comes from a
+    // second pass of the recurrence
+    if (const char *ek = std::getenv("PHUB_LANCZOS_KEEP")) st.cap = std::max(4, std::atoi(ek));
     if (d_v0)
-        hipLaunchKernelGGL(k_convert<T>, dim3(g), dim3(256), 0, c->stream, d_v0, b3.p, n, 1.0);
+        hipLaunchKernelGGL(k_convert<T>, dim3(g), dim3(256), 0, c->stream, d_v0, st.slot[0], n, 1.0);
     else
-        hipLaunchKernelGGL(k_randvec<T>, dim3(g), dim3(256), 0, c->stream, b3.p, n, o->seed);
+        hipLaunchKernelGGL(k_randvec<T>, dim3(g), dim3(256), 0, c->stream, st.slot[0], n, o->seed);
     PHUB_HIP(c, hipGetLastError());
-    if (o->want_psi) {
-        // the recurrence rotates three buffers; keep the start vector for the second pass
-        PHUB_HIP(c, hipMalloc(&v0.p, n * sizeof(T)));
-        PHUB_HIP(c, hipMemcpyAsync(v0.p, b3.p, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-    }
-    int rc = lanczos_plain<T>(c, op, oo, b3.p, b1.p, b2.p, s, tm, fixed);
+    int rc = lanczos_plain<T>(c, op, oo, st.slot[0], st.slot[1], st.slot[2], s, tm, fixed, &st);
     if (rc) return rc;
     *t_lcz = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     res->e0_ritz = res->e0 = s.e0;
@@ -879,22 +994,88 @@ static int plain_mode(phub_ctx *c, const phub_op *op, const phub_lanczos_opts *o
         std::vector<std::vector<double>> ys;
         if (ritz_states(s, std::max(1, o->n_states), vals, ys))
             return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
-        rc = plain_second_pass<T>(c, op, v0.p, b1.p, b2.p, b3.p, s, ys, d_psi);
+        if (st.growing) {
+            // every vector is stored: psi_q = sum_j y_qj x_j / beta_j in one pass each
+            DevBuf<const T *> dptr;
+            PHUB_HIP(c, hipMalloc(&dptr.p, (size_t)s.steps * sizeof(T *)));
+            std::vector<double> cf(s.steps);
+            for (size_t q = 0; q < ys.size(); ++q) {
+                for (int j = 0; j < s.steps; ++j)
+                    cf[j] = s.beta[j] > 0.0 ? ys[q][j] / s.beta[j] : 0.0;
+                rc = dev_combine_ptrs<T, double>(c, st.slot.data(), s.steps, cf.data(),
+                                                 d_psi + q * n, n, coef, dptr.p);
+                if (rc) return rc;
+            }
+        } else {
+            rc = plain_second_pass<T>(c, op, st.slot[0], st.slot[1], st.slot[2], st.slot[3], s,
+                                      ys, d_psi);
+            if (rc) return rc;
+        }
+        if (sizeof(T) == 4) {
+            LczState tmp = s;   // FP32 vectors drift more: no norm check here
+            rc = normalise_states(c, d_psi, (int)ys.size(), n, tmp);
+        } else {
+            rc = normalise_states(c, d_psi, (int)ys.size(), n, s);
+        }
         if (rc) return rc;
-        hipFree(b2.p);
-        b2.p = nullptr;
-        hipFree(b3.p);
-        b3.p = nullptr;
-        rc = normalise_states(c, d_psi, (int)ys.size(), n, s);
-        if (rc) return rc;
+        const int rs = sizeof(T) == 4 ? std::max(0, std::min(o->refine_steps, 4)) : 0;
+        // the Krylov vectors are no longer needed: their memory serves what follows
         PHUB_HIP(c, hipStreamSynchronize(c->stream));
+        st.slot.clear();
+        ar.release();
+        if (rs > 0) {
+            // FP32 recurrence, then FP64 Rayleigh-Ritz refinement of the ground state
+            // (as the MIXED mode)
+            const int m = 3;
+            double *b64 = (double *)ar.take((size_t)(m + 1) * basis_ld(n) * sizeof(double));
+            if (!b64) return phub_fail(c, PHUB_ERR_OOM, "FP64 refinement vectors do not fit");
+            for (int it = 0; it < rs; ++it) {
+                double er = 0;
+                rc = refine64(c, op, d_psi, b64, m, coef, &er);
+                if (rc) return rc;
+                res->e0 = er;
+            }
+            PHUB_HIP(c, hipStreamSynchronize(c->stream));
+            res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            double erq = 0;
+            return rq_residual(c, op, d_psi, b64, &erq, &res->residual);
+        }
         res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        DevBuf<double> tmp;
-        PHUB_HIP(c, hipMalloc(&tmp.p, n * sizeof(double)));
+        double *tmp = (double *)ar.take(n * sizeof(double));
+        if (!tmp) return phub_fail(c, PHUB_ERR_OOM, "residual vector does not fit");
         double erq = 0;   // e0 stays the Ritz value, as the reference reports it
-        rc = rq_residual(c, op, d_psi, tmp.p, &erq, &res->residual);
+        rc = rq_residual(c, op, d_psi, tmp, &erq, &res->residual);
         if (rc) return rc;
     }
+    return PHUB_OK;
+}
+
+extern "C" int phub_lanczos_workspace_bytes(const phub_op *op, const phub_lanczos_opts *o,
+                                            size_t *bytes) {
+    if (!op || !o || !bytes) return PHUB_ERR_ARG;
+    const int64_t n = op->dim;
+    const size_t ld = (size_t)basis_ld(n);
+    int steps = o->fixed_steps > 0 ? o->fixed_steps : o->max_steps;
+    steps = (int)std::min<int64_t>(std::max(steps, 1), std::max<int64_t>(n, 1));
+    const size_t m1 = (size_t)steps + 1;
+    const size_t refine = 4 * ld * sizeof(double);   // refine64 with m = 3
+    size_t b = 0;
+    switch (o->mode) {
+    case PHUB_LCZ_FULL64: b = m1 * ld * sizeof(double); break;
+    case PHUB_LCZ_MIXED:
+        b = std::max(m1 * ld * sizeof(float), (o->want_psi || o->refine_steps > 0) ? refine : 0);
+        break;
+    case PHUB_LCZ_PLAIN64:
+        b = o->want_psi ? std::max(m1, (size_t)4) * ld * sizeof(double) : 3 * ld * sizeof(double);
+        break;
+    case PHUB_LCZ_PLAIN32:
+        b = o->want_psi ? std::max(std::max(m1, (size_t)4) * ld * sizeof(float),
+                                   o->refine_steps > 0 ? refine : ld * sizeof(double))
+                        : 3 * ld * sizeof(float);
+        break;
+    default: return PHUB_ERR_ARG;
+    }
+    *bytes = b + 256;   // alignment slack
     return PHUB_OK;
 }
 
@@ -923,24 +1104,25 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
     PHUB_HIP(c, hipMalloc(&coef.p, (size_t)(oo.max_steps + 8) * sizeof(double)));
     int rc = PHUB_OK;
     double t_lcz = 0.0;
+    Arena ar(o->workspace, o->workspace_bytes);
 
     if (o->mode == PHUB_LCZ_PLAIN64) {
-        rc = plain_mode<double>(c, op, o, &oo, d_v0, d_psi, s, &tm, fixed, res, t0, &t_lcz);
+        rc = plain_mode<double>(c, op, o, &oo, d_v0, d_psi, s, &tm, fixed, res, t0, &t_lcz, coef.p,
+                                ar);
         if (rc) return rc;
     } else if (o->mode == PHUB_LCZ_FULL64) {
-        DevBuf<double> basis;
         const int64_t ld = basis_ld(n);
         size_t bytes = (size_t)(oo.max_steps + 1) * ld * sizeof(double);
-        hipError_t e = hipMalloc(&basis.p, bytes);
-        if (e != hipSuccess)
+        double *basis = (double *)ar.take(bytes);
+        if (!basis)
             return phub_fail(c, PHUB_ERR_OOM, "FP64 Krylov basis does not fit: " +
                                                   std::to_string(bytes / 1e9) + " GB requested");
         if (d_v0)
-            PHUB_HIP(c, hipMemcpyAsync(basis.p, d_v0, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            PHUB_HIP(c, hipMemcpyAsync(basis, d_v0, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         else
-            hipLaunchKernelGGL(k_randvec<double>, dim3(g), dim3(256), 0, c->stream, basis.p, n, o->seed);
+            hipLaunchKernelGGL(k_randvec<double>, dim3(g), dim3(256), 0, c->stream, basis, n, o->seed);
         std::vector<double> sig;
-        rc = lanczos_full<double>(c, op, &oo, basis.p, ld, s, sig, coef.p, &tm, fixed);
+        rc = lanczos_full<double>(c, op, &oo, basis, ld, s, sig, coef.p, &tm, fixed);
         if (rc) return rc;
         t_lcz = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         res->e0_ritz = res->e0 = s.e0;
@@ -953,7 +1135,7 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
                 std::vector<double> cf(s.steps);
                 for (int j = 0; j < s.steps; ++j) cf[j] = ys[k][j] * sig[j];
                 double nn = 0;
-                rc = dev_combine<double, double>(c, basis.p, ld, s.steps, cf.data(), d_psi + k * n,
+                rc = dev_combine<double, double>(c, basis, ld, s.steps, cf.data(), d_psi + k * n,
                                                  n, true, &nn, coef.p);
                 if (rc) return rc;
             }
@@ -962,26 +1144,26 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
             PHUB_HIP(c, hipStreamSynchronize(c->stream));
             res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             double E = 0;
-            rc = rq_residual(c, op, d_psi, basis.p, &E, &res->residual);
+            rc = rq_residual(c, op, d_psi, basis, &E, &res->residual);
             if (rc) return rc;
         }
     } else if (o->mode == PHUB_LCZ_PLAIN32) {
-        rc = plain_mode<float>(c, op, o, &oo, d_v0, d_psi, s, &tm, fixed, res, t0, &t_lcz);
+        rc = plain_mode<float>(c, op, o, &oo, d_v0, d_psi, s, &tm, fixed, res, t0, &t_lcz, coef.p,
+                               ar);
         if (rc) return rc;
     } else {   // MIXED
-        DevBuf<float> basis;
         const int64_t ld = basis_ld(n);
         size_t bytes = (size_t)(oo.max_steps + 1) * ld * sizeof(float);
-        hipError_t e = hipMalloc(&basis.p, bytes);
-        if (e != hipSuccess)
+        float *basis = (float *)ar.take(bytes);
+        if (!basis)
             return phub_fail(c, PHUB_ERR_OOM, "FP32 Krylov basis does not fit: " +
                                                   std::to_string(bytes / 1e9) + " GB requested");
         if (d_v0)
-            hipLaunchKernelGGL(k_convert<float>, dim3(g), dim3(256), 0, c->stream, d_v0, basis.p, n, 1.0);
+            hipLaunchKernelGGL(k_convert<float>, dim3(g), dim3(256), 0, c->stream, d_v0, basis, n, 1.0);
         else
-            hipLaunchKernelGGL(k_randvec<float>, dim3(g), dim3(256), 0, c->stream, basis.p, n, o->seed);
+            hipLaunchKernelGGL(k_randvec<float>, dim3(g), dim3(256), 0, c->stream, basis, n, o->seed);
         std::vector<double> sig;
-        rc = lanczos_full<float>(c, op, &oo, basis.p, ld, s, sig, coef.p, &tm, fixed);
+        rc = lanczos_full<float>(c, op, &oo, basis, ld, s, sig, coef.p, &tm, fixed);
         if (rc) return rc;
         t_lcz = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         res->e0_ritz = res->e0 = s.e0;
@@ -995,7 +1177,7 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
                 std::vector<double> cf(s.steps);
                 for (int j = 0; j < s.steps; ++j) cf[j] = ys[k][j] * sig[j];
                 double nn = 0;
-                rc = dev_combine<float, double>(c, basis.p, ld, s.steps, cf.data(), d_psi + k * n,
+                rc = dev_combine<float, double>(c, basis, ld, s.steps, cf.data(), d_psi + k * n,
                                                 n, true, &nn, coef.p);
                 if (rc) return rc;
             }
@@ -1004,23 +1186,23 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
                 rc = normalise_states(c, d_psi, (int)ys.size(), n, tmp);
                 if (rc) return rc;
             }
-            // the FP32 basis is no longer needed: free it before the FP64 work
-            hipFree(basis.p);
-            basis.p = nullptr;
+            // the FP32 basis is no longer needed: its memory serves the FP64 work
+            PHUB_HIP(c, hipStreamSynchronize(c->stream));
+            ar.release();
             const int rs = std::max(0, std::min(o->refine_steps, 4));
-            DevBuf<double> b64;
             const int m = 3;
-            PHUB_HIP(c, hipMalloc(&b64.p, (size_t)(m + 1) * basis_ld(n) * sizeof(double)));
+            double *b64 = (double *)ar.take((size_t)(m + 1) * basis_ld(n) * sizeof(double));
+            if (!b64) return phub_fail(c, PHUB_ERR_OOM, "FP64 refinement vectors do not fit");
             for (int it = 0; it < rs; ++it) {
                 double er = 0;
-                rc = refine64(c, op, d_psi, b64.p, m, coef.p, &er);
+                rc = refine64(c, op, d_psi, b64, m, coef.p, &er);
                 if (rc) return rc;
                 res->e0 = er;
             }
             PHUB_HIP(c, hipStreamSynchronize(c->stream));
             res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             double E = 0;
-            rc = rq_residual(c, op, d_psi, b64.p, &E, &res->residual);
+            rc = rq_residual(c, op, d_psi, b64, &E, &res->residual);
             if (rc) return rc;
             if (rs == 0) res->e0 = E;
         }
@@ -1033,7 +1215,9 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
         std::vector<double> rv;
         if (s.steps > 0 && all_ritz(s.steps, s.alpha.data(), s.beta.data(), rv) == 0)
             for (int k = 0; k < PHUB_MAX_STATES && k < s.steps; ++k) res->e_states[k] = rv[k];
-        if (o->mode == PHUB_LCZ_MIXED) res->e_states[0] = res->e0;
+        if (o->mode == PHUB_LCZ_MIXED || (o->mode == PHUB_LCZ_PLAIN32 && o->want_psi &&
+                                          o->refine_steps > 0))
+            res->e_states[0] = res->e0;
     }
     res->steps = s.steps;
     res->converged = (s.converged || fixed) ? 1 : 0;

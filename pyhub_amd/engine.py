@@ -140,11 +140,47 @@ class _Op:
                                                   C.c_void_p(dn.data_ptr())))
         return up, dn[: self.dim_dn]
 
+    def lanczos_workspace_bytes(self, mode=L.LCZ_PLAIN64, max_steps=600, fixed_steps=0,
+                                refine_steps=0, want_psi=True) -> int:
+        """Device bytes the Krylov vectors of such a run need (phub_lanczos_workspace_bytes)."""
+        o = L.LanczosOpts(mode, max_steps, 0.0, 5, fixed_steps, refine_steps, 1,
+                          1 if want_psi else 0, 1, None, 0)
+        nb = C.c_size_t(0)
+        rc = self.ctx.lib.phub_lanczos_workspace_bytes(self.h, C.byref(o), C.byref(nb))
+        if rc:
+            raise ValueError("bad Lanczos options")
+        return int(nb.value)
+
+    def _workspace(self, mode, max_steps, fixed_steps, refine_steps, want_psi):
+        """A torch-owned workspace for the Krylov vectors (torch's caching allocator
+        reuses it across calls instead of returning it to the driver).  The plain
+        modes take what fits in free memory; they keep fewer vectors and rebuild psi
+        by a second pass when it is smaller than the full need."""
+        need = self.lanczos_workspace_bytes(mode, max_steps, fixed_steps, refine_steps, want_psi)
+        dev = self.ctx.device
+        if mode in (L.LCZ_PLAIN64, L.LCZ_PLAIN32):
+            free, _ = torch.cuda.mem_get_info(dev)
+            cached = torch.cuda.memory_reserved(dev) - torch.cuda.memory_allocated(dev)
+            need = min(need, max(int(0.85 * (free + cached)), 0))
+        try:
+            return torch.empty(max(need, 1), dtype=torch.uint8, device=dev)
+        except torch.cuda.OutOfMemoryError:
+            return None
+
     def lanczos(self, mode=L.LCZ_PLAIN64, max_steps=600, tol=1e-8, check_every=5,
                 fixed_steps=0, refine_steps=0, seed=1, v0: torch.Tensor | None = None,
-                want_psi=True, raise_noconv=False, n_states=1) -> LanczosOutput:
+                want_psi=True, raise_noconv=False, n_states=1,
+                workspace: torch.Tensor | None = None) -> LanczosOutput:
+        """workspace: device memory for the Krylov vectors (any dtype, contiguous);
+        by default one is taken from torch's caching allocator for the call."""
+        ws = workspace if workspace is not None else self._workspace(
+            mode, max_steps, fixed_steps, refine_steps, want_psi)
+        if ws is not None and not ws.is_contiguous():
+            raise ValueError("workspace must be contiguous")
         o = L.LanczosOpts(mode, max_steps, tol, check_every, fixed_steps, refine_steps, seed,
-                          1 if want_psi else 0, int(n_states))
+                          1 if want_psi else 0, int(n_states),
+                          C.c_void_p(ws.data_ptr()) if ws is not None else None,
+                          ws.numel() * ws.element_size() if ws is not None else 0)
         res = L.LanczosResult()
         nal = max(max_steps, fixed_steps) + 2
         alpha = np.zeros(nal)

@@ -157,6 +157,37 @@ def test_lanczos_path_temperature_and_excited_states(ctx):
         fl.spgf()
 
 
+def test_self_energy_dimer_closed_form(ctx):
+    """self_energy / qpw (reduced_quantities.py:134-169) on the half-filled Hubbard
+    dimer, whose self-energy is known in closed form: in the bonding (e = -t) and
+    antibonding (e = +t) orbitals Sigma(z) = U/2 + U^2 / (4 (z - U/2 -+ 3t)), so the
+    quasiparticle weight at mu = U/2 is 1 / (1 + U^2 / (36 t^2)); U = 0 gives 0."""
+    from pyhub_amd.solver.fermi_hubbard import FermiHubbard
+    t, U = 1.0, 3.0
+    T = np.array([[0.0, -t], [-t, 0.0]])
+    fh = FermiHubbard(2, 1, 1, T, U=U)
+    fh.kernel(compute_spgf=True)
+    assert not fh.is_lanczos
+    ek, Vk = np.linalg.eigh(T)
+    for w, eta in ((0.37, 0.2), (-2.5, 0.05), (4.0, 0.5)):
+        Sk = Vk.T @ fh.self_energy(w, eta=eta)["up"] @ Vk
+        z = w + 1j * eta
+        exact = U / 2 + U * U / (4 * (z - U / 2 + np.array([-3 * t, 3 * t])))
+        assert np.max(np.abs(np.diag(Sk) - exact)) < 1e-10
+        assert abs(Sk[0, 1]) < 1e-10
+    ws = np.array([0.1, 0.7, 1.3])
+    S = fh.self_energy(ws, eta=0.1)
+    assert S["up"].shape == (2, 2, 3)
+    for k, w in enumerate(ws):
+        assert np.max(np.abs(S["down"][:, :, k].T - fh.self_energy(w, eta=0.1)["down"])) < 1e-12
+    assert abs(fh.mu["up"] - U / 2) < 1e-10
+    assert np.max(np.abs(fh.qpw - np.identity(2) / (1 + U * U / (36 * t * t)))) < 1e-5
+    assert fh.galitskii_migdal is None
+    f0 = FermiHubbard(2, 1, 1, T, U=0.0)
+    f0.kernel(compute_spgf=True)
+    assert np.max(np.abs(f0.self_energy(0.3, eta=0.1)["up"])) < 1e-10
+
+
 def test_fermi_hubbard_4index_u(ctx):
     """FermiHubbard with a 4-index U (is_Uloc = 3): energies against dense ED of the
     oracle operator, two_rdm against the oracle's DBL_OCC of the same psi, ni equal to

@@ -76,7 +76,7 @@ def test_hubbard_dense_matrix_bit_exact(ctx, n, nu, nd, seed):
     assert np.array_equal(Hg, Ho)
 
 
-@pytest.mark.parametrize("tile", list(range(10)))
+@pytest.mark.parametrize("tile", list(range(14)))
 def test_hubbard_down_tiles_vs_oracle(ctx, tile, monkeypatch):
     """Every down-hop tile the H*v tuner can pick gives the oracle's H*v (FP64 to
     1e-13, FP32 to 2e-5 relative), on a sector whose dimensions are not
@@ -187,14 +187,23 @@ def test_heisenberg_large_sector_properties(ctx, Lx, Ly):
 CFG1_E0 = -4.603526299989137
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
-def test_config1_lanczos_modes(ctx, mode):
+@pytest.mark.parametrize("mode,keep", [(0, None), (0, "4"), (1, None), (2, None), (3, None),
+                                       (3, "6")])
+def test_config1_lanczos_modes(ctx, mode, keep, monkeypatch):
+    """Every Lanczos mode on config 1.  The plain modes keep their vectors and
+    combine psi from them; with PHUB_LANCZOS_KEEP capping the vectors kept, psi
+    comes from a second pass of the recurrence instead.  PLAIN32 refines E0 in
+    FP64 (refine_steps = 2), as MIXED does."""
     from pyhub_amd.engine import HubbardOp
+    if keep:
+        monkeypatch.setenv("PHUB_LANCZOS_KEEP", keep)
     op = HubbardOp(ctx, 8, 4, 4, O.ring_T(8), 4.0)
     v0 = np.random.default_rng(1).uniform(-0.5, 0.5, op.dim)
-    r = op.lanczos(mode=mode, max_steps=300, tol=1e-12, v0=dev(v0), refine_steps=2)
+    # the FP32 recurrence's Ritz values settle at ~1e-9, not 1e-12
+    r = op.lanczos(mode=mode, max_steps=300, tol=1e-12 if mode != 3 else 1e-9, v0=dev(v0),
+                   refine_steps=2)
     assert r.converged
-    tol = 1e-10 if mode != 2 else 1e-9
+    tol = 1e-10 if mode in (0, 1) else 1e-9
     assert abs(r.e0 - CFG1_E0) / abs(CFG1_E0) < tol, (r.e0, r.steps)
     assert r.residual < (1e-6 if mode == 0 else 1e-5)
     m = O.HubbardModel(8, 4, 4, O.ring_T(8), 4.0)
@@ -208,6 +217,32 @@ def test_config1_lanczos_modes(ctx, mode):
     rd = m.one_rdm(V[:, 0])
     # psi error ~ residual / gap (gap 0.3035 above E0)
     assert np.max(np.abs(up - rd[0])) < 1e-10 + 20 * r.residual / 0.3
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_lanczos_workspace(ctx, mode):
+    """A caller workspace of phub_lanczos_workspace_bytes holds every Krylov vector;
+    a smaller one is accepted (the plain modes keep fewer vectors and rebuild psi by
+    a second pass, the stored-basis modes allocate the rest), with the same result."""
+    from pyhub_amd.engine import HubbardOp
+    op = HubbardOp(ctx, 8, 4, 4, O.ring_T(8), 4.0)
+    v0 = dev(np.random.default_rng(1).uniform(-0.5, 0.5, op.dim))
+    kw = dict(mode=mode, max_steps=200, tol=1e-12 if mode != 3 else 1e-9, v0=v0, refine_steps=2)
+    need = op.lanczos_workspace_bytes(mode, 200, 0, 2, True)
+    ld = (op.dim + 63) // 64 * 64
+    s = 4 if mode in (2, 3) else 8
+    assert need >= 201 * ld * s
+    full = op.lanczos(workspace=torch.empty(need, dtype=torch.uint8, device=ctx.device), **kw)
+    small = op.lanczos(workspace=torch.empty(5 * ld * 8 + 256, dtype=torch.uint8,
+                                             device=ctx.device), **kw)
+    auto = op.lanczos(**kw)
+    for r in (full, small, auto):
+        assert r.converged
+    assert full.steps == small.steps == auto.steps
+    tol = 1e-12 if mode != 3 else 1e-9
+    assert abs(full.e0 - small.e0) < tol and abs(full.e0 - auto.e0) < tol
+    ov = abs(float(torch.dot(full.psi, small.psi)))
+    assert abs(ov - 1.0) < (1e-10 if mode in (0, 1) else 1e-6)
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
