@@ -62,7 +62,9 @@ def main():
     ap.add_argument("--lanczos-tol", type=float, default=1e-9)
     ap.add_argument("--lanczos-max", type=int, default=300)
     ap.add_argument("--calib", action="store_true", help="add a 1 GiB device copy (counter calibration)")
-    ap.add_argument("--workload", default="hubbard4x4", choices=["hubbard4x4", "heis6x6", "batch256"])
+    ap.add_argument("--workload", default="hubbard4x4",
+                    choices=["hubbard4x4", "heis6x6", "batch256", "band4x4", "u4"])
+    ap.add_argument("--band-steps", type=int, default=192, help="band Lanczos steps (band4x4)")
     ap.add_argument("--lx", type=int, default=6, help="Heisenberg torus width")
     ap.add_argument("--ly", type=int, default=6, help="Heisenberg torus height")
     ap.add_argument("--lanczos-steps", type=int, default=150, help="Heisenberg Lanczos steps")
@@ -83,6 +85,12 @@ def main():
         return
     if args.workload == "batch256":
         _batch_bench(args, ctx, torch)
+        return
+    if args.workload == "band4x4":
+        _band_bench(args, ctx, torch, L)
+        return
+    if args.workload == "u4":
+        _u4_bench(args, ctx, torch)
         return
     Lx = Ly = 4
     n = Lx * Ly
@@ -335,6 +343,125 @@ def _batch_bench(args, ctx, torch):
         "batch": {"solve_s": ts, "gf_eval_s": tg, "chains": nchains, "max_dim": max(dims),
                   "sum_dim": int(sum(dims)), "e0_first": float(e0[0]),
                   "g_finite": bool(np.all(np.isfinite(G)))},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _band_bench(args, ctx, torch, L):
+    """Band Lanczos G_ij (GF_BANDLANCZOS) of the 4x4 ground state: the 16 hole
+    excitations c_i,up psi0 start a band run in the (7 up, 8 down) sector
+    (D = 147,232,800) for a fixed number of steps (tol = 0)."""
+    from pyhub_amd.engine import HubbardOp
+    T = square_T(4, 4, 1.0, -0.2)
+    op = HubbardOp(ctx, 16, 8, 8, T, 8.0)
+    g = torch.Generator(device=ctx.device)
+    g.manual_seed(2)
+    v0 = (torch.rand(op.dim, generator=g, device=ctx.device, dtype=torch.float64) - 0.5)
+    r = op.lanczos(mode=L.LCZ_PLAIN32, max_steps=300, tol=1e-9 * 8.5, v0=v0, refine_steps=2)
+    psi, e0 = r.psi, r.e0
+    del r, v0
+    tgt, xs = None, []
+    for site in range(1, 17):
+        tgt, x = op.excite(psi, site, 0, False, target=tgt)
+        xs.append(x)
+    V = torch.stack(xs)
+    del xs, psi
+    torch.cuda.empty_cache()
+    p = V.shape[0]
+    D = tgt.dim
+    steps = (args.band_steps + p - 1) // p * p
+    # one short untimed run (tuner, first-touch), then the timed run
+    tgt.band_lanczos(V, max_steps=2 * p + 10, tol=0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pos, Q = tgt.band_lanczos(V, max_steps=steps + 10, tol=0.0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = dt / steps * 1e3
+    # bytes per band step: FP64 H*v (read v, write w) + two multidots and two
+    # subtractions over the p previous and p - 1 next vectors + the alpha update and scale
+    hbar = _mean_hops(T, 7) / 2 + _mean_hops(T, 8) / 2
+    hv_bytes = 2 * 8 * D
+    orth_bytes = ((p + 1) + (p + 2) + p + (p + 1) + 3 + 2) * 8 * D
+    b_step = hv_bytes + orth_bytes
+    gbs = b_step / (ms * 1e-3) / 1e9
+    # zeroth moment: sum_k Q_ik Q_jk = <v_i|v_j> (exact at any step count)
+    G = V @ V.T
+    m0 = float(np.max(np.abs(Q.T @ Q - G.cpu().numpy())))
+    print(f"# band4x4: D={D} p={p} steps={steps} {dt:.2f} s ({ms:.2f} ms/step, {gbs:.0f} GB/s), "
+          f"{len(pos)} poles, lowest {pos[0]:.6f}, zeroth-moment error {m0:.1e}",
+          file=sys.stderr, flush=True)
+    out = {
+        "metric": "band Lanczos step time (G_ij of the 4x4 ground state, 16 start vectors)",
+        "value": round(ms, 3), "unit": "ms/step", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp64",
+        "data": "synthetic: ground state of the 4x4 model from a seeded start vector",
+        "config": {"workload": "band Lanczos, 4x4 t-t' Hubbard U=8, lesser up sector (7, 8), p=16",
+                   "dim": D, "parallelism": "single GPU"},
+        "band": {"seconds": dt, "poles": len(pos), "e0": e0, "zeroth_moment_err": m0,
+                 "bytes_per_step": b_step, "mean_hops": hbar},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(gbs / 8000.0, 4), "traffic": None,
+                     "kernel": "one band step: FP64 H*v + k_multidot_v/k_combine_v over 2p-1 vectors"},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _u4_bench(args, ctx, torch):
+    """4-index interaction H*v (is_Uloc = 3): 3x4 open cluster, N_up = N_dn = 6
+    (D = 853,776), dense random UL with UL[J,K,M,N] = UL[K,J,N,M]."""
+    from pyhub_amd.engine import HubbardOp
+    n, N = 12, 6
+    T = np.zeros((n, n))
+    for y in range(3):
+        for x in range(4):
+            i = x + 4 * y
+            if x + 1 < 4:
+                T[i, i + 1] = T[i + 1, i] = -1.0
+            if y + 1 < 3:
+                T[i, i + 4] = T[i + 4, i] = -1.0
+    rng = np.random.default_rng(4)
+    A = rng.uniform(-0.5, 0.5, (n, n, n, n))
+    U4 = (A + A.transpose(1, 0, 3, 2)) / 2
+    op = HubbardOp(ctx, n, N, N, T, U4)
+    D = op.dim
+    res = {}
+    for name, dt_ in (("fp64", torch.float64), ("fp32", torch.float32)):
+        g = torch.Generator(device=ctx.device)
+        g.manual_seed(4)
+        v = (torch.rand(D, generator=g, device=ctx.device, dtype=torch.float64) - 0.5).to(dt_)
+        w = torch.empty_like(v)
+        for _ in range(max(1, args.warmup)):
+            op.hv(v, w)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(max(1, args.steps))]
+        stream = torch.cuda.current_stream()
+        for a, b in evs:
+            a.record(stream)
+            op.hv(v, w)
+            b.record(stream)
+        torch.cuda.synchronize()
+        res[name] = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    # terms per state: (up pairs incl. occupied sites) x (down pairs incl. occupied sites)
+    pairs = N * (n - N) + N
+    flops = 2.0 * pairs * pairs * D
+    tf = flops / (res["fp64"] * 1e-3) / 1e12
+    print(f"# u4: D={D} FP64 {res['fp64']:.3f} ms FP32 {res['fp32']:.3f} ms, "
+          f"{pairs * pairs} terms/state, {tf:.2f} TFLOP/s", file=sys.stderr, flush=True)
+    out = {
+        "metric": "4-index U H*v time", "value": round(res["fp64"], 4), "unit": "ms",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(res["fp64"], 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp64",
+        "data": "synthetic: seeded uniform vector and UL tensor",
+        "config": {"workload": "3x4 open Hubbard, dense random 4-index U, N_up=N_dn=6",
+                   "dim": D, "parallelism": "single GPU"},
+        "u4": {"fp64_ms": res["fp64"], "fp32_ms": res["fp32"], "terms_per_state": pairs * pairs},
+        "roofline": {"bound": "mfma", "achieved": round(tf, 3), "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": round(tf / 78.6, 4), "traffic": None,
+                     "kernel": "k_hv_up<double,1> + k_hv_dn + k_hv_ul<double> (one H*v)"},
     }
     print(json.dumps(out), flush=True)
 
