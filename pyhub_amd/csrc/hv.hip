@@ -631,6 +631,150 @@ __global__ __launch_bounds__(256) void k_hv_ul(UlArgs a, const T *__restrict__ x
     if (EPI) block_sum2<T>(s_xo, s_oo, part);
 }
 
+// Row-staged form of the same sum.  A workgroup takes one target row d' and 256
+// up targets u'.  The down pairs of d' (c^dag_N c_M d = d', occupied sites as M = N)
+// are uniform over the workgroup: RB of them at a time, their source rows x[d][:]
+// and the UL slices UL(:, :, M, N) times the down sign are staged in LDS, then each
+// lane walks its own up pairs (J, K, u, sign) once per batch and reads both
+// operands from LDS.  Used when RB >= 2 rows fit the LDS budget.
+constexpr int UL_LDS_BUDGET = 48 * 1024;
+constexpr int UL_MAXPAIRS = 288;   // pairs of one species state, n <= 32: N (n - N + 1) <= 272
+
+template <typename T, bool EPI, int RB>
+__global__ __launch_bounds__(256) void k_hv_ul_rows(UlArgs a, int nchunk, const T *__restrict__ x,
+                                                    T *__restrict__ out, double scale,
+                                                    const T *__restrict__ prev, double beta,
+                                                    double *__restrict__ part) {
+    // LDS, batch-interleaved so a lane reads the RB values of one (J,K) or one u
+    // contiguously: s_ul[jk * RB + b] = UL(J,K,M_b,N_b) * sign_b, s_x[u * RB + b] = x[d_b][u]
+    extern __shared__ double smem_ul[];
+    __shared__ int s_mn[UL_MAXPAIRS];
+    __shared__ int64_t s_ds[UL_MAXPAIRS];
+    __shared__ double s_sg[UL_MAXPAIRS];
+    __shared__ int s_cnt;
+    const int n = a.n, n2 = n * n;
+    const int64_t nu = a.nu;
+    const int64_t dp = blockIdx.x / nchunk;
+    const int64_t upr = (int64_t)(blockIdx.x % nchunk) * 256 + threadIdx.x;
+    const bool live = upr < nu;
+    double *s_ul = smem_ul;
+    T *s_x = reinterpret_cast<T *>(smem_ul + (size_t)RB * n2);
+    if (threadIdx.x == 0) {
+        // the down pairs of d', in the order of the pull kernel (table, then sites)
+        const uint32_t xd = a.md[dp];
+        const int cntd = a.cd[dp];
+        int c = 0;
+        for (int ib = 0; ib < cntd; ++ib) {
+            const size_t o = (size_t)dp * a.maxh_d + ib;
+            const int q = a.pd[o];
+            s_mn[c] = (q % n) * n + q / n;
+            s_ds[c] = a.id[o];
+            s_sg[c] = a.sd[o];
+            ++c;
+        }
+        for (int site = 0; site < n; ++site)
+            if ((xd >> (n - 1 - site)) & 1u) {
+                s_mn[c] = site * n + site;
+                s_ds[c] = dp;
+                s_sg[c] = 1.0;
+                ++c;
+            }
+        s_cnt = c;
+    }
+    __syncthreads();
+    const int nbp = s_cnt;
+    uint32_t xu = 0;
+    int cntu = 0;
+    if (live) {
+        xu = a.mu[upr];
+        cntu = a.cu[upr];
+    }
+    double acc = 0.0;
+    for (int b0 = 0; b0 < nbp; b0 += RB) {
+        const int nb = min(RB, nbp - b0);
+        // batch slots past nb hold zeros (coefficient and value), so they add nothing
+        for (int t = threadIdx.x; t < RB * n2; t += 256) {
+            const int jk = t / RB, b = t - jk * RB;
+            s_ul[t] = b < nb ? a.ul[(size_t)jk * n2 + s_mn[b0 + b]] * s_sg[b0 + b] : 0.0;
+        }
+        for (int b = 0; b < RB; ++b) {
+            if (b < nb) {
+                const T *row = x + s_ds[b0 + b] * nu;
+                for (int64_t u = threadIdx.x; u < nu; u += 256) s_x[(size_t)u * RB + b] = row[u];
+            } else {
+                for (int64_t u = threadIdx.x; u < nu; u += 256) s_x[(size_t)u * RB + b] = (T)0;
+            }
+        }
+        __syncthreads();
+        if (live) {
+            for (int ia = 0; ia < cntu + n; ++ia) {
+                int jk;
+                double sg;
+                int64_t us;
+                if (ia < cntu) {
+                    const size_t o = (size_t)ia * nu + upr;
+                    const int pr = a.pu[o];
+                    jk = (pr % n) * n + pr / n;
+                    sg = a.su[o];
+                    us = a.iu[o];
+                } else {
+                    const int site = ia - cntu;
+                    if (!((xu >> (n - 1 - site)) & 1u)) continue;
+                    jk = site * n + site;
+                    sg = 1.0;
+                    us = upr;
+                }
+                const double *cu = s_ul + (size_t)jk * RB;
+                const T *xv = s_x + (size_t)us * RB;
+                double t = 0.0;
+#pragma unroll
+                for (int b = 0; b < RB; ++b) t += cu[b] * (double)xv[b];
+                acc += sg * t;
+            }
+        }
+        __syncthreads();
+    }
+    double s_xo = 0.0, s_oo = 0.0;
+    if (live) {
+        const int64_t e = dp * nu + upr;
+        if (EPI) {
+            double val = scale * ((double)out[e] + acc);
+            if (prev) val -= beta * (double)prev[e];
+            const T vt = (T)val;
+            out[e] = vt;
+            s_xo += scale * (double)x[e] * (double)vt;
+            s_oo += (double)vt * (double)vt;
+        } else {
+            out[e] = (T)((double)out[e] + acc);
+        }
+    }
+    if (EPI) block_sum2<T>(s_xo, s_oo, part);
+}
+
+template <typename T, bool EPI>
+static void launch_ul_rows(phub_ctx *c, int rb, unsigned grid, size_t lds, const UlArgs &a,
+                           int nchunk, const T *x, T *out, double scale, const T *prev,
+                           double beta) {
+    switch (rb) {
+    case 16:
+        hipLaunchKernelGGL((k_hv_ul_rows<T, EPI, 16>), dim3(grid), dim3(256), lds, c->stream, a,
+                           nchunk, x, out, scale, prev, beta, c->d_red);
+        break;
+    case 8:
+        hipLaunchKernelGGL((k_hv_ul_rows<T, EPI, 8>), dim3(grid), dim3(256), lds, c->stream, a,
+                           nchunk, x, out, scale, prev, beta, c->d_red);
+        break;
+    case 4:
+        hipLaunchKernelGGL((k_hv_ul_rows<T, EPI, 4>), dim3(grid), dim3(256), lds, c->stream, a,
+                           nchunk, x, out, scale, prev, beta, c->d_red);
+        break;
+    default:
+        hipLaunchKernelGGL((k_hv_ul_rows<T, EPI, 2>), dim3(grid), dim3(256), lds, c->stream, a,
+                           nchunk, x, out, scale, prev, beta, c->d_red);
+        break;
+    }
+}
+
 template <typename T>
 int phub_hv_ul(phub_ctx *c, const phub_op *op, const T *x, T *out, double scale, const T *prev,
                double beta, bool epi, int *nblocks_out) {
@@ -650,6 +794,30 @@ int phub_hv_ul(phub_ctx *c, const phub_op *op, const T *x, T *out, double scale,
     a.pd = op->rdn.pair;
     a.maxh_d = std::max(op->rdn.maxh, 1);
     a.ul = op->d_ul;
+    const int n2 = op->n * op->n;
+    // rows per batch: the largest power of two up to 16 that fits the LDS budget
+    const int64_t fit = UL_LDS_BUDGET / ((int64_t)op->dim_up * sizeof(T) + (int64_t)n2 * 8);
+    int rb = 16;
+    while (rb > 1 && rb > fit) rb /= 2;
+    const char *force_old = std::getenv("PHUB_UL_PULL");   // 1: the one-lane-per-target kernel
+    if (rb >= 2 && op->rdn.maxh + op->n <= UL_MAXPAIRS && !(force_old && force_old[0] == '1')) {
+        const int64_t nchunk = (op->dim_up + 255) / 256;
+        const int64_t nbr = nchunk * op->dim_dn;
+        if (nbr > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
+        const size_t lds = (size_t)rb * n2 * 8 + (size_t)rb * op->dim_up * sizeof(T);
+        if (epi) {
+            int rc = phub_ensure_red(c, (size_t)nbr * 2 + 64);
+            if (rc) return rc;
+            launch_ul_rows<T, true>(c, rb, (unsigned)nbr, lds, a, (int)nchunk, x, out, scale,
+                                    prev, beta);
+        } else {
+            launch_ul_rows<T, false>(c, rb, (unsigned)nbr, lds, a, (int)nchunk, x, out, scale,
+                                     prev, beta);
+        }
+        PHUB_HIP(c, hipGetLastError());
+        if (nblocks_out) *nblocks_out = (int)nbr;
+        return PHUB_OK;
+    }
     const int64_t nb = (op->dim + 255) / 256;
     if (nb > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
     if (epi) {

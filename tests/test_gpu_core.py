@@ -367,12 +367,18 @@ def _rand_u4(rng, n, density=0.15):
     return U
 
 
-@pytest.mark.parametrize("n,nu,nd,seed", [(6, 3, 2, 0), (6, 3, 3, 1), (7, 2, 4, 2)])
-def test_hubbard_4index_u_vs_oracle(ctx, n, nu, nd, seed):
+@pytest.mark.parametrize("n,nu,nd,seed,pull", [(6, 3, 2, 0, False), (6, 3, 3, 1, False),
+                                               (7, 2, 4, 2, False), (6, 3, 2, 0, True),
+                                               (7, 2, 4, 2, True)])
+def test_hubbard_4index_u_vs_oracle(ctx, n, nu, nd, seed, pull, monkeypatch):
     """is_Uloc = 3: H*v (FP64 and FP32) and DBL_OCC against the oracle's HPROD /
-    TWO_RDM restatements, and the Lanczos ground energy against dense ED."""
+    TWO_RDM restatements, and the Lanczos ground energy against dense ED; both the
+    row-staged kernel and the one-lane-per-target kernel (PHUB_UL_PULL=1, the form
+    used when the rows do not fit in LDS)."""
     from pyhub_amd import _lib as L
     from pyhub_amd.engine import HubbardOp
+    if pull:
+        monkeypatch.setenv("PHUB_UL_PULL", "1")
     rng = np.random.default_rng(80 + seed)
     T, _ = rand_cluster(rng, n)
     U4 = _rand_u4(rng, n)
