@@ -272,7 +272,9 @@ __global__ void k_hop_pack(const int32_t *__restrict__ idx, const double *__rest
         if (k < c) {
             int q = 0;
             while (q < ncls - 1 && cls[q] != c64[o]) ++q;
-            e = ((uint32_t)idx[o] << 11) | (uint32_t)q;   // (idx * 8) << 8: LDS byte offset
+            // (idx * 8) << 11 | q * 8: the LDS byte offset of column idx (8 bytes per
+            // column) and of the coefficient slot q (8 bytes per slot)
+            e = ((uint32_t)idx[o] << 14) | ((uint32_t)q << 3);
         }
         packed[o] = e;
     }
@@ -379,7 +381,7 @@ int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor
             PHUB_HIP(c, hipMalloc(&out->cls32, cls.size() * sizeof(float)));
             PHUB_HIP(c, hipMemcpy(out->cls64, cls.data(), cls.size() * sizeof(double), hipMemcpyHostToDevice));
             PHUB_HIP(c, hipMemcpy(out->cls32, cls32.data(), cls.size() * sizeof(float), hipMemcpyHostToDevice));
-            if (kmajor && dim < (1ll << 21)) {   // the byte offset idx*8 must fit 24 bits
+            if (kmajor && dim < (1ll << 18)) {   // the byte offset idx*8 must fit 21 bits
                 PHUB_HIP(c, hipMalloc(&out->packed, (size_t)dim * mh * sizeof(uint32_t)));
                 hipLaunchKernelGGL(k_hop_pack, dim3(nblk(dim, 256)), dim3(256), 0, c->stream,
                                    out->idx, out->c64, out->cnt, dim, mh, out->cls64, out->ncls,

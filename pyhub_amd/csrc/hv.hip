@@ -146,6 +146,8 @@ __global__ __launch_bounds__(256) void k_hv_hub(HubArgs<T> a, const T *__restric
 //            out = scale * (out + sum_dn) - beta * prev, dot partials.
 // ---------------------------------------------------------------------------
 constexpr int UP_THREADS = 1024;
+constexpr int UP_CLS_BYTES = 256 * 8;   // coefficient slots ahead of the rows in LDS
+constexpr int UP_LDS_MAX = 150 * 1024;  // coefficient slots + staged rows
 constexpr int UP_UNROLL = 8;    // independent hop-entry loads in flight per lane
 
 template <typename T, int R>
@@ -154,18 +156,21 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
                                                       const T *__restrict__ cls, int ncls,
                                                       const T *__restrict__ x,
                                                       T *__restrict__ out) {
-    // rows interleaved in LDS: element u of row r at rows[u * R + r], so one gather
-    // of column j fetches all R rows (a single 64-bit LDS read for two FP32 rows)
-    extern __shared__ double smem_d[];
-    T *rows = reinterpret_cast<T *>(smem_d);
-    const int64_t nu = a.dim_up;
-    T *cl = rows + R * nu;
+    // LDS: 256 coefficient slots of 8 bytes (UP_CLS_BYTES), then the R rows
+    // interleaved: element u of row r at rows[u * R + r], so one gather of column j
+    // fetches all R rows (a single 64-bit LDS read for two FP32 rows)
+    // a static array (not dynamic shared memory): its LDS address is a compile-time
+    // constant, so both decoded offsets go straight into the ds_read address
+    __shared__ double smem_up[UP_LDS_MAX / 8];
+    char *lds = reinterpret_cast<char *>(smem_up);
+    T *rows = reinterpret_cast<T *>(lds + UP_CLS_BYTES);
+    const int nu = (int)a.dim_up;
     const int64_t d0 = (int64_t)blockIdx.x * R;
-    for (int64_t u = threadIdx.x; u < nu; u += UP_THREADS) {
+    for (int u = threadIdx.x; u < nu; u += UP_THREADS) {
         T v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            v[r] = (d0 + r < a.dim_dn) ? __builtin_nontemporal_load(x + (d0 + r) * nu + u) : (T)0;
+            v[r] = (d0 + r < a.dim_dn) ? __builtin_nontemporal_load(x + (d0 + r) * (int64_t)nu + u) : (T)0;
         if constexpr (R == 2 && sizeof(T) == 4) {
             reinterpret_cast<float2 *>(rows)[u] = make_float2(v[0], v[1]);
         } else {
@@ -173,12 +178,14 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
             for (int r = 0; r < R; ++r) rows[u * R + r] = v[r];
         }
     }
-    for (int i = threadIdx.x; i < ncls; i += UP_THREADS) cl[i] = cls[i];
+    for (int i = threadIdx.x; i < ncls; i += UP_THREADS) *reinterpret_cast<T *>(lds + 8 * i) = cls[i];
     __syncthreads();
-    // one column of the R staged rows is 8 bytes (two FP32 or one FP64); the packed
-    // entries carry that byte offset directly: e = (j * 8) << 8 | class
+    // one column of the R staged rows is 8 bytes (two FP32 or one FP64); a packed
+    // entry carries both LDS byte offsets: e = (j * 8) << 11 | class * 8, so the
+    // decode is one shift and one mask
     static_assert(R * sizeof(T) == 8, "8-byte LDS columns");
-    const char *rowb = reinterpret_cast<const char *>(rows);
+    const char *rowb = lds + UP_CLS_BYTES;
+
     auto gather = [&](uint32_t off, T *v) {
         if constexpr (R == 2) {
             const float2 p = *reinterpret_cast<const float2 *>(rowb + off);
@@ -196,7 +203,7 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
         xd[r] = ok ? a.md[d0 + r] : 0u;
         ed[r] = ok ? a.ed[d0 + r] : 0.0;
     }
-    for (int64_t u = threadIdx.x; u < nu; u += UP_THREADS) {
+    for (int u = threadIdx.x; u < nu; u += UP_THREADS) {
         const uint32_t xu = a.mu[u];
         const double eu = a.eu[u];
         T acc[R], self[R];
@@ -218,32 +225,35 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
             acc[r] = (T)dg * self[r];
         }
         const int cnt = a.cu[u];
+        // row k of the k-major entry table is a wave-uniform (SGPR) base and the
+        // lane's column a 32-bit byte offset: saddr loads, no per-entry address VALU
+        const uint32_t uoff = (uint32_t)u * 4u;
         int k = 0;
         for (; k + UP_UNROLL <= cnt; k += UP_UNROLL) {
             uint32_t e[UP_UNROLL];
-            // the entry row (k-major) is a wave-uniform base; u the per-lane offset
 #pragma unroll
-            for (int q = 0; q < UP_UNROLL; ++q) e[q] = (packed + (size_t)(k + q) * nu)[u];
+            for (int q = 0; q < UP_UNROLL; ++q)
+                e[q] = *(const uint32_t *)((const char *)(packed + (size_t)(k + q) * nu) + uoff);
 #pragma unroll
             for (int q = 0; q < UP_UNROLL; ++q) {
-                const T c = cl[e[q] & 255u];
+                const T c = *reinterpret_cast<const T *>(lds + (e[q] & 0x7f8u));
                 T v[R];
-                gather(e[q] >> 8, v);
+                gather(e[q] >> 11, v);
 #pragma unroll
                 for (int r = 0; r < R; ++r) acc[r] += c * v[r];
             }
         }
         for (; k < cnt; ++k) {
-            const uint32_t e = (packed + (size_t)k * nu)[u];
-            const T c = cl[e & 255u];
+            const uint32_t e = *(const uint32_t *)((const char *)(packed + (size_t)k * nu) + uoff);
+            const T c = *reinterpret_cast<const T *>(lds + (e & 0x7f8u));
             T v[R];
-            gather(e >> 8, v);
+            gather(e >> 11, v);
 #pragma unroll
             for (int r = 0; r < R; ++r) acc[r] += c * v[r];
         }
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            if (d0 + r < a.dim_dn) __builtin_nontemporal_store(acc[r], out + (d0 + r) * nu + u);
+            if (d0 + r < a.dim_dn) __builtin_nontemporal_store(acc[r], out + (d0 + r) * (int64_t)nu + u);
     }
 }
 
@@ -283,6 +293,71 @@ __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict
         int64_t cj[VW];
 #pragma unroll
         for (int j = 0; j < VW; ++j) cj[j] = ok[j] ? col0 + j * LPR : col0;
+        if constexpr (RPW == 1) {
+            // one row per wave: its hop list is wave-uniform, so the entries, the
+            // coefficients and each source row's base address live in SGPRs (scalar
+            // loads through the constant address space; the tables are read-only
+            // here) and a gather is a saddr load with the lane's 32-bit byte offset
+            using ci32 = const __attribute__((address_space(4))) int32_t;
+            using cu32 = const __attribute__((address_space(4))) uint32_t;
+            using cT = const __attribute__((address_space(4))) T;
+            ci32 *rp = (ci32 *)a.rp_dn;
+            cu32 *csr = (cu32 *)a.csr_dn;
+            cT *cl = (cT *)a.cls_dn;
+            const int wv = __builtin_amdgcn_readfirstlane(wave);
+            uint32_t cb[VW];
+#pragma unroll
+            for (int j = 0; j < VW; ++j) cb[j] = (uint32_t)cj[j] * (uint32_t)sizeof(T);
+            for (int rr = wv; rr < ROWS; rr += 4) {
+                const int64_t d = (int64_t)rb * ROWS + rr;
+                if (d >= a.dim_dn) break;
+                const int32_t r0 = rp[d], r1 = rp[d + 1];
+                T sacc[VW];
+#pragma unroll
+                for (int j = 0; j < VW; ++j) sacc[j] = (T)0;
+                int32_t k = r0;
+                for (; k + UN <= r1; k += UN) {
+                    uint32_t e[UN];
+#pragma unroll
+                    for (int q = 0; q < UN; ++q) e[q] = csr[k + q];
+                    T v[UN][VW];
+#pragma unroll
+                    for (int q = 0; q < UN; ++q) {
+                        const char *row = (const char *)(x + (int64_t)(e[q] >> 8) * nu);
+#pragma unroll
+                        for (int j = 0; j < VW; ++j) v[q][j] = *(const T *)(row + cb[j]);
+                    }
+#pragma unroll
+                    for (int q = 0; q < UN; ++q) {
+                        const T c = cl[e[q] & 255u];
+#pragma unroll
+                        for (int j = 0; j < VW; ++j) sacc[j] += c * v[q][j];
+                    }
+                }
+                for (; k < r1; ++k) {
+                    const uint32_t e = csr[k];
+                    const char *row = (const char *)(x + (int64_t)(e >> 8) * nu);
+                    const T c = cl[e & 255u];
+#pragma unroll
+                    for (int j = 0; j < VW; ++j) sacc[j] += c * *(const T *)(row + cb[j]);
+                }
+#pragma unroll
+                for (int j = 0; j < VW; ++j) {
+                    if (!ok[j]) continue;
+                    const size_t o = (size_t)d * nu + cj[j];
+                    if (EPI) {
+                        double val = scale * ((double)__builtin_nontemporal_load(out + o) + (double)sacc[j]);
+                        if (prev) val -= beta * (double)__builtin_nontemporal_load(prev + o);
+                        const T vt = (T)val;
+                        __builtin_nontemporal_store(vt, out + o);
+                        s_xo += scale * (double)x[o] * (double)vt;
+                        s_oo += (double)vt * (double)vt;
+                    } else {
+                        __builtin_nontemporal_store(__builtin_nontemporal_load(out + o) + sacc[j], out + o);
+                    }
+                }
+            }
+        } else
         for (int rr = wave * RPW + sub; rr < ROWS; rr += 4 * RPW) {
             const int64_t d = (int64_t)rb * ROWS + rr;
             if (d >= a.dim_dn) break;
@@ -1057,16 +1132,9 @@ static int hv_launch_main(phub_ctx *c, const phub_op *op, const T *x, T *out, do
     if (op->kind == 0 && op->up.packed && op->dn.csr) {
         // two-kernel path: R rows of the state in LDS for the up hops
         constexpr int R = sizeof(T) == 4 ? 2 : 1;
-        const size_t lds = ((size_t)R * op->dim_up + op->up.ncls) * sizeof(T);
-        if (lds <= 150 * 1024) {
+        const size_t lds = 0;   // k_hv_up's LDS is static (UP_LDS_MAX)
+        if (UP_CLS_BYTES + (size_t)R * op->dim_up * sizeof(T) <= UP_LDS_MAX) {
             HubArgs<T> a = hub_args<T>(op);
-            static bool attr_set[2] = {false, false};
-            if (!attr_set[sizeof(T) == 8]) {
-                PHUB_HIP(c, hipFuncSetAttribute((const void *)k_hv_up<T, R>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                160 * 1024));
-                attr_set[sizeof(T) == 8] = true;
-            }
             const T *cls = (sizeof(T) == 8) ? (const T *)op->up.cls64 : (const T *)op->up.cls32;
             const unsigned g1 = (unsigned)((op->dim_dn + R - 1) / R);
             hipLaunchKernelGGL((k_hv_up<T, R>), dim3(g1), dim3(UP_THREADS), lds, c->stream, a,
