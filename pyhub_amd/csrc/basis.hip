@@ -99,26 +99,39 @@ int phub_ensure_red(phub_ctx *c, size_t nd) {
 // reductions
 // ---------------------------------------------------------------------------
 // partial layout: [block][comp]; one workgroup per component sums all blocks.
-__global__ void k_reduce_partials(const double *__restrict__ part, int nblocks, int ncomp,
-                                  double *__restrict__ out) {
-    __shared__ double sh[256];
-    int c = blockIdx.x;
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += part[(size_t)b * ncomp + c];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
+// one block of 1024 threads per component; four independent partial sums per
+// thread keep four loads in flight (the partial arrays of the H*v grids hold ~1e5
+// entries per component)
+__global__ __launch_bounds__(1024) void k_reduce_partials(const double *__restrict__ part,
+                                                          int nblocks, int ncomp,
+                                                          double *__restrict__ out) {
+    __shared__ double sh[16];
+    const int c = blockIdx.x;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int b = threadIdx.x;
+    for (; b + 3 * 1024 < nblocks; b += 4 * 1024) {
+        s0 += part[(size_t)b * ncomp + c];
+        s1 += part[(size_t)(b + 1024) * ncomp + c];
+        s2 += part[(size_t)(b + 2048) * ncomp + c];
+        s3 += part[(size_t)(b + 3072) * ncomp + c];
     }
-    if (threadIdx.x == 0) out[c] = sh[0];
+    for (; b < nblocks; b += 1024) s0 += part[(size_t)b * ncomp + c];
+    double s = (s0 + s1) + (s2 + s3);
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < 16; ++w) t += sh[w];
+        out[c] = t;
+    }
 }
 
 int phub_reduce_partials(phub_ctx *c, int nblocks, int ncomp, double *host_out) {
     if ((size_t)nblocks * ncomp + ncomp > c->red_cap || (size_t)ncomp > c->h_red_cap)
         return phub_fail(c, PHUB_ERR_ARG, "reduction buffer too small");
     double *dst = c->d_red + (size_t)nblocks * ncomp;
-    hipLaunchKernelGGL(k_reduce_partials, dim3(ncomp), dim3(256), 0, c->stream, c->d_red,
+    hipLaunchKernelGGL(k_reduce_partials, dim3(ncomp), dim3(1024), 0, c->stream, c->d_red,
                        nblocks, ncomp, dst);
     PHUB_HIP(c, hipGetLastError());
     PHUB_HIP(c, hipMemcpyAsync(c->h_red, dst, ncomp * sizeof(double), hipMemcpyDeviceToHost,
