@@ -289,6 +289,39 @@ int phub_adjoint_minres(phub_ctx *ctx, const phub_op *op, const double *d_psi, d
                         const double *d_b, double *d_x, int max_iter, double tol,
                         int *iters, double *rel_res);
 
+/* ---- generic operator strings (operators.x) ---------------------------------------
+ * Replaces compute_operators.f90:1-72 (the driver) and the elementary routines of
+ * operators.f90 (C_DAGGER_C :1-55, C_DAGGER_C_UNRES :58-144, NI :148-172, NIM1
+ * :196-220, SZ_OPE :224-245, SISJ :303-394, C_DAGGER :397-446, C_ :450-499).
+ * A phub_fock is the product basis of the concatenated fixed-N sectors of each
+ * species (compute_operators.f90:10-26; state index d * nsup + u).  A program is a
+ * sum of products: term t has coefficient coef[t] and term_len[t] factors, each
+ * five ints (code, site1, spin1, site2, spin2) with the reference's OPE codes
+ * (1 c^dag c same spin, 2 n, 3 id, 4 1 - n, 5 sz, 6 S+, 7 S-, 8 S.S, 9 c^dag c
+ * spin-flip, 10 c^dag, 11 c), sites 1-based, spin 1 = up; the factors of a product
+ * act in list order (the first one first), as in the reference. */
+typedef struct phub_fock phub_fock;
+typedef struct phub_opprog phub_opprog;
+int phub_fock_create(phub_ctx *ctx, int n_sites, const int *n_up, int n_up_count,
+                     const int *n_dn, int n_dn_count, phub_fock **out);
+void phub_fock_destroy(phub_fock *f);
+int phub_fock_dims(const phub_fock *f, int64_t *nsup, int64_t *nsdn);
+int phub_fock_masks(phub_ctx *ctx, const phub_fock *f, uint64_t *d_up, uint64_t *d_dn);
+int phub_opprog_create(phub_ctx *ctx, int n_terms, const double *coef, const int *term_len,
+                       const int *elems, phub_opprog **out);
+void phub_opprog_destroy(phub_opprog *p);
+int phub_opprog_terms(const phub_opprog *p);
+/* d_out[v] = O d_in[v] for nvec vectors of nsup*nsdn (vector v at offset v*dim). */
+int phub_operator_apply(phub_ctx *ctx, const phub_fock *f, const phub_opprog *p,
+                        const double *d_in, double *d_out, int nvec);
+/* d_diag[i] = <e_i|O|e_i> for every basis element (the basis-element mode,
+ * operators_basis.f90, used by the comparisons and bit_repr). */
+int phub_operator_diag(phub_ctx *ctx, const phub_fock *f, const phub_opprog *p,
+                       double *d_diag);
+/* d_mat[i * dim + j] = (O e_i)[j], the reference's to_matrix layout (at most 16 GB). */
+int phub_operator_matrix(phub_ctx *ctx, const phub_fock *f, const phub_opprog *p,
+                         double *d_mat);
+
 #ifdef __cplusplus
 }
 #endif

@@ -485,3 +485,284 @@ def lieb_wu_ring(L, nup, ndn, U, t=1.0, I=None, Jq=None, tol=1e-14):
     x, info, ier, msg = fsolve(eqs, np.concatenate([k0, lam0]), xtol=tol, full_output=True)
     res = np.max(np.abs(eqs(x))) if len(x) else 0.0
     return float(-2 * t * np.sum(np.cos(x[:N]))), float(res)
+
+
+# ---------------------------------------------------------------------------
+# generic operator strings (operators.x), restated in Python loops
+# ---------------------------------------------------------------------------
+def _anticom(n, pos, m):
+    """ANTICOM (funcmod.f90:29-41): (-1)^(occupied sites 1..pos-1), site J = bit n-J."""
+    s = 1.0
+    for i2 in range(1, pos):
+        if (m >> (n - i2)) & 1:
+            s = -s
+    return s
+
+
+def _bit(n, pos):
+    return 1 << (n - pos)
+
+
+class OperatorOracle:
+    """Restates operators.x on a product basis of fixed-N sectors.
+
+    bup / bdn are the concatenated sector lists in the given order
+    (compute_operators.f90:10-26); the state index is d * nsup + u.  Terms are
+    (coeff, [(code, site1, spin1, site2, spin2), ...]) with the OPE codes of
+    compute_operators.f90:36-63; factors act in list order (first one first).
+
+    apply() follows the vector path: C_DAGGER_C (operators.f90:1-55),
+    C_DAGGER_C_UNRES (:58-144), NI (:148-172), NIM1 (:196-220), SZ_OPE (:224-245),
+    SISJ (:303-394), C_DAGGER (:397-446), C_ (:450-499).  matrix() follows the
+    basis-element path OPERATOR_BASIS (operators_basis.f90:1-56) with
+    C_DAGGER_C_BASIS (:59-100), NI/NIM1_BASIS (:103-130), SZ_BASIS (:134-143),
+    SISJ_BASIS (:147-186), C_DAGGER_C_UNRES_BASIS, C_DAGGER_BASIS, C_BASIS.
+    Three places differ from the Fortran text, all stated: SISJ with K = J reads
+    the current up state (the Fortran reads BUP(K), the K-th up state); a
+    basis-element path whose next state is missing from the lists is dropped (the
+    Fortran keeps the previous position); and SISJ_BASIS's second try restarts
+    from the state the factor started on (the Fortran restarts from the product's
+    first basis element, the same state when S.S is the first factor)."""
+
+    def __init__(self, n, nup, ndn):
+        self.n = n
+        self.bup = np.concatenate([hilbert(n, k) for k in dict.fromkeys(nup)]).astype(np.int64)
+        self.bdn = np.concatenate([hilbert(n, k) for k in dict.fromkeys(ndn)]).astype(np.int64)
+        self.nsup, self.nsdn = len(self.bup), len(self.bdn)
+        self.dim = self.nsup * self.nsdn
+        self.pu = {int(m): i for i, m in enumerate(self.bup)}
+        self.pd = {int(m): i for i, m in enumerate(self.bdn)}
+
+    # -- vector path ---------------------------------------------------------------
+    def _cdc(self, J, K, spin, v):
+        n, out = self.n, np.zeros_like(v)
+        if spin == 1:
+            for i, x in enumerate(self.bup):
+                x = int(x)
+                a, b = _bit(n, K), _bit(n, J)
+                new = 0
+                if x & a and not x & b:
+                    new = x - a + b
+                if x & a and J == K:
+                    new = x
+                if new > 0 and new in self.pu:
+                    pos = self.pu[new]
+                    ac = _anticom(n, K, x) * _anticom(n, J, new)
+                    for l1 in range(self.nsdn):
+                        out[l1 * self.nsup + pos] += ac * v[l1 * self.nsup + i]
+        else:
+            for l1, x in enumerate(self.bdn):
+                x = int(x)
+                a, b = _bit(n, K), _bit(n, J)
+                new = 0
+                if x & a and not x & b:
+                    new = x - a + b
+                if x & a and J == K:
+                    new = x
+                if new > 0 and new in self.pd:
+                    pos = self.pd[new]
+                    ac = _anticom(n, K, x) * _anticom(n, J, new)
+                    for i in range(self.nsup):
+                        out[pos * self.nsup + i] += ac * v[l1 * self.nsup + i]
+        return out
+
+    def _unres(self, J, K, spink, v):
+        n, out = self.n, np.zeros_like(v)
+        if spink == 1:   # remove up at K, add down at J
+            for i, x in enumerate(self.bup):
+                x = int(x)
+                if not x & _bit(n, K):
+                    continue
+                new = x - _bit(n, K)
+                if new not in self.pu:
+                    continue
+                pos, ac = self.pu[new], _anticom(n, K, new)
+                for l1, y in enumerate(self.bdn):
+                    y = int(y)
+                    if y & _bit(n, J):
+                        continue
+                    new2 = y + _bit(n, J)
+                    if new2 in self.pd:
+                        ac2 = _anticom(n, J, new2)
+                        out[self.pd[new2] * self.nsup + pos] += ac * ac2 * v[l1 * self.nsup + i]
+        else:            # remove down at K, add up at J
+            for i, x in enumerate(self.bdn):
+                x = int(x)
+                if not x & _bit(n, K):
+                    continue
+                new = x - _bit(n, K)
+                if new not in self.pd:
+                    continue
+                pos, ac = self.pd[new], _anticom(n, K, new)
+                for l1, y in enumerate(self.bup):
+                    y = int(y)
+                    if y & _bit(n, J):
+                        continue
+                    new2 = y + _bit(n, J)
+                    if new2 in self.pu:
+                        ac2 = _anticom(n, J, new2)
+                        out[pos * self.nsup + self.pu[new2]] += ac * ac2 * v[i * self.nsup + l1]
+        return out
+
+    def _diag(self, fn, v):
+        out = np.zeros_like(v)
+        for d, y in enumerate(self.bdn):
+            for u, x in enumerate(self.bup):
+                out[d * self.nsup + u] = fn(int(x), int(y)) * v[d * self.nsup + u]
+        return out
+
+    def _sz1(self, J, x, y):
+        n = self.n
+        return (0.5 if x & _bit(n, J) else 0.0) - (0.5 if y & _bit(n, J) else 0.0)
+
+    def _create_annihilate(self, J, spin, v, create):
+        n, out = self.n, np.zeros_like(v)
+        b = _bit(n, J)
+        if spin == 1:
+            for i, x in enumerate(self.bup):
+                x = int(x)
+                if bool(x & b) == create:
+                    continue
+                new = x + b if create else x - b
+                if new in self.pu:
+                    ac = _anticom(n, J, x)   # ANTICOM(NORB, 0, .) = 1 for the down part
+                    for l2 in range(self.nsdn):
+                        out[l2 * self.nsup + self.pu[new]] += ac * v[l2 * self.nsup + i]
+        else:
+            for i, x in enumerate(self.bdn):
+                x = int(x)
+                if bool(x & b) == create:
+                    continue
+                new = x + b if create else x - b
+                if new in self.pd:
+                    ac = _anticom(n, J, x)
+                    for l1 in range(self.nsup):
+                        out[self.pd[new] * self.nsup + l1] += ac * v[i * self.nsup + l1]
+        return out
+
+    def _sisj(self, K, J, v):
+        n, out = self.n, np.zeros_like(v)
+        if K == J:
+            b = _bit(n, J)
+            out += self._diag(lambda x, y: 0.5 if bool(x & b) != bool(y & b) else 0.0, v)
+        else:
+            # up J -> K with down K -> J, then down J -> K with up K -> J, each times -1/2
+            out -= 0.5 * self._cdc(J, K, 2, self._cdc(K, J, 1, v))
+            out -= 0.5 * self._cdc(J, K, 1, self._cdc(K, J, 2, v))
+        out += self._diag(lambda x, y: self._sz1(J, x, y) * self._sz1(K, x, y), v)
+        return out
+
+    def _factor(self, f, v):
+        code, s1, p1, s2, p2 = f
+        n = self.n
+        if code == 1:
+            return self._cdc(s1, s2, p1, v)
+        if code == 2:
+            return self._diag(lambda x, y: 1.0 if ((x if p1 == 1 else y) & _bit(n, s1)) else 0.0, v)
+        if code == 3:
+            return v.copy()
+        if code == 4:
+            return self._diag(lambda x, y: 0.0 if ((x if p1 == 1 else y) & _bit(n, s1)) else 1.0, v)
+        if code == 5:   # SZ_OPE: (1 - n_up)/2 - (1 - n_dn)/2
+            return self._diag(lambda x, y: (0.0 if x & _bit(n, s1) else 0.5)
+                              - (0.0 if y & _bit(n, s1) else 0.5), v)
+        if code == 6:
+            return self._unres(s1, s1, 2, v)
+        if code == 7:
+            return self._unres(s1, s1, 1, v)
+        if code == 8:
+            return self._sisj(s1, s2, v)
+        if code == 9:
+            return self._unres(s1, s2, p2, v)
+        if code == 10:
+            return self._create_annihilate(s1, p1, v, True)
+        if code == 11:
+            return self._create_annihilate(s1, p1, v, False)
+        raise ValueError(code)
+
+    def apply(self, terms, v):
+        v = np.asarray(v, dtype=np.float64)
+        out = np.zeros(self.dim)
+        for coeff, factors in terms:
+            if not abs(coeff) > 1e-14:
+                continue
+            cur = v.copy()
+            for f in factors:
+                cur = self._factor(f, cur)
+            out += coeff * cur
+        return out
+
+    # -- basis-element path ----------------------------------------------------------
+    def _walk(self, f, x, y, val):
+        """One factor on the single state (x, y) with value val (operators_basis.f90)."""
+        code, s1, p1, s2, p2 = f
+        n = self.n
+        if code == 1:
+            m = x if p1 == 1 else y
+            a, b = _bit(n, s2), _bit(n, s1)
+            new = 0
+            if m & a and not m & b:
+                new = m - a + b
+            if m & a and s1 == s2:
+                new = m
+            if new == 0:
+                return None
+            val *= _anticom(n, s2, m) * _anticom(n, s1, new)
+            return (new, y, val) if p1 == 1 else (x, new, val)
+        if code == 2:
+            return (x, y, val) if ((x if p1 == 1 else y) & _bit(n, s1)) else None
+        if code == 3:
+            return x, y, val
+        if code == 4:
+            return None if ((x if p1 == 1 else y) & _bit(n, s1)) else (x, y, val)
+        if code == 5:   # SZ_BASIS: funcmod's SZ
+            z = self._sz1(s1, x, y)
+            return (x, y, val * z) if z != 0.0 else None
+        if code in (6, 7, 9):
+            J, K = (s1, s1) if code in (6, 7) else (s1, s2)
+            spk = 2 if code == 6 else (1 if code == 7 else p2)
+            rem, add = (x, y) if spk == 1 else (y, x)
+            if not rem & _bit(n, K) or add & _bit(n, J):
+                return None
+            val *= _anticom(n, K, rem) * _anticom(n, J, add)
+            rem, add = rem - _bit(n, K), add + _bit(n, J)
+            return (rem, add, val) if spk == 1 else (add, rem, val)
+        if code == 8:
+            K, J = s1, s2
+            if K == J:
+                b = _bit(n, J)
+                return (x, y, val / 2) if bool(x & b) != bool(y & b) else None
+            r = self._walk((1, K, 1, J, 1), x, y, val)
+            r = self._walk((1, J, 2, K, 2), *r) if r else None
+            if r is None:
+                r = self._walk((1, K, 2, J, 2), x, y, val)
+                r = self._walk((1, J, 1, K, 1), *r) if r else None
+            return (r[0], r[1], -r[2] / 2) if r else None
+        if code in (10, 11):
+            m = x if p1 == 1 else y
+            b = _bit(n, s1)
+            if bool(m & b) == (code == 10):
+                return None
+            val *= _anticom(n, s1, m)
+            m = m + b if code == 10 else m - b
+            return (m, y, val) if p1 == 1 else (x, m, val)
+        raise ValueError(code)
+
+    def matrix(self, terms):
+        """Row i = O e_i through the basis-element path (to_matrix)."""
+        M = np.zeros((self.dim, self.dim))
+        for d, y0 in enumerate(self.bdn):
+            for u, x0 in enumerate(self.bup):
+                i = d * self.nsup + u
+                for coeff, factors in terms:
+                    if not abs(coeff) > 1e-14:
+                        continue
+                    st = (int(x0), int(y0), 1.0)
+                    for f in factors:
+                        st = self._walk(f, *st)
+                        if st is None or st[0] not in self.pu or st[1] not in self.pd:
+                            st = None
+                            break
+                    if st is not None:
+                        M[i, self.pd[st[1]] * self.nsup + self.pu[st[0]]] += coeff * st[2]
+        return M

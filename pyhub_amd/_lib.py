@@ -103,6 +103,18 @@ SIGNATURES = {
     "phub_hf_grad_heisenberg": (C.c_int, [_vp, _vp, _vp, _dp]),
     "phub_adjoint_minres": (C.c_int, [_vp, _vp, _vp, C.c_double, _vp, _vp, C.c_int, C.c_double,
                                       C.POINTER(C.c_int), _dp]),
+    "phub_fock_create": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
+                                   C.c_int, C.POINTER(_vp)]),
+    "phub_fock_destroy": (None, [_vp]),
+    "phub_fock_dims": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "phub_fock_masks": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "phub_opprog_create": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(_vp)]),
+    "phub_opprog_destroy": (None, [_vp]),
+    "phub_opprog_terms": (C.c_int, [_vp]),
+    "phub_operator_apply": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int]),
+    "phub_operator_diag": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "phub_operator_matrix": (C.c_int, [_vp, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -8,7 +8,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libphub.so")
-SOURCES = ["basis.hip", "ops.hip", "hv.hip", "krylov.hip", "rdm.hip", "gf.hip", "batch.hip"]
+SOURCES = ["basis.hip", "ops.hip", "hv.hip", "krylov.hip", "rdm.hip", "gf.hip", "batch.hip",
+           "operators.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -188,6 +188,25 @@ def test_self_energy_dimer_closed_form(ctx):
     assert np.max(np.abs(f0.self_energy(0.3, eta=0.1)["up"])) < 1e-10
 
 
+def test_empty_species_and_green_functions(ctx):
+    """No up electron (N_up = 0): the lesser up Green's function has no pole (an
+    empty N-1 sector), the greater one carries the whole sum rule; E0 and the
+    1-RDM match dense ED of the oracle's matrix."""
+    from pyhub_amd.solver.fermi_hubbard import FermiHubbard
+    T = O.ring_T(5)
+    fh = FermiHubbard(5, 0, 2, T, U=3.0)
+    fh.kernel(compute_spgf=True)
+    m = O.HubbardModel(5, 0, 2, T, 3.0)
+    ev, V = np.linalg.eigh(m.dense())
+    assert abs(fh.e0 - ev[0]) < 1e-12
+    assert np.max(np.abs(fh.one_rdm["up"])) < 1e-14
+    assert abs(np.trace(fh.one_rdm["down"]) - 2.0) < 1e-12
+    assert fh.nb_poles[0, 0] == 0
+    for i in range(5):
+        w = fh.gf["up"][i, i].pole["weights"]
+        assert abs(np.sum(w) - 1.0) < 1e-10
+
+
 def test_fermi_hubbard_4index_u(ctx):
     """FermiHubbard with a 4-index U (is_Uloc = 3): energies against dense ED of the
     oracle operator, two_rdm against the oracle's DBL_OCC of the same psi, ni equal to

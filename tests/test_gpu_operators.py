@@ -1,0 +1,139 @@
+"""GPU tests of the generic operator engine (Operators, tools/operators.py,
+tools/models.py) against the operators.x restatement in the oracle, modelled on the
+reference's examples 02 and 07."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_terms(rng, n, nterms, maxlen):
+    """Random products of every factor code (sites 1..n, both spins)."""
+    terms = []
+    for _ in range(nterms):
+        fs = []
+        for _ in range(rng.integers(1, maxlen + 1)):
+            code = int(rng.choice([1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11]))
+            s1, s2 = (int(x) for x in rng.integers(1, n + 1, size=2))
+            p1, p2 = (int(x) for x in rng.integers(1, 3, size=2))
+            if code == 9 and p1 == p2:
+                p1 = 3 - p2
+            fs.append((code, s1, p1, s2, p2))
+        terms.append((float(rng.normal()), fs))
+    return terms
+
+
+def _program(ctx, n, nup, ndn, terms):
+    from pyhub_amd.core._operators import _Compiled
+    return _Compiled(ctx, n, nup, ndn, [c for c, _ in terms], [f for _, f in terms])
+
+
+@pytest.mark.parametrize("n,nup,ndn,seed", [(3, [1], [2], 0), (4, [1, 2], [2, 1], 1),
+                                            (4, [0, 1, 2, 3, 4], [0, 1, 2, 3, 4], 2),
+                                            (5, [2], [2, 3], 3)])
+def test_operator_programs_vs_oracle(ctx, n, nup, ndn, seed):
+    """Random sums of products of all eleven factor codes on single- and
+    multi-sector bases: O v (vector path) and the to_matrix rows / diagonal
+    (basis-element path) equal the oracle's restatement to 1e-12."""
+    rng = np.random.default_rng(seed)
+    terms = _random_terms(rng, n, 24, 4)
+    # one product of S.S with itself and a zero-coefficient term (skipped)
+    terms.append((0.7, [(8, 1, 0, 2, 0), (8, 2, 0, 2, 0)]))
+    terms.append((0.0, [(2, 1, 1, 0, 0)]))
+    oc = O.OperatorOracle(n, nup, ndn)
+    prog = _program(ctx, n, nup, ndn, terms)
+    assert prog.nstates == oc.dim
+    V = rng.normal(size=(3, oc.dim))
+    out = prog.apply(torch.as_tensor(V)).cpu().numpy()
+    for k in range(3):
+        ref = oc.apply(terms, V[k])
+        assert np.max(np.abs(out[k] - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+    M = oc.matrix(terms)
+    assert np.max(np.abs(prog.matrix() - M)) < 1e-12 * max(1.0, np.max(np.abs(M)))
+    assert np.max(np.abs(prog.diag() - np.diag(M))) < 1e-12 * max(1.0, np.max(np.abs(M)))
+
+
+def test_hubbard_operator_api_vs_solver(ctx):
+    """fermi_hubbard(t, U) as an operator sum (tools/models.py:5-11) on Basis(6,
+    (3, 2)): to_matrix equals the pinned Hubbard matrix, O @ v equals the H*v
+    kernel, the operator's own Lanczos (_operators.py:325-366) reaches the dense
+    ground energy, avg and the complex call follow the reference."""
+    from pyhub_amd.core.basis import Basis
+    from pyhub_amd.engine import HubbardOp
+    from pyhub_amd.tools.models import fermi_hubbard
+    T = O.disordered_ring_T(6)
+    U = 2.5
+    H = fermi_hubbard(T, U)
+    H.set_basis(Basis(6, hilbert=(3, 2)))
+    m = O.HubbardModel(6, 3, 2, T, np.full(6, U))
+    Hd = m.dense()
+    assert np.max(np.abs(H.to_matrix - Hd)) < 1e-12
+    rng = np.random.default_rng(0)
+    v = rng.normal(size=H.nstates)
+    hv = HubbardOp(ctx, 6, 3, 2, T, U).hv(torch.as_tensor(v, device=ctx.device)).cpu().numpy()
+    assert np.max(np.abs(H @ v - hv)) < 1e-12
+    assert abs(H.avg(v) - v @ Hd @ v) < 1e-9
+    w = v + 1j * rng.normal(size=H.nstates)
+    assert np.max(np.abs(H(w) - (Hd @ w.real + 1j * (Hd @ w.imag)))) < 1e-12
+    np.random.seed(4)
+    ek, psi = H.lanczos(maxstep=200, acc_lcz=1e-10)
+    e_ref = np.linalg.eigvalsh(Hd)[0]
+    assert abs(ek[0] - e_ref) < 1e-8
+    assert abs(abs(np.linalg.norm(psi[:, 0])) - 1.0) < 1e-6
+    assert np.max(np.abs(Hd @ psi[:, 0] - ek[0] * psi[:, 0])) < 1e-4
+
+
+def test_counting_operator_and_restriction(ctx):
+    """Example 02 (N on a 2-site (1,1) basis: avg = 2) and the example 07 flow:
+    the comparisons list basis elements up-index-major, and H[idx] acts on the
+    selected elements (its to_matrix is the submatrix)."""
+    from itertools import product
+    from pyhub_amd.core.basis import Basis
+    from pyhub_amd.tools.models import heisenberg
+    from pyhub_amd.tools.operators import n, opesum
+    N = opesum([n((i, s)) for i, s in product(range(2), ("up", "down"))])
+    N.set_basis(Basis(2, hilbert=(1, 1)))
+    v = np.random.default_rng(10).random(N.nstates) - 0.5
+    v /= np.linalg.norm(v)
+    assert abs(N.avg(v) - 2.0) < 1e-12
+    assert np.max(np.abs(N.to_matrix - 2 * np.eye(4))) < 1e-14
+    nb = 4
+    J = np.diag(np.ones(nb - 1), 1) + np.diag(np.ones(nb - 1), -1)
+    J[0, -1] = J[-1, 0] = 1.0
+    B = Basis(nb, hilbert=(2, 2))
+    H = heisenberg(J)
+    H.set_basis(B)
+    N2 = opesum([n((i, "u")) * n((i, "d")) for i in range(nb)])
+    N2.set_basis(B)
+    eq = N2 == 0
+    order = [d * B.nsup + u for u in range(B.nsup) for d in range(B.nsdown)]
+    diag = np.diag(N2.to_matrix)
+    assert eq == [bool(np.isclose(diag[s], 0.0)) for s in order]
+    idx = np.where(eq)[0]
+    Hs = H[idx]
+    assert np.max(np.abs(Hs.to_matrix - H.to_matrix[np.ix_(idx, idx)])) < 1e-14
+    x = np.random.default_rng(1).normal(size=len(idx))
+    full = np.zeros(H.nstates)
+    full[idx] = x
+    assert np.max(np.abs(Hs @ x - (H @ full)[idx])) < 1e-12
+    assert isinstance(H.bit_repr(), str) and H.bit_repr().count("\n") == 8 + H.nstates
+
+
+def test_operator_errors(ctx):
+    """An operator that names a site beyond the basis, an unknown factor string and
+    an application before set_basis are rejected."""
+    from pyhub_amd.core.basis import Basis
+    from pyhub_amd.core._operators import Operators
+    from pyhub_amd.tools.operators import n
+    op = n((5, "up"))
+    with pytest.raises(AttributeError):
+        op(np.zeros(4))
+    with pytest.raises(ValueError):
+        op.set_basis(Basis(3, hilbert=(1, 1)))
+    bad = Operators((0, "u"))
+    bad.str_index = [["foo"]]
+    with pytest.raises((ValueError, NotImplementedError)):
+        bad.set_basis(Basis(2, hilbert=(1, 1)))

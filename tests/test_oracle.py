@@ -239,3 +239,96 @@ def test_library_exports_header_symbols():
     assert not missing, missing
     from pyhub_amd import _lib
     assert names <= set(_lib.SIGNATURES), names - set(_lib.SIGNATURES)
+
+
+# ---------------------------------------------------------------------------
+# generic operator strings (operators.x restatement)
+# ---------------------------------------------------------------------------
+def _hubbard_terms(T, U):
+    n = T.shape[0]
+    terms = [(T[i, j], [(1, i + 1, s, j + 1, s)]) for i in range(n) for j in range(n)
+             for s in (1, 2) if T[i, j] != 0.0]
+    return terms + [(U[i], [(2, i + 1, 1, 0, 0), (2, i + 1, 2, 0, 0)]) for i in range(n)]
+
+
+@pytest.mark.parametrize("n,nu,nd,seed", [(4, 2, 1, 0), (5, 2, 3, 1)])
+def test_operator_oracle_hubbard_equals_pinned_hprod(n, nu, nd, seed):
+    """The operator sum of fermi_hubbard (tools/models.py:5-11) through both paths
+    of the operators.x restatement equals HubbardModel (pinned bit for bit to the
+    reference HPROD): the vector path on random vectors, the basis-element path as
+    the dense matrix."""
+    rng = np.random.default_rng(seed)
+    T = rng.normal(size=(n, n))
+    T = (T + T.T) / 2
+    U = rng.uniform(1, 4, n)
+    m = O.HubbardModel(n, nu, nd, T, U)
+    op = O.OperatorOracle(n, [nu], [nd])
+    terms = _hubbard_terms(T, U)
+    v = rng.normal(size=m.dim)
+    assert np.max(np.abs(op.apply(terms, v) - m.hv(v))) < 1e-13
+    assert np.max(np.abs(op.matrix(terms) - m.dense())) < 1e-13
+
+
+def test_operator_oracle_anticommutation_and_products():
+    """On the Fock basis of 4 sites (every N of both species):
+    - {c_i, c^dag_j} = delta_ij within a species (list order: first factor first);
+    - an up and a down operator commute: the reference's c / c^dag carry no sign
+      from the other species (ANTICOM(NORB, 0, .) = 1, operators.f90:424,477);
+    - c^dag_i c_j (code 1) = [c_j, c^dag_i]; S+ (6) = [c_dn, c^dag_up]; S- (7) =
+      [c_up, c^dag_dn]; the spin-flip c^dag c (9) = [c_K, c^dag_J] of the two spins."""
+    n = 4
+    op = O.OperatorOracle(n, list(range(n + 1)), list(range(n + 1)))
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=op.dim)
+    for s in (1, 2):
+        for i in range(1, n + 1):
+            for j in range(1, n + 1):
+                ac = op.apply([(1.0, [(10, j, s, 0, 0), (11, i, s, 0, 0)]),
+                               (1.0, [(11, i, s, 0, 0), (10, j, s, 0, 0)])], v)
+                assert np.max(np.abs(ac - (v if i == j else 0.0))) < 1e-14, (s, i, j)
+                assert np.max(np.abs(op.apply([(1.0, [(1, i, s, j, s)])], v)
+                                     - op.apply([(1.0, [(11, j, s, 0, 0), (10, i, s, 0, 0)])], v))) < 1e-14
+    for i in range(1, n + 1):
+        for j in range(1, n + 1):
+            cm = op.apply([(1.0, [(10, j, 2, 0, 0), (11, i, 1, 0, 0)]),
+                           (-1.0, [(11, i, 1, 0, 0), (10, j, 2, 0, 0)])], v)
+            assert np.max(np.abs(cm)) < 1e-14
+            for K, J, sk in ((i, j, 1), (i, j, 2)):
+                a = op.apply([(1.0, [(9, J, 3 - sk, K, sk)])], v)
+                b = op.apply([(1.0, [(11, K, sk, 0, 0), (10, J, 3 - sk, 0, 0)])], v)
+                assert np.max(np.abs(a - b)) < 1e-14
+        a6 = op.apply([(1.0, [(6, i, 0, 0, 0)])], v)
+        assert np.max(np.abs(a6 - op.apply([(1.0, [(11, i, 2, 0, 0), (10, i, 1, 0, 0)])], v))) < 1e-14
+        a7 = op.apply([(1.0, [(7, i, 0, 0, 0)])], v)
+        assert np.max(np.abs(a7 - op.apply([(1.0, [(11, i, 1, 0, 0), (10, i, 2, 0, 0)])], v))) < 1e-14
+
+
+def test_operator_oracle_sisj_equals_pinned_jspin():
+    """<psi| sisj(i, j) |psi> with sisj built as in tools/operators.py:36-48 equals
+    the J_SPIN restatement (HubbardModel.sisj, static_rq.f90:323-395) for i != j.
+    For i = j the helper adds n_up (1 - n_dn) / 2 twice (operators.py:39) where
+    S_i^2 needs n_dn (1 - n_up) / 2 once, so its value is
+    <n_up (1 - n_dn) + sz^2>; and the SZ factor on a vector is SZ_OPE's
+    (n_dn - n_up) / 2."""
+    from pyhub_amd.tools.operators import sisj, sz
+    n, nu, nd = 4, 2, 2
+    m = O.HubbardModel(n, nu, nd, O.ring_T(n), np.full(n, 3.0))
+    op = O.OperatorOracle(n, [nu], [nd])
+    rng = np.random.default_rng(5)
+    psi = rng.normal(size=m.dim)
+    S = m.sisj(psi)
+    for i in range(n):
+        for j in range(n):
+            val = psi @ op.apply(sisj((i,), (j,))._terms(), psi)
+            if i != j:
+                assert abs(val - S[i, j]) < 1e-12, (i, j)
+                continue
+            b = n - 1 - i
+            w = np.array([(((int(x) >> b) & 1) * (1 - ((int(y) >> b) & 1))
+                           + 0.25 * (((int(x) >> b) & 1) - ((int(y) >> b) & 1)) ** 2)
+                          for y in op.bdn for x in op.bup])
+            assert abs(val - psi @ (w * psi)) < 1e-12, i
+    szv = op.apply(sz((1,))._terms(), psi)
+    d = np.array([((int(y) >> (n - 2)) & 1) - ((int(x) >> (n - 2)) & 1)
+                  for y in op.bdn for x in op.bup]) / 2.0
+    assert np.max(np.abs(szv - d * psi)) < 1e-15
