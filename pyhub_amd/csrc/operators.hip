@@ -13,6 +13,7 @@
 #include "phub_internal.h"
 
 #include <atomic>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -799,9 +800,11 @@ static OpArgs op_args(phub_ctx *c, const phub_fock *f, const OpProgDev &p) {
 }
 
 // Build (or reuse) the source maps of the vector program on basis f.  They are used
-// when both lists stay below 2^30 elements and the maps within 1 GiB; otherwise the
-// kernel walks the factors itself.
+// when both lists stay below 2^30 elements and the maps within 1 GiB; otherwise (or
+// with PHUB_OP_MAPS=0 in the environment) the kernel walks the factors itself.
 static int op_maps(phub_ctx *c, const phub_fock *f, const phub_opprog *pc, OpArgs &a) {
+    if (const char *env = getenv("PHUB_OP_MAPS"))
+        if (env[0] == '0') return PHUB_OK;
     phub_opprog *p = const_cast<phub_opprog *>(pc);
     const OpProgDev &v = p->vec;
     if (f->nsup >= (1ll << 30) || f->nsdn >= (1ll << 30)) return PHUB_OK;

@@ -56,6 +56,34 @@ def test_operator_programs_vs_oracle(ctx, n, nup, ndn, seed):
     assert np.max(np.abs(prog.diag() - np.diag(M))) < 1e-12 * max(1.0, np.max(np.abs(M)))
 
 
+@pytest.mark.parametrize("maps", ["1", "0"])
+def test_operator_walk_and_map_paths(ctx, monkeypatch, maps):
+    """O v through the per-species source maps (the default) and through the
+    in-kernel factor walk (PHUB_OP_MAPS=0, the path for lists beyond 2^30 elements or
+    maps beyond 1 GiB) on a multi-sector basis: hops, densities, c / c^dag and S.S
+    products (mixed-species factors included), against the oracle to 1e-12; both
+    paths on an odd number of single-species products and a restricted basis."""
+    monkeypatch.setenv("PHUB_OP_MAPS", maps)
+    rng = np.random.default_rng(7)
+    n, nup, ndn = 5, [1, 2, 3], [2, 3]
+    terms = _random_terms(rng, n, 31, 3)
+    terms += [(0.3, [(1, 2, 1, 4, 1)]), (-0.8, [(1, 5, 2, 1, 2), (2, 3, 1, 0, 0)]),
+              (1.1, [(10, 2, 2, 0, 0), (11, 4, 2, 0, 0)]), (0.4, [(8, 1, 0, 3, 0)])]
+    oc = O.OperatorOracle(n, nup, ndn)
+    prog = _program(ctx, n, nup, ndn, terms)
+    V = rng.normal(size=(2, oc.dim))
+    out = prog.apply(torch.as_tensor(V)).cpu().numpy()
+    for k in range(2):
+        ref = oc.apply(terms, V[k])
+        assert np.max(np.abs(out[k] - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+    single = [(0.5, [(1, 1, 1, 2, 1)]), (0.25, [(2, 3, 2, 0, 0)]), (-1.5, [(1, 4, 2, 2, 2)])]
+    oc1 = O.OperatorOracle(4, [2], [1, 2])
+    prog1 = _program(ctx, 4, [2], [1, 2], single)
+    v = rng.normal(size=oc1.dim)
+    ref = oc1.apply(single, v)
+    assert np.max(np.abs(prog1.apply(torch.as_tensor(v)).cpu().numpy() - ref)) < 1e-12
+
+
 def test_hubbard_operator_api_vs_solver(ctx):
     """fermi_hubbard(t, U) as an operator sum (tools/models.py:5-11) on Basis(6,
     (3, 2)): to_matrix equals the pinned Hubbard matrix, O @ v equals the H*v
