@@ -311,7 +311,9 @@ int phub_opprog_create(phub_ctx *ctx, int n_terms, const double *coef, const int
                        const int *elems, phub_opprog **out);
 void phub_opprog_destroy(phub_opprog *p);
 int phub_opprog_terms(const phub_opprog *p);
-/* d_out[v] = O d_in[v] for nvec vectors of nsup*nsdn (vector v at offset v*dim). */
+/* d_out[v] = O d_in[v] for nvec vectors of nsup*nsdn (vector v at offset v*dim).  The
+ * first call on a basis builds the program's per-species source maps (at most 1 GiB,
+ * kept with the program); calls on one program must not overlap. */
 int phub_operator_apply(phub_ctx *ctx, const phub_fock *f, const phub_opprog *p,
                         const double *d_in, double *d_out, int nvec);
 /* d_diag[i] = <e_i|O|e_i> for every basis element (the basis-element mode,
