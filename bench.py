@@ -99,6 +99,9 @@ def main():
     n = Lx * Ly
     T = square_T(Lx, Ly, 1.0, -0.2)
     U = 8.0
+    # the reference's own HPROD on one host core takes about a minute per 4x4 H*v: it
+    # runs in a thread beside the GPU work and is collected with the CPU baseline
+    ref_run = None if args.no_cpu else _start_ref_hprod(T, U)
     t0 = time.perf_counter()
     op = HubbardOp(ctx, n, 8, 8, T, U)
     torch.cuda.synchronize()
@@ -188,8 +191,30 @@ def main():
     if not args.no_lanczos:
         out["lanczos"] = _lanczos_bench(op, args, L, torch)
     if not args.no_cpu:
-        out["cpu_baseline"] = _cpu_baseline(T, U, D, b_alg)
+        out["cpu_baseline"] = _cpu_baseline(T, U, D, b_alg, ref_run)
     print(json.dumps(out), flush=True)
+
+
+def _start_ref_hprod(T, U):
+    """One FP64 H*v of the 4x4 state by the reference's own HPROD
+    (hprod_hubbard.f90, compiled from its sources into oracle/_ref) on one host core, in
+    a thread (the call releases the GIL).  None when that build is absent."""
+    import threading
+    from oracle import oracle as O
+    if not O.ref_available():
+        return None
+    res = {}
+
+    def run():
+        m = O.HubbardModel(16, 8, 8, T, U)
+        v = np.random.default_rng(2).uniform(-0.5, 0.5, m.dim)
+        t0 = time.perf_counter()
+        m.hv_ref(v)
+        res["seconds"] = time.perf_counter() - t0
+
+    th = threading.Thread(target=run, daemon=True)
+    th.start()
+    return th, res
 
 
 def _profile_traffic(name):
@@ -640,9 +665,27 @@ def _lanczos_bench(op, args, L, torch):
     return res
 
 
-def _cpu_baseline(T, U, D, b_alg):
-    """CPU oracle H*v (rank tables, OpenMP over the host cores) on the same 4x4 state."""
+def _cpu_baseline(T, U, D, b_alg, ref_run=None):
+    """CPU oracle H*v (rank tables, OpenMP over the host cores) on the same 4x4 state,
+    next to the reference's own HPROD (one core) when it was started."""
     from oracle import oracle as O
+    # the reference's run ends first, so that it does not share the host cores with
+    # the OpenMP restatement timed below
+    ref = {}
+    if ref_run is not None:
+        th, res = ref_run
+        if th.is_alive():
+            print("# waiting for the reference HPROD (one core)", file=sys.stderr, flush=True)
+        th.join()
+        if "seconds" in res:
+            sr = res["seconds"]
+            ref = {"reference_hprod_1core_s": round(sr, 2),
+                   "reference_hprod_1core_gbs": round(2 * 8 * D / sr / 1e9, 4),
+                   "reference_lanczos_300_steps_s_extrapolated": round(300 * sr, 0),
+                   "reference_sample": "one FP64 HPROD of the full 4x4 state by hprod_hubbard.f90 "
+                                       "compiled from the reference's sources (amdflang -O2), one "
+                                       "host core, run beside the GPU work; the 300-step Lanczos "
+                                       "figure is 300 x that time (extrapolated)"}
     cores = len(os.sched_getaffinity(0))
     cores = min(cores, 16)
     m = O.HubbardModel(16, 8, 8, T, U)
@@ -659,7 +702,7 @@ def _cpu_baseline(T, U, D, b_alg):
     r1 = m1.lanczos(v1, maxlcz=600, acc=1e-12, fast=False)
     dt1 = time.perf_counter() - t1
     return {"value": round(2 * 8 * D / dt / 1e9, 3), "unit": "GB/s", "cores": cores,
-            "kind": "port", "seconds_per_hv": round(dt, 3),
+            "kind": "port", "seconds_per_hv": round(dt, 3), **ref,
             "sample": "one FP64 H*v of the full 4x4 state with the oracle's rank-table "
                       "restatement, OpenMP over the host cores (table build included); "
                       "value = FP64 algorithmic bytes / time",
