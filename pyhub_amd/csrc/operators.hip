@@ -840,11 +840,13 @@ extern "C" int phub_operator_apply(phub_ctx *c, const phub_fock *f, const phub_o
                                    const double *d_in, double *d_out, int nvec) {
     if (!c || !f || !p || !d_in || !d_out) return phub_fail(c, PHUB_ERR_ARG, "null argument");
     if (nvec < 1) return phub_fail(c, PHUB_ERR_ARG, "nvec must be >= 1");
-    if (d_in == d_out) return phub_fail(c, PHUB_ERR_ARG, "out must not alias in");
     if (p->max_site > f->n) return phub_fail(c, PHUB_ERR_ARG, "operator site beyond the basis");
-    PHUB_HIP(c, hipSetDevice(c->device));
     const int64_t ns = f->nsup * f->nsdn;
     if (ns == 0) return PHUB_OK;
+    const size_t span = (size_t)nvec * (size_t)ns;
+    if (d_in < d_out + span && d_out < d_in + span)
+        return phub_fail(c, PHUB_ERR_ARG, "out must not overlap in");
+    PHUB_HIP(c, hipSetDevice(c->device));
     OpArgs a = op_args(c, f, p->vec);
     if (int rc = op_maps(c, f, p, a)) return rc;
     const int64_t tiles = ((f->nsup + 255) / 256) * ((f->nsdn + OPR_R - 1) / OPR_R);

@@ -151,8 +151,8 @@ def test_counting_operator_and_restriction(ctx):
 
 
 def test_operator_errors(ctx):
-    """An operator that names a site beyond the basis, an unknown factor string and
-    an application before set_basis are rejected."""
+    """An operator that names a site beyond the basis, an unknown factor string, an
+    application before set_basis and overlapping in / out blocks are rejected."""
     from pyhub_amd.core.basis import Basis
     from pyhub_amd.core._operators import Operators
     from pyhub_amd.tools.operators import n
@@ -165,3 +165,10 @@ def test_operator_errors(ctx):
     bad.str_index = [["foo"]]
     with pytest.raises((ValueError, NotImplementedError)):
         bad.set_basis(Basis(2, hilbert=(1, 1)))
+    # overlapping input and output blocks are rejected before any launch
+    import ctypes as C
+    prog = _program(ctx, 3, [1], [1], [(1.0, [(1, 1, 1, 2, 1)])])
+    buf = torch.zeros(3 * prog.nstates, dtype=torch.float64, device=ctx.device)
+    with pytest.raises(ValueError):
+        ctx.check(ctx.lib.phub_operator_apply(ctx.h, prog.fock, prog.prog, C.c_void_p(buf.data_ptr()),
+                                              C.c_void_p(buf.data_ptr() + 8 * prog.nstates), 2))
