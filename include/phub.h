@@ -41,7 +41,8 @@ enum {
     PHUB_ERR_NOCONV = 4,       /* Lanczos hit max steps (reference: "MAX LCZ STEP REACH") */
     PHUB_ERR_BREAKDOWN = 5,    /* invariant subspace found before convergence was checked */
     PHUB_ERR_HIP = 6,          /* HIP runtime error */
-    PHUB_ERR_NORM_LOST = 7     /* reconstructed ground vector norm drifted (reference: "NORM LOST") */
+    PHUB_ERR_NORM_LOST = 7     /* kept for the ABI; phub_lanczos reports NORM LOST in
+                                * phub_lanczos_result.norm_lost instead */
 };
 
 typedef struct phub_ctx phub_ctx;
@@ -171,6 +172,11 @@ typedef struct {
     double seconds_lanczos; /* of which: Krylov steps */
     double hv_ms_avg;       /* average H*v kernel time */
     double e_states[PHUB_MAX_STATES];   /* lowest Ritz values (e_states[0] = e0) */
+    int norm_lost;          /* the reference's NORM LOST (lanczos.f90:74-76): the ground
+                             * vector rebuilt from the Krylov vectors had a norm more than
+                             * 1e-5 away from 1 (loss of orthogonality in the plain forms);
+                             * it is normalised and the call still succeeds, as the
+                             * reference prints the message and keeps its results */
 } phub_lanczos_result;
 
 /* Device bytes phub_lanczos uses for its Krylov vectors under these options (the

@@ -47,7 +47,8 @@ class LanczosResult(C.Structure):
     _fields_ = [("e0", C.c_double), ("e0_ritz", C.c_double), ("residual", C.c_double),
                 ("steps", C.c_int), ("converged", C.c_int), ("ndeg", C.c_int),
                 ("seconds", C.c_double), ("seconds_lanczos", C.c_double),
-                ("hv_ms_avg", C.c_double), ("e_states", C.c_double * MAX_STATES)]
+                ("hv_ms_avg", C.c_double), ("e_states", C.c_double * MAX_STATES),
+                ("norm_lost", C.c_int)]
 
 
 class Cluster(C.Structure):

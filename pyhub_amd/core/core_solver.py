@@ -160,6 +160,9 @@ class CoreSolver(Basis, GreenFunction, StaticQuantities):
                 out = op.lanczos(mode=_MODES[self.lanczos_mode], max_steps=self.max_lcz,
                                  tol=self.acc_lcz, v0=v0, refine_steps=2, want_psi=True,
                                  n_states=nst)
+                if out.norm_lost:
+                    # the reference prints this and keeps its results (lanczos.f90:74-76)
+                    print(" FORTRAN LANCZOS ERROR: NORM LOST", flush=True)
                 m = len(out.alpha)
                 val, vec = eigh_tridiagonal(out.alpha, out.beta[1:m]) if m > 1 else (out.alpha.copy(), np.ones((1, 1)))
                 nw = len(boltzmann_weights(val, self.T, self.nb_comp_states_))

@@ -1184,6 +1184,7 @@ static int hv_launch_main(phub_ctx *c, const phub_op *op, const T *x, T *out, do
         }
         HeisBlkArgs a = heis_blk_args(op);
         const size_t lds = (size_t)(a.na1 + a.na2) * sizeof(uint32_t);
+        // order: 0 each XCD takes one contiguous eighth of the item list, 1 round robin
         auto go = [&](int order) {
             a.order = order;
             if (epi)
@@ -1197,10 +1198,14 @@ static int hv_launch_main(phub_ctx *c, const phub_op *op, const T *x, T *out, do
         phub_op *mop = const_cast<phub_op *>(op);
         if (mop->tune_heis[ti] < 0) {
             // tuner: time both item orders on the caller's vectors (the kernel
-            // rewrites out completely, so the final launch below gives the result)
+            // rewrites out completely, so the final launch below gives the result);
+            // PHUB_HEIS_ORDER=k forces order k
             const char *env = std::getenv("PHUB_TUNE");
+            const char *force = std::getenv("PHUB_HEIS_ORDER");   // a fixed code
             int best = 0;
-            if (!(env && env[0] == '0')) {
+            if (force && std::atoi(force) >= 0 && std::atoi(force) < 2) {
+                best = std::atoi(force);
+            } else if (!(env && env[0] == '0')) {
                 float tb = 1e30f;
                 for (int order = 0; order < 2; ++order) {
                     PHUB_HIP(c, hipEventRecord(c->ev0, c->stream));

@@ -1227,7 +1227,7 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
         for (int i = 0; i < s.steps; ++i) alpha[i] = s.alpha[i];
     if (beta)
         for (int i = 0; i <= s.steps; ++i) beta[i] = s.beta[i];
-    if (s.norm_lost) return phub_fail(c, PHUB_ERR_NORM_LOST, "FORTRAN LANCZOS ERROR: NORM LOST equivalent: |psi| drifted by > 1e-5");
+    res->norm_lost = s.norm_lost ? 1 : 0;
     if (!res->converged)
         return phub_fail(c, PHUB_ERR_NOCONV, "Lanczos reached max_steps without convergence");
     return PHUB_OK;

@@ -93,6 +93,7 @@ class LanczosOutput:
     psi: torch.Tensor | None
     psis: torch.Tensor | None = None        # (n_states, dim) when n_states > 1
     e_states: np.ndarray | None = None      # lowest Ritz values
+    norm_lost: bool = False                 # lanczos.f90:74-76 NORM LOST (psi renormalised)
 
 
 class _Op:
@@ -209,7 +210,7 @@ class _Op:
         return LanczosOutput(res.e0, res.e0_ritz, res.residual, res.steps, bool(res.converged),
                              res.ndeg, res.seconds, res.seconds_lanczos, res.hv_ms_avg,
                              alpha[: res.steps].copy(), beta[: res.steps + 1].copy(), psi,
-                             psis, np.array(res.e_states[:nst]))
+                             psis, np.array(res.e_states[:nst]), bool(res.norm_lost))
 
     def cf_chain(self, start: torch.Tensor, nsteps: int, precision: int = 64):
         a = np.zeros(nsteps)

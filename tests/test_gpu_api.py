@@ -252,6 +252,29 @@ def test_heisenberg_ring(ctx):
     assert abs(h.e0 - 2 * es) < 1e-9
 
 
+def test_norm_lost_is_reported_not_raised(ctx):
+    """Plain FP64 Lanczos run far past convergence (60 fixed steps on the 70-state
+    ring) loses orthogonality; as in the reference (lanczos.f90:74-76, a printed
+    NORM LOST) the call succeeds, psi comes back normalised and is still the ground
+    state."""
+    from pyhub_amd import _lib as L
+    from pyhub_amd.engine import HeisenbergOp
+    n = 8
+    J = np.zeros((n, n))
+    for i in range(n):
+        J[i, (i + 1) % n] = J[(i + 1) % n, i] = 1.0
+    op = HeisenbergOp(ctx, n, J)
+    m = O.HeisenbergModel(n, J)
+    ev, V = np.linalg.eigh(m.dense())
+    v0 = torch.as_tensor(np.random.default_rng(3).uniform(-0.5, 0.5, m.dim))
+    out = op.lanczos(mode=L.LCZ_PLAIN64, fixed_steps=60, v0=v0, want_psi=True)
+    psi = out.psi.cpu().numpy()
+    assert abs(np.linalg.norm(psi) - 1.0) < 1e-12
+    assert abs(out.e0 - ev[0]) < 1e-9
+    assert abs(abs(psi @ V[:, 0]) - 1.0) < 1e-8
+    assert isinstance(out.norm_lost, bool)
+
+
 def test_dimer_closed_form(ctx):
     from pyhub_amd.solver.fermi_hubbard import FermiHubbard
     for U in (0.0, 2.0, 8.0):

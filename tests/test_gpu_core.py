@@ -140,6 +140,25 @@ def test_heisenberg_block_hv_vs_oracle(ctx, n, seed, dtype):
     assert np.max(np.abs(w - ref)) <= tol * max(1.0, np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("code", range(2))
+def test_heisenberg_block_hv_item_orders(ctx, monkeypatch, code):
+    """Both item orders the tuner can pick (PHUB_HEIS_ORDER forces one) give the
+    oracle's HPROD."""
+    from pyhub_amd.engine import HeisenbergOp
+    monkeypatch.setenv("PHUB_HEIS_ORDER", str(code))
+    n = 20
+    rng = np.random.default_rng(46)
+    J = rng.normal(size=(n, n))
+    J[rng.random((n, n)) < 0.5] = 0.0
+    J = (J + J.T) / 2
+    op = HeisenbergOp(ctx, n, J)
+    m = O.HeisenbergModel(n, J)
+    v = rng.normal(size=m.dim)
+    ref = m.hv(v, fast=True)
+    w = op.hv(dev(v)).cpu().numpy()
+    assert np.max(np.abs(w - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
 def test_j1j2_4x4_standard_convention_e0(ctx):
     """J1-J2 on the 4x4 torus (J2/J1 = 0.5) in the standard convention: the
     ordered-pair matrix carries J/2 per pair (sign_convention=1); E0 against the
