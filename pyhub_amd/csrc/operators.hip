@@ -59,6 +59,7 @@ struct OpProgDev {
     // product (-1: none), rowq_up / rowq_dn the product of each row
     int *urow = nullptr, *drow = nullptr, *rowq_up = nullptr, *rowq_dn = nullptr;
     int nrow_up = 0, nrow_dn = 0;
+    bool has_mix = false;        // some product has a factor that reads both species
 };
 
 struct phub_opprog {
@@ -305,7 +306,7 @@ static hipError_t upload_prog(OpProgDev &d, const std::vector<double> &cf,
         for (int f = st[q]; f < st[q + 1]; ++f) touch[q] |= factor_touch(&el[5 * (size_t)f]);
     std::vector<int> urow(touch.size(), -1), drow(touch.size(), -1), rq_up, rq_dn;
     for (size_t q = 0; q < cf.size(); ++q) {
-        if (touch[q] & 8) continue;
+        if (touch[q] & 8) { d.has_mix = true; continue; }
         bool up = false, dn = false;
         for (int f = st[q]; f < st[q + 1]; ++f) {
             const int *e = &el[5 * (size_t)f];
@@ -662,14 +663,20 @@ __global__ __launch_bounds__(256) void k_op_map(OpArgs a, int sp, const int *__r
 // Products with a factor that reads both species take the per-state walk.
 constexpr int OPR_R = 8;
 
+// MAPONLY: every product takes the map path (no mixed-species factor, maps built):
+// the per-state walks are compiled out, which frees registers for occupancy.
+template <bool MAPONLY>
 __global__ __launch_bounds__(256) void k_op_apply_rows(OpArgs a, const double *__restrict__ in,
                                                        double *__restrict__ out) {
-    __shared__ uint32_t s_nib[OP_NIB_MAX];
-    __shared__ int64_t s_base[2 * (OP_MAXN + 1)];
-    const int nn = a.nchunk * (a.n + 1) * 16;
-    for (int i = threadIdx.x; i < nn; i += blockDim.x) s_nib[i] = a.nib[i];
-    for (int i = threadIdx.x; i < 2 * (OP_MAXN + 1); i += blockDim.x) s_base[i] = a.base[i];
-    __syncthreads();
+    // the ranking tables serve the per-state walks only
+    __shared__ uint32_t s_nib[MAPONLY ? 1 : OP_NIB_MAX];
+    __shared__ int64_t s_base[MAPONLY ? 1 : 2 * (OP_MAXN + 1)];
+    if constexpr (!MAPONLY) {
+        const int nn = a.nchunk * (a.n + 1) * 16;
+        for (int i = threadIdx.x; i < nn; i += blockDim.x) s_nib[i] = a.nib[i];
+        for (int i = threadIdx.x; i < 2 * (OP_MAXN + 1); i += blockDim.x) s_base[i] = a.base[i];
+        __syncthreads();
+    }
     op_ci32 *st = (op_ci32 *)a.start;
     op_ci32 *el = (op_ci32 *)a.elem;
     op_ci32 *tc = (op_ci32 *)a.touch;
@@ -690,6 +697,38 @@ __global__ __launch_bounds__(256) void k_op_apply_rows(OpArgs a, const double *_
         double acc[OPR_R];
 #pragma unroll
         for (int r = 0; r < OPR_R; ++r) acc[r] = 0.0;
+        if constexpr (MAPONLY) {
+            // every product from the maps: the next product's up entry is loaded
+            // before this product's gathers, each gather is issued as soon as its
+            // row's source is known, and the row signs stay wave-uniform bits
+            using ci32m = const __attribute__((address_space(4))) int32_t;
+            auto up_entry = [&](int q) -> int32_t {
+                const int ur = ((op_ci32 *)a.urow)[q];
+                return ur < 0 ? (int32_t)(tu << 1) : a.map_up[(int64_t)ur * a.nsup + tu];
+            };
+            int32_t eu_next = a.nterms > 0 ? up_entry(0) : -1;
+            for (int q = 0; q < a.nterms; ++q) {
+                const int32_t eu = eu_next;
+                if (q + 1 < a.nterms) eu_next = up_entry(q + 1);
+                if (__ballot(eu >= 0) == 0) continue;
+                const double c = cf[q];
+                const int dr = ((op_ci32 *)a.drow)[q];
+                const int64_t su = eu >= 0 ? (int64_t)(eu >> 1) : tu;
+                const double wu = eu < 0 ? 0.0 : ((eu & 1) ? -c : c);
+                double g[OPR_R];
+                uint32_t neg = 0u;
+#pragma unroll
+                for (int r = 0; r < OPR_R; ++r) {
+                    const int64_t d = d0 + r;
+                    const int32_t ed = d >= a.nsdn ? -1
+                        : dr < 0 ? (int32_t)(d << 1) : ((ci32m *)a.map_dn)[(int64_t)dr * a.nsdn + d];
+                    neg |= (ed >= 0 && (ed & 1)) ? (1u << r) : 0u;
+                    g[r] = (ed >= 0 && eu >= 0) ? in[(int64_t)(ed >> 1) * a.nsup + su] : 0.0;
+                }
+#pragma unroll
+                for (int r = 0; r < OPR_R; ++r) acc[r] += ((neg >> r) & 1u ? -wu : wu) * g[r];
+            }
+        } else
         for (int q = 0; q < a.nterms; ++q) {
             const int f0 = st[q], f1 = st[q + 1], touch = tc[q];
             const double c = cf[q];
@@ -851,9 +890,15 @@ extern "C" int phub_operator_apply(phub_ctx *c, const phub_fock *f, const phub_o
     if (int rc = op_maps(c, f, p, a)) return rc;
     const int64_t tiles = ((f->nsup + 255) / 256) * ((f->nsdn + OPR_R - 1) / OPR_R);
     const unsigned grid = (unsigned)std::min<int64_t>(tiles, 16384);
-    for (int v = 0; v < nvec; ++v)
-        hipLaunchKernelGGL(k_op_apply_rows, dim3(grid), dim3(256), 0, c->stream, a,
-                           d_in + (size_t)v * ns, d_out + (size_t)v * ns);
+    const bool maponly = a.use_map && !p->vec.has_mix;
+    for (int v = 0; v < nvec; ++v) {
+        if (maponly)
+            hipLaunchKernelGGL(k_op_apply_rows<true>, dim3(grid), dim3(256), 0, c->stream, a,
+                               d_in + (size_t)v * ns, d_out + (size_t)v * ns);
+        else
+            hipLaunchKernelGGL(k_op_apply_rows<false>, dim3(grid), dim3(256), 0, c->stream, a,
+                               d_in + (size_t)v * ns, d_out + (size_t)v * ns);
+    }
     PHUB_HIP(c, hipGetLastError());
     return PHUB_OK;
 }
