@@ -639,6 +639,7 @@ def _lanczos_bench(op, args, L, torch):
     need = max(op.lanczos_workspace_bytes(m, cap, 0, 2, True) for _, m, cap in runs)
     ws = torch.empty(min(need, int(0.9 * free)), dtype=torch.uint8, device=op.ctx.device)
     res["workspace_gb"] = ws.numel() / 1e9
+    rq, rq_s = {}, {}
     for name, mode, cap in runs:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -650,6 +651,13 @@ def _lanczos_bench(op, args, L, torch):
                      "converged": r.converged, "residual": r.residual,
                      "seconds": r.seconds, "seconds_lanczos": r.seconds_lanczos,
                      "wall_incl_residual": wall, "hv_ms_avg": r.hv_ms_avg}
+        # reduced quantities of the state: 1-RDM per spin, double occupancy and the
+        # S.S matrix in one pass over psi (FP64)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rq[name] = op.rdm(r.psi)
+        torch.cuda.synchronize()
+        rq_s[name] = time.perf_counter() - t1
         print(f"# lanczos {name}: E0={r.e0:.12f} steps={r.steps} {r.seconds:.2f}s "
               f"res={r.residual:.2e}", file=sys.stderr, flush=True)
         del r
@@ -658,6 +666,28 @@ def _lanczos_bench(op, args, L, torch):
     e64 = res["fp64_full"]["e0"]
     for k in ("mixed", "mixed_plain", "fp64"):
         res[k]["rel_err_vs_fp64_full"] = abs(res[k]["e0"] - e64) / abs(e64)
+    # 1-RDM and nearest-neighbour <S.S> of every form against the FP64 run with full
+    # reorthogonalisation
+    n = op.n
+    L4 = int(round(n ** 0.5))
+    nn = [(x + L4 * y, (x + 1) % L4 + L4 * y) for y in range(L4) for x in range(L4)]
+    nn += [(x + L4 * y, x + L4 * ((y + 1) % L4)) for y in range(L4) for x in range(L4)]
+    up64, dn64, ni64, ss64 = rq["fp64_full"]
+    nn64 = float(np.mean([ss64[i, j] for i, j in nn]))
+    res["fp64_full"]["nn_sisj"] = nn64
+    res["fp64_full"]["rdm_pass_s"] = rq_s["fp64_full"]
+    for k in ("mixed", "mixed_plain", "fp64"):
+        up, dn, ni, ss = rq[k]
+        res[k]["nn_sisj"] = float(np.mean([ss[i, j] for i, j in nn]))
+        res[k]["one_rdm_max_abs_err_vs_fp64_full"] = float(max(np.max(np.abs(up - up64)),
+                                                               np.max(np.abs(dn - dn64))))
+        res[k]["nn_sisj_abs_err_vs_fp64_full"] = abs(res[k]["nn_sisj"] - nn64)
+        res[k]["double_occ_max_abs_err_vs_fp64_full"] = float(np.max(np.abs(ni - ni64)))
+        res[k]["rdm_pass_s"] = rq_s[k]
+    print(f"# nearest-neighbour <S.S> {nn64:.12f}; max |d gamma| mixed "
+          f"{res['mixed']['one_rdm_max_abs_err_vs_fp64_full']:.2e}, mixed_plain "
+          f"{res['mixed_plain']['one_rdm_max_abs_err_vs_fp64_full']:.2e}; RDM pass "
+          f"{rq_s['fp64_full'] * 1e3:.1f} ms", file=sys.stderr, flush=True)
     res["speedup_mixed_vs_fp64_full"] = res["fp64_full"]["seconds"] / res["mixed"]["seconds"]
     res["speedup_mixed_vs_fp64_plain"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
     res["speedup_mixed_plain_vs_fp64_plain"] = res["fp64"]["seconds"] / res["mixed_plain"]["seconds"]
