@@ -271,6 +271,118 @@ void phub_free_hop_table(HopTable *t) {
     *t = HopTable();
 }
 
+void phub_free_hop_batch(HopBatch *b) {
+    for (void *&p : b->arena)
+        if (p) { hipFree(p); p = nullptr; }
+    b->tabs.clear();
+}
+
+int phub_build_hop_tables(phub_ctx *c, const std::vector<HopSpec> &specs, HopBatch *out) {
+    phub_free_hop_batch(out);
+    const size_t nt = specs.size();
+    out->tabs.assign(nt, HopTable());
+    if (nt == 0) return PHUB_OK;
+    // host staging of every table's pair list (the rules of phub_build_hop_table)
+    std::vector<int> bj, bk;
+    std::vector<double> tv, td;
+    std::vector<int16_t> pid;
+    std::vector<size_t> poff(nt), np(nt), toff(nt);
+    std::vector<int64_t> dim(nt), moff(nt);
+    int64_t mtot = 0;
+    for (size_t t = 0; t < nt; ++t) {
+        const HopSpec &h = specs[t];
+        const int n = h.n;
+        if (n < 1 || n > 32 || h.k < 0 || h.k > n) return phub_fail(c, PHUB_ERR_ARG, "hop table sector out of range");
+        poff[t] = bj.size();
+        toff[t] = td.size();
+        for (int J = 1; J <= n; ++J) {
+            const double dj = h.T[(J - 1) * n + (J - 1)];
+            td.push_back(std::fabs(dj) >= 1e-14 ? dj : 0.0);
+            for (int K = 1; K <= n; ++K) {
+                if (J == K) continue;
+                const double tjk = h.T[(J - 1) * n + (K - 1)];
+                if (!(std::fabs(tjk) >= 1e-14)) continue;
+                bj.push_back(n - J);
+                bk.push_back(n - K);
+                tv.push_back(tjk);
+                pid.push_back((int16_t)((J - 1) * n + (K - 1)));
+            }
+        }
+        np[t] = bj.size() - poff[t];
+        dim[t] = (int64_t)phub_binom(n, h.k);
+        moff[t] = mtot;
+        mtot += dim[t];
+    }
+    // arena 0: pairs and per-table maxima, one copy
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t npair = std::max<size_t>(bj.size(), 1);
+    const size_t o_bk = al(npair * sizeof(int)), o_t = o_bk + al(npair * sizeof(int));
+    const size_t o_pid = o_t + al(npair * sizeof(double)), o_td = o_pid + al(npair * sizeof(int16_t));
+    const size_t o_max = o_td + al(std::max<size_t>(td.size(), 1) * sizeof(double));
+    const size_t bytes0 = o_max + al(nt * sizeof(int));
+    std::vector<char> stage(bytes0, 0);
+    if (!bj.empty()) {
+        std::memcpy(stage.data(), bj.data(), bj.size() * sizeof(int));
+        std::memcpy(stage.data() + o_bk, bk.data(), bk.size() * sizeof(int));
+        std::memcpy(stage.data() + o_t, tv.data(), tv.size() * sizeof(double));
+        std::memcpy(stage.data() + o_pid, pid.data(), pid.size() * sizeof(int16_t));
+    }
+    std::memcpy(stage.data() + o_td, td.data(), td.size() * sizeof(double));
+    PHUB_HIP(c, hipMalloc(&out->arena[0], bytes0));
+    char *a0 = (char *)out->arena[0];
+    PHUB_HIP(c, hipMemcpyAsync(a0, stage.data(), bytes0, hipMemcpyHostToDevice, c->stream));
+    // arena 1: masks (u32), eloc (f64), cnt (i32) of every table
+    const size_t o_el = al((size_t)mtot * sizeof(uint32_t)), o_cn = o_el + al((size_t)mtot * sizeof(double));
+    PHUB_HIP(c, hipMalloc(&out->arena[1], o_cn + al((size_t)mtot * sizeof(int32_t))));
+    char *a1 = (char *)out->arena[1];
+    auto pl_of = [&](size_t t) {
+        return PairList{(int)np[t], (const int *)(a0) + poff[t], (const int *)(a0 + o_bk) + poff[t],
+                        (const double *)(a0 + o_t) + poff[t], (const int16_t *)(a0 + o_pid) + poff[t]};
+    };
+    for (size_t t = 0; t < nt; ++t) {
+        HopTable &h = out->tabs[t];
+        h.n = specs[t].n;
+        h.k = specs[t].k;
+        h.kmajor = specs[t].kmajor;
+        h.dim = dim[t];
+        h.masks = (uint32_t *)a1 + moff[t];
+        h.eloc = (double *)(a1 + o_el) + moff[t];
+        h.cnt = (int32_t *)(a1 + o_cn) + moff[t];
+        hipLaunchKernelGGL(k_unrank<uint32_t>, dim3(nblk(dim[t], 256)), dim3(256), 0, c->stream,
+                           c->d_binom, h.n, h.k, dim[t], h.masks);
+        hipLaunchKernelGGL(k_hop_count, dim3(nblk(dim[t], 256)), dim3(256), 0, c->stream, h.masks,
+                           dim[t], pl_of(t), h.cnt, (int *)(a0 + o_max) + t);
+    }
+    PHUB_HIP(c, hipGetLastError());
+    std::vector<int> mh(nt, 0);
+    PHUB_HIP(c, hipMemcpyAsync(mh.data(), a0 + o_max, nt * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    // arena 2: idx (i32), c64, c32 of every table
+    std::vector<size_t> eoff(nt);
+    size_t etot = 0;
+    for (size_t t = 0; t < nt; ++t) {
+        eoff[t] = etot;
+        etot += (size_t)dim[t] * std::max(mh[t], 1);
+    }
+    const size_t o_c64 = al(etot * sizeof(int32_t)), o_c32 = o_c64 + al(etot * sizeof(double));
+    const size_t bytes2 = o_c32 + al(etot * sizeof(float));
+    PHUB_HIP(c, hipMalloc(&out->arena[2], bytes2));
+    char *a2 = (char *)out->arena[2];
+    PHUB_HIP(c, hipMemsetAsync(a2, 0, bytes2, c->stream));
+    for (size_t t = 0; t < nt; ++t) {
+        HopTable &h = out->tabs[t];
+        h.maxh = mh[t];
+        h.idx = (int32_t *)a2 + eoff[t];
+        h.c64 = (double *)(a2 + o_c64) + eoff[t];
+        h.c32 = (float *)(a2 + o_c32) + eoff[t];
+        hipLaunchKernelGGL(k_hop_fill, dim3(nblk(dim[t], 256)), dim3(256), 0, c->stream, c->d_binom,
+                           h.n, h.k, h.masks, dim[t], pl_of(t), mh[t], h.kmajor, h.idx, h.c64, h.c32,
+                           (int16_t *)nullptr, h.eloc, (const double *)(a0 + o_td) + toff[t]);
+    }
+    PHUB_HIP(c, hipGetLastError());
+    return PHUB_OK;
+}
+
 // compressed entries: (target index << 8) | class of the signed coefficient
 __global__ void k_hop_pack(const int32_t *__restrict__ idx, const double *__restrict__ c64,
                            const int32_t *__restrict__ cnt, int64_t dim, int maxh,

@@ -12,13 +12,19 @@
 // pulling tasks from an atomic counter: the first solves every ground state
 // (largest cluster first), the second runs every chain (largest sector first).
 // Stream order makes the ground states visible to the chains, so no workgroup
-// ever waits on another.
+// ever waits on another.  A ground-state sector of BIG_GS states or more would
+// hold one workgroup far longer than the rest of the launch, so those ground
+// states are solved instead by the full-device Lanczos (FP64 vectors, full
+// reorthogonalisation) from host threads on streams of their own while the first
+// launch runs; the chain launch is queued after they finish.
 #include "phub_internal.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -26,6 +32,7 @@ namespace {
 constexpr int BT = 1024;          // threads per workgroup
 constexpr int MAXN = 16;          // sites per cluster in the batched path
 constexpr int MAXSTEPS = 256;     // Lanczos steps per task
+constexpr int64_t BIG_GS = 100000;  // ground states of at least this many states: full device
 
 struct SectorDev {
     int64_t nu, nd;
@@ -388,6 +395,76 @@ struct DB {
 
 }  // namespace
 
+// Ground states of the clusters in `big` by phub_lanczos (FULL64, psi into the
+// cluster's psi0 slot, e0 into d_e0), beside the persistent launch already queued on
+// c->stream.  Each solve is bound by its host round trips, so up to BIG_THREADS host
+// threads run them at once, each on a context with a stream of its own; every thread
+// synchronises its stream before it ends, so the chain launch queued after the join
+// sees psi0 and e0.  The stopping rule is the Ritz value's change, 1e-3 of gs_tol,
+// which leaves psi0 at least as converged as the persistent path's residual rule.
+constexpr int BIG_THREADS = 8;
+
+static int big_ground_states(phub_ctx *c, const std::vector<int> &big,
+                             const std::vector<phub_op *> &ops_gs,
+                             const std::vector<ClusterDev> &hc, const phub_batch_opts *o,
+                             double *d_e0) {
+    const int nth = std::min<int>(BIG_THREADS, (int)big.size());
+    std::vector<int> rcs(nth, PHUB_OK);
+    std::vector<std::string> msgs(nth);
+    auto worker = [&](int w) {
+        hipStream_t s2 = nullptr;
+        if (hipSetDevice(c->device) != hipSuccess ||
+            hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess) {
+            rcs[w] = PHUB_ERR_HIP;
+            msgs[w] = "stream for the large ground states";
+            return;
+        }
+        phub_ctx *c2 = nullptr;
+        int rc = phub_ctx_create(c->device, (void *)s2, &c2);
+        std::vector<double> al(MAXSTEPS + 2), be(MAXSTEPS + 3);
+        // clusters w, w + nth, ... of the list (largest first)
+        for (size_t j = w; j < big.size() && rc == PHUB_OK; j += nth) {
+            const int i = big[j];
+            phub_lanczos_opts lo{};
+            lo.mode = PHUB_LCZ_FULL64;
+            lo.max_steps = o->gs_steps;
+            lo.tol = 1e-3 * o->gs_tol;
+            lo.check_every = 5;
+            lo.seed = o->seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
+            lo.want_psi = 1;
+            lo.n_states = 1;
+            phub_lanczos_result res{};
+            rc = phub_lanczos(c2, ops_gs[i], &lo, nullptr, al.data(), be.data(), &res, hc[i].psi0);
+            if (rc == PHUB_ERR_NOCONV) rc = PHUB_OK;   // the step cap, as in the persistent path
+            if (rc == PHUB_OK) {
+                const double e = res.e0;
+                if (hipMemcpyAsync(d_e0 + i, &e, sizeof(double), hipMemcpyHostToDevice, s2) != hipSuccess ||
+                    hipStreamSynchronize(s2) != hipSuccess)
+                    rc = PHUB_ERR_HIP;
+            }
+        }
+        if (hipStreamSynchronize(s2) != hipSuccess && rc == PHUB_OK) rc = PHUB_ERR_HIP;
+        rcs[w] = rc;
+        if (rc) msgs[w] = c2 ? c2->err : std::string("context for the large ground states");
+        if (c2) phub_ctx_destroy(c2);
+        hipStreamDestroy(s2);
+    };
+    std::vector<std::thread> th;
+    std::vector<int> here{0};   // workers run on this thread (all of them if threads fail)
+    for (int w = 1; w < nth; ++w) {
+        try {
+            th.emplace_back(worker, w);
+        } catch (...) {
+            here.push_back(w);
+        }
+    }
+    for (int w : here) worker(w);
+    for (auto &t : th) t.join();
+    for (int w = 0; w < nth; ++w)
+        if (rcs[w]) return phub_fail(c, rcs[w], msgs[w].empty() ? "large ground state" : msgs[w]);
+    return PHUB_OK;
+}
+
 extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
                                   const phub_batch_opts *o, double *e0, double *gf_alpha,
                                   double *gf_beta, double *gf_norm2) {
@@ -396,14 +473,50 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
         return phub_fail(c, PHUB_ERR_ARG, "steps must be in [2, 256]");
     PHUB_HIP(c, hipSetDevice(c->device));
     std::vector<phub_op *> ops;
+    HopBatch tabs;
     auto cleanup = [&]() {
+        phub_free_hop_batch(&tabs);
         for (auto *p : ops) phub_op_destroy(p);
     };
     std::vector<ClusterDev> hc(ncl);
-    int64_t dmax = 1, psi_total = 0;
+    std::vector<phub_op *> ops_gs(ncl, nullptr);   // ground-sector operator of the large clusters
+    // the per-species tables of every cluster: up tables of N_up - 1, N_up, N_up + 1
+    // (k-major) and down tables of N_dn - 1, N_dn, N_dn + 1 (row-major), shared by the
+    // five sectors, all built in one batch
+    std::vector<HopSpec> specs;
+    std::vector<int> tab_of(ncl * 6, -1);   // [cluster][0..2 up, 3..5 down] -> table
     for (int i = 0; i < ncl; ++i) {
         const phub_cluster &q = cls[i];
         if (q.n_sites < 1 || q.n_sites > MAXN) { cleanup(); return phub_fail(c, PHUB_ERR_UNSUPPORTED, "batched clusters need n <= 16"); }
+        if (q.n_up < 0 || q.n_up > q.n_sites || q.n_dn < 0 || q.n_dn > q.n_sites) {
+            cleanup();
+            return phub_fail(c, PHUB_ERR_ARG, "particle numbers out of range");
+        }
+        for (int j = 0; j < q.n_sites * q.n_sites; ++j)
+            if (!std::isfinite(q.t[j])) { cleanup(); return phub_fail(c, PHUB_ERR_ARG, "t_matrix has non-finite entries"); }
+        for (int j = 0; j < q.n_sites; ++j)
+            if (!std::isfinite(q.u[j])) { cleanup(); return phub_fail(c, PHUB_ERR_ARG, "U has non-finite entries"); }
+        for (int J = 0; J < q.n_sites; ++J)
+            for (int K = 0; K < J; ++K)
+                if (q.t[J * q.n_sites + K] != q.t[K * q.n_sites + J]) {
+                    cleanup();
+                    return phub_fail(c, PHUB_ERR_ARG, "t_matrix must be symmetric");
+                }
+        for (int sp = 0; sp < 2; ++sp)
+            for (int dk = -1; dk <= 1; ++dk) {
+                const int k = (sp == 0 ? q.n_up : q.n_dn) + dk;
+                if (k < 0 || k > q.n_sites) continue;
+                tab_of[i * 6 + sp * 3 + dk + 1] = (int)specs.size();
+                specs.push_back(HopSpec{q.n_sites, k, q.t, sp == 0});
+            }
+    }
+    {
+        int rc = phub_build_hop_tables(c, specs, &tabs);
+        if (rc) { cleanup(); return rc; }
+    }
+    int64_t dmax = 1, psi_total = 0;
+    for (int i = 0; i < ncl; ++i) {
+        const phub_cluster &q = cls[i];
         ClusterDev &h = hc[i];
         std::memset(&h, 0, sizeof(h));
         h.n = q.n_sites;
@@ -413,38 +526,42 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
             h.U[j] = q.u[j];
             if (q.u[j] != q.u[0]) h.u_uniform = 0;
         }
-        const int secs[5][2] = {{q.n_up, q.n_dn}, {q.n_up - 1, q.n_dn}, {q.n_up + 1, q.n_dn},
-                                {q.n_up, q.n_dn - 1}, {q.n_up, q.n_dn + 1}};
+        // sector s: (up offset, down offset) from (N_up, N_dn)
+        const int secs[5][2] = {{0, 0}, {-1, 0}, {1, 0}, {0, -1}, {0, 1}};
         for (int s = 0; s < 5; ++s) {
             SectorDev &sd = h.sec[s];
-            const int nu = secs[s][0], nd = secs[s][1];
-            sd.nup = nu;
-            sd.ndn = nd;
-            if (nu < 0 || nu > q.n_sites || nd < 0 || nd > q.n_sites) continue;   // empty sector
-            phub_hubbard_desc desc{q.n_sites, nu, nd, q.t, q.u, nullptr};
+            sd.nup = q.n_up + secs[s][0];
+            sd.ndn = q.n_dn + secs[s][1];
+            const int tu = tab_of[i * 6 + secs[s][0] + 1], tdn = tab_of[i * 6 + 3 + secs[s][1] + 1];
+            if (tu < 0 || tdn < 0) continue;   // empty sector
+            const HopTable &up = tabs.tabs[tu], &dn = tabs.tabs[tdn];
+            sd.nu = up.dim;
+            sd.nd = dn.dim;
+            sd.mu = up.masks;
+            sd.md = dn.masks;
+            sd.eu = up.eloc;
+            sd.ed = dn.eloc;
+            sd.cu = up.cnt;
+            sd.iu = up.idx;
+            sd.cfu = up.c32;
+            sd.cfu64 = up.c64;
+            sd.cd = dn.cnt;
+            sd.id = dn.idx;
+            sd.cfd = dn.c32;
+            sd.cfd64 = dn.c64;
+            sd.maxh_dn = std::max(dn.maxh, 1);
+            dmax = std::max<int64_t>(dmax, sd.nu * sd.nd);
+        }
+        h.psi_off = psi_total;
+        psi_total += std::max<int64_t>(h.sec[0].nu * h.sec[0].nd, 1);
+        if (h.sec[0].nu * h.sec[0].nd >= BIG_GS) {
+            phub_hubbard_desc desc{q.n_sites, q.n_up, q.n_dn, q.t, q.u, nullptr};
             phub_op *op = nullptr;
             int rc = phub_op_hubbard(c, &desc, &op);
             if (rc) { cleanup(); return rc; }
             ops.push_back(op);
-            sd.nu = op->dim_up;
-            sd.nd = op->dim_dn;
-            sd.mu = op->up.masks;
-            sd.md = op->dn.masks;
-            sd.eu = op->up.eloc;
-            sd.ed = op->dn.eloc;
-            sd.cu = op->up.cnt;
-            sd.iu = op->up.idx;
-            sd.cfu = op->up.c32;
-            sd.cfu64 = op->up.c64;
-            sd.cd = op->dn.cnt;
-            sd.id = op->dn.idx;
-            sd.cfd = op->dn.c32;
-            sd.cfd64 = op->dn.c64;
-            sd.maxh_dn = std::max(op->dn.maxh, 1);
-            dmax = std::max<int64_t>(dmax, op->dim);
+            ops_gs[i] = op;
         }
-        h.psi_off = psi_total;
-        psi_total += std::max<int64_t>(h.sec[0].nu * h.sec[0].nd, 1);
     }
     // tasks: ground states (largest first), then chains (largest first)
     std::vector<Task> tasks;
@@ -453,7 +570,12 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
     std::sort(order.begin(), order.end(), [&](int a, int b) {
         return hc[a].sec[0].nu * hc[a].sec[0].nd > hc[b].sec[0].nu * hc[b].sec[0].nd;
     });
-    for (int i : order) tasks.push_back(Task{i, 0, 0, 0, 0, 0});
+    std::vector<int> big;   // clusters whose ground state runs on the full device
+    for (int i : order) {
+        if (hc[i].sec[0].nu * hc[i].sec[0].nd >= BIG_GS) big.push_back(i);
+        else tasks.push_back(Task{i, 0, 0, 0, 0, 0});
+    }
+    const int ngs = (int)tasks.size();
     std::vector<Task> chains;
     for (int i = 0; i < ncl; ++i)
         for (int site = 1; site <= hc[i].n; ++site)
@@ -512,15 +634,30 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
     B.gn = gn.p;
     B.binom = c->d_binom;
     B.tri = tri.p;
-    // launch 1: ground states; launch 2: chains (stream order publishes psi0)
-    for (int phase = 0; phase < 2; ++phase) {
-        B.counter = counter.p + phase;
-        B.first = phase == 0 ? 0 : ncl;
-        B.last = phase == 0 ? ncl : (int)tasks.size();
-        const int g = std::max(1, std::min(grid, B.last - B.first));
-        if (B.last > B.first) hipLaunchKernelGGL(k_batch, dim3(g), dim3(BT), 0, c->stream, B);
-        PHUB_HIP(c, hipGetLastError());
+    // launch 1: the small ground states; meanwhile the large ones on a second stream;
+    // launch 2: chains, after both (stream order and an event publish psi0 and e0)
+    B.counter = counter.p;
+    B.first = 0;
+    B.last = ngs;
+    if (ngs > 0)
+        hipLaunchKernelGGL(k_batch, dim3(std::max(1, std::min(grid, ngs))), dim3(BT), 0, c->stream, B);
+    PHUB_HIP(c, hipGetLastError());
+    if (!big.empty()) {
+        int rc = big_ground_states(c, big, ops_gs, hc, o, de0.p);
+        if (rc) {
+            // the first launch still reads the buffers freed on return
+            hipStreamSynchronize(c->stream);
+            cleanup();
+            return rc;
+        }
     }
+    B.counter = counter.p + 1;
+    B.first = ngs;
+    B.last = (int)tasks.size();
+    if (B.last > B.first)
+        hipLaunchKernelGGL(k_batch, dim3(std::max(1, std::min(grid, B.last - B.first))), dim3(BT), 0,
+                           c->stream, B);
+    PHUB_HIP(c, hipGetLastError());
     PHUB_HIP(c, hipStreamSynchronize(c->stream));
     PHUB_HIP(c, hipMemcpy(e0, de0.p, ncl * sizeof(double), hipMemcpyDeviceToHost));
     if (gf_alpha) PHUB_HIP(c, hipMemcpy(gf_alpha, ga.p, (size_t)nslots * o->gf_steps * sizeof(double), hipMemcpyDeviceToHost));

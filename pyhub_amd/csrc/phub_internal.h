@@ -160,6 +160,20 @@ __device__ inline int64_t rank_ref_dev(const uint64_t *binom, uint64_t x, int n,
 int phub_build_hop_table(phub_ctx *ctx, int n, int k, const double *T_host, bool kmajor,
                          bool all_pairs, HopTable *out);
 void phub_free_hop_table(HopTable *t);
+// Many uncompressed hop tables at once (masks, eloc, cnt, idx, c64, c32 of each),
+// in three device arenas: one host-device synchronisation for all of them.  The
+// HopTable entries point into the arenas and must not be freed one by one.
+struct HopSpec {
+    int n, k;
+    const double *T;      // host, n*n
+    bool kmajor;
+};
+struct HopBatch {
+    std::vector<HopTable> tabs;
+    void *arena[3] = {nullptr, nullptr, nullptr};
+};
+int phub_build_hop_tables(phub_ctx *ctx, const std::vector<HopSpec> &specs, HopBatch *out);
+void phub_free_hop_batch(HopBatch *b);
 int phub_build_heis(phub_ctx *ctx, int n, const double *J_host, int sign_convention,
                     HeisTables *out);
 void phub_free_heis(HeisTables *t);
