@@ -14,8 +14,8 @@
 // Stream order makes the ground states visible to the chains, so no workgroup
 // ever waits on another.  A ground-state sector of BIG_GS states or more would
 // hold one workgroup far longer than the rest of the launch, so those ground
-// states are solved instead by the full-device Lanczos (FP64 vectors, full
-// reorthogonalisation) from host threads on streams of their own while the first
+// states are solved instead by the full-device Lanczos (FP64 vectors, the same
+// three-term recurrence) from host threads on streams of their own while the first
 // launch runs; the chain launch is queued after they finish.
 #include "phub_internal.h"
 
@@ -395,9 +395,11 @@ struct DB {
 
 }  // namespace
 
-// Ground states of the clusters in `big` by phub_lanczos (FULL64, psi into the
-// cluster's psi0 slot, e0 into d_e0), beside the persistent launch already queued on
-// c->stream.  Each solve is bound by its host round trips, so up to BIG_THREADS host
+// Ground states of the clusters in `big` by phub_lanczos (PLAIN64: the three-term
+// recurrence of the persistent path, with the vectors kept and psi combined from
+// them; psi into the cluster's psi0 slot, e0 into d_e0), beside the persistent
+// launch already queued on c->stream.  Each solve is bound by its host round trips,
+// so up to BIG_THREADS host
 // threads run them at once, each on a context with a stream of its own; every thread
 // synchronises its stream before it ends, so the chain launch queued after the join
 // sees psi0 and e0.  The stopping rule is the Ritz value's change, 1e-3 of gs_tol,
@@ -426,7 +428,7 @@ static int big_ground_states(phub_ctx *c, const std::vector<int> &big,
         for (size_t j = w; j < big.size() && rc == PHUB_OK; j += nth) {
             const int i = big[j];
             phub_lanczos_opts lo{};
-            lo.mode = PHUB_LCZ_FULL64;
+            lo.mode = PHUB_LCZ_PLAIN64;
             lo.max_steps = o->gs_steps;
             lo.tol = 1e-3 * o->gs_tol;
             lo.check_every = 5;
