@@ -130,6 +130,7 @@ extern "C" void phub_op_destroy(phub_op *op) {
     phub_free_hop_table(&op->dn);
     phub_free_hop_table(&op->rup);
     phub_free_hop_table(&op->rdn);
+    if (op->dn_pairmap) hipFree(op->dn_pairmap);
     phub_free_heis(&op->h);
     if (op->d_ul) hipFree(op->d_ul);
     delete op;

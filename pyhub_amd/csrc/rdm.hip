@@ -14,6 +14,7 @@
 #include "phub_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -79,7 +80,9 @@ __global__ __launch_bounds__(256) void k_rdm_up(const int64_t nu, const int64_t 
         if (acc[i] != 0.0) atomicAdd(&out[i], acc[i]);
 }
 
-// down-spin hop expectations: one workgroup per row d, entries uniform
+// down-spin hop expectations: one workgroup per row d, entries uniform; RDM_CHUNK
+// entries per pass over the row (the bra row read once for them), their sums
+// reduced together
 __global__ __launch_bounds__(256) void k_rdm_dn(const int64_t nu, const int64_t nd,
                                                 const int32_t *__restrict__ cnt,
                                                 const int32_t *__restrict__ idx,
@@ -88,16 +91,36 @@ __global__ __launch_bounds__(256) void k_rdm_dn(const int64_t nu, const int64_t 
                                                 const double *__restrict__ bra,
                                                 const double *__restrict__ ket,
                                                 double *__restrict__ out) {
-    __shared__ double sh[4];
+    __shared__ double sh[4 * RDM_CHUNK];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int64_t d = blockIdx.x; d < nd; d += gridDim.x) {
         const int c = cnt[d];
-        for (int k = 0; k < c; ++k) {
-            const size_t o = (size_t)d * maxh + k;
-            const int64_t dp = idx[o];
-            double s = 0.0;
-            for (int64_t u = threadIdx.x; u < nu; u += blockDim.x) s += bra[d * nu + u] * ket[dp * nu + u];
-            s = block_sum(s, sh);
-            if (threadIdx.x == 0) atomicAdd(&out[pair[o]], sgn[o] * s);
+        for (int k0 = 0; k0 < c; k0 += RDM_CHUNK) {
+            int64_t dp[RDM_CHUNK];
+            double a[RDM_CHUNK];
+#pragma unroll
+            for (int e = 0; e < RDM_CHUNK; ++e) {
+                dp[e] = (k0 + e < c) ? (int64_t)idx[(size_t)d * maxh + k0 + e] : d;
+                a[e] = 0.0;
+            }
+            for (int64_t u = threadIdx.x; u < nu; u += blockDim.x) {
+                const double b = bra[d * nu + u];
+#pragma unroll
+                for (int e = 0; e < RDM_CHUNK; ++e) a[e] += b * ket[dp[e] * nu + u];
+            }
+#pragma unroll
+            for (int e = 0; e < RDM_CHUNK; ++e) a[e] = wsum(a[e]);
+            __syncthreads();   // sh is free (the previous chunk's readers are done)
+            if (lane == 0)
+#pragma unroll
+                for (int e = 0; e < RDM_CHUNK; ++e) sh[wave * RDM_CHUNK + e] = a[e];
+            __syncthreads();
+            if (threadIdx.x < RDM_CHUNK && k0 + (int)threadIdx.x < c) {
+                const int e = threadIdx.x;
+                const double s = sh[e] + sh[RDM_CHUNK + e] + sh[2 * RDM_CHUNK + e] + sh[3 * RDM_CHUNK + e];
+                const size_t o = (size_t)d * maxh + k0 + e;
+                atomicAdd(&out[pair[o]], sgn[o] * s);
+            }
         }
     }
 }
@@ -130,6 +153,29 @@ __global__ __launch_bounds__(256) void k_rdm_diag(const int64_t nu, const int64_
     if (u < nu) atomicAdd(&q[u], qs);
 }
 
+// the down-spin pair map: c^dag_J c_K applied to every down state, for every
+// ordered pair p = (J-1)*n + (K-1) (J == K: -1)
+__global__ __launch_bounds__(256) void k_pairmap(const uint64_t *__restrict__ binom, int n, int k,
+                                                 int64_t nd, const uint32_t *__restrict__ md,
+                                                 int32_t *__restrict__ pm) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n * n * nd) return;
+    const int p = (int)(i / nd);
+    const int64_t d = i - (int64_t)p * nd;
+    const int J = p / n + 1, K = p % n + 1;
+    const int bJ = n - J, bK = n - K;
+    const uint32_t xd = md[d];
+    int32_t e = -1;
+    if (J != K && ((xd >> bK) & 1u) && !((xd >> bJ) & 1u)) {
+        const int lo = min(bJ, bK), hi = max(bJ, bK);
+        const uint32_t between = (hi - lo > 1) ? (((1u << (hi - lo - 1)) - 1u) << (lo + 1)) : 0u;
+        const uint32_t xn = xd ^ (1u << bK) ^ (1u << bJ);
+        const int64_t dp = rank_ref_dev(binom, xn, n, k);
+        e = (int32_t)((dp << 1) | (__popc(xd & between) & 1));
+    }
+    pm[i] = e;
+}
+
 // spin-flip part of J_SPIN (static_rq.f90:345-383): for J != K,
 //   F[J][K] = sum ac*ac2 * bra[d][u] * ket[d'][u'], u' = c^dag_K c_J u (up),
 //   d' = c^dag_J c_K d (down).  Accumulated in LDS per ordered pair.
@@ -143,7 +189,8 @@ __global__ __launch_bounds__(256) void k_rdm_flip(const uint64_t *__restrict__ b
                                                   const int16_t *__restrict__ pair,
                                                   const double *__restrict__ bra,
                                                   const double *__restrict__ ket,
-                                                  double *__restrict__ out, int ncb) {
+                                                  double *__restrict__ out, int ncb,
+                                                  const int32_t *__restrict__ pm) {
     extern __shared__ double acc[];   // n*n
     for (int i = threadIdx.x; i < n * n; i += blockDim.x) acc[i] = 0.0;
     __syncthreads();
@@ -163,6 +210,14 @@ __global__ __launch_bounds__(256) void k_rdm_flip(const uint64_t *__restrict__ b
             const int lo = min(bJ, bK), hi = max(bJ, bK);
             const uint32_t between = (hi - lo > 1) ? (((1u << (hi - lo - 1)) - 1u) << (lo + 1)) : 0u;
             double s = 0.0;
+            if (pm) {   // the down hop from the pair map: one load per row
+                const int32_t *pmp = pm + (size_t)p * nd;
+                for (int64_t d = d0; d < d1; ++d) {
+                    const int32_t e = pmp[d];
+                    if (e >= 0)
+                        s += ((e & 1) ? -1.0 : 1.0) * bra[d * nu + u] * ket[(int64_t)(e >> 1) * nu + up2];
+                }
+            } else
             for (int64_t d = d0; d < d1; ++d) {
                 const uint32_t xd = md[d];
                 if (((xd >> bK) & 1u) && !((xd >> bJ) & 1u)) {
@@ -242,11 +297,21 @@ static int hubbard_raw(phub_ctx *c, phub_op *op, const double *bra, const double
                            op->rdn.pair, op->rdn.maxh, bra, ket, dout.p + nn);
     hipLaunchKernelGGL(k_rdm_diag, dim3(grid), dim3(256), 0, c->stream, nu, nd, n, op->up.masks,
                        bra, ket, dR.p, dp.p, dq.p, ncb);
-    if (want_flip && op->rup.maxh > 0)
+    if (want_flip && op->rup.maxh > 0) {
+        // the pair map when it fits 1 GiB and its entries 31 bits; else ranks per row
+        const size_t pm_bytes = (size_t)nn * (size_t)nd * sizeof(int32_t);
+        const char *env = std::getenv("PHUB_RDM_PAIRMAP");   // "0": ranks per row
+        const bool use_pm = !(env && env[0] == '0');
+        if (use_pm && !op->dn_pairmap && pm_bytes <= (size_t(1) << 30) && nd < (1ll << 30)) {
+            PHUB_HIP(c, hipMalloc(&op->dn_pairmap, pm_bytes));
+            hipLaunchKernelGGL(k_pairmap, dim3((unsigned)(((size_t)nn * nd + 255) / 256)), dim3(256), 0,
+                               c->stream, c->d_binom, n, op->ndn, nd, op->dn.masks, op->dn_pairmap);
+        }
         hipLaunchKernelGGL(k_rdm_flip, dim3(grid), dim3(256), nn * sizeof(double), c->stream,
                            c->d_binom, nu, nd, n, op->ndn, op->up.masks, op->dn.masks,
                            op->rup.cnt, op->rup.idx, op->rup.c64, op->rup.pair, bra, ket,
-                           dout.p + 2 * nn, ncb);
+                           dout.p + 2 * nn, ncb, use_pm ? (const int32_t *)op->dn_pairmap : nullptr);
+    }
     PHUB_HIP(c, hipGetLastError());
     std::vector<double> hout(3 * nn), hR(nd * n), hp(nd), hq(nu);
     std::vector<uint64_t> mup(nu), mdn(nd);

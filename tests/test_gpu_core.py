@@ -300,10 +300,14 @@ def test_lanczos_excited_states(ctx, mode):
     assert r1.psis.shape == (1, op.dim)
 
 
+@pytest.mark.parametrize("pairmap", ["1", "0"])
 @pytest.mark.parametrize("n,nu,nd,seed", [(6, 3, 2, 20), (7, 3, 3, 21), (8, 4, 3, 22), (6, 3, 3, 23)])
-def test_rdm_transition_vs_oracle(ctx, n, nu, nd, seed):
-    """ONE_RDM, NI and J_SPIN for bra != ket (all loops of static_rq.f90)."""
+def test_rdm_transition_vs_oracle(ctx, monkeypatch, n, nu, nd, seed, pairmap):
+    """ONE_RDM, NI and J_SPIN for bra != ket (all loops of static_rq.f90); the
+    spin-flip term through the down-spin pair map and through per-row ranks
+    (PHUB_RDM_PAIRMAP=0, the path for maps beyond 1 GiB)."""
     from pyhub_amd.engine import HubbardOp
+    monkeypatch.setenv("PHUB_RDM_PAIRMAP", pairmap)
     rng = np.random.default_rng(seed)
     T, U = rand_cluster(rng, n)
     op = HubbardOp(ctx, n, nu, nd, T, U)
