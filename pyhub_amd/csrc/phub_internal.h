@@ -110,7 +110,7 @@ struct phub_op {
     HopTable up, dn;               // up KMAJOR, dn ROWMAJOR (hop tables of T)
     HopTable rup, rdn;             // all-pairs tables for reduced quantities (lazy)
     // down-spin pair map (lazy, when it fits 1 GiB): for the ordered pair p = (J-1)*n +
-    // (K-1) and down state d, (d' << 1) | sign of c^dag_J c_K d, or -1; [p][d]
+    // (K-1) and down state d, (d' << 1) | sign of c^dag_J c_K d, or -1; [d][p]
     int32_t *dn_pairmap = nullptr;
     double U[PHUB_MAXN];
     int u_uniform = 1;

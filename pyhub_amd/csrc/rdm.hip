@@ -125,43 +125,73 @@ __global__ __launch_bounds__(256) void k_rdm_dn(const int64_t nu, const int64_t 
     }
 }
 
-// diagonal pass: R[d][J] (bit position b), p[d] via block reduction per row chunk;
-// q[u] accumulated per lane over the rows of the workgroup.
-__global__ __launch_bounds__(256) void k_rdm_diag(const int64_t nu, const int64_t nd, int n,
-                                                  const uint32_t *__restrict__ mu,
-                                                  const double *__restrict__ bra,
-                                                  const double *__restrict__ ket,
-                                                  double *__restrict__ R, double *__restrict__ p,
-                                                  double *__restrict__ q, int ncb) {
-    __shared__ double sh[4];
+// diagonal pass, rows: one workgroup per row d gives p[d] = sum_u w[d][u] and
+// R[d][b] = sum_u w[d][u] n_b(u) (bit position b), w = bra * ket, all n + 1 sums
+// reduced together
+__global__ __launch_bounds__(256) void k_rdm_diag_rows(const int64_t nu, const int64_t nd, int n,
+                                                       const uint32_t *__restrict__ mu,
+                                                       const double *__restrict__ bra,
+                                                       const double *__restrict__ ket,
+                                                       double *__restrict__ R, double *__restrict__ p) {
+    __shared__ double sh[4][33];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int64_t d = blockIdx.x; d < nd; d += gridDim.x) {
+        double ps = 0.0, r[32];
+#pragma unroll
+        for (int b = 0; b < 32; ++b) r[b] = 0.0;
+        for (int64_t u = threadIdx.x; u < nu; u += blockDim.x) {
+            const double w = bra[d * nu + u] * ket[d * nu + u];
+            const uint32_t x = mu[u];
+            ps += w;
+#pragma unroll
+            for (int b = 0; b < 32; ++b) r[b] += ((x >> b) & 1u) ? w : 0.0;
+        }
+        ps = wsum(ps);
+#pragma unroll
+        for (int b = 0; b < 32; ++b)
+            if (b < n) r[b] = wsum(r[b]);
+        __syncthreads();   // sh is free (the previous row's readers are done)
+        if (lane == 0) {
+            sh[wave][32] = ps;
+#pragma unroll
+            for (int b = 0; b < 32; ++b) sh[wave][b] = r[b];
+        }
+        __syncthreads();
+        if ((int)threadIdx.x <= n) {
+            const int b = threadIdx.x == (unsigned)n ? 32 : (int)threadIdx.x;
+            const double v = sh[0][b] + sh[1][b] + sh[2][b] + sh[3][b];
+            if (b == 32) p[d] = v;
+            else R[d * n + b] = v;
+        }
+    }
+}
+
+// diagonal pass, columns: q[u] = sum_d w[d][u], lanes over u, a chunk of rows per
+// workgroup
+__global__ __launch_bounds__(256) void k_rdm_diag_cols(const int64_t nu, const int64_t nd,
+                                                       const double *__restrict__ bra,
+                                                       const double *__restrict__ ket,
+                                                       double *__restrict__ q, int ncb) {
     const int cb = blockIdx.x % ncb;
     const int64_t d0 = (int64_t)(blockIdx.x / ncb) * RDM_ROWS;
     const int64_t d1 = min(d0 + RDM_ROWS, nd);
     const int64_t u = (int64_t)cb * 256 + threadIdx.x;
-    const uint32_t x = (u < nu) ? mu[u] : 0u;
+    if (u >= nu) return;
     double qs = 0.0;
-    for (int64_t d = d0; d < d1; ++d) {
-        const double w = (u < nu) ? bra[d * nu + u] * ket[d * nu + u] : 0.0;
-        qs += w;
-        const double ps = block_sum(w, sh);
-        if (threadIdx.x == 0) atomicAdd(&p[d], ps);
-        for (int b = 0; b < n; ++b) {
-            const double r = block_sum(((x >> b) & 1u) ? w : 0.0, sh);
-            if (threadIdx.x == 0) atomicAdd(&R[d * n + b], r);
-        }
-    }
-    if (u < nu) atomicAdd(&q[u], qs);
+    for (int64_t d = d0; d < d1; ++d) qs += bra[d * nu + u] * ket[d * nu + u];
+    atomicAdd(&q[u], qs);
 }
 
 // the down-spin pair map: c^dag_J c_K applied to every down state, for every
-// ordered pair p = (J-1)*n + (K-1) (J == K: -1)
+// ordered pair p = (J-1)*n + (K-1) (J == K: -1); layout [d][p], so that the lanes
+// of a workgroup (different pairs, the same rows) read the same few rows
 __global__ __launch_bounds__(256) void k_pairmap(const uint64_t *__restrict__ binom, int n, int k,
                                                  int64_t nd, const uint32_t *__restrict__ md,
                                                  int32_t *__restrict__ pm) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)n * n * nd) return;
-    const int p = (int)(i / nd);
-    const int64_t d = i - (int64_t)p * nd;
+    const int64_t d = i / (n * n);
+    const int p = (int)(i - d * n * n);
     const int J = p / n + 1, K = p % n + 1;
     const int bJ = n - J, bK = n - K;
     const uint32_t xd = md[d];
@@ -211,9 +241,8 @@ __global__ __launch_bounds__(256) void k_rdm_flip(const uint64_t *__restrict__ b
             const uint32_t between = (hi - lo > 1) ? (((1u << (hi - lo - 1)) - 1u) << (lo + 1)) : 0u;
             double s = 0.0;
             if (pm) {   // the down hop from the pair map: one load per row
-                const int32_t *pmp = pm + (size_t)p * nd;
                 for (int64_t d = d0; d < d1; ++d) {
-                    const int32_t e = pmp[d];
+                    const int32_t e = pm[(size_t)d * n * n + p];
                     if (e >= 0)
                         s += ((e & 1) ? -1.0 : 1.0) * bra[d * nu + u] * ket[(int64_t)(e >> 1) * nu + up2];
                 }
@@ -295,8 +324,10 @@ static int hubbard_raw(phub_ctx *c, phub_op *op, const double *bra, const double
         hipLaunchKernelGGL(k_rdm_dn, dim3((unsigned)std::min<int64_t>(nd, 8192)), dim3(256), 0,
                            c->stream, nu, nd, op->rdn.cnt, op->rdn.idx, op->rdn.c64,
                            op->rdn.pair, op->rdn.maxh, bra, ket, dout.p + nn);
-    hipLaunchKernelGGL(k_rdm_diag, dim3(grid), dim3(256), 0, c->stream, nu, nd, n, op->up.masks,
-                       bra, ket, dR.p, dp.p, dq.p, ncb);
+    hipLaunchKernelGGL(k_rdm_diag_rows, dim3((unsigned)std::min<int64_t>(nd, 16384)), dim3(256), 0,
+                       c->stream, nu, nd, n, op->up.masks, bra, ket, dR.p, dp.p);
+    hipLaunchKernelGGL(k_rdm_diag_cols, dim3(grid), dim3(256), 0, c->stream, nu, nd, bra, ket, dq.p,
+                       ncb);
     if (want_flip && op->rup.maxh > 0) {
         // the pair map when it fits 1 GiB and its entries 31 bits; else ranks per row
         const size_t pm_bytes = (size_t)nn * (size_t)nd * sizeof(int32_t);
