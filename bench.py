@@ -63,7 +63,8 @@ def main():
     ap.add_argument("--lanczos-max", type=int, default=300)
     ap.add_argument("--calib", action="store_true", help="add a 1 GiB device copy (counter calibration)")
     ap.add_argument("--workload", default="hubbard4x4",
-                    choices=["hubbard4x4", "heis6x6", "batch256", "band4x4", "u4", "ops4x4"])
+                    choices=["hubbard4x4", "heis6x6", "batch256", "band4x4", "u4", "ops4x4", "grad64"])
+    ap.add_argument("--sets", type=int, default=64, help="parameter sets (grad64)")
     ap.add_argument("--band-steps", type=int, default=192, help="band Lanczos steps (band4x4)")
     ap.add_argument("--lx", type=int, default=6, help="Heisenberg torus width")
     ap.add_argument("--ly", type=int, default=6, help="Heisenberg torus height")
@@ -94,6 +95,9 @@ def main():
         return
     if args.workload == "ops4x4":
         _ops_bench(args, ctx, torch)
+        return
+    if args.workload == "grad64":
+        _grad_bench(args, ctx, torch)
         return
     Lx = Ly = 4
     n = Lx * Ly
@@ -580,6 +584,82 @@ def _ops_bench(args, ctx, torch):
     }
     if not args.no_cpu:
         out["cpu_baseline"] = _ops_cpu_baseline(T, D, nprod)
+    print(json.dumps(out), flush=True)
+
+
+def _grad_bench(args, ctx, torch):
+    """Differentiation configuration: the 3x4 open cluster at half filling (17 bonds,
+    D = 853,776), parameter sets p = 0..63 from default_rng([4, p]) (t_b ~ U(0.8, 1.2),
+    U ~ U(4, 8), eps_i ~ N(0, 0.2)), the 1-RDM target from the parameters perturbed by
+    N(0, 0.05) with default_rng([4, p, 1]).  Per set: the target's ground state and
+    1-RDM, the ground state, E0 with its Hellmann-Feynman gradient, and the loss
+    L = sum_s ||gamma_s - target_s||_F^2 with its adjoint-MINRES gradient.  Set 0's
+    gradient is checked against central differences (h = 1e-5) on two bonds."""
+    from pyhub_amd.grad import ClusterSpec, lattice_bonds, rdm_loss_and_grad, solve
+    spec = ClusterSpec(n=12, bonds=lattice_bonds(4, 3), n_up=6, n_dn=6, lanczos_tol=1e-12)
+    nsets = max(1, args.sets)
+
+    def params(p):
+        rng = np.random.default_rng([4, p])
+        t = rng.uniform(0.8, 1.2, 17)
+        U = rng.uniform(4, 8)
+        eps = rng.normal(0, 0.2, 12)
+        r1 = np.random.default_rng([4, p, 1])
+        return t, eps, U, (t + r1.normal(0, 0.05, 17), eps + r1.normal(0, 0.05, 12),
+                           U + r1.normal(0, 0.05))
+
+    t, eps, U, pert = params(0)
+    solve(spec, t, eps, U)   # warm-up: tables, tuner
+    torch.cuda.synchronize()
+    res = []
+    t0 = time.perf_counter()
+    for p in range(nsets):
+        t, eps, U, (tq, eq, uq) = params(p)
+        op_t, _, psi_t = solve(spec, tq, eq, uq)
+        tg_up, tg_dn = op_t.one_rdm_spin(psi_t)
+        del op_t, psi_t
+        L_val, gt, ge, gu, info = rdm_loss_and_grad(spec, t, eps, U, tg_up, tg_dn,
+                                                    want_energy_grad=True)
+        res.append({"L": L_val, "e0": info["e0"], "minres_iters": info["minres_iters"],
+                    "gt0": float(gt[0]), "gt9": float(gt[9]), "tg": (tg_up, tg_dn),
+                    "dE_dU": float(info["energy_grad"][2])})
+        if p % 8 == 7:
+            print(f"# grad64: {p + 1} sets, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                  flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # central differences of set 0 on bonds 0 and 9
+    t, eps, U, _ = params(0)
+    tg_up, tg_dn = res[0]["tg"]
+
+    def loss(tq):
+        op, _, psi = solve(spec, tq, eps, U)
+        a, b = op.one_rdm_spin(psi)
+        return float(np.sum((a - tg_up) ** 2) + np.sum((b - tg_dn) ** 2))
+
+    fd_err = 0.0
+    for b, key in ((0, "gt0"), (9, "gt9")):
+        tp, tm = t.copy(), t.copy()
+        tp[b] += 1e-5
+        tm[b] -= 1e-5
+        fd = (loss(tp) - loss(tm)) / 2e-5
+        fd_err = max(fd_err, abs(fd - res[0][key]) / max(1e-12, abs(fd)))
+    rate = nsets / wall
+    print(f"# grad64: {nsets} sets in {wall:.2f} s ({rate:.2f} sets/s), mean MINRES iterations "
+          f"{np.mean([r['minres_iters'] for r in res]):.0f}, FD rel. error {fd_err:.2e}",
+          file=sys.stderr, flush=True)
+    out = {
+        "metric": "parameter sets per second (E0 + HF gradient + adjoint 1-RDM loss gradient)",
+        "value": round(rate, 3), "unit": "sets/s", "n_gpus": 1, "steps": nsets, "warmup": 1,
+        "ms_per_step": round(wall / nsets * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp64",
+        "data": "synthetic parameter sets from default_rng([4, p]), targets from default_rng([4, p, 1])",
+        "config": {"workload": "3x4 open Hubbard, N_up=N_dn=6, 17 bonds, 64 parameter sets",
+                   "dim": 853776, "parallelism": "single GPU"},
+        "grad": {"seconds": wall, "mean_minres_iters": float(np.mean([r["minres_iters"] for r in res])),
+                 "fd_rel_err_set0_bonds_0_9": fd_err, "loss_set0": res[0]["L"],
+                 "e0_set0": res[0]["e0"]},
+    }
     print(json.dumps(out), flush=True)
 
 

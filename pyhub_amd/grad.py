@@ -81,8 +81,11 @@ def hellmann_feynman(spec: ClusterSpec, t, eps, U, ctx=None):
     return e0, gt, ge, gu
 
 
-def rdm_loss_and_grad(spec: ClusterSpec, t, eps, U, target_up, target_dn, ctx=None):
-    """L = sum_s ||gamma_s - target_s||_F^2 and dL/d(t, eps, U) by the adjoint solve."""
+def rdm_loss_and_grad(spec: ClusterSpec, t, eps, U, target_up, target_dn, ctx=None,
+                      want_energy_grad=False):
+    """L = sum_s ||gamma_s - target_s||_F^2 and dL/d(t, eps, U) by the adjoint solve.
+    With want_energy_grad, info["energy_grad"] holds (dE/dt, dE/deps, dE/dU) of the same
+    ground state (Hellmann-Feynman, one more pass over psi)."""
     ctx = ctx or Context.default()
     op, e0, psi = solve(spec, t, eps, U, ctx)
     n = spec.n
@@ -100,6 +103,9 @@ def rdm_loss_and_grad(spec: ClusterSpec, t, eps, U, target_up, target_dn, ctx=No
     gt, ge, gu = _param_grads(spec, -gT, -gU, np.ndim(U) == 0)
     info = {"minres_iters": iters, "minres_rel_res": rr, "e0": e0,
             "gamma_up": gam_up, "gamma_dn": gam_dn}
+    if want_energy_grad:
+        hT, hU = op.hf_grad(psi)
+        info["energy_grad"] = _param_grads(spec, hT, hU, np.ndim(U) == 0)
     return L_val, gt, ge, gu, info
 
 

@@ -63,8 +63,14 @@ def test_rdm_loss_adjoint_vs_fd(ctx, nup, ndn):
     tt, et, Ut = t + rng.normal(0, 0.05, len(t)), eps + rng.normal(0, 0.05, spec.n), U + rng.normal(0, 0.05)
     op_t, _, psi_t = solve(spec, tt, et, Ut)
     tg_up, tg_dn = op_t.one_rdm_spin(psi_t)
-    L0, gt, ge, gu, info = rdm_loss_and_grad(spec, t, eps, U, tg_up, tg_dn)
+    L0, gt, ge, gu, info = rdm_loss_and_grad(spec, t, eps, U, tg_up, tg_dn, want_energy_grad=True)
     assert info["minres_rel_res"] < 1e-9
+    # the energy gradient of the same ground state equals the Hellmann-Feynman path
+    from pyhub_amd.grad import hellmann_feynman
+    _, hgt, hge, hgu = hellmann_feynman(spec, t, eps, U)
+    egt, ege, egu = info["energy_grad"]
+    assert np.max(np.abs(egt - hgt)) < 1e-8 and np.max(np.abs(ege - hge)) < 1e-8
+    assert abs(egu - hgu) < 1e-8
     # spin-resolved RDMs equal the oracle's ONE_RDM of the same psi
     m = O.HubbardModel(spec.n, nup, ndn, spec.T(t, eps), U)
     ev, V = np.linalg.eigh(m.dense())
