@@ -182,6 +182,55 @@ __global__ __launch_bounds__(256) void k_rdm_diag_cols(const int64_t nu, const i
     atomicAdd(&q[u], qs);
 }
 
+// Diagonal quantities from the row sums p, column sums q and R[d][b] (site J is bit
+// n - J), accumulated in LDS and added to out:
+//   [0, n) densities up, [n, 2n) densities down, [2n, 3n) NI, then n x n each of
+//   <n_J,up n_K,up>, <n_K,up n_J,dn> at [(K-1) n + J-1], <n_J,dn n_K,dn>
+__global__ __launch_bounds__(256) void k_rdm_assemble(int n, int64_t nu, int64_t nd,
+                                                      const uint32_t *__restrict__ mu,
+                                                      const uint32_t *__restrict__ md,
+                                                      const double *__restrict__ q,
+                                                      const double *__restrict__ p,
+                                                      const double *__restrict__ R,
+                                                      double *__restrict__ out) {
+    extern __shared__ double acc[];
+    const int nn = n * n, tot = 3 * n + 3 * nn;
+    double *dup = acc, *ddn = acc + n, *ni = acc + 2 * n;
+    double *uu = acc + 3 * n, *ud = uu + nn, *dd = ud + nn;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) acc[i] = 0.0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nu + nd;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (i < nu) {
+            const uint32_t x = mu[i];
+            const double w = q[i];
+            for (int J = 1; J <= n; ++J) {
+                if (!((x >> (n - J)) & 1u)) continue;
+                atomicAdd(&dup[J - 1], w);
+                for (int K = 1; K <= n; ++K)
+                    if ((x >> (n - K)) & 1u) atomicAdd(&uu[(J - 1) * n + (K - 1)], w);
+            }
+        } else {
+            const int64_t d = i - nu;
+            const uint32_t x = md[d];
+            const double w = p[d];
+            const double *Rd = R + d * n;
+            for (int J = 1; J <= n; ++J) {
+                if (!((x >> (n - J)) & 1u)) continue;
+                atomicAdd(&ddn[J - 1], w);
+                atomicAdd(&ni[J - 1], Rd[n - J]);
+                for (int K = 1; K <= n; ++K) {
+                    if ((x >> (n - K)) & 1u) atomicAdd(&dd[(J - 1) * n + (K - 1)], w);
+                    atomicAdd(&ud[(K - 1) * n + (J - 1)], Rd[n - K]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < tot; i += blockDim.x)
+        if (acc[i] != 0.0) atomicAdd(&out[i], acc[i]);
+}
+
 // the down-spin pair map: c^dag_J c_K applied to every down state, for every
 // ordered pair p = (J-1)*n + (K-1) (J == K: -1); layout [d][p], so that the lanes
 // of a workgroup (different pairs, the same rows) read the same few rows
@@ -343,49 +392,28 @@ static int hubbard_raw(phub_ctx *c, phub_op *op, const double *bra, const double
                            op->rup.cnt, op->rup.idx, op->rup.c64, op->rup.pair, bra, ket,
                            dout.p + 2 * nn, ncb, use_pm ? (const int32_t *)op->dn_pairmap : nullptr);
     }
+    // the diagonal quantities assembled on the device (site J <-> bit n-J)
+    const int ndiag = 3 * n + 3 * nn;
+    DBuf<double> ddiag;
+    PHUB_HIP(c, hipMalloc(&ddiag.p, ndiag * sizeof(double)));
+    PHUB_HIP(c, hipMemsetAsync(ddiag.p, 0, ndiag * sizeof(double), c->stream));
+    hipLaunchKernelGGL(k_rdm_assemble, dim3((unsigned)std::min<int64_t>((nu + nd + 255) / 256, 256)),
+                       dim3(256), ndiag * sizeof(double), c->stream, n, nu, nd, op->up.masks,
+                       op->dn.masks, dq.p, dp.p, dR.p, ddiag.p);
     PHUB_HIP(c, hipGetLastError());
-    std::vector<double> hout(3 * nn), hR(nd * n), hp(nd), hq(nu);
-    std::vector<uint64_t> mup(nu), mdn(nd);
-    std::vector<uint32_t> m32u(nu), m32d(nd);
+    std::vector<double> hout(3 * nn), hdiag(ndiag);
     PHUB_HIP(c, hipMemcpyAsync(hout.data(), dout.p, 3 * nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    PHUB_HIP(c, hipMemcpyAsync(hR.data(), dR.p, nd * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    PHUB_HIP(c, hipMemcpyAsync(hp.data(), dp.p, nd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    PHUB_HIP(c, hipMemcpyAsync(hq.data(), dq.p, nu * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    PHUB_HIP(c, hipMemcpyAsync(m32u.data(), op->up.masks, nu * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    PHUB_HIP(c, hipMemcpyAsync(m32d.data(), op->dn.masks, nd * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    PHUB_HIP(c, hipMemcpyAsync(hdiag.data(), ddiag.p, ndiag * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     PHUB_HIP(c, hipStreamSynchronize(c->stream));
     r.hop_up.assign(hout.begin(), hout.begin() + nn);
     r.hop_dn.assign(hout.begin() + nn, hout.begin() + 2 * nn);
     r.flip.assign(hout.begin() + 2 * nn, hout.end());
-    r.dens_up.assign(n, 0.0);
-    r.dens_dn.assign(n, 0.0);
-    r.ni.assign(n, 0.0);
-    r.nn_uu.assign(nn, 0.0);
-    r.nn_ud.assign(nn, 0.0);
-    r.nn_dd.assign(nn, 0.0);
-    // site J <-> bit n-J
-    for (int64_t u = 0; u < nu; ++u) {
-        const uint32_t x = m32u[u];
-        for (int J = 1; J <= n; ++J) {
-            if (!((x >> (n - J)) & 1u)) continue;
-            r.dens_up[J - 1] += hq[u];
-            for (int K = 1; K <= n; ++K)
-                if ((x >> (n - K)) & 1u) r.nn_uu[(J - 1) * n + (K - 1)] += hq[u];
-        }
-    }
-    for (int64_t d = 0; d < nd; ++d) {
-        const uint32_t x = m32d[d];
-        for (int J = 1; J <= n; ++J) {
-            if (!((x >> (n - J)) & 1u)) continue;
-            r.dens_dn[J - 1] += hp[d];
-            r.ni[J - 1] += hR[d * n + (n - J)];
-            for (int K = 1; K <= n; ++K) {
-                if ((x >> (n - K)) & 1u) r.nn_dd[(J - 1) * n + (K - 1)] += hp[d];
-                // <n_K,up n_J,dn>
-                r.nn_ud[(K - 1) * n + (J - 1)] += hR[d * n + (n - K)];
-            }
-        }
-    }
+    r.dens_up.assign(hdiag.begin(), hdiag.begin() + n);
+    r.dens_dn.assign(hdiag.begin() + n, hdiag.begin() + 2 * n);
+    r.ni.assign(hdiag.begin() + 2 * n, hdiag.begin() + 3 * n);
+    r.nn_uu.assign(hdiag.begin() + 3 * n, hdiag.begin() + 3 * n + nn);
+    r.nn_ud.assign(hdiag.begin() + 3 * n + nn, hdiag.begin() + 3 * n + 2 * nn);
+    r.nn_dd.assign(hdiag.begin() + 3 * n + 2 * nn, hdiag.end());
     return PHUB_OK;
 }
 
