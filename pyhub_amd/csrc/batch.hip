@@ -140,12 +140,44 @@ __device__ void wg_hv(const ClusterDev &c, const SectorDev &s, const V *__restri
             }
         }
         V acc = (V)dg * x[e];
+        // hops four at a time: their table loads, then their gathers, are in flight
+        // together (the loop is bound by memory latency, not by issue)
         const int cu = s.cu[u];
-        for (int k = 0; k < cu; ++k)
-            acc += cf_up<V>(s, (size_t)k * s.nu + u) * x[d * s.nu + s.iu[(size_t)k * s.nu + u]];
+        const V *xr = x + d * s.nu;
+        int k = 0;
+        for (; k + 4 <= cu; k += 4) {
+            int32_t j[4];
+            V cf[4], g[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                j[q] = s.iu[(size_t)(k + q) * s.nu + u];
+                cf[q] = cf_up<V>(s, (size_t)(k + q) * s.nu + u);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) g[q] = xr[j[q]];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc += cf[q] * g[q];
+        }
+        for (; k < cu; ++k)
+            acc += cf_up<V>(s, (size_t)k * s.nu + u) * xr[s.iu[(size_t)k * s.nu + u]];
         const int cd = s.cd[d];
-        for (int k = 0; k < cd; ++k)
-            acc += cf_dn<V>(s, (size_t)d * s.maxh_dn + k) * x[(int64_t)s.id[(size_t)d * s.maxh_dn + k] * s.nu + u];
+        const int32_t *idr = s.id + (size_t)d * s.maxh_dn;
+        k = 0;
+        for (; k + 4 <= cd; k += 4) {
+            int32_t j[4];
+            V cf[4], g[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                j[q] = idr[k + q];
+                cf[q] = cf_dn<V>(s, (size_t)d * s.maxh_dn + k + q);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) g[q] = x[(int64_t)j[q] * s.nu + u];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc += cf[q] * g[q];
+        }
+        for (; k < cd; ++k)
+            acc += cf_dn<V>(s, (size_t)d * s.maxh_dn + k) * x[(int64_t)idr[k] * s.nu + u];
         double v = scale * (double)acc;
         if (xp) v -= bprev * (double)xp[e];
         const V vt = (V)v;
