@@ -340,6 +340,9 @@ __device__ double wg_excite(const BatchDev &B, const ClusterDev &c, const Sector
     return s;
 }
 
+// KIND 0: ground-state tasks only (FP64); KIND 1: chain tasks only (FP32).  Two
+// instantiations, so that each gets the registers of its own path.
+template <int KIND>
 __global__ __launch_bounds__(BT) void k_batch(BatchDev B) {
     __shared__ double sh[2 * BT / 64];
     __shared__ int task_sh;
@@ -359,7 +362,7 @@ __global__ __launch_bounds__(BT) void k_batch(BatchDev B) {
         const ClusterDev &c = B.cl[tk.cluster];
         const SectorDev &g = c.sec[0];
         const int64_t Dg = g.nu * g.nd;
-        if (tk.kind == 0) {
+        if constexpr (KIND == 0) {
             // ground state in FP64 vectors: random start, plain recurrence, second pass for psi0
             double n2 = 0.0, dummy = 0.0;
             for (int64_t e = threadIdx.x; e < Dg; e += BT) {
@@ -674,7 +677,7 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
     B.first = 0;
     B.last = ngs;
     if (ngs > 0)
-        hipLaunchKernelGGL(k_batch, dim3(std::max(1, std::min(grid, ngs))), dim3(BT), 0, c->stream, B);
+        hipLaunchKernelGGL(k_batch<0>, dim3(std::max(1, std::min(grid, ngs))), dim3(BT), 0, c->stream, B);
     PHUB_HIP(c, hipGetLastError());
     if (!big.empty()) {
         int rc = big_ground_states(c, big, ops_gs, hc, o, de0.p);
@@ -689,7 +692,7 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
     B.first = ngs;
     B.last = (int)tasks.size();
     if (B.last > B.first)
-        hipLaunchKernelGGL(k_batch, dim3(std::max(1, std::min(grid, B.last - B.first))), dim3(BT), 0,
+        hipLaunchKernelGGL(k_batch<1>, dim3(std::max(1, std::min(grid, B.last - B.first))), dim3(BT), 0,
                            c->stream, B);
     PHUB_HIP(c, hipGetLastError());
     PHUB_HIP(c, hipStreamSynchronize(c->stream));
