@@ -64,6 +64,7 @@ extern "C" void phub_ctx_destroy(phub_ctx *c) {
     hipSetDevice(c->device);
     if (c->d_binom) hipFree(c->d_binom);
     if (c->d_red) hipFree(c->d_red);
+    for (void *p : c->deferred) hipFree(p);
     if (c->h_red) hipHostFree(c->h_red);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -82,7 +83,7 @@ extern "C" double phub_last_kernel_ms(const phub_ctx *c) { return c ? c->last_ms
 
 int phub_ensure_red(phub_ctx *c, size_t nd) {
     if (c->red_cap < nd) {
-        if (c->d_red) hipFree(c->d_red);
+        phub_dfree(c, c->d_red);
         c->d_red = nullptr;
         c->red_cap = 0;
         PHUB_HIP(c, hipMalloc(&c->d_red, nd * sizeof(double)));

@@ -433,74 +433,120 @@ struct DB {
 // Ground states of the clusters in `big` by phub_lanczos (PLAIN64: the three-term
 // recurrence of the persistent path, with the vectors kept and psi combined from
 // them; psi into the cluster's psi0 slot, e0 into d_e0), beside the persistent
-// launch already queued on c->stream.  Each solve is bound by its host round trips,
-// so up to BIG_THREADS host
-// threads run them at once, each on a context with a stream of its own; every thread
-// synchronises its stream before it ends, so the chain launch queued after the join
-// sees psi0 and e0.  The stopping rule is the Ritz value's change, 1e-3 of gs_tol,
-// which leaves psi0 at least as converged as the persistent path's residual rule.
+// launch of the small ground states on c->stream.  Each solve is bound by its host
+// round trips, so up to BIG_THREADS host threads run them at once, each on a context
+// with a stream of its own; every thread synchronises its stream before it ends, so
+// the chain launch queued after the join sees psi0 and e0.  The stopping rule is the
+// Ritz value's change, 1e-3 of gs_tol, which leaves psi0 at least as converged as the
+// persistent path's residual rule.
+//
+// hipMemcpy and hipFree wait for kernels on every stream, the persistent launch
+// included, so the workers' streams, contexts (whose creation copies with hipMemcpy)
+// and Krylov workspaces are made before that launch (BigWorkers::prepare), their
+// contexts keep freed memory until they are destroyed (defer_free), and all of it is
+// released after the chains (the destructor).
 constexpr int BIG_THREADS = 8;
 
-static int big_ground_states(phub_ctx *c, const std::vector<int> &big,
-                             const std::vector<phub_op *> &ops_gs,
-                             const std::vector<ClusterDev> &hc, const phub_batch_opts *o,
-                             double *d_e0) {
-    const int nth = std::min<int>(BIG_THREADS, (int)big.size());
-    std::vector<int> rcs(nth, PHUB_OK);
-    std::vector<std::string> msgs(nth);
-    auto worker = [&](int w) {
-        hipStream_t s2 = nullptr;
-        if (hipSetDevice(c->device) != hipSuccess ||
-            hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess) {
-            rcs[w] = PHUB_ERR_HIP;
-            msgs[w] = "stream for the large ground states";
-            return;
-        }
-        phub_ctx *c2 = nullptr;
-        int rc = phub_ctx_create(c->device, (void *)s2, &c2);
-        std::vector<double> al(MAXSTEPS + 2), be(MAXSTEPS + 3);
-        // clusters w, w + nth, ... of the list (largest first)
-        for (size_t j = w; j < big.size() && rc == PHUB_OK; j += nth) {
-            const int i = big[j];
-            phub_lanczos_opts lo{};
-            lo.mode = PHUB_LCZ_PLAIN64;
-            lo.max_steps = o->gs_steps;
-            lo.tol = 1e-3 * o->gs_tol;
-            lo.check_every = 5;
-            lo.seed = o->seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
-            lo.want_psi = 1;
-            lo.n_states = 1;
-            phub_lanczos_result res{};
-            rc = phub_lanczos(c2, ops_gs[i], &lo, nullptr, al.data(), be.data(), &res, hc[i].psi0);
-            if (rc == PHUB_ERR_NOCONV) rc = PHUB_OK;   // the step cap, as in the persistent path
-            if (rc == PHUB_OK) {
-                const double e = res.e0;
-                if (hipMemcpyAsync(d_e0 + i, &e, sizeof(double), hipMemcpyHostToDevice, s2) != hipSuccess ||
-                    hipStreamSynchronize(s2) != hipSuccess)
-                    rc = PHUB_ERR_HIP;
-            }
-        }
-        if (hipStreamSynchronize(s2) != hipSuccess && rc == PHUB_OK) rc = PHUB_ERR_HIP;
-        rcs[w] = rc;
-        if (rc) msgs[w] = c2 ? c2->err : std::string("context for the large ground states");
-        if (c2) phub_ctx_destroy(c2);
-        hipStreamDestroy(s2);
+struct BigWorkers {
+    struct W {
+        hipStream_t s = nullptr;
+        phub_ctx *c = nullptr;
+        void *ws = nullptr;
+        size_t ws_bytes = 0;
     };
-    std::vector<std::thread> th;
-    std::vector<int> here{0};   // workers run on this thread (all of them if threads fail)
-    for (int w = 1; w < nth; ++w) {
-        try {
-            th.emplace_back(worker, w);
-        } catch (...) {
-            here.push_back(w);
+    std::vector<W> w;
+    int device = 0;
+    ~BigWorkers() {
+        hipSetDevice(device);
+        for (W &x : w) {
+            if (x.s) hipStreamSynchronize(x.s);
+            if (x.c) phub_ctx_destroy(x.c);
+            if (x.ws) hipFree(x.ws);
+            if (x.s) hipStreamDestroy(x.s);
         }
     }
-    for (int w : here) worker(w);
-    for (auto &t : th) t.join();
-    for (int w = 0; w < nth; ++w)
-        if (rcs[w]) return phub_fail(c, rcs[w], msgs[w].empty() ? "large ground state" : msgs[w]);
-    return PHUB_OK;
-}
+
+    static phub_lanczos_opts opts(const phub_batch_opts *o, int i) {
+        phub_lanczos_opts lo{};
+        lo.mode = PHUB_LCZ_PLAIN64;
+        lo.max_steps = o->gs_steps;
+        lo.tol = 1e-3 * o->gs_tol;
+        lo.check_every = 5;
+        lo.seed = o->seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
+        lo.want_psi = 1;
+        lo.n_states = 1;
+        return lo;
+    }
+
+    // worker k solves clusters big[k], big[k + nth], ...
+    int prepare(phub_ctx *c, const std::vector<int> &big, const std::vector<phub_op *> &ops_gs,
+                const phub_batch_opts *o) {
+        device = c->device;
+        const int nth = std::min<int>(BIG_THREADS, (int)big.size());
+        w.resize(nth);
+        for (int k = 0; k < nth; ++k) {
+            W &x = w[k];
+            PHUB_HIP(c, hipStreamCreateWithFlags(&x.s, hipStreamNonBlocking));
+            if (phub_ctx_create(c->device, (void *)x.s, &x.c) != PHUB_OK)
+                return phub_fail(c, PHUB_ERR_HIP, "context for the large ground states: " +
+                                                      std::string(x.c ? x.c->err : ""));
+            x.c->defer_free = true;
+            for (size_t j = k; j < big.size(); j += nth) {
+                const phub_lanczos_opts lo = opts(o, big[j]);
+                size_t b = 0;
+                if (phub_lanczos_workspace_bytes(ops_gs[big[j]], &lo, &b) != PHUB_OK)
+                    return phub_fail(c, PHUB_ERR_ARG, "large ground state workspace");
+                x.ws_bytes = std::max(x.ws_bytes, b);
+            }
+            PHUB_HIP(c, hipMalloc(&x.ws, x.ws_bytes));
+        }
+        return PHUB_OK;
+    }
+
+    int run(phub_ctx *c, const std::vector<int> &big, const std::vector<phub_op *> &ops_gs,
+            const std::vector<ClusterDev> &hc, const phub_batch_opts *o, double *d_e0) {
+        const int nth = (int)w.size();
+        std::vector<int> rcs(nth, PHUB_OK);
+        std::vector<std::string> msgs(nth);
+        auto worker = [&](int k) {
+            W &x = w[k];
+            int rc = hipSetDevice(device) == hipSuccess ? PHUB_OK : PHUB_ERR_HIP;
+            std::vector<double> al(MAXSTEPS + 2), be(MAXSTEPS + 3);
+            for (size_t j = k; j < big.size() && rc == PHUB_OK; j += nth) {
+                const int i = big[j];
+                phub_lanczos_opts lo = opts(o, i);
+                lo.workspace = x.ws;
+                lo.workspace_bytes = x.ws_bytes;
+                phub_lanczos_result res{};
+                rc = phub_lanczos(x.c, ops_gs[i], &lo, nullptr, al.data(), be.data(), &res, hc[i].psi0);
+                if (rc == PHUB_ERR_NOCONV) rc = PHUB_OK;   // the step cap, as in the persistent path
+                if (rc == PHUB_OK) {
+                    const double e = res.e0;
+                    if (hipMemcpyAsync(d_e0 + i, &e, sizeof(double), hipMemcpyHostToDevice, x.s) != hipSuccess ||
+                        hipStreamSynchronize(x.s) != hipSuccess)
+                        rc = PHUB_ERR_HIP;
+                }
+            }
+            if (hipStreamSynchronize(x.s) != hipSuccess && rc == PHUB_OK) rc = PHUB_ERR_HIP;
+            rcs[k] = rc;
+            if (rc) msgs[k] = x.c->err;
+        };
+        std::vector<std::thread> th;
+        std::vector<int> here{0};   // workers run on this thread (all of them if threads fail)
+        for (int k = 1; k < nth; ++k) {
+            try {
+                th.emplace_back(worker, k);
+            } catch (...) {
+                here.push_back(k);
+            }
+        }
+        for (int k : here) worker(k);
+        for (auto &t : th) t.join();
+        for (int k = 0; k < nth; ++k)
+            if (rcs[k]) return phub_fail(c, rcs[k], msgs[k].empty() ? "large ground state" : msgs[k]);
+        return PHUB_OK;
+    }
+};
 
 extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
                                   const phub_batch_opts *o, double *e0, double *gf_alpha,
@@ -676,11 +722,16 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
     B.counter = counter.p;
     B.first = 0;
     B.last = ngs;
+    BigWorkers bw;   // declared after the buffers: destroyed (synchronised) before them
+    if (!big.empty()) {
+        int rc = bw.prepare(c, big, ops_gs, o);
+        if (rc) { cleanup(); return rc; }
+    }
     if (ngs > 0)
         hipLaunchKernelGGL(k_batch<0>, dim3(std::max(1, std::min(grid, ngs))), dim3(BT), 0, c->stream, B);
     PHUB_HIP(c, hipGetLastError());
     if (!big.empty()) {
-        int rc = big_ground_states(c, big, ops_gs, hc, o, de0.p);
+        int rc = bw.run(c, big, ops_gs, hc, o, de0.p);
         if (rc) {
             // the first launch still reads the buffers freed on return
             hipStreamSynchronize(c->stream);

@@ -630,10 +630,11 @@ static int ritz_states(const LczState &s, int nst, std::vector<double> &vals,
 // Device memory for Krylov vectors: carved from a caller workspace while it lasts
 // (256-byte aligned pieces), then allocated here and freed with the arena.
 struct Arena {
+    phub_ctx *c;
     char *base = nullptr;
     size_t size = 0, used = 0;
     std::vector<void *> owned;
-    Arena(void *ws, size_t bytes) {
+    Arena(phub_ctx *ctx, void *ws, size_t bytes) : c(ctx) {
         if (ws) {
             const uintptr_t a = ((uintptr_t)ws + 255) & ~(uintptr_t)255;
             const size_t skip = a - (uintptr_t)ws;
@@ -666,7 +667,7 @@ struct Arena {
     }
     // every piece handed out so far becomes invalid
     void release() {
-        for (void *p : owned) hipFree(p);
+        for (void *p : owned) phub_dfree(c, p);
         owned.clear();
         used = 0;
     }
@@ -955,10 +956,12 @@ static int rq_residual(phub_ctx *c, const phub_op *op, const double *psi, double
 
 template <typename T>
 struct DevBuf {
+    phub_ctx *c;
     T *p = nullptr;
-    ~DevBuf() {
-        if (p) hipFree(p);
-    }
+    explicit DevBuf(phub_ctx *ctx) : c(ctx) {}
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { phub_dfree(c, p); }
 };
 
 // plain recurrence with 3 resident vectors of type T (FP64 scalars); psi by a
@@ -996,7 +999,7 @@ static int plain_mode(phub_ctx *c, const phub_op *op, const phub_lanczos_opts *o
             return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
         if (st.growing) {
             // every vector is stored: psi_q = sum_j y_qj x_j / beta_j in one pass each
-            DevBuf<const T *> dptr;
+            DevBuf<const T *> dptr(c);
             PHUB_HIP(c, hipMalloc(&dptr.p, (size_t)s.steps * sizeof(T *)));
             std::vector<double> cf(s.steps);
             for (size_t q = 0; q < ys.size(); ++q) {
@@ -1100,11 +1103,11 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
     auto t0 = std::chrono::steady_clock::now();
     EvTimer tm(c);
     LczState s;
-    DevBuf<double> coef;
+    DevBuf<double> coef(c);
     PHUB_HIP(c, hipMalloc(&coef.p, (size_t)(oo.max_steps + 8) * sizeof(double)));
     int rc = PHUB_OK;
     double t_lcz = 0.0;
-    Arena ar(o->workspace, o->workspace_bytes);
+    Arena ar(c, o->workspace, o->workspace_bytes);
 
     if (o->mode == PHUB_LCZ_PLAIN64) {
         rc = plain_mode<double>(c, op, o, &oo, d_v0, d_psi, s, &tm, fixed, res, t0, &t_lcz, coef.p,
@@ -1241,7 +1244,7 @@ static int cf_chain_t(phub_ctx *c, const phub_op *op, const double *start, int n
                       double *alpha, double *beta, double *norm2) {
     const int64_t n = op->dim;
     const unsigned g = (unsigned)((n + 255) / 256);
-    DevBuf<T> v0, b1, b2;
+    DevBuf<T> v0(c), b1(c), b2(c);
     PHUB_HIP(c, hipMalloc(&v0.p, n * sizeof(T)));
     PHUB_HIP(c, hipMalloc(&b1.p, n * sizeof(T)));
     PHUB_HIP(c, hipMalloc(&b2.p, n * sizeof(T)));
@@ -1411,8 +1414,8 @@ extern "C" int phub_band_lanczos(phub_ctx *c, const phub_op *op, const double *d
     const double DF = 1e-10;   // deflation threshold (spgf_bandlanczos.f90:312)
     // 2p + 1 vector slots: the p previous vectors, the current one and its p - 1
     // successors, and the work vector; rotated by pointer, never copied
-    DevBuf<double> slots, coefd;
-    DevBuf<const double *> dptr;
+    DevBuf<double> slots(c), coefd(c);
+    DevBuf<const double *> dptr(c);
     PHUB_HIP(c, hipMalloc(&slots.p, (size_t)(2 * p + 1) * ld * sizeof(double)));
     PHUB_HIP(c, hipMalloc(&coefd.p, (size_t)(2 * p + 8) * sizeof(double)));
     PHUB_HIP(c, hipMalloc(&dptr.p, (size_t)(2 * p + 1) * sizeof(double *)));
@@ -1574,7 +1577,7 @@ extern "C" int phub_adjoint_minres(phub_ctx *c, const phub_op *op, const double 
                                    int *iters, double *rel_res) {
     if (!c || !op || !psi || !b || !x) return phub_fail(c, PHUB_ERR_ARG, "null argument");
     const int64_t n = op->dim;
-    DevBuf<double> r1, r2, y, v, w, w1, w2;
+    DevBuf<double> r1(c), r2(c), y(c), v(c), w(c), w1(c), w2(c);
     for (auto *bf : {&r1, &r2, &y, &v, &w, &w1, &w2})
         PHUB_HIP(c, hipMalloc(&bf->p, n * sizeof(double)));
     PHUB_HIP(c, hipMemsetAsync(x, 0, n * sizeof(double), c->stream));

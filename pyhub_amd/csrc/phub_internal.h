@@ -19,7 +19,19 @@ struct phub_ctx {
     size_t red_cap = 0;              // doubles
     double *h_red = nullptr;         // pinned host mirror of reduction results
     size_t h_red_cap = 0;
+    // hipFree waits for every stream of the device: a context that works beside a
+    // long kernel on another stream sets defer_free and its device memory is kept
+    // until the context is destroyed
+    bool defer_free = false;
+    std::vector<void *> deferred;
 };
+
+// Frees device memory that context c allocated (c may be null: plain hipFree).
+inline void phub_dfree(phub_ctx *c, void *p) {
+    if (!p) return;
+    if (c && c->defer_free) c->deferred.push_back(p);
+    else hipFree(p);
+}
 
 // Per-species hop table: for every state s the list of (target index, coefficient)
 // produced by c^dag_K c_J with J != K and |T(J,K)| >= 1e-14 (hprod_hubbard.f90:37,39-50).
