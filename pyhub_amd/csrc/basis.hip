@@ -51,7 +51,7 @@ extern "C" int phub_ctx_create(int device, void *stream, phub_ctx **out) {
     host_binom(tab.data());
     e = hipMalloc(&c->d_binom, tab.size() * sizeof(uint64_t));
     if (e == hipSuccess)
-        e = hipMemcpy(c->d_binom, tab.data(), tab.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
+        e = phub_copy(c, c->d_binom, tab.data(), tab.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     *out = c;
@@ -460,13 +460,13 @@ int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor
     PHUB_HIP(c, hipMalloc(&d_td, std::max(n, 1) * sizeof(double)));
     PHUB_HIP(c, hipMalloc(&d_max, sizeof(int)));
     if (np) {
-        PHUB_HIP(c, hipMemcpy(d_bj, bj.data(), np * sizeof(int), hipMemcpyHostToDevice));
-        PHUB_HIP(c, hipMemcpy(d_bk, bk.data(), np * sizeof(int), hipMemcpyHostToDevice));
-        PHUB_HIP(c, hipMemcpy(d_t, tv.data(), np * sizeof(double), hipMemcpyHostToDevice));
-        PHUB_HIP(c, hipMemcpy(d_pid, pid.data(), np * sizeof(int16_t), hipMemcpyHostToDevice));
+        PHUB_HIP(c, phub_copy(c, d_bj, bj.data(), np * sizeof(int), hipMemcpyHostToDevice));
+        PHUB_HIP(c, phub_copy(c, d_bk, bk.data(), np * sizeof(int), hipMemcpyHostToDevice));
+        PHUB_HIP(c, phub_copy(c, d_t, tv.data(), np * sizeof(double), hipMemcpyHostToDevice));
+        PHUB_HIP(c, phub_copy(c, d_pid, pid.data(), np * sizeof(int16_t), hipMemcpyHostToDevice));
     }
-    if (n) PHUB_HIP(c, hipMemcpy(d_td, tdiag.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    PHUB_HIP(c, hipMemset(d_max, 0, sizeof(int)));
+    if (n) PHUB_HIP(c, phub_copy(c, d_td, tdiag.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    PHUB_HIP(c, phub_clear(c, d_max, sizeof(int)));
     PHUB_HIP(c, hipMalloc(&out->masks, dim * sizeof(uint32_t)));
     PHUB_HIP(c, hipMalloc(&out->eloc, dim * sizeof(double)));
     PHUB_HIP(c, hipMalloc(&out->cnt, dim * sizeof(int32_t)));
@@ -505,8 +505,8 @@ int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor
             std::vector<float> cls32(cls.begin(), cls.end());
             PHUB_HIP(c, hipMalloc(&out->cls64, cls.size() * sizeof(double)));
             PHUB_HIP(c, hipMalloc(&out->cls32, cls.size() * sizeof(float)));
-            PHUB_HIP(c, hipMemcpy(out->cls64, cls.data(), cls.size() * sizeof(double), hipMemcpyHostToDevice));
-            PHUB_HIP(c, hipMemcpy(out->cls32, cls32.data(), cls.size() * sizeof(float), hipMemcpyHostToDevice));
+            PHUB_HIP(c, phub_copy(c, out->cls64, cls.data(), cls.size() * sizeof(double), hipMemcpyHostToDevice));
+            PHUB_HIP(c, phub_copy(c, out->cls32, cls32.data(), cls.size() * sizeof(float), hipMemcpyHostToDevice));
             if (kmajor && dim < (1ll << 18)) {   // the byte offset idx*8 must fit 21 bits
                 PHUB_HIP(c, hipMalloc(&out->packed, (size_t)dim * mh * sizeof(uint32_t)));
                 hipLaunchKernelGGL(k_hop_pack, dim3(nblk(dim, 256)), dim3(256), 0, c->stream,
@@ -514,11 +514,11 @@ int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor
                                    out->packed);
             } else if (!kmajor) {
                 std::vector<int32_t> hcnt(dim), rp(dim + 1, 0);
-                PHUB_HIP(c, hipMemcpy(hcnt.data(), out->cnt, dim * sizeof(int32_t), hipMemcpyDeviceToHost));
+                PHUB_HIP(c, phub_copy(c, hcnt.data(), out->cnt, dim * sizeof(int32_t), hipMemcpyDeviceToHost));
                 for (int64_t s = 0; s < dim; ++s) rp[s + 1] = rp[s] + hcnt[s];
                 PHUB_HIP(c, hipMalloc(&out->rowptr, (dim + 1) * sizeof(int32_t)));
                 PHUB_HIP(c, hipMalloc(&out->csr, std::max<int64_t>(rp[dim], 1) * sizeof(uint32_t)));
-                PHUB_HIP(c, hipMemcpy(out->rowptr, rp.data(), (dim + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+                PHUB_HIP(c, phub_copy(c, out->rowptr, rp.data(), (dim + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
                 hipLaunchKernelGGL(k_hop_pack_csr, dim3(nblk(dim, 256)), dim3(256), 0, c->stream,
                                    out->idx, out->c64, out->cnt, out->rowptr, dim, mh, out->cls64,
                                    out->ncls, out->csr);
@@ -626,7 +626,7 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
         std::vector<int64_t> hoff(nh + 2, 0);
         for (int m = 0; m <= nh; ++m) hoff[m + 1] = hoff[m] + (int64_t)phub_binom(nh, m);
         PHUB_HIP(c, hipMalloc(&out->hoff, (nh + 2) * sizeof(int64_t)));
-        PHUB_HIP(c, hipMemcpy(out->hoff, hoff.data(), (nh + 2) * sizeof(int64_t), hipMemcpyHostToDevice));
+        PHUB_HIP(c, phub_copy(c, out->hoff, hoff.data(), (nh + 2) * sizeof(int64_t), hipMemcpyHostToDevice));
         PHUB_HIP(c, hipMalloc(&out->headpat, ((size_t)1 << nh) * sizeof(uint32_t)));
         hipLaunchKernelGGL(k_headpat, dim3(nblk(1ll << nh, 256)), dim3(256), 0, c->stream,
                            out->tabA, nh, out->hoff, out->headpat);
@@ -656,13 +656,13 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
                 }
             PHUB_HIP(c, hipMalloc(&out->tabA1, a1.size() * sizeof(uint32_t)));
             PHUB_HIP(c, hipMalloc(&out->tabA2, a2.size() * sizeof(uint32_t)));
-            PHUB_HIP(c, hipMemcpy(out->tabA1, a1.data(), a1.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-            PHUB_HIP(c, hipMemcpy(out->tabA2, a2.data(), a2.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+            PHUB_HIP(c, phub_copy(c, out->tabA1, a1.data(), a1.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+            PHUB_HIP(c, phub_copy(c, out->tabA2, a2.data(), a2.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         }
         // work items: every low pattern xl with a valid high popcount m = n/2 - popc(xl)
         // owns the contiguous block of C(nh, m) states that ends at C-1-B[m][xl]
         std::vector<uint64_t> Bh(((size_t)1 << L) * (nh + 1));
-        PHUB_HIP(c, hipMemcpy(Bh.data(), out->tabB, Bh.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        PHUB_HIP(c, phub_copy(c, Bh.data(), out->tabB, Bh.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
         const int k = n / 2;
         std::vector<std::pair<uint64_t, uint32_t>> blocks;   // (first index, xl)
         for (uint32_t xl = 0; xl < (1u << L); ++xl) {
@@ -680,7 +680,7 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
         }
         out->nitems = (int64_t)items.size();
         PHUB_HIP(c, hipMalloc(&out->items, std::max<size_t>(items.size(), 1) * sizeof(uint32_t)));
-        PHUB_HIP(c, hipMemcpy(out->items, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        PHUB_HIP(c, phub_copy(c, out->items, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
     // bonds: unordered {J<K} with either J(J,K) or J(K,J) non-zero
     std::vector<int> bj, bk;
@@ -743,8 +743,8 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
             std::vector<float> e32(e64.begin(), e64.end());
             PHUB_HIP(c, hipMalloc(&out->ehh64, e64.size() * sizeof(double)));
             PHUB_HIP(c, hipMalloc(&out->ehh32, e32.size() * sizeof(float)));
-            PHUB_HIP(c, hipMemcpy(out->ehh64, e64.data(), e64.size() * sizeof(double), hipMemcpyHostToDevice));
-            PHUB_HIP(c, hipMemcpy(out->ehh32, e32.data(), e32.size() * sizeof(float), hipMemcpyHostToDevice));
+            PHUB_HIP(c, phub_copy(c, out->ehh64, e64.data(), e64.size() * sizeof(double), hipMemcpyHostToDevice));
+            PHUB_HIP(c, phub_copy(c, out->ehh32, e32.data(), e32.size() * sizeof(float), hipMemcpyHostToDevice));
         }
         out->ntt = (int)g[0].size();
         out->nhh = (int)g[1].size();
@@ -753,7 +753,7 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
         for (auto &v : g) all.insert(all.end(), v.begin(), v.end());
         PHUB_HIP(c, hipMalloc(&out->sbonds, std::max<size_t>(all.size(), 1) * sizeof(HeisBond)));
         if (!all.empty())
-            PHUB_HIP(c, hipMemcpy(out->sbonds, all.data(), all.size() * sizeof(HeisBond), hipMemcpyHostToDevice));
+            PHUB_HIP(c, phub_copy(c, out->sbonds, all.data(), all.size() * sizeof(HeisBond), hipMemcpyHostToDevice));
     }
     out->diag_const = dconst;
     size_t nba = std::max(nb, 1);
@@ -764,12 +764,12 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
     PHUB_HIP(c, hipMalloc(&out->bsign, nba * sizeof(double)));
     PHUB_HIP(c, hipMalloc(&out->bond_ising, nba * sizeof(double)));
     if (nb) {
-        PHUB_HIP(c, hipMemcpy(out->bj, bj.data(), nb * sizeof(int), hipMemcpyHostToDevice));
-        PHUB_HIP(c, hipMemcpy(out->bk, bk.data(), nb * sizeof(int), hipMemcpyHostToDevice));
-        PHUB_HIP(c, hipMemcpy(out->bJ, bJ.data(), nb * sizeof(double), hipMemcpyHostToDevice));
-        PHUB_HIP(c, hipMemcpy(out->bJr, bJr.data(), nb * sizeof(double), hipMemcpyHostToDevice));
-        PHUB_HIP(c, hipMemcpy(out->bsign, bs.data(), nb * sizeof(double), hipMemcpyHostToDevice));
-        PHUB_HIP(c, hipMemcpy(out->bond_ising, bis.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+        PHUB_HIP(c, phub_copy(c, out->bj, bj.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+        PHUB_HIP(c, phub_copy(c, out->bk, bk.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+        PHUB_HIP(c, phub_copy(c, out->bJ, bJ.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+        PHUB_HIP(c, phub_copy(c, out->bJr, bJr.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+        PHUB_HIP(c, phub_copy(c, out->bsign, bs.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+        PHUB_HIP(c, phub_copy(c, out->bond_ising, bis.data(), nb * sizeof(double), hipMemcpyHostToDevice));
     }
     PHUB_HIP(c, hipStreamSynchronize(c->stream));
     return PHUB_OK;

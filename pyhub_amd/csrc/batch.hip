@@ -695,12 +695,12 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
     PHUB_HIP(c, hipMalloc(&gb.p, (size_t)nslots * o->gf_steps * sizeof(double)));
     PHUB_HIP(c, hipMalloc(&gn.p, nslots * sizeof(double)));
     PHUB_HIP(c, hipMalloc(&tri.p, (size_t)grid * 6 * MAXSTEPS * sizeof(double)));
-    PHUB_HIP(c, hipMemcpy(dcl.p, hc.data(), ncl * sizeof(ClusterDev), hipMemcpyHostToDevice));
-    PHUB_HIP(c, hipMemcpy(dtasks.p, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice));
-    PHUB_HIP(c, hipMemset(counter.p, 0, 2 * sizeof(unsigned int)));
-    PHUB_HIP(c, hipMemset(gn.p, 0, nslots * sizeof(double)));
-    PHUB_HIP(c, hipMemset(ga.p, 0, (size_t)nslots * o->gf_steps * sizeof(double)));
-    PHUB_HIP(c, hipMemset(gb.p, 0, (size_t)nslots * o->gf_steps * sizeof(double)));
+    PHUB_HIP(c, phub_copy(c, dcl.p, hc.data(), ncl * sizeof(ClusterDev), hipMemcpyHostToDevice));
+    PHUB_HIP(c, phub_copy(c, dtasks.p, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice));
+    PHUB_HIP(c, phub_clear(c, counter.p, 2 * sizeof(unsigned int)));
+    PHUB_HIP(c, phub_clear(c, gn.p, nslots * sizeof(double)));
+    PHUB_HIP(c, phub_clear(c, ga.p, (size_t)nslots * o->gf_steps * sizeof(double)));
+    PHUB_HIP(c, phub_clear(c, gb.p, (size_t)nslots * o->gf_steps * sizeof(double)));
     BatchDev B{};
     B.cl = dcl.p;
     B.tasks = dtasks.p;
@@ -747,10 +747,10 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
                            c->stream, B);
     PHUB_HIP(c, hipGetLastError());
     PHUB_HIP(c, hipStreamSynchronize(c->stream));
-    PHUB_HIP(c, hipMemcpy(e0, de0.p, ncl * sizeof(double), hipMemcpyDeviceToHost));
-    if (gf_alpha) PHUB_HIP(c, hipMemcpy(gf_alpha, ga.p, (size_t)nslots * o->gf_steps * sizeof(double), hipMemcpyDeviceToHost));
-    if (gf_beta) PHUB_HIP(c, hipMemcpy(gf_beta, gb.p, (size_t)nslots * o->gf_steps * sizeof(double), hipMemcpyDeviceToHost));
-    if (gf_norm2) PHUB_HIP(c, hipMemcpy(gf_norm2, gn.p, nslots * sizeof(double), hipMemcpyDeviceToHost));
+    PHUB_HIP(c, phub_copy(c, e0, de0.p, ncl * sizeof(double), hipMemcpyDeviceToHost));
+    if (gf_alpha) PHUB_HIP(c, phub_copy(c, gf_alpha, ga.p, (size_t)nslots * o->gf_steps * sizeof(double), hipMemcpyDeviceToHost));
+    if (gf_beta) PHUB_HIP(c, phub_copy(c, gf_beta, gb.p, (size_t)nslots * o->gf_steps * sizeof(double), hipMemcpyDeviceToHost));
+    if (gf_norm2) PHUB_HIP(c, phub_copy(c, gf_norm2, gn.p, nslots * sizeof(double), hipMemcpyDeviceToHost));
     cleanup();
     return PHUB_OK;
 }

@@ -150,9 +150,9 @@ extern "C" int phub_fock_create(phub_ctx *c, int n, const int *n_up, int n_up_co
     hipError_t e = hipMalloc(&f->d_base, base.size() * sizeof(int64_t));
     if (e == hipSuccess) e = hipMalloc(&f->d_nib, nib.size() * sizeof(uint32_t));
     if (e == hipSuccess)
-        e = hipMemcpy(f->d_base, base.data(), base.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+        e = phub_copy(c, f->d_base, base.data(), base.size() * sizeof(int64_t), hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = hipMemcpy(f->d_nib, nib.data(), nib.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+        e = phub_copy(c, f->d_nib, nib.data(), nib.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fock_fail_free(c, f, e, "ranking tables");
     *out = f;
@@ -298,7 +298,7 @@ static int factor_touch(const int *e) {
     }
 }
 
-static hipError_t upload_prog(OpProgDev &d, const std::vector<double> &cf,
+static hipError_t upload_prog(phub_ctx *c, OpProgDev &d, const std::vector<double> &cf,
                               const std::vector<int> &st, const std::vector<int> &el) {
     d.nterms = (int)cf.size();
     std::vector<int> touch(std::max<size_t>(cf.size(), 1), 0);
@@ -325,7 +325,7 @@ static hipError_t upload_prog(OpProgDev &d, const std::vector<double> &cf,
                     std::make_pair(&d.rowq_up, &rq_up), std::make_pair(&d.rowq_dn, &rq_dn)}) {
         if (e == hipSuccess) e = hipMalloc(pr.first, pr.second->size() * sizeof(int));
         if (e == hipSuccess)
-            e = hipMemcpy(*pr.first, pr.second->data(), pr.second->size() * sizeof(int),
+            e = phub_copy(c, *pr.first, pr.second->data(), pr.second->size() * sizeof(int),
                           hipMemcpyHostToDevice);
     }
     if (e == hipSuccess) e = hipMalloc(&d.coef, std::max<size_t>(cf.size(), 1) * sizeof(double));
@@ -333,12 +333,12 @@ static hipError_t upload_prog(OpProgDev &d, const std::vector<double> &cf,
     if (e == hipSuccess) e = hipMalloc(&d.elem, std::max<size_t>(el.size(), 1) * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&d.touch, touch.size() * sizeof(int));
     if (e == hipSuccess)
-        e = hipMemcpy(d.touch, touch.data(), touch.size() * sizeof(int), hipMemcpyHostToDevice);
+        e = phub_copy(c, d.touch, touch.data(), touch.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess && !cf.empty())
-        e = hipMemcpy(d.coef, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d.start, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice);
+        e = phub_copy(c, d.coef, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = phub_copy(c, d.start, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess && !el.empty())
-        e = hipMemcpy(d.elem, el.data(), el.size() * sizeof(int), hipMemcpyHostToDevice);
+        e = phub_copy(c, d.elem, el.data(), el.size() * sizeof(int), hipMemcpyHostToDevice);
     return e;
 }
 
@@ -353,7 +353,7 @@ extern "C" int phub_opprog_create(phub_ctx *c, int nterms, const double *coef,
         std::vector<int> st, el;
         int rc = expand(c, nterms, coef, term_len, elems, b == 1, cf, st, el, p->max_site);
         if (rc) { phub_opprog_destroy(p); return rc; }
-        hipError_t e = upload_prog(b == 0 ? p->vec : p->bas, cf, st, el);
+        hipError_t e = upload_prog(c, b == 0 ? p->vec : p->bas, cf, st, el);
         if (e != hipSuccess) {
             phub_opprog_destroy(p);
             return phub_hip_fail(c, e, "operator program upload");

@@ -82,7 +82,7 @@ extern "C" int phub_op_hubbard(phub_ctx *c, const phub_hubbard_desc *d, phub_op 
             }
         }
         hipError_t e = hipMalloc(&op->d_ul, n4 * sizeof(double));
-        if (e == hipSuccess) e = hipMemcpy(op->d_ul, ul.data(), n4 * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = phub_copy(c, op->d_ul, ul.data(), n4 * sizeof(double), hipMemcpyHostToDevice);
         if (e != hipSuccess) { phub_op_destroy(op); return phub_hip_fail(c, e, "4-index U upload"); }
         rc = phub_ensure_pair_tables(c, op);
         if (rc) { phub_op_destroy(op); return rc; }

@@ -26,6 +26,22 @@ struct phub_ctx {
     std::vector<void *> deferred;
 };
 
+// Copies and clears ordered on the context's stream.  The synchronous hipMemcpy /
+// hipMemset run on the null stream, which neither waits for nor is waited on by a
+// non-blocking stream (torch's side streams are such streams): a table uploaded or
+// a counter cleared that way can still be in flight when a kernel queued on
+// c->stream reads it.  phub_copy returns when the copy is complete, so the host
+// buffer may be reused or freed at once.
+inline hipError_t phub_copy(phub_ctx *c, void *dst, const void *src, size_t bytes,
+                            hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e;
+}
+inline hipError_t phub_clear(phub_ctx *c, void *dst, size_t bytes) {
+    return hipMemsetAsync(dst, 0, bytes, c->stream);
+}
+
 // Frees device memory that context c allocated (c may be null: plain hipFree).
 inline void phub_dfree(phub_ctx *c, void *p) {
     if (!p) return;
