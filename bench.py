@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--workload", default="hubbard4x4",
                     choices=["hubbard4x4", "heis6x6", "batch256", "band4x4", "u4", "ops4x4", "grad64"])
     ap.add_argument("--sets", type=int, default=64, help="parameter sets (grad64)")
+    ap.add_argument("--threads", type=int, default=8,
+                    help="host threads sharing the GPU, one stream each (grad64)")
     ap.add_argument("--band-steps", type=int, default=192, help="band Lanczos steps (band4x4)")
     ap.add_argument("--lx", type=int, default=6, help="Heisenberg torus width")
     ap.add_argument("--ly", type=int, default=6, help="Heisenberg torus height")
@@ -595,9 +597,10 @@ def _grad_bench(args, ctx, torch):
     1-RDM, the ground state, E0 with its Hellmann-Feynman gradient, and the loss
     L = sum_s ||gamma_s - target_s||_F^2 with its adjoint-MINRES gradient.  Set 0's
     gradient is checked against central differences (h = 1e-5) on two bonds."""
-    from pyhub_amd.grad import ClusterSpec, lattice_bonds, rdm_loss_and_grad, solve
+    from pyhub_amd.grad import ClusterSpec, lattice_bonds, rdm_loss_and_grad, run_sets, solve
     spec = ClusterSpec(n=12, bonds=lattice_bonds(4, 3), n_up=6, n_dn=6, lanczos_tol=1e-12)
     nsets = max(1, args.sets)
+    nthreads = max(1, args.threads)
 
     def params(p):
         rng = np.random.default_rng([4, p])
@@ -611,21 +614,20 @@ def _grad_bench(args, ctx, torch):
     t, eps, U, pert = params(0)
     solve(spec, t, eps, U)   # warm-up: tables, tuner
     torch.cuda.synchronize()
-    res = []
-    t0 = time.perf_counter()
-    for p in range(nsets):
+    def one_set(c, p):
         t, eps, U, (tq, eq, uq) = params(p)
-        op_t, _, psi_t = solve(spec, tq, eq, uq)
+        op_t, _, psi_t = solve(spec, tq, eq, uq, c)
         tg_up, tg_dn = op_t.one_rdm_spin(psi_t)
         del op_t, psi_t
-        L_val, gt, ge, gu, info = rdm_loss_and_grad(spec, t, eps, U, tg_up, tg_dn,
+        L_val, gt, ge, gu, info = rdm_loss_and_grad(spec, t, eps, U, tg_up, tg_dn, ctx=c,
                                                     want_energy_grad=True)
-        res.append({"L": L_val, "e0": info["e0"], "minres_iters": info["minres_iters"],
-                    "gt0": float(gt[0]), "gt9": float(gt[9]), "tg": (tg_up, tg_dn),
-                    "dE_dU": float(info["energy_grad"][2])})
-        if p % 8 == 7:
-            print(f"# grad64: {p + 1} sets, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
-                  flush=True)
+        return {"L": L_val, "e0": info["e0"], "minres_iters": info["minres_iters"],
+                "gt0": float(gt[0]), "gt9": float(gt[9]), "tg": (tg_up, tg_dn),
+                "dE_dU": float(info["energy_grad"][2])}
+
+    # the sets run on `nthreads` host threads, each with a stream and context of its own
+    t0 = time.perf_counter()
+    res = run_sets(one_set, range(nsets), nthreads=nthreads)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     # central differences of set 0 on bonds 0 and 9
@@ -645,7 +647,7 @@ def _grad_bench(args, ctx, torch):
         fd = (loss(tp) - loss(tm)) / 2e-5
         fd_err = max(fd_err, abs(fd - res[0][key]) / max(1e-12, abs(fd)))
     rate = nsets / wall
-    print(f"# grad64: {nsets} sets in {wall:.2f} s ({rate:.2f} sets/s), mean MINRES iterations "
+    print(f"# grad64: {nsets} sets on {nthreads} threads in {wall:.2f} s ({rate:.2f} sets/s), mean MINRES iterations "
           f"{np.mean([r['minres_iters'] for r in res]):.0f}, FD rel. error {fd_err:.2e}",
           file=sys.stderr, flush=True)
     out = {
@@ -655,7 +657,7 @@ def _grad_bench(args, ctx, torch):
         "vs_baseline": None, "dtype": "fp64",
         "data": "synthetic parameter sets from default_rng([4, p]), targets from default_rng([4, p, 1])",
         "config": {"workload": "3x4 open Hubbard, N_up=N_dn=6, 17 bonds, 64 parameter sets",
-                   "dim": 853776, "parallelism": "single GPU"},
+                   "dim": 853776, "parallelism": f"single GPU, {nthreads} host threads"},
         "grad": {"seconds": wall, "mean_minres_iters": float(np.mean([r["minres_iters"] for r in res])),
                  "fd_rel_err_set0_bonds_0_9": fd_err, "loss_set0": res[0]["L"],
                  "e0_set0": res[0]["e0"]},

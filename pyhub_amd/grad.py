@@ -113,6 +113,47 @@ def rdm_loss_and_grad(spec: ClusterSpec, t, eps, U, target_up, target_dn, ctx=No
     return L_val, gt, ge, gu, info
 
 
+def run_sets(fn, items, nthreads: int = 8, device: int = 0):
+    """fn(ctx, item) for every item; returns the results in item order.
+
+    The items are dealt round-robin to `nthreads` host threads. Each thread makes
+    a torch stream and a Context on it, and runs its items inside that stream.
+    Parameter sets are independent, and one set is a chain of small launches with
+    a host round trip per Krylov or MINRES step, so several sets in flight keep
+    the GPU busy where one leaves it idle between steps. ctypes releases the GIL
+    during every library call. After every thread has ended, the first exception
+    a thread raised is raised here."""
+    import threading
+    items = list(items)
+    nth = max(1, min(int(nthreads), len(items)))
+    L.load()   # load the library once, before the threads
+    out = [None] * len(items)
+    errs = []
+
+    def work(k):
+        try:
+            s = torch.cuda.Stream(device=device)
+            ctx = Context(device, stream=s)
+            with torch.cuda.stream(s):
+                for i in range(k, len(items), nth):
+                    out[i] = fn(ctx, items[i])
+            s.synchronize()
+        except BaseException as e:   # re-raised by the caller's thread
+            errs.append(e)
+
+    if nth == 1:
+        work(0)
+    else:
+        th = [threading.Thread(target=work, args=(k,)) for k in range(nth)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+    if errs:
+        raise errs[0]
+    return out
+
+
 class E0Function(torch.autograd.Function):
     """E0(t, eps, U) with the Hellmann-Feynman backward."""
 

@@ -144,3 +144,35 @@ def test_config4_geometry_single_set(ctx):
         tm[b] -= h
         fd = (loss(tp) - loss(tm)) / (2 * h)
         assert abs(fd - gt[b]) < 1e-4 * max(1.0, abs(gt[b])), (b, fd, gt[b])
+
+
+def test_run_sets_threads_match_sequential(ctx):
+    """run_sets: six parameter sets (target solve, 1-RDM loss, adjoint gradient) on
+    three host threads, each with a stream and context of its own, give the results
+    of the same sets on one thread, in item order; an exception in one set reaches
+    the caller after the threads end."""
+    from pyhub_amd.grad import rdm_loss_and_grad, run_sets, solve
+    spec = small_spec(3, 3)
+
+    def one_set(c, p):
+        t, eps, U = params(spec, 100 + p)
+        tq, eq, uq = params(spec, 200 + p)
+        op_t, _, psi_t = solve(spec, tq, eq, uq, c)
+        tg_up, tg_dn = op_t.one_rdm_spin(psi_t)
+        L_val, gt, ge, gu, info = rdm_loss_and_grad(spec, t, eps, U, tg_up, tg_dn, ctx=c)
+        return np.concatenate([[L_val, info["e0"], gu], gt, ge])
+
+    seq = run_sets(one_set, range(6), nthreads=1)
+    par = run_sets(one_set, range(6), nthreads=3)
+    assert len(par) == 6
+    for a, b in zip(seq, par):
+        assert np.max(np.abs(a - b)) < 1e-9 * max(1.0, np.max(np.abs(a)))
+    assert not np.allclose(seq[0], seq[1])   # the sets differ
+
+    def failing(c, p):
+        if p == 4:
+            raise ValueError("set 4")
+        return p
+
+    with pytest.raises(ValueError, match="set 4"):
+        run_sets(failing, range(6), nthreads=3)
