@@ -180,6 +180,88 @@ __global__ __launch_bounds__(256) void k_axpby(double a, const TX *__restrict__ 
     }
 }
 
+// Adjoint MINRES, the vector work of one iteration in three passes (FP64).
+// k_minres_scale: v = s * y.
+__global__ __launch_bounds__(256) void k_minres_scale(double *__restrict__ v,
+                                                      const double *__restrict__ y, double s,
+                                                      int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = s * y[i];
+}
+
+// after t = H v: t <- t - e0 v - c1 r1 (no r1 term when r1 is null); partials
+// [block][3] of (psi.(t - e0 v), v.t, v.psi).  psi is projected out of (H - e0) v
+// before the r1 term: taking it from the whole of t would also strip r1's own
+// small psi component, which breaks the three-term relation the MINRES update
+// relies on (slower convergence, 1e-11 instead of 1e-13 in a 60-state test).
+__global__ __launch_bounds__(256) void k_minres_a(double *__restrict__ t,
+                                                  const double *__restrict__ v,
+                                                  const double *__restrict__ r1,
+                                                  const double *__restrict__ psi, double e0,
+                                                  double c1, int64_t n,
+                                                  double *__restrict__ part) {
+    __shared__ double sh[3][4];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double vi = v[i], pi = psi[i];
+        double ti = t[i] - e0 * vi;
+        s0 += pi * ti;
+        if (r1) ti -= c1 * r1[i];
+        t[i] = ti;
+        s1 += vi * ti;
+        s2 += vi * pi;
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = s0;
+        sh[1][threadIdx.x >> 6] = s1;
+        sh[2][threadIdx.x >> 6] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3)
+        part[(size_t)blockIdx.x * 3 + threadIdx.x] =
+            sh[threadIdx.x][0] + sh[threadIdx.x][1] + sh[threadIdx.x][2] + sh[threadIdx.x][3];
+}
+
+// r = t - p psi - c2 r2 (r may not alias t, psi or r2); partial |r|^2
+__global__ __launch_bounds__(256) void k_minres_b(const double *__restrict__ t,
+                                                  const double *__restrict__ psi,
+                                                  const double *__restrict__ r2, double p,
+                                                  double c2, double *__restrict__ r, int64_t n,
+                                                  double *__restrict__ part) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double ri = t[i] - p * psi[i] - c2 * r2[i];
+        r[i] = ri;
+        s += ri * ri;
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// w = (v - oe wp - de wl) * ig, x += phi w
+__global__ __launch_bounds__(256) void k_minres_c(const double *__restrict__ v,
+                                                  const double *__restrict__ wp,
+                                                  const double *__restrict__ wl, double oe,
+                                                  double de, double ig, double phi,
+                                                  double *__restrict__ w, double *__restrict__ x,
+                                                  int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double wi = ((v[i] - oe * wp[i]) - de * wl[i]) * ig;
+        w[i] = wi;
+        x[i] += phi * wi;
+    }
+}
+
 // second pass of the plain recurrence: out = (out - a*x) * binv ; psi += y*out
 template <typename T>
 __global__ __launch_bounds__(256) void k_second_pass(T *__restrict__ out, const T *__restrict__ x,
@@ -1577,17 +1659,21 @@ extern "C" int phub_adjoint_minres(phub_ctx *c, const phub_op *op, const double 
                                    int *iters, double *rel_res) {
     if (!c || !op || !psi || !b || !x) return phub_fail(c, PHUB_ERR_ARG, "null argument");
     const int64_t n = op->dim;
-    DevBuf<double> r1(c), r2(c), y(c), v(c), w(c), w1(c), w2(c);
-    for (auto *bf : {&r1, &r2, &y, &v, &w, &w1, &w2})
+    // r2 is also the unnormalised Lanczos vector y of the iteration; wl / wp are the
+    // last two search directions and wf the slot the next one is written to
+    DevBuf<double> r1(c), r2(c), v(c), t(c), wl(c), wp(c), wf(c);
+    for (auto *bf : {&r1, &r2, &v, &t, &wl, &wp, &wf})
         PHUB_HIP(c, hipMalloc(&bf->p, n * sizeof(double)));
+    const unsigned g = grid_for(n);
+    int rc = phub_ensure_red(c, (size_t)g * 3 + 3 + 64);
+    if (rc) return rc;
     PHUB_HIP(c, hipMemsetAsync(x, 0, n * sizeof(double), c->stream));
-    PHUB_HIP(c, hipMemsetAsync(w.p, 0, n * sizeof(double), c->stream));
-    PHUB_HIP(c, hipMemsetAsync(w2.p, 0, n * sizeof(double), c->stream));
+    PHUB_HIP(c, hipMemsetAsync(wl.p, 0, n * sizeof(double), c->stream));
+    PHUB_HIP(c, hipMemsetAsync(wp.p, 0, n * sizeof(double), c->stream));
     PHUB_HIP(c, hipMemcpyAsync(r1.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    int rc = project_out(c, psi, r1.p, n);
+    rc = project_out(c, psi, r1.p, n);
     if (rc) return rc;
     PHUB_HIP(c, hipMemcpyAsync(r2.p, r1.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    PHUB_HIP(c, hipMemcpyAsync(y.p, r1.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     double b1sq = 0;
     rc = dev_dot<double, double>(c, r1.p, r1.p, n, &b1sq);
     if (rc) return rc;
@@ -1599,28 +1685,27 @@ extern "C" int phub_adjoint_minres(phub_ctx *c, const phub_op *op, const double 
     int it = 0;
     double rnorm = beta1;
     for (it = 1; it <= max_iter; ++it) {
-        // v = y / beta
-        PHUB_HIP(c, hipMemcpyAsync(v.p, y.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / beta, v.p, n, nullptr);
+        // v = y / beta; t = (H - e0) v - (beta / oldb) r1; p = psi . (H - e0) v
+        hipLaunchKernelGGL(k_minres_scale, dim3(g), dim3(256), 0, c->stream, v.p, r2.p, 1.0 / beta, n);
+        rc = phub_hv_launch<double>(c, op, v.p, t.p, 1.0, (const double *)nullptr, 0.0, false, nullptr);
         if (rc) return rc;
-        // y = P (H - e0) v
-        rc = phub_hv_launch<double>(c, op, v.p, y.p, 1.0, (const double *)nullptr, 0.0, false, nullptr);
+        hipLaunchKernelGGL(k_minres_a, dim3(g), dim3(256), 0, c->stream, t.p, v.p,
+                           it >= 2 ? (const double *)r1.p : (const double *)nullptr, psi, e0,
+                           it >= 2 ? beta / oldb : 0.0, n, c->d_red);
+        PHUB_HIP(c, hipGetLastError());
+        double d3[3];
+        rc = phub_reduce_partials(c, (int)g, 3, d3);
         if (rc) return rc;
-        rc = dev_axpby<double, double>(c, -e0, v.p, 1.0, y.p, n, nullptr);
-        if (!rc) rc = project_out(c, psi, y.p, n);
-        if (rc) return rc;
-        if (it >= 2) {
-            rc = dev_axpby<double, double>(c, -beta / oldb, r1.p, 1.0, y.p, n, nullptr);
-            if (rc) return rc;
-        }
-        double alfa = 0;
-        rc = dev_dot<double, double>(c, v.p, y.p, n, &alfa);
-        if (rc) return rc;
+        const double p = d3[0];                    // psi . (H - e0) v
+        const double alfa = d3[1] - p * d3[2];     // v . (t - p psi)
+        // y_new = t - p psi - (alfa / beta) r2, into r1's slot; then r1 <- r2, r2 <- y_new
+        hipLaunchKernelGGL(k_minres_b, dim3(g), dim3(256), 0, c->stream, t.p, psi, r2.p, p,
+                           alfa / beta, r1.p, n, c->d_red);
+        PHUB_HIP(c, hipGetLastError());
         double yy = 0;
-        rc = dev_axpby<double, double>(c, -alfa / beta, r2.p, 1.0, y.p, n, &yy);
+        rc = phub_reduce_partials(c, (int)g, 1, &yy);
         if (rc) return rc;
-        std::swap(r1.p, r2.p);   // r1 <- r2
-        PHUB_HIP(c, hipMemcpyAsync(r2.p, y.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        std::swap(r1.p, r2.p);
         oldb = beta;
         beta = std::sqrt(std::max(yy, 0.0));
         const double oldeps = epsln;
@@ -1633,15 +1718,13 @@ extern "C" int phub_adjoint_minres(phub_ctx *c, const phub_op *op, const double 
         sn = beta / gamma;
         const double phi = cs * phibar;
         phibar = sn * phibar;
-        // w1 <- w2, w2 <- w, w = (v - oldeps*w1 - delta*w2)/gamma
-        std::swap(w1.p, w2.p);
-        std::swap(w2.p, w.p);
-        PHUB_HIP(c, hipMemcpyAsync(w.p, v.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        rc = dev_axpby<double, double>(c, -oldeps, w1.p, 1.0, w.p, n, nullptr);
-        if (!rc) rc = dev_axpby<double, double>(c, -delta, w2.p, 1.0, w.p, n, nullptr);
-        if (!rc) rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / gamma, w.p, n, nullptr);
-        if (!rc) rc = dev_axpby<double, double>(c, phi, w.p, 1.0, x, n, nullptr);
-        if (rc) return rc;
+        // w = (v - oldeps wp - delta wl) / gamma, x += phi w; wp <- wl, wl <- w
+        hipLaunchKernelGGL(k_minres_c, dim3(g), dim3(256), 0, c->stream, v.p, wp.p, wl.p, oldeps,
+                           delta, 1.0 / gamma, phi, wf.p, x, n);
+        PHUB_HIP(c, hipGetLastError());
+        // (wf, wp, wl) <- (old wp, old wl, the one just written)
+        std::swap(wf.p, wp.p);
+        std::swap(wp.p, wl.p);
         rnorm = phibar;
         if (rnorm / beta1 < tol || !(beta > 0.0)) break;
     }
