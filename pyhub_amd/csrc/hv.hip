@@ -1059,8 +1059,11 @@ static int launch_dn(phub_ctx *c, const phub_op *op, const HubArgs<T> &a, const 
 
 // Tuner: on an operator's first H*v in a precision, time every down-hop tile on
 // the caller's own vectors (up kernel, then the candidate; the best of two runs
-// each) and keep the fastest.  PHUB_TUNE=0 keeps the default tile, PHUB_DN_TILE=k
-// forces tile k.  Every run
+// each) and keep the fastest.  The pick is remembered on the context by operator
+// shape (sector dimensions, maximal hop counts, precision), and a later operator
+// of the same shape takes it without timing: workloads that build one operator
+// per parameter set would otherwise pay about 24 extra H*v per operator.
+// PHUB_TUNE=0 keeps the default tile, PHUB_DN_TILE=k forces tile k.  Every run
 // rewrites out completely, so the caller's result is recomputed afterwards.
 template <typename T>
 static int tune_dn(phub_ctx *c, phub_op *op, const HubArgs<T> &a, const T *x, T *out,
@@ -1078,6 +1081,12 @@ static int tune_dn(phub_ctx *c, phub_op *op, const HubArgs<T> &a, const T *x, T 
         op->tune_dn[ti] = std::atoi(force);
         return PHUB_OK;
     }
+    for (const auto &t : c->tuned)
+        if (t.dim_up == op->dim_up && t.dim_dn == op->dim_dn && t.maxh_up == op->up.maxh &&
+            t.maxh_dn == op->dn.maxh && t.fp64 == ti) {
+            op->tune_dn[ti] = t.cfg;
+            return PHUB_OK;
+        }
     float best = 1e30f;
     int bestc = DN_DEFAULT;
     for (int cfg = 0; cfg < DN_NCFG; ++cfg) {
@@ -1102,6 +1111,7 @@ static int tune_dn(phub_ctx *c, phub_op *op, const HubArgs<T> &a, const T *x, T 
         if (tmin < best) { best = tmin; bestc = cfg; }
     }
     op->tune_dn[ti] = bestc;
+    c->tuned.push_back(phub_ctx::TunedTile{op->dim_up, op->dim_dn, op->up.maxh, op->dn.maxh, ti, bestc});
     return PHUB_OK;
 }
 

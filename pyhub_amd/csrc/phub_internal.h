@@ -24,6 +24,13 @@ struct phub_ctx {
     // until the context is destroyed
     bool defer_free = false;
     std::vector<void *> deferred;
+    // down-hop tiles the tuner picked on this context, by operator shape: operators
+    // with the same sector dimensions and maximal hop counts reuse the pick
+    struct TunedTile {
+        int64_t dim_up, dim_dn;
+        int maxh_up, maxh_dn, fp64, cfg;
+    };
+    std::vector<TunedTile> tuned;
 };
 
 // Copies and clears ordered on the context's stream.  The synchronous hipMemcpy /
