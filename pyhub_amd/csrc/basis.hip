@@ -90,7 +90,7 @@ int phub_ensure_red(phub_ctx *c, size_t nd) {
         c->red_cap = nd;
     }
     if (c->h_red_cap < 4096) {
-        PHUB_HIP(c, hipHostMalloc(&c->h_red, 4096 * sizeof(double), hipHostMallocDefault));
+        PHUB_HIP(c, hipHostMalloc(&c->h_red, 4096 * sizeof(double), hipHostMallocCoherent));
         c->h_red_cap = 4096;
     }
     return PHUB_OK;
@@ -131,12 +131,11 @@ __global__ __launch_bounds__(1024) void k_reduce_partials(const double *__restri
 int phub_reduce_partials(phub_ctx *c, int nblocks, int ncomp, double *host_out) {
     if ((size_t)nblocks * ncomp + ncomp > c->red_cap || (size_t)ncomp > c->h_red_cap)
         return phub_fail(c, PHUB_ERR_ARG, "reduction buffer too small");
-    double *dst = c->d_red + (size_t)nblocks * ncomp;
+    // the sums go straight to the pinned host buffer (coherent host memory, mapped
+    // into the device address space): no device-to-host copy after the kernel
     hipLaunchKernelGGL(k_reduce_partials, dim3(ncomp), dim3(1024), 0, c->stream, c->d_red,
-                       nblocks, ncomp, dst);
+                       nblocks, ncomp, c->h_red);
     PHUB_HIP(c, hipGetLastError());
-    PHUB_HIP(c, hipMemcpyAsync(c->h_red, dst, ncomp * sizeof(double), hipMemcpyDeviceToHost,
-                               c->stream));
     PHUB_HIP(c, hipStreamSynchronize(c->stream));
     std::memcpy(host_out, c->h_red, ncomp * sizeof(double));
     return PHUB_OK;

@@ -564,7 +564,7 @@ static int dev_multidot(phub_ctx *c, const T *basis, int64_t ld, int k, const TW
     // reduce k components; host buffer may need to grow
     if ((size_t)k > c->h_red_cap) {
         if (c->h_red) hipHostFree(c->h_red);
-        PHUB_HIP(c, hipHostMalloc(&c->h_red, (size_t)k * sizeof(double), hipHostMallocDefault));
+        PHUB_HIP(c, hipHostMalloc(&c->h_red, (size_t)k * sizeof(double), hipHostMallocCoherent));
         c->h_red_cap = k;
     }
     return phub_reduce_partials(c, (int)g, k, out);
