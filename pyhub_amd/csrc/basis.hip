@@ -221,38 +221,58 @@ __global__ void k_hop_count(const uint32_t *__restrict__ masks, int64_t dim, Pai
     if ((threadIdx.x & 63) == 0) atomicMax(maxh, c);
 }
 
-__global__ void k_hop_fill(const uint64_t *__restrict__ binom, int n, int k,
+// One wave per state: lane p of each 64-pair chunk tests pair p, a ballot gives every
+// valid pair its slot (the number of valid pairs before it, so the entries keep the
+// pair order), and only the valid lanes rank their target.  Launch with
+// hop_fill_grid(dim) blocks of 256 threads (four states per block, grid-stride).
+__global__ __launch_bounds__(256) void k_hop_fill(const uint64_t *__restrict__ binom, int n, int k,
                            const uint32_t *__restrict__ masks, int64_t dim, PairList pl, int maxh,
                            bool kmajor, int32_t *__restrict__ idx, double *__restrict__ c64,
                            float *__restrict__ c32, int16_t *__restrict__ pair,
                            double *__restrict__ eloc, const double *__restrict__ tdiag) {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= dim) return;
-    uint32_t x = masks[s];
-    int c = 0;
-    for (int p = 0; p < pl.np; ++p) {
-        int bj = pl.bj[p], bk = pl.bk[p];
-        if (((x >> bj) & 1u) && !((x >> bk) & 1u)) {
-            uint32_t xn = x ^ (1u << bj) ^ (1u << bk);
-            // JW sign: occupied sites strictly between J and K (ANTICOM(J,x)*ANTICOM(K,x'))
-            int lo = min(bj, bk), hi = max(bj, bk);
-            uint32_t between = (hi - lo > 1) ? (((1u << (hi - lo - 1)) - 1u) << (lo + 1)) : 0u;
-            double sg = (__popc(x & between) & 1) ? -1.0 : 1.0;
-            int64_t pos = rank_ref_dev(binom, xn, n, k);
-            size_t o = kmajor ? ((size_t)c * dim + s) : ((size_t)s * maxh + c);
-            idx[o] = (int32_t)pos;
-            double cc = sg * pl.t[p];
-            c64[o] = cc;
-            c32[o] = (float)cc;
-            if (pair) pair[o] = pl.pid[p];
-            ++c;
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = (int64_t)gridDim.x * 4;
+    for (int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); s < dim; s += nwaves) {
+        const uint32_t x = masks[s];
+        int c = 0;   // entries of this state so far (wave-uniform)
+        for (int p0 = 0; p0 < pl.np; p0 += 64) {
+            const int p = p0 + lane;
+            int bj = 0, bk = 0;
+            bool ok = false;
+            if (p < pl.np) {
+                bj = pl.bj[p];
+                bk = pl.bk[p];
+                ok = ((x >> bj) & 1u) && !((x >> bk) & 1u);
+            }
+            const uint64_t m = __ballot(ok);
+            if (ok) {
+                const int slot = c + __popcll(m & ((1ull << lane) - 1ull));
+                const uint32_t xn = x ^ (1u << bj) ^ (1u << bk);
+                // JW sign: occupied sites strictly between J and K (ANTICOM(J,x)*ANTICOM(K,x'))
+                const int lo = min(bj, bk), hi = max(bj, bk);
+                const uint32_t between = (hi - lo > 1) ? (((1u << (hi - lo - 1)) - 1u) << (lo + 1)) : 0u;
+                const double sg = (__popc(x & between) & 1) ? -1.0 : 1.0;
+                const int64_t pos = rank_ref_dev(binom, xn, n, k);
+                const size_t o = kmajor ? ((size_t)slot * dim + s) : ((size_t)s * maxh + slot);
+                idx[o] = (int32_t)pos;
+                const double cc = sg * pl.t[p];
+                c64[o] = cc;
+                c32[o] = (float)cc;
+                if (pair) pair[o] = pl.pid[p];
+            }
+            c += __popcll(m);
+        }
+        if (lane == 0) {
+            double e = 0.0;
+            for (int b = 0; b < n; ++b)
+                if ((x >> b) & 1u) e += tdiag[n - 1 - b];
+            eloc[s] = e;
         }
     }
-    // pad the tail of row-major tables (uniform loops read cnt only)
-    double e = 0.0;
-    for (int b = 0; b < n; ++b)
-        if ((x >> b) & 1u) e += tdiag[n - 1 - b];
-    eloc[s] = e;
+}
+
+static inline unsigned hop_fill_grid(int64_t dim) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((dim + 3) / 4, 1 << 22));
 }
 
 void phub_free_hop_table(HopTable *t) {
@@ -375,7 +395,7 @@ int phub_build_hop_tables(phub_ctx *c, const std::vector<HopSpec> &specs, HopBat
         h.idx = (int32_t *)a2 + eoff[t];
         h.c64 = (double *)(a2 + o_c64) + eoff[t];
         h.c32 = (float *)(a2 + o_c32) + eoff[t];
-        hipLaunchKernelGGL(k_hop_fill, dim3(nblk(dim[t], 256)), dim3(256), 0, c->stream, c->d_binom,
+        hipLaunchKernelGGL(k_hop_fill, dim3(hop_fill_grid(dim[t])), dim3(256), 0, c->stream, c->d_binom,
                            h.n, h.k, h.masks, dim[t], pl_of(t), mh[t], h.kmajor, h.idx, h.c64, h.c32,
                            (int16_t *)nullptr, h.eloc, (const double *)(a0 + o_td) + toff[t]);
     }
@@ -487,7 +507,7 @@ int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor
     PHUB_HIP(c, hipMemsetAsync(out->c64, 0, ne * sizeof(double), c->stream));
     PHUB_HIP(c, hipMemsetAsync(out->c32, 0, ne * sizeof(float), c->stream));
     if (all_pairs) PHUB_HIP(c, hipMalloc(&out->pair, ne * sizeof(int16_t)));
-    hipLaunchKernelGGL(k_hop_fill, dim3(nblk(dim, 256)), dim3(256), 0, c->stream, c->d_binom, n,
+    hipLaunchKernelGGL(k_hop_fill, dim3(hop_fill_grid(dim)), dim3(256), 0, c->stream, c->d_binom, n,
                        k, out->masks, dim, pl, mh, kmajor, out->idx, out->c64, out->c32,
                        out->pair, out->eloc, d_td);
     PHUB_HIP(c, hipGetLastError());
