@@ -183,52 +183,89 @@ __global__ __launch_bounds__(256) void k_rdm_diag_cols(const int64_t nu, const i
 }
 
 // Diagonal quantities from the row sums p, column sums q and R[d][b] (site J is bit
-// n - J), accumulated in LDS and added to out:
+// n - J), added to out:
 //   [0, n) densities up, [n, 2n) densities down, [2n, 3n) NI, then n x n each of
 //   <n_J,up n_K,up>, <n_K,up n_J,dn> at [(K-1) n + J-1], <n_J,dn n_K,dn>
-__global__ __launch_bounds__(256) void k_rdm_assemble(int n, int64_t nu, int64_t nd,
-                                                      const uint32_t *__restrict__ mu,
-                                                      const uint32_t *__restrict__ md,
-                                                      const double *__restrict__ q,
-                                                      const double *__restrict__ p,
-                                                      const double *__restrict__ R,
-                                                      double *__restrict__ out) {
-    extern __shared__ double acc[];
+// One workgroup per chunk of ASM_CHUNK states (the up states, then the down states,
+// in one index range), staged in LDS; one thread per output sum walks the chunk and
+// adds its register total to out once.  Launch asm_grid(nu + nd) workgroups of
+// ASM_THREADS; n <= 32.
+constexpr int ASM_CHUNK = 256;
+constexpr int ASM_THREADS = 1024;
+__global__ __launch_bounds__(ASM_THREADS) void k_rdm_assemble(int n, int64_t nu, int64_t nd,
+                                                              const uint32_t *__restrict__ mu,
+                                                              const uint32_t *__restrict__ md,
+                                                              const double *__restrict__ q,
+                                                              const double *__restrict__ p,
+                                                              const double *__restrict__ R,
+                                                              double *__restrict__ out) {
+    __shared__ uint32_t sm[ASM_CHUNK];
+    __shared__ double sw[ASM_CHUNK];
+    __shared__ double sr[ASM_CHUNK * 16];   // R rows of the chunk's down states, n <= 16
     const int nn = n * n, tot = 3 * n + 3 * nn;
-    double *dup = acc, *ddn = acc + n, *ni = acc + 2 * n;
-    double *uu = acc + 3 * n, *ud = uu + nn, *dd = ud + nn;
-    for (int i = threadIdx.x; i < tot; i += blockDim.x) acc[i] = 0.0;
-    __syncthreads();
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nu + nd;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        if (i < nu) {
-            const uint32_t x = mu[i];
-            const double w = q[i];
-            for (int J = 1; J <= n; ++J) {
-                if (!((x >> (n - J)) & 1u)) continue;
-                atomicAdd(&dup[J - 1], w);
-                for (int K = 1; K <= n; ++K)
-                    if ((x >> (n - K)) & 1u) atomicAdd(&uu[(J - 1) * n + (K - 1)], w);
-            }
+    const int64_t g0 = (int64_t)blockIdx.x * ASM_CHUNK;
+    const int cnt = (int)min<int64_t>(ASM_CHUNK, nu + nd - g0);
+    const bool rows_in_lds = n <= 16;
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const int64_t gi = g0 + i;
+        if (gi < nu) {
+            sm[i] = mu[gi];
+            sw[i] = q[gi];
         } else {
-            const int64_t d = i - nu;
-            const uint32_t x = md[d];
-            const double w = p[d];
-            const double *Rd = R + d * n;
-            for (int J = 1; J <= n; ++J) {
-                if (!((x >> (n - J)) & 1u)) continue;
-                atomicAdd(&ddn[J - 1], w);
-                atomicAdd(&ni[J - 1], Rd[n - J]);
-                for (int K = 1; K <= n; ++K) {
-                    if ((x >> (n - K)) & 1u) atomicAdd(&dd[(J - 1) * n + (K - 1)], w);
-                    atomicAdd(&ud[(K - 1) * n + (J - 1)], Rd[n - K]);
-                }
-            }
+            sm[i] = md[gi - nu];
+            sw[i] = p[gi - nu];
         }
     }
+    if (rows_in_lds)
+        for (int i = threadIdx.x; i < cnt * n; i += blockDim.x) {
+            const int64_t gi = g0 + i / n;
+            sr[i] = gi >= nu ? R[(gi - nu) * n + i % n] : 0.0;
+        }
     __syncthreads();
-    for (int i = threadIdx.x; i < tot; i += blockDim.x)
-        if (acc[i] != 0.0) atomicAdd(&out[i], acc[i]);
+    // states [0, nup_here) of the chunk are up states, the rest down states
+    const int nup_here = (int)max<int64_t>(0, min<int64_t>(cnt, nu - g0));
+    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+        double acc = 0.0;
+        auto rrow = [&](int i, int b) -> double {   // R[d][b] of chunk state i (a down state)
+            return rows_in_lds ? sr[i * n + b] : R[(g0 + i - nu) * n + b];
+        };
+        if (t < n) {                       // density up, site J = t + 1
+            const uint32_t bj = 1u << (n - 1 - t);
+            for (int i = 0; i < nup_here; ++i)
+                if (sm[i] & bj) acc += sw[i];
+        } else if (t < 2 * n) {            // density down
+            const uint32_t bj = 1u << (n - 1 - (t - n));
+            for (int i = nup_here; i < cnt; ++i)
+                if (sm[i] & bj) acc += sw[i];
+        } else if (t < 3 * n) {            // NI: R[d][n - J] over down states with J occupied
+            const int J = t - 2 * n + 1;
+            const uint32_t bj = 1u << (n - J);
+            for (int i = nup_here; i < cnt; ++i)
+                if (sm[i] & bj) acc += rrow(i, n - J);
+        } else {
+            const int u = t - 3 * n, blk = u / nn, jk = u % nn;
+            const int a = jk / n + 1, b = jk % n + 1;
+            if (blk == 0) {                // <n_J,up n_K,up>: J = a, K = b
+                const uint32_t m2 = (1u << (n - a)) | (1u << (n - b));
+                for (int i = 0; i < nup_here; ++i)
+                    if ((sm[i] & m2) == m2) acc += sw[i];
+            } else if (blk == 1) {         // [(K-1) n + J-1] = R[d][n - K] over J occupied
+                const int K = a, J = b;
+                const uint32_t bj = 1u << (n - J);
+                for (int i = nup_here; i < cnt; ++i)
+                    if (sm[i] & bj) acc += rrow(i, n - K);
+            } else {                       // <n_J,dn n_K,dn>
+                const uint32_t m2 = (1u << (n - a)) | (1u << (n - b));
+                for (int i = nup_here; i < cnt; ++i)
+                    if ((sm[i] & m2) == m2) acc += sw[i];
+            }
+        }
+        if (acc != 0.0) atomicAdd(&out[t], acc);
+    }
+}
+
+static inline unsigned asm_grid(int64_t states) {
+    return (unsigned)std::max<int64_t>(1, (states + ASM_CHUNK - 1) / ASM_CHUNK);
 }
 
 // the down-spin pair map: c^dag_J c_K applied to every down state, for every
@@ -397,9 +434,8 @@ static int hubbard_raw(phub_ctx *c, phub_op *op, const double *bra, const double
     DBuf<double> ddiag;
     PHUB_HIP(c, hipMalloc(&ddiag.p, ndiag * sizeof(double)));
     PHUB_HIP(c, hipMemsetAsync(ddiag.p, 0, ndiag * sizeof(double), c->stream));
-    hipLaunchKernelGGL(k_rdm_assemble, dim3((unsigned)std::min<int64_t>((nu + nd + 255) / 256, 256)),
-                       dim3(256), ndiag * sizeof(double), c->stream, n, nu, nd, op->up.masks,
-                       op->dn.masks, dq.p, dp.p, dR.p, ddiag.p);
+    hipLaunchKernelGGL(k_rdm_assemble, dim3(asm_grid(nu + nd)), dim3(ASM_THREADS), 0, c->stream, n,
+                       nu, nd, op->up.masks, op->dn.masks, dq.p, dp.p, dR.p, ddiag.p);
     PHUB_HIP(c, hipGetLastError());
     std::vector<double> hout(3 * nn), hdiag(ndiag);
     PHUB_HIP(c, hipMemcpyAsync(hout.data(), dout.p, 3 * nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
