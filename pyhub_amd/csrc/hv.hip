@@ -161,7 +161,7 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
     // fetches all R rows (a single 64-bit LDS read for two FP32 rows)
     // a static array (not dynamic shared memory): its LDS address is a compile-time
     // constant, so both decoded offsets go straight into the ds_read address
-    __shared__ double smem_up[UP_LDS_MAX / 8];
+    __shared__ __attribute__((aligned(16))) double smem_up[UP_LDS_MAX / 8];
     char *lds = reinterpret_cast<char *>(smem_up);
     T *rows = reinterpret_cast<T *>(lds + UP_CLS_BYTES);
     const int nu = (int)a.dim_up;
@@ -180,15 +180,21 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
     }
     for (int i = threadIdx.x; i < ncls; i += UP_THREADS) *reinterpret_cast<T *>(lds + 8 * i) = cls[i];
     __syncthreads();
-    // one column of the R staged rows is 8 bytes (two FP32 or one FP64); a packed
-    // entry carries both LDS byte offsets: e = (j * 8) << 11 | class * 8, so the
-    // decode is one shift and one mask
-    static_assert(R * sizeof(T) == 8, "8-byte LDS columns");
+    // one column of the R staged rows is 8 bytes (two FP32 or one FP64) or 16 bytes
+    // (two FP64); a packed entry carries both LDS byte offsets for 8-byte columns:
+    // e = (j * 8) << 11 | class * 8, so the decode is one shift and one mask (one
+    // more shift for 16-byte columns)
+    static_assert(R * sizeof(T) == 8 || R * sizeof(T) == 16, "8- or 16-byte LDS columns");
+    constexpr int CSH = R * sizeof(T) == 16 ? 1 : 0;   // column offset = (j * 8) << CSH
     const char *rowb = lds + UP_CLS_BYTES;
 
     auto gather = [&](uint32_t off, T *v) {
-        if constexpr (R == 2) {
+        if constexpr (R == 2 && sizeof(T) == 4) {
             const float2 p = *reinterpret_cast<const float2 *>(rowb + off);
+            v[0] = p.x;
+            v[1] = p.y;
+        } else if constexpr (R == 2) {
+            const double2 p = *reinterpret_cast<const double2 *>(rowb + off);
             v[0] = p.x;
             v[1] = p.y;
         } else {
@@ -207,7 +213,7 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
         const uint32_t xu = a.mu[u];
         const double eu = a.eu[u];
         T acc[R], self[R];
-        gather((uint32_t)u * 8u, self);
+        gather(((uint32_t)u * 8u) << CSH, self);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const uint32_t both = xu & xd[r];
@@ -238,7 +244,7 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
             for (int q = 0; q < UP_UNROLL; ++q) {
                 const T c = *reinterpret_cast<const T *>(lds + (e[q] & 0x7f8u));
                 T v[R];
-                gather(e[q] >> 11, v);
+                gather((e[q] >> 11) << CSH, v);
 #pragma unroll
                 for (int r = 0; r < R; ++r) acc[r] += c * v[r];
             }
@@ -247,7 +253,7 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
             const uint32_t e = *(const uint32_t *)((const char *)(packed + (size_t)k * nu) + uoff);
             const T c = *reinterpret_cast<const T *>(lds + (e & 0x7f8u));
             T v[R];
-            gather(e >> 11, v);
+            gather((e >> 11) << CSH, v);
 #pragma unroll
             for (int r = 0; r < R; ++r) acc[r] += c * v[r];
         }
@@ -1065,11 +1071,10 @@ static int launch_dn(phub_ctx *c, const phub_op *op, const HubArgs<T> &a, const 
 // per parameter set would otherwise pay about 24 extra H*v per operator.
 // PHUB_TUNE=0 keeps the default tile, PHUB_DN_TILE=k forces tile k.  Every run
 // rewrites out completely, so the caller's result is recomputed afterwards.
-template <typename T>
+template <typename T, int R>
 static int tune_dn(phub_ctx *c, phub_op *op, const HubArgs<T> &a, const T *x, T *out,
                    double scale, const T *prev, double beta, bool epi, unsigned g1, size_t lds,
                    const T *cls) {
-    constexpr int R = sizeof(T) == 4 ? 2 : 1;
     const int ti = sizeof(T) == 8;
     const char *env = std::getenv("PHUB_TUNE");
     if (env && env[0] == '0') {
@@ -1133,6 +1138,35 @@ int phub_hv_launch(phub_ctx *c, const phub_op *op, const T *x, T *out, double sc
     return hv_launch_main<T>(c, op, x, out, scale, prev, beta, want_dots, nblocks_out);
 }
 
+// the two-kernel Hubbard H*v with R rows per up-kernel workgroup; the operator's
+// first call in a precision runs the down-tile tuner
+template <typename T, int R>
+static int hv_two_kernel(phub_ctx *c, const phub_op *op, const T *x, T *out, double scale,
+                         const T *prev, double beta, bool epi, int *nblocks_out) {
+    int nblocks = 0;
+    const size_t lds = 0;   // k_hv_up's LDS is static (UP_LDS_MAX)
+    HubArgs<T> a = hub_args<T>(op);
+    const T *cls = (sizeof(T) == 8) ? (const T *)op->up.cls64 : (const T *)op->up.cls32;
+    const unsigned g1 = (unsigned)((op->dim_dn + R - 1) / R);
+    hipLaunchKernelGGL((k_hv_up<T, R>), dim3(g1), dim3(UP_THREADS), lds, c->stream, a,
+                       op->up.packed, cls, op->up.ncls, x, out);
+    PHUB_HIP(c, hipGetLastError());
+    const int ti = sizeof(T) == 8;
+    phub_op *mop = const_cast<phub_op *>(op);
+    if (mop->tune_dn[ti] < 0) {
+        int rc = tune_dn<T, R>(c, mop, a, x, out, scale, prev, beta, epi, g1, lds, cls);
+        if (rc) return rc;
+        // the tuner's runs overwrote out: recompute the up part
+        hipLaunchKernelGGL((k_hv_up<T, R>), dim3(g1), dim3(UP_THREADS), lds, c->stream, a,
+                           op->up.packed, cls, op->up.ncls, x, out);
+    }
+    int rc = launch_dn<T>(c, op, a, x, out, scale, prev, beta, epi, mop->tune_dn[ti], &nblocks);
+    if (rc) return rc;
+    PHUB_HIP(c, hipGetLastError());
+    if (nblocks_out) *nblocks_out = nblocks;
+    return PHUB_OK;
+}
+
 // the one-body (T, per-site U) Hubbard operator and the Heisenberg operator
 template <typename T>
 static int hv_launch_main(phub_ctx *c, const phub_op *op, const T *x, T *out, double scale,
@@ -1140,31 +1174,14 @@ static int hv_launch_main(phub_ctx *c, const phub_op *op, const T *x, T *out, do
     int nblocks = 0;
     const bool epi = want_dots || prev != nullptr || scale != 1.0;
     if (op->kind == 0 && op->up.packed && op->dn.csr) {
-        // two-kernel path: R rows of the state in LDS for the up hops
+        // two-kernel path: R rows of the state in LDS for the up hops (FP32: 2; FP64: 2
+        // when two rows fit, else 1)
+        const size_t row_bytes = (size_t)op->dim_up * sizeof(T);
+        if (sizeof(T) == 8 && UP_CLS_BYTES + 2 * row_bytes <= UP_LDS_MAX)
+            return hv_two_kernel<T, 2>(c, op, x, out, scale, prev, beta, epi, nblocks_out);
         constexpr int R = sizeof(T) == 4 ? 2 : 1;
-        const size_t lds = 0;   // k_hv_up's LDS is static (UP_LDS_MAX)
-        if (UP_CLS_BYTES + (size_t)R * op->dim_up * sizeof(T) <= UP_LDS_MAX) {
-            HubArgs<T> a = hub_args<T>(op);
-            const T *cls = (sizeof(T) == 8) ? (const T *)op->up.cls64 : (const T *)op->up.cls32;
-            const unsigned g1 = (unsigned)((op->dim_dn + R - 1) / R);
-            hipLaunchKernelGGL((k_hv_up<T, R>), dim3(g1), dim3(UP_THREADS), lds, c->stream, a,
-                               op->up.packed, cls, op->up.ncls, x, out);
-            PHUB_HIP(c, hipGetLastError());
-            const int ti = sizeof(T) == 8;
-            phub_op *mop = const_cast<phub_op *>(op);
-            if (mop->tune_dn[ti] < 0) {
-                int rc = tune_dn<T>(c, mop, a, x, out, scale, prev, beta, epi, g1, lds, cls);
-                if (rc) return rc;
-                // the tuner's runs overwrote out: recompute the up part
-                hipLaunchKernelGGL((k_hv_up<T, R>), dim3(g1), dim3(UP_THREADS), lds, c->stream, a,
-                                   op->up.packed, cls, op->up.ncls, x, out);
-            }
-            int rc = launch_dn<T>(c, op, a, x, out, scale, prev, beta, epi, mop->tune_dn[ti], &nblocks);
-            if (rc) return rc;
-            PHUB_HIP(c, hipGetLastError());
-            if (nblocks_out) *nblocks_out = nblocks;
-            return PHUB_OK;
-        }
+        if (UP_CLS_BYTES + R * row_bytes <= UP_LDS_MAX)
+            return hv_two_kernel<T, R>(c, op, x, out, scale, prev, beta, epi, nblocks_out);
     }
     if (op->kind == 0) {
         if (op->n > 32) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "Hubbard H*v needs n <= 32");
