@@ -290,7 +290,9 @@ int phub_grad_hubbard(phub_ctx *ctx, const phub_op *op, const double *d_bra, con
 int phub_hf_grad_heisenberg(phub_ctx *ctx, const phub_op *op, const double *d_psi,
                             double *grad_j);
 /* Adjoint solve (H - e0) x = b with b orthogonal to psi, by MINRES restricted to
- * the complement of psi (FP64).  Returns iterations in *iters. */
+ * the complement of psi (FP64; seven work vectors of the operator dimension are
+ * allocated for the call; one iteration is one H*v, four vector passes and two host
+ * round trips).  Returns iterations in *iters. */
 int phub_adjoint_minres(phub_ctx *ctx, const phub_op *op, const double *d_psi, double e0,
                         const double *d_b, double *d_x, int max_iter, double tol,
                         int *iters, double *rel_res);

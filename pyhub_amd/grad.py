@@ -12,7 +12,7 @@ against central finite differences of the engine's energies and RDMs.
   adjoint method.  With G_s = dL/dgamma_s = 2 (gamma_s - target_s) (symmetric),
       g = dL/dpsi = 2 sum_s sum_JK G_s[J][K] c^dag_Js c_Ks psi
   is one H*v of an operator with T_up = G_up, T_dn = G_dn, U = 0 (same kernels);
-      (H - E0) lam = P_perp g      (MINRES on the complement of psi, 5 vectors),
+      (H - E0) lam = P_perp g      (MINRES on the complement of psi, 7 work vectors),
       dL/dtheta = -<lam| dH/dtheta |psi>      (phub_grad_hubbard).
   No Krylov basis is stored for the adjoint.
 
