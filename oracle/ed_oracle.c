@@ -29,9 +29,20 @@
  *   orc_two_rdm        its (J,J,J,J) entries equal the double occupancy NI;
  *                      E_int = sum UL * DBL_OCC equals <psi|H_U|psi>.
  *   orc_lanczos        converged E0 against dense FP64 eigh.
- *   orc_one_rdm/...    sum rules (tr rho = N), Hellmann-Feynman vs finite
- *                      differences of E0, brute-force operator matrices.
- *   orc_excite         dense-ED poles/weights sum rule sum_k w_k = 1.
+ *   orc_one_rdm, orc_ni, orc_two_rdm, orc_j_spin
+ *                      bit-exact against the reference's own ONE_RDM / TWO_RDM
+ *                      / J_SPIN (static_rq.f90:153-395, oracle/_ref); sum rules
+ *                      (tr rho = N), Hellmann-Feynman vs finite differences.
+ *   orc_j_spin_heis    bit-exact against the reference's Heisenberg J_SPIN
+ *                      (static_rq_heis.f90:58-116, oracle/_ref).
+ *   orc_excite         bit-exact against the reference's GF_E_UP / GF_E_DOWN /
+ *                      GF_H_UP / GF_H_DOWN (spgf_ed.f90:304-429, oracle/_ref);
+ *                      sum rule sum_k w_k = 1.
+ *   orc_cf_chain       on a sector the chain exhausts, its continued fraction
+ *                      equals the dense-ED spectral sum sum_k |<k|x>|^2/(z-E_k);
+ *                      alpha_0 = <x|H|x>/<x|x> of the pinned HPROD.
+ *   orc_tridiag_eig    through orc_lanczos against dense eigh.
+ * The whole file also runs under ASan + UBSan (Makefile target `sanitize`).
  */
 #include <stdint.h>
 #include <stdlib.h>

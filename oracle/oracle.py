@@ -36,7 +36,9 @@ i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 def lib():
     global _lib
     if _lib is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # PHUB_ORACLE_SAN=1 loads the ASan/UBSan build (oracle/Makefile `sanitize`)
+        name = "liboracle_san.so" if os.environ.get("PHUB_ORACLE_SAN") == "1" else "liboracle.so"
+        path = os.path.join(_HERE, name)
         if not os.path.exists(path):
             raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
         L = C.CDLL(path)
@@ -88,9 +90,15 @@ def _ref_libs():
         _ref_hub.phref_hub_setup.argtypes = [C.c_int] * 5 + [i32p, i32p, f64p, f64p]
         _ref_hub.phref_hub_hprod.argtypes = [f64p, f64p]
         _ref_hub.phref_hub_setup_ul.argtypes = [C.c_int] * 5 + [i32p, i32p, f64p, f64p, f64p]
+        _ref_hub.phref_hub_one_rdm.argtypes = [f64p, f64p, f64p]
+        _ref_hub.phref_hub_two_rdm.argtypes = [f64p, f64p, C.c_int, f64p, f64p]
+        _ref_hub.phref_hub_j_spin.argtypes = [f64p, f64p, f64p]
+        _ref_hub.phref_hub_gf.argtypes = ([C.c_int] * 6 + [i32p, i32p, C.c_int, C.c_int, i32p, i32p,
+                                           f64p, C.c_int, f64p])
         _ref_heis = C.CDLL(os.path.join(_HERE, "_ref", "libref_heis.so"))
         _ref_heis.phref_heis_setup.argtypes = [C.c_int, C.c_int, i32p, f64p]
         _ref_heis.phref_heis_hprod.argtypes = [f64p, f64p]
+        _ref_heis.phref_heis_j_spin.argtypes = [f64p, f64p, f64p]
     return _ref_hub, _ref_heis
 
 
@@ -174,6 +182,61 @@ class HubbardModel:
         w = np.empty_like(v)
         hub.phref_hub_hprod(v, w)
         return w
+
+    def _ref_setup(self):
+        hub, _ = _ref_libs()
+        hub.phref_hub_setup(self.n, self.nup, self.ndn, self.nsup, self.nsdn,
+                            self.bup.astype(np.int32), self.bdn.astype(np.int32),
+                            self.T, self.U if self.UL is None else np.zeros(self.n))
+        return hub
+
+    def one_rdm_ref(self, bra, ket=None):
+        """DENS_MAT of the reference's own ONE_RDM (static_rq.f90:153-244), as
+        one_rdm(): [s][J-1][K-1]."""
+        hub = self._ref_setup()
+        ket = bra if ket is None else ket
+        out = np.zeros(2 * self.n * self.n)
+        hub.phref_hub_one_rdm(np.ascontiguousarray(bra, dtype=np.float64),
+                              np.ascontiguousarray(ket, dtype=np.float64), out)
+        return out.reshape(2, self.n, self.n).transpose(0, 2, 1).copy()
+
+    def two_rdm_ref(self, bra, ket=None, full=False):
+        """The reference's own TWO_RDM (static_rq.f90:247-320): (NI, DBL_OCC or
+        None); DBL_OCC as two_rdm(): [J-1][K-1][M-1][N-1]."""
+        hub = self._ref_setup()
+        ket = bra if ket is None else ket
+        db = np.zeros(self.n ** 4)
+        ni = np.zeros(self.n)
+        hub.phref_hub_two_rdm(np.ascontiguousarray(bra, dtype=np.float64),
+                              np.ascontiguousarray(ket, dtype=np.float64), int(full), db, ni)
+        d4 = db.reshape(self.n, self.n, self.n, self.n).transpose(3, 2, 1, 0).copy() if full else None
+        return ni, d4
+
+    def sisj_ref(self, bra, ket=None):
+        """SMATRIX of the reference's own J_SPIN (static_rq.f90:323-395)."""
+        hub = self._ref_setup()
+        ket = bra if ket is None else ket
+        out = np.zeros(self.n * self.n)
+        hub.phref_hub_j_spin(np.ascontiguousarray(bra, dtype=np.float64),
+                             np.ascontiguousarray(ket, dtype=np.float64), out)
+        return out.reshape(self.n, self.n).T.copy()
+
+    def excite_ref(self, psi, spin, create, ncomp=None):
+        """Excitation vectors of the reference's own GF_E_UP / GF_E_DOWN / GF_H_UP /
+        GF_H_DOWN (spgf_ed.f90:304-429) for sites 1..ncomp: (target model,
+        array [site-1][target state])."""
+        hub, _ = _ref_libs()
+        ncomp = self.n if ncomp is None else int(ncomp)
+        dn = 1 if create else -1
+        tgt = HubbardModel(self.n, self.nup + (dn if spin == 0 else 0),
+                           self.ndn + (dn if spin == 1 else 0), self.T, self.U)
+        kind = (0 if create else 2) + (1 if spin == 1 else 0)
+        q = np.zeros(tgt.dim * ncomp)
+        hub.phref_hub_gf(kind, self.n, self.nup, self.ndn, self.nsup, self.nsdn,
+                         self.bup.astype(np.int32), self.bdn.astype(np.int32),
+                         tgt.nsup, tgt.nsdn, tgt.bup.astype(np.int32), tgt.bdn.astype(np.int32),
+                         np.ascontiguousarray(psi, dtype=np.float64), ncomp, q)
+        return tgt, q.reshape(ncomp, tgt.dim)
 
     def dense(self, fast=True):
         H = np.zeros((self.dim, self.dim))
@@ -287,6 +350,16 @@ class HeisenbergModel:
         args = (1, self.n, self.n // 2, 0, self.nsup, 1, self.bup, self.bup,
                 np.zeros(1), np.zeros(1), self.Jm, 0, 1)
         return _lanczos(args, v0, maxlcz, acc, self.dim)
+
+    def sisj_ref(self, bra, ket=None):
+        """SMATRIX of the reference's own Heisenberg J_SPIN (static_rq_heis.f90:58-116)."""
+        _, heis = _ref_libs()
+        heis.phref_heis_setup(self.n, self.nsup, self.bup.astype(np.int32), self.Jm)
+        ket = bra if ket is None else ket
+        out = np.zeros(self.n * self.n)
+        heis.phref_heis_j_spin(np.ascontiguousarray(bra, dtype=np.float64),
+                               np.ascontiguousarray(ket, dtype=np.float64), out)
+        return out.reshape(self.n, self.n).T.copy()
 
     def sisj(self, bra, ket=None):
         ket = bra if ket is None else ket

@@ -1,6 +1,7 @@
 ! ref_driver_heis.f90 -- test harness (not product code) that exposes the
-! reference's own Heisenberg HPROD (pyhub/libs/src/hprod_heis.f90:1-61),
-! compiled from its sources, through a C ABI for the oracle checks.
+! reference's own Heisenberg HPROD (pyhub/libs/src/hprod_heis.f90:1-61) and
+! its J_SPIN (static_rq_heis.f90:58-116), compiled from their sources, through
+! a C ABI for the oracle checks.
 MODULE PHREF_HEIS
   USE ISO_C_BINDING
   USE HUBMOD
@@ -22,5 +23,15 @@ CONTAINS
     REAL(C_DOUBLE), INTENT(IN) :: V(*)
     REAL(C_DOUBLE), INTENT(OUT) :: W(*)
     CALL HPROD(V(1:NSTATES), W(1:NSTATES))
+  END SUBROUTINE
+  ! SMATRIX(J,K) of the Heisenberg J_SPIN (Fortran order)
+  SUBROUTINE PHREF_HEIS_J_SPIN(BRA, KET, SM) BIND(C, NAME="phref_heis_j_spin")
+    REAL(C_DOUBLE), INTENT(IN) :: BRA(*), KET(*)
+    REAL(C_DOUBLE), INTENT(OUT) :: SM(*)
+    REAL*8, ALLOCATABLE :: S(:,:)
+    ALLOCATE(S(NORB,NORB))
+    CALL J_SPIN(BRA(1:NSTATES), KET(1:NSTATES), S)
+    SM(1:NORB*NORB) = RESHAPE(S, (/NORB*NORB/))
+    DEALLOCATE(S)
   END SUBROUTINE
 END MODULE

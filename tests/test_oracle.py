@@ -121,6 +121,137 @@ def test_oracle_excitation_sum_rule():
             assert abs(p @ p + h @ h - 1.0) < 1e-12
 
 
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) absent")
+@pytest.mark.parametrize("n,nu,nd,seed", [(5, 2, 3, 0), (6, 3, 3, 1), (6, 1, 4, 2), (7, 3, 2, 3),
+                                          (5, 0, 2, 4), (6, 5, 5, 5)])
+def test_rdm_restatement_bit_exact_vs_reference(n, nu, nd, seed):
+    """orc_one_rdm, orc_ni, orc_two_rdm and orc_j_spin against the reference's own
+    ONE_RDM / TWO_RDM / J_SPIN (static_rq.f90:153-395), compiled from their line
+    ranges: both ONE_RDM branches (N_up != N_dn and the copy when equal), NI and
+    the full DBL_OCC, with a bra different from the ket."""
+    rng = np.random.default_rng(seed)
+    T = rng.normal(size=(n, n))
+    T = (T + T.T) / 2
+    m = O.HubbardModel(n, nu, nd, T, rng.uniform(1, 5, n))
+    bra, ket = rng.normal(size=m.dim), rng.normal(size=m.dim)
+    assert np.array_equal(m.one_rdm(bra, ket), m.one_rdm_ref(bra, ket))
+    ni, d4 = m.two_rdm_ref(bra, ket, full=True)
+    assert np.array_equal(m.ni(bra, ket), ni)
+    assert np.array_equal(m.two_rdm(bra, ket), d4)
+    assert np.array_equal(m.sisj(bra, ket), m.sisj_ref(bra, ket))
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) absent")
+@pytest.mark.parametrize("n,seed", [(4, 0), (6, 1), (8, 2)])
+def test_heisenberg_jspin_bit_exact_vs_reference(n, seed):
+    """orc_j_spin_heis against the reference's own Heisenberg J_SPIN
+    (static_rq_heis.f90:58-116), every (J, K) including the diagonal."""
+    rng = np.random.default_rng(seed)
+    J = rng.normal(size=(n, n))
+    m = O.HeisenbergModel(n, (J + J.T) / 2)
+    bra, ket = rng.normal(size=m.dim), rng.normal(size=m.dim)
+    assert np.array_equal(m.sisj(bra, ket), m.sisj_ref(bra, ket))
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) absent")
+@pytest.mark.parametrize("n,nu,nd,seed", [(5, 2, 2, 0), (6, 3, 3, 1), (6, 2, 3, 2), (5, 0, 1, 3),
+                                          (6, 5, 4, 4)])
+def test_excitation_bit_exact_vs_reference(n, nu, nd, seed):
+    """orc_excite against the reference's GF_E_UP / GF_E_DOWN / GF_H_UP / GF_H_DOWN
+    (spgf_ed.f90:304-429) for every site: the ANTICOM signs, the (-1)^N_dn factor of
+    the up operators (odd and even N_dn) and the target placement."""
+    rng = np.random.default_rng(seed)
+    m = O.HubbardModel(n, nu, nd, O.ring_T(n), 2.0)
+    psi = rng.normal(size=m.dim)
+    for spin in (0, 1):
+        for create in (1, 0):
+            full = (nu if spin == 0 else nd) == (n if create else 0)
+            if full:
+                continue
+            tgt, q = m.excite_ref(psi, spin, create)
+            for site in range(1, n + 1):
+                t2, v = m.excite(psi, site, spin, create)
+                assert t2.dim == tgt.dim
+                assert np.array_equal(v, q[site - 1]), (spin, create, site)
+
+
+def _cf_value(alpha, beta, norm2, z):
+    """norm2 / (z - a0 - b0^2 / (z - a1 - b1^2 / ...)), evaluated from the bottom."""
+    g = np.zeros_like(z)
+    for k in range(len(alpha) - 1, -1, -1):
+        tail = beta[k] ** 2 * g if k < len(alpha) - 1 else 0.0
+        g = 1.0 / (z - alpha[k] - tail)
+    return norm2 * g
+
+
+def test_cf_chain_pinned_by_dense_spectral_sum():
+    """orc_cf_chain: on a sector the chain exhausts, the continued fraction equals
+    the dense-ED spectral sum sum_k |<k|x>|^2 / (z - E_k) at every z; its first
+    coefficient is the Rayleigh quotient <x|H|x>/<x|x> of the pinned HPROD and its
+    norm is <x|x>.  A dropped beta term, a shifted beta index or a wrong sign moves
+    the poles and fails the comparison."""
+    rng = np.random.default_rng(7)
+    T = O.square_T(2, 3, t=1.0, tp=0.3, periodic=False) + np.diag(rng.normal(0, 0.3, 6))
+    m = O.HubbardModel(6, 2, 1, T, 3.0)          # dim 15 x 6 = 90
+    x = rng.normal(size=m.dim)
+    # past the sector dimension: rounding keeps beta from vanishing, and the extra
+    # (ghost) levels it adds carry no weight
+    a, b, n2 = m.cf_chain(x, m.dim + 20, fast=False)
+    assert abs(n2 - x @ x) < 1e-12 * (x @ x)
+    assert abs(a[0] - x @ m.hv(x) / (x @ x)) < 1e-12
+    ev, V = np.linalg.eigh(m.dense(fast=False))
+    w = (V.T @ x) ** 2
+    z = np.linspace(-12, 12, 97) + 0.4j
+    g_dense = (w[None, :] / (z[:, None] - ev[None, :])).sum(axis=1)
+    # truncate where the Krylov space is exhausted (beta at rounding level)
+    m_eff = next((k + 1 for k in range(len(b)) if b[k] < 1e-9), len(b))
+    g_cf = _cf_value(a[:m_eff], b[:m_eff], n2, z)
+    assert np.max(np.abs(g_cf - g_dense)) < 1e-10 * np.max(np.abs(g_dense))
+    # the lowest Ritz value of the chain's tridiagonal is the ground energy
+    tri = np.diag(a[:m_eff]) + np.diag(b[:m_eff - 1], 1) + np.diag(b[:m_eff - 1], -1)
+    assert abs(np.linalg.eigvalsh(tri)[0] - ev[0]) < 1e-10
+
+
+@pytest.mark.skipif(os.environ.get("PHUB_SKIP_SAN") == "1", reason="sanitizer run disabled")
+def test_oracle_under_asan_ubsan(tmp_path):
+    """The C oracle built with -fsanitize=address,undefined (oracle/Makefile target
+    `sanitize`) runs the Hubbard and Heisenberg restatements, the RDMs, the
+    excitations, the chain and the Lanczos without a report."""
+    import subprocess
+    import sys
+    odir = os.path.join(ROOT, "oracle")
+    r = subprocess.run(["make", "-C", odir, "liboracle_san.so"], capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.skip("sanitizer build unavailable: " + r.stderr[-200:])
+    asan = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True,
+                          text=True).stdout.strip()
+    if not os.path.isabs(asan):
+        pytest.skip("libasan runtime not found")
+    script = tmp_path / "san_run.py"
+    script.write_text(
+        "import numpy as np\n"
+        "from oracle import oracle as O\n"
+        "rng = np.random.default_rng(0)\n"
+        "m = O.HubbardModel(6, 3, 2, O.ring_T(6), 2.0)\n"
+        "v = rng.normal(size=m.dim)\n"
+        "m.hv(v); m.hv(v, fast=True, nthreads=2)\n"
+        "m.one_rdm(v); m.ni(v); m.sisj(v); m.two_rdm(v)\n"
+        "m.excite(v, 2, 0, 1); m.excite(v, 3, 1, 0)\n"
+        "m.cf_chain(v, 12)\n"
+        "m.lanczos(v, maxlcz=60)\n"
+        "h = O.HeisenbergModel(6, -O.ring_T(6))\n"
+        "w = rng.normal(size=h.dim)\n"
+        "h.hv(w); h.hv(w, fast=True); h.sisj(w)\n"
+        "print('san ok')\n")
+    env = dict(os.environ, PHUB_ORACLE_SAN="1", LD_PRELOAD=asan,
+               ASAN_OPTIONS="detect_leaks=0:abort_on_error=1",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1",
+               PYTHONPATH=ROOT)
+    r = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, env=env,
+                       cwd=ROOT, timeout=300)
+    assert r.returncode == 0 and "san ok" in r.stdout, r.stderr[-2000:]
+
+
 def test_heisenberg_reference_convention():
     """The reference Heisenberg HPROD is 2x the standard model on bipartite bond
     graphs; with frustrating bonds it is not (its flip sign is -(-1)^{J+K})."""
