@@ -921,3 +921,67 @@ void orc_cf_chain(int model, int n, int nup, int ndn, long long nsup, long long 
     }
     free(V0); free(V); free(V2);
 }
+
+/* ---------------------------------------------------------------------------
+ * One column of the Heisenberg HPROD matrix for clusters far beyond a dense or
+ * even a full-vector restatement (n = 36: 9.1e9 states, indices past 2^32).
+ * orc_unrank_ref inverts the closed-form reference rank (the order of
+ * compute_basis.f90:67-167): the colex index C(n,k)-1-idx of rev_n(x), taken
+ * greedily from the highest bit.  orc_heis_column gives H e_idx as (row, value)
+ * pairs: for every ordered pair (J,K) the same KINOP / ANTICOM factors as
+ * hprod_heis.f90:22-53 (sign_convention 0: the reference's -ac*J(J,K); 1: +ac*J(J,K),
+ * the element of the physical spin model), the J=J term of :11-21 and the Ising
+ * products of :54-58 on the diagonal.  Rows are unsorted; a row can repeat when
+ * two pairs reach it.  Returns the number of pairs written (at most n*n+1).
+ * ------------------------------------------------------------------------- */
+u64 orc_unrank_ref(int n, int k, u64 idx) {
+    u64 c = orc_binom(n, k) - 1 - idx, y = 0;
+    int p = n - 1;
+    for (int i = k; i >= 1; --i) {
+        while (p >= 0 && orc_binom(p, i) > c) --p;
+        y |= (u64)1 << p;
+        c -= orc_binom(p, i);
+        --p;
+    }
+    u64 x = 0;                                   /* x = rev_n(y) */
+    for (int b = 0; b < n; ++b)
+        if (y >> b & 1) x |= (u64)1 << (n - 1 - b);
+    return x;
+}
+
+long long orc_heis_column(int n, const double *Jm, int sign_convention, u64 idx,
+                          u64 *rows, double *vals) {
+    const int k = n / 2;
+    const u64 full = (n >= 64) ? ~(u64)0 : (((u64)1 << n) - 1);
+    const u64 x = orc_unrank_ref(n, k, idx), bd = full - x;
+    long long cnt = 0;
+    double dg = 0.0;
+    for (int J = 1; J <= n; ++J) {
+        double jjj = Jm[(J - 1) * n + (J - 1)];
+        if (fabs(jjj) > 1.e-14)
+            if ((is_part(n, J, x) && !is_part(n, J, bd)) || (!is_part(n, J, x) && is_part(n, J, bd)))
+                dg += 0.5 * jjj;
+        for (int K = 1; K <= n; ++K) {
+            double jjk = Jm[(J - 1) * n + (K - 1)];
+            if (fabs(jjk) > 1.e-14 && J != K) {
+                long long nw = kinop(n, x, J, K);
+                if (nw > 0) {
+                    double ac = anticom(n, J, x);
+                    ac = ac * anticom(n, K, (u64)nw);
+                    long long nw2 = kinop(n, bd, K, J);
+                    if ((u64)nw2 == full - (u64)nw) {
+                        ac = ac * anticom(n, J, bd);
+                        ac = ac * anticom(n, K, (u64)nw2);
+                        rows[cnt] = (u64)colex_rank_ref(n, k, (u64)nw);
+                        vals[cnt] = sign_convention ? ac * jjk : -ac * jjk;
+                        ++cnt;
+                    }
+                }
+            }
+            dg += szf(n, J, x, bd) * szf(n, K, x, bd) * jjk;
+        }
+    }
+    rows[cnt] = idx;
+    vals[cnt] = dg;
+    return cnt + 1;
+}

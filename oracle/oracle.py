@@ -74,6 +74,10 @@ def lib():
         L.orc_cf_chain.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_longlong, C.c_longlong,
                                    u64p, u64p, f64p, f64p, f64p, C.c_int, C.c_int, f64p, C.c_int,
                                    f64p, f64p, C.POINTER(C.c_double)]
+        L.orc_unrank_ref.restype = C.c_uint64
+        L.orc_unrank_ref.argtypes = [C.c_int, C.c_int, C.c_uint64]
+        L.orc_heis_column.restype = C.c_longlong
+        L.orc_heis_column.argtypes = [C.c_int, f64p, C.c_int, C.c_uint64, u64p, f64p]
         _lib = L
     return _lib
 
@@ -119,6 +123,24 @@ def hilbert(n: int, k: int) -> np.ndarray:
 
 def rank_ref(n: int, k: int, x: int) -> int:
     return int(lib().orc_rank_ref(n, k, int(x)))
+
+
+def unrank_ref(n: int, k: int, idx: int) -> int:
+    """Mask of reference-order index idx (inverse of rank_ref; any n <= 63)."""
+    return int(lib().orc_unrank_ref(n, k, int(idx)))
+
+
+def heis_column(n: int, Jm, idx: int, sign_convention: int = 0) -> dict:
+    """Column idx of the Heisenberg HPROD matrix (orc_heis_column) as {row: value}
+    with repeated rows summed; works for n = 36 (indices past 2^32)."""
+    Jm = np.ascontiguousarray(Jm, dtype=np.float64)
+    rows = np.zeros(n * n + 1, dtype=np.uint64)
+    vals = np.zeros(n * n + 1)
+    m = int(lib().orc_heis_column(n, Jm, int(sign_convention), int(idx), rows, vals))
+    out: dict = {}
+    for r, v in zip(rows[:m].tolist(), vals[:m].tolist()):
+        out[r] = out.get(r, 0.0) + v
+    return out
 
 
 def brute_basis(n: int, k: int) -> np.ndarray:

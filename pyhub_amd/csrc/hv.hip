@@ -1167,6 +1167,13 @@ static int hv_two_kernel(phub_ctx *c, const phub_op *op, const T *x, T *out, dou
     return PHUB_OK;
 }
 
+// PHUB_UP_ROWS=1 keeps FP64 on the one-row up kernel even where two rows fit
+// (the path of the large sectors, exercised on small ones by the tests)
+static bool phub_force_r1() {
+    const char *e = std::getenv("PHUB_UP_ROWS");
+    return e && e[0] == '1';
+}
+
 // the one-body (T, per-site U) Hubbard operator and the Heisenberg operator
 template <typename T>
 static int hv_launch_main(phub_ctx *c, const phub_op *op, const T *x, T *out, double scale,
@@ -1177,7 +1184,7 @@ static int hv_launch_main(phub_ctx *c, const phub_op *op, const T *x, T *out, do
         // two-kernel path: R rows of the state in LDS for the up hops (FP32: 2; FP64: 2
         // when two rows fit, else 1)
         const size_t row_bytes = (size_t)op->dim_up * sizeof(T);
-        if (sizeof(T) == 8 && UP_CLS_BYTES + 2 * row_bytes <= UP_LDS_MAX)
+        if (sizeof(T) == 8 && UP_CLS_BYTES + 2 * row_bytes <= UP_LDS_MAX && !phub_force_r1())
             return hv_two_kernel<T, 2>(c, op, x, out, scale, prev, beta, epi, nblocks_out);
         constexpr int R = sizeof(T) == 4 ? 2 : 1;
         if (UP_CLS_BYTES + R * row_bytes <= UP_LDS_MAX)

@@ -1,0 +1,105 @@
+"""GPU tests of the up-hop kernel paths of the two-kernel Hubbard H*v against the
+oracle (hprod_hubbard.f90:17-118 restated, bit-exact against the reference):
+
+- lattice hoppings (few signed values) and random hoppings (many); a sector whose
+  rows do not fit LDS (the fallback kernel);
+- FP64 with two rows per workgroup where they fit (dim_up <= 9,472), and with one
+  row: forced by PHUB_UP_ROWS=1 on small sectors, and on a sector whose up
+  dimension (11,440) is past the two-row limit, over every down-hop tile.
+
+Tolerances: FP64 within 1e-13 of the oracle relative to max|w|; FP32 within 2e-5."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(x, dtype=torch.float64):
+    return torch.as_tensor(np.asarray(x), dtype=dtype, device="cuda")
+
+
+def _check(op, m, rng):
+    v = rng.normal(size=m.dim)
+    ref = m.hv(v, fast=True, nthreads=8)
+    sc = max(1.0, np.max(np.abs(ref)))
+    w64 = op.hv(dev(v)).cpu().numpy()
+    assert np.max(np.abs(w64 - ref)) <= 1e-13 * sc
+    w32 = op.hv(dev(v, torch.float32)).double().cpu().numpy()
+    assert np.max(np.abs(w32 - ref)) <= 2e-5 * sc
+
+
+def _lattice(n, rng, tp=0.2, eps=True):
+    """A ring (n not a square) or torus with t = 1, t' on the next-nearest bonds
+    and random on-site energies: four signed hopping values at most."""
+    if n in (9, 16):
+        L = int(round(np.sqrt(n)))
+        T = O.square_T(L, L, t=1.0, tp=-tp, periodic=True)
+    else:
+        T = O.ring_T(n)
+        for i in range(n):
+            j = (i + 2) % n
+            T[i, j] = T[j, i] = tp
+    if eps:
+        T[np.arange(n), np.arange(n)] = rng.normal(0, 0.4, n)
+    return T
+
+
+@pytest.mark.parametrize("n,nu,nd,seed", [(8, 4, 4, 0), (9, 4, 5, 1), (10, 3, 6, 2), (6, 0, 3, 3),
+                                          (12, 6, 2, 4), (16, 8, 1, 5)])
+def test_lattice_hoppings(ctx, n, nu, nd, seed):
+    from pyhub_amd.engine import HubbardOp
+    rng = np.random.default_rng(seed)
+    T = _lattice(n, rng)
+    U = rng.uniform(2, 6, n)
+    op = HubbardOp(ctx, n, nu, nd, T, U)
+    _check(op, O.HubbardModel(n, nu, nd, T, U), rng)
+
+
+def test_lattice_hoppings_large_up_sector(ctx):
+    """18 sites, 8 up (43,758 states): one row (171 KB in FP32) does not fit LDS, so
+    the H*v runs the one-lane-per-state fallback kernel (k_hv_hub)."""
+    from pyhub_amd.engine import HubbardOp
+    rng = np.random.default_rng(11)
+    n, nu, nd = 18, 8, 1
+    T = _lattice(n, rng)
+    op = HubbardOp(ctx, n, nu, nd, T, 4.0)
+    _check(op, O.HubbardModel(n, nu, nd, T, 4.0), rng)
+
+
+@pytest.mark.parametrize("lattice", [True, False])
+def test_fp64_one_row_forced(ctx, monkeypatch, lattice):
+    """PHUB_UP_ROWS=1: the FP64 one-row up kernel on sectors where two rows fit."""
+    from pyhub_amd.engine import HubbardOp
+    monkeypatch.setenv("PHUB_UP_ROWS", "1")
+    rng = np.random.default_rng(21)
+    n, nu, nd = 10, 5, 4
+    if lattice:
+        T = _lattice(n, rng)
+    else:
+        T = rng.normal(size=(n, n))
+        T = (T + T.T) / 2
+    U = rng.uniform(2, 6, n)
+    op = HubbardOp(ctx, n, nu, nd, T, U)
+    _check(op, O.HubbardModel(n, nu, nd, T, U), rng)
+
+
+@pytest.mark.parametrize("tile", list(range(14)))
+@pytest.mark.parametrize("lattice", [True, False])
+def test_fp64_one_row_large_up_sector_all_tiles(ctx, monkeypatch, tile, lattice):
+    """n = 16, 7 up and 1 down (dim_up = 11,440: two FP64 rows do not fit LDS, so
+    FP64 runs the one-row kernel, the config-2 FP64 path), every down tile."""
+    from pyhub_amd.engine import HubbardOp
+    monkeypatch.setenv("PHUB_DN_TILE", str(tile))
+    rng = np.random.default_rng(300 + tile)
+    n, nu, nd = 16, 7, 1
+    if lattice:
+        T = _lattice(n, rng)
+    else:
+        T = rng.normal(size=(n, n)) * 0.5
+        T = (T + T.T) / 2
+    U = rng.uniform(2, 6, n)
+    op = HubbardOp(ctx, n, nu, nd, T, U)
+    _check(op, O.HubbardModel(n, nu, nd, T, U), rng)
