@@ -243,7 +243,11 @@ __device__ int wg_lanczos(const ClusterDev &c, const SectorDev &s, V *v0, V *b1,
         bk = psi ? beta[k] : b;
         sig = (bk > 0.0) ? 1.0 / bk : 0.0;
         __syncthreads();
-        if (!(b > 1e-10 * fmax(1.0, fabs(a)))) break;   // invariant subspace
+        if (!(b > 1e-10 * fmax(1.0, fabs(a)))) {        // invariant subspace
+            // the chain ends here: no coupling to a further level
+            if (!psi && threadIdx.x == 0) beta[k] = 0.0;
+            break;
+        }
         if (tol > 0.0 && !psi && m >= 20 && m % 10 == 0 && m < steps) {
             if (threadIdx.x == 0) {
                 const double e = tri_lowest(m, alpha, beta, ty, twk);
@@ -412,8 +416,12 @@ __global__ __launch_bounds__(BT) void k_batch(BatchDev B) {
             for (int k = threadIdx.x; k < B.gf_steps; k += BT) { ga[k] = 0.0; gb[k] = 0.0; }
             __syncthreads();
             if (n2 > 0.0) {
-                const int steps = (int)min<int64_t>(B.gf_steps, Dt);
-                wg_lanczos<float>(c, t, w0, w1, w2, steps, ga, gb, sqrt(n2), nullptr, nullptr, sh);
+                // every chain runs gf_steps steps, also on sectors with fewer states: in
+                // FP32 the basis loses orthogonality before the sector is exhausted, so
+                // stopping at the sector dimension would keep an O(1) last beta (and
+                // drop poles not yet converged); the extra steps only add ghost copies
+                // of converged poles, whose weights sum to the pole's own
+                wg_lanczos<float>(c, t, w0, w1, w2, B.gf_steps, ga, gb, sqrt(n2), nullptr, nullptr, sh);
             }
             __syncthreads();
         }
