@@ -143,6 +143,17 @@ def heis_column(n: int, Jm, idx: int, sign_convention: int = 0) -> dict:
     return out
 
 
+def heis_dense(n: int, Jm, sign_convention: int = 0) -> np.ndarray:
+    """Dense S_z = 0 Heisenberg matrix from orc_heis_column (reference order), in
+    either flip-sign convention; small n only."""
+    D = binom(n, n // 2)
+    H = np.zeros((D, D))
+    for c in range(D):
+        for r, v in heis_column(n, Jm, c, sign_convention).items():
+            H[r, c] += v
+    return H
+
+
 def brute_basis(n: int, k: int) -> np.ndarray:
     """All n-bit masks with k set bits, sorted lexicographically by their
     increasing list of set-bit positions (bit 0 first).  This is the order

@@ -359,17 +359,9 @@ class HeisenbergOp(_Op):
                                                         C.c_void_p(ket.data_ptr()), _dptr(ss)))
         return ss.reshape(n, n)
 
-    def two_rdm(self, bra: torch.Tensor, ket: torch.Tensor | None = None):
-        """DBL_OCC(J,K,M,N) (TWO_RDM, static_rq.f90:247-306), shape (n, n, n, n)."""
-        ket = bra if ket is None else ket
-        n = self.n
-        out = np.zeros(n ** 4)
-        self.ctx.check(self.ctx.lib.phub_two_rdm_hubbard(self.ctx.h, self.h,
-                                                         C.c_void_p(bra.data_ptr()),
-                                                         C.c_void_p(ket.data_ptr()), _dptr(out)))
-        return out.reshape(n, n, n, n)
-
     def hf_grad(self, psi: torch.Tensor):
+        """dE0/dJ(J,K) over ordered pairs: <psi| dH/dJ(J,K) |psi> (the diagonal is
+        0.75 <psi|psi>)."""
         n = self.n
         g = np.zeros(n * n)
         self.ctx.check(self.ctx.lib.phub_hf_grad_heisenberg(self.ctx.h, self.h,

@@ -18,6 +18,14 @@ against central finite differences of the engine's energies and RDMs.
 
 `E0Function` / `RDMLossFunction` wrap both as torch.autograd.Functions over the
 parameter tensors (t, eps, U).
+
+* Heisenberg energy (hprod_heis.f90 operator): Hellmann-Feynman over psi,
+      dE0/dJ(J,K) = <dH/dJ(J,K)>  (phub_hf_grad_heisenberg, ordered pairs),
+  mapped to bond couplings J_b with J(i,j) = J(j,i) = scale * J_b (scale 1/2 in the
+  standard convention H = sum_b J_b S_i.S_j, 1 in the reference's ordered-pair
+  one).  In the physical sign convention dE0/dJ_b = <S_i.S_j> of the reference's
+  J_SPIN (static_rq_heis.f90:58-116); in the reference's flip sign it differs on
+  frustrated bond graphs.  `HeisenbergE0Function` wraps it.
 """
 from __future__ import annotations
 
@@ -27,7 +35,7 @@ import numpy as np
 import torch
 
 from pyhub_amd import _lib as L
-from pyhub_amd.engine import Context, HubbardOp
+from pyhub_amd.engine import Context, HeisenbergOp, HubbardOp
 
 
 @dataclass
@@ -198,13 +206,83 @@ def rdm_loss(t, eps, U, spec: ClusterSpec, target_up, target_dn):
 
 
 def lattice_bonds(Lx, Ly, periodic=False):
-    """Nearest-neighbour bonds of an Lx x Ly lattice, site s = x + Lx*y."""
-    b = []
+    """Nearest-neighbour bonds of an Lx x Ly lattice, site s = x + Lx*y.  Each
+    unordered bond appears once: on a periodic side of length 2 the wrap-around
+    bond is the bond already listed, and on a side of length 1 it would join a
+    site to itself; neither is added again (every listed bond owns its own
+    matrix entries, so its gradient is the whole derivative of that entry)."""
+    b, seen = [], set()
+
+    def add(i, j):
+        key = (min(i, j), max(i, j))
+        if i != j and key not in seen:
+            seen.add(key)
+            b.append((i, j))
+
     for y in range(Ly):
         for x in range(Lx):
             s = x + Lx * y
             if x + 1 < Lx or periodic:
-                b.append((s, (x + 1) % Lx + Lx * y))
+                add(s, (x + 1) % Lx + Lx * y)
             if y + 1 < Ly or periodic:
-                b.append((s, x + Lx * ((y + 1) % Ly)))
+                add(s, x + Lx * ((y + 1) % Ly))
     return b
+
+
+@dataclass
+class HeisenbergSpec:
+    """Heisenberg cluster: n sites (even), bonds (i, j) 0-based.  convention
+    "standard": H = sum_b J_b S_i.S_j, the physical flip sign (J(i,j) = J(j,i) =
+    J_b / 2, sign_convention 1); "reference": the reference's ordered-pair matrix
+    and flip sign (J(i,j) = J(j,i) = J_b, sign_convention 0)."""
+    n: int
+    bonds: list
+    convention: str = "standard"
+    lanczos_tol: float = 1e-13
+    max_steps: int = 400
+
+    @property
+    def scale(self):
+        return 0.5 if self.convention == "standard" else 1.0
+
+    @property
+    def sign_convention(self):
+        return 1 if self.convention == "standard" else 0
+
+    def Jm(self, Jb):
+        J = np.zeros((self.n, self.n))
+        for b, (i, j) in enumerate(self.bonds):
+            J[i, j] += self.scale * float(Jb[b])
+            J[j, i] += self.scale * float(Jb[b])
+        return J
+
+
+def heisenberg_hellmann_feynman(spec: HeisenbergSpec, Jb, ctx: Context | None = None):
+    """E0 and dE0/dJ_b from one pass over psi (FP64 Lanczos, full reorthogonalisation)."""
+    ctx = ctx or Context.default()
+    op = HeisenbergOp(ctx, spec.n, spec.Jm(Jb), sign_convention=spec.sign_convention)
+    r = op.lanczos(mode=L.LCZ_FULL64, max_steps=min(spec.max_steps, op.dim),
+                   tol=spec.lanczos_tol, check_every=5, seed=12345)
+    e0 = float(torch.dot(r.psi, op.hv(r.psi)))
+    g = op.hf_grad(r.psi)
+    gb = np.array([spec.scale * (g[i, j] + g[j, i]) for (i, j) in spec.bonds])
+    return e0, gb
+
+
+class HeisenbergE0Function(torch.autograd.Function):
+    """E0(J_b) of a Heisenberg cluster with the Hellmann-Feynman backward."""
+
+    @staticmethod
+    def forward(ctx, Jb, spec):
+        e0, gb = heisenberg_hellmann_feynman(spec, Jb.detach().cpu().numpy())
+        ctx.save_for_backward(torch.as_tensor(gb))
+        return torch.tensor(e0, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, go):
+        (gb,) = ctx.saved_tensors
+        return go * gb, None
+
+
+def heisenberg_energy(Jb, spec: HeisenbergSpec):
+    return HeisenbergE0Function.apply(Jb, spec)

@@ -301,11 +301,15 @@ def test_lanczos_excited_states(ctx, mode):
 
 
 @pytest.mark.parametrize("pairmap", ["1", "0"])
-@pytest.mark.parametrize("n,nu,nd,seed", [(6, 3, 2, 20), (7, 3, 3, 21), (8, 4, 3, 22), (6, 3, 3, 23)])
+@pytest.mark.parametrize("n,nu,nd,seed", [(6, 3, 2, 20), (7, 3, 3, 21), (8, 4, 3, 22), (6, 3, 3, 23),
+                                          (10, 5, 4, 24), (18, 2, 1, 25)])
 def test_rdm_transition_vs_oracle(ctx, monkeypatch, n, nu, nd, seed, pairmap):
     """ONE_RDM, NI and J_SPIN for bra != ket (all loops of static_rq.f90); the
     spin-flip term through the down-spin pair map and through per-row ranks
-    (PHUB_RDM_PAIRMAP=0, the path for maps beyond 1 GiB)."""
+    (PHUB_RDM_PAIRMAP=0, the path for maps beyond 1 GiB).  The on-device
+    assembly of the diagonal sums runs over chunks of 256 up-then-down states:
+    10 sites (252 + 210 states: the up/down boundary inside the first chunk, two
+    chunks) and 18 sites (the branch that keeps its tables in global memory)."""
     from pyhub_amd.engine import HubbardOp
     monkeypatch.setenv("PHUB_RDM_PAIRMAP", pairmap)
     rng = np.random.default_rng(seed)

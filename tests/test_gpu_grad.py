@@ -176,3 +176,59 @@ def test_run_sets_threads_match_sequential(ctx):
 
     with pytest.raises(ValueError, match="set 4"):
         run_sets(failing, range(6), nthreads=3)
+
+
+def _j1j2_bonds(Lx, Ly):
+    """Nearest- and next-nearest-neighbour bonds of an Lx x Ly torus, each once."""
+    Jstd = O.j1j2_square(Lx, Ly, 1.0, 0.5)
+    nn, nnn = [], []
+    for i in range(Lx * Ly):
+        for j in range(i + 1, Lx * Ly):
+            if Jstd[i, j] != 0.0:
+                (nn if abs(Jstd[i, j] - 1.0) < 1e-12 else nnn).append((i, j))
+    return nn, nnn
+
+
+@pytest.mark.parametrize("convention", ["standard", "reference"])
+@pytest.mark.parametrize("frustrated", [True, False])
+def test_heisenberg_hf_grad_vs_fd(ctx, convention, frustrated):
+    """dE0/dJ_b of phub_hf_grad_heisenberg (both flip-sign conventions, through
+    HeisenbergE0Function) against central differences (h = 1e-5) of the lowest
+    eigenvalue of the oracle's dense matrix (orc_heis_column, pinned against the
+    reference HPROD and the standard spin model), on a 4x3 torus with and without
+    the frustrating next-nearest bonds; in the physical convention also against
+    <S_i.S_j> of the oracle's J_SPIN restatement on the GPU ground state."""
+    from pyhub_amd.engine import HeisenbergOp
+    from pyhub_amd.grad import HeisenbergSpec, heisenberg_energy
+    nn, nnn = _j1j2_bonds(4, 3)
+    bonds = nn + (nnn if frustrated else [])
+    spec = HeisenbergSpec(n=12, bonds=bonds, convention=convention)
+    rng = np.random.default_rng(40 + frustrated)
+    Jb = torch.tensor(np.where(np.arange(len(bonds)) < len(nn), 1.0, 0.5)
+                      * rng.uniform(0.8, 1.2, len(bonds)), requires_grad=True)
+    e0 = heisenberg_energy(Jb, spec)
+    e0.backward()
+    g = Jb.grad.numpy()
+    Jn = Jb.detach().numpy()
+    H = O.heis_dense(12, spec.Jm(Jn), spec.sign_convention)
+    assert abs(float(e0.detach()) - np.linalg.eigvalsh(H)[0]) < 1e-10
+    h = 1e-5
+    fd = []
+    for b in range(len(bonds)):
+        Jp, Jm_ = Jn.copy(), Jn.copy()
+        Jp[b] += h
+        Jm_[b] -= h
+        ep = np.linalg.eigvalsh(O.heis_dense(12, spec.Jm(Jp), spec.sign_convention))[0]
+        em = np.linalg.eigvalsh(O.heis_dense(12, spec.Jm(Jm_), spec.sign_convention))[0]
+        fd.append((ep - em) / (2 * h))
+    err = np.max(np.abs(g - np.array(fd)))
+    print(f"heisenberg {convention} frustrated={frustrated}: max |dE/dJ - FD| = {err:.1e}", flush=True)
+    assert err < 1e-7
+    if convention == "standard":
+        op = HeisenbergOp(ctx, 12, spec.Jm(Jn), sign_convention=1)
+        from pyhub_amd import _lib as L
+        r = op.lanczos(mode=L.LCZ_FULL64, max_steps=400, tol=1e-13)
+        psi = r.psi.cpu().numpy()
+        ss = O.HeisenbergModel(12, spec.Jm(Jn)).sisj(psi)
+        assert np.max(np.abs(g - np.array([ss[i, j] for (i, j) in bonds]))) < 1e-8
+

@@ -1,0 +1,25 @@
+"""CPU tests of host-side logic of the package (no GPU calls)."""
+from pyhub_amd.grad import HeisenbergSpec, lattice_bonds
+
+
+def test_lattice_bonds_small_periodic_sides():
+    """Periodic sides of length 2 and 1: every unordered bond once, no self bond
+    (each listed bond owns its matrix entries, so its gradient is the whole
+    derivative of those entries)."""
+    b = lattice_bonds(2, 3, periodic=True)
+    keys = [tuple(sorted(x)) for x in b]
+    assert len(keys) == len(set(keys)) and all(i != j for i, j in b)
+    assert len(b) == 3 + 6          # one horizontal bond per row, a 3-ring per column
+    b1 = lattice_bonds(1, 4, periodic=True)
+    assert len(b1) == 4 and all(i != j for i, j in b1)
+    assert len(lattice_bonds(4, 3)) == 17     # the 3x4 open cluster of the gradient workload
+
+
+def test_heisenberg_spec_coupling_matrix():
+    """Standard convention: J(i,j) = J(j,i) = J_b / 2 (the ordered-pair sum then
+    gives J_b S_i.S_j); reference convention: J_b on both ordered pairs."""
+    s = HeisenbergSpec(n=4, bonds=[(0, 1), (1, 2)], convention="standard")
+    J = s.Jm([2.0, 4.0])
+    assert J[0, 1] == J[1, 0] == 1.0 and J[1, 2] == J[2, 1] == 2.0 and s.sign_convention == 1
+    r = HeisenbergSpec(n=4, bonds=[(0, 1)], convention="reference")
+    assert r.Jm([3.0])[0, 1] == 3.0 and r.sign_convention == 0
