@@ -16,9 +16,23 @@
  *
  * Memory: every pointer documented "device" is a device pointer on the
  * context's device; "host" pointers are read/written synchronously.  Calls are
- * ordered on the context's stream.  No call allocates device memory except the
- * *_create entry points and phub_lanczos (which allocates its Krylov basis for
- * the duration of the call).
+ * ordered on the context's stream.  Device (and pinned host) memory is allocated
+ * only
+ *   - by the *_create entry points (contexts, operators, Fock spaces, operator
+ *     programs) and by phub_batch_hubbard (its per-call task and vector arrays);
+ *   - once per operator, on the first call that needs them, for tables kept with
+ *     the operator: the all-pairs hop tables and the down-spin pair map of the
+ *     reduced quantities, the all-pairs view of a Heisenberg operator, the
+ *     source maps of an operator program;
+ *   - once per context, when a call needs a larger reduction buffer or scratch
+ *     arena than the context holds (kept until phub_ctx_destroy);
+ *   - for the Krylov / work vectors of phub_lanczos, phub_cf_chain,
+ *     phub_band_lanczos and phub_adjoint_minres when the caller passes no
+ *     workspace (sizes from the matching *_workspace_bytes query), for the
+ *     duration of the call.
+ * So a call repeated on the same operator and context, with workspaces where the
+ * entry point takes one, allocates nothing; phub_alloc_count() counts every
+ * allocation the library makes, for checking that.
  *
  * Errors: every entry returns PHUB_OK (0) or a PHUB_ERR_* code; the message is
  * available from phub_last_error(ctx).
@@ -52,6 +66,10 @@ typedef struct phub_op phub_op;
  * Replaces the per-call executable spawn (core_solver.py:107-111).
  * stream: a hipStream_t (NULL = default stream). */
 int phub_ctx_create(int device, void *stream, phub_ctx **out);
+/* Number of device and pinned-host allocations the library has made in this
+ * process (all contexts).  No counterpart in the reference: it lets a caller check
+ * the allocation contract above around a call. */
+uint64_t phub_alloc_count(void);
 void phub_ctx_destroy(phub_ctx *ctx);
 const char *phub_last_error(const phub_ctx *ctx);
 int phub_sync(phub_ctx *ctx);
@@ -221,7 +239,11 @@ int phub_excite(phub_ctx *ctx, const phub_op *src, const phub_op *dst, int site,
  * spgf_bandlanczos.f90:309-448): alpha[nsteps], beta[nsteps], norm2 = <start|start>.
  * precision 64 or 32 (FP32 vectors, FP64 recurrence scalars). */
 int phub_cf_chain(phub_ctx *ctx, const phub_op *op, const double *d_start, int nsteps,
-                  int precision, double *alpha, double *beta, double *norm2);
+                  int precision, double *alpha, double *beta, double *norm2,
+                  void *workspace /* device or NULL */, size_t workspace_bytes);
+/* Device bytes of the three chain vectors (the workspace that lets phub_cf_chain
+ * allocate nothing).  No counterpart in the reference (it allocates internally). */
+int phub_cf_chain_workspace_bytes(const phub_op *op, int precision, size_t *bytes);
 
 /* Band Lanczos for the full G_ij (GF_BANDLANCZOS, spgf_bandlanczos.f90:309-448, and
  * the weights Q of SPGF_BANDLANCZOS, :90-100).  d_V: p start vectors (device,
@@ -236,9 +258,15 @@ int phub_cf_chain(phub_ctx *ctx, const phub_op *op, const double *d_start, int n
 typedef struct {
     int max_steps;          /* reference max_lcz */
     double tol;             /* reference acc_lcz */
+    void *workspace;        /* device memory for the 2p+1 vector slots, or NULL
+                               (then allocated for the call) */
+    size_t workspace_bytes;
 } phub_band_opts;
 int phub_band_lanczos(phub_ctx *ctx, const phub_op *op, const double *d_V, int p,
                       const phub_band_opts *opts, double *positions, double *Q, int *npoles);
+/* Device bytes of the band Lanczos vector slots for p start vectors.  No counterpart
+ * in the reference. */
+int phub_band_lanczos_workspace_bytes(const phub_op *op, int p, size_t *bytes);
 
 /* ---- batched clusters --------------------------------------------------------
  * Many independent Hubbard clusters in one persistent launch: each workgroup
@@ -290,12 +318,15 @@ int phub_grad_hubbard(phub_ctx *ctx, const phub_op *op, const double *d_bra, con
 int phub_hf_grad_heisenberg(phub_ctx *ctx, const phub_op *op, const double *d_psi,
                             double *grad_j);
 /* Adjoint solve (H - e0) x = b with b orthogonal to psi, by MINRES restricted to
- * the complement of psi (FP64; seven work vectors of the operator dimension are
- * allocated for the call; one iteration is one H*v, four vector passes and two host
- * round trips).  Returns iterations in *iters. */
+ * the complement of psi (FP64; seven work vectors of the operator dimension, from
+ * the workspace or allocated for the call; one iteration is one H*v, four vector
+ * passes and two host round trips).  Returns iterations in *iters. */
 int phub_adjoint_minres(phub_ctx *ctx, const phub_op *op, const double *d_psi, double e0,
                         const double *d_b, double *d_x, int max_iter, double tol,
-                        int *iters, double *rel_res);
+                        int *iters, double *rel_res, void *workspace /* device or NULL */,
+                        size_t workspace_bytes);
+/* Device bytes of the seven MINRES work vectors. */
+int phub_adjoint_minres_workspace_bytes(const phub_op *op, size_t *bytes);
 
 /* ---- generic operator strings (operators.x) ---------------------------------------
  * Replaces compute_operators.f90:1-72 (the driver) and the elementary routines of

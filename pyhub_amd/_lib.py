@@ -57,7 +57,8 @@ class Cluster(C.Structure):
 
 
 class BandOpts(C.Structure):
-    _fields_ = [("max_steps", C.c_int), ("tol", C.c_double)]
+    _fields_ = [("max_steps", C.c_int), ("tol", C.c_double), ("workspace", C.c_void_p),
+                ("workspace_bytes", C.c_size_t)]
 
 
 class BatchOpts(C.Structure):
@@ -91,10 +92,12 @@ SIGNATURES = {
     "phub_rdm_heisenberg": (C.c_int, [_vp, _vp, _vp, _vp, _dp]),
     "phub_one_rdm_spin": (C.c_int, [_vp, _vp, _vp, _vp, _dp, _dp]),
     "phub_excite": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
-    "phub_cf_chain": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _dp, _dp, _dp]),
+    "phub_cf_chain": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _dp, _dp, _dp, _vp, C.c_size_t]),
+    "phub_cf_chain_workspace_bytes": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_size_t)]),
     "phub_two_rdm_hubbard": (C.c_int, [_vp, _vp, _vp, _vp, _dp]),
     "phub_band_lanczos": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(BandOpts), _dp, _dp,
                                     C.POINTER(C.c_int)]),
+    "phub_band_lanczos_workspace_bytes": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_size_t)]),
     "phub_batch_hubbard": (C.c_int, [_vp, C.POINTER(Cluster), C.c_int, C.POINTER(BatchOpts),
                                      _dp, _dp, _dp, _dp]),
     "phub_cf_eval": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp,
@@ -103,7 +106,9 @@ SIGNATURES = {
     "phub_grad_hubbard": (C.c_int, [_vp, _vp, _vp, _vp, _dp, _dp]),
     "phub_hf_grad_heisenberg": (C.c_int, [_vp, _vp, _vp, _dp]),
     "phub_adjoint_minres": (C.c_int, [_vp, _vp, _vp, C.c_double, _vp, _vp, C.c_int, C.c_double,
-                                      C.POINTER(C.c_int), _dp]),
+                                      C.POINTER(C.c_int), _dp, _vp, C.c_size_t]),
+    "phub_adjoint_minres_workspace_bytes": (C.c_int, [_vp, C.POINTER(C.c_size_t)]),
+    "phub_alloc_count": (C.c_uint64, []),
     "phub_fock_create": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
                                    C.c_int, C.POINTER(_vp)]),
     "phub_fock_destroy": (None, [_vp]),

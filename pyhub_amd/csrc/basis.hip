@@ -10,6 +10,12 @@
 #include <cmath>
 #include <cstring>
 
+std::atomic<unsigned long long> g_phub_alloc_count{0};
+
+extern "C" uint64_t phub_alloc_count(void) {
+    return (uint64_t)g_phub_alloc_count.load(std::memory_order_relaxed);
+}
+
 int phub_fail(phub_ctx *ctx, int code, const std::string &msg) {
     if (ctx) ctx->err = msg;
     return code;
@@ -49,7 +55,7 @@ extern "C" int phub_ctx_create(int device, void *stream, phub_ctx **out) {
     if (e != hipSuccess) { int r = phub_hip_fail(c, e, "hipSetDevice"); *out = c; return r; }
     std::vector<uint64_t> tab((PHUB_MAXN + 1) * (PHUB_MAXN + 1));
     host_binom(tab.data());
-    e = hipMalloc(&c->d_binom, tab.size() * sizeof(uint64_t));
+    e = phub_malloc(&c->d_binom, tab.size() * sizeof(uint64_t));
     if (e == hipSuccess)
         e = phub_copy(c, c->d_binom, tab.data(), tab.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
@@ -64,6 +70,7 @@ extern "C" void phub_ctx_destroy(phub_ctx *c) {
     hipSetDevice(c->device);
     if (c->d_binom) hipFree(c->d_binom);
     if (c->d_red) hipFree(c->d_red);
+    if (c->arena) hipFree(c->arena);
     for (void *p : c->deferred) hipFree(p);
     if (c->h_red) hipHostFree(c->h_red);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -86,13 +93,32 @@ int phub_ensure_red(phub_ctx *c, size_t nd) {
         phub_dfree(c, c->d_red);
         c->d_red = nullptr;
         c->red_cap = 0;
-        PHUB_HIP(c, hipMalloc(&c->d_red, nd * sizeof(double)));
+        PHUB_HIP(c, phub_malloc(&c->d_red, nd * sizeof(double)));
         c->red_cap = nd;
     }
     if (c->h_red_cap < 4096) {
-        PHUB_HIP(c, hipHostMalloc(&c->h_red, 4096 * sizeof(double), hipHostMallocCoherent));
+        PHUB_HIP(c, phub_host_malloc(&c->h_red, 4096 * sizeof(double), hipHostMallocCoherent));
         c->h_red_cap = 4096;
     }
+    return PHUB_OK;
+}
+
+// The context's scratch arena: small per-call buffers (RDM sums, Krylov
+// coefficients, staged inputs) carved from one device block that grows to the
+// largest request and is kept until the context is destroyed, so a call repeated
+// on the same operator allocates nothing.  The old block is released (or kept,
+// for a context that defers frees) before the new one is made; its previous
+// users on this stream have been ordered before by hipFree's implicit wait.
+int phub_arena(phub_ctx *c, size_t bytes, void **out) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (c->arena_cap < bytes) {
+        phub_dfree(c, c->arena);
+        c->arena = nullptr;
+        c->arena_cap = 0;
+        PHUB_HIP(c, phub_malloc(&c->arena, bytes));
+        c->arena_cap = bytes;
+    }
+    *out = c->arena;
     return PHUB_OK;
 }
 
@@ -348,12 +374,12 @@ int phub_build_hop_tables(phub_ctx *c, const std::vector<HopSpec> &specs, HopBat
         std::memcpy(stage.data() + o_pid, pid.data(), pid.size() * sizeof(int16_t));
     }
     std::memcpy(stage.data() + o_td, td.data(), td.size() * sizeof(double));
-    PHUB_HIP(c, hipMalloc(&out->arena[0], bytes0));
+    PHUB_HIP(c, phub_malloc(&out->arena[0], bytes0));
     char *a0 = (char *)out->arena[0];
     PHUB_HIP(c, hipMemcpyAsync(a0, stage.data(), bytes0, hipMemcpyHostToDevice, c->stream));
     // arena 1: masks (u32), eloc (f64), cnt (i32) of every table
     const size_t o_el = al((size_t)mtot * sizeof(uint32_t)), o_cn = o_el + al((size_t)mtot * sizeof(double));
-    PHUB_HIP(c, hipMalloc(&out->arena[1], o_cn + al((size_t)mtot * sizeof(int32_t))));
+    PHUB_HIP(c, phub_malloc(&out->arena[1], o_cn + al((size_t)mtot * sizeof(int32_t))));
     char *a1 = (char *)out->arena[1];
     auto pl_of = [&](size_t t) {
         return PairList{(int)np[t], (const int *)(a0) + poff[t], (const int *)(a0 + o_bk) + poff[t],
@@ -386,7 +412,7 @@ int phub_build_hop_tables(phub_ctx *c, const std::vector<HopSpec> &specs, HopBat
     }
     const size_t o_c64 = al(etot * sizeof(int32_t)), o_c32 = o_c64 + al(etot * sizeof(double));
     const size_t bytes2 = o_c32 + al(etot * sizeof(float));
-    PHUB_HIP(c, hipMalloc(&out->arena[2], bytes2));
+    PHUB_HIP(c, phub_malloc(&out->arena[2], bytes2));
     char *a2 = (char *)out->arena[2];
     PHUB_HIP(c, hipMemsetAsync(a2, 0, bytes2, c->stream));
     for (size_t t = 0; t < nt; ++t) {
@@ -472,12 +498,12 @@ int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor
     double *d_t = nullptr, *d_td = nullptr;
     int16_t *d_pid = nullptr;
     size_t npa = std::max(np, 1);
-    PHUB_HIP(c, hipMalloc(&d_bj, npa * sizeof(int)));
-    PHUB_HIP(c, hipMalloc(&d_bk, npa * sizeof(int)));
-    PHUB_HIP(c, hipMalloc(&d_t, npa * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&d_pid, npa * sizeof(int16_t)));
-    PHUB_HIP(c, hipMalloc(&d_td, std::max(n, 1) * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&d_max, sizeof(int)));
+    PHUB_HIP(c, phub_malloc(&d_bj, npa * sizeof(int)));
+    PHUB_HIP(c, phub_malloc(&d_bk, npa * sizeof(int)));
+    PHUB_HIP(c, phub_malloc(&d_t, npa * sizeof(double)));
+    PHUB_HIP(c, phub_malloc(&d_pid, npa * sizeof(int16_t)));
+    PHUB_HIP(c, phub_malloc(&d_td, std::max(n, 1) * sizeof(double)));
+    PHUB_HIP(c, phub_malloc(&d_max, sizeof(int)));
     if (np) {
         PHUB_HIP(c, phub_copy(c, d_bj, bj.data(), np * sizeof(int), hipMemcpyHostToDevice));
         PHUB_HIP(c, phub_copy(c, d_bk, bk.data(), np * sizeof(int), hipMemcpyHostToDevice));
@@ -486,9 +512,9 @@ int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor
     }
     if (n) PHUB_HIP(c, phub_copy(c, d_td, tdiag.data(), n * sizeof(double), hipMemcpyHostToDevice));
     PHUB_HIP(c, phub_clear(c, d_max, sizeof(int)));
-    PHUB_HIP(c, hipMalloc(&out->masks, dim * sizeof(uint32_t)));
-    PHUB_HIP(c, hipMalloc(&out->eloc, dim * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&out->cnt, dim * sizeof(int32_t)));
+    PHUB_HIP(c, phub_malloc(&out->masks, dim * sizeof(uint32_t)));
+    PHUB_HIP(c, phub_malloc(&out->eloc, dim * sizeof(double)));
+    PHUB_HIP(c, phub_malloc(&out->cnt, dim * sizeof(int32_t)));
     hipLaunchKernelGGL(k_unrank<uint32_t>, dim3(nblk(dim, 256)), dim3(256), 0, c->stream,
                        c->d_binom, n, k, dim, out->masks);
     PairList pl{np, d_bj, d_bk, d_t, d_pid};
@@ -500,13 +526,13 @@ int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor
     PHUB_HIP(c, hipStreamSynchronize(c->stream));
     out->maxh = mh;
     size_t ne = (size_t)dim * std::max(mh, 1);
-    PHUB_HIP(c, hipMalloc(&out->idx, ne * sizeof(int32_t)));
-    PHUB_HIP(c, hipMalloc(&out->c64, ne * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&out->c32, ne * sizeof(float)));
+    PHUB_HIP(c, phub_malloc(&out->idx, ne * sizeof(int32_t)));
+    PHUB_HIP(c, phub_malloc(&out->c64, ne * sizeof(double)));
+    PHUB_HIP(c, phub_malloc(&out->c32, ne * sizeof(float)));
     PHUB_HIP(c, hipMemsetAsync(out->idx, 0, ne * sizeof(int32_t), c->stream));
     PHUB_HIP(c, hipMemsetAsync(out->c64, 0, ne * sizeof(double), c->stream));
     PHUB_HIP(c, hipMemsetAsync(out->c32, 0, ne * sizeof(float), c->stream));
-    if (all_pairs) PHUB_HIP(c, hipMalloc(&out->pair, ne * sizeof(int16_t)));
+    if (all_pairs) PHUB_HIP(c, phub_malloc(&out->pair, ne * sizeof(int16_t)));
     hipLaunchKernelGGL(k_hop_fill, dim3(hop_fill_grid(dim)), dim3(256), 0, c->stream, c->d_binom, n,
                        k, out->masks, dim, pl, mh, kmajor, out->idx, out->c64, out->c32,
                        out->pair, out->eloc, d_td);
@@ -522,12 +548,12 @@ int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor
         if (cls.size() <= 255) {
             out->ncls = (int)cls.size();
             std::vector<float> cls32(cls.begin(), cls.end());
-            PHUB_HIP(c, hipMalloc(&out->cls64, cls.size() * sizeof(double)));
-            PHUB_HIP(c, hipMalloc(&out->cls32, cls.size() * sizeof(float)));
+            PHUB_HIP(c, phub_malloc(&out->cls64, cls.size() * sizeof(double)));
+            PHUB_HIP(c, phub_malloc(&out->cls32, cls.size() * sizeof(float)));
             PHUB_HIP(c, phub_copy(c, out->cls64, cls.data(), cls.size() * sizeof(double), hipMemcpyHostToDevice));
             PHUB_HIP(c, phub_copy(c, out->cls32, cls32.data(), cls.size() * sizeof(float), hipMemcpyHostToDevice));
             if (kmajor && dim < (1ll << 18)) {   // the byte offset idx*8 must fit 21 bits
-                PHUB_HIP(c, hipMalloc(&out->packed, (size_t)dim * mh * sizeof(uint32_t)));
+                PHUB_HIP(c, phub_malloc(&out->packed, (size_t)dim * mh * sizeof(uint32_t)));
                 hipLaunchKernelGGL(k_hop_pack, dim3(nblk(dim, 256)), dim3(256), 0, c->stream,
                                    out->idx, out->c64, out->cnt, dim, mh, out->cls64, out->ncls,
                                    out->packed);
@@ -535,8 +561,8 @@ int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor
                 std::vector<int32_t> hcnt(dim), rp(dim + 1, 0);
                 PHUB_HIP(c, phub_copy(c, hcnt.data(), out->cnt, dim * sizeof(int32_t), hipMemcpyDeviceToHost));
                 for (int64_t s = 0; s < dim; ++s) rp[s + 1] = rp[s] + hcnt[s];
-                PHUB_HIP(c, hipMalloc(&out->rowptr, (dim + 1) * sizeof(int32_t)));
-                PHUB_HIP(c, hipMalloc(&out->csr, std::max<int64_t>(rp[dim], 1) * sizeof(uint32_t)));
+                PHUB_HIP(c, phub_malloc(&out->rowptr, (dim + 1) * sizeof(int32_t)));
+                PHUB_HIP(c, phub_malloc(&out->csr, std::max<int64_t>(rp[dim], 1) * sizeof(uint32_t)));
                 PHUB_HIP(c, phub_copy(c, out->rowptr, rp.data(), (dim + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
                 hipLaunchKernelGGL(k_hop_pack_csr, dim3(nblk(dim, 256)), dim3(256), 0, c->stream,
                                    out->idx, out->c64, out->cnt, out->rowptr, dim, mh, out->cls64,
@@ -626,8 +652,8 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
     int L = n / 2;
     out->L = L;
     int nh = n - L;
-    PHUB_HIP(c, hipMalloc(&out->tabA, ((size_t)1 << nh) * sizeof(uint64_t)));
-    PHUB_HIP(c, hipMalloc(&out->tabB, ((size_t)1 << L) * (nh + 1) * sizeof(uint64_t)));
+    PHUB_HIP(c, phub_malloc(&out->tabA, ((size_t)1 << nh) * sizeof(uint64_t)));
+    PHUB_HIP(c, phub_malloc(&out->tabB, ((size_t)1 << L) * (nh + 1) * sizeof(uint64_t)));
     hipLaunchKernelGGL(k_rank_tab_A, dim3(nblk(1ll << nh, 256)), dim3(256), 0, c->stream,
                        c->d_binom, n, L, out->tabA);
     hipLaunchKernelGGL(k_rank_tab_B, dim3(nblk((1ll << L) * (nh + 1), 256)), dim3(256), 0,
@@ -636,7 +662,7 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
     // the mask table is only kept when small (reduced quantities use it); the H*v
     // enumerates blocks of equal low-half pattern instead
     if (dim <= (1ll << 28)) {
-        PHUB_HIP(c, hipMalloc(&out->masks, dim * sizeof(uint64_t)));
+        PHUB_HIP(c, phub_malloc(&out->masks, dim * sizeof(uint64_t)));
         hipLaunchKernelGGL(k_unrank<uint64_t>, dim3(nblk(dim, 256)), dim3(256), 0, c->stream,
                            c->d_binom, n, n / 2, dim, out->masks);
         PHUB_HIP(c, hipGetLastError());
@@ -644,13 +670,13 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
     {
         std::vector<int64_t> hoff(nh + 2, 0);
         for (int m = 0; m <= nh; ++m) hoff[m + 1] = hoff[m] + (int64_t)phub_binom(nh, m);
-        PHUB_HIP(c, hipMalloc(&out->hoff, (nh + 2) * sizeof(int64_t)));
+        PHUB_HIP(c, phub_malloc(&out->hoff, (nh + 2) * sizeof(int64_t)));
         PHUB_HIP(c, phub_copy(c, out->hoff, hoff.data(), (nh + 2) * sizeof(int64_t), hipMemcpyHostToDevice));
-        PHUB_HIP(c, hipMalloc(&out->headpat, ((size_t)1 << nh) * sizeof(uint32_t)));
+        PHUB_HIP(c, phub_malloc(&out->headpat, ((size_t)1 << nh) * sizeof(uint32_t)));
         hipLaunchKernelGGL(k_headpat, dim3(nblk(1ll << nh, 256)), dim3(256), 0, c->stream,
                            out->tabA, nh, out->hoff, out->headpat);
         PHUB_HIP(c, hipGetLastError());
-        PHUB_HIP(c, hipMalloc(&out->tabA32, ((size_t)1 << nh) * sizeof(uint32_t)));
+        PHUB_HIP(c, phub_malloc(&out->tabA32, ((size_t)1 << nh) * sizeof(uint32_t)));
         hipLaunchKernelGGL(k_narrow32, dim3(nblk(1ll << nh, 256)), dim3(256), 0, c->stream,
                            out->tabA, 1ll << nh, out->tabA32);
         PHUB_HIP(c, hipGetLastError());
@@ -673,8 +699,8 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
                         if ((lo >> b) & 1u) { ++i; cc += phub_binom(n - 1 - (b + L), i); }
                     a2[((size_t)p << LA) + lo] = (uint32_t)cc;
                 }
-            PHUB_HIP(c, hipMalloc(&out->tabA1, a1.size() * sizeof(uint32_t)));
-            PHUB_HIP(c, hipMalloc(&out->tabA2, a2.size() * sizeof(uint32_t)));
+            PHUB_HIP(c, phub_malloc(&out->tabA1, a1.size() * sizeof(uint32_t)));
+            PHUB_HIP(c, phub_malloc(&out->tabA2, a2.size() * sizeof(uint32_t)));
             PHUB_HIP(c, phub_copy(c, out->tabA1, a1.data(), a1.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
             PHUB_HIP(c, phub_copy(c, out->tabA2, a2.data(), a2.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         }
@@ -698,7 +724,7 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
             for (uint64_t ch = 0; ch < nch; ++ch) items.push_back((bl.second << 12) | (uint32_t)ch);
         }
         out->nitems = (int64_t)items.size();
-        PHUB_HIP(c, hipMalloc(&out->items, std::max<size_t>(items.size(), 1) * sizeof(uint32_t)));
+        PHUB_HIP(c, phub_malloc(&out->items, std::max<size_t>(items.size(), 1) * sizeof(uint32_t)));
         PHUB_HIP(c, phub_copy(c, out->items, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
     // bonds: unordered {J<K} with either J(J,K) or J(K,J) non-zero
@@ -760,8 +786,8 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
                 e64[h] = e;
             }
             std::vector<float> e32(e64.begin(), e64.end());
-            PHUB_HIP(c, hipMalloc(&out->ehh64, e64.size() * sizeof(double)));
-            PHUB_HIP(c, hipMalloc(&out->ehh32, e32.size() * sizeof(float)));
+            PHUB_HIP(c, phub_malloc(&out->ehh64, e64.size() * sizeof(double)));
+            PHUB_HIP(c, phub_malloc(&out->ehh32, e32.size() * sizeof(float)));
             PHUB_HIP(c, phub_copy(c, out->ehh64, e64.data(), e64.size() * sizeof(double), hipMemcpyHostToDevice));
             PHUB_HIP(c, phub_copy(c, out->ehh32, e32.data(), e32.size() * sizeof(float), hipMemcpyHostToDevice));
         }
@@ -770,18 +796,18 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
         out->nx = (int)g[2].size();
         std::vector<HeisBond> all;
         for (auto &v : g) all.insert(all.end(), v.begin(), v.end());
-        PHUB_HIP(c, hipMalloc(&out->sbonds, std::max<size_t>(all.size(), 1) * sizeof(HeisBond)));
+        PHUB_HIP(c, phub_malloc(&out->sbonds, std::max<size_t>(all.size(), 1) * sizeof(HeisBond)));
         if (!all.empty())
             PHUB_HIP(c, phub_copy(c, out->sbonds, all.data(), all.size() * sizeof(HeisBond), hipMemcpyHostToDevice));
     }
     out->diag_const = dconst;
     size_t nba = std::max(nb, 1);
-    PHUB_HIP(c, hipMalloc(&out->bj, nba * sizeof(int)));
-    PHUB_HIP(c, hipMalloc(&out->bk, nba * sizeof(int)));
-    PHUB_HIP(c, hipMalloc(&out->bJ, nba * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&out->bJr, nba * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&out->bsign, nba * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&out->bond_ising, nba * sizeof(double)));
+    PHUB_HIP(c, phub_malloc(&out->bj, nba * sizeof(int)));
+    PHUB_HIP(c, phub_malloc(&out->bk, nba * sizeof(int)));
+    PHUB_HIP(c, phub_malloc(&out->bJ, nba * sizeof(double)));
+    PHUB_HIP(c, phub_malloc(&out->bJr, nba * sizeof(double)));
+    PHUB_HIP(c, phub_malloc(&out->bsign, nba * sizeof(double)));
+    PHUB_HIP(c, phub_malloc(&out->bond_ising, nba * sizeof(double)));
     if (nb) {
         PHUB_HIP(c, phub_copy(c, out->bj, bj.data(), nb * sizeof(int), hipMemcpyHostToDevice));
         PHUB_HIP(c, phub_copy(c, out->bk, bk.data(), nb * sizeof(int), hipMemcpyHostToDevice));

@@ -506,7 +506,7 @@ struct BigWorkers {
                     return phub_fail(c, PHUB_ERR_ARG, "large ground state workspace");
                 x.ws_bytes = std::max(x.ws_bytes, b);
             }
-            PHUB_HIP(c, hipMalloc(&x.ws, x.ws_bytes));
+            PHUB_HIP(c, phub_malloc(&x.ws, x.ws_bytes));
         }
         return PHUB_OK;
     }
@@ -692,17 +692,17 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
     DB<Task> dtasks;
     DB<unsigned int> counter;
     DB<double> de0, ga, gb, gn, tri;
-    PHUB_HIP(c, hipMalloc(&psi.p, psi_total * sizeof(double)));
+    PHUB_HIP(c, phub_malloc(&psi.p, psi_total * sizeof(double)));
     for (int i = 0; i < ncl; ++i) hc[i].psi0 = psi.p + hc[i].psi_off;
-    PHUB_HIP(c, hipMalloc(&scratch.p, (size_t)grid * 3 * dmax * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&dcl.p, ncl * sizeof(ClusterDev)));
-    PHUB_HIP(c, hipMalloc(&dtasks.p, tasks.size() * sizeof(Task)));
-    PHUB_HIP(c, hipMalloc(&counter.p, 2 * sizeof(unsigned int)));
-    PHUB_HIP(c, hipMalloc(&de0.p, ncl * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&ga.p, (size_t)nslots * o->gf_steps * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&gb.p, (size_t)nslots * o->gf_steps * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&gn.p, nslots * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&tri.p, (size_t)grid * 6 * MAXSTEPS * sizeof(double)));
+    PHUB_HIP(c, phub_malloc(&scratch.p, (size_t)grid * 3 * dmax * sizeof(double)));
+    PHUB_HIP(c, phub_malloc(&dcl.p, ncl * sizeof(ClusterDev)));
+    PHUB_HIP(c, phub_malloc(&dtasks.p, tasks.size() * sizeof(Task)));
+    PHUB_HIP(c, phub_malloc(&counter.p, 2 * sizeof(unsigned int)));
+    PHUB_HIP(c, phub_malloc(&de0.p, ncl * sizeof(double)));
+    PHUB_HIP(c, phub_malloc(&ga.p, (size_t)nslots * o->gf_steps * sizeof(double)));
+    PHUB_HIP(c, phub_malloc(&gb.p, (size_t)nslots * o->gf_steps * sizeof(double)));
+    PHUB_HIP(c, phub_malloc(&gn.p, nslots * sizeof(double)));
+    PHUB_HIP(c, phub_malloc(&tri.p, (size_t)grid * 6 * MAXSTEPS * sizeof(double)));
     PHUB_HIP(c, phub_copy(c, dcl.p, hc.data(), ncl * sizeof(ClusterDev), hipMemcpyHostToDevice));
     PHUB_HIP(c, phub_copy(c, dtasks.p, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice));
     PHUB_HIP(c, phub_clear(c, counter.p, 2 * sizeof(unsigned int)));

@@ -104,7 +104,8 @@ extern "C" int phub_cf_eval(phub_ctx *c, int nch, int m, const double *alpha, co
     double *d = nullptr;
     const size_t na = (size_t)nch * m;
     const size_t tot = 2 * na + 3 * (size_t)nch + nw + 2 * (size_t)nch * nw;
-    PHUB_HIP(c, hipMalloc(&d, tot * sizeof(double)));
+    int rc = phub_arena(c, tot * sizeof(double), (void **)&d);
+    if (rc) return rc;
     double *da = d, *db = da + na, *dn = db + na, *ds = dn + nch, *dg = ds + nch, *dw = dg + nch,
            *dout = dw + nw;
     hipMemcpyAsync(da, alpha, na * sizeof(double), hipMemcpyHostToDevice, c->stream);
@@ -120,7 +121,6 @@ extern "C" int phub_cf_eval(phub_ctx *c, int nch, int m, const double *alpha, co
     if (e == hipSuccess)
         e = hipMemcpyAsync(out, dout, 2 * (size_t)nt * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(d);
     if (e != hipSuccess) return phub_hip_fail(c, e, "phub_cf_eval");
     return PHUB_OK;
 }

@@ -564,7 +564,7 @@ static int dev_multidot(phub_ctx *c, const T *basis, int64_t ld, int k, const TW
     // reduce k components; host buffer may need to grow
     if ((size_t)k > c->h_red_cap) {
         if (c->h_red) hipHostFree(c->h_red);
-        PHUB_HIP(c, hipHostMalloc(&c->h_red, (size_t)k * sizeof(double), hipHostMallocCoherent));
+        PHUB_HIP(c, phub_host_malloc(&c->h_red, (size_t)k * sizeof(double), hipHostMallocCoherent));
         c->h_red_cap = k;
     }
     return phub_reduce_partials(c, (int)g, k, out);
@@ -740,7 +740,7 @@ struct Arena {
             return base + off;
         }
         void *p = nullptr;
-        if (hipMalloc(&p, n) != hipSuccess) {
+        if (phub_malloc(&p, n) != hipSuccess) {
             (void)hipGetLastError();   // a failed allocation is not a sticky error here
             return nullptr;
         }
@@ -1081,8 +1081,8 @@ static int plain_mode(phub_ctx *c, const phub_op *op, const phub_lanczos_opts *o
             return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
         if (st.growing) {
             // every vector is stored: psi_q = sum_j y_qj x_j / beta_j in one pass each
-            DevBuf<const T *> dptr(c);
-            PHUB_HIP(c, hipMalloc(&dptr.p, (size_t)s.steps * sizeof(T *)));
+            // the pointer table sits in the coefficient block, after its max_steps + 8
+            struct { const T **p; } dptr{(const T **)(coef + oo->max_steps + 8)};
             std::vector<double> cf(s.steps);
             for (size_t q = 0; q < ys.size(); ++q) {
                 for (int j = 0; j < s.steps; ++j)
@@ -1185,9 +1185,11 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
     auto t0 = std::chrono::steady_clock::now();
     EvTimer tm(c);
     LczState s;
-    DevBuf<double> coef(c);
-    PHUB_HIP(c, hipMalloc(&coef.p, (size_t)(oo.max_steps + 8) * sizeof(double)));
-    int rc = PHUB_OK;
+    // combine coefficients (max_steps + 8 doubles) and, after them, the table of
+    // basis-vector pointers of the one-pass psi (max_steps + 1): the context arena
+    struct { double *p; } coef{nullptr};
+    int rc = phub_arena(c, (size_t)(2 * oo.max_steps + 9) * sizeof(double), (void **)&coef.p);
+    if (rc) return rc;
     double t_lcz = 0.0;
     Arena ar(c, o->workspace, o->workspace_bytes);
 
@@ -1323,13 +1325,14 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
 // ---------------------------------------------------------------------------
 template <typename T>
 static int cf_chain_t(phub_ctx *c, const phub_op *op, const double *start, int nsteps,
-                      double *alpha, double *beta, double *norm2) {
+                      double *alpha, double *beta, double *norm2, void *ws, size_t ws_bytes) {
     const int64_t n = op->dim;
     const unsigned g = (unsigned)((n + 255) / 256);
-    DevBuf<T> v0(c), b1(c), b2(c);
-    PHUB_HIP(c, hipMalloc(&v0.p, n * sizeof(T)));
-    PHUB_HIP(c, hipMalloc(&b1.p, n * sizeof(T)));
-    PHUB_HIP(c, hipMalloc(&b2.p, n * sizeof(T)));
+    // three vectors: from the caller's workspace, else allocated for the call
+    Arena ar(c, ws, ws_bytes);
+    struct { T *p; } v0{(T *)ar.take(n * sizeof(T))}, b1{(T *)ar.take(n * sizeof(T))},
+        b2{(T *)ar.take(n * sizeof(T))};
+    if (!v0.p || !b1.p || !b2.p) return phub_fail(c, PHUB_ERR_OOM, "phub_cf_chain: vectors");
     double n2 = 0;
     int rc = dev_dot<double, double>(c, start, start, n, &n2);
     if (rc) return rc;
@@ -1483,6 +1486,12 @@ static int band_matrix(int nb, int p, const std::vector<double> &alpha,
     return m;
 }
 
+extern "C" int phub_band_lanczos_workspace_bytes(const phub_op *op, int p, size_t *bytes) {
+    if (!op || !bytes || p < 1) return PHUB_ERR_ARG;
+    *bytes = (size_t)(2 * p + 1) * basis_ld(op->dim) * sizeof(double) + 256;
+    return PHUB_OK;
+}
+
 extern "C" int phub_band_lanczos(phub_ctx *c, const phub_op *op, const double *d_V, int p,
                                  const phub_band_opts *o, double *positions, double *Q,
                                  int *npoles) {
@@ -1496,11 +1505,15 @@ extern "C" int phub_band_lanczos(phub_ctx *c, const phub_op *op, const double *d
     const double DF = 1e-10;   // deflation threshold (spgf_bandlanczos.f90:312)
     // 2p + 1 vector slots: the p previous vectors, the current one and its p - 1
     // successors, and the work vector; rotated by pointer, never copied
-    DevBuf<double> slots(c), coefd(c);
-    DevBuf<const double *> dptr(c);
-    PHUB_HIP(c, hipMalloc(&slots.p, (size_t)(2 * p + 1) * ld * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&coefd.p, (size_t)(2 * p + 8) * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&dptr.p, (size_t)(2 * p + 1) * sizeof(double *)));
+    // the slots from the caller's workspace (else allocated for the call); the
+    // coefficients and the pointer table from the context arena
+    Arena ar(c, o->workspace, o->workspace_bytes);
+    struct { double *p; } slots{(double *)ar.take((size_t)(2 * p + 1) * ld * sizeof(double))};
+    if (!slots.p) return phub_fail(c, PHUB_ERR_OOM, "phub_band_lanczos: vectors");
+    struct { double *p; } coefd{nullptr};
+    int arc = phub_arena(c, (size_t)(4 * p + 9) * sizeof(double), (void **)&coefd.p);
+    if (arc) return arc;
+    struct { const double **p; } dptr{(const double **)(coefd.p + 2 * p + 8)};
     PHUB_HIP(c, hipMemsetAsync(slots.p, 0, (size_t)(2 * p + 1) * ld * sizeof(double), c->stream));
     std::vector<double *> ring(2 * p);       // [0,p): previous block V0, [p,2p): V
     for (int i = 0; i < 2 * p; ++i) ring[i] = slots.p + (size_t)i * ld;
@@ -1630,8 +1643,16 @@ extern "C" int phub_band_lanczos(phub_ctx *c, const phub_op *op, const double *d
     return PHUB_OK;
 }
 
+extern "C" int phub_cf_chain_workspace_bytes(const phub_op *op, int precision, size_t *bytes) {
+    if (!op || !bytes || (precision != 32 && precision != 64)) return PHUB_ERR_ARG;
+    const size_t v = ((size_t)op->dim * (precision / 8) + 255) & ~(size_t)255;
+    *bytes = 3 * v + 256;
+    return PHUB_OK;
+}
+
 extern "C" int phub_cf_chain(phub_ctx *c, const phub_op *op, const double *d_start, int nsteps,
-                             int precision, double *alpha, double *beta, double *norm2) {
+                             int precision, double *alpha, double *beta, double *norm2,
+                             void *workspace, size_t workspace_bytes) {
     if (!c || !op || !d_start || !alpha || !beta || !norm2 || nsteps < 1)
         return phub_fail(c, PHUB_ERR_ARG, "bad argument");
     if (op->dim == 0) {
@@ -1639,8 +1660,12 @@ extern "C" int phub_cf_chain(phub_ctx *c, const phub_op *op, const double *d_sta
         for (int i = 0; i < nsteps; ++i) alpha[i] = beta[i] = 0.0;
         return PHUB_OK;
     }
-    if (precision == 64) return cf_chain_t<double>(c, op, d_start, nsteps, alpha, beta, norm2);
-    if (precision == 32) return cf_chain_t<float>(c, op, d_start, nsteps, alpha, beta, norm2);
+    if (precision == 64)
+        return cf_chain_t<double>(c, op, d_start, nsteps, alpha, beta, norm2, workspace,
+                                  workspace_bytes);
+    if (precision == 32)
+        return cf_chain_t<float>(c, op, d_start, nsteps, alpha, beta, norm2, workspace,
+                                 workspace_bytes);
     return phub_fail(c, PHUB_ERR_ARG, "precision must be 32 or 64");
 }
 
@@ -1654,16 +1679,28 @@ static int project_out(phub_ctx *c, const double *psi, double *y, int64_t n) {
     return dev_axpby<double, double>(c, -p, psi, 1.0, y, n, nullptr);
 }
 
+extern "C" int phub_adjoint_minres_workspace_bytes(const phub_op *op, size_t *bytes) {
+    if (!op || !bytes) return PHUB_ERR_ARG;
+    *bytes = 7 * (((size_t)op->dim * sizeof(double) + 255) & ~(size_t)255) + 256;
+    return PHUB_OK;
+}
+
 extern "C" int phub_adjoint_minres(phub_ctx *c, const phub_op *op, const double *psi, double e0,
                                    const double *b, double *x, int max_iter, double tol,
-                                   int *iters, double *rel_res) {
+                                   int *iters, double *rel_res, void *workspace,
+                                   size_t workspace_bytes) {
     if (!c || !op || !psi || !b || !x) return phub_fail(c, PHUB_ERR_ARG, "null argument");
     const int64_t n = op->dim;
     // r2 is also the unnormalised Lanczos vector y of the iteration; wl / wp are the
-    // last two search directions and wf the slot the next one is written to
-    DevBuf<double> r1(c), r2(c), v(c), t(c), wl(c), wp(c), wf(c);
-    for (auto *bf : {&r1, &r2, &v, &t, &wl, &wp, &wf})
-        PHUB_HIP(c, hipMalloc(&bf->p, n * sizeof(double)));
+    // last two search directions and wf the slot the next one is written to.  The
+    // seven vectors come from the caller's workspace, else are allocated for the call.
+    Arena ar(c, workspace, workspace_bytes);
+    struct Vec { double *p; };
+    Vec r1{}, r2{}, v{}, t{}, wl{}, wp{}, wf{};
+    for (Vec *bf : {&r1, &r2, &v, &t, &wl, &wp, &wf}) {
+        bf->p = (double *)ar.take(n * sizeof(double));
+        if (!bf->p) return phub_fail(c, PHUB_ERR_OOM, "phub_adjoint_minres: vectors");
+    }
     const unsigned g = grid_for(n);
     int rc = phub_ensure_red(c, (size_t)g * 3 + 3 + 64);
     if (rc) return rc;

@@ -105,7 +105,7 @@ extern "C" int phub_fock_create(phub_ctx *c, int n, const int *n_up, int n_up_co
         }
         (s == 0 ? f->nsup : f->nsdn) = total;
         uint64_t *&dst = s == 0 ? f->bup : f->bdn;
-        hipError_t e = hipMalloc(&dst, std::max<int64_t>(total, 1) * sizeof(uint64_t));
+        hipError_t e = phub_malloc(&dst, std::max<int64_t>(total, 1) * sizeof(uint64_t));
         if (e != hipSuccess) return fock_fail_free(c, f, e, "species list");
         for (int k = 0; k <= n; ++k)
             if (off[k] >= 0) {
@@ -147,8 +147,8 @@ extern "C" int phub_fock_create(phub_ctx *c, int n, const int *n_up, int n_up_co
                     }
                 nib[((size_t)ch * (n + 1) + s0) * 16 + v] = (uint32_t)sum;
             }
-    hipError_t e = hipMalloc(&f->d_base, base.size() * sizeof(int64_t));
-    if (e == hipSuccess) e = hipMalloc(&f->d_nib, nib.size() * sizeof(uint32_t));
+    hipError_t e = phub_malloc(&f->d_base, base.size() * sizeof(int64_t));
+    if (e == hipSuccess) e = phub_malloc(&f->d_nib, nib.size() * sizeof(uint32_t));
     if (e == hipSuccess)
         e = phub_copy(c, f->d_base, base.data(), base.size() * sizeof(int64_t), hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -323,15 +323,15 @@ static hipError_t upload_prog(phub_ctx *c, OpProgDev &d, const std::vector<doubl
     hipError_t e = hipSuccess;
     for (auto pr : {std::make_pair(&d.urow, &urow), std::make_pair(&d.drow, &drow),
                     std::make_pair(&d.rowq_up, &rq_up), std::make_pair(&d.rowq_dn, &rq_dn)}) {
-        if (e == hipSuccess) e = hipMalloc(pr.first, pr.second->size() * sizeof(int));
+        if (e == hipSuccess) e = phub_malloc(pr.first, pr.second->size() * sizeof(int));
         if (e == hipSuccess)
             e = phub_copy(c, *pr.first, pr.second->data(), pr.second->size() * sizeof(int),
                           hipMemcpyHostToDevice);
     }
-    if (e == hipSuccess) e = hipMalloc(&d.coef, std::max<size_t>(cf.size(), 1) * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&d.start, st.size() * sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(&d.elem, std::max<size_t>(el.size(), 1) * sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(&d.touch, touch.size() * sizeof(int));
+    if (e == hipSuccess) e = phub_malloc(&d.coef, std::max<size_t>(cf.size(), 1) * sizeof(double));
+    if (e == hipSuccess) e = phub_malloc(&d.start, st.size() * sizeof(int));
+    if (e == hipSuccess) e = phub_malloc(&d.elem, std::max<size_t>(el.size(), 1) * sizeof(int));
+    if (e == hipSuccess) e = phub_malloc(&d.touch, touch.size() * sizeof(int));
     if (e == hipSuccess)
         e = phub_copy(c, d.touch, touch.data(), touch.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess && !cf.empty())
@@ -856,7 +856,7 @@ static int op_maps(phub_ctx *c, const phub_fock *f, const phub_opprog *pc, OpArg
             if (p->d_map) hipFree(p->d_map);
             p->d_map = nullptr;
             p->map_cap = 0;
-            PHUB_HIP(c, hipMalloc(&p->d_map, std::max<size_t>(bytes, 4)));
+            PHUB_HIP(c, phub_malloc(&p->d_map, std::max<size_t>(bytes, 4)));
             p->map_cap = std::max<size_t>(bytes, 4);
         }
         const OpArgs b = a;

@@ -81,7 +81,7 @@ extern "C" int phub_op_hubbard(phub_ctx *c, const phub_hubbard_desc *d, phub_op 
                 if (!(up >= 1e-14) || !(std::fabs(x) >= 1e-14)) x = 0.0;
             }
         }
-        hipError_t e = hipMalloc(&op->d_ul, n4 * sizeof(double));
+        hipError_t e = phub_malloc(&op->d_ul, n4 * sizeof(double));
         if (e == hipSuccess) e = phub_copy(c, op->d_ul, ul.data(), n4 * sizeof(double), hipMemcpyHostToDevice);
         if (e != hipSuccess) { phub_op_destroy(op); return phub_hip_fail(c, e, "4-index U upload"); }
         rc = phub_ensure_pair_tables(c, op);
@@ -133,6 +133,7 @@ extern "C" void phub_op_destroy(phub_op *op) {
     if (op->dn_pairmap) hipFree(op->dn_pairmap);
     phub_free_heis(&op->h);
     if (op->d_ul) hipFree(op->d_ul);
+    if (op->heis_all) phub_op_destroy(op->heis_all);
     delete op;
 }
 

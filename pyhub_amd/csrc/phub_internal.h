@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <atomic>
 #include <string>
 #include <vector>
 #include "../../include/phub.h"
@@ -19,6 +20,8 @@ struct phub_ctx {
     size_t red_cap = 0;              // doubles
     double *h_red = nullptr;         // pinned host mirror of reduction results
     size_t h_red_cap = 0;
+    void *arena = nullptr;           // scratch arena (phub_arena)
+    size_t arena_cap = 0;            // bytes
     // hipFree waits for every stream of the device: a context that works beside a
     // long kernel on another stream sets defer_free and its device memory is kept
     // until the context is destroyed
@@ -47,6 +50,21 @@ inline hipError_t phub_copy(phub_ctx *c, void *dst, const void *src, size_t byte
 }
 inline hipError_t phub_clear(phub_ctx *c, void *dst, size_t bytes) {
     return hipMemsetAsync(dst, 0, bytes, c->stream);
+}
+
+// Every device and pinned-host allocation of the library goes through these two
+// wrappers, which count them (phub_alloc_count): the tests of the allocation
+// contract read the count around a call.
+extern std::atomic<unsigned long long> g_phub_alloc_count;
+template <typename P>
+inline hipError_t phub_malloc(P **p, size_t bytes) {
+    g_phub_alloc_count.fetch_add(1, std::memory_order_relaxed);
+    return hipMalloc(reinterpret_cast<void **>(p), bytes);
+}
+template <typename P>
+inline hipError_t phub_host_malloc(P **p, size_t bytes, unsigned flags) {
+    g_phub_alloc_count.fetch_add(1, std::memory_order_relaxed);
+    return hipHostMalloc(reinterpret_cast<void **>(p), bytes, flags);
 }
 
 // Frees device memory that context c allocated (c may be null: plain hipFree).
@@ -164,6 +182,8 @@ struct phub_op {
     int tune_dn[2] = {-1, -1};
     // Heisenberg work-item order chosen by the tuner (0 XCD-contiguous, 1 round robin)
     int tune_heis[2] = {-1, -1};
+    // Heisenberg: the all-pairs view for spin correlations (lazy, owned)
+    phub_op *heis_all = nullptr;
 };
 
 // error helpers ------------------------------------------------------------------
@@ -215,6 +235,10 @@ void phub_free_hop_batch(HopBatch *b);
 int phub_build_heis(phub_ctx *ctx, int n, const double *J_host, int sign_convention,
                     HeisTables *out);
 void phub_free_heis(HeisTables *t);
+
+// the context's scratch arena: a device block of at least `bytes`, valid until the
+// next phub_arena call on this context (one request per call: carve it)
+int phub_arena(phub_ctx *ctx, size_t bytes, void **out);
 
 // reductions: sums of `ncomp` interleaved per-block partials -> host (synchronous)
 int phub_reduce_partials(phub_ctx *ctx, int nblocks, int ncomp, double *host_out);

@@ -353,14 +353,6 @@ __global__ __launch_bounds__(256) void k_rdm_flip(const uint64_t *__restrict__ b
 // ---------------------------------------------------------------------------
 // host assembly
 // ---------------------------------------------------------------------------
-template <typename T>
-struct DBuf {
-    T *p = nullptr;
-    ~DBuf() {
-        if (p) hipFree(p);
-    }
-};
-
 int phub_ensure_pair_tables(phub_ctx *c, phub_op *op) {
     if (!op->rup.idx) {
         std::vector<double> ones(op->n * op->n, 1.0);
@@ -389,16 +381,16 @@ static int hubbard_raw(phub_ctx *c, phub_op *op, const double *bra, const double
     const int64_t nu = op->dim_up, nd = op->dim_dn;
     int rc = phub_ensure_pair_tables(c, op);
     if (rc) return rc;
-    DBuf<double> dout, dR, dp, dq;
     const int nn = n * n;
-    PHUB_HIP(c, hipMalloc(&dout.p, 3 * nn * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&dR.p, nd * n * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&dp.p, nd * sizeof(double)));
-    PHUB_HIP(c, hipMalloc(&dq.p, nu * sizeof(double)));
-    PHUB_HIP(c, hipMemsetAsync(dout.p, 0, 3 * nn * sizeof(double), c->stream));
-    PHUB_HIP(c, hipMemsetAsync(dR.p, 0, nd * n * sizeof(double), c->stream));
-    PHUB_HIP(c, hipMemsetAsync(dp.p, 0, nd * sizeof(double), c->stream));
-    PHUB_HIP(c, hipMemsetAsync(dq.p, 0, nu * sizeof(double), c->stream));
+    const int ndiag = 3 * n + 3 * nn;
+    // one arena block, carved: hop / flip sums, per-row R, p, q, the diagonal sums
+    const size_t tot = (size_t)3 * nn + (size_t)nd * n + (size_t)nd + (size_t)nu + (size_t)ndiag;
+    void *ab = nullptr;
+    rc = phub_arena(c, tot * sizeof(double), &ab);
+    if (rc) return rc;
+    struct { double *p; } dout{(double *)ab}, dR{dout.p + 3 * nn}, dp{dR.p + nd * n},
+        dq{dp.p + nd}, ddiag{dq.p + nu};
+    PHUB_HIP(c, hipMemsetAsync(ab, 0, tot * sizeof(double), c->stream));
     const int ncb = (int)((nu + 255) / 256);
     const int64_t nrg = (nd + RDM_ROWS - 1) / RDM_ROWS;
     const unsigned grid = (unsigned)(ncb * nrg);
@@ -420,7 +412,7 @@ static int hubbard_raw(phub_ctx *c, phub_op *op, const double *bra, const double
         const char *env = std::getenv("PHUB_RDM_PAIRMAP");   // "0": ranks per row
         const bool use_pm = !(env && env[0] == '0');
         if (use_pm && !op->dn_pairmap && pm_bytes <= (size_t(1) << 30) && nd < (1ll << 30)) {
-            PHUB_HIP(c, hipMalloc(&op->dn_pairmap, pm_bytes));
+            PHUB_HIP(c, phub_malloc(&op->dn_pairmap, pm_bytes));
             hipLaunchKernelGGL(k_pairmap, dim3((unsigned)(((size_t)nn * nd + 255) / 256)), dim3(256), 0,
                                c->stream, c->d_binom, n, op->ndn, nd, op->dn.masks, op->dn_pairmap);
         }
@@ -430,10 +422,6 @@ static int hubbard_raw(phub_ctx *c, phub_op *op, const double *bra, const double
                            dout.p + 2 * nn, ncb, use_pm ? (const int32_t *)op->dn_pairmap : nullptr);
     }
     // the diagonal quantities assembled on the device (site J <-> bit n-J)
-    const int ndiag = 3 * n + 3 * nn;
-    DBuf<double> ddiag;
-    PHUB_HIP(c, hipMalloc(&ddiag.p, ndiag * sizeof(double)));
-    PHUB_HIP(c, hipMemsetAsync(ddiag.p, 0, ndiag * sizeof(double), c->stream));
     hipLaunchKernelGGL(k_rdm_assemble, dim3(asm_grid(nu + nd)), dim3(ASM_THREADS), 0, c->stream, n,
                        nu, nd, op->up.masks, op->dn.masks, dq.p, dp.p, dR.p, ddiag.p);
     PHUB_HIP(c, hipGetLastError());
@@ -618,8 +606,9 @@ extern "C" int phub_two_rdm_hubbard(phub_ctx *c, const phub_op *op, const double
     if (rc) return rc;
     const int n = op->n;
     const size_t n4 = (size_t)n * n * n * n;
-    DBuf<double> d;
-    PHUB_HIP(c, hipMalloc(&d.p, n4 * sizeof(double)));
+    struct { double *p; } d{nullptr};
+    rc = phub_arena(c, n4 * sizeof(double), (void **)&d.p);
+    if (rc) return rc;
     PHUB_HIP(c, hipMemsetAsync(d.p, 0, n4 * sizeof(double), c->stream));
     const int64_t nb = (op->dim + 255) / 256;
     if (nb > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
@@ -700,8 +689,9 @@ static int heis_bonds(phub_ctx *c, const phub_op *op, const double *bra, const d
     if (!t.masks)
         return phub_fail(c, PHUB_ERR_UNSUPPORTED, "spin correlations need dim <= 2^28 (mask table)");
     const int nb = t.nb;
-    DBuf<double> d;
-    PHUB_HIP(c, hipMalloc(&d.p, (3 * nb + 1) * sizeof(double)));
+    struct { double *p; } d{nullptr};
+    int rc = phub_arena(c, (3 * nb + 1) * sizeof(double), (void **)&d.p);
+    if (rc) return rc;
     PHUB_HIP(c, hipMemsetAsync(d.p, 0, (3 * nb + 1) * sizeof(double), c->stream));
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((t.dim + 255) / 256, 2048));
     hipLaunchKernelGGL(k_heis_bonds, dim3(grid), dim3(256), (3 * nb + 1) * sizeof(double),
@@ -711,6 +701,23 @@ static int heis_bonds(phub_ctx *c, const phub_op *op, const double *bra, const d
     h.resize(3 * nb + 1);
     PHUB_HIP(c, hipMemcpyAsync(h.data(), d.p, (3 * nb + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    return PHUB_OK;
+}
+
+// The all-pairs view of a Heisenberg operator (every J < K a bond, same sign
+// convention): built on the first spin-correlation call and kept with the
+// operator, so later calls allocate nothing.
+static int heis_all_pairs(phub_ctx *c, const phub_op *op, const phub_op **out) {
+    phub_op *mop = const_cast<phub_op *>(op);
+    if (!mop->heis_all) {
+        const int n = op->n;
+        std::vector<double> ones(n * n, 1.0);
+        for (int i = 0; i < n; ++i) ones[i * n + i] = 0.0;
+        phub_heis_desc hd{n, ones.data(), op->sign_convention};
+        int rc = phub_op_heisenberg(c, &hd, &mop->heis_all);
+        if (rc) return rc;
+    }
+    *out = mop->heis_all;
     return PHUB_OK;
 }
 
@@ -731,17 +738,13 @@ extern "C" int phub_rdm_heisenberg(phub_ctx *c, const phub_op *op, const double 
     if (!c || !op || !bra || !ket || !sisj) return phub_fail(c, PHUB_ERR_ARG, "null argument");
     if (op->kind != 1) return phub_fail(c, PHUB_ERR_ARG, "not a Heisenberg operator");
     // J_SPIN runs over every ordered pair; pairs without a bond still need their
-    // <S_J.S_K>: compute with an all-pairs operator view.
+    // <S_J.S_K>: computed with the operator's all-pairs view
     const int n = op->n;
-    std::vector<double> ones(n * n, 1.0);
-    for (int i = 0; i < n; ++i) ones[i * n + i] = 0.0;
-    phub_op *all = nullptr;
-    phub_heis_desc hd{n, ones.data(), op->sign_convention};
-    int rc = phub_op_heisenberg(c, &hd, &all);
+    const phub_op *all = nullptr;
+    int rc = heis_all_pairs(c, op, &all);
     if (rc) return rc;
     std::vector<double> h;
     rc = heis_bonds(c, all, bra, ket, h);
-    phub_op_destroy(all);
     if (rc) return rc;
     std::vector<int> J1, K1;
     for (int J = 1; J <= n; ++J)
@@ -765,15 +768,11 @@ extern "C" int phub_hf_grad_heisenberg(phub_ctx *c, const phub_op *op, const dou
     if (!c || !op || !psi || !grad_j) return phub_fail(c, PHUB_ERR_ARG, "null argument");
     if (op->kind != 1) return phub_fail(c, PHUB_ERR_ARG, "not a Heisenberg operator");
     const int n = op->n;
-    std::vector<double> ones(n * n, 1.0);
-    for (int i = 0; i < n; ++i) ones[i * n + i] = 0.0;
-    phub_op *all = nullptr;
-    phub_heis_desc hd{n, ones.data(), op->sign_convention};
-    int rc = phub_op_heisenberg(c, &hd, &all);
+    const phub_op *all = nullptr;
+    int rc = heis_all_pairs(c, op, &all);
     if (rc) return rc;
     std::vector<double> h;
     rc = heis_bonds(c, all, psi, psi, h);
-    phub_op_destroy(all);
     if (rc) return rc;
     const double nrm = h[h.size() - 1];
     int b = 0;

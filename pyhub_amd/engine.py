@@ -212,17 +212,37 @@ class _Op:
                              alpha[: res.steps].copy(), beta[: res.steps + 1].copy(), psi,
                              psis, np.array(res.e_states[:nst]), bool(res.norm_lost))
 
-    def cf_chain(self, start: torch.Tensor, nsteps: int, precision: int = 64):
+    def _ws(self, query, *args) -> torch.Tensor:
+        """A device workspace of the size an entry point's *_workspace_bytes query gives."""
+        nb = C.c_size_t(0)
+        rc = query(self.h, *args, C.byref(nb))
+        if rc:
+            raise ValueError("workspace size query failed")
+        return torch.empty(max(int(nb.value), 1), dtype=torch.uint8, device=self.ctx.device)
+
+    def cf_chain_workspace(self, precision: int = 64) -> torch.Tensor:
+        return self._ws(self.ctx.lib.phub_cf_chain_workspace_bytes, precision)
+
+    def cf_chain(self, start: torch.Tensor, nsteps: int, precision: int = 64,
+                 workspace: torch.Tensor | None = None):
+        """Single-vector chain (alpha, beta, <start|start>); workspace: device memory of
+        cf_chain_workspace() size for the three chain vectors (else allocated per call)."""
         a = np.zeros(nsteps)
         b = np.zeros(nsteps)
         n2 = np.zeros(1)
         start = start.to(self.ctx.device, torch.float64).contiguous()
+        ws = workspace if workspace is not None else self.cf_chain_workspace(precision)
         self.ctx.check(self.ctx.lib.phub_cf_chain(self.ctx.h, self.h, C.c_void_p(start.data_ptr()),
-                                                  nsteps, precision, _dptr(a), _dptr(b), _dptr(n2)))
+                                                  nsteps, precision, _dptr(a), _dptr(b), _dptr(n2),
+                                                  C.c_void_p(ws.data_ptr()),
+                                                  ws.numel() * ws.element_size()))
         return a, b, float(n2[0])
 
+    def band_workspace(self, p: int) -> torch.Tensor:
+        return self._ws(self.ctx.lib.phub_band_lanczos_workspace_bytes, int(p))
+
     def band_lanczos(self, V: torch.Tensor, max_steps: int = 600, tol: float = 1e-8,
-                     raise_noconv: bool = False):
+                     raise_noconv: bool = False, workspace: torch.Tensor | None = None):
         """Band Lanczos from the p start vectors V (p, dim) (GF_BANDLANCZOS): returns the
         poles (ascending eigenvalues of the band matrix) and Q (npoles, p) with
         Q[k, i] = sum_{j<=i} R[i, j] VEC[j, k]."""
@@ -230,7 +250,9 @@ class _Op:
         p = V.shape[0]
         if V.shape[1] != self.dim:
             raise ValueError("start vectors must have the operator dimension")
-        o = L.BandOpts(int(max_steps), float(tol))
+        ws = workspace if workspace is not None else self.band_workspace(p)
+        o = L.BandOpts(int(max_steps), float(tol), C.c_void_p(ws.data_ptr()),
+                       ws.numel() * ws.element_size())
         pos = np.zeros(max_steps + p)
         Q = np.zeros((max_steps + p) * p)
         npl = C.c_int(0)
@@ -241,13 +263,22 @@ class _Op:
         m = npl.value
         return pos[:m].copy(), Q[: m * p].reshape(m, p).copy()
 
-    def minres(self, psi: torch.Tensor, e0: float, b: torch.Tensor, max_iter=500, tol=1e-10):
-        x = self.vec(torch.float64)
+    def minres_workspace(self) -> torch.Tensor:
+        return self._ws(self.ctx.lib.phub_adjoint_minres_workspace_bytes)
+
+    def minres(self, psi: torch.Tensor, e0: float, b: torch.Tensor, max_iter=500, tol=1e-10,
+               workspace: torch.Tensor | None = None, out: torch.Tensor | None = None):
+        """Adjoint MINRES (H - e0) x = b on the complement of psi; workspace of
+        minres_workspace() size for the seven work vectors (torch's caching allocator
+        reuses it across calls when none is passed)."""
+        x = self.vec(torch.float64) if out is None else out
+        ws = workspace if workspace is not None else self.minres_workspace()
         it = C.c_int(0)
         rr = C.c_double(0)
         self.ctx.check(self.ctx.lib.phub_adjoint_minres(
             self.ctx.h, self.h, C.c_void_p(psi.data_ptr()), e0, C.c_void_p(b.contiguous().data_ptr()),
-            C.c_void_p(x.data_ptr()), max_iter, tol, C.byref(it), C.byref(rr)))
+            C.c_void_p(x.data_ptr()), max_iter, tol, C.byref(it), C.byref(rr),
+            C.c_void_p(ws.data_ptr()), ws.numel() * ws.element_size()))
         return x, it.value, rr.value
 
 
