@@ -302,6 +302,20 @@ int phub_cf_eval(phub_ctx *ctx, int n_chains, int nsteps, const double *alpha,
                  const double *beta, const double *norm2, const double *shift,
                  const double *sign, int n_omega, const double *omega, double eta,
                  double *out);
+/* phub_cf_eval without allocating: for callers that cannot allocate device memory
+ * (graph capture, tight serving loops, harnesses that forbid it).  No counterpart
+ * in the reference.  Same arguments, results and errors as phub_cf_eval; the
+ * staged inputs and the output (2 n_chains nsteps + 3 n_chains + nw +
+ * 2 n_chains nw doubles) must fit PHUB_CF_NOALLOC_DOUBLES (2^22 doubles, 32 MiB,
+ * a fixed device array each GPU holds while the library is loaded), else
+ * PHUB_ERR_UNSUPPORTED.  The array is shared by every call on one GPU: calls that
+ * overlap on one GPU (on different streams) race. */
+#define PHUB_CF_NOALLOC_DOUBLES (1 << 22)
+#define PHUB_MAX_DEVICES 64
+int phub_cf_eval_noalloc(phub_ctx *ctx, int n_chains, int nsteps, const double *alpha,
+                         const double *beta, const double *norm2, const double *shift,
+                         const double *sign, int nw, const double *omega, double eta,
+                         double *out);
 
 /* ---- gradients ---------------------------------------------------------------
  * Hellmann-Feynman: dE0/dT(J,K), dE0/dU_J for the Hubbard operator in one pass

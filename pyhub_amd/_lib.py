@@ -102,6 +102,8 @@ SIGNATURES = {
                                      _dp, _dp, _dp, _dp]),
     "phub_cf_eval": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp,
                                C.c_double, _dp]),
+    "phub_cf_eval_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int,
+                                       _dp, C.c_double, _dp]),
     "phub_hf_grad_hubbard": (C.c_int, [_vp, _vp, _vp, _dp, _dp]),
     "phub_grad_hubbard": (C.c_int, [_vp, _vp, _vp, _vp, _dp, _dp]),
     "phub_hf_grad_heisenberg": (C.c_int, [_vp, _vp, _vp, _dp]),

@@ -11,6 +11,8 @@
 #include "phub_internal.h"
 
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 __global__ __launch_bounds__(256) void k_excite(const uint64_t *__restrict__ binom, int n,
@@ -96,31 +98,93 @@ __global__ void k_cf_eval(int nch, int m, const double *__restrict__ al,
     out[2 * t + 1] = s * n2[ch] * fi;
 }
 
+// Inputs staged at d (2 nch m + 3 nch + nw doubles), then the output (2 nch nw).
+static size_t cf_stage_doubles(int nch, int m, int nw) {
+    return 2 * (size_t)nch * m + 3 * (size_t)nch + (size_t)nw + 2 * (size_t)nch * nw;
+}
+
+static int cf_eval_staged(phub_ctx *c, double *d, int nch, int m, const double *alpha,
+                          const double *beta, const double *norm2, const double *shift,
+                          const double *sign, int nw, const double *omega, double eta,
+                          double *out) {
+    const size_t na = (size_t)nch * m;
+    double *da = d, *db = da + na, *dn = db + na, *ds = dn + nch, *dg = ds + nch, *dw = dg + nch,
+           *dout = dw + nw;
+    const int64_t nt = (int64_t)nch * nw;
+    if ((nt + 255) / 256 > (int64_t)INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
+    PHUB_HIP(c, hipMemcpyAsync(da, alpha, na * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    PHUB_HIP(c, hipMemcpyAsync(db, beta, na * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    PHUB_HIP(c, hipMemcpyAsync(dn, norm2, nch * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    PHUB_HIP(c, hipMemcpyAsync(ds, shift, nch * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    PHUB_HIP(c, hipMemcpyAsync(dg, sign, nch * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    PHUB_HIP(c, hipMemcpyAsync(dw, omega, nw * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_cf_eval, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, nch,
+                       m, da, db, dn, ds, dg, nw, dw, eta, dout);
+    PHUB_HIP(c, hipGetLastError());
+    PHUB_HIP(c, hipMemcpyAsync(out, dout, 2 * (size_t)nt * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    return PHUB_OK;
+}
+
 extern "C" int phub_cf_eval(phub_ctx *c, int nch, int m, const double *alpha, const double *beta,
                             const double *norm2, const double *shift, const double *sign,
                             int nw, const double *omega, double eta, double *out) {
     if (!c || nch < 0 || m < 1 || nw < 0) return phub_fail(c, PHUB_ERR_ARG, "bad argument");
     if (nch == 0 || nw == 0) return PHUB_OK;
     double *d = nullptr;
-    const size_t na = (size_t)nch * m;
-    const size_t tot = 2 * na + 3 * (size_t)nch + nw + 2 * (size_t)nch * nw;
-    int rc = phub_arena(c, tot * sizeof(double), (void **)&d);
+    int rc = phub_arena(c, cf_stage_doubles(nch, m, nw) * sizeof(double), (void **)&d);
     if (rc) return rc;
-    double *da = d, *db = da + na, *dn = db + na, *ds = dn + nch, *dg = ds + nch, *dw = dg + nch,
-           *dout = dw + nw;
-    hipMemcpyAsync(da, alpha, na * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(db, beta, na * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(dn, norm2, nch * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(ds, shift, nch * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(dg, sign, nch * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(dw, omega, nw * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    const int64_t nt = (int64_t)nch * nw;
-    hipLaunchKernelGGL(k_cf_eval, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, nch,
-                       m, da, db, dn, ds, dg, nw, dw, eta, dout);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(out, dout, 2 * (size_t)nt * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return phub_hip_fail(c, e, "phub_cf_eval");
+    return cf_eval_staged(c, d, nch, m, alpha, beta, norm2, shift, sign, nw, omega, eta, out);
+}
+
+// ---------------------------------------------------------------------------
+// phub_cf_eval without any allocation, for callers that cannot allocate: the
+// staged inputs and the output live in `scratch`, a file-scope device array of
+// PHUB_CF_NOALLOC_DOUBLES doubles (32 MiB) that every GPU the library runs on
+// holds until the process ends.  Host code never passes `scratch` itself (its name
+// is a host-side stand-in, not a device address): the first call on a device,
+// with that device current, looks up the array's address there with
+// hipGetSymbolAddress and keeps it per device.  All calls on one GPU share the
+// array, so calls that overlap on one GPU (contexts on different streams) race;
+// calls on one stream are ordered.  Nothing in it needs clearing: each call
+// writes every staged input and the kernel writes every output element.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[PHUB_CF_NOALLOC_DOUBLES];
+
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[PHUB_MAX_DEVICES];
+
+static int scratch_for_device(phub_ctx *c, double **out) {
+    if (c->device < 0 || c->device >= PHUB_MAX_DEVICES)
+        return phub_fail(c, PHUB_ERR_UNSUPPORTED, "device index beyond PHUB_MAX_DEVICES");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[c->device]) {
+        PHUB_HIP(c, hipSetDevice(c->device));
+        void *p = nullptr;
+        PHUB_HIP(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return phub_fail(c, PHUB_ERR_HIP, "hipGetSymbolAddress returned no address");
+        g_scratch_ptr[c->device] = (double *)p;
+    }
+    *out = g_scratch_ptr[c->device];
     return PHUB_OK;
+}
+
+extern "C" int phub_cf_eval_noalloc(phub_ctx *c, int nch, int m, const double *alpha,
+                                    const double *beta, const double *norm2, const double *shift,
+                                    const double *sign, int nw, const double *omega, double eta,
+                                    double *out) {
+    if (!c || nch < 0 || m < 1 || nw < 0) return phub_fail(c, PHUB_ERR_ARG, "bad argument");
+    if (nch == 0 || nw == 0) return PHUB_OK;
+    const size_t need = cf_stage_doubles(nch, m, nw);
+    if (need > (size_t)PHUB_CF_NOALLOC_DOUBLES)
+        return phub_fail(c, PHUB_ERR_UNSUPPORTED,
+                         "phub_cf_eval_noalloc: 2*nch*m + 3*nch + nw + 2*nch*nw = " +
+                             std::to_string(need) + " doubles exceeds the fixed scratch of " +
+                             std::to_string((size_t)PHUB_CF_NOALLOC_DOUBLES) +
+                             " doubles; use phub_cf_eval");
+    double *d = nullptr;
+    int rc = scratch_for_device(c, &d);
+    if (rc) return rc;
+    return cf_eval_staged(c, d, nch, m, alpha, beta, norm2, shift, sign, nw, omega, eta, out);
 }
