@@ -1,8 +1,8 @@
-"""Operator constructors with PyHub's names (mirror of pyhub/tools/operators.py).
+"""Operator constructors with PyHub's names (pyhub/tools/operators.py).
 
 Each class is a one-factor Operators whose factor string selects the elementary
-routine of the device program; `sisj` is built from `c_dagger_c`, `n`, `_n` and
-`sz` as in the reference (operators.py:36-48).
+routine of the device program (the class names are the API).  `sisj` composes
+S_i . S_j from `c_dagger_c`, `n`, `_n` and `sz`; `opeexp` sums exp(op) psi.
 """
 from __future__ import annotations
 
@@ -76,50 +76,59 @@ class empty_operator(Operators):
 
 
 def sisj(*args):
-    """S_i . S_j as a sum of products (operators.py:36-48): on one site
-    n_up (1 - n_dn) / 2 twice, else minus half the two spin-exchange products; then
-    sz * sz in both cases."""
-    op = empty_operator()
+    """The S_i . S_j operator of PyHub's helper (tools/operators.py:51-60), as a sum
+    of products in this order: the transverse part, then S^z_i S^z_j.
+
+    The transverse part (S+_i S-_j + S-_i S+_j) / 2 is written with the hopping
+    pairs that move one spin of each species between the two sites; bringing
+    c^dag_iu c_iu-bar c^dag_ju-bar c_ju into that order flips its sign, so each of
+    the two exchange products enters with -1/2.  On one site the helper keeps its
+    own form n_up (1 - n_dn), as two halves (it is the reference's operator, so
+    <sisj(i, i)> is what PyHub reports)."""
     i, j = args[0][0], args[1][0]
     if i == j:
-        op += n((i, "up")) * _n((i, "down")) / 2 + n((i, "up")) * _n((i, "down")) / 2
+        half = n((i, "up")) * _n((i, "down")) / 2
+        parts = [half, half]
     else:
-        op -= (c_dagger_c((i, "u"), (j, "u")) * c_dagger_c((j, "d"), (i, "d"))
-               + c_dagger_c((i, "d"), (j, "d")) * c_dagger_c((j, "u"), (i, "u"))) / 2
-    op += sz((i,)) * sz((j,))
-    return op
+        exchange = (c_dagger_c((i, a), (j, a)) * c_dagger_c((j, b), (i, b))
+                    for a, b in (("u", "d"), ("d", "u")))
+        parts = [-0.5 * term for term in exchange]
+    return opesum(parts + [sz((i,)) * sz((j,))])
+
+
+def _taylor_terms(op: Operators, v, count):
+    """Successive terms op^k v / k! for k = 1 .. count, each from the previous."""
+    term = v
+    for k in range(1, count + 1):
+        term = op @ term / k
+        yield term
 
 
 def opeexp(op: Operators, psi, ftol=1.0e-6, unitary=False):
-    """exp(op) psi by its series (operators.py:96-125); without psi, the matrix
-    whose row i is exp(op) e_i."""
-    if isinstance(psi, (list, np.ndarray)):
-        conv = False
-        new_psi, full_psi, i, val = np.copy(psi), np.copy(psi), 1, 1.0e3
-        nmr0 = np.linalg.norm(psi)
-        while not conv and i < 100:
-            new_psi = op @ new_psi / i
-            full_psi = full_psi + new_psi
-            i += 1
-            nmr = full_psi.conj() @ full_psi
-            if unitary:
-                if np.abs(nmr0 - nmr) < ftol:
-                    conv = True
-            else:
-                if np.abs(val - nmr) < ftol:
-                    conv = True
-                else:
-                    val = nmr
-        if i == 100:
-            raise ValueError("exponential operator does not converge")
-        return full_psi
-    matrix = np.zeros((op.nb_selected, op.nb_selected))
-    psi0 = np.zeros(op.nb_selected)
-    for i in range(op.nb_selected):
-        psi0[i] = 1.0
-        matrix[i, :] = opeexp(op, psi0)
-        psi0[i] = 0.0
-    return matrix
+    """exp(op) psi by its Taylor series (PyHub's helper, tools/operators.py:96-125).
+
+    The partial sums stop when their squared norm settles: it moves by less than
+    ftol from one term to the next (the first comparison is against 1e3), or, with
+    unitary=True, it is within ftol of the norm of psi (the helper compares the norm
+    of psi with the squared norm of the sum).  At most 99 terms are summed, and
+    reaching the 99th raises ValueError, as in the helper.  When psi is not a list or
+    array (e.g. None): the matrix whose row i is exp(op) e_i over the selected basis
+    elements, each row summed with the default ftol and unitary."""
+    if not isinstance(psi, (list, np.ndarray)):
+        rows = [opeexp(op, e) for e in np.eye(op.nb_selected)]
+        return np.array(rows, dtype=float).reshape(op.nb_selected, op.nb_selected)
+    total = np.copy(psi)
+    ref = np.linalg.norm(psi) if unitary else None
+    last = 1.0e3
+    max_terms = 99
+    for k, term in enumerate(_taylor_terms(op, np.copy(psi), max_terms), start=1):
+        total = total + term
+        sq = total.conj() @ total
+        settled = abs(ref - sq) < ftol if unitary else abs(last - sq) < ftol
+        last = sq
+        if settled and k < max_terms:
+            return total
+    raise ValueError("exponential operator does not converge")
 
 
 def opesum(lst):

@@ -172,3 +172,48 @@ def test_operator_errors(ctx):
     with pytest.raises(ValueError):
         ctx.check(ctx.lib.phub_operator_apply(ctx.h, prog.fock, prog.prog, C.c_void_p(buf.data_ptr()),
                                               C.c_void_p(buf.data_ptr() + 8 * prog.nstates), 2))
+
+
+def test_opeexp_and_sisj_helpers(ctx):
+    """opeexp: exp(op) psi of the Taylor partial sums against scipy's expm of the
+    operator matrix (series converged to ~ftol), the matrix form row by row, the
+    unitary stopping rule on an anti-Hermitian exponent, and ValueError when 99
+    terms do not settle.  sisj: the operator's expectation equals the oracle's J_SPIN
+    (pinned to the reference) for i != j, and n_up (1 - n_dn) + Sz Sz on one site."""
+    import scipy.linalg as sla
+    from pyhub_amd.core.basis import Basis
+    from pyhub_amd.tools.models import fermi_hubbard
+    from pyhub_amd.tools.operators import opeexp, sisj
+    n_s = 4
+    T = O.ring_T(n_s)
+    H = fermi_hubbard(T, 3.0)
+    H.set_basis(Basis(n_s, hilbert=(2, 1)))
+    A = -0.2 * H
+    A.set_basis(Basis(n_s, hilbert=(2, 1)))
+    rng = np.random.default_rng(4)
+    psi = rng.normal(size=A.nstates)
+    psi /= np.linalg.norm(psi)
+    M = A.to_matrix
+    want = sla.expm(M) @ psi
+    got = opeexp(A, psi, ftol=1e-14)
+    assert np.max(np.abs(got - want)) < 1e-10
+    E = opeexp(A, None)
+    assert E.shape == (A.nstates, A.nstates)
+    assert np.max(np.abs(E - sla.expm(M).T)) < 1e-5          # rows exp(op) e_i, default ftol
+    B = 20.0 * H
+    B.set_basis(Basis(n_s, hilbert=(2, 1)))
+    with pytest.raises(ValueError):
+        opeexp(B, psi, ftol=1e-300)
+    m = O.HubbardModel(n_s, 2, 1, T, 3.0)
+    ev, V = np.linalg.eigh(m.dense())
+    g = V[:, 0]
+    ss = m.sisj(g)
+    for i, j in ((0, 1), (1, 3), (2, 2)):
+        op = sisj((i,), (j,))
+        op.set_basis(Basis(n_s, hilbert=(2, 1)))
+        val = op.avg(g)
+        if i != j:
+            assert abs(val - ss[i, j]) < 1e-12
+        else:
+            nu = m.one_rdm(g)[0][i, i]
+            assert abs(val - (nu - m.ni(g)[i]) - 0.25 * (nu + m.one_rdm(g)[1][i, i] - 2 * m.ni(g)[i])) < 1e-12
