@@ -61,6 +61,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanczos-tol", type=float, default=1e-9)
     ap.add_argument("--lanczos-max", type=int, default=300)
+    ap.add_argument("--psi-tol", type=float, default=1e-10,
+                    help="residual target ||H psi - E psi|| of every Lanczos form (FP64: a "
+                         "stopping condition; mixed: FP64 corrections of psi)")
+    ap.add_argument("--refine", type=int, default=4, help="at most this many FP64 corrections")
     ap.add_argument("--calib", action="store_true", help="add a 1 GiB device copy (counter calibration)")
     ap.add_argument("--workload", default="hubbard4x4",
                     choices=["hubbard4x4", "heis6x6", "batch256", "band4x4", "u4", "ops4x4", "grad64"])
@@ -718,7 +722,7 @@ def _lanczos_bench(op, args, L, torch):
     # rebuilds psi by a second pass beyond that).  Allocation is outside the timing:
     # a fresh allocation after large frees can cost seconds of driver-side clearing.
     free, _ = torch.cuda.mem_get_info(op.ctx.device)
-    need = max(op.lanczos_workspace_bytes(m, cap, 0, 2, True) for _, m, cap in runs)
+    need = max(op.lanczos_workspace_bytes(m, cap, 0, args.refine, True) for _, m, cap in runs)
     ws = torch.empty(min(need, int(0.9 * free)), dtype=torch.uint8, device=op.ctx.device)
     res["workspace_gb"] = ws.numel() / 1e9
     rq, rq_s = {}, {}
@@ -726,13 +730,15 @@ def _lanczos_bench(op, args, L, torch):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         r = op.lanczos(mode=mode, max_steps=cap, tol=args.lanczos_tol, v0=v0,
-                       refine_steps=2, want_psi=True, workspace=ws)
+                       refine_steps=args.refine, want_psi=True, workspace=ws,
+                       psi_tol=args.psi_tol)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         res[name] = {"e0": r.e0, "e0_ritz": r.e0_ritz, "steps": r.steps,
                      "converged": r.converged, "residual": r.residual,
                      "seconds": r.seconds, "seconds_lanczos": r.seconds_lanczos,
-                     "wall_incl_residual": wall, "hv_ms_avg": r.hv_ms_avg}
+                     "wall_incl_residual": wall, "hv_ms_avg": r.hv_ms_avg,
+                     "fp64_corrections": r.corrections, "correction_minres_iters": r.correction_iters}
         # reduced quantities of the state: 1-RDM per spin, double occupancy and the
         # S.S matrix in one pass over psi (FP64)
         torch.cuda.synchronize()
@@ -741,7 +747,8 @@ def _lanczos_bench(op, args, L, torch):
         torch.cuda.synchronize()
         rq_s[name] = time.perf_counter() - t1
         print(f"# lanczos {name}: E0={r.e0:.12f} steps={r.steps} {r.seconds:.2f}s "
-              f"res={r.residual:.2e}", file=sys.stderr, flush=True)
+              f"res={r.residual:.2e} corrections={r.corrections} ({r.correction_iters} inner)",
+              file=sys.stderr, flush=True)
         del r
     del ws
     torch.cuda.empty_cache()

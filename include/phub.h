@@ -162,7 +162,7 @@ typedef struct {
     double tol;             /* convergence of the lowest Ritz value (reference acc_lcz) */
     int check_every;        /* reference NADD = 5 */
     int fixed_steps;        /* >0: run exactly this many steps, no convergence test */
-    int refine_steps;       /* FP64 refinement steps after MIXED / PLAIN32 (0..4) */
+    int refine_steps;       /* FP64 corrections of psi after MIXED / PLAIN32 (0..8) */
     uint64_t seed;          /* start vector when d_v0 is NULL */
     int want_psi;           /* reconstruct the ground vector into d_psi */
     int n_states;           /* Ritz vectors to reconstruct (READ_PSI_HUBBARD for EXC_STATE
@@ -175,6 +175,13 @@ typedef struct {
                                one the plain modes keep fewer vectors (psi by a second
                                pass) and FULL64 / MIXED allocate their basis instead */
     size_t workspace_bytes;
+    double psi_tol;         /* > 0: target for ||H psi - E psi|| (psi normalised).  The
+                               run also continues until the Lanczos residual estimate
+                               beta_m |y_m| of the lowest Ritz pair is below it (FP32
+                               vectors: below max(psi_tol, 1e-6 max(1, |E|)), their
+                               floor); MIXED / PLAIN32 then apply FP64 corrections of psi
+                               (at most refine_steps) until the FP64 residual is below
+                               it.  0: the Ritz-value test only */
 } phub_lanczos_opts;
 
 #define PHUB_MAX_STATES 16
@@ -195,6 +202,8 @@ typedef struct {
                              * 1e-5 away from 1 (loss of orthogonality in the plain forms);
                              * it is normalised and the call still succeeds, as the
                              * reference prints the message and keeps its results */
+    int corrections;        /* FP64 corrections of psi applied (MIXED / PLAIN32) */
+    int correction_iters;   /* their inner MINRES iterations, summed */
 } phub_lanczos_result;
 
 /* Device bytes phub_lanczos uses for its Krylov vectors under these options (the

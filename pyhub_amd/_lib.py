@@ -40,7 +40,8 @@ class LanczosOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("max_steps", C.c_int), ("tol", C.c_double),
                 ("check_every", C.c_int), ("fixed_steps", C.c_int), ("refine_steps", C.c_int),
                 ("seed", C.c_uint64), ("want_psi", C.c_int), ("n_states", C.c_int),
-                ("workspace", C.c_void_p), ("workspace_bytes", C.c_size_t)]
+                ("workspace", C.c_void_p), ("workspace_bytes", C.c_size_t),
+                ("psi_tol", C.c_double)]
 
 
 class LanczosResult(C.Structure):
@@ -48,7 +49,8 @@ class LanczosResult(C.Structure):
                 ("steps", C.c_int), ("converged", C.c_int), ("ndeg", C.c_int),
                 ("seconds", C.c_double), ("seconds_lanczos", C.c_double),
                 ("hv_ms_avg", C.c_double), ("e_states", C.c_double * MAX_STATES),
-                ("norm_lost", C.c_int)]
+                ("norm_lost", C.c_int), ("corrections", C.c_int),
+                ("correction_iters", C.c_int)]
 
 
 class Cluster(C.Structure):

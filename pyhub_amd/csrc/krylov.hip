@@ -180,14 +180,26 @@ __global__ __launch_bounds__(256) void k_axpby(double a, const TX *__restrict__ 
     }
 }
 
-// Adjoint MINRES, the vector work of one iteration in three passes (FP64).
-// k_minres_scale: v = s * y.
-__global__ __launch_bounds__(256) void k_minres_scale(double *__restrict__ v,
-                                                      const double *__restrict__ y, double s,
-                                                      int64_t n) {
+// r = a * psi + b * y (psi FP64, y and r of type V)
+template <typename V>
+__global__ __launch_bounds__(256) void k_axpby_to(double a, const double *__restrict__ psi,
+                                                  double b, const V *__restrict__ y,
+                                                  V *__restrict__ r, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        v[i] = s * y[i];
+        r[i] = (V)(a * psi[i] + b * (double)y[i]);
+}
+
+// Adjoint MINRES, the vector work of one iteration in three passes.  Vectors of
+// type V (FP64 for the adjoint solve, FP32 for the loose correction solves of
+// jd_refine), psi and every sum in FP64.
+// k_minres_scale: v = s * y.
+template <typename V>
+__global__ __launch_bounds__(256) void k_minres_scale(V *__restrict__ v, const V *__restrict__ y,
+                                                      double s, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = (V)(s * (double)y[i]);
 }
 
 // after t = H v: t <- t - e0 v - c1 r1 (no r1 term when r1 is null); partials
@@ -195,9 +207,9 @@ __global__ __launch_bounds__(256) void k_minres_scale(double *__restrict__ v,
 // before the r1 term: taking it from the whole of t would also strip r1's own
 // small psi component, which breaks the three-term relation the MINRES update
 // relies on (slower convergence, 1e-11 instead of 1e-13 in a 60-state test).
-__global__ __launch_bounds__(256) void k_minres_a(double *__restrict__ t,
-                                                  const double *__restrict__ v,
-                                                  const double *__restrict__ r1,
+template <typename V>
+__global__ __launch_bounds__(256) void k_minres_a(V *__restrict__ t, const V *__restrict__ v,
+                                                  const V *__restrict__ r1,
                                                   const double *__restrict__ psi, double e0,
                                                   double c1, int64_t n,
                                                   double *__restrict__ part) {
@@ -205,12 +217,13 @@ __global__ __launch_bounds__(256) void k_minres_a(double *__restrict__ t,
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const double vi = v[i], pi = psi[i];
-        double ti = t[i] - e0 * vi;
+        const double vi = (double)v[i], pi = psi[i];
+        double ti = (double)t[i] - e0 * vi;
         s0 += pi * ti;
-        if (r1) ti -= c1 * r1[i];
-        t[i] = ti;
-        s1 += vi * ti;
+        if (r1) ti -= c1 * (double)r1[i];
+        const V tv = (V)ti;
+        t[i] = tv;
+        s1 += vi * (double)tv;
         s2 += vi * pi;
     }
     s0 = wave_sum(s0);
@@ -228,18 +241,19 @@ __global__ __launch_bounds__(256) void k_minres_a(double *__restrict__ t,
 }
 
 // r = t - p psi - c2 r2 (r may not alias t, psi or r2); partial |r|^2
-__global__ __launch_bounds__(256) void k_minres_b(const double *__restrict__ t,
+template <typename V>
+__global__ __launch_bounds__(256) void k_minres_b(const V *__restrict__ t,
                                                   const double *__restrict__ psi,
-                                                  const double *__restrict__ r2, double p,
-                                                  double c2, double *__restrict__ r, int64_t n,
+                                                  const V *__restrict__ r2, double p, double c2,
+                                                  V *__restrict__ r, int64_t n,
                                                   double *__restrict__ part) {
     __shared__ double sh[4];
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const double ri = t[i] - p * psi[i] - c2 * r2[i];
+        const V ri = (V)((double)t[i] - p * psi[i] - c2 * (double)r2[i]);
         r[i] = ri;
-        s += ri * ri;
+        s += (double)ri * (double)ri;
     }
     s = wave_sum(s);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
@@ -248,17 +262,16 @@ __global__ __launch_bounds__(256) void k_minres_b(const double *__restrict__ t,
 }
 
 // w = (v - oe wp - de wl) * ig, x += phi w
-__global__ __launch_bounds__(256) void k_minres_c(const double *__restrict__ v,
-                                                  const double *__restrict__ wp,
-                                                  const double *__restrict__ wl, double oe,
-                                                  double de, double ig, double phi,
-                                                  double *__restrict__ w, double *__restrict__ x,
-                                                  int64_t n) {
+template <typename V>
+__global__ __launch_bounds__(256) void k_minres_c(const V *__restrict__ v, const V *__restrict__ wp,
+                                                  const V *__restrict__ wl, double oe, double de,
+                                                  double ig, double phi, V *__restrict__ w,
+                                                  V *__restrict__ x, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const double wi = ((v[i] - oe * wp[i]) - de * wl[i]) * ig;
-        w[i] = wi;
-        x[i] += phi * wi;
+        const double wi = (((double)v[i] - oe * (double)wp[i]) - de * (double)wl[i]) * ig;
+        w[i] = (V)wi;
+        x[i] = (V)((double)x[i] + phi * wi);
     }
 }
 
@@ -686,6 +699,19 @@ static void convergence(LczState &s, double tol, int max_steps, const std::vecto
     if (!stable) s.converged = false;
 }
 
+// With a psi_tol, convergence also needs the Lanczos residual estimate of the lowest
+// Ritz pair, beta_m |y_m| = ||H x - theta x|| in exact arithmetic, below psi_tol.
+// FP32 vectors cannot represent psi below about 1e-6 relative, so their target is
+// raised to that floor (the FP64 corrections take psi the rest of the way).
+static void residual_rule(LczState &s, double psi_tol, bool fp32) {
+    if (!s.converged || !(psi_tol > 0.0) || s.breakdown) return;
+    double e0 = 0, e1 = 0;
+    std::vector<double> y;
+    if (lowest_ritz(s.steps, s.alpha.data(), s.beta.data(), &e0, &e1, &y)) return;
+    const double target = fp32 ? std::max(psi_tol, 1e-6 * std::max(1.0, std::fabs(e0))) : psi_tol;
+    if (std::fabs(s.beta[s.steps] * y[s.steps - 1]) > target) s.converged = false;
+}
+
 static int all_ritz(int m, const double *alpha, const double *beta, std::vector<double> &out) {
     out.assign(alpha, alpha + m);
     std::vector<double> e(std::max(m - 1, 1), 0.0);
@@ -859,6 +885,7 @@ static int lanczos_plain(phub_ctx *c, const phub_op *op, const phub_lanczos_opts
             s.e0 = ritz[0];
             if (s.breakdown) { s.converged = true; break; }
             convergence(s, o->tol, maxs, ritz, o->n_states);
+            residual_rule(s, o->psi_tol, sizeof(T) == 4);
         }
     }
     if (fixed || s.steps == 0 || !s.converged) {
@@ -978,41 +1005,10 @@ static int lanczos_full(phub_ctx *c, const phub_op *op, const phub_lanczos_opts 
             s.e0 = ritz[0];
             if (s.breakdown) { s.converged = true; break; }
             convergence(s, o->tol, maxs, ritz, o->n_states);
+            residual_rule(s, o->psi_tol, sizeof(T) == 4);
         }
     }
     if (s.steps && all_ritz(s.steps, s.alpha.data(), s.beta.data(), ritz) == 0) s.e0 = ritz[0];
-    return PHUB_OK;
-}
-
-// ---------------------------------------------------------------------------
-// Rayleigh-Ritz refinement of psi in FP64: a short Lanczos (m steps, full
-// reorthogonalisation) started from psi; psi <- lowest Ritz vector.
-// ---------------------------------------------------------------------------
-static int refine64(phub_ctx *c, const phub_op *op, double *psi, double *basis64, int m,
-                    double *d_coef, double *e_out) {
-    const int64_t n = op->dim;
-    PHUB_HIP(c, hipMemcpyAsync(basis64, psi, n * sizeof(double), hipMemcpyDeviceToDevice,
-                               c->stream));
-    phub_lanczos_opts o{};
-    o.max_steps = m;
-    o.check_every = 1000000;
-    LczState s;
-    std::vector<double> sig;
-    int rc = lanczos_full<double>(c, op, &o, basis64, basis_ld(n), s, sig, d_coef, nullptr, true);
-    if (rc) return rc;
-    double e0 = 0, e1 = 0;
-    std::vector<double> y;
-    if (lowest_ritz(s.steps, s.alpha.data(), s.beta.data(), &e0, &e1, &y))
-        return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
-    std::vector<double> cf(s.steps);
-    for (int j = 0; j < s.steps; ++j) cf[j] = y[j] * sig[j];
-    double nn = 0.0;
-    rc = dev_combine<double, double>(c, basis64, basis_ld(n), s.steps, cf.data(), psi, n, true, &nn, d_coef);
-    if (rc) return rc;
-    rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / std::sqrt(nn), psi, n,
-                                   nullptr);
-    if (rc) return rc;
-    *e_out = e0;
     return PHUB_OK;
 }
 
@@ -1033,6 +1029,62 @@ static int rq_residual(phub_ctx *c, const phub_op *op, const double *psi, double
     if (rc) return rc;
     *e = E;
     *res = std::sqrt(r2 / pp);
+    return PHUB_OK;
+}
+
+// Jacobi-Davidson corrections of a normalised FP64 psi (Sleijpen and van der Vorst,
+// SIAM J. Matrix Anal. Appl. 17, 401 (1996)): with E = <psi|H|psi> and
+// r = H psi - E psi (orthogonal to psi), both in FP64, solve the correction equation
+// (H - E) d = -r on the complement of psi loosely, by MINRES with FP32 vectors and
+// FP32 H*v, and take psi <- (psi + d) / |psi + d|.  The residual falls by about the
+// inner tolerance per step, so that tolerance aims at psi_tol in one step
+// (between 1e-5 and 1e-1 relative; at most 120 inner iterations).  At most rs
+// steps, fewer once |r| <= psi_tol.  On return e0 is the Rayleigh quotient of psi
+// and res its residual.
+template <typename V>
+static int minres_core(phub_ctx *c, const phub_op *op, const double *psi, double e0, const V *b,
+                       V *x, int max_iter, double tol, int *iters, double *rel_res, Arena &ar);
+
+__global__ __launch_bounds__(256) void k_jd_rhs(const double *__restrict__ r, float *__restrict__ b,
+                                                int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (float)(-r[i]);
+}
+
+static int jd_refine(phub_ctx *c, const phub_op *op, double *psi, Arena &ar, int rs,
+                     double psi_tol, double *e0, double *res, phub_lanczos_result *out) {
+    const int64_t n = op->dim;
+    const unsigned g = grid_for(n);
+    double *tmp = (double *)ar.take(n * sizeof(double));
+    float *b32 = (float *)ar.take(n * sizeof(float));
+    float *x32 = (float *)ar.take(n * sizeof(float));
+    if (!tmp || !b32 || !x32) return phub_fail(c, PHUB_ERR_OOM, "FP64 correction vectors do not fit");
+    int rc = rq_residual(c, op, psi, tmp, e0, res);
+    if (rc) return rc;
+    for (int it = 0; it < rs && !(psi_tol > 0.0 && *res <= psi_tol); ++it) {
+        // tmp holds r = H psi - E psi; the right-hand side is -r
+        hipLaunchKernelGGL(k_jd_rhs, dim3(g), dim3(256), 0, c->stream, tmp, b32, n);
+        PHUB_HIP(c, hipGetLastError());
+        const double tol_in = psi_tol > 0.0 ? std::min(1e-1, std::max(1e-5, 0.3 * psi_tol / *res))
+                                            : 1e-2;
+        int iters = 0;
+        double rr = 0.0;
+        const size_t mark = ar.used;
+        rc = minres_core<float>(c, op, psi, *e0, b32, x32, 120, tol_in, &iters, &rr, ar);
+        ar.used = mark;   // the seven MINRES vectors are free again
+        if (rc) return rc;
+        out->corrections += 1;
+        out->correction_iters += iters;
+        double nn = 0.0;
+        rc = dev_axpby<float, double>(c, 1.0, x32, 1.0, psi, n, &nn);
+        if (rc) return rc;
+        rc = dev_axpby<double, double>(c, 0.0, (const double *)nullptr, 1.0 / std::sqrt(nn), psi, n,
+                                       nullptr);
+        if (rc) return rc;
+        rc = rq_residual(c, op, psi, tmp, e0, res);
+        if (rc) return rc;
+    }
     return PHUB_OK;
 }
 
@@ -1103,27 +1155,18 @@ static int plain_mode(phub_ctx *c, const phub_op *op, const phub_lanczos_opts *o
             rc = normalise_states(c, d_psi, (int)ys.size(), n, s);
         }
         if (rc) return rc;
-        const int rs = sizeof(T) == 4 ? std::max(0, std::min(o->refine_steps, 4)) : 0;
+        const int rs = sizeof(T) == 4 ? std::max(0, std::min(o->refine_steps, 8)) : 0;
         // the Krylov vectors are no longer needed: their memory serves what follows
         PHUB_HIP(c, hipStreamSynchronize(c->stream));
         st.slot.clear();
         ar.release();
         if (rs > 0) {
-            // FP32 recurrence, then FP64 Rayleigh-Ritz refinement of the ground state
-            // (as the MIXED mode)
-            const int m = 3;
-            double *b64 = (double *)ar.take((size_t)(m + 1) * basis_ld(n) * sizeof(double));
-            if (!b64) return phub_fail(c, PHUB_ERR_OOM, "FP64 refinement vectors do not fit");
-            for (int it = 0; it < rs; ++it) {
-                double er = 0;
-                rc = refine64(c, op, d_psi, b64, m, coef, &er);
-                if (rc) return rc;
-                res->e0 = er;
-            }
+            // FP32 recurrence, then FP64 corrections of the ground state (as MIXED)
+            rc = jd_refine(c, op, d_psi, ar, rs, o->psi_tol, &res->e0, &res->residual, res);
+            if (rc) return rc;
             PHUB_HIP(c, hipStreamSynchronize(c->stream));
             res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            double erq = 0;
-            return rq_residual(c, op, d_psi, b64, &erq, &res->residual);
+            return PHUB_OK;
         }
         res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         double *tmp = (double *)ar.take(n * sizeof(double));
@@ -1143,7 +1186,9 @@ extern "C" int phub_lanczos_workspace_bytes(const phub_op *op, const phub_lanczo
     int steps = o->fixed_steps > 0 ? o->fixed_steps : o->max_steps;
     steps = (int)std::min<int64_t>(std::max(steps, 1), std::max<int64_t>(n, 1));
     const size_t m1 = (size_t)steps + 1;
-    const size_t refine = 4 * ld * sizeof(double);   // refine64 with m = 3
+    // jd_refine: one FP64 vector, nine FP32 ones
+    const size_t refine = ((ld * sizeof(double) + 255) & ~(size_t)255) +
+                          9 * ((ld * sizeof(float) + 255) & ~(size_t)255) + 512;
     size_t b = 0;
     switch (o->mode) {
     case PHUB_LCZ_FULL64: b = m1 * ld * sizeof(double); break;
@@ -1276,22 +1321,11 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
             // the FP32 basis is no longer needed: its memory serves the FP64 work
             PHUB_HIP(c, hipStreamSynchronize(c->stream));
             ar.release();
-            const int rs = std::max(0, std::min(o->refine_steps, 4));
-            const int m = 3;
-            double *b64 = (double *)ar.take((size_t)(m + 1) * basis_ld(n) * sizeof(double));
-            if (!b64) return phub_fail(c, PHUB_ERR_OOM, "FP64 refinement vectors do not fit");
-            for (int it = 0; it < rs; ++it) {
-                double er = 0;
-                rc = refine64(c, op, d_psi, b64, m, coef.p, &er);
-                if (rc) return rc;
-                res->e0 = er;
-            }
+            const int rs = std::max(0, std::min(o->refine_steps, 8));
+            rc = jd_refine(c, op, d_psi, ar, rs, o->psi_tol, &res->e0, &res->residual, res);
+            if (rc) return rc;
             PHUB_HIP(c, hipStreamSynchronize(c->stream));
             res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            double E = 0;
-            rc = rq_residual(c, op, d_psi, b64, &E, &res->residual);
-            if (rc) return rc;
-            if (rs == 0) res->e0 = E;
         }
     }
     PHUB_HIP(c, hipStreamSynchronize(c->stream));
@@ -1672,47 +1706,42 @@ extern "C" int phub_cf_chain(phub_ctx *c, const phub_op *op, const double *d_sta
 // ---------------------------------------------------------------------------
 // adjoint MINRES: (H - e0) x = b on the complement of psi (Paige-Saunders MINRES)
 // ---------------------------------------------------------------------------
-static int project_out(phub_ctx *c, const double *psi, double *y, int64_t n) {
-    double p = 0;
-    int rc = dev_dot<double, double>(c, psi, y, n, &p);
-    if (rc) return rc;
-    return dev_axpby<double, double>(c, -p, psi, 1.0, y, n, nullptr);
-}
-
 extern "C" int phub_adjoint_minres_workspace_bytes(const phub_op *op, size_t *bytes) {
     if (!op || !bytes) return PHUB_ERR_ARG;
     *bytes = 7 * (((size_t)op->dim * sizeof(double) + 255) & ~(size_t)255) + 256;
     return PHUB_OK;
 }
 
-extern "C" int phub_adjoint_minres(phub_ctx *c, const phub_op *op, const double *psi, double e0,
-                                   const double *b, double *x, int max_iter, double tol,
-                                   int *iters, double *rel_res, void *workspace,
-                                   size_t workspace_bytes) {
-    if (!c || !op || !psi || !b || !x) return phub_fail(c, PHUB_ERR_ARG, "null argument");
+// MINRES on (H - e0) restricted to the complement of psi (Paige-Saunders), vectors
+// of type V: x (zeroed here) solves it for the right-hand side b projected off psi,
+// to the relative residual tol or max_iter iterations.  Seven work vectors from ar.
+template <typename V>
+static int minres_core(phub_ctx *c, const phub_op *op, const double *psi, double e0, const V *b,
+                       V *x, int max_iter, double tol, int *iters, double *rel_res, Arena &ar) {
     const int64_t n = op->dim;
     // r2 is also the unnormalised Lanczos vector y of the iteration; wl / wp are the
-    // last two search directions and wf the slot the next one is written to.  The
-    // seven vectors come from the caller's workspace, else are allocated for the call.
-    Arena ar(c, workspace, workspace_bytes);
-    struct Vec { double *p; };
+    // last two search directions and wf the slot the next one is written to
+    struct Vec { V *p; };
     Vec r1{}, r2{}, v{}, t{}, wl{}, wp{}, wf{};
     for (Vec *bf : {&r1, &r2, &v, &t, &wl, &wp, &wf}) {
-        bf->p = (double *)ar.take(n * sizeof(double));
-        if (!bf->p) return phub_fail(c, PHUB_ERR_OOM, "phub_adjoint_minres: vectors");
+        bf->p = (V *)ar.take(n * sizeof(V));
+        if (!bf->p) return phub_fail(c, PHUB_ERR_OOM, "MINRES work vectors");
     }
     const unsigned g = grid_for(n);
     int rc = phub_ensure_red(c, (size_t)g * 3 + 3 + 64);
     if (rc) return rc;
-    PHUB_HIP(c, hipMemsetAsync(x, 0, n * sizeof(double), c->stream));
-    PHUB_HIP(c, hipMemsetAsync(wl.p, 0, n * sizeof(double), c->stream));
-    PHUB_HIP(c, hipMemsetAsync(wp.p, 0, n * sizeof(double), c->stream));
-    PHUB_HIP(c, hipMemcpyAsync(r1.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    rc = project_out(c, psi, r1.p, n);
+    PHUB_HIP(c, hipMemsetAsync(x, 0, n * sizeof(V), c->stream));
+    PHUB_HIP(c, hipMemsetAsync(wl.p, 0, n * sizeof(V), c->stream));
+    PHUB_HIP(c, hipMemsetAsync(wp.p, 0, n * sizeof(V), c->stream));
+    // r1 = b - (psi . b) psi
+    double pb = 0;
+    rc = dev_dot<double, V>(c, psi, b, n, &pb);
     if (rc) return rc;
-    PHUB_HIP(c, hipMemcpyAsync(r2.p, r1.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_axpby_to<V>, dim3(g), dim3(256), 0, c->stream, -pb, psi, 1.0, b, r1.p, n);
+    PHUB_HIP(c, hipGetLastError());
+    PHUB_HIP(c, hipMemcpyAsync(r2.p, r1.p, n * sizeof(V), hipMemcpyDeviceToDevice, c->stream));
     double b1sq = 0;
-    rc = dev_dot<double, double>(c, r1.p, r1.p, n, &b1sq);
+    rc = dev_dot<V, V>(c, r1.p, r1.p, n, &b1sq);
     if (rc) return rc;
     const double beta1 = std::sqrt(b1sq);
     if (iters) *iters = 0;
@@ -1723,11 +1752,12 @@ extern "C" int phub_adjoint_minres(phub_ctx *c, const phub_op *op, const double 
     double rnorm = beta1;
     for (it = 1; it <= max_iter; ++it) {
         // v = y / beta; t = (H - e0) v - (beta / oldb) r1; p = psi . (H - e0) v
-        hipLaunchKernelGGL(k_minres_scale, dim3(g), dim3(256), 0, c->stream, v.p, r2.p, 1.0 / beta, n);
-        rc = phub_hv_launch<double>(c, op, v.p, t.p, 1.0, (const double *)nullptr, 0.0, false, nullptr);
+        hipLaunchKernelGGL(k_minres_scale<V>, dim3(g), dim3(256), 0, c->stream, v.p, r2.p,
+                           1.0 / beta, n);
+        rc = phub_hv_launch<V>(c, op, v.p, t.p, 1.0, (const V *)nullptr, 0.0, false, nullptr);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_minres_a, dim3(g), dim3(256), 0, c->stream, t.p, v.p,
-                           it >= 2 ? (const double *)r1.p : (const double *)nullptr, psi, e0,
+        hipLaunchKernelGGL(k_minres_a<V>, dim3(g), dim3(256), 0, c->stream, t.p, v.p,
+                           it >= 2 ? (const V *)r1.p : (const V *)nullptr, psi, e0,
                            it >= 2 ? beta / oldb : 0.0, n, c->d_red);
         PHUB_HIP(c, hipGetLastError());
         double d3[3];
@@ -1736,7 +1766,7 @@ extern "C" int phub_adjoint_minres(phub_ctx *c, const phub_op *op, const double 
         const double p = d3[0];                    // psi . (H - e0) v
         const double alfa = d3[1] - p * d3[2];     // v . (t - p psi)
         // y_new = t - p psi - (alfa / beta) r2, into r1's slot; then r1 <- r2, r2 <- y_new
-        hipLaunchKernelGGL(k_minres_b, dim3(g), dim3(256), 0, c->stream, t.p, psi, r2.p, p,
+        hipLaunchKernelGGL(k_minres_b<V>, dim3(g), dim3(256), 0, c->stream, t.p, psi, r2.p, p,
                            alfa / beta, r1.p, n, c->d_red);
         PHUB_HIP(c, hipGetLastError());
         double yy = 0;
@@ -1756,8 +1786,8 @@ extern "C" int phub_adjoint_minres(phub_ctx *c, const phub_op *op, const double 
         const double phi = cs * phibar;
         phibar = sn * phibar;
         // w = (v - oldeps wp - delta wl) / gamma, x += phi w; wp <- wl, wl <- w
-        hipLaunchKernelGGL(k_minres_c, dim3(g), dim3(256), 0, c->stream, v.p, wp.p, wl.p, oldeps,
-                           delta, 1.0 / gamma, phi, wf.p, x, n);
+        hipLaunchKernelGGL(k_minres_c<V>, dim3(g), dim3(256), 0, c->stream, v.p, wp.p, wl.p,
+                           oldeps, delta, 1.0 / gamma, phi, wf.p, x, n);
         PHUB_HIP(c, hipGetLastError());
         // (wf, wp, wl) <- (old wp, old wl, the one just written)
         std::swap(wf.p, wp.p);
@@ -1769,4 +1799,14 @@ extern "C" int phub_adjoint_minres(phub_ctx *c, const phub_op *op, const double 
     if (iters) *iters = std::min(it, max_iter);
     if (rel_res) *rel_res = rnorm / beta1;
     return PHUB_OK;
+}
+
+extern "C" int phub_adjoint_minres(phub_ctx *c, const phub_op *op, const double *psi, double e0,
+                                   const double *b, double *x, int max_iter, double tol,
+                                   int *iters, double *rel_res, void *workspace,
+                                   size_t workspace_bytes) {
+    if (!c || !op || !psi || !b || !x) return phub_fail(c, PHUB_ERR_ARG, "null argument");
+    // the seven vectors come from the caller's workspace, else are allocated for the call
+    Arena ar(c, workspace, workspace_bytes);
+    return minres_core<double>(c, op, psi, e0, b, x, max_iter, tol, iters, rel_res, ar);
 }

@@ -94,6 +94,8 @@ class LanczosOutput:
     psis: torch.Tensor | None = None        # (n_states, dim) when n_states > 1
     e_states: np.ndarray | None = None      # lowest Ritz values
     norm_lost: bool = False                 # lanczos.f90:74-76 NORM LOST (psi renormalised)
+    corrections: int = 0                    # FP64 corrections of psi (MIXED / PLAIN32)
+    correction_iters: int = 0               # their inner MINRES iterations
 
 
 class _Op:
@@ -171,9 +173,12 @@ class _Op:
     def lanczos(self, mode=L.LCZ_PLAIN64, max_steps=600, tol=1e-8, check_every=5,
                 fixed_steps=0, refine_steps=0, seed=1, v0: torch.Tensor | None = None,
                 want_psi=True, raise_noconv=False, n_states=1,
-                workspace: torch.Tensor | None = None) -> LanczosOutput:
+                workspace: torch.Tensor | None = None, psi_tol: float = 0.0) -> LanczosOutput:
         """workspace: device memory for the Krylov vectors (any dtype, contiguous);
-        by default one is taken from torch's caching allocator for the call."""
+        by default one is taken from torch's caching allocator for the call.
+        psi_tol > 0: residual target ||H psi - E psi|| (FP64 vectors: a stopping
+        condition besides tol; MIXED / PLAIN32: the FP64 corrections of psi, at most
+        refine_steps of them, stop once it is met)."""
         ws = workspace if workspace is not None else self._workspace(
             mode, max_steps, fixed_steps, refine_steps, want_psi)
         if ws is not None and not ws.is_contiguous():
@@ -181,7 +186,8 @@ class _Op:
         o = L.LanczosOpts(mode, max_steps, tol, check_every, fixed_steps, refine_steps, seed,
                           1 if want_psi else 0, int(n_states),
                           C.c_void_p(ws.data_ptr()) if ws is not None else None,
-                          ws.numel() * ws.element_size() if ws is not None else 0)
+                          ws.numel() * ws.element_size() if ws is not None else 0,
+                          float(psi_tol))
         res = L.LanczosResult()
         nal = max(max_steps, fixed_steps) + 2
         alpha = np.zeros(nal)
@@ -210,7 +216,8 @@ class _Op:
         return LanczosOutput(res.e0, res.e0_ritz, res.residual, res.steps, bool(res.converged),
                              res.ndeg, res.seconds, res.seconds_lanczos, res.hv_ms_avg,
                              alpha[: res.steps].copy(), beta[: res.steps + 1].copy(), psi,
-                             psis, np.array(res.e_states[:nst]), bool(res.norm_lost))
+                             psis, np.array(res.e_states[:nst]), bool(res.norm_lost),
+                             int(res.corrections), int(res.correction_iters))
 
     def _ws(self, query, *args) -> torch.Tensor:
         """A device workspace of the size an entry point's *_workspace_bytes query gives."""
