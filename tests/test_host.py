@@ -23,3 +23,17 @@ def test_heisenberg_spec_coupling_matrix():
     assert J[0, 1] == J[1, 0] == 1.0 and J[1, 2] == J[2, 1] == 2.0 and s.sign_convention == 1
     r = HeisenbergSpec(n=4, bonds=[(0, 1)], convention="reference")
     assert r.Jm([3.0])[0, 1] == 3.0 and r.sign_convention == 0
+
+
+def test_pyhub_import_paths_alias_the_engine():
+    """The reference's module paths resolve to this engine's modules (same objects)."""
+    import importlib
+    import pyhub_amd.solver.fermi_hubbard as ours
+    ref = importlib.import_module("pyhub.solver.fermi_hubbard")
+    assert ref is ours and ref.FermiHubbard is ours.FermiHubbard
+    from pyhub.core.basis import Basis
+    from pyhub_amd.core.basis import Basis as B2
+    assert Basis is B2
+    from pyhub.tools.operators import c_dagger_c, opeexp, sisj  # noqa: F401
+    from pyhub.tools.pole import Pole  # noqa: F401
+    from pyhub.solver.heisenberg import Heisenberg  # noqa: F401
