@@ -71,6 +71,8 @@ extern "C" void phub_ctx_destroy(phub_ctx *c) {
     if (c->d_binom) hipFree(c->d_binom);
     if (c->d_red) hipFree(c->d_red);
     if (c->arena) hipFree(c->arena);
+    if (c->capture_stream) hipStreamDestroy(c->capture_stream);
+    if (c->h_lcz) hipHostFree(c->h_lcz);
     for (void *p : c->deferred) hipFree(p);
     if (c->h_red) hipHostFree(c->h_red);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -152,6 +154,16 @@ __global__ __launch_bounds__(1024) void k_reduce_partials(const double *__restri
         for (int w = 0; w < 16; ++w) t += sh[w];
         out[c] = t;
     }
+}
+
+// the same sums into device memory, asynchronously (no host wait; graph-capturable)
+int phub_reduce_partials_dev(phub_ctx *c, int nblocks, int ncomp, double *dev_out) {
+    if ((size_t)nblocks * ncomp + ncomp > c->red_cap)
+        return phub_fail(c, PHUB_ERR_ARG, "reduction buffer too small");
+    hipLaunchKernelGGL(k_reduce_partials, dim3(ncomp), dim3(1024), 0, c->stream, c->d_red,
+                       nblocks, ncomp, dev_out);
+    PHUB_HIP(c, hipGetLastError());
+    return PHUB_OK;
 }
 
 int phub_reduce_partials(phub_ctx *c, int nblocks, int ncomp, double *host_out) {

@@ -41,6 +41,9 @@ struct HubArgs {
     int u_uniform;
     double u0;
     double U[32];
+    // when set, the fused epilogue takes its scale and beta from device memory
+    // (dsc[0], dsc[1]) instead of the kernel arguments: graph-captured Lanczos steps
+    const double *dsc;
 };
 
 template <typename T>
@@ -116,11 +119,12 @@ __global__ __launch_bounds__(256) void k_hv_hub(HubArgs<T> a, const T *__restric
                 for (int k = 0; k < cd; ++k) s += cdr[k] * x[(int64_t)idr[k] * nu + u];
                 const size_t o = (size_t)d * nu + u;
                 if (EPI) {
-                    double val = scale * (double)s;
-                    if (prev) val -= beta * (double)prev[o];
+                    const double sc = a.dsc ? a.dsc[0] : scale, bt = a.dsc ? a.dsc[1] : beta;
+                    double val = sc * (double)s;
+                    if (prev) val -= bt * (double)prev[o];
                     const T vt = (T)val;
                     out[o] = vt;
-                    const double xs = scale * (double)x[o];
+                    const double xs = sc * (double)x[o];
                     s_xo += xs * (double)vt;
                     s_oo += (double)vt * (double)vt;
                 } else {
@@ -293,6 +297,7 @@ __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict
     __shared__ T cls[256];
     for (int q = threadIdx.x; q < a.ncls_dn; q += 256) cls[q] = a.cls_dn[q];
     __syncthreads();
+    const double sc = (EPI && a.dsc) ? a.dsc[0] : scale, bt = (EPI && a.dsc) ? a.dsc[1] : beta;
     double s_xo = 0.0, s_oo = 0.0;
     if (any) {
         // columns past the end read column col0 (in range) and are not stored
@@ -352,11 +357,11 @@ __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict
                     if (!ok[j]) continue;
                     const size_t o = (size_t)d * nu + cj[j];
                     if (EPI) {
-                        double val = scale * ((double)__builtin_nontemporal_load(out + o) + (double)sacc[j]);
-                        if (prev) val -= beta * (double)__builtin_nontemporal_load(prev + o);
+                        double val = sc * ((double)__builtin_nontemporal_load(out + o) + (double)sacc[j]);
+                        if (prev) val -= bt * (double)__builtin_nontemporal_load(prev + o);
                         const T vt = (T)val;
                         __builtin_nontemporal_store(vt, out + o);
-                        s_xo += scale * (double)x[o] * (double)vt;
+                        s_xo += sc * (double)x[o] * (double)vt;
                         s_oo += (double)vt * (double)vt;
                     } else {
                         __builtin_nontemporal_store(__builtin_nontemporal_load(out + o) + sacc[j], out + o);
@@ -403,11 +408,11 @@ __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict
                 const size_t o = (size_t)d * nu + cj[j];
                 if (EPI) {
                     // streamed once: non-temporal, so the slab keeps the L2
-                    double val = scale * ((double)__builtin_nontemporal_load(out + o) + (double)s[j]);
-                    if (prev) val -= beta * (double)__builtin_nontemporal_load(prev + o);
+                    double val = sc * ((double)__builtin_nontemporal_load(out + o) + (double)s[j]);
+                    if (prev) val -= bt * (double)__builtin_nontemporal_load(prev + o);
                     const T vt = (T)val;
                     __builtin_nontemporal_store(vt, out + o);
-                    s_xo += scale * (double)x[o] * (double)vt;
+                    s_xo += sc * (double)x[o] * (double)vt;
                     s_oo += (double)vt * (double)vt;
                 } else {
                     __builtin_nontemporal_store(__builtin_nontemporal_load(out + o) + s[j], out + o);
@@ -921,8 +926,9 @@ template int phub_hv_ul<float>(phub_ctx *, const phub_op *, const float *, float
                                const float *, double, bool, int *);
 
 template <typename T>
-static HubArgs<T> hub_args(const phub_op *op) {
+static HubArgs<T> hub_args(const phub_op *op, const phub_ctx *c = nullptr) {
     HubArgs<T> a{};
+    a.dsc = c ? c->dev_scalars : nullptr;
     a.dim_up = op->dim_up;
     a.dim_dn = op->dim_dn;
     a.mu = op->up.masks;
@@ -1145,7 +1151,7 @@ static int hv_two_kernel(phub_ctx *c, const phub_op *op, const T *x, T *out, dou
                          const T *prev, double beta, bool epi, int *nblocks_out) {
     int nblocks = 0;
     const size_t lds = 0;   // k_hv_up's LDS is static (UP_LDS_MAX)
-    HubArgs<T> a = hub_args<T>(op);
+    HubArgs<T> a = hub_args<T>(op, c);
     const T *cls = (sizeof(T) == 8) ? (const T *)op->up.cls64 : (const T *)op->up.cls32;
     const unsigned g1 = (unsigned)((op->dim_dn + R - 1) / R);
     hipLaunchKernelGGL((k_hv_up<T, R>), dim3(g1), dim3(UP_THREADS), lds, c->stream, a,
@@ -1201,7 +1207,7 @@ static int hv_launch_main(phub_ctx *c, const phub_op *op, const T *x, T *out, do
             int rc = phub_ensure_red(c, (size_t)nblocks * 2 + 64);
             if (rc) return rc;
         }
-        HubArgs<T> a = hub_args<T>(op);
+        HubArgs<T> a = hub_args<T>(op, c);
         if (epi)
             hipLaunchKernelGGL((k_hv_hub<T, HUB_R, true>), dim3(nblocks), dim3(256), 0, c->stream,
                                a, x, out, scale, prev, beta, c->d_red, ncb);

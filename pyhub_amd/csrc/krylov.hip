@@ -162,8 +162,11 @@ __global__ __launch_bounds__(256) void k_dot(const TA *__restrict__ a, const TB 
 template <typename TX, typename TY>
 __global__ __launch_bounds__(256) void k_axpby(double a, const TX *__restrict__ x, double b,
                                                TY *__restrict__ y, int64_t n,
-                                               double *__restrict__ part) {
+                                               double *__restrict__ part,
+                                               const double *__restrict__ dsc = nullptr) {
     __shared__ double sh[4];
+    // dsc: a graph-captured Lanczos step takes a = -alpha_k * scale from device memory
+    if (dsc) a = -dsc[2] * dsc[0];
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -178,6 +181,34 @@ __global__ __launch_bounds__(256) void k_axpby(double a, const TX *__restrict__ 
         __syncthreads();
         if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
     }
+}
+
+// Kernels of a graph-captured three-term Lanczos step, whose scalars live in device
+// memory: dsc[0] the scale of x_k, dsc[1] beta_k times the scale of x_{k-1},
+// dsc[2] alpha_k = <scale x_k, out>, dsc[3] |out|^2 after the alpha update (the
+// alpha update itself is k_axpby with its coefficient from dsc).
+// one thread: alpha_k and beta_{k+1} into their arrays, the scales of the next step
+// (beta_{k+1} = |out|, scale 1 / beta_{k+1}), k advanced
+__global__ void k_lcz_step_end(double *__restrict__ dsc, double *__restrict__ alpha,
+                               double *__restrict__ beta, int *__restrict__ steps) {
+    const int k = steps[0];
+    const double a = dsc[2], b = sqrt(fmax(dsc[3], 0.0));
+    alpha[k] = a;
+    beta[k + 1] = b;
+    dsc[1] = b * dsc[0];
+    dsc[0] = (b > 0.0) ? 1.0 / b : 0.0;
+    steps[0] = k + 1;
+}
+
+// the new Lanczos vector into its slot of the store: slots[k] after the step ended
+template <typename T>
+__global__ __launch_bounds__(256) void k_store_vec(const T *__restrict__ src,
+                                                   T *const *__restrict__ slots,
+                                                   const int *__restrict__ steps, int64_t n) {
+    T *dst = slots[steps[0]];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 // r = a * psi + b * y (psi FP64, y and r of type V)
@@ -894,6 +925,194 @@ static int lanczos_plain(phub_ctx *c, const phub_op *op, const phub_lanczos_opts
     return PHUB_OK;
 }
 
+// The three-term recurrence of lanczos_plain with each step captured once into a
+// hipGraph and replayed: the step's scalars stay in device memory (GraphBufs) and
+// alpha, beta go to the context's coherent pinned array, so a step is one graph
+// launch and the host waits only at the convergence checks (every check_every
+// steps).  Three graphs, one per rotation
+// of the three work vectors r[0..2]; with a store, each step also copies its new
+// vector into slots[k] (a device table of the store's slot pointers, every slot
+// allocated before the run).  The arithmetic is that of lanczos_plain, so both give
+// the same bits.  Steps past a breakdown inside a block of check_every steps are
+// discarded at the check.  Used for the Hubbard operators (their fused epilogue reads
+// device scalars); step 0 runs outside the graphs (it tunes the operator's tiles and
+// sizes the reduction buffer).
+struct GraphBufs {
+    double *dsc;       // 8 scalars
+    int *steps;        // 1
+    void **slots;      // max_steps + 2 (store mode)
+};
+
+static bool graph_capable(const phub_op *op) {
+    const char *e = std::getenv("PHUB_LANCZOS_GRAPH");   // 0: never, 1: any size
+    if (e && e[0] == '0') return false;
+    if (op->kind != 0 || op->d_ul) return false;
+    return (e && e[0] == '1') || op->dim <= ((int64_t)1 << 24);
+}
+
+template <typename T>
+static int lanczos_plain_graph(phub_ctx *c, const phub_op *op, const phub_lanczos_opts *o,
+                               const T *start, T *const r[3], LczState &s, EvTimer *tm,
+                               bool fixed, KrylovStore<T> *st, const GraphBufs &gb) {
+    const int64_t n = op->dim;
+    const int maxs = o->max_steps;
+    const int every = o->check_every > 0 ? o->check_every : 5;
+    const unsigned g = grid_for(n);
+    s.alpha.assign(maxs + 2, 0.0);
+    s.beta.assign(maxs + 2, 0.0);
+    double n2 = 0.0;
+    int rc = dev_dot<T, T>(c, start, start, n, &n2);
+    if (rc) return rc;
+    if (!(n2 > 0.0)) return phub_fail(c, PHUB_ERR_ARG, "start vector is zero");
+    s.beta[0] = std::sqrt(n2);
+    // step 0 as lanczos_plain takes it (outside the graphs: it also tunes the operator's
+    // tiles and sizes the reduction buffer), into r[2] with r[0] = x_0
+    PHUB_HIP(c, hipMemcpyAsync(r[0], start, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    double sig = 1.0 / s.beta[0];
+    double dots[2];
+    int nb = 0;
+    {
+        if (tm) tm->start();
+        rc = phub_hv_launch<T>(c, op, r[0], r[2], sig, (const T *)nullptr, 0.0, true, &nb);
+        if (rc) return rc;
+        if (tm) tm->stop();
+        rc = phub_reduce_partials(c, nb, 2, dots);
+        if (rc) return rc;
+    }
+    const double a0 = dots[0];
+    s.alpha[0] = a0;
+    double nn = 0.0;
+    rc = dev_axpby<T, T>(c, -a0 * sig, r[0], 1.0, r[2], n, &nn);
+    if (rc) return rc;
+    const double b1 = std::sqrt(std::max(nn, 0.0));
+    s.steps = 1;
+    s.beta[1] = b1;
+    if (st) {
+        PHUB_HIP(c, hipMemcpyAsync(st->slot[1], r[2], n * sizeof(T), hipMemcpyDeviceToDevice,
+                                   c->stream));
+    }
+    if (!(b1 > 1e-13 * std::max(1.0, std::fabs(a0)))) s.breakdown = true;
+    rc = phub_ensure_red(c, (size_t)std::max<int64_t>(2 * (int64_t)nb, g) + 64);
+    if (rc) return rc;
+    // alpha and beta go straight to coherent pinned host memory: a check only waits
+    if (c->h_lcz_cap < 2 * (size_t)(maxs + 2)) {
+        if (c->h_lcz) hipHostFree(c->h_lcz);
+        c->h_lcz = nullptr;
+        c->h_lcz_cap = 0;
+        PHUB_HIP(c, phub_host_malloc(&c->h_lcz, 2 * (size_t)(maxs + 2) * sizeof(double),
+                                     hipHostMallocCoherent));
+        c->h_lcz_cap = 2 * (size_t)(maxs + 2);
+    }
+    double *h_alpha = c->h_lcz, *h_beta = c->h_lcz + maxs + 2;
+    // device scalars after step 0: scale 1/beta_1, beta_1 * scale of x_0
+    const double sig1 = (b1 > 0.0) ? 1.0 / b1 : 0.0;
+    const double init[4] = {sig1, b1 * sig, 0.0, 0.0};
+    PHUB_HIP(c, hipMemcpyAsync(gb.dsc, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    h_alpha[0] = s.alpha[0];
+    h_beta[0] = s.beta[0];
+    h_beta[1] = s.beta[1];
+    const int one = 1;
+    PHUB_HIP(c, hipMemcpyAsync(gb.steps, &one, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (st) {
+        PHUB_HIP(c, hipMemcpyAsync(gb.slots, st->slot.data(), st->slot.size() * sizeof(void *),
+                                   hipMemcpyHostToDevice, c->stream));
+    }
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    if (!c->capture_stream)
+        PHUB_HIP(c, hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking));
+    // (x_k, x_{k-1}, out) of step k: rotation phase k % 3
+    const int rot[3][3] = {{0, 1, 2}, {2, 0, 1}, {1, 2, 0}};
+    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+    struct Cleanup {
+        hipGraphExec_t *e;
+        ~Cleanup() {
+            for (int i = 0; i < 3; ++i)
+                if (e[i]) hipGraphExecDestroy(e[i]);
+        }
+    } cleanup{exec};
+    hipStream_t user = c->stream;
+    for (int ph = 0; ph < 3; ++ph) {
+        T *xk = r[rot[ph][0]], *xp = r[rot[ph][1]], *out = r[rot[ph][2]];
+        c->stream = c->capture_stream;
+        c->dev_scalars = gb.dsc;
+        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        int nb2 = 0;
+        if (e == hipSuccess) {
+            rc = phub_hv_launch<T>(c, op, xk, out, 1.0, xp, 0.0, true, &nb2);
+            if (!rc) rc = phub_reduce_partials_dev(c, nb2, 2, gb.dsc + 2);
+            if (!rc) {
+                hipLaunchKernelGGL((k_axpby<T, T>), dim3(g), dim3(256), 0, c->stream, 0.0,
+                                   (const T *)xk, 1.0, out, n, c->d_red, (const double *)gb.dsc);
+                rc = phub_reduce_partials_dev(c, (int)g, 1, gb.dsc + 3);
+            }
+            if (!rc) {
+                hipLaunchKernelGGL(k_lcz_step_end, dim3(1), dim3(1), 0, c->stream, gb.dsc, h_alpha,
+                                   h_beta, gb.steps);
+                if (st)
+                    hipLaunchKernelGGL(k_store_vec<T>, dim3(g), dim3(256), 0, c->stream, out,
+                                       (T *const *)gb.slots, gb.steps, n);
+            }
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+        c->stream = user;
+        c->dev_scalars = nullptr;
+        if (e != hipSuccess) return phub_hip_fail(c, e, "hipStreamBeginCapture");
+        if (rc) {
+            if (graph) hipGraphDestroy(graph);
+            return rc;
+        }
+        if (ee != hipSuccess) return phub_hip_fail(c, ee, "hipStreamEndCapture");
+        if (nb2 != nb) {
+            hipGraphDestroy(graph);
+            return phub_fail(c, PHUB_ERR_HIP, "captured H*v changed its grid");
+        }
+        e = hipGraphInstantiate(&exec[ph], graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) return phub_hip_fail(c, e, "hipGraphInstantiate");
+    }
+    std::vector<double> ritz;
+    int launched = 1;
+    if (!fixed && s.breakdown) {
+        if (all_ritz(s.steps, s.alpha.data(), s.beta.data(), ritz))
+            return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
+        s.e0 = ritz[0];
+        s.converged = true;
+    }
+    while (!s.converged && !s.breakdown && launched < maxs) {
+        const int blk = std::min(every - launched % every, maxs - launched);
+        for (int q = 0; q < blk; ++q, ++launched)
+            PHUB_HIP(c, hipGraphLaunch(exec[launched % 3], c->stream));
+        PHUB_HIP(c, hipStreamSynchronize(c->stream));
+        std::memcpy(&s.alpha[0], h_alpha, launched * sizeof(double));
+        std::memcpy(&s.beta[1], h_beta + 1, launched * sizeof(double));
+        // the step count of the host-driven loop: up to the first breakdown
+        int k = s.steps;
+        while (k < launched) {
+            ++k;
+            if (!(s.beta[k] > 1e-13 * std::max(1.0, std::fabs(s.alpha[k - 1])))) {
+                s.breakdown = true;
+                break;
+            }
+        }
+        s.steps = k;
+        if (!fixed && (s.steps % every == 0 || s.breakdown || s.steps == maxs)) {
+            if (all_ritz(s.steps, s.alpha.data(), s.beta.data(), ritz))
+                return phub_fail(c, PHUB_ERR_HIP, "tridiagonal QL failed");
+            s.e0 = ritz[0];
+            if (s.breakdown) { s.converged = true; break; }
+            convergence(s, o->tol, maxs, ritz, o->n_states);
+            residual_rule(s, o->psi_tol, sizeof(T) == 4);
+        }
+    }
+    for (int k = s.steps + 1; k < maxs + 2; ++k) s.beta[k] = 0.0;
+    for (int k = s.steps; k < maxs + 2; ++k) s.alpha[k] = 0.0;
+    if (fixed || s.steps == 0 || !s.converged) {
+        if (all_ritz(s.steps, s.alpha.data(), s.beta.data(), ritz) == 0 && s.steps) s.e0 = ritz[0];
+    }
+    return PHUB_OK;
+}
+
 // second pass of the plain recurrence to rebuild psi = sum_k y_k x_k
 template <typename T>
 static int plain_second_pass(phub_ctx *c, const phub_op *op, const T *v0t, T *b1, T *b2, T *b3,
@@ -1098,6 +1317,18 @@ struct DevBuf {
     ~DevBuf() { phub_dfree(c, p); }
 };
 
+// The graph buffers inside the Lanczos arena block (3 max_steps + 40 doubles):
+// coefficients [0, max_steps + 8), psi pointers [max_steps + 8, 2 max_steps + 9),
+// then the scalars, the step counter and the slot table.
+static GraphBufs graph_bufs(double *coef, int max_steps) {
+    double *p = coef + 2 * max_steps + 16;
+    GraphBufs g;
+    g.dsc = p;
+    g.steps = (int *)(p + 8);
+    g.slots = (void **)(p + 10);
+    return g;
+}
+
 // plain recurrence with 3 resident vectors of type T (FP64 scalars); psi by a
 // second pass from a kept copy of the start vector
 template <typename T>
@@ -1122,8 +1353,27 @@ static int plain_mode(phub_ctx *c, const phub_op *op, const phub_lanczos_opts *o
     else
         hipLaunchKernelGGL(k_randvec<T>, dim3(g), dim3(256), 0, c->stream, st.slot[0], n, o->seed);
     PHUB_HIP(c, hipGetLastError());
-    int rc = lanczos_plain<T>(c, op, oo, st.slot[0], st.slot[1], st.slot[2], s, tm, fixed, &st);
-    if (rc) return rc;
+    int rc = PHUB_OK;
+    // graph-captured steps where the operator allows them and every vector the run
+    // may keep fits (the store cannot grow between graph launches)
+    bool graphed = false;
+    if (graph_capable(op) && !st.cap) {
+        T *r[3];
+        bool ok = true;
+        for (int i = 0; i < 3 && ok; ++i) ok = (r[i] = (T *)ar.take(n * sizeof(T))) != nullptr;
+        if (ok && st.growing)
+            while ((int)st.slot.size() < oo->max_steps + 1 && st.grow()) {}
+        if (ok && (!st.growing || (int)st.slot.size() >= oo->max_steps + 1)) {
+            rc = lanczos_plain_graph<T>(c, op, oo, st.slot[0], r, s, tm, fixed,
+                                        st.growing ? &st : nullptr, graph_bufs(coef, oo->max_steps));
+            if (rc) return rc;
+            graphed = true;
+        }
+    }
+    if (!graphed) {
+        rc = lanczos_plain<T>(c, op, oo, st.slot[0], st.slot[1], st.slot[2], s, tm, fixed, &st);
+        if (rc) return rc;
+    }
     *t_lcz = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     res->e0_ritz = res->e0 = s.e0;
     if (o->want_psi) {
@@ -1195,13 +1445,17 @@ extern "C" int phub_lanczos_workspace_bytes(const phub_op *op, const phub_lanczo
     case PHUB_LCZ_MIXED:
         b = std::max(m1 * ld * sizeof(float), (o->want_psi || o->refine_steps > 0) ? refine : 0);
         break;
+    // the plain modes: the kept vectors, plus three work vectors when the steps run as
+    // graphs (graph_capable)
     case PHUB_LCZ_PLAIN64:
-        b = o->want_psi ? std::max(m1, (size_t)4) * ld * sizeof(double) : 3 * ld * sizeof(double);
+        b = (o->want_psi ? std::max(m1, (size_t)4) : 3) * ld * sizeof(double) +
+            (graph_capable(op) ? 3 * ld * sizeof(double) + 768 : 0);
         break;
     case PHUB_LCZ_PLAIN32:
         b = o->want_psi ? std::max(std::max(m1, (size_t)4) * ld * sizeof(float),
                                    o->refine_steps > 0 ? refine : ld * sizeof(double))
                         : 3 * ld * sizeof(float);
+        b += graph_capable(op) ? 3 * ld * sizeof(float) + 768 : 0;
         break;
     default: return PHUB_ERR_ARG;
     }
@@ -1230,10 +1484,11 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
     auto t0 = std::chrono::steady_clock::now();
     EvTimer tm(c);
     LczState s;
-    // combine coefficients (max_steps + 8 doubles) and, after them, the table of
-    // basis-vector pointers of the one-pass psi (max_steps + 1): the context arena
+    // combine coefficients (max_steps + 8 doubles), the table of basis-vector pointers
+    // of the one-pass psi (max_steps + 1) and the buffers of the graph-captured plain
+    // recurrence (graph_bufs): the context arena
     struct { double *p; } coef{nullptr};
-    int rc = phub_arena(c, (size_t)(2 * oo.max_steps + 9) * sizeof(double), (void **)&coef.p);
+    int rc = phub_arena(c, (size_t)(3 * oo.max_steps + 40) * sizeof(double), (void **)&coef.p);
     if (rc) return rc;
     double t_lcz = 0.0;
     Arena ar(c, o->workspace, o->workspace_bytes);

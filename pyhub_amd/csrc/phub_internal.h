@@ -22,6 +22,16 @@ struct phub_ctx {
     size_t h_red_cap = 0;
     void *arena = nullptr;           // scratch arena (phub_arena)
     size_t arena_cap = 0;            // bytes
+    // set only while a Lanczos step is captured into a graph: the Hubbard H*v
+    // epilogue then reads its scale and beta from this device array ([0], [1])
+    const double *dev_scalars = nullptr;
+    // the stream graph captures are recorded on (lazily created; the graphs are then
+    // launched on `stream`, which may be the legacy null stream)
+    hipStream_t capture_stream = nullptr;
+    // pinned (coherent) host array the graph-captured Lanczos steps write alpha and
+    // beta into; grown on demand and kept
+    double *h_lcz = nullptr;
+    size_t h_lcz_cap = 0;            // doubles
     // hipFree waits for every stream of the device: a context that works beside a
     // long kernel on another stream sets defer_free and its device memory is kept
     // until the context is destroyed
@@ -242,6 +252,8 @@ int phub_arena(phub_ctx *ctx, size_t bytes, void **out);
 
 // reductions: sums of `ncomp` interleaved per-block partials -> host (synchronous)
 int phub_reduce_partials(phub_ctx *ctx, int nblocks, int ncomp, double *host_out);
+// the same sums into device memory, asynchronously (graph-capturable)
+int phub_reduce_partials_dev(phub_ctx *ctx, int nblocks, int ncomp, double *dev_out);
 int phub_ensure_red(phub_ctx *ctx, size_t ndoubles);
 
 // H*v launchers with a fused Lanczos epilogue:
