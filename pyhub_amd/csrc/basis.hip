@@ -588,89 +588,225 @@ int phub_build_hop_table(phub_ctx *c, int n, int k, const double *T, bool kmajor
 }
 
 // ---------------------------------------------------------------------------
-// Heisenberg tables: masks, two-half colex rank tables (built on the device)
+// Heisenberg tables: masks, bonds and the tables of the staged H*v
 // ---------------------------------------------------------------------------
-// A[xh] for the n-L high bits of x; B[m][xl] for the L low bits given m = popc(xh).
-// colex(rev_n(x)) = A[x >> L] + B[popc(x >> L)][x & (2^L - 1)].
-__global__ void k_rank_tab_A(const uint64_t *__restrict__ binom, int n, int L,
-                             uint64_t *__restrict__ A) {
-    int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int nh = n - L;
-    if (h >= (1ll << nh)) return;
-    uint64_t c = 0;
-    int i = 0;
-    for (int b = nh - 1; b >= 0; --b)
-        if ((h >> b) & 1) { ++i; c += binom_lookup(binom, n - 1 - (b + L), i); }
-    A[h] = c;
-}
-
-__global__ void k_rank_tab_B(const uint64_t *__restrict__ binom, int n, int L,
-                             uint64_t *__restrict__ B) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int nh = n - L;
-    int64_t per = 1ll << L;
-    if (t >= per * (nh + 1)) return;
-    int m = (int)(t / per);
-    int64_t xl = t % per;
-    uint64_t c = 0;
-    int i = m;
-    for (int b = L - 1; b >= 0; --b)
-        if ((xl >> b) & 1) { ++i; c += binom_lookup(binom, n - 1 - b, i); }
-    B[t] = c;
-}
-
-__global__ void k_narrow32(const uint64_t *__restrict__ a, int64_t n, uint32_t *__restrict__ b) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = (uint32_t)a[i];
-}
-
-// inverse of A per popcount: headpat[hoff[popc(xh)] + A[xh]] = xh
-__global__ void k_headpat(const uint64_t *__restrict__ A, int nh, const int64_t *__restrict__ hoff,
-                          uint32_t *__restrict__ headpat) {
-    int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= (1ll << nh)) return;
-    headpat[hoff[__popcll((uint64_t)h)] + (int64_t)A[h]] = (uint32_t)h;
-}
+static void free_heis_stage(HeisStage *s);
 
 void phub_free_heis(HeisTables *t) {
+    free_heis_stage(&t->stg);
     if (t->masks) hipFree(t->masks);
-    if (t->tabA) hipFree(t->tabA);
-    if (t->tabB) hipFree(t->tabB);
     if (t->bj) hipFree(t->bj);
     if (t->bk) hipFree(t->bk);
     if (t->bJ) hipFree(t->bJ);
     if (t->bJr) hipFree(t->bJr);
     if (t->bsign) hipFree(t->bsign);
     if (t->bond_ising) hipFree(t->bond_ising);
-    if (t->headpat) hipFree(t->headpat);
-    if (t->hoff) hipFree(t->hoff);
-    if (t->tabA32) hipFree(t->tabA32);
-    if (t->tabA1) hipFree(t->tabA1);
-    if (t->tabA2) hipFree(t->tabA2);
-    if (t->sbonds) hipFree(t->sbonds);
-    if (t->items) hipFree(t->items);
-    if (t->ehh32) hipFree(t->ehh32);
-    if (t->ehh64) hipFree(t->ehh64);
     *t = HeisTables();
+}
+
+static void free_heis_stage(HeisStage *s) {
+    hipFree(s->cnt);
+    hipFree(s->goff);
+    hipFree(s->first);
+    hipFree(s->pos);
+    hipFree(s->blk);
+    hipFree(s->blkm);
+    hipFree(s->hht);
+    hipFree(s->xt);
+    hipFree(s->cx);
+    hipFree(s->ehh32);
+    hipFree(s->ehh64);
+    hipFree(s->ll);
+    hipFree(s->hp);
+    hipFree(s->xb);
+    hipFree(s->ph);
+    *s = HeisStage();
+}
+
+template <typename V>
+static int stage_upload(phub_ctx *c, const std::vector<V> &h, V **d, double *bytes) {
+    const size_t nb = std::max<size_t>(h.size(), 1) * sizeof(V);
+    PHUB_HIP(c, phub_malloc(d, nb));
+    if (!h.empty()) PHUB_HIP(c, phub_copy(c, *d, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice));
+    *bytes += (double)nb;
+    return PHUB_OK;
+}
+
+// Tables of the staged Heisenberg H*v.  Bonds come as bit positions p > q (bit = n - site)
+// with their flip coefficient cf and half Ising weight hw.  The index of a state (xh, xl)
+// is C(n, n/2) - 1 - (A(xh) + B(m, xl)): the closed form of the basis order above
+// (compute_basis.f90:67-96) split at bit L, A summing the set bits of the high half
+// counted from the top and B continuing that count into the low half.  The block of xl
+// (all xh of popcount m = n/2 - popc(xl)) is contiguous; a state sits at position
+// j = C(nh, m) - 1 - A(xh) from the block's first index.  Per-pattern tables are
+// indexed by goff[m] + j, flip targets are 4 * (position in the source block).
+static int build_heis_stage(phub_ctx *c, int n, int64_t dim, const std::vector<int> &bp,
+                            const std::vector<int> &bq, const std::vector<double> &cf,
+                            const std::vector<double> &hw, HeisStage *s) {
+    free_heis_stage(s);
+    int nh = std::min(n - n / 2, HEIS_STG_NHMAX);
+    if (const char *e = std::getenv("PHUB_HEIS_NH")) {   // a smaller high half
+        const int v = std::atoi(e);
+        if (v >= 1 && v < nh) nh = v;
+    }
+    const int L = n - nh, k = n / 2;
+    if (L > 30) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "Heisenberg H*v needs n - 15 <= 30");
+    s->nh = nh;
+    s->L = L;
+    s->k = k;
+    const int64_t GS = (int64_t)1 << nh;
+    s->GS = GS;
+    s->cntmax = (int)phub_binom(nh, nh / 2);
+    std::vector<uint64_t> bn((size_t)(n + 1) * (n + 1), 0);
+    for (int a = 0; a <= n; ++a) {
+        bn[(size_t)a * (n + 1)] = 1;
+        for (int b = 1; b <= a; ++b)
+            bn[(size_t)a * (n + 1) + b] = bn[(size_t)(a - 1) * (n + 1) + b - 1] +
+                                          (b <= a - 1 ? bn[(size_t)(a - 1) * (n + 1) + b] : 0);
+    }
+    auto C = [&](int a, int b) -> uint64_t {
+        return (b < 0 || b > a) ? 0 : bn[(size_t)a * (n + 1) + b];
+    };
+    std::vector<int32_t> cnt(nh + 1);
+    std::vector<int64_t> goff(nh + 1);
+    int64_t acc = 0;
+    for (int m = 0; m <= nh; ++m) {
+        cnt[m] = (int32_t)C(nh, m);
+        goff[m] = acc;
+        acc += cnt[m];
+    }
+    std::vector<uint32_t> pos(GS);   // position j of a high pattern in its block
+    std::vector<int64_t> row(GS);
+    for (int64_t h = 0; h < GS; ++h) {
+        uint64_t a = 0;
+        int i = 0;
+        for (int b = nh - 1; b >= 0; --b)
+            if ((h >> b) & 1) { ++i; a += C(nh - 1 - b, i); }
+        const int m = __builtin_popcountll((uint64_t)h);
+        pos[h] = (uint32_t)(cnt[m] - 1 - (int64_t)a);
+        row[h] = goff[m] + pos[h];
+    }
+    std::vector<int> hh, ll, xx;
+    for (size_t b = 0; b < bp.size(); ++b) {
+        if (bq[b] >= L) hh.push_back((int)b);
+        else if (bp[b] < L) ll.push_back((int)b);
+        else xx.push_back((int)b);
+    }
+    const uint32_t zero = 4u * (uint32_t)s->cntmax;   // byte offset of the zero slot
+    auto anti = [](uint64_t v, int p, int q) { return ((v >> p) ^ (v >> q)) & 1u; };
+    // high-high: two targets per word, a missing flip points at the zero slot
+    const int npair = ((int)hh.size() + 1) / 2;
+    std::vector<uint32_t> hht((size_t)npair * GS, 0);
+    std::vector<HeisStgPair> hp(npair);
+    std::vector<double> ehh(GS, 0.0);
+    for (int pi = 0; pi < npair; ++pi)
+        for (int e = 0; e < 2; ++e) {
+            const int bi = 2 * pi + e;
+            const bool have = bi < (int)hh.size();
+            const int b = have ? hh[bi] : 0;
+            hp[pi].c64[e] = have ? cf[b] : 0.0;
+            hp[pi].c32[e] = (float)hp[pi].c64[e];
+            const int p = have ? bp[b] - L : 0, q = have ? bq[b] - L : 0;
+            for (int64_t h = 0; h < GS; ++h) {
+                uint32_t t = zero;
+                if (have && anti((uint64_t)h, p, q)) {
+                    t = 4u * pos[h ^ (((int64_t)1 << p) | ((int64_t)1 << q))];
+                    ehh[row[h]] += hw[b];
+                }
+                hht[(size_t)pi * GS + row[h]] |= t << (16 * e);
+            }
+        }
+    // cross: the distinct high sites, and one phase per distinct low site
+    std::vector<int> xs;
+    for (int b : xx)
+        if (std::find(xs.begin(), xs.end(), bp[b]) == xs.end()) xs.push_back(bp[b]);
+    std::sort(xs.begin(), xs.end());
+    const int ncx = (int)xs.size();
+    if (ncx > HEIS_STG_NXMAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "too many cross sites");
+    // per high site: 4 * the target's position in the source block, bit 15 set when
+    // the pattern's high bit is 0.  A target flips only where its high bit differs
+    // from its low bit tb, i.e. where bit 15 of (entry ^ (tb << 15)) is clear; the
+    // kernel takes min(entry ^ (tb << 15), zero slot), which sends the others to the
+    // zero slot (4 * position < 2^15 <= every entry with bit 15 set).
+    std::vector<uint16_t> xt((size_t)std::max(ncx, 1) * GS, 0);
+    std::vector<uint32_t> cxp(GS, 0);
+    for (int jx = 0; jx < ncx; ++jx) {
+        const int hj = xs[jx] - L;
+        for (int64_t h = 0; h < GS; ++h) {
+            const uint32_t hb = (uint32_t)(h >> hj) & 1u;
+            cxp[row[h]] |= hb << jx;
+            const uint32_t t = 4u * pos[h ^ ((int64_t)1 << hj)];
+            xt[(size_t)jx * GS + row[h]] = (uint16_t)(t | ((1u - hb) << 15));
+        }
+    }
+    std::vector<int> ks;
+    for (int b : xx)
+        if (std::find(ks.begin(), ks.end(), bq[b]) == ks.end()) ks.push_back(bq[b]);
+    std::sort(ks.begin(), ks.end());
+    std::vector<HeisStgX> xb;
+    std::vector<HeisStgPhase> ph;
+    for (int kk : ks) {
+        HeisStgPhase P{kk, (int)xb.size(), 0, 0};
+        for (int b : xx) {
+            if (bq[b] != kk) continue;
+            const int jx = (int)(std::find(xs.begin(), xs.end(), bp[b]) - xs.begin());
+            xb.push_back(HeisStgX{jx, kk, (float)cf[b], (float)hw[b], cf[b], hw[b]});
+            ++P.nb;
+        }
+        ph.push_back(P);
+    }
+    std::vector<HeisStgLL> lv;
+    for (int b : ll) lv.push_back(HeisStgLL{bp[b], bq[b], (float)cf[b], (float)hw[b], cf[b], hw[b]});
+    // blocks: every xl whose high popcount m = n/2 - popc(xl) exists, in memory order
+    const int64_t NL = (int64_t)1 << L;
+    std::vector<int64_t> first(NL, -1);
+    std::vector<std::pair<int64_t, uint32_t>> blocks;
+    for (int64_t xl = 0; xl < NL; ++xl) {
+        const int m = k - __builtin_popcountll((uint64_t)xl);
+        if (m < 0 || m > nh) continue;
+        uint64_t B = 0;
+        int i = m;
+        for (int b = L - 1; b >= 0; --b)
+            if ((xl >> b) & 1) { ++i; B += C(n - 1 - b, i); }
+        first[xl] = dim - 1 - (int64_t)B - (cnt[m] - 1);
+        blocks.push_back({first[xl], (uint32_t)xl});
+    }
+    std::sort(blocks.begin(), blocks.end());
+    std::vector<uint32_t> blk(blocks.size()), blkm;
+    for (size_t i = 0; i < blocks.size(); ++i) blk[i] = blocks[i].second;
+    blkm.reserve(blk.size());
+    for (int m = 0; m <= nh; ++m)   // popcount-major: one popcount's tables at a time
+        for (uint32_t xl : blk)
+            if (k - __builtin_popcount(xl) == m) blkm.push_back(xl);
+    std::vector<float> ehh32(ehh.begin(), ehh.end());
+    s->ncx = ncx;
+    s->nlo = ncx / 2;
+    s->nll = (int)lv.size();
+    s->npair = npair;
+    s->nx = (int)xb.size();
+    s->nphase = (int)ph.size();
+    s->nblk = (int64_t)blk.size();
+    double by = 0.0;
+    int rc = 0;
+    if ((rc = stage_upload(c, cnt, &s->cnt, &by)) || (rc = stage_upload(c, goff, &s->goff, &by)) ||
+        (rc = stage_upload(c, first, &s->first, &by)) || (rc = stage_upload(c, pos, &s->pos, &by)) || (rc = stage_upload(c, blk, &s->blk, &by)) ||
+        (rc = stage_upload(c, blkm, &s->blkm, &by)) ||
+        (rc = stage_upload(c, hht, &s->hht, &by)) || (rc = stage_upload(c, xt, &s->xt, &by)) ||
+        (rc = stage_upload(c, cxp, &s->cx, &by)) || (rc = stage_upload(c, ehh32, &s->ehh32, &by)) ||
+        (rc = stage_upload(c, ehh, &s->ehh64, &by)) || (rc = stage_upload(c, lv, &s->ll, &by)) ||
+        (rc = stage_upload(c, hp, &s->hp, &by)) || (rc = stage_upload(c, xb, &s->xb, &by)) ||
+        (rc = stage_upload(c, ph, &s->ph, &by)))
+        return rc;
+    s->bytes = by;
+    return PHUB_OK;
 }
 
 int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *out) {
     phub_free_heis(out);
-    // the block H*v packs the low pattern in 20 bits and a chunk number in 12
     if (n < 2 || n > 40) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "Heisenberg needs 2 <= n <= 40");
     int64_t dim = (int64_t)phub_binom(n, n / 2);
     out->n = n;
     out->dim = dim;
-    int L = n / 2;
-    out->L = L;
-    int nh = n - L;
-    PHUB_HIP(c, phub_malloc(&out->tabA, ((size_t)1 << nh) * sizeof(uint64_t)));
-    PHUB_HIP(c, phub_malloc(&out->tabB, ((size_t)1 << L) * (nh + 1) * sizeof(uint64_t)));
-    hipLaunchKernelGGL(k_rank_tab_A, dim3(nblk(1ll << nh, 256)), dim3(256), 0, c->stream,
-                       c->d_binom, n, L, out->tabA);
-    hipLaunchKernelGGL(k_rank_tab_B, dim3(nblk((1ll << L) * (nh + 1), 256)), dim3(256), 0,
-                       c->stream, c->d_binom, n, L, out->tabB);
-    PHUB_HIP(c, hipGetLastError());
     // the mask table is only kept when small (reduced quantities use it); the H*v
     // enumerates blocks of equal low-half pattern instead
     if (dim <= (1ll << 28)) {
@@ -678,66 +814,6 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
         hipLaunchKernelGGL(k_unrank<uint64_t>, dim3(nblk(dim, 256)), dim3(256), 0, c->stream,
                            c->d_binom, n, n / 2, dim, out->masks);
         PHUB_HIP(c, hipGetLastError());
-    }
-    {
-        std::vector<int64_t> hoff(nh + 2, 0);
-        for (int m = 0; m <= nh; ++m) hoff[m + 1] = hoff[m] + (int64_t)phub_binom(nh, m);
-        PHUB_HIP(c, phub_malloc(&out->hoff, (nh + 2) * sizeof(int64_t)));
-        PHUB_HIP(c, phub_copy(c, out->hoff, hoff.data(), (nh + 2) * sizeof(int64_t), hipMemcpyHostToDevice));
-        PHUB_HIP(c, phub_malloc(&out->headpat, ((size_t)1 << nh) * sizeof(uint32_t)));
-        hipLaunchKernelGGL(k_headpat, dim3(nblk(1ll << nh, 256)), dim3(256), 0, c->stream,
-                           out->tabA, nh, out->hoff, out->headpat);
-        PHUB_HIP(c, hipGetLastError());
-        PHUB_HIP(c, phub_malloc(&out->tabA32, ((size_t)1 << nh) * sizeof(uint32_t)));
-        hipLaunchKernelGGL(k_narrow32, dim3(nblk(1ll << nh, 256)), dim3(256), 0, c->stream,
-                           out->tabA, 1ll << nh, out->tabA32);
-        PHUB_HIP(c, hipGetLastError());
-        {   // two-half split of A for LDS (same construction as A/B one level up)
-            const int LA = nh / 2, nh2 = nh - LA;
-            out->LA = LA;
-            std::vector<uint32_t> a1((size_t)1 << nh2), a2((size_t)(nh2 + 1) << LA);
-            for (uint32_t h = 0; h < (1u << nh2); ++h) {
-                uint64_t cc = 0;
-                int i = 0;
-                for (int b = nh2 - 1; b >= 0; --b)
-                    if ((h >> b) & 1u) { ++i; cc += phub_binom(n - 1 - (b + LA + L), i); }
-                a1[h] = (uint32_t)cc;
-            }
-            for (int p = 0; p <= nh2; ++p)
-                for (uint32_t lo = 0; lo < (1u << LA); ++lo) {
-                    uint64_t cc = 0;
-                    int i = p;
-                    for (int b = LA - 1; b >= 0; --b)
-                        if ((lo >> b) & 1u) { ++i; cc += phub_binom(n - 1 - (b + L), i); }
-                    a2[((size_t)p << LA) + lo] = (uint32_t)cc;
-                }
-            PHUB_HIP(c, phub_malloc(&out->tabA1, a1.size() * sizeof(uint32_t)));
-            PHUB_HIP(c, phub_malloc(&out->tabA2, a2.size() * sizeof(uint32_t)));
-            PHUB_HIP(c, phub_copy(c, out->tabA1, a1.data(), a1.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-            PHUB_HIP(c, phub_copy(c, out->tabA2, a2.data(), a2.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-        }
-        // work items: every low pattern xl with a valid high popcount m = n/2 - popc(xl)
-        // owns the contiguous block of C(nh, m) states that ends at C-1-B[m][xl]
-        std::vector<uint64_t> Bh(((size_t)1 << L) * (nh + 1));
-        PHUB_HIP(c, phub_copy(c, Bh.data(), out->tabB, Bh.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
-        const int k = n / 2;
-        std::vector<std::pair<uint64_t, uint32_t>> blocks;   // (first index, xl)
-        for (uint32_t xl = 0; xl < (1u << L); ++xl) {
-            const int m = k - __builtin_popcount(xl);
-            if (m < 0 || m > nh) continue;
-            const uint64_t len = phub_binom(nh, m);
-            blocks.push_back({(uint64_t)dim - 1 - Bh[((size_t)m << L) + xl] - (len - 1), xl});
-        }
-        std::sort(blocks.begin(), blocks.end());
-        std::vector<uint32_t> items;
-        for (auto &bl : blocks) {
-            const int m = k - __builtin_popcount(bl.second);
-            const uint64_t nch = (phub_binom(nh, m) + HEIS_CHUNK - 1) / HEIS_CHUNK;
-            for (uint64_t ch = 0; ch < nch; ++ch) items.push_back((bl.second << 12) | (uint32_t)ch);
-        }
-        out->nitems = (int64_t)items.size();
-        PHUB_HIP(c, phub_malloc(&out->items, std::max<size_t>(items.size(), 1) * sizeof(uint32_t)));
-        PHUB_HIP(c, phub_copy(c, out->items, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
     // bonds: unordered {J<K} with either J(J,K) or J(K,J) non-zero
     std::vector<int> bj, bk;
@@ -768,51 +844,16 @@ int phub_build_heis(phub_ctx *c, int n, const double *Jm, int conv, HeisTables *
     }
     int nb = (int)bj.size();
     out->nb = nb;
-    {   // bonds grouped by type for the block H*v (the arrays above keep pair order)
-        std::vector<HeisBond> g[3];
-        for (int b = 0; b < nb; ++b) {
-            const bool jl = bj[b] < L, kl = bk[b] < L;
-            const int ty = (jl && kl) ? 0 : ((!jl && !kl) ? 1 : 2);
-            const double cj = bs[b] * bJ[b], cr = bs[b] * bJr[b], hi = 0.5 * bis[b];
-            g[ty].push_back(HeisBond{bj[b], bk[b], cj, cr, hi, (float)cj, (float)cr, (float)hi, 0});
-        }
-        std::stable_sort(g[2].begin(), g[2].end(),
-                         [](const HeisBond &p, const HeisBond &q) { return p.j > q.j; });
-        // high-high bonds by position relative to the A split at bit LA of the high half
-        const int LA = (n - L) / 2;
-        auto hcls = [&](const HeisBond &b) {
-            const bool jl = (b.j - L) < LA, kl = (b.k - L) < LA;
-            return (jl && kl) ? 0 : ((!jl && !kl) ? 1 : 2);
-        };
-        std::stable_sort(g[1].begin(), g[1].end(),
-                         [&](const HeisBond &p, const HeisBond &q) { return hcls(p) < hcls(q); });
-        out->nhh_lo = (int)std::count_if(g[1].begin(), g[1].end(), [&](const HeisBond &b) { return hcls(b) == 0; });
-        out->nhh_hi = (int)std::count_if(g[1].begin(), g[1].end(), [&](const HeisBond &b) { return hcls(b) == 1; });
-        {   // Ising weight of the high-high bonds per high pattern
-            const int nhp = n - L;
-            std::vector<double> e64((size_t)1 << nhp, 0.0);
-            for (uint32_t h = 0; h < (1u << nhp); ++h) {
-                double e = 0.0;
-                for (const HeisBond &b : g[1])
-                    if (((h >> (b.j - L)) ^ (h >> (b.k - L))) & 1u) e += b.hi;
-                e64[h] = e;
-            }
-            std::vector<float> e32(e64.begin(), e64.end());
-            PHUB_HIP(c, phub_malloc(&out->ehh64, e64.size() * sizeof(double)));
-            PHUB_HIP(c, phub_malloc(&out->ehh32, e32.size() * sizeof(float)));
-            PHUB_HIP(c, phub_copy(c, out->ehh64, e64.data(), e64.size() * sizeof(double), hipMemcpyHostToDevice));
-            PHUB_HIP(c, phub_copy(c, out->ehh32, e32.data(), e32.size() * sizeof(float), hipMemcpyHostToDevice));
-        }
-        out->ntt = (int)g[0].size();
-        out->nhh = (int)g[1].size();
-        out->nx = (int)g[2].size();
-        std::vector<HeisBond> all;
-        for (auto &v : g) all.insert(all.end(), v.begin(), v.end());
-        PHUB_HIP(c, phub_malloc(&out->sbonds, std::max<size_t>(all.size(), 1) * sizeof(HeisBond)));
-        if (!all.empty())
-            PHUB_HIP(c, phub_copy(c, out->sbonds, all.data(), all.size() * sizeof(HeisBond), hipMemcpyHostToDevice));
-    }
     out->diag_const = dconst;
+    {
+        std::vector<double> cf(nb), hw(nb);
+        for (int b = 0; b < nb; ++b) {
+            cf[b] = bs[b] * bJ[b];
+            hw[b] = 0.5 * bis[b];
+        }
+        int rc = build_heis_stage(c, n, dim, bj, bk, cf, hw, &out->stg);
+        if (rc) return rc;
+    }
     size_t nba = std::max(nb, 1);
     PHUB_HIP(c, phub_malloc(&out->bj, nba * sizeof(int)));
     PHUB_HIP(c, phub_malloc(&out->bk, nba * sizeof(int)));

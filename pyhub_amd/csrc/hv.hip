@@ -11,8 +11,9 @@
 //   they are scalar table loads and fully coalesced vector loads of other rows.
 //   A workgroup owns R rows x 256 columns: each lane reads its up-hop list once
 //   and applies it to R rows.
-// Heisenberg (reference HPROD, pyhub/libs/src/hprod_heis.f90:1-61): blocks of
-//   states sharing their low-half pattern, no mask table (k_hv_heis_blk below).
+// Heisenberg (reference HPROD, pyhub/libs/src/hprod_heis.f90:1-61): one workgroup
+//   per block of states sharing their low-half pattern, the block and one
+//   neighbouring block at a time staged in LDS, no mask table (k_hv_heis_stg below).
 //
 // Fused Lanczos epilogue (optional): out = scale*(H x) - beta*prev and per-block
 // partial sums <scale*x, out>, <out, out> (FP64) written to the context's
@@ -423,208 +424,350 @@ __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict
     if (EPI) block_sum2<T>(s_xo, s_oo, part);
 }
 
-// Heisenberg H*v without a mask table (replaces HPROD, hprod_heis.f90:1-61).
-// A state x splits into its high bits xh = x >> L and low bits xl; its reference
-// index is C-1-B[m][xl]-A[xh] with m = popc(xh).  For a fixed xl the C(n-L, m)
-// states with m = n/2 - popc(xl) are contiguous (inner index A[xh] counted down
-// from the block's last index), and headpat lists their xh in that order.  One
-// workgroup takes HEIS_CHUNK consecutive inner indices of one block:
-//   low-low bonds  flip xl only: the source is the same inner index of another
-//                  block, a coalesced read, and whether the flip exists is
-//                  uniform over the workgroup;
-//   high-high bonds flip xh only: a gather inside the block through A;
-//   cross bonds    flip one bit of each half: the low bit is uniform, so the
-//                  source block and the coefficient are uniform, the inner
-//                  index is A[xh ^ bit].
-// Matrix elements as the mask kernel: pull coefficient bsign*J(K,J) when the
-// target has site J up, bsign*J(J,K) otherwise; diagonal dconst minus half the
-// Ising weight of every antiparallel bond.
-struct HeisBlkArgs {
-    int L, nh, k;
-    int ntt, nhh, nx, nhh_lo, nhh_hi;
-    int LA, na1, na2;    // split A tables: sizes of A1 and A2
-    int order;           // 0: each XCD takes one contiguous eighth of the items; 1: round robin
-    int64_t nitems;
-    uint64_t total_m1;   // C(n, n/2) - 1
-    const uint64_t *tabB;
-    const uint32_t *tabA1, *tabA2, *headpat, *items;
-    const int64_t *hoff;
-    const HeisBond *bonds;
+// Staged Heisenberg H*v (replaces HPROD, hprod_heis.f90:1-61).  A state is (xh, xl),
+// xh the nh top bits and xl the L low bits.  The states of one xl form a contiguous
+// block of C(nh, m) states (m = n/2 - popc(xl)), state xh at position j(xh) from the
+// block's first index.  One workgroup owns one block and keeps it in LDS (S0):
+//   high-high flips  read S0 through a per-pattern table of targets, two 16-bit
+//                    targets per word (a bond that does not flip points at a zero
+//                    slot after the block);
+//   low-low flips    read the same position in another block of the same popcount:
+//                    a coalesced global read whose existence is uniform;
+//   cross flips      one phase per low site of the cross bonds: the neighbouring
+//                    block (popcount m +- 1) is staged in S1 and read through a
+//                    per-pattern table of targets for the target's low bit (the zero
+//                    slot where the flip does not exist).
+// Every global read is coalesced, every scattered read is an LDS read, and every
+// address is a wave-uniform base plus a 32-bit byte offset.  Targets are 4 * position:
+// LDS byte offsets in FP32, doubled in FP64.  The diagonal is dconst minus half the
+// Ising weight of each antiparallel bond: uniform for the low-low bonds, a per-pattern
+// table for the high-high bonds, and two LDS tables indexed by the cross-site bits of
+// xh for the cross bonds (built per block from the low bits of xl).  Matrix elements
+// as HPROD's: the pull coefficient of a flip is bsign * J(J,K) (J symmetric), bsign
+// the bond's sign convention factor.
+struct HeisStgArgs {
+    int nh, k, cntmax, ncx, nlo, nll, npair, nphase, nx;
+    int order;          // workgroup -> block map: 0 each XCD takes one contiguous eighth of
+                        // the list, 1 round robin, 2 / 3 runs of 16 / 64 consecutive blocks
+                        // per XCD, the XCDs taking runs in turn
+    int64_t nblk, GS;
+    const int32_t *cnt;
+    const int64_t *goff, *first;
+    const uint32_t *blk, *hht, *cx;
+    const uint16_t *xt;
     const float *ehh32;
     const double *ehh64;
+    const HeisStgLL *ll;
+    const HeisStgPair *hp;
+    const HeisStgX *xb;
+    const HeisStgPhase *ph;
     double dconst;
 };
 
+template <typename T, typename B>
+__device__ __forceinline__ T stg_c(const B &b) {
+    if constexpr (sizeof(T) == 4) return b.c32;
+    else return b.c64;
+}
+template <typename T, typename B>
+__device__ __forceinline__ T stg_h(const B &b) {
+    if constexpr (sizeof(T) == 4) return b.h32;
+    else return b.h64;
+}
+// element at 4-byte-unit offset o4 (4 * position) of an array of T
 template <typename T>
-__device__ __forceinline__ T bond_c(const HeisBond &b) {
-    // J is symmetric (checked when the operator is built), so cj == cr
-    if constexpr (sizeof(T) == 4) return b.fj;
-    else return b.cj;
+__device__ __forceinline__ const T &stg_at(const T *base, uint32_t o4) {
+    if constexpr (sizeof(T) == 4) return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + o4);
+    else return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (o4 << 1));
 }
 template <typename T>
-__device__ __forceinline__ T bond_h(const HeisBond &b) {
-    if constexpr (sizeof(T) == 4) return b.fh;
-    else return b.hi;
+__device__ __forceinline__ T &stg_at(T *base, uint32_t o4) {
+    if constexpr (sizeof(T) == 4) return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + o4);
+    else return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (o4 << 1));
+}
+// a value every lane holds alike (read from one LDS address), moved to scalar registers
+__device__ __forceinline__ int64_t stg_uniform(int64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ uint32_t stg_word(const uint32_t *base, uint32_t o4) {
+    return *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(base) + o4);
+}
+__device__ __forceinline__ uint32_t stg_half(const uint16_t *base, uint32_t o4) {
+    return *reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(base) + (o4 >> 1));
 }
 
-// Every load below is issued for all lanes (inactive lanes read an in-range
-// element and contribute zero), so gathers of different bonds overlap.  A of a
-// high pattern h = (hi, lo) split at bit LA is A1[hi] + A2[popc(hi)][lo]: a flip
-// inside lo keeps A1[hi] and the A2 row, a flip inside hi keeps A2[popc(hi)][lo],
-// so only straddling flips need both lookups.
-template <typename T, bool EPI>
-__global__ __launch_bounds__(256) void k_hv_heis_blk(HeisBlkArgs a, const T *__restrict__ x,
-                                                     T *__restrict__ out, double scale,
-                                                     const T *__restrict__ prev, double beta,
-                                                     double *__restrict__ part) {
-    constexpr int R = HEIS_CHUNK / 256;
-    extern __shared__ uint32_t sA[];   // A1 then A2
-    uint32_t *sA1 = sA, *sA2 = sA + a.na1;
-    for (int i = threadIdx.x; i < a.na1; i += 256) sA1[i] = a.tabA1[i];
-    for (int i = threadIdx.x; i < a.na2; i += 256) sA2[i] = a.tabA2[i];
-    const int64_t per = (a.nitems + 7) / 8;
-    const int64_t it = a.order ? (int64_t)blockIdx.x
-                               : (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    double s_xo = 0.0, s_oo = 0.0;
-    __syncthreads();
-    if (it < a.nitems) {
-        const uint32_t item = a.items[it];
-        const uint32_t xl = item >> 12;
-        const int ch = (int)(item & 4095u);
-        const int m = a.k - __popc(xl);
-        const int64_t h0 = a.hoff[m];
-        const int len = (int)(a.hoff[m + 1] - h0);
-        const int L = a.L, LA = a.LA;
-        const uint32_t lomask = (1u << LA) - 1u;
-        const int64_t last = (int64_t)(a.total_m1 - a.tabB[((uint64_t)m << L) + xl]);
-        const T *__restrict__ xb = x + last;   // inner index i lives at xb[-i]
-        int ai[R];
-        bool ok[R];
-        uint32_t xh[R], lo[R], hi[R], row[R];
-        int A1h[R], A2l[R];
-        T dg[R], acc[R];
+// dynamic LDS of k_hv_heis_stg: two block buffers of C(15, 7) + 1 elements, the cross
+// Ising tables and the per-block bond lists (at most 160 KB less the static LDS)
+constexpr int HEIS_STG_SLOT = HEIS_STG_CNTMAX + 1;
+constexpr size_t HEIS_STG_LDS_MAX = 160 * 1024 - 512;
+
+// Copies the n <= C(15, 7) elements of a block into LDS: every load is issued before
+// the first store (a loop with a run-time bound would wait on each load in turn).
+template <typename T, int NT>
+__device__ __forceinline__ void stg_stage(T *dst, const T *__restrict__ src, uint32_t n) {
+    constexpr int K = (HEIS_STG_CNTMAX + NT - 1) / NT;
+    const uint32_t tid = threadIdx.x;
+    T v[K];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int i = ch * HEIS_CHUNK + r * 256 + (int)threadIdx.x;
-            ok[r] = i < len;
-            ai[r] = ok[r] ? i : 0;   // inactive lanes read element 0 of the block
-            xh[r] = ok[r] ? a.headpat[h0 + i] : 0u;
-            hi[r] = xh[r] >> LA;
-            lo[r] = xh[r] & lomask;
-            row[r] = (uint32_t)__popc(hi[r]) << LA;
-            A1h[r] = (int)sA1[hi[r]];
-            A2l[r] = (int)sA2[row[r] + lo[r]];
-            if constexpr (sizeof(T) == 4) dg[r] = a.ehh32[xh[r]];
-            else dg[r] = a.ehh64[xh[r]];
-            acc[r] = (T)0;
+    for (int q = 0; q < K; ++q) {
+        const uint32_t j = (uint32_t)q * NT + tid;
+        if ((uint32_t)q * NT < n) v[q] = src[j < n ? j : n - 1u];
+    }
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const uint32_t j = (uint32_t)q * NT + tid;
+        if ((uint32_t)q * NT < n && j < n) dst[j] = v[q];
+    }
+}
+
+// The per-state work of one block, SPL states per lane (lanes past the block work on
+// position cnt-1 and store nothing): no branch depends on the state index, so the
+// accumulators stay in registers.
+template <typename T, int NT, int SPL, bool EPI>
+__device__ __forceinline__ void heis_stg_body(const HeisStgArgs &a, const T *__restrict__ x,
+                                              T *__restrict__ out, double scale,
+                                              const T *__restrict__ prev, double beta,
+                                              uint32_t xl, int m, uint32_t cnt, int64_t go,
+                                              int64_t first, T dgu, T *S0, T *S1,
+                                              const T *Tlo, const T *Thi, const int64_t *llf,
+                                              const T *llc, int nact, const int64_t *phf,
+                                              double &s_xo, double &s_oo) {
+    const uint32_t tid = threadIdx.x;
+    uint32_t o4[SPL];
+#pragma unroll
+    for (int q = 0; q < SPL; ++q) {
+        const uint32_t j = (uint32_t)q * NT + tid;
+        o4[q] = 4u * (j < cnt ? j : cnt - 1u);
+    }
+    const uint32_t mlo = (1u << a.nlo) - 1u;
+    T acc[SPL];
+    {
+        const uint32_t *cxb = a.cx + go;
+        const float *e32 = a.ehh32 + go;
+        const double *e64 = a.ehh64 + go;
+#pragma unroll
+        for (int q = 0; q < SPL; ++q) {
+            const uint32_t cv = stg_word(cxb, o4[q]);
+            T eh;
+            if constexpr (sizeof(T) == 4) eh = stg_at<float>(e32, o4[q]);
+            else eh = stg_at<double>(e64, o4[q]);
+            acc[q] = (dgu - eh - Tlo[cv & mlo] - Thi[cv >> a.nlo]) * stg_at<T>(S0, o4[q]);
         }
-        T dgu = (T)a.dconst;
-        // low-low bonds: existence and source block uniform
-        for (int b = 0; b < a.ntt; ++b) {
-            const HeisBond bd = a.bonds[b];
-            const uint32_t tj = (xl >> bd.j) & 1u, tk = (xl >> bd.k) & 1u;
-            if (tj != tk) {
-                dgu -= bond_h<T>(bd);
-                const uint32_t xl2 = xl ^ ((1u << bd.j) | (1u << bd.k));
-                const int64_t last2 = (int64_t)(a.total_m1 - a.tabB[((uint64_t)m << L) + xl2]);
-                const T cf = bond_c<T>(bd);
-                const T *__restrict__ xs = x + last2;
+    }
+    // high-high bonds, two words (four bonds) per pass so that their loads overlap
+    for (int p = 0; p < a.npair; p += 2) {
+        const int p1 = p + 1 < a.npair ? p + 1 : p;   // an odd last pass repeats p with zeros
+        const HeisStgPair h0 = a.hp[p], h1 = a.hp[p1];
+        T c0, c1, c2, c3;
+        if constexpr (sizeof(T) == 4) { c0 = h0.c32[0]; c1 = h0.c32[1]; c2 = h1.c32[0]; c3 = h1.c32[1]; }
+        else { c0 = h0.c64[0]; c1 = h0.c64[1]; c2 = h1.c64[0]; c3 = h1.c64[1]; }
+        if (p1 == p) { c2 = (T)0; c3 = (T)0; }
+        const uint32_t *tb0 = a.hht + (int64_t)p * a.GS + go;
+        const uint32_t *tb1 = a.hht + (int64_t)p1 * a.GS + go;
+        uint32_t w0[SPL], w1[SPL];
 #pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] += cf * xs[-ai[r]];
-            }
-        }
-        // high-high bonds (their Ising part is in dg already), by position in the A split
-        const int b0 = a.ntt, b1 = b0 + a.nhh_lo, b2 = b1 + a.nhh_hi, b3 = b0 + a.nhh;
-        for (int b = b0; b < b1; ++b) {   // both bits below LA
-            const HeisBond bd = a.bonds[b];
-            const int hj = bd.j - L, hk = bd.k - L;
-            const uint32_t f = (1u << hj) | (1u << hk);
-            const T c = bond_c<T>(bd);
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const bool on = ok[r] && (((xh[r] >> hj) ^ (xh[r] >> hk)) & 1u);
-                const int a2 = on ? A1h[r] + (int)sA2[row[r] + (lo[r] ^ f)] : ai[r];
-                acc[r] += (on ? c : (T)0) * xb[-a2];
-            }
-        }
-        for (int b = b1; b < b2; ++b) {   // both bits at or above LA
-            const HeisBond bd = a.bonds[b];
-            const int hj = bd.j - L, hk = bd.k - L;
-            const uint32_t f = ((1u << hj) | (1u << hk)) >> LA;
-            const T c = bond_c<T>(bd);
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const bool on = ok[r] && (((xh[r] >> hj) ^ (xh[r] >> hk)) & 1u);
-                const int a2 = on ? (int)sA1[hi[r] ^ f] + A2l[r] : ai[r];
-                acc[r] += (on ? c : (T)0) * xb[-a2];
-            }
-        }
-        for (int b = b2; b < b3; ++b) {   // straddling the split
-            const HeisBond bd = a.bonds[b];
-            const int hj = bd.j - L, hk = bd.k - L;
-            const uint32_t f = (1u << hj) | (1u << hk);
-            const T c = bond_c<T>(bd);
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const bool on = ok[r] && (((xh[r] >> hj) ^ (xh[r] >> hk)) & 1u);
-                const uint32_t h2 = xh[r] ^ f, hh2 = h2 >> LA;
-                const int av = (int)(sA1[hh2] + sA2[((uint32_t)__popc(hh2) << LA) + (h2 & lomask)]);
-                const int a2 = on ? av : ai[r];
-                acc[r] += (on ? c : (T)0) * xb[-a2];
-            }
-        }
-        // cross bonds, grouped by their high site: one A lookup per group
-        const int bx0 = a.ntt + a.nhh, bx1 = bx0 + a.nx;
-        for (int b = bx0; b < bx1;) {
-            const int j = a.bonds[b].j;
-            const int hj = j - L;
-            int af[R];
-            uint32_t hb[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                hb[r] = (xh[r] >> hj) & 1u;
-                const uint32_t h2 = xh[r] ^ (1u << hj), hh2 = h2 >> LA;
-                af[r] = (int)(sA1[hh2] + sA2[((uint32_t)__popc(hh2) << LA) + (h2 & lomask)]);
-            }
-            T w0 = (T)0, w1 = (T)0;   // Ising weights of the group's bonds whose low bit is 0 / 1
-            for (; b < bx1 && a.bonds[b].j == j; ++b) {
-                const HeisBond bd = a.bonds[b];
-                const uint32_t tb = (xl >> bd.k) & 1u;   // target's low bit; its high bit must differ
-                if (tb) w1 += bond_h<T>(bd);
-                else w0 += bond_h<T>(bd);
-                const int m2 = tb ? m + 1 : m - 1;        // popcount of the source's high half
-                if (m2 < 0 || m2 > a.nh) continue;        // no lane of this block can flip
-                const int64_t last2 = (int64_t)(a.total_m1 - a.tabB[((uint64_t)m2 << L) + (xl ^ (1u << bd.k))]);
-                const T cf = bond_c<T>(bd);
-                const T *__restrict__ xs = x + last2;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const bool on = ok[r] && (hb[r] != tb);
-                    const int a2 = on ? af[r] : 0;
-                    acc[r] += (on ? cf : (T)0) * xs[-a2];
-                }
-            }
-            // antiparallel: high bit 1 with low bit 0, or high bit 0 with low bit 1
-#pragma unroll
-            for (int r = 0; r < R; ++r) dg[r] += hb[r] ? w0 : w1;
+        for (int q = 0; q < SPL; ++q) {
+            w0[q] = stg_word(tb0, o4[q]);
+            w1[q] = stg_word(tb1, o4[q]);
         }
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (!ok[r]) continue;
-            const int64_t s = last - ai[r];
-            const T xv = x[s];
-            acc[r] += (dgu - dg[r]) * xv;
+        for (int q = 0; q < SPL; ++q)
+            acc[q] += c0 * stg_at<T>(S0, w0[q] & 0xffffu) + c1 * stg_at<T>(S0, w0[q] >> 16) +
+                      c2 * stg_at<T>(S0, w1[q] & 0xffffu) + c3 * stg_at<T>(S0, w1[q] >> 16);
+    }
+    // the antiparallel low-low bonds (compacted at the block start), two at a time
+    {
+        int b = 0;
+        for (; b + 3 < nact; b += 4) {
+            const T c0 = llc[b], c1 = llc[b + 1], c2 = llc[b + 2], c3 = llc[b + 3];
+            const T *__restrict__ xs0 = x + stg_uniform(llf[b]);
+            const T *__restrict__ xs1 = x + stg_uniform(llf[b + 1]);
+            const T *__restrict__ xs2 = x + stg_uniform(llf[b + 2]);
+            const T *__restrict__ xs3 = x + stg_uniform(llf[b + 3]);
+            T v0[SPL], v1[SPL], v2[SPL], v3[SPL];
+#pragma unroll
+            for (int q = 0; q < SPL; ++q) {
+                v0[q] = stg_at<T>(xs0, o4[q]);
+                v1[q] = stg_at<T>(xs1, o4[q]);
+                v2[q] = stg_at<T>(xs2, o4[q]);
+                v3[q] = stg_at<T>(xs3, o4[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < SPL; ++q) acc[q] += (c0 * v0[q] + c1 * v1[q]) + (c2 * v2[q] + c3 * v3[q]);
+        }
+        for (; b < nact; ++b) {
+            const T c = llc[b];
+            const T *__restrict__ xs = x + stg_uniform(llf[b]);
+#pragma unroll
+            for (int q = 0; q < SPL; ++q) acc[q] += c * stg_at<T>(xs, o4[q]);
+        }
+    }
+    // cross phases: one neighbouring block (high popcount m +- 1) staged in S1 each
+    const uint32_t zero = 4u * (uint32_t)a.cntmax;
+    for (int ph = 0; ph < a.nphase; ++ph) {
+        const int64_t f2 = stg_uniform(phf[ph]);
+        if (f2 < 0) continue;                // no source block: no lane flips
+        const HeisStgPhase P = a.ph[ph];
+        const uint32_t tb = (xl >> P.kk) & 1u;
+        const uint32_t flip = tb << 15;
+        __syncthreads();                     // the previous phase's reads of S1 are done
+        stg_stage<T, NT>(S1, x + f2, (uint32_t)a.cnt[tb ? m + 1 : m - 1]);
+        __syncthreads();
+        for (int b = P.b0; b < P.b0 + P.nb; ++b) {
+            const HeisStgX xb = a.xb[b];
+            const T c = stg_c<T>(xb);
+            const uint16_t *tt = a.xt + (int64_t)xb.jx * a.GS + go;
+#pragma unroll
+            for (int q = 0; q < SPL; ++q)
+                acc[q] += c * stg_at<T>(S1, min(stg_half(tt, o4[q]) ^ flip, zero));
+        }
+    }
+    T *__restrict__ ob = out + first;
+    const T *__restrict__ pb = prev ? prev + first : nullptr;
+#pragma unroll
+    for (int q = 0; q < SPL; ++q) {
+        const uint32_t j = (uint32_t)q * NT + tid;
+        if (j < cnt) {
             if (EPI) {
-                double val = scale * (double)acc[r];
-                if (prev) val -= beta * (double)prev[s];
+                double val = scale * (double)acc[q];
+                if (pb) val -= beta * (double)stg_at<T>(pb, o4[q]);
                 const T vt = (T)val;
-                out[s] = vt;
-                s_xo += scale * (double)xv * (double)vt;
+                stg_at<T>(ob, o4[q]) = vt;
+                s_xo += scale * (double)stg_at<T>(S0, o4[q]) * (double)vt;
                 s_oo += (double)vt * (double)vt;
             } else {
-                out[s] = acc[r];
+                stg_at<T>(ob, o4[q]) = acc[q];
             }
         }
+    }
+}
+
+// states per lane of the five bodies: ceil(C(15, m) / NT) for m = 3..7, so that the
+// blocks of a 15-site high half run without idle slots
+template <int NT, int I>
+constexpr int heis_stg_spl() {
+    constexpr int c[5] = {455, 1365, 3003, 5005, HEIS_STG_CNTMAX};
+    return (c[I] + NT - 1) / NT;
+}
+
+// FP32 runs two 512-thread workgroups per CU (four waves per SIMD: at most 128 VGPRs);
+// FP64 one (its LDS holds one)
+template <typename T, int NT, bool EPI>
+__global__ __launch_bounds__(NT, sizeof(T) == 4 ? 4 : 2) void k_hv_heis_stg(HeisStgArgs a, const T *__restrict__ x,
+                                                    T *__restrict__ out, double scale,
+                                                    const T *__restrict__ prev, double beta,
+                                                    double *__restrict__ part) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char stg_smem[];
+    T *S0 = reinterpret_cast<T *>(stg_smem);
+    T *S1 = S0 + HEIS_STG_SLOT;
+    T *Wt = S1 + HEIS_STG_SLOT;           // Wt[hb][jx]: cross Ising weight for high bit hb
+    T *Tlo = Wt + 2 * HEIS_STG_NXMAX;
+    T *Thi = Tlo + (1 << a.nlo);
+    // after the T arrays (rounded to 8 bytes): the compacted low-low sources and the
+    // source block of every cross phase, then the low-low coefficients
+    const size_t tbytes = ((2 * (size_t)HEIS_STG_SLOT + 2 * HEIS_STG_NXMAX + (1u << a.nlo) +
+                            (1u << (a.ncx - a.nlo))) * sizeof(T) + 7) & ~(size_t)7;
+    int64_t *llf = reinterpret_cast<int64_t *>(stg_smem + tbytes);
+    int64_t *phf = llf + a.nll;
+    T *llc = reinterpret_cast<T *>(phf + a.nphase);
+    __shared__ int nact_sh;
+    __shared__ T dgu_sh;
+    const uint32_t tid = threadIdx.x;
+    const int64_t bid = blockIdx.x, xcd = bid & 7, k = bid >> 3;
+    int64_t it;
+    if (a.order == 0) {
+        const int64_t per = (a.nblk + 7) / 8;
+        it = k < per ? xcd * per + k : a.nblk;
+    }
+    else if (a.order == 1) it = bid;
+    else {
+        const int64_t g = a.order == 2 ? 16 : 64;
+        it = ((k / g) * 8 + xcd) * g + k % g;
+    }
+    double s_xo = 0.0, s_oo = 0.0;
+    if (it < a.nblk) {
+        const uint32_t xl = a.blk[it];
+        const int m = a.k - __popc(xl);
+        const uint32_t cnt = (uint32_t)a.cnt[m];
+        const int64_t go = a.goff[m];
+        const int64_t first = a.first[xl];
+        const int nhi = a.ncx - a.nlo;
+        stg_stage<T, NT>(S0, x + first, cnt);
+        if (tid == 0) {
+            S0[a.cntmax] = (T)0;
+            S1[a.cntmax] = (T)0;
+        }
+        if ((int)tid < a.ncx) {
+            T w0 = (T)0, w1 = (T)0;
+            for (int b = 0; b < a.nx; ++b) {
+                const HeisStgX xb = a.xb[b];
+                if (xb.jx != (int)tid) continue;
+                if ((xl >> xb.kk) & 1u) w0 += stg_h<T>(xb);   // high bit 0 against low bit 1
+                else w1 += stg_h<T>(xb);
+            }
+            Wt[tid] = w0;
+            Wt[HEIS_STG_NXMAX + tid] = w1;
+        }
+        if (tid < 64) {
+            // wave 0 compacts the antiparallel low-low bonds (their source blocks are
+            // looked up here in parallel) and sums their Ising weight
+            T dsum = (T)0;
+            int base = 0;
+            for (int b0 = 0; b0 < a.nll; b0 += 64) {
+                const int b = b0 + (int)tid;
+                bool f = false;
+                HeisStgLL l{};
+                if (b < a.nll) {
+                    l = a.ll[b];
+                    f = ((xl >> l.p) ^ (xl >> l.q)) & 1u;
+                }
+                const uint64_t bal = __ballot(f);
+                if (f) {
+                    const int slot = base + __popcll(bal & ((1ull << tid) - 1ull));
+                    llf[slot] = a.first[xl ^ ((1u << l.p) | (1u << l.q))];
+                    llc[slot] = stg_c<T>(l);
+                    dsum += stg_h<T>(l);
+                }
+                base += __popcll(bal);
+            }
+            for (int off = 32; off > 0; off >>= 1) dsum += __shfl_xor(dsum, off);
+            if (tid == 0) {
+                nact_sh = base;
+                dgu_sh = (T)a.dconst - dsum;
+            }
+        }
+        for (int p = (int)tid; p < a.nphase; p += NT) {
+            const HeisStgPhase P = a.ph[p];
+            const int m2 = ((xl >> P.kk) & 1u) ? m + 1 : m - 1;   // the source's high popcount
+            phf[p] = (m2 < 0 || m2 > a.nh) ? (int64_t)-1 : a.first[xl ^ (1u << P.kk)];
+        }
+        __syncthreads();
+        const T dgu = dgu_sh;
+        const int nact = nact_sh;
+        for (uint32_t sx = tid; sx < (1u << a.nlo); sx += NT) {
+            T v = (T)0;
+            for (int j = 0; j < a.nlo; ++j) v += Wt[((sx >> j) & 1u) * HEIS_STG_NXMAX + j];
+            Tlo[sx] = v;
+        }
+        for (uint32_t sx = tid; sx < (1u << nhi); sx += NT) {
+            T v = (T)0;
+            for (int j = 0; j < nhi; ++j) v += Wt[((sx >> j) & 1u) * HEIS_STG_NXMAX + a.nlo + j];
+            Thi[sx] = v;
+        }
+        __syncthreads();
+        const uint32_t qn = (cnt + NT - 1) / NT;
+#define PHUB_STG_BODY(I)                                                                      \
+    heis_stg_body<T, NT, heis_stg_spl<NT, I>(), EPI>(a, x, out, scale, prev, beta, xl, m, cnt, \
+                                                     go, first, dgu, S0, S1, Tlo, Thi, llf,   \
+                                                     llc, nact, phf, s_xo, s_oo)
+        if (qn <= (uint32_t)heis_stg_spl<NT, 0>()) PHUB_STG_BODY(0);
+        else if (qn <= (uint32_t)heis_stg_spl<NT, 1>()) PHUB_STG_BODY(1);
+        else if (qn <= (uint32_t)heis_stg_spl<NT, 2>()) PHUB_STG_BODY(2);
+        else if (qn <= (uint32_t)heis_stg_spl<NT, 3>()) PHUB_STG_BODY(3);
+        else PHUB_STG_BODY(4);
+#undef PHUB_STG_BODY
     }
     if (EPI) block_sum2<T>(s_xo, s_oo, part);
 }
@@ -958,35 +1101,6 @@ static HubArgs<T> hub_args(const phub_op *op, const phub_ctx *c = nullptr) {
     return a;
 }
 
-static HeisBlkArgs heis_blk_args(const phub_op *op) {
-    const HeisTables &h = op->h;
-    HeisBlkArgs a{};
-    a.L = h.L;
-    a.nh = h.n - h.L;
-    a.k = h.n / 2;
-    a.ntt = h.ntt;
-    a.nhh = h.nhh;
-    a.nx = h.nx;
-    a.LA = h.LA;
-    a.nhh_lo = h.nhh_lo;
-    a.nhh_hi = h.nhh_hi;
-    a.ehh32 = h.ehh32;
-    a.ehh64 = h.ehh64;
-    a.na1 = 1 << (a.nh - h.LA);
-    a.na2 = (a.nh - h.LA + 1) << h.LA;
-    a.nitems = h.nitems;
-    a.total_m1 = (uint64_t)h.dim - 1;
-    a.tabB = h.tabB;
-    a.tabA1 = h.tabA1;
-    a.tabA2 = h.tabA2;
-    a.headpat = h.headpat;
-    a.items = h.items;
-    a.hoff = h.hoff;
-    a.bonds = h.sbonds;
-    a.dconst = h.diag_const;
-    return a;
-}
-
 constexpr int HUB_R = 4;
 
 // Candidate tiles of the down-hop kernel: (columns per row segment, rows per
@@ -1175,6 +1289,122 @@ static int hv_two_kernel(phub_ctx *c, const phub_op *op, const T *x, T *out, dou
 
 // PHUB_UP_ROWS=1 keeps FP64 on the one-row up kernel even where two rows fit
 // (the path of the large sectors, exercised on small ones by the tests)
+static HeisStgArgs heis_stg_args(const phub_op *op) {
+    const HeisStage &s = op->h.stg;
+    HeisStgArgs a{};
+    a.nh = s.nh;
+    a.k = s.k;
+    a.cntmax = s.cntmax;
+    a.ncx = s.ncx;
+    a.nlo = s.nlo;
+    a.nll = s.nll;
+    a.npair = s.npair;
+    a.nphase = s.nphase;
+    a.nx = s.nx;
+    a.nblk = s.nblk;
+    a.GS = s.GS;
+    a.cnt = s.cnt;
+    a.goff = s.goff;
+    a.first = s.first;
+    a.blk = s.blk;
+    a.hht = s.hht;
+    a.xt = s.xt;
+    a.cx = s.cx;
+    a.ehh32 = s.ehh32;
+    a.ehh64 = s.ehh64;
+    a.ll = s.ll;
+    a.hp = s.hp;
+    a.xb = s.xb;
+    a.ph = s.ph;
+    a.dconst = op->h.diag_const;
+    return a;
+}
+
+// FP32: 256 threads, three workgroups per CU in LDS; FP64: 512 threads, one per CU
+template <typename T>
+constexpr int heis_stg_threads() { return 512; }
+
+template <typename T, bool EPI>
+static int heis_stg_go(phub_ctx *c, const HeisStgArgs &a, int nblocks, size_t lds, const T *x,
+                       T *out, double scale, const T *prev, double beta) {
+    constexpr int NT = heis_stg_threads<T>();
+    static bool attr_set[PHUB_MAX_DEVICES];   // set once per device and instantiation
+    if (c->device < 0 || c->device >= PHUB_MAX_DEVICES)
+        return phub_fail(c, PHUB_ERR_UNSUPPORTED, "device index beyond PHUB_MAX_DEVICES");
+    if (!attr_set[c->device]) {
+        PHUB_HIP(c, hipFuncSetAttribute((const void *)k_hv_heis_stg<T, NT, EPI>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)HEIS_STG_LDS_MAX));
+        attr_set[c->device] = true;
+    }
+    hipLaunchKernelGGL((k_hv_heis_stg<T, NT, EPI>), dim3(nblocks), dim3(NT), lds, c->stream, a, x,
+                       out, scale, prev, beta, c->d_red);
+    PHUB_HIP(c, hipGetLastError());
+    return PHUB_OK;
+}
+
+template <typename T>
+static int hv_heis_stage(phub_ctx *c, const phub_op *op, const T *x, T *out, double scale,
+                         const T *prev, double beta, bool epi, int *nblocks_out) {
+    const HeisStage &s = op->h.stg;
+    // the grid covers whole runs of 64 blocks on each of the 8 XCDs (every map below
+    // sends workgroup ids past the list to no block)
+    const int64_t nb = 512 * ((s.nblk + 511) / 512);
+    if (nb > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
+    const int nblocks = (int)nb;
+    if (epi) {
+        int rc = phub_ensure_red(c, (size_t)nblocks * 2 + 64);
+        if (rc) return rc;
+    }
+    HeisStgArgs a = heis_stg_args(op);
+    // LDS: the own block, one source block, the cross Ising tables, then the low-low
+    // sources and coefficients and the phase sources (FP32: two workgroups per CU)
+    const size_t tbytes = (((size_t)2 * HEIS_STG_SLOT + 2 * HEIS_STG_NXMAX + ((size_t)1 << s.nlo) +
+                            ((size_t)1 << (s.ncx - s.nlo))) * sizeof(T) + 7) & ~(size_t)7;
+    const size_t lds = tbytes + (size_t)(s.nll + s.nphase) * sizeof(int64_t) + (size_t)s.nll * sizeof(T);
+    if (lds > HEIS_STG_LDS_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "Heisenberg blocks exceed LDS");
+    // code: bits 0-1 the workgroup -> block map, bit 2 the block list (0 memory
+    // order, 1 popcount-major)
+    auto go = [&](int code) {
+        a.order = code & 3;
+        a.blk = (code & 4) ? s.blkm : s.blk;
+        return epi ? heis_stg_go<T, true>(c, a, nblocks, lds, x, out, scale, prev, beta)
+                   : heis_stg_go<T, false>(c, a, nblocks, lds, x, out, scale, prev, beta);
+    };
+    const int ti = sizeof(T) == 8;
+    phub_op *mop = const_cast<phub_op *>(op);
+    if (mop->tune_heis[ti] < 0) {
+        // time the eight block orders on the caller's vectors (every launch rewrites
+        // out completely); PHUB_HEIS_ORDER=k forces code k, PHUB_TUNE=0 takes code 0
+        const char *env = std::getenv("PHUB_TUNE");
+        const char *force = std::getenv("PHUB_HEIS_ORDER");
+        int best = 0;
+        if (force && std::atoi(force) >= 0 && std::atoi(force) < 8) {
+            best = std::atoi(force);
+        } else if (!(env && env[0] == '0')) {
+            float tb = 1e30f;
+            for (int order = 0; order < 8; ++order) {
+                PHUB_HIP(c, hipEventRecord(c->ev0, c->stream));
+                int rc = go(order);
+                if (rc) return rc;
+                PHUB_HIP(c, hipEventRecord(c->ev1, c->stream));
+                PHUB_HIP(c, hipEventSynchronize(c->ev1));
+                float ms = 0.f;
+                PHUB_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+                if (std::getenv("PHUB_TUNE_VERBOSE"))
+                    std::fprintf(stderr, "phub tune: %s heisenberg block order %d: %.3f ms\n",
+                                 sizeof(T) == 4 ? "fp32" : "fp64", order, ms);
+                if (ms < tb) { tb = ms; best = order; }
+            }
+        }
+        mop->tune_heis[ti] = best;
+    }
+    int rc = go(mop->tune_heis[ti]);
+    if (rc) return rc;
+    if (nblocks_out) *nblocks_out = nblocks;
+    return PHUB_OK;
+}
+
 static bool phub_force_r1() {
     const char *e = std::getenv("PHUB_UP_ROWS");
     return e && e[0] == '1';
@@ -1215,55 +1445,7 @@ static int hv_launch_main(phub_ctx *c, const phub_op *op, const T *x, T *out, do
             hipLaunchKernelGGL((k_hv_hub<T, HUB_R, false>), dim3(nblocks), dim3(256), 0,
                                c->stream, a, x, out, scale, prev, beta, c->d_red, ncb);
     } else {
-        const int64_t nb = 8 * ((op->h.nitems + 7) / 8);
-        if (nb > INT32_MAX) return phub_fail(c, PHUB_ERR_UNSUPPORTED, "grid too large");
-        nblocks = (int)nb;
-        if (epi) {
-            int rc = phub_ensure_red(c, (size_t)nblocks * 2 + 64);
-            if (rc) return rc;
-        }
-        HeisBlkArgs a = heis_blk_args(op);
-        const size_t lds = (size_t)(a.na1 + a.na2) * sizeof(uint32_t);
-        // order: 0 each XCD takes one contiguous eighth of the item list, 1 round robin
-        auto go = [&](int order) {
-            a.order = order;
-            if (epi)
-                hipLaunchKernelGGL((k_hv_heis_blk<T, true>), dim3(nblocks), dim3(256), lds,
-                                   c->stream, a, x, out, scale, prev, beta, c->d_red);
-            else
-                hipLaunchKernelGGL((k_hv_heis_blk<T, false>), dim3(nblocks), dim3(256), lds,
-                                   c->stream, a, x, out, scale, prev, beta, c->d_red);
-        };
-        const int ti = sizeof(T) == 8;
-        phub_op *mop = const_cast<phub_op *>(op);
-        if (mop->tune_heis[ti] < 0) {
-            // tuner: time both item orders on the caller's vectors (the kernel
-            // rewrites out completely, so the final launch below gives the result);
-            // PHUB_HEIS_ORDER=k forces order k
-            const char *env = std::getenv("PHUB_TUNE");
-            const char *force = std::getenv("PHUB_HEIS_ORDER");   // a fixed code
-            int best = 0;
-            if (force && std::atoi(force) >= 0 && std::atoi(force) < 2) {
-                best = std::atoi(force);
-            } else if (!(env && env[0] == '0')) {
-                float tb = 1e30f;
-                for (int order = 0; order < 2; ++order) {
-                    PHUB_HIP(c, hipEventRecord(c->ev0, c->stream));
-                    go(order);
-                    PHUB_HIP(c, hipGetLastError());
-                    PHUB_HIP(c, hipEventRecord(c->ev1, c->stream));
-                    PHUB_HIP(c, hipEventSynchronize(c->ev1));
-                    float ms = 0.f;
-                    PHUB_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-                    if (std::getenv("PHUB_TUNE_VERBOSE"))
-                        std::fprintf(stderr, "phub tune: %s heisenberg item order %d: %.3f ms\n",
-                                     sizeof(T) == 4 ? "fp32" : "fp64", order, ms);
-                    if (ms < tb) { tb = ms; best = order; }
-                }
-            }
-            mop->tune_heis[ti] = best;
-        }
-        go(mop->tune_heis[ti]);
+        return hv_heis_stage<T>(c, op, x, out, scale, prev, beta, epi, nblocks_out);
     }
     PHUB_HIP(c, hipGetLastError());
     if (nblocks_out) *nblocks_out = nblocks;

@@ -114,12 +114,7 @@ extern "C" int phub_op_heisenberg(phub_ctx *c, const phub_heis_desc *d, phub_op 
     op->dim_up = op->h.dim;
     op->dim_dn = 1;
     op->dim = op->h.dim;
-    {
-        const double nh = (double)(1ull << (n - op->h.L));
-        op->table_bytes = (op->h.masks ? (double)op->h.dim * 8.0 : 0.0) + nh * 16.0 +
-                          (double)(1ull << op->h.L) * (n - op->h.L + 1) * 8.0 +
-                          (double)op->h.nitems * 4.0;
-    }
+    op->table_bytes = (op->h.masks ? (double)op->h.dim * 8.0 : 0.0) + op->h.stg.bytes;
     *out = op;
     return PHUB_OK;
 }

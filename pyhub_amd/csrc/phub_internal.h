@@ -111,26 +111,61 @@ struct HopTable {
     float *cls32 = nullptr;
 };
 
-// one bond {J<K} for the block H*v: bit positions j > k of sites J, K; the pull
-// coefficient for a target whose site J is down (cj) or up (cr); half the Ising weight
-struct HeisBond {
-    int j, k;
-    double cj, cr, hi;
-    float fj, fr, fh;     // the same three in FP32
-    int pad;
+// Staged Heisenberg H*v (k_hv_heis_stg).  The high half holds the top nh bits of a
+// state (sites 1..nh), nh <= HEIS_STG_NHMAX so that one block of C(nh, nh/2) states
+// and a neighbouring block fit in LDS together in FP64, and so that LDS byte offsets
+// of FP32 positions (4 * j < 2^16) fit the 16-bit flip targets.
+constexpr int HEIS_STG_NHMAX = 15;
+constexpr int HEIS_STG_CNTMAX = 6435;    // C(15, 7)
+constexpr int HEIS_STG_NXMAX = 16;       // cross high sites (<= nh)
+
+struct HeisStgLL {      // low-low bond: low bit positions p > q
+    int p, q;
+    float c32, h32;     // flip coefficient, half the Ising weight
+    double c64, h64;
+};
+struct HeisStgPair {    // two high-high bonds sharing one word of the target table
+    float c32[2];
+    double c64[2];
+};
+struct HeisStgX {       // cross bond: high site index jx (into the cross-site list), low bit kk
+    int jx, kk;
+    float c32, h32;
+    double c64, h64;
+};
+struct HeisStgPhase {   // cross bonds [b0, b0 + nb) share the low bit kk
+    int kk, b0, nb, pad;
 };
 
-// elements per work item of the block Heisenberg H*v
-constexpr int HEIS_CHUNK = 1024;
+struct HeisStage {
+    int nh = 0, L = 0, k = 0;
+    int cntmax = 0;                // C(nh, nh/2): index of the zero slot
+    int ncx = 0, nlo = 0;          // cross high sites; the first nlo index the low Ising table
+    int nll = 0, npair = 0, nx = 0, nphase = 0;
+    int64_t GS = 0;                // 2^nh: per-pattern table stride
+    int64_t nblk = 0;              // valid low patterns
+    int32_t *cnt = nullptr;        // device, nh+1: C(nh, m)
+    int64_t *goff = nullptr;       // device, nh+1: first table row of popcount m
+    int64_t *first = nullptr;      // device, 2^L: first index of the block of xl (-1: none)
+    uint32_t *pos = nullptr;       // device, 2^nh: position of a high pattern in its block
+    uint32_t *blk = nullptr;       // device, nblk: valid xl in memory order
+    uint32_t *blkm = nullptr;      // device, nblk: the same ordered by popcount, then memory
+    uint32_t *hht = nullptr;       // device, npair x GS: two 16-bit high-high flip targets
+    uint16_t *xt = nullptr;        // device, ncx x GS: cross flip targets and direction bit
+    uint32_t *cx = nullptr;        // device, GS: bits of xh at the cross high sites
+    float *ehh32 = nullptr;        // device, GS: half Ising weight of antiparallel high-high bonds
+    double *ehh64 = nullptr;
+    HeisStgLL *ll = nullptr;       // device, nll
+    HeisStgPair *hp = nullptr;     // device, npair
+    HeisStgX *xb = nullptr;        // device, nx (phase order)
+    HeisStgPhase *ph = nullptr;    // device, nphase
+    double bytes = 0.0;            // device bytes of the tables above
+};
 
 struct HeisTables {
     int n = 0;
     int64_t dim = 0;
-    uint64_t *masks = nullptr;     // device, dim
-    // two-half rank tables: rank(x) = C - 1 - (A[x >> L] + B[popc(x>>L)][x & (2^L-1)])
-    int L = 0;
-    uint64_t *tabA = nullptr;      // device, 2^(n-L)
-    uint64_t *tabB = nullptr;      // device, (n-L+1) * 2^L
+    uint64_t *masks = nullptr;     // device, dim (kept when dim <= 2^28)
     int nb = 0;                    // number of undirected bonds
     int *bj = nullptr, *bk = nullptr;   // device, bit positions of the two sites
     double *bJ = nullptr;          // device, J(J,K) used when site J is up in the source
@@ -138,31 +173,7 @@ struct HeisTables {
     double *bsign = nullptr;       // device, flip sign factor per bond
     double diag_const = 0.0;       // sum of 0.75 J(J,J) + 0.5 * (on-site) terms
     double *bond_ising = nullptr;  // device, Ising weight per bond
-    // block enumeration (no mask table needed): states with the same low-half
-    // pattern xl are contiguous; headpat lists the high-half patterns of each
-    // popcount m in the order of A (headpat[hoff[m] + A[xh]] = xh)
-    uint32_t *headpat = nullptr;   // device, 2^(n-L)
-    int64_t *hoff = nullptr;       // device, n-L+2 offsets
-    uint32_t *tabA32 = nullptr;    // device, A as 32-bit (A < C(n-L, m))
-    // A itself split in two halves (small enough for LDS):
-    // A[h] = A1[h >> LA] + A2[popc(h >> LA)][h & (2^LA - 1)]
-    int LA = 0;
-    uint32_t *tabA1 = nullptr;     // device, 2^(n-L-LA)
-    uint32_t *tabA2 = nullptr;     // device, (n-L-LA+1) * 2^LA
-    // bonds grouped by where their sites lie: [0,ntt) both low half, [ntt,ntt+nhh)
-    // both high half, then cross bonds (site J in the high half, K in the low
-    // half) ordered by J so that bonds sharing their high site are adjacent
-    HeisBond *sbonds = nullptr;    // device
-    int ntt = 0, nhh = 0, nx = 0;
-    // high-high bonds are further ordered by where their two bits fall in the A split:
-    // [0,nhh_lo) both below LA, [nhh_lo,nhh_lo+nhh_hi) both at or above, then straddling
-    int nhh_lo = 0, nhh_hi = 0;
-    // half Ising weight of the antiparallel high-high bonds of each high pattern
-    float *ehh32 = nullptr;        // device, 2^(n-L)
-    double *ehh64 = nullptr;       // device, 2^(n-L)
-    // work items of the block H*v: (low pattern << 12) | chunk, in memory order
-    uint32_t *items = nullptr;     // device
-    int64_t nitems = 0;
+    HeisStage stg;                 // tables of the H*v
 };
 
 struct phub_op {

@@ -640,10 +640,10 @@ extern "C" int phub_two_rdm_hubbard(phub_ctx *c, const phub_op *op, const double
 // flip[b] = sum over states i with site J up, K down of bra[pos] * ket[i]
 // (pos = flipped state), for bond b = {J<K}; and the reverse (K up, J down).
 // zz[b] = sum_i bra[i]*ket[i] * (same spin ? +1 : -1); norm = sum bra*ket.
-__global__ __launch_bounds__(256) void k_heis_bonds(int64_t dim, int L, uint64_t total_m1,
+__global__ __launch_bounds__(256) void k_heis_bonds(int64_t dim, int L,
                                                     const uint64_t *__restrict__ masks,
-                                                    const uint64_t *__restrict__ tabA,
-                                                    const uint64_t *__restrict__ tabB, int nb,
+                                                    const int64_t *__restrict__ first,
+                                                    const uint32_t *__restrict__ pos, int nb,
                                                     const int *__restrict__ bj,
                                                     const int *__restrict__ bk,
                                                     const double *__restrict__ bra,
@@ -666,12 +666,12 @@ __global__ __launch_bounds__(256) void k_heis_bonds(int64_t dim, int L, uint64_t
             const bool diff = ((x >> j) ^ (x >> kk)) & 1ull;
             atomicAdd(&acc[3 * b + 2], diff ? -w : w);
             if (diff) {
+                // the index of x2: its low-half block's first index plus the position
+                // of its high half within the block
                 const uint64_t x2 = x ^ ((1ull << j) | (1ull << kk));
-                const uint64_t xh = x2 >> L;
-                const uint64_t colex = tabA[xh] + tabB[((uint64_t)__popcll(xh) << L) + (x2 & lmask)];
-                const int64_t pos = (int64_t)(total_m1 - colex);
+                const int64_t i2 = first[x2 & lmask] + (int64_t)pos[x2 >> L];
                 // source i = s has J up (bit j set): contributes to ordered (J,K)
-                atomicAdd(&acc[3 * b + (((x >> j) & 1ull) ? 0 : 1)], bra[pos] * k);
+                atomicAdd(&acc[3 * b + (((x >> j) & 1ull) ? 0 : 1)], bra[i2] * k);
             }
         }
     }
@@ -695,7 +695,7 @@ static int heis_bonds(phub_ctx *c, const phub_op *op, const double *bra, const d
     PHUB_HIP(c, hipMemsetAsync(d.p, 0, (3 * nb + 1) * sizeof(double), c->stream));
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((t.dim + 255) / 256, 2048));
     hipLaunchKernelGGL(k_heis_bonds, dim3(grid), dim3(256), (3 * nb + 1) * sizeof(double),
-                       c->stream, t.dim, t.L, (uint64_t)t.dim - 1, t.masks, t.tabA, t.tabB, nb,
+                       c->stream, t.dim, t.stg.L, t.masks, t.stg.first, t.stg.pos, nb,
                        t.bj, t.bk, bra, ket, d.p);
     PHUB_HIP(c, hipGetLastError());
     h.resize(3 * nb + 1);

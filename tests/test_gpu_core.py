@@ -140,10 +140,10 @@ def test_heisenberg_block_hv_vs_oracle(ctx, n, seed, dtype):
     assert np.max(np.abs(w - ref)) <= tol * max(1.0, np.max(np.abs(ref)))
 
 
-@pytest.mark.parametrize("code", range(2))
+@pytest.mark.parametrize("code", range(8))
 def test_heisenberg_block_hv_item_orders(ctx, monkeypatch, code):
-    """Both item orders the tuner can pick (PHUB_HEIS_ORDER forces one) give the
-    oracle's HPROD."""
+    """Every block order the tuner can pick (PHUB_HEIS_ORDER forces one: workgroup to
+    XCD map and memory or popcount-major block list) gives the oracle's HPROD."""
     from pyhub_amd.engine import HeisenbergOp
     monkeypatch.setenv("PHUB_HEIS_ORDER", str(code))
     n = 20
