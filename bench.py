@@ -324,8 +324,8 @@ def _heis_bench(args, ctx, torch, L):
                "alg_bytes": b_alg, "alg_flops": flops, "mean_flips_per_state": hbar},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(gbs / 8000.0, 4),
-                     "traffic": _profile_traffic("k_hv_heis_blk_j1j2_6x6.md") if (Lx, Ly) == (6, 6) else None,
-                     "kernel": "k_hv_heis_blk<float,false>",
+                     "traffic": _profile_traffic("k_hv_heis_stg_j1j2_6x6.md") if (Lx, Ly) == (6, 6) else None,
+                     "kernel": "k_hv_heis_stg<float,512,false>",
                      "valu_fp32_tflops": round(flops / (ms_avg * 1e-3) / 1e12, 2)},
     }
     if args.calib:

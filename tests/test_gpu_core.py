@@ -140,6 +140,30 @@ def test_heisenberg_block_hv_vs_oracle(ctx, n, seed, dtype):
     assert np.max(np.abs(w - ref)) <= tol * max(1.0, np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("n,nh,seed", [(14, 3, 47), (16, 5, 48), (18, 7, 49), (20, 6, 50)])
+def test_heisenberg_hv_high_half_sizes(ctx, monkeypatch, n, nh, seed):
+    """The H*v with fewer high sites than n - n/2 (PHUB_HEIS_NH): more low-low and
+    cross bonds, more and smaller blocks, popcounts with no neighbouring block; FP64
+    and FP32 against the oracle's HPROD, both sign conventions' flip signs."""
+    from pyhub_amd.engine import HeisenbergOp
+    monkeypatch.setenv("PHUB_HEIS_NH", str(nh))
+    rng = np.random.default_rng(seed)
+    J = rng.normal(size=(n, n))
+    J[rng.random((n, n)) < 0.4] = 0.0
+    J = (J + J.T) / 2
+    op = HeisenbergOp(ctx, n, J)
+    m = O.HeisenbergModel(n, J)
+    v = rng.normal(size=m.dim)
+    ref = m.hv(v, fast=True)
+    w64 = op.hv(dev(v)).cpu().numpy()
+    w32 = op.hv(dev(v, torch.float32)).double().cpu().numpy()
+    scale = max(1.0, np.max(np.abs(ref)))
+    print(f"n={n} nh={nh} fp64 {np.max(np.abs(w64 - ref)):.2e} fp32 {np.max(np.abs(w32 - ref)):.2e}",
+          flush=True)
+    assert np.max(np.abs(w64 - ref)) <= 1e-12 * scale
+    assert np.max(np.abs(w32 - ref)) <= 2e-5 * scale
+
+
 @pytest.mark.parametrize("code", range(8))
 def test_heisenberg_block_hv_item_orders(ctx, monkeypatch, code):
     """Every block order the tuner can pick (PHUB_HEIS_ORDER forces one: workgroup to
