@@ -67,7 +67,8 @@ def main():
     ap.add_argument("--refine", type=int, default=4, help="at most this many FP64 corrections")
     ap.add_argument("--calib", action="store_true", help="add a 1 GiB device copy (counter calibration)")
     ap.add_argument("--workload", default="hubbard4x4",
-                    choices=["hubbard4x4", "heis6x6", "batch256", "band4x4", "u4", "ops4x4", "grad64"])
+                    choices=["hubbard4x4", "heis6x6", "batch256", "band4x4", "u4", "ops4x4", "grad64",
+                             "rdm4x4"])
     ap.add_argument("--sets", type=int, default=64, help="parameter sets (grad64)")
     ap.add_argument("--threads", type=int, default=8,
                     help="host threads sharing the GPU, one stream each (grad64)")
@@ -104,6 +105,9 @@ def main():
         return
     if args.workload == "grad64":
         _grad_bench(args, ctx, torch)
+        return
+    if args.workload == "rdm4x4":
+        _rdm_bench(args, ctx, torch)
         return
     Lx = Ly = 4
     n = Lx * Ly
@@ -330,6 +334,8 @@ def _heis_bench(args, ctx, torch, L):
     }
     if args.calib:
         _calib(out, torch, dev)
+    if not args.no_cpu:
+        out["cpu_baseline"] = _heis_cpu_baseline(n, Jm, D)
     if not args.no_lanczos:
         steps = args.lanczos_steps
         os.environ["PHUB_PROGRESS"] = "10"   # a progress line every 10 steps
@@ -345,6 +351,76 @@ def _heis_bench(args, ctx, torch, L):
         print(f"# lanczos plain32: {r.steps} steps, E0={r.e0:.10f}, E0/site={r.e0 / n:.10f}, "
               f"{wall:.1f}s", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
+
+
+def _rdm_bench(args, ctx, torch):
+    """Reduced quantities of the 4x4 t-t' state (N_up = N_dn = 8, D = 165,636,900, FP64):
+    one phub_rdm_hubbard call gives the 2 x 16 x 16 hop expectations, the densities and
+    double occupancies and the 16 x 16 <S_J.S_K> matrix (ONE_RDM, TWO_RDM's NI and
+    J_SPIN of static_rq.f90:153-395), on one seeded normalised vector."""
+    from pyhub_amd.engine import HubbardOp
+    T = square_T(4, 4, 1.0, -0.2)
+    op = HubbardOp(ctx, 16, 8, 8, T, 8.0)
+    g = torch.Generator(device=ctx.device)
+    g.manual_seed(4)
+    v = torch.rand(op.dim, generator=g, device=ctx.device, dtype=torch.float64) - 0.5
+    v /= torch.linalg.vector_norm(v)
+    for _ in range(max(1, args.warmup)):
+        op.rdm(v)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        up, dn, ni, ss = op.rdm(v)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = wall / args.steps * 1e3
+    # every kernel of the call reads the state at least once: 8 B x D per pass
+    gbs = 8 * op.dim / (ms * 1e-3) / 1e9
+    out = {
+        "metric": "reduced-quantity evaluations per second (1-RDM per spin, <n>, <n n>, S.S)",
+        "value": round(1e3 / ms, 3), "unit": "calls/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
+        "data": "synthetic (seeded uniform vector, normalised)",
+        "config": {"workload": "4x4 t-t' Hubbard state, N_up=N_dn=8, one phub_rdm_hubbard call",
+                   "dim": op.dim, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(gbs / 8000.0, 4), "traffic": None,
+                     "kernel": "whole call (five kernels), one pass of the FP64 state counted"},
+        "rdm": {"trace_up": float(np.trace(up)), "trace_dn": float(np.trace(dn)),
+                "sum_ni": float(np.sum(ni)), "ss_00": float(ss[0, 0])},
+        "cpu_baseline": None,
+        "cpu_baseline_note": "not measured for this auxiliary workload: the oracle's 1-RDM of a "
+                             "1.66e8-state vector takes minutes on the host",
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _heis_cpu_baseline(n, Jm, D, budget_s=10.0):
+    """The oracle's Heisenberg matrix column (unrank the state, every flip and the Ising
+    diagonal, rank the flipped states in 64 bits: the per-state work of HPROD) over
+    states spread across the 6x6 index range, one core, until `budget_s` has passed;
+    value = states/s x 16 B (the FP64 H*v's algorithmic bytes per state)."""
+    import ctypes as C
+    from oracle import oracle as O
+    L = O.lib()
+    rows = np.zeros(4 * n * n + 8, dtype=np.uint64)
+    vals = np.zeros(4 * n * n + 8, dtype=np.float64)
+    Jc = np.ascontiguousarray(Jm, dtype=np.float64)
+    rng = np.random.default_rng(7)
+    idx = rng.integers(0, D, 1 << 22, dtype=np.int64)
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and done < len(idx):
+        for i in idx[done:done + 4096]:
+            L.orc_heis_column(n, Jc, 1, C.c_uint64(int(i)), rows, vals)
+        done += 4096
+    dt = time.perf_counter() - t0
+    rate = done / dt
+    return {"value": round(rate * 16 / 1e9, 6), "unit": "GB/s", "cores": 1, "kind": "port",
+            "states_per_s": round(rate, 1), "seconds": round(dt, 2),
+            "sample": f"{done} matrix columns of the 6x6 J1-J2 HPROD at random state indices "
+                      "(oracle restatement, unrank/flip/rank in 64 bits, called from Python "
+                      "one column at a time), one core; value = columns/s x 16 B"}
 
 
 def _batch_bench(args, ctx, torch):
@@ -382,7 +458,124 @@ def _batch_bench(args, ctx, torch):
                   "sum_dim": int(sum(dims)), "e0_first": float(e0[0]),
                   "g_finite": bool(np.all(np.isfinite(G)))},
     }
+    if not args.no_cpu:
+        G05 = gii_omega(e0, ga, gb, gn, clusters, omega, eta=0.5)   # resolved by 120 steps
+        out["cpu_baseline"] = _batch_cpu_baseline(clusters, e0, G, G05, omega)
     print(json.dumps(out), flush=True)
+
+
+def _batch_cpu_baseline(clusters, e0, G, G05, omega, budget_s=15.0, dmax=1500, nsteps=120):
+    """The oracle's FP64 path on a bounded sample of the batch, one core: clusters 0, 1,
+    ... whose ground sector has at most `dmax` states, until `budget_s` has passed: the
+    ground state (oracle Lanczos to 1e-10) and every particle and hole G_ii chain (120
+    FP64 steps).  value = the batch's clusters/s at the sample's rate in state-steps
+    (sector dimension x steps, summed over the ground state and the chains).
+
+    The same pass checks the GPU's results on the sample (the oracle as checker):
+    E0 against dense FP64 ED; G_ii(omega) at eta = 0.05 against dense ED where 120 steps
+    exhaust the target sector, and against the oracle's FP64 chains at eta = 0.05 and
+    0.5 elsewhere (clusters with a degenerate ground state are skipped: psi0 is not
+    unique there)."""
+    from math import comb
+    from oracle import oracle as O
+
+    def cf(a, b, n2, zz):
+        f = np.zeros_like(zz)
+        ln = len(a)
+        while ln > 1 and a[ln - 1] == 0.0 and b[ln - 2] == 0.0:
+            ln -= 1
+        for k in range(ln - 1, -1, -1):
+            f = 1.0 / (zz - a[k] - (b[k] ** 2 if k < ln - 1 else 0.0) * f)
+        return n2 * f
+
+    def work(cl):   # state-steps of the ground state (256-step cap) and the chains
+        d0 = comb(cl.n, cl.n_up) * comb(cl.n, cl.n_dn)
+        w = d0 * 256
+        for nu, nd in ((cl.n_up + 1, cl.n_dn), (cl.n_up - 1, cl.n_dn),
+                       (cl.n_up, cl.n_dn + 1), (cl.n_up, cl.n_dn - 1)):
+            if 0 <= nu <= cl.n and 0 <= nd <= cl.n:
+                w += cl.n * comb(cl.n, nu) * comb(cl.n, nd) * nsteps
+        return w
+
+    cpu_s, sample, wsum = 0.0, [], 0
+    e_err, g_ex, g005, g05 = [], [], [], []
+    for ci, cl in enumerate(clusters):
+        if cpu_s > budget_s:
+            break
+        m = O.HubbardModel(cl.n, cl.n_up, cl.n_dn, cl.T, cl.U)
+        if m.dim > dmax:
+            continue
+        t0 = time.perf_counter()
+        r = m.lanczos(np.random.default_rng(ci).uniform(-0.5, 0.5, m.dim), maxlcz=256, acc=1e-10)
+        psi0 = r["psi"] / np.linalg.norm(r["psi"])
+        chains = []
+        for site in range(1, cl.n + 1):
+            for spin in (0, 1):
+                for create in (1, 0):
+                    nsp = (cl.n_up if spin == 0 else cl.n_dn) + (1 if create else -1)
+                    if not 0 <= nsp <= cl.n:
+                        continue
+                    tgt, x = m.excite(psi0, site, spin, create)
+                    if x @ x == 0.0:
+                        continue
+                    chains.append((site, spin, create, tgt, x, tgt.cf_chain(x, nsteps)))
+        cpu_s += time.perf_counter() - t0
+        sample.append(ci)
+        wsum += work(cl)
+        # checks (untimed)
+        ev, V = np.linalg.eigh(m.dense(fast=True))
+        e_err.append(abs(e0[ci] - ev[0]) / max(1.0, abs(ev[0])))
+        if ev[1] - ev[0] < 1e-6:
+            continue
+        psi = V[:, 0]
+        for site in range(1, cl.n + 1):
+            for spin in (0, 1):
+                mine = [c for c in chains if c[0] == site and c[1] == spin]
+                exact = all(c[3].dim <= nsteps for c in mine)
+                gx = np.zeros(len(omega), dtype=complex)
+                g_ch = {0.05: np.zeros(len(omega), dtype=complex),
+                        0.5: np.zeros(len(omega), dtype=complex)}
+                for _, _, create, tgt, _, (a, b, n2) in mine:
+                    sgn = 1.0 if create else -1.0
+                    if exact:
+                        _, x = m.excite(psi, site, spin, create)
+                        evt, Vt = np.linalg.eigh(tgt.dense(fast=True))
+                        w = (Vt.T @ x) ** 2
+                        zz = sgn * (omega + 0.05j) + ev[0]
+                        gx += sgn * (w[None, :] / (zz[:, None] - evt[None, :])).sum(axis=1)
+                    for eta in (0.05, 0.5):
+                        g_ch[eta] += sgn * cf(a, b, n2, sgn * (omega + 1j * eta) + r["e0"])
+                gg = G[ci, site - 1, spin]
+                if exact:
+                    g_ex.append(np.max(np.abs(gg - gx)) / np.max(np.abs(gx)))
+                else:
+                    g005.append(np.max(np.abs(gg - g_ch[0.05])) / np.max(np.abs(g_ch[0.05])))
+                    g05.append(np.max(np.abs(G05[ci, site - 1, spin] - g_ch[0.5])) /
+                               np.max(np.abs(g_ch[0.5])))
+    wall = sum(work(cl) for cl in clusters)
+    rate = wsum / cpu_s
+
+    def st(v):
+        v = np.sort(np.asarray(v))
+        if len(v) == 0:
+            return None
+        return {"n": int(len(v)), "median": float(v[len(v) // 2]),
+                "p90": float(v[int(0.9 * (len(v) - 1))]), "max": float(v[-1])}
+
+    return {"value": round(len(clusters) / (wall / rate), 3), "unit": "clusters/s", "cores": 1,
+            "kind": "port", "seconds": round(cpu_s, 2), "clusters_sampled": len(sample),
+            "sample": f"clusters {sample[0]}..{sample[-1]} with ground sectors of at most {dmax} "
+                      f"states ({len(sample)} clusters, {cpu_s:.1f} s): oracle FP64 Lanczos ground "
+                      f"state and every particle/hole chain of {nsteps} steps, one core; value = "
+                      f"256 / (the batch's state-steps / the sample's state-steps per second)",
+            "e0_rel_err_vs_dense_ed": st(e_err),
+            "gii_rel_err_eta005_vs_dense_ed_exhausted": st(g_ex),
+            "gii_rel_err_eta005_vs_fp64_chain": st(g005),
+            "gii_rel_err_eta05_vs_fp64_chain": st(g05),
+            "gii_err_note": "errors are max|G_gpu - G_ref| / max|G_ref| per function (site, spin) "
+                            "on 2,048 points; the GPU chains run FP32 vectors with FP64 scalars; "
+                            "where 120 steps do not exhaust the sector, FP32 and FP64 chains "
+                            "place the unresolved poles apart at eta = 0.05"}
 
 
 def _band_bench(args, ctx, torch, L):
@@ -666,7 +859,59 @@ def _grad_bench(args, ctx, torch):
                  "fd_rel_err_set0_bonds_0_9": fd_err, "loss_set0": res[0]["L"],
                  "e0_set0": res[0]["e0"]},
     }
+    # roofline of the dominant kernels, the FP64 H*v of the sector (up and down kernels
+    # of one H*v, as the Lanczos and MINRES iterations call it), timed with HIP events
+    # on the stream it runs on
+    op, _, psi = solve(spec, t, eps, U)
+    w = torch.empty_like(psi)
+    stream = torch.cuda.current_stream()
+    for _ in range(3):
+        op.hv(psi, w)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(20)]
+    for a, b in evs:
+        a.record(stream)
+        op.hv(psi, w)
+        b.record(stream)
+    torch.cuda.synchronize()
+    hv_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    D = op.dim
+    gbs = 2 * 8 * D / (hv_ms * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                       "frac": round(gbs / 8000.0, 4), "traffic": None,
+                       "kernel": "k_hv_up<double,1> + k_hv_dn<double,false> (one FP64 H*v, "
+                                 f"D = {D}, {hv_ms * 1e3:.1f} us)"}
+    if not args.no_cpu:
+        out["cpu_baseline"] = _grad_cpu_baseline(spec, t, eps, U, out["grad"]["mean_minres_iters"])
     print(json.dumps(out), flush=True)
+
+
+def _grad_cpu_baseline(spec, t, eps, U, minres_iters):
+    """The oracle's FP64 path for one parameter set, one core: the ground state (oracle
+    Lanczos to 1e-12) and its 1-RDM, timed, and one oracle H*v, timed.  A set needs two
+    ground states with their 1-RDMs (the target's and the parameters') and one H*v per
+    MINRES iteration of the adjoint solve: value = 1 / (2 (t_gs + t_rdm) + iters x t_hv),
+    a lower bound of the oracle's time per set (the Hellmann-Feynman pass and the
+    gradient contraction are left out)."""
+    from oracle import oracle as O
+    m = O.HubbardModel(spec.n, spec.n_up, spec.n_dn, spec.T(t, eps), spec.U_vec(U))
+    v = np.random.default_rng(5).uniform(-0.5, 0.5, m.dim)
+    t0 = time.perf_counter()
+    m.hv(v, fast=True)
+    t_hv = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r = m.lanczos(v, maxlcz=400, acc=1e-12)
+    t_gs = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    m.one_rdm(r["psi"] / np.linalg.norm(r["psi"]))
+    t_rdm = time.perf_counter() - t0
+    per_set = 2 * (t_gs + t_rdm) + minres_iters * t_hv
+    return {"value": round(1.0 / per_set, 4), "unit": "sets/s", "cores": 1, "kind": "port",
+            "seconds_ground_state": round(t_gs, 2), "lanczos_steps": int(r["nblcz"]),
+            "seconds_one_rdm": round(t_rdm, 3), "seconds_hv": round(t_hv, 4),
+            "sample": "parameter set 0: oracle FP64 Lanczos ground state (1e-12) and 1-RDM, and "
+                      "one oracle H*v, one core; value = 1 / (2 (ground state + 1-RDM) + "
+                      "mean MINRES iterations x H*v)"}
 
 
 def _ops_cpu_baseline(T, D, nprod):
