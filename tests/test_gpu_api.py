@@ -157,6 +157,49 @@ def test_lanczos_path_temperature_and_excited_states(ctx):
         fl.spgf()
 
 
+@pytest.mark.parametrize("temp,path", [(0.1, "lanczos"), (0.1, "dense"), (0.6, "dense")])
+def test_thermal_ensemble_vs_canonical_average(ctx, temp, path):
+    """BOLTZMANN (boltzmann.f90:1-29) weights against the canonical ensemble of the
+    oracle's full spectrum: Z, the ensemble energy e0 = sum_k w_k E_k / Z and the
+    ensemble 1-RDM, densities and spin correlations equal the exact thermal averages
+    sum_k exp(-E_k/T) <k|A|k> / sum_k exp(-E_k/T) over all states (dense ED of the
+    pinned HPROD, expectations from the pinned ONE_RDM / NI / J_SPIN restatements),
+    to the 1e-8 weight cut of BOLTZMANN.  A wrong sign in the exponent, a missing
+    1/Z or weights paired with the wrong states fails these comparisons."""
+    from pyhub_amd.solver.fermi_hubbard import FermiHubbard
+    T = O.disordered_ring_T(6)
+    U = 2.5
+    m = O.HubbardModel(6, 3, 2, T, U)                      # dim 300, N_up != N_dn
+    ev, V = np.linalg.eigh(m.dense())
+    p = np.exp((ev[0] - ev) / temp)
+    Zx = p.sum()
+    p /= Zx
+    E_x = p @ ev
+    rho = sum(pk * m.one_rdm(V[:, k]) for k, pk in enumerate(p) if pk > 1e-14)
+    ni = sum(pk * m.ni(V[:, k]) for k, pk in enumerate(p) if pk > 1e-14)
+    ss = sum(pk * m.sisj(V[:, k]) for k, pk in enumerate(p) if pk > 1e-14)
+    if path == "lanczos":
+        f = FermiHubbard(6, 3, 2, T, U=U, T=temp, lanczos_mode="full64")
+        f.kernel(max_lcz=250, acc_lcz=1e-12, nb_comp_states=3)
+        assert f.is_lanczos
+    else:
+        f = FermiHubbard(6, 3, 2, T, U=U, T=temp)
+        f.kernel(max_lcz=600, nb_comp_states=3)
+        assert not f.is_lanczos
+    nw = len(f.boltzmann)
+    # NW: the states whose weight exceeds 1e-8 (at least nb_comp_states)
+    assert nw == max(3, int(np.sum(np.exp((ev[0] - ev) / temp) > 1e-8)))
+    cut = np.exp((ev[0] - ev[nw:]) / temp).sum() / Zx      # weight BOLTZMANN leaves out
+    print(f"T={temp} {path}: NW={nw}, weight left out {cut:.1e}", flush=True)
+    tol = 1e-8 + 4 * cut
+    assert abs(f.Z - Zx) <= tol * Zx
+    assert abs(f.e0 - E_x) <= tol * max(1.0, abs(E_x))
+    assert np.max(np.abs(f.one_rdm["up"] - rho[0])) <= tol
+    assert np.max(np.abs(f.one_rdm["down"] - rho[1])) <= tol
+    assert np.max(np.abs(f.ni - ni)) <= tol
+    assert np.max(np.abs(f.spin_cor_matrix - ss)) <= tol
+
+
 def test_self_energy_dimer_closed_form(ctx):
     """self_energy / qpw (reduced_quantities.py:134-169) on the half-filled Hubbard
     dimer, whose self-energy is known in closed form: in the bonding (e = -t) and
