@@ -233,8 +233,11 @@ def _spread_indices(D, rng, count):
     return sorted(i for i in idx if i < D)
 
 
-@pytest.mark.parametrize("conv", [1, 0])
-def test_config5_heis6x6_sparse_columns_exact(ctx, conv):
+@pytest.mark.parametrize("conv,dtype", [(1, torch.float32), (0, torch.float32), (1, torch.float64)])
+def test_config5_heis6x6_sparse_columns_exact(ctx, conv, dtype):
+    """H applied to 1,000 scaled unit vectors spread over the 6x6 index range (indices past
+    2^32 included) equals the oracle's columns exactly (dyadic coefficients), FP32 and FP64
+    (145 GB of vectors), both sign conventions; no other row is touched."""
     from pyhub_amd.engine import HeisenbergOp
     n = 36
     Jm = _j1j2_6x6()
@@ -250,9 +253,9 @@ def test_config5_heis6x6_sparse_columns_exact(ctx, conv):
             expect[r] = expect.get(r, 0.0) + a * val
     rows = np.array(sorted(expect), dtype=np.int64)
     want = np.array([expect[r] for r in rows])
-    v = torch.zeros(D, dtype=torch.float32, device="cuda")
+    v = torch.zeros(D, dtype=dtype, device="cuda")
     v[torch.as_tensor(np.array(cols, dtype=np.int64), device="cuda")] = torch.as_tensor(
-        coef, dtype=torch.float32, device="cuda")
+        coef, dtype=dtype, device="cuda")
     w = op.hv(v)
     del v
     got = w[torch.as_tensor(rows, device="cuda")].double().cpu().numpy()
@@ -260,7 +263,7 @@ def test_config5_heis6x6_sparse_columns_exact(ctx, conv):
     nz = int(torch.count_nonzero(w).item())
     del w
     _free()
-    print(f"6x6 conv {conv}: {len(cols)} columns, {len(rows)} rows reached, max |diff| "
+    print(f"6x6 conv {conv} {dtype}: {len(cols)} columns, {len(rows)} rows reached, max |diff| "
           f"{np.max(np.abs(got - want)):.1e}, nonzeros {nz} (expected {nz_expect})", flush=True)
     assert np.array_equal(got, want)
     assert nz == nz_expect
