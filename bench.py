@@ -65,6 +65,8 @@ def main():
                     help="residual target ||H psi - E psi|| of every Lanczos form (FP64: a "
                          "stopping condition; mixed: FP64 corrections of psi)")
     ap.add_argument("--refine", type=int, default=4, help="at most this many FP64 corrections")
+    ap.add_argument("--fixed-steps", type=int, default=300,
+                    help="also run MIXED and PLAIN64 for exactly this many steps (0: skip)")
     ap.add_argument("--calib", action="store_true", help="add a 1 GiB device copy (counter calibration)")
     ap.add_argument("--workload", default="hubbard4x4",
                     choices=["hubbard4x4", "heis6x6", "batch256", "band4x4", "u4", "ops4x4", "grad64",
@@ -1026,7 +1028,45 @@ def _lanczos_bench(op, args, L, torch):
     res["speedup_mixed_vs_fp64_plain"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
     res["speedup_mixed_plain_vs_fp64_plain"] = res["fp64"]["seconds"] / res["mixed_plain"]["seconds"]
     res["speedup_mixed_plain_vs_fp64_full"] = res["fp64_full"]["seconds"] / res["mixed_plain"]["seconds"]
+    if args.fixed_steps > 0:
+        res["fixed_steps"] = _lanczos_fixed(op, args, L, torch, v0, rq["fp64_full"], nn,
+                                            res["fp64_full"]["e0"])
     return res
+
+
+def _lanczos_fixed(op, args, L, torch, v0, rq64, nn, e64):
+    """The configuration's exact step count (300) without a stopping rule: MIXED (FP32
+    basis with full reorthogonalisation, FP64 scalars, then at most 2 FP64 corrections of
+    psi) and PLAIN64 (the reference's FP64 three-term recurrence).  FULL64 at 300 steps
+    needs 397 GB and does not fit, so both are compared with the converged FULL64 run."""
+    steps = args.fixed_steps
+    runs = (("mixed", L.LCZ_MIXED, 2), ("fp64", L.LCZ_PLAIN64, 0))
+    free, _ = torch.cuda.mem_get_info(op.ctx.device)
+    need = max(op.lanczos_workspace_bytes(m, steps, steps, rs, True) for _, m, rs in runs)
+    ws = torch.empty(min(need, int(0.9 * free)), dtype=torch.uint8, device=op.ctx.device)
+    out = {"steps": steps, "workspace_gb": ws.numel() / 1e9}
+    up64, dn64, ni64, ss64 = rq64
+    nn64 = float(np.mean([ss64[i, j] for i, j in nn]))
+    for name, mode, rs in runs:
+        torch.cuda.synchronize()
+        r = op.lanczos(mode=mode, max_steps=steps, fixed_steps=steps, tol=args.lanczos_tol,
+                       v0=v0, refine_steps=rs, want_psi=True, workspace=ws,
+                       psi_tol=args.psi_tol)
+        torch.cuda.synchronize()
+        up, dn, ni, ss = op.rdm(r.psi)
+        snn = float(np.mean([ss[i, j] for i, j in nn]))
+        out[name] = {"e0": r.e0, "steps": r.steps, "seconds": r.seconds, "residual": r.residual,
+                     "fp64_corrections": r.corrections,
+                     "rel_err_vs_converged_fp64_full": abs(r.e0 - e64) / abs(e64),
+                     "one_rdm_max_abs_err_vs_converged_fp64_full":
+                         float(max(np.max(np.abs(up - up64)), np.max(np.abs(dn - dn64)))),
+                     "nn_sisj": snn, "nn_sisj_abs_err_vs_converged_fp64_full": abs(snn - nn64)}
+        print(f"# lanczos {name}, {steps} fixed steps: E0={r.e0:.12f} {r.seconds:.2f}s "
+              f"res={r.residual:.2e}", file=sys.stderr, flush=True)
+        del r
+    del ws
+    torch.cuda.empty_cache()
+    return out
 
 
 def _cpu_baseline(T, U, D, b_alg, ref_run=None):
