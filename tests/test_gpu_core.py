@@ -76,7 +76,7 @@ def test_hubbard_dense_matrix_bit_exact(ctx, n, nu, nd, seed):
     assert np.array_equal(Hg, Ho)
 
 
-@pytest.mark.parametrize("tile", list(range(14)))
+@pytest.mark.parametrize("tile", list(range(17)))
 def test_hubbard_down_tiles_vs_oracle(ctx, tile, monkeypatch):
     """Every down-hop tile the H*v tuner can pick gives the oracle's H*v (FP64 to
     1e-13, FP32 to 2e-5 relative), on a sector whose dimensions are not
