@@ -424,6 +424,131 @@ __global__ __launch_bounds__(256) void k_hv_dn(HubArgs<T> a, const T *__restrict
     if (EPI) block_sum2<T>(s_xo, s_oo, part);
 }
 
+// Down hops with buffer loads (k_hv_dn2): one row per wave, VW columns per lane spaced
+// 64 apart, the same XCD-aware slabs as k_hv_dn.  A hop entry carries the source row's
+// byte offset in the state and its coefficient (DnEnt), so it is loaded by the scalar
+// unit straight into the SGPRs that the gathers use as their buffer soffset and the FMAs
+// as their operand: no per-entry decode.  Entries are taken UN at a time with two groups
+// in flight (the gathers of group g + 1 are issued before group g is summed).
+template <typename T>
+__device__ __forceinline__ T dn_bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+    else
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+
+template <typename T, bool EPI, int ROWS, int VW, int UN>
+__global__ __launch_bounds__(256) void k_hv_dn2(HubArgs<T> a, const DnEnt<T> *__restrict__ ent,
+                                                uint32_t nrec, const T *__restrict__ x,
+                                                T *__restrict__ out, double scale,
+                                                const T *__restrict__ prev, double beta,
+                                                double *__restrict__ part, int nchunks, int nrb) {
+    constexpr int COLS = 64 * VW;
+    const int b = blockIdx.x;
+    const int xcd = b & 7, i = b >> 3;
+    const int chunk = (i / nrb) * 8 + xcd;
+    const int rb = i % nrb;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t nu = a.dim_up;
+    const int64_t col0 = (int64_t)chunk * COLS + lane;
+    bool ok[VW];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < VW; ++j) {
+        ok[j] = (chunk < nchunks) && (col0 + 64 * j < nu);
+        any |= ok[j];
+    }
+    const double sc = (EPI && a.dsc) ? a.dsc[0] : scale, bt = (EPI && a.dsc) ? a.dsc[1] : beta;
+    double s_xo = 0.0, s_oo = 0.0;
+    if (any) {
+        // columns past the end read column col0 (in range) and are not stored
+        int64_t cj[VW];
+        uint32_t cb[VW];
+#pragma unroll
+        for (int j = 0; j < VW; ++j) {
+            cj[j] = ok[j] ? col0 + 64 * j : col0;
+            cb[j] = (uint32_t)cj[j] * (uint32_t)sizeof(T);
+        }
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, (int)nrec, 0x00020000);
+        using ci32 = const __attribute__((address_space(4))) int32_t;
+        using cE = const __attribute__((address_space(4))) DnEnt<T>;
+        ci32 *rp = (ci32 *)a.rp_dn;
+        cE *E = (cE *)ent;
+        const int wv = __builtin_amdgcn_readfirstlane(wave);
+        for (int rr = wv; rr < ROWS; rr += 4) {
+            const int64_t d = (int64_t)rb * ROWS + rr;
+            if (d >= a.dim_dn) break;
+            const int32_t r0 = rp[d], r1 = rp[d + 1];
+            T acc[VW];
+#pragma unroll
+            for (int j = 0; j < VW; ++j) acc[j] = (T)0;
+            auto issue = [&](int32_t k, T (&v)[UN][VW], T (&c)[UN]) {
+#pragma unroll
+                for (int q = 0; q < UN; ++q) {
+                    const uint32_t off = E[k + q].off;
+                    c[q] = E[k + q].c;
+#pragma unroll
+                    for (int j = 0; j < VW; ++j) v[q][j] = dn_bload<T>(rs, cb[j], off);
+                }
+            };
+            auto consume = [&](const T (&v)[UN][VW], const T (&c)[UN]) {
+                // keep the gathers just issued ahead of these sums
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int q = 0; q < UN; ++q)
+#pragma unroll
+                    for (int j = 0; j < VW; ++j) acc[j] += c[q] * v[q][j];
+            };
+            const int nfull = (r1 - r0) / UN;
+            if (nfull > 0) {
+                // the steady-state body issues unconditionally, so the wait before each
+                // sum leaves the other group's gathers in flight
+                T va[UN][VW], vb[UN][VW], ca[UN], cbq[UN];
+                int g = 0;
+                issue(r0, va, ca);
+                while (g + 2 < nfull) {
+                    issue(r0 + (g + 1) * UN, vb, cbq);
+                    consume(va, ca);
+                    issue(r0 + (g + 2) * UN, va, ca);
+                    consume(vb, cbq);
+                    g += 2;
+                }
+                if (g + 1 < nfull) {
+                    issue(r0 + (g + 1) * UN, vb, cbq);
+                    consume(va, ca);
+                    consume(vb, cbq);
+                } else {
+                    consume(va, ca);
+                }
+            }
+            for (int32_t k = r0 + nfull * UN; k < r1; ++k) {
+                const uint32_t off = E[k].off;
+                const T c = E[k].c;
+#pragma unroll
+                for (int j = 0; j < VW; ++j) acc[j] += c * dn_bload<T>(rs, cb[j], off);
+            }
+#pragma unroll
+            for (int j = 0; j < VW; ++j) {
+                if (!ok[j]) continue;
+                const size_t o = (size_t)d * nu + cj[j];
+                if (EPI) {
+                    double val = sc * ((double)__builtin_nontemporal_load(out + o) + (double)acc[j]);
+                    if (prev) val -= bt * (double)__builtin_nontemporal_load(prev + o);
+                    const T vt = (T)val;
+                    __builtin_nontemporal_store(vt, out + o);
+                    s_xo += sc * (double)x[o] * (double)vt;
+                    s_oo += (double)vt * (double)vt;
+                } else {
+                    __builtin_nontemporal_store(__builtin_nontemporal_load(out + o) + acc[j], out + o);
+                }
+            }
+        }
+    }
+    if (EPI) block_sum2<T>(s_xo, s_oo, part);
+}
+
 // Staged Heisenberg H*v (replaces HPROD, hprod_heis.f90:1-61).  A state is (xh, xl),
 // xh the nh top bits and xl the L low bits.  The states of one xl form a contiguous
 // block of C(nh, m) states (m = n/2 - popc(xl)), state xh at position j(xh) from the
@@ -1103,28 +1228,34 @@ static HubArgs<T> hub_args(const phub_op *op, const phub_ctx *c = nullptr) {
 
 constexpr int HUB_R = 4;
 
-// Candidate tiles of the down-hop kernel: (columns per row segment, rows per
-// workgroup, columns per lane).  FP32 and FP64 list the same byte widths, from
-// 64 B to 1 KB per row segment.
-// un: row gathers in flight per lane (each of vw elements)
+// Candidate tiles of the down-hop kernels: (columns per row segment, rows per
+// workgroup, columns per lane, row gathers in flight per lane, kernel).  Kernel 0 is
+// k_hv_dn (compressed CSR entries, any tile); kernel 1 is k_hv_dn2 (byte-offset
+// entries, buffer loads, one row per wave: columns = 64 * columns per lane).  FP32 and
+// FP64 list the same byte widths, from 64 B to 1 KB per row segment.
 struct DnTile {
-    int cols, rows, vw, un;
+    int cols, rows, vw, un, v2;
 };
-// The last three keep fewer rows per workgroup: the rows in flight on an XCD then
-// span a narrower window of the state, so more of their down-hop sources are still in
-// that XCD's L2 (4x4 FP32: 512 B x 4 rows, 8 in flight, 1.63 ms against 1.69 ms at
-// 16 rows).
-constexpr int DN_NCFG = 17;
+// The three with 4 or 8 rows per workgroup keep fewer rows in flight per XCD: their
+// down-hop sources then span a narrower window of the state, more of it still in that
+// XCD's L2 (4x4 FP32: 512 B x 4 rows, 8 in flight, 1.63 ms against 1.69 ms at 16 rows).
+constexpr int DN_NCFG = 25;
 constexpr DnTile DN_F32[DN_NCFG] = {
-    {16, 16, 1, 8},  {32, 16, 1, 8},  {32, 32, 1, 8},  {64, 8, 1, 8},   {64, 16, 1, 8},
-    {64, 32, 1, 8},  {128, 16, 2, 4}, {128, 32, 2, 4}, {256, 16, 4, 2}, {256, 32, 4, 2},
-    {128, 16, 2, 8}, {128, 32, 2, 8}, {256, 16, 4, 4}, {256, 32, 4, 4},
-    {128, 4, 2, 8},  {128, 8, 2, 8},  {256, 4, 4, 4}};
+    {16, 16, 1, 8, 0},  {32, 16, 1, 8, 0},  {32, 32, 1, 8, 0},  {64, 8, 1, 8, 0},
+    {64, 16, 1, 8, 0},  {64, 32, 1, 8, 0},  {128, 16, 2, 4, 0}, {128, 32, 2, 4, 0},
+    {256, 16, 4, 2, 0}, {256, 32, 4, 2, 0}, {128, 16, 2, 8, 0}, {128, 32, 2, 8, 0},
+    {256, 16, 4, 4, 0}, {256, 32, 4, 4, 0}, {128, 4, 2, 8, 0},  {128, 8, 2, 8, 0},
+    {256, 4, 4, 4, 0},
+    {64, 4, 1, 8, 1},   {64, 8, 1, 8, 1},   {64, 16, 1, 8, 1},  {128, 4, 2, 8, 1},
+    {128, 8, 2, 8, 1},  {128, 16, 2, 4, 1}, {256, 4, 4, 4, 1},  {256, 8, 4, 4, 1}};
 constexpr DnTile DN_F64[DN_NCFG] = {
-    {8, 16, 1, 8},   {16, 16, 1, 8},  {16, 32, 1, 8},  {32, 8, 1, 8},   {32, 16, 1, 8},
-    {32, 32, 1, 8},  {64, 16, 2, 4},  {64, 32, 2, 4},  {128, 16, 4, 2}, {128, 32, 4, 2},
-    {64, 16, 2, 8},  {64, 32, 2, 8},  {128, 16, 4, 4}, {128, 32, 4, 4},
-    {64, 4, 2, 8},   {64, 8, 2, 8},   {128, 4, 4, 4}};
+    {8, 16, 1, 8, 0},   {16, 16, 1, 8, 0},  {16, 32, 1, 8, 0},  {32, 8, 1, 8, 0},
+    {32, 16, 1, 8, 0},  {32, 32, 1, 8, 0},  {64, 16, 2, 4, 0},  {64, 32, 2, 4, 0},
+    {128, 16, 4, 2, 0}, {128, 32, 4, 2, 0}, {64, 16, 2, 8, 0},  {64, 32, 2, 8, 0},
+    {128, 16, 4, 4, 0}, {128, 32, 4, 4, 0}, {64, 4, 2, 8, 0},   {64, 8, 2, 8, 0},
+    {128, 4, 4, 4, 0},
+    {64, 4, 1, 8, 1},   {64, 8, 1, 8, 1},   {64, 16, 1, 8, 1},  {128, 4, 2, 4, 1},
+    {128, 8, 2, 8, 1},  {128, 16, 2, 4, 1}, {256, 4, 4, 2, 1},  {256, 8, 4, 4, 1}};
 constexpr int DN_DEFAULT = 4;   // 256 B segments, 16 rows
 
 template <typename T>
@@ -1135,7 +1266,61 @@ constexpr DnTile dn_tile(int cfg) {
 int phub_dn_tile_code(int cfg, bool fp64) {
     if (cfg < 0) return -1;
     const DnTile t = fp64 ? DN_F64[cfg] : DN_F32[cfg];
-    return t.cols * (fp64 ? 8 : 4) * 100000 + t.rows * 100 + t.un;
+    return t.cols * (fp64 ? 8 : 4) * 100000 + t.rows * 100 + t.un + (t.v2 ? 50 : 0);
+}
+
+// byte-offset entries from the row-major down table: entry k of row s (CSR order)
+template <typename T>
+__global__ void k_dn_ent(const int32_t *__restrict__ idx, const double *__restrict__ c64,
+                         const int32_t *__restrict__ cnt, const int32_t *__restrict__ rowptr,
+                         int64_t dim, int maxh, uint64_t rowbytes, DnEnt<T> *__restrict__ ent) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= dim) return;
+    const int c = cnt[s];
+    const int32_t base = rowptr[s];
+    for (int k = 0; k < c; ++k) {
+        const size_t o = (size_t)s * maxh + k;
+        DnEnt<T> e{};
+        e.off = (uint32_t)((uint64_t)idx[o] * rowbytes);
+        e.c = (T)c64[o];
+        ent[base + k] = e;
+    }
+}
+
+int phub_build_dn_ent(phub_ctx *c, phub_op *op) {
+    const HopTable &t = op->dn;
+    if (!t.rowptr || t.dim <= 0 || t.maxh <= 0) return PHUB_OK;
+    int32_t nnz = 0;
+    PHUB_HIP(c, phub_copy(c, &nnz, t.rowptr + t.dim, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (nnz <= 0) return PHUB_OK;
+    for (int ti = 0; ti < 2; ++ti) {
+        const uint64_t es = ti ? 8 : 4;
+        if ((uint64_t)op->dim_up * (uint64_t)op->dim_dn * es >= (1ull << 32)) continue;
+        const uint64_t rowbytes = (uint64_t)op->dim_up * es;
+        if (ti == 0) {
+            DnEnt<float> *e = nullptr;
+            PHUB_HIP(c, phub_malloc(&e, (size_t)nnz * sizeof(DnEnt<float>)));
+            op->dn_ent[0] = e;
+            hipLaunchKernelGGL(k_dn_ent<float>, dim3((unsigned)((t.dim + 255) / 256)), dim3(256), 0, c->stream, t.idx,
+                               t.c64, t.cnt, t.rowptr, t.dim, t.maxh, rowbytes, e);
+        } else {
+            DnEnt<double> *e = nullptr;
+            PHUB_HIP(c, phub_malloc(&e, (size_t)nnz * sizeof(DnEnt<double>)));
+            op->dn_ent[1] = e;
+            hipLaunchKernelGGL(k_dn_ent<double>, dim3((unsigned)((t.dim + 255) / 256)), dim3(256), 0, c->stream, t.idx,
+                               t.c64, t.cnt, t.rowptr, t.dim, t.maxh, rowbytes, e);
+        }
+        PHUB_HIP(c, hipGetLastError());
+        op->table_bytes += (double)nnz * (ti ? sizeof(DnEnt<double>) : sizeof(DnEnt<float>));
+    }
+    PHUB_HIP(c, hipStreamSynchronize(c->stream));
+    return PHUB_OK;
+}
+
+// the buffer-load kernel needs the byte-offset entries (a state below 4 GiB)
+template <typename T>
+static bool dn_tile_ok(const phub_op *op, int cfg) {
+    return !dn_tile<T>(cfg).v2 || op->dn_ent[sizeof(T) == 8];
 }
 
 template <typename T, bool EPI, int CFG>
@@ -1145,37 +1330,40 @@ static void dn_go(phub_ctx *c, const phub_op *op, const HubArgs<T> &a, const T *
     const int nchunks = (int)((op->dim_up + t.cols - 1) / t.cols);
     const int nrb = (int)((op->dim_dn + t.rows - 1) / t.rows);
     *nblocks = (int)std::min<int64_t>((int64_t)8 * ((nchunks + 7) / 8) * nrb, INT32_MAX);
-    hipLaunchKernelGGL((k_hv_dn<T, EPI, t.cols, t.rows, t.vw, t.un>), dim3(*nblocks), dim3(256), 0,
-                       c->stream, a, x, out, scale, prev, beta, c->d_red, nchunks, nrb);
+    if constexpr (t.v2) {
+        static_assert(t.cols == 64 * t.vw, "one row per wave");
+        const uint32_t nrec = (uint32_t)std::min<uint64_t>((uint64_t)op->dim * sizeof(T), 0xffffffffull);
+        hipLaunchKernelGGL((k_hv_dn2<T, EPI, t.rows, t.vw, t.un>), dim3(*nblocks), dim3(256), 0,
+                           c->stream, a, (const DnEnt<T> *)op->dn_ent[sizeof(T) == 8], nrec, x, out,
+                           scale, prev, beta, c->d_red, nchunks, nrb);
+    } else {
+        hipLaunchKernelGGL((k_hv_dn<T, EPI, t.cols, t.rows, t.vw, t.un>), dim3(*nblocks), dim3(256), 0,
+                           c->stream, a, x, out, scale, prev, beta, c->d_red, nchunks, nrb);
+    }
 }
+
+#define PHUB_DN_CASE(k) \
+    case k: dn_go<T, EPI, k>(c, op, a, x, out, scale, prev, beta, nb); break;
 
 template <typename T, bool EPI>
 static void dn_dispatch(phub_ctx *c, const phub_op *op, const HubArgs<T> &a, const T *x, T *out,
                         double scale, const T *prev, double beta, int cfg, int *nb) {
+    static_assert(DN_NCFG == 25, "dispatch cases");
     switch (cfg) {
-    case 0: dn_go<T, EPI, 0>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 1: dn_go<T, EPI, 1>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 2: dn_go<T, EPI, 2>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 3: dn_go<T, EPI, 3>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 4: dn_go<T, EPI, 4>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 5: dn_go<T, EPI, 5>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 6: dn_go<T, EPI, 6>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 7: dn_go<T, EPI, 7>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 8: dn_go<T, EPI, 8>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 9: dn_go<T, EPI, 9>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 10: dn_go<T, EPI, 10>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 11: dn_go<T, EPI, 11>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 12: dn_go<T, EPI, 12>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 13: dn_go<T, EPI, 13>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 14: dn_go<T, EPI, 14>(c, op, a, x, out, scale, prev, beta, nb); break;
-    case 15: dn_go<T, EPI, 15>(c, op, a, x, out, scale, prev, beta, nb); break;
-    default: dn_go<T, EPI, 16>(c, op, a, x, out, scale, prev, beta, nb); break;
+    PHUB_DN_CASE(0) PHUB_DN_CASE(1) PHUB_DN_CASE(2) PHUB_DN_CASE(3) PHUB_DN_CASE(4)
+    PHUB_DN_CASE(5) PHUB_DN_CASE(6) PHUB_DN_CASE(7) PHUB_DN_CASE(8) PHUB_DN_CASE(9)
+    PHUB_DN_CASE(10) PHUB_DN_CASE(11) PHUB_DN_CASE(12) PHUB_DN_CASE(13) PHUB_DN_CASE(14)
+    PHUB_DN_CASE(15) PHUB_DN_CASE(16) PHUB_DN_CASE(17) PHUB_DN_CASE(18) PHUB_DN_CASE(19)
+    PHUB_DN_CASE(20) PHUB_DN_CASE(21) PHUB_DN_CASE(22) PHUB_DN_CASE(23)
+    default: dn_go<T, EPI, 24>(c, op, a, x, out, scale, prev, beta, nb); break;
     }
 }
+#undef PHUB_DN_CASE
 
 template <typename T>
 static int launch_dn(phub_ctx *c, const phub_op *op, const HubArgs<T> &a, const T *x, T *out,
                      double scale, const T *prev, double beta, bool epi, int cfg, int *nblocks) {
+    if (!dn_tile_ok<T>(op, cfg)) cfg = DN_DEFAULT;
     const DnTile t = dn_tile<T>(cfg);
     const int cols = t.cols, rows = t.rows;
     const int64_t nchunks = (op->dim_up + cols - 1) / cols, nrb = (op->dim_dn + rows - 1) / rows;
@@ -1224,6 +1412,7 @@ static int tune_dn(phub_ctx *c, phub_op *op, const HubArgs<T> &a, const T *x, T 
     float best = 1e30f;
     int bestc = DN_DEFAULT;
     for (int cfg = 0; cfg < DN_NCFG; ++cfg) {
+        if (!dn_tile_ok<T>(op, cfg)) continue;
         float tmin = 1e30f;
         for (int rep = 0; rep < 2; ++rep) {
             hipLaunchKernelGGL((k_hv_up<T, R>), dim3(g1), dim3(UP_THREADS), lds, c->stream, a,
@@ -1239,9 +1428,11 @@ static int tune_dn(phub_ctx *c, phub_op *op, const HubArgs<T> &a, const T *x, T 
             tmin = std::min(tmin, ms);
         }
         if (std::getenv("PHUB_TUNE_VERBOSE"))
-            std::fprintf(stderr, "phub tune: %s down tile %d: %d cols x %d rows (%d per lane, %d in flight): %.3f ms\n",
-                         sizeof(T) == 4 ? "fp32" : "fp64", cfg, dn_tile<T>(cfg).cols,
-                         dn_tile<T>(cfg).rows, dn_tile<T>(cfg).vw, dn_tile<T>(cfg).un, tmin);
+            std::fprintf(stderr, "phub tune: %s down tile %d: %s %d cols x %d rows (%d per lane, %d in flight): %.3f ms\n",
+                         sizeof(T) == 4 ? "fp32" : "fp64", cfg,
+                         dn_tile<T>(cfg).v2 ? "k_hv_dn2" : "k_hv_dn",
+                         dn_tile<T>(cfg).cols, dn_tile<T>(cfg).rows, dn_tile<T>(cfg).vw,
+                         dn_tile<T>(cfg).un, tmin);
         if (tmin < best) { best = tmin; bestc = cfg; }
     }
     op->tune_dn[ti] = bestc;

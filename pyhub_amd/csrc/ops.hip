@@ -69,6 +69,8 @@ extern "C" int phub_op_hubbard(phub_ctx *c, const phub_hubbard_desc *d, phub_op 
         return (double)t.dim * (4.0 + 8.0 + 4.0) + (double)t.dim * t.maxh * (4.0 + 8.0 + 4.0);
     };
     op->table_bytes = tb(op->up) + tb(op->dn);
+    rc = phub_build_dn_ent(c, op);
+    if (rc) { phub_op_destroy(op); return rc; }
     if (d->u4) {
         // UPARTIAL(J,K) = sum_{MN} UL(J,K,M,N)^2; a term enters when UPARTIAL(J,K) >= 1e-14
         // and |UL(J,K,M,N)| >= 1e-14 (hprod_hubbard.f90:62-65): keep exactly those
@@ -126,6 +128,9 @@ extern "C" void phub_op_destroy(phub_op *op) {
     phub_free_hop_table(&op->rup);
     phub_free_hop_table(&op->rdn);
     if (op->dn_pairmap) hipFree(op->dn_pairmap);
+    for (void *p : op->dn_ent)
+        if (p) hipFree(p);
+
     phub_free_heis(&op->h);
     if (op->d_ul) hipFree(op->d_ul);
     if (op->heis_all) phub_op_destroy(op->heis_all);

@@ -176,6 +176,21 @@ struct HeisTables {
     HeisStage stg;                 // tables of the H*v
 };
 
+// A down-hop entry of the buffer-load kernel: the byte offset of the source row in the
+// state (source index * dim_up * sizeof(T), below 2^32) and the signed coefficient.
+template <typename T>
+struct DnEnt;
+template <>
+struct DnEnt<float> {
+    uint32_t off;
+    float c;
+};
+template <>
+struct DnEnt<double> {
+    uint32_t off, pad;
+    double c;
+};
+
 struct phub_op {
     int kind = 0;                  // 0 Hubbard, 1 Heisenberg
     int n = 0, nup = 0, ndn = 0;
@@ -199,6 +214,10 @@ struct phub_op {
     int sign_convention = 0;
     double table_bytes = 0.0;
     phub_ctx *ctx = nullptr;
+    // down-hop entries with the source row's byte offset in the state (CSR order of
+    // dn.rowptr), per precision (FP32 DnEnt<float>, FP64 DnEnt<double>); null when the
+    // state's bytes do not fit 32 bits or the down table has no CSR
+    void *dn_ent[2] = {nullptr, nullptr};
     // down-hop tile chosen by the H*v tuner, per precision (FP32, FP64); -1 = not yet
     int tune_dn[2] = {-1, -1};
     // Heisenberg work-item order chosen by the tuner (0 XCD-contiguous, 1 round robin)
@@ -253,6 +272,8 @@ struct HopBatch {
 };
 int phub_build_hop_tables(phub_ctx *ctx, const std::vector<HopSpec> &specs, HopBatch *out);
 void phub_free_hop_batch(HopBatch *b);
+// down-hop entries of the buffer-load down kernel (op->dn_ent), both precisions (hv.hip)
+int phub_build_dn_ent(phub_ctx *ctx, phub_op *op);
 int phub_build_heis(phub_ctx *ctx, int n, const double *J_host, int sign_convention,
                     HeisTables *out);
 void phub_free_heis(HeisTables *t);

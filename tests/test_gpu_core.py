@@ -76,7 +76,10 @@ def test_hubbard_dense_matrix_bit_exact(ctx, n, nu, nd, seed):
     assert np.array_equal(Hg, Ho)
 
 
-@pytest.mark.parametrize("tile", list(range(17)))
+N_DN_TILES = 25   # the down-hop tuner's candidates: 17 of k_hv_dn, 8 of k_hv_dn2
+
+
+@pytest.mark.parametrize("tile", list(range(N_DN_TILES)))
 def test_hubbard_down_tiles_vs_oracle(ctx, tile, monkeypatch):
     """Every down-hop tile the H*v tuner can pick gives the oracle's H*v (FP64 to
     1e-13, FP32 to 2e-5 relative), on a sector whose dimensions are not
@@ -94,6 +97,25 @@ def test_hubbard_down_tiles_vs_oracle(ctx, tile, monkeypatch):
     assert np.max(np.abs(w64 - ref)) <= 1e-13 * max(1.0, np.max(np.abs(ref)))
     w32 = op.hv(dev(v, torch.float32)).double().cpu().numpy()
     assert np.max(np.abs(w32 - ref)) <= 2e-5 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("tile", list(range(N_DN_TILES)))
+def test_hubbard_down_tiles_lanczos_vs_dense(ctx, tile, monkeypatch):
+    """The fused Lanczos epilogue of every down-hop tile (scale, beta * prev and the
+    dot partials): FP64 three-term Lanczos E0 against the dense eigenvalue of the
+    oracle's matrix to 1e-10 relative, on a sector of 1,568 states."""
+    from pyhub_amd import _lib as L
+    from pyhub_amd.engine import HubbardOp
+    monkeypatch.setenv("PHUB_DN_TILE", str(tile))
+    rng = np.random.default_rng(170 + tile)
+    n, nu, nd = 8, 3, 2
+    T, U = rand_cluster(rng, n)
+    op = HubbardOp(ctx, n, nu, nd, T, U)
+    m = O.HubbardModel(n, nu, nd, T, U)
+    e_dense = float(np.linalg.eigvalsh(m.dense())[0])
+    v0 = rng.uniform(-0.5, 0.5, m.dim)
+    r = op.lanczos(mode=L.LCZ_PLAIN64, max_steps=400, tol=1e-12, v0=dev(v0))
+    assert abs(r.e0 - e_dense) <= 1e-10 * abs(e_dense)
 
 
 @pytest.mark.parametrize("n,conv,seed", [(4, 0, 0), (6, 0, 1), (8, 0, 2), (10, 0, 3), (6, 1, 4), (8, 1, 5)])
