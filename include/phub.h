@@ -124,7 +124,8 @@ typedef struct {
     double table_bytes;  /* device bytes of the hop / rank tables */
     int dn_tile[2];      /* Hubbard down-hop tile the H*v tuner chose, FP32 / FP64:
                             (bytes per row segment) * 100000 + (rows per workgroup) * 100
-                            + (row gathers in flight per lane); -1 = not yet tuned */
+                            + (row gathers in flight per lane), + 50 for the buffer-load
+                            kernel; -1 = not yet tuned */
 } phub_op_info;
 
 int phub_op_hubbard(phub_ctx *ctx, const phub_hubbard_desc *desc, phub_op **out);

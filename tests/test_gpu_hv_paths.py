@@ -86,7 +86,7 @@ def test_fp64_one_row_forced(ctx, monkeypatch, lattice):
     _check(op, O.HubbardModel(n, nu, nd, T, U), rng)
 
 
-@pytest.mark.parametrize("tile", list(range(17)))
+@pytest.mark.parametrize("tile", list(range(25)))
 @pytest.mark.parametrize("lattice", [True, False])
 def test_fp64_one_row_large_up_sector_all_tiles(ctx, monkeypatch, tile, lattice):
     """n = 16, 7 up and 1 down (dim_up = 11,440: two FP64 rows do not fit LDS, so
@@ -103,3 +103,4 @@ def test_fp64_one_row_large_up_sector_all_tiles(ctx, monkeypatch, tile, lattice)
     U = rng.uniform(2, 6, n)
     op = HubbardOp(ctx, n, nu, nd, T, U)
     _check(op, O.HubbardModel(n, nu, nd, T, U), rng)
+
