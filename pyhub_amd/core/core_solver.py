@@ -23,6 +23,31 @@ from pyhub_amd.engine import Context, HeisenbergOp, HubbardOp
 
 _MODES = {"plain64": L.LCZ_PLAIN64, "full64": L.LCZ_FULL64, "mixed": L.LCZ_MIXED,
           "plain32": L.LCZ_PLAIN32}
+# relative distance within which a Ritz value of the three-term recurrence is a copy of
+# a lower one (FP64 / FP32 vectors); the full-reorthogonalisation modes keep repeats
+_GHOST_TOL = {"plain64": 1e-8, "plain32": 1e-5}
+
+
+def _genuine_ritz(val, u1, tol):
+    """Positions of the genuine values among the ascending Ritz values of the three-term
+    recurrence (u1: first components of their tridiagonal eigenvectors).  Without
+    reorthogonalisation, rounding adds copies of converged values (Paige's ghosts) and,
+    while a copy forms, values with no weight on the start vector (Cullum and
+    Willoughby's spurious values); either would enter the spectrum as an excited state.
+    Values within tol (relative) of a cluster's first member form one cluster, kept once
+    through its member of largest |u1|; a value alone with |u1| < 1e-10 is dropped.  The
+    same rule as the engine's excited-state selection."""
+    keep, i, m = [], 0, len(val)
+    while i < m:
+        j, best = i + 1, i
+        while j < m and val[j] - val[i] <= tol * max(1.0, abs(val[j])):
+            if abs(u1[j]) > abs(u1[best]):
+                best = j
+            j += 1
+        if j - i > 1 or abs(u1[i]) >= 1e-10:
+            keep.append(best)
+        i = j
+    return np.asarray(keep, dtype=int)
 
 
 def boltzmann_weights(val, T, nb_comp_states):
@@ -165,6 +190,9 @@ class CoreSolver(Basis, GreenFunction, StaticQuantities):
                     print(" FORTRAN LANCZOS ERROR: NORM LOST", flush=True)
                 m = len(out.alpha)
                 val, vec = eigh_tridiagonal(out.alpha, out.beta[1:m]) if m > 1 else (out.alpha.copy(), np.ones((1, 1)))
+                if self.lanczos_mode in _GHOST_TOL:
+                    keep = _genuine_ritz(val, vec[0], _GHOST_TOL[self.lanczos_mode])
+                    val, vec = val[keep], vec[:, keep]
                 nw = len(boltzmann_weights(val, self.T, self.nb_comp_states_))
                 if nw <= nst or nst >= m:
                     break

@@ -1226,6 +1226,43 @@ static HubArgs<T> hub_args(const phub_op *op, const phub_ctx *c = nullptr) {
     return a;
 }
 
+// minv[d * dim_up + u] = 1 / max(H_ii - e, floor): the inverse shifted diagonal of the
+// one-body Hubbard operator (the diagonal the H*v kernels add: eloc of both species
+// plus U over the doubly occupied sites), a Jacobi preconditioner for H - e near the
+// ground state (every H_ii >= e0, so the entries are positive)
+__global__ __launch_bounds__(256) void k_hub_minv(HubArgs<float> a, double e, double floor_,
+                                                 float *__restrict__ minv) {
+    const int64_t n = a.dim_up * a.dim_dn;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t d = i / a.dim_up, u = i - d * a.dim_up;
+        const uint32_t both = a.mu[u] & a.md[d];
+        double dg = a.eu[u] + a.ed[d];
+        if (a.u_uniform) {
+            dg += a.u0 * (double)__popc(both);
+        } else {
+            uint32_t bb = both;
+            while (bb) {
+                const int b = __ffs(bb) - 1;
+                dg += a.U[a.n - 1 - b];
+                bb &= bb - 1;
+            }
+        }
+        minv[i] = (float)(1.0 / fmax(dg - e, floor_));
+    }
+}
+
+int phub_hub_minv(phub_ctx *c, const phub_op *op, double e, float *minv) {
+    if (op->kind != 0 || op->d_ul)
+        return phub_fail(c, PHUB_ERR_UNSUPPORTED, "diagonal preconditioner: one-body Hubbard operators only");
+    const int64_t n = op->dim;
+    const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_hub_minv, dim3(g), dim3(256), 0, c->stream, hub_args<float>(op), e, 1e-3,
+                       minv);
+    PHUB_HIP(c, hipGetLastError());
+    return PHUB_OK;
+}
+
 constexpr int HUB_R = 4;
 
 // Candidate tiles of the down-hop kernels: (columns per row segment, rows per

@@ -29,13 +29,16 @@
 // host tridiagonal eigensolver (implicit QL, Wilkinson shift)
 // ---------------------------------------------------------------------------
 // d[m] diagonal, e[m-1] off-diagonal.  Returns eigenvalues ascending in d; if z
-// is non-null it receives the eigenvectors, column j = z[j*m .. j*m+m-1].
-static int tridiag_ql(int m, double *d, const double *e_in, double *z) {
+// is non-null it receives the leading zr components (zr <= 0: all m) of the
+// eigenvectors, vector j = z[j*zr .. j*zr+zr-1] (zr = 1: their first components, in
+// O(m^2)).
+static int tridiag_ql(int m, double *d, const double *e_in, double *z, int zr = 0) {
+    if (zr <= 0 || zr > m) zr = m;
     std::vector<double> e(m + 1, 0.0);
     for (int i = 0; i + 1 < m; ++i) e[i] = e_in[i];
     if (z) {
-        std::fill(z, z + (size_t)m * m, 0.0);
-        for (int i = 0; i < m; ++i) z[(size_t)i * m + i] = 1.0;
+        std::fill(z, z + (size_t)m * zr, 0.0);
+        for (int i = 0; i < zr; ++i) z[(size_t)i * zr + i] = 1.0;
     }
     for (int l = 0; l < m; ++l) {
         int it = 0;
@@ -71,8 +74,8 @@ static int tridiag_ql(int m, double *d, const double *e_in, double *z) {
                 d[i + 1] = g + p;
                 g = c * r - b;
                 if (z) {
-                    double *zi = z + (size_t)i * m, *zi1 = z + (size_t)(i + 1) * m;
-                    for (int k = 0; k < m; ++k) {
+                    double *zi = z + (size_t)i * zr, *zi1 = z + (size_t)(i + 1) * zr;
+                    for (int k = 0; k < zr; ++k) {
                         double t = zi1[k];
                         zi1[k] = s * zi[k] + c * t;
                         zi[k] = c * zi[k] - s * t;
@@ -93,7 +96,7 @@ static int tridiag_ql(int m, double *d, const double *e_in, double *z) {
         if (k != i) {
             std::swap(d[i], d[k]);
             if (z)
-                for (int q = 0; q < m; ++q) std::swap(z[(size_t)i * m + q], z[(size_t)k * m + q]);
+                for (int q = 0; q < zr; ++q) std::swap(z[(size_t)i * zr + q], z[(size_t)k * zr + q]);
         }
     }
     return 0;
@@ -143,6 +146,16 @@ __device__ __forceinline__ double wave_sum(double a) {
     return a;
 }
 
+// 16-byte packs for the streaming passes: four FP32 or two FP64 elements per lane
+// access.  A pass takes the pack loop when every pointer it touches is 16-byte aligned
+// (a wave-uniform test) and finishes the last n mod W elements, or all of them when a
+// pointer is not aligned, element by element.
+template <typename T, int W>
+using vpk = T __attribute__((ext_vector_type(W)));
+template <typename V>
+constexpr int vpk_w() { return 16 / (int)sizeof(V); }
+__device__ __forceinline__ bool al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
 // per-block partial sums of a.b over a grid-stride range
 template <typename TA, typename TB>
 __global__ __launch_bounds__(256) void k_dot(const TA *__restrict__ a, const TB *__restrict__ b,
@@ -168,8 +181,31 @@ __global__ __launch_bounds__(256) void k_axpby(double a, const TX *__restrict__ 
     // dsc: a graph-captured Lanczos step takes a = -alpha_k * scale from device memory
     if (dsc) a = -dsc[2] * dsc[0];
     double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    int64_t i0 = 0;
+    if constexpr (sizeof(TX) == sizeof(TY)) {
+        constexpr int W = vpk_w<TY>();
+        using PX = vpk<TX, W>;
+        using PY = vpk<TY, W>;
+        if (al16(x) && al16(y)) {
+            const int64_t nq = n / W;
+            for (int64_t q = t0; q < nq; q += st) {
+                PY yv = reinterpret_cast<const PY *>(y)[q];
+                PX xv{};
+                if (x) xv = reinterpret_cast<const PX *>(x)[q];
+#pragma unroll
+                for (int e = 0; e < W; ++e) {
+                    const TY t = (TY)((x ? a * (double)xv[e] : 0.0) + b * (double)yv[e]);
+                    yv[e] = t;
+                    s += (double)t * (double)t;
+                }
+                reinterpret_cast<PY *>(y)[q] = yv;
+            }
+            i0 = nq * W;
+        }
+    }
+    for (int64_t i = i0 + t0; i < n; i += st) {
         double v = (x ? a * (double)x[i] : 0.0) + b * (double)y[i];
         TY t = (TY)v;
         y[i] = t;
@@ -213,24 +249,59 @@ __global__ __launch_bounds__(256) void k_store_vec(const T *__restrict__ src,
 
 // r = a * psi + b * y (psi FP64, y and r of type V)
 template <typename V>
-__global__ __launch_bounds__(256) void k_axpby_to(double a, const double *__restrict__ psi,
+__global__ __launch_bounds__(256) void k_axpby_to(double a, const V *__restrict__ psi,
                                                   double b, const V *__restrict__ y,
                                                   V *__restrict__ r, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        r[i] = (V)(a * psi[i] + b * (double)y[i]);
+        r[i] = (V)(a * (double)psi[i] + b * (double)y[i]);
 }
 
 // Adjoint MINRES, the vector work of one iteration in three passes.  Vectors of
 // type V (FP64 for the adjoint solve, FP32 for the loose correction solves of
-// jd_refine), psi and every sum in FP64.
+// jd_refine, which pass an FP32 copy of psi as well), every sum in FP64.
 // k_minres_scale: v = s * y.
 template <typename V>
 __global__ __launch_bounds__(256) void k_minres_scale(V *__restrict__ v, const V *__restrict__ y,
                                                       double s, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        v[i] = (V)(s * (double)y[i]);
+    constexpr int W = vpk_w<V>();
+    using P = vpk<V, W>;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nq = (al16(v) && al16(y)) ? n / W : 0;
+    for (int64_t q = t0; q < nq; q += st) {
+        const P a = reinterpret_cast<const P *>(y)[q];
+        P o;
+#pragma unroll
+        for (int e = 0; e < W; ++e) o[e] = (V)(s * (double)a[e]);
+        reinterpret_cast<P *>(v)[q] = o;
+    }
+    for (int64_t i = nq * W + t0; i < n; i += st) v[i] = (V)(s * (double)y[i]);
+}
+
+// preconditioned form: v = s * (minv * y - gam * psi), the preconditioned residual
+// P M^-1 y (y orthogonal to psi, gam = psi . (minv * y)) scaled by s
+template <typename V>
+__global__ __launch_bounds__(256) void k_minres_pscale(V *__restrict__ v, const V *__restrict__ y,
+                                                       const V *__restrict__ minv,
+                                                       const V *__restrict__ psi, double gam,
+                                                       double s, int64_t n) {
+    constexpr int W = vpk_w<V>();
+    using P = vpk<V, W>;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nq = (al16(v) && al16(y) && al16(minv) && al16(psi)) ? n / W : 0;
+    for (int64_t q = t0; q < nq; q += st) {
+        const P a = reinterpret_cast<const P *>(y)[q];
+        const P m = reinterpret_cast<const P *>(minv)[q];
+        const P p = reinterpret_cast<const P *>(psi)[q];
+        P o;
+#pragma unroll
+        for (int e = 0; e < W; ++e) o[e] = (V)(s * ((double)m[e] * (double)a[e] - gam * (double)p[e]));
+        reinterpret_cast<P *>(v)[q] = o;
+    }
+    for (int64_t i = nq * W + t0; i < n; i += st)
+        v[i] = (V)(s * ((double)minv[i] * (double)y[i] - gam * (double)psi[i]));
 }
 
 // after t = H v: t <- t - e0 v - c1 r1 (no r1 term when r1 is null); partials
@@ -241,22 +312,37 @@ __global__ __launch_bounds__(256) void k_minres_scale(V *__restrict__ v, const V
 template <typename V>
 __global__ __launch_bounds__(256) void k_minres_a(V *__restrict__ t, const V *__restrict__ v,
                                                   const V *__restrict__ r1,
-                                                  const double *__restrict__ psi, double e0,
+                                                  const V *__restrict__ psi, double e0,
                                                   double c1, int64_t n,
                                                   double *__restrict__ part) {
     __shared__ double sh[3][4];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const double vi = (double)v[i], pi = psi[i];
-        double ti = (double)t[i] - e0 * vi;
+    auto one = [&](double ti, double vi, double pi, double ri) -> V {
+        ti -= e0 * vi;
         s0 += pi * ti;
-        if (r1) ti -= c1 * (double)r1[i];
+        ti -= c1 * ri;
         const V tv = (V)ti;
-        t[i] = tv;
         s1 += vi * (double)tv;
         s2 += vi * pi;
+        return tv;
+    };
+    constexpr int W = vpk_w<V>();
+    using P = vpk<V, W>;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nq = (al16(t) && al16(v) && al16(r1) && al16(psi)) ? n / W : 0;
+    for (int64_t q = t0; q < nq; q += st) {
+        P tv = reinterpret_cast<const P *>(t)[q];
+        const P vv = reinterpret_cast<const P *>(v)[q];
+        const P pv = reinterpret_cast<const P *>(psi)[q];
+        P rv{};
+        if (r1) rv = reinterpret_cast<const P *>(r1)[q];
+#pragma unroll
+        for (int e = 0; e < W; ++e) tv[e] = one((double)tv[e], (double)vv[e], (double)pv[e], (double)rv[e]);
+        reinterpret_cast<P *>(t)[q] = tv;
     }
+    for (int64_t i = nq * W + t0; i < n; i += st)
+        t[i] = one((double)t[i], (double)v[i], (double)psi[i], r1 ? (double)r1[i] : 0.0);
     s0 = wave_sum(s0);
     s1 = wave_sum(s1);
     s2 = wave_sum(s2);
@@ -271,25 +357,57 @@ __global__ __launch_bounds__(256) void k_minres_a(V *__restrict__ t, const V *__
             sh[threadIdx.x][0] + sh[threadIdx.x][1] + sh[threadIdx.x][2] + sh[threadIdx.x][3];
 }
 
-// r = t - p psi - c2 r2 (r may not alias t, psi or r2); partial |r|^2
+// r = t - p psi - c2 r2 (r may not alias t, psi or r2); partials [block][2]: without
+// minv |r|^2 and 0, with it r . (minv r) and psi . (minv r)
 template <typename V>
 __global__ __launch_bounds__(256) void k_minres_b(const V *__restrict__ t,
-                                                  const double *__restrict__ psi,
+                                                  const V *__restrict__ psi,
                                                   const V *__restrict__ r2, double p, double c2,
+                                                  const V *__restrict__ minv,
                                                   V *__restrict__ r, int64_t n,
                                                   double *__restrict__ part) {
-    __shared__ double sh[4];
-    double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const V ri = (V)((double)t[i] - p * psi[i] - c2 * (double)r2[i]);
-        r[i] = ri;
-        s += (double)ri * (double)ri;
+    __shared__ double sh[2][4];
+    double s0 = 0.0, s1 = 0.0;
+    auto one = [&](double ti, double pi, double r2i, double mi) -> V {
+        const V ri = (V)(ti - p * pi - c2 * r2i);
+        const double rd = (double)ri;
+        if (minv) {
+            const double mr = mi * rd;
+            s0 += rd * mr;
+            s1 += pi * mr;
+        } else {
+            s0 += rd * rd;
+        }
+        return ri;
+    };
+    constexpr int W = vpk_w<V>();
+    using P = vpk<V, W>;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nq = (al16(t) && al16(psi) && al16(r2) && al16(minv) && al16(r)) ? n / W : 0;
+    for (int64_t q = t0; q < nq; q += st) {
+        const P tv = reinterpret_cast<const P *>(t)[q];
+        const P pv = reinterpret_cast<const P *>(psi)[q];
+        const P r2v = reinterpret_cast<const P *>(r2)[q];
+        P mv{};
+        if (minv) mv = reinterpret_cast<const P *>(minv)[q];
+        P o;
+#pragma unroll
+        for (int e = 0; e < W; ++e) o[e] = one((double)tv[e], (double)pv[e], (double)r2v[e], (double)mv[e]);
+        reinterpret_cast<P *>(r)[q] = o;
     }
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    for (int64_t i = nq * W + t0; i < n; i += st)
+        r[i] = one((double)t[i], (double)psi[i], (double)r2[i], minv ? (double)minv[i] : 0.0);
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = s0;
+        sh[1][threadIdx.x >> 6] = s1;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    if (threadIdx.x < 2)
+        part[(size_t)blockIdx.x * 2 + threadIdx.x] =
+            sh[threadIdx.x][0] + sh[threadIdx.x][1] + sh[threadIdx.x][2] + sh[threadIdx.x][3];
 }
 
 // w = (v - oe wp - de wl) * ig, x += phi w
@@ -298,8 +416,27 @@ __global__ __launch_bounds__(256) void k_minres_c(const V *__restrict__ v, const
                                                   const V *__restrict__ wl, double oe, double de,
                                                   double ig, double phi, V *__restrict__ w,
                                                   V *__restrict__ x, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    constexpr int W = vpk_w<V>();
+    using P = vpk<V, W>;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nq = (al16(v) && al16(wp) && al16(wl) && al16(w) && al16(x)) ? n / W : 0;
+    for (int64_t q = t0; q < nq; q += st) {
+        const P vv = reinterpret_cast<const P *>(v)[q];
+        const P pv = reinterpret_cast<const P *>(wp)[q];
+        const P lv = reinterpret_cast<const P *>(wl)[q];
+        P xv = reinterpret_cast<const P *>(x)[q];
+        P wv;
+#pragma unroll
+        for (int e = 0; e < W; ++e) {
+            const double wi = (((double)vv[e] - oe * (double)pv[e]) - de * (double)lv[e]) * ig;
+            wv[e] = (V)wi;
+            xv[e] = (V)((double)xv[e] + phi * wi);
+        }
+        reinterpret_cast<P *>(w)[q] = wv;
+        reinterpret_cast<P *>(x)[q] = xv;
+    }
+    for (int64_t i = nq * W + t0; i < n; i += st) {
         const double wi = (((double)v[i] - oe * (double)wp[i]) - de * (double)wl[i]) * ig;
         w[i] = (V)wi;
         x[i] = (V)((double)x[i] + phi * wi);
@@ -702,6 +839,9 @@ struct LczState {
     bool norm_lost = false;
     double ev = 0.0, e0 = 0.0;
     std::vector<double> prev;          // lowest Ritz values at the previous check
+    // three-term recurrence: Ritz values within ghost_tol (relative) of a lower one are
+    // copies of it (0: full reorthogonalisation, where a repeat is a degenerate level)
+    double ghost_tol = 0.0;
 };
 
 // convergence test in the reference's form (lanczos.f90:74-89) on the Ritz values
@@ -716,17 +856,72 @@ static void ref_convergence(LczState &s, double tol, int max_steps, const double
     }
 }
 
+// The positions of the nst lowest genuine Ritz values (ascending d, u1 the first
+// components of their tridiagonal eigenvectors).  In exact arithmetic the Ritz values
+// of a single-vector Lanczos run are simple; without reorthogonalisation, rounding
+// adds copies of the values that have converged (Paige's ghosts) and, while a copy
+// forms, a value whose eigenvector has no weight on the start vector (Cullum and
+// Willoughby's spurious values: also eigenvalues of T without its first row and
+// column).  Either would pass for an excited state.  So: values within tol (relative)
+// of a cluster's first member are one cluster (the copies agree to about 1e-8 with FP64
+// vectors, 1e-5 with FP32 ones), represented by its member of largest |u1|; a value
+// alone with |u1| < 1e-10 is spurious (measured: below 1e-18 for the spurious values,
+// above 1e-6 for the weakest genuine cluster of a 300-state test).  When fewer genuine
+// values exist than asked for, the rest fill up in ascending order.  tol = 0 (full
+// reorthogonalisation, where a repeated value is a degenerate level): the nst lowest.
+static std::vector<int> genuine_ritz(const double *d, const double *u1, int m, int nst, double tol) {
+    std::vector<int> keep;
+    if (!(tol > 0.0) || !u1) {
+        for (int i = 0; i < std::min(m, nst); ++i) keep.push_back(i);
+        return keep;
+    }
+    std::vector<char> used(std::max(m, 1), 0);
+    for (int i = 0; i < m && (int)keep.size() < nst;) {
+        int j = i + 1, best = i;
+        while (j < m && d[j] - d[i] <= tol * std::max(1.0, std::fabs(d[j]))) {
+            if (std::fabs(u1[j]) > std::fabs(u1[best])) best = j;
+            ++j;
+        }
+        if (j - i > 1 || std::fabs(u1[i]) >= 1e-10) {
+            keep.push_back(best);
+            used[best] = 1;
+        }
+        i = j;
+    }
+    for (int i = 0; i < m && (int)keep.size() < nst; ++i)
+        if (!used[i]) keep.push_back(i);
+    std::sort(keep.begin(), keep.end());
+    return keep;
+}
+
+// Ritz values of the m x m Lanczos tridiagonal and the first components of their
+// eigenvectors
+static int ritz_first(int m, const double *alpha, const double *beta, std::vector<double> &vals,
+                      std::vector<double> &u1) {
+    vals.assign(alpha, alpha + m);
+    std::vector<double> e(std::max(m - 1, 1), 0.0);
+    for (int i = 0; i + 1 < m; ++i) e[i] = beta[i + 1];
+    u1.assign(std::max(m, 1), 0.0);
+    return tridiag_ql(m, vals.data(), e.data(), u1.data(), 1);
+}
+
 // With several requested states the run also continues until each of the nst lowest
-// Ritz values has moved by less than tol since the previous check (the reference's
-// test alone follows the two lowest distinct levels).  nst = 1 is the reference's test.
+// distinct Ritz values has moved by less than tol since the previous check (the
+// reference's test alone follows the two lowest distinct levels).  nst = 1 is the
+// reference's test.
 static void convergence(LczState &s, double tol, int max_steps, const std::vector<double> &ritz,
                         int nst) {
     ref_convergence(s, tol, max_steps, ritz.data());
     if (nst <= 1) return;
-    const int m = (int)ritz.size();
-    bool stable = m >= nst && (int)s.prev.size() >= nst;
-    for (int k = 0; stable && k < nst; ++k) stable = std::fabs(ritz[k] - s.prev[k]) < tol;
-    s.prev.assign(ritz.begin(), ritz.begin() + std::min(m, nst));
+    std::vector<double> vals = ritz, u1;
+    if (s.ghost_tol > 0.0 && ritz_first(s.steps, s.alpha.data(), s.beta.data(), vals, u1)) u1.clear();
+    const std::vector<int> k_of = genuine_ritz(vals.data(), u1.empty() ? nullptr : u1.data(),
+                                               (int)vals.size(), nst, s.ghost_tol);
+    std::vector<double> cur;
+    for (int k : k_of) cur.push_back(vals[k]);
+    bool stable = (int)cur.size() >= nst && (int)s.prev.size() >= nst;
+    for (int k = 0; stable && k < nst; ++k) stable = std::fabs(cur[k] - s.prev[k]) < tol;
+    s.prev = cur;
     if (!stable) s.converged = false;
 }
 
@@ -759,10 +954,15 @@ static int ritz_states(const LczState &s, int nst, std::vector<double> &vals,
     std::vector<double> z((size_t)m * m);
     if (tridiag_ql(m, d.data(), e.data(), z.data())) return 1;
     nst = std::max(1, std::min(nst, m));
-    vals.assign(d.begin(), d.begin() + nst);
+    std::vector<double> u1(m);
+    for (int k = 0; k < m; ++k) u1[k] = z[(size_t)k * m];
+    const std::vector<int> k_of = genuine_ritz(d.data(), u1.data(), m, nst, s.ghost_tol);
+    vals.resize(nst);
     ys.assign(nst, std::vector<double>(m));
-    for (int k = 0; k < nst; ++k)
-        for (int j = 0; j < m; ++j) ys[k][j] = z[(size_t)k * m + j];
+    for (int k = 0; k < nst; ++k) {
+        vals[k] = d[k_of[k]];
+        for (int j = 0; j < m; ++j) ys[k][j] = z[(size_t)k_of[k] * m + j];
+    }
     return 0;
 }
 
@@ -1261,8 +1461,9 @@ static int rq_residual(phub_ctx *c, const phub_op *op, const double *psi, double
 // steps, fewer once |r| <= psi_tol.  On return e0 is the Rayleigh quotient of psi
 // and res its residual.
 template <typename V>
-static int minres_core(phub_ctx *c, const phub_op *op, const double *psi, double e0, const V *b,
-                       V *x, int max_iter, double tol, int *iters, double *rel_res, Arena &ar);
+static int minres_core(phub_ctx *c, const phub_op *op, const V *psi, double e0, const V *b,
+                       V *x, int max_iter, double tol, int *iters, double *rel_res, Arena &ar,
+                       const V *minv);
 
 __global__ __launch_bounds__(256) void k_jd_rhs(const double *__restrict__ r, float *__restrict__ b,
                                                 int64_t n) {
@@ -1279,6 +1480,17 @@ static int jd_refine(phub_ctx *c, const phub_op *op, double *psi, Arena &ar, int
     float *b32 = (float *)ar.take(n * sizeof(float));
     float *x32 = (float *)ar.take(n * sizeof(float));
     if (!tmp || !b32 || !x32) return phub_fail(c, PHUB_ERR_OOM, "FP64 correction vectors do not fit");
+    // Jacobi preconditioner of the inner solves on one-body Hubbard operators (their
+    // diagonal, U times the double occupancy, spreads the spectrum of H - E far more
+    // than the hops do); PHUB_JD_PRECOND=0 solves unpreconditioned
+    float *psi32 = (float *)ar.take(n * sizeof(float));
+    if (!psi32) return phub_fail(c, PHUB_ERR_OOM, "FP64 correction vectors do not fit");
+    float *minv = nullptr;
+    const char *pe = std::getenv("PHUB_JD_PRECOND");
+    if (op->kind == 0 && !op->d_ul && !(pe && pe[0] == '0')) {
+        minv = (float *)ar.take(n * sizeof(float));
+        if (!minv) return phub_fail(c, PHUB_ERR_OOM, "FP64 correction vectors do not fit");
+    }
     int rc = rq_residual(c, op, psi, tmp, e0, res);
     if (rc) return rc;
     for (int it = 0; it < rs && !(psi_tol > 0.0 && *res <= psi_tol); ++it) {
@@ -1289,8 +1501,18 @@ static int jd_refine(phub_ctx *c, const phub_op *op, double *psi, Arena &ar, int
                                             : 1e-2;
         int iters = 0;
         double rr = 0.0;
+        if (minv) {
+            rc = phub_hub_minv(c, op, *e0, minv);
+            if (rc) return rc;
+        }
+        // the inner solve projects with an FP32 copy of psi: a psi component it leaves
+        // in the correction only rescales psi + d, which is normalised below
+        hipLaunchKernelGGL(k_convert<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                           psi, psi32, n, 1.0);   // one element per thread: a full grid
+        PHUB_HIP(c, hipGetLastError());
         const size_t mark = ar.used;
-        rc = minres_core<float>(c, op, psi, *e0, b32, x32, 120, tol_in, &iters, &rr, ar);
+        rc = minres_core<float>(c, op, (const float *)psi32, *e0, b32, x32, 120, tol_in, &iters,
+                                &rr, ar, (const float *)minv);
         ar.used = mark;   // the seven MINRES vectors are free again
         if (rc) return rc;
         out->corrections += 1;
@@ -1436,9 +1658,10 @@ extern "C" int phub_lanczos_workspace_bytes(const phub_op *op, const phub_lanczo
     int steps = o->fixed_steps > 0 ? o->fixed_steps : o->max_steps;
     steps = (int)std::min<int64_t>(std::max(steps, 1), std::max<int64_t>(n, 1));
     const size_t m1 = (size_t)steps + 1;
-    // jd_refine: one FP64 vector, nine FP32 ones
+    // jd_refine: one FP64 vector, eleven FP32 ones (the preconditioner and the FP32
+    // copy of psi included)
     const size_t refine = ((ld * sizeof(double) + 255) & ~(size_t)255) +
-                          9 * ((ld * sizeof(float) + 255) & ~(size_t)255) + 512;
+                          11 * ((ld * sizeof(float) + 255) & ~(size_t)255) + 512;
     size_t b = 0;
     switch (o->mode) {
     case PHUB_LCZ_FULL64: b = m1 * ld * sizeof(double); break;
@@ -1484,6 +1707,7 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
     auto t0 = std::chrono::steady_clock::now();
     EvTimer tm(c);
     LczState s;
+    s.ghost_tol = o->mode == PHUB_LCZ_PLAIN64 ? 1e-8 : o->mode == PHUB_LCZ_PLAIN32 ? 1e-5 : 0.0;
     // combine coefficients (max_steps + 8 doubles), the table of basis-vector pointers
     // of the one-pass psi (max_steps + 1) and the buffers of the graph-captured plain
     // recurrence (graph_bufs): the context arena
@@ -1589,8 +1813,12 @@ extern "C" int phub_lanczos(phub_ctx *c, const phub_op *op, const phub_lanczos_o
     res->seconds_lanczos = t_lcz;
     {
         std::vector<double> rv;
-        if (s.steps > 0 && all_ritz(s.steps, s.alpha.data(), s.beta.data(), rv) == 0)
-            for (int k = 0; k < PHUB_MAX_STATES && k < s.steps; ++k) res->e_states[k] = rv[k];
+        std::vector<double> u1;
+        if (s.steps > 0 && ritz_first(s.steps, s.alpha.data(), s.beta.data(), rv, u1) == 0) {
+            const std::vector<int> k_of = genuine_ritz(rv.data(), u1.data(), (int)rv.size(),
+                                                       std::min(PHUB_MAX_STATES, s.steps), s.ghost_tol);
+            for (size_t k = 0; k < k_of.size(); ++k) res->e_states[k] = rv[k_of[k]];
+        }
         if (o->mode == PHUB_LCZ_MIXED || (o->mode == PHUB_LCZ_PLAIN32 && o->want_psi &&
                                           o->refine_steps > 0))
             res->e_states[0] = res->e0;
@@ -1970,9 +2198,16 @@ extern "C" int phub_adjoint_minres_workspace_bytes(const phub_op *op, size_t *by
 // MINRES on (H - e0) restricted to the complement of psi (Paige-Saunders), vectors
 // of type V: x (zeroed here) solves it for the right-hand side b projected off psi,
 // to the relative residual tol or max_iter iterations.  Seven work vectors from ar.
+// With minv (a positive diagonal M^-1) the iteration is preconditioned by P M^-1 P,
+// P = 1 - psi psi^T: symmetric and positive definite on the complement, where every
+// iterate lies.  The residual vectors r stay orthogonal to psi, so the preconditioned
+// residual is minv * r - (psi . minv r) psi and its norm^2 is r . (minv r); both dots
+// come from the pass that forms r, and the projection happens where v is formed.  tol
+// then bounds the residual in the M^-1 norm.
 template <typename V>
-static int minres_core(phub_ctx *c, const phub_op *op, const double *psi, double e0, const V *b,
-                       V *x, int max_iter, double tol, int *iters, double *rel_res, Arena &ar) {
+static int minres_core(phub_ctx *c, const phub_op *op, const V *psi, double e0, const V *b,
+                       V *x, int max_iter, double tol, int *iters, double *rel_res, Arena &ar,
+                       const V *minv) {
     const int64_t n = op->dim;
     // r2 is also the unnormalised Lanczos vector y of the iteration; wl / wp are the
     // last two search directions and wf the slot the next one is written to
@@ -1990,15 +2225,19 @@ static int minres_core(phub_ctx *c, const phub_op *op, const double *psi, double
     PHUB_HIP(c, hipMemsetAsync(wp.p, 0, n * sizeof(V), c->stream));
     // r1 = b - (psi . b) psi
     double pb = 0;
-    rc = dev_dot<double, V>(c, psi, b, n, &pb);
+    rc = dev_dot<V, V>(c, psi, b, n, &pb);
     if (rc) return rc;
     hipLaunchKernelGGL(k_axpby_to<V>, dim3(g), dim3(256), 0, c->stream, -pb, psi, 1.0, b, r1.p, n);
     PHUB_HIP(c, hipGetLastError());
-    PHUB_HIP(c, hipMemcpyAsync(r2.p, r1.p, n * sizeof(V), hipMemcpyDeviceToDevice, c->stream));
-    double b1sq = 0;
-    rc = dev_dot<V, V>(c, r1.p, r1.p, n, &b1sq);
+    // r2 = r1, with r1 . (M^-1 r1) and psi . (M^-1 r1) (or |r1|^2)
+    hipLaunchKernelGGL(k_minres_b<V>, dim3(g), dim3(256), 0, c->stream, (const V *)r1.p, psi,
+                       (const V *)r1.p, 0.0, 0.0, minv, r2.p, n, c->d_red);
+    PHUB_HIP(c, hipGetLastError());
+    double d2[2];
+    rc = phub_reduce_partials(c, (int)g, 2, d2);
     if (rc) return rc;
-    const double beta1 = std::sqrt(b1sq);
+    const double beta1 = std::sqrt(std::max(d2[0], 0.0));
+    double gam = d2[1];
     if (iters) *iters = 0;
     if (rel_res) *rel_res = 0.0;
     if (!(beta1 > 0.0)) return PHUB_OK;
@@ -2006,9 +2245,14 @@ static int minres_core(phub_ctx *c, const phub_op *op, const double *psi, double
     int it = 0;
     double rnorm = beta1;
     for (it = 1; it <= max_iter; ++it) {
-        // v = y / beta; t = (H - e0) v - (beta / oldb) r1; p = psi . (H - e0) v
-        hipLaunchKernelGGL(k_minres_scale<V>, dim3(g), dim3(256), 0, c->stream, v.p, r2.p,
-                           1.0 / beta, n);
+        // v = y / beta (y = r2, or its preconditioned form); t = (H - e0) v - (beta / oldb) r1;
+        // p = psi . (H - e0) v
+        if (minv)
+            hipLaunchKernelGGL(k_minres_pscale<V>, dim3(g), dim3(256), 0, c->stream, v.p,
+                               (const V *)r2.p, minv, psi, gam, 1.0 / beta, n);
+        else
+            hipLaunchKernelGGL(k_minres_scale<V>, dim3(g), dim3(256), 0, c->stream, v.p, r2.p,
+                               1.0 / beta, n);
         rc = phub_hv_launch<V>(c, op, v.p, t.p, 1.0, (const V *)nullptr, 0.0, false, nullptr);
         if (rc) return rc;
         hipLaunchKernelGGL(k_minres_a<V>, dim3(g), dim3(256), 0, c->stream, t.p, v.p,
@@ -2021,15 +2265,16 @@ static int minres_core(phub_ctx *c, const phub_op *op, const double *psi, double
         const double p = d3[0];                    // psi . (H - e0) v
         const double alfa = d3[1] - p * d3[2];     // v . (t - p psi)
         // y_new = t - p psi - (alfa / beta) r2, into r1's slot; then r1 <- r2, r2 <- y_new
-        hipLaunchKernelGGL(k_minres_b<V>, dim3(g), dim3(256), 0, c->stream, t.p, psi, r2.p, p,
-                           alfa / beta, r1.p, n, c->d_red);
+        hipLaunchKernelGGL(k_minres_b<V>, dim3(g), dim3(256), 0, c->stream, (const V *)t.p, psi,
+                           (const V *)r2.p, p, alfa / beta, minv, r1.p, n, c->d_red);
         PHUB_HIP(c, hipGetLastError());
-        double yy = 0;
-        rc = phub_reduce_partials(c, (int)g, 1, &yy);
+        double yy[2];
+        rc = phub_reduce_partials(c, (int)g, 2, yy);
         if (rc) return rc;
         std::swap(r1.p, r2.p);
         oldb = beta;
-        beta = std::sqrt(std::max(yy, 0.0));
+        beta = std::sqrt(std::max(yy[0], 0.0));
+        gam = yy[1];
         const double oldeps = epsln;
         const double delta = cs * dbar + sn * alfa;
         const double gbar = sn * dbar - cs * alfa;
@@ -2063,5 +2308,6 @@ extern "C" int phub_adjoint_minres(phub_ctx *c, const phub_op *op, const double 
     if (!c || !op || !psi || !b || !x) return phub_fail(c, PHUB_ERR_ARG, "null argument");
     // the seven vectors come from the caller's workspace, else are allocated for the call
     Arena ar(c, workspace, workspace_bytes);
-    return minres_core<double>(c, op, psi, e0, b, x, max_iter, tol, iters, rel_res, ar);
+    return minres_core<double>(c, op, psi, e0, b, x, max_iter, tol, iters, rel_res, ar,
+                               (const double *)nullptr);
 }

@@ -304,3 +304,7 @@ int phub_dn_tile_code(int cfg, bool fp64);
 template <typename T>
 int phub_hv_launch(phub_ctx *ctx, const phub_op *op, const T *x, T *out, double scale,
                    const T *prev, double beta, bool want_dots, int *nblocks_out);
+
+// minv = 1 / max(diag(H) - e, 1e-3) of a one-body Hubbard operator, FP32, dim entries
+// (hv.hip); PHUB_ERR_UNSUPPORTED for other operators
+int phub_hub_minv(phub_ctx *ctx, const phub_op *op, double e, float *minv);

@@ -372,6 +372,58 @@ def test_rdm_transition_vs_oracle(ctx, monkeypatch, n, nu, nd, seed, pairmap):
     assert np.max(np.abs(ss - m.sisj(a, b))) < 1e-10
 
 
+@pytest.mark.parametrize("mode", [0, 3])
+def test_plain_recurrence_excited_states_skip_ghosts(ctx, mode):
+    """The three-term recurrence run far past convergence (200 fixed steps on 300
+    states) makes copies of its converged Ritz values; the excited states skip them:
+    the three lowest energies match the oracle's dense spectrum, and each returned
+    state is an eigenvector of the oracle's matrix (FP64: residual 1e-6; FP32 vectors
+    with FP64 scalars: 1e-3)."""
+    from pyhub_amd.engine import HubbardOp
+    T = O.disordered_ring_T(6)
+    op = HubbardOp(ctx, 6, 3, 2, T, 2.5)
+    m = O.HubbardModel(6, 3, 2, T, 2.5)
+    H = m.dense()
+    e = np.linalg.eigvalsh(H)
+    r = op.lanczos(mode=mode, fixed_steps=200, max_steps=200, n_states=3, want_psi=True, seed=3)
+    print(f"mode {mode}: e_states {r.e_states} dense {e[:3]}", flush=True)
+    tol_e, tol_r = (1e-9, 1e-6) if mode == 0 else (1e-5, 1e-3)
+    assert np.max(np.abs(np.asarray(r.e_states[:3]) - e[:3])) < tol_e
+    for k in range(3):
+        p = r.psis[k].cpu().numpy()
+        p = p / np.linalg.norm(p)
+        assert np.linalg.norm(H @ p - e[k] * p) < tol_r, k
+
+
+def test_jd_correction_preconditioned(ctx, monkeypatch):
+    """FP64 corrections of the FP32 ground state with the Jacobi-preconditioned inner
+    MINRES (the default) and without it (PHUB_JD_PRECOND=0), on the 4x3 t-t' torus at
+    half filling (853,776 states): both reach the residual target, agree with the FP64
+    three-term recurrence on E0 to 1e-12 relative and on the 1-RDM, double occupancy
+    and <S.S> to 1e-8, and the preconditioned solve takes fewer inner iterations."""
+    from pyhub_amd import _lib as L
+    from pyhub_amd.engine import HubbardOp
+    T = O.square_T(4, 3, t=1.0, tp=-0.2, periodic=True)
+    op = HubbardOp(ctx, 12, 6, 6, T, 8.0)
+    v0 = dev(np.random.default_rng(9).uniform(-0.5, 0.5, op.dim))
+    out = {}
+    for name, mode, pc in (("fp64", L.LCZ_PLAIN64, "1"), ("pc", L.LCZ_PLAIN32, "1"),
+                           ("nopc", L.LCZ_PLAIN32, "0")):
+        monkeypatch.setenv("PHUB_JD_PRECOND", pc)
+        r = op.lanczos(mode=mode, max_steps=400, tol=1e-9, v0=v0, refine_steps=4, psi_tol=1e-10)
+        out[name] = (r.e0, r.residual, r.correction_iters, op.rdm(r.psi))
+        print(f"4x3 {name}: E0 {r.e0:.13f} residual {r.residual:.1e} {r.corrections} corrections "
+              f"({r.correction_iters} inner)", flush=True)
+        del r
+    e64 = out["fp64"][0]
+    for name in ("pc", "nopc"):
+        e, res, _, q = out[name]
+        assert res <= 1e-9
+        assert abs(e - e64) <= 1e-12 * abs(e64)
+        assert max(float(np.max(np.abs(a - b))) for a, b in zip(q, out["fp64"][3])) <= 1e-8
+    assert 0 < out["pc"][2] < out["nopc"][2]
+
+
 @pytest.mark.parametrize("n,seed", [(6, 30), (8, 31)])
 def test_heisenberg_sisj_vs_oracle(ctx, n, seed):
     from pyhub_amd.engine import HeisenbergOp
