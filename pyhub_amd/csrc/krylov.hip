@@ -146,10 +146,12 @@ __device__ __forceinline__ double wave_sum(double a) {
     return a;
 }
 
-// 16-byte packs for the streaming passes: four FP32 or two FP64 elements per lane
-// access.  A pass takes the pack loop when every pointer it touches is 16-byte aligned
-// (a wave-uniform test) and finishes the last n mod W elements, or all of them when a
-// pointer is not aligned, element by element.
+// 16-byte packs (four FP32 elements per lane access) for the FP32 y = a x + b y pass,
+// where they measured faster (0.47 -> 0.40 ms on 1.66e8 elements).  The passes with
+// more streams or FP64 elements measured slower with them (the MINRES passes 2.61 ->
+// 2.96 ms per iteration, the FP64 y = a x + b y 0.72 -> 0.80 ms) and loop per element.
+// The pack loop runs when every pointer is 16-byte aligned (a wave-uniform test); the
+// last n mod 4 elements, or all of them otherwise, go element by element.
 template <typename T, int W>
 using vpk = T __attribute__((ext_vector_type(W)));
 template <typename V>
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(256) void k_axpby(double a, const TX *__restrict__ 
     const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t st = (int64_t)gridDim.x * blockDim.x;
     int64_t i0 = 0;
-    if constexpr (sizeof(TX) == sizeof(TY)) {
+    if constexpr (sizeof(TX) == 4 && sizeof(TY) == 4) {
         constexpr int W = vpk_w<TY>();
         using PX = vpk<TX, W>;
         using PY = vpk<TY, W>;
@@ -264,19 +266,9 @@ __global__ __launch_bounds__(256) void k_axpby_to(double a, const V *__restrict_
 template <typename V>
 __global__ __launch_bounds__(256) void k_minres_scale(V *__restrict__ v, const V *__restrict__ y,
                                                       double s, int64_t n) {
-    constexpr int W = vpk_w<V>();
-    using P = vpk<V, W>;
-    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t st = (int64_t)gridDim.x * blockDim.x;
-    const int64_t nq = (al16(v) && al16(y)) ? n / W : 0;
-    for (int64_t q = t0; q < nq; q += st) {
-        const P a = reinterpret_cast<const P *>(y)[q];
-        P o;
-#pragma unroll
-        for (int e = 0; e < W; ++e) o[e] = (V)(s * (double)a[e]);
-        reinterpret_cast<P *>(v)[q] = o;
-    }
-    for (int64_t i = nq * W + t0; i < n; i += st) v[i] = (V)(s * (double)y[i]);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = (V)(s * (double)y[i]);
 }
 
 // preconditioned form: v = s * (minv * y - gam * psi), the preconditioned residual
@@ -286,30 +278,19 @@ __global__ __launch_bounds__(256) void k_minres_pscale(V *__restrict__ v, const 
                                                        const V *__restrict__ minv,
                                                        const V *__restrict__ psi, double gam,
                                                        double s, int64_t n) {
-    constexpr int W = vpk_w<V>();
-    using P = vpk<V, W>;
-    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t st = (int64_t)gridDim.x * blockDim.x;
-    const int64_t nq = (al16(v) && al16(y) && al16(minv) && al16(psi)) ? n / W : 0;
-    for (int64_t q = t0; q < nq; q += st) {
-        const P a = reinterpret_cast<const P *>(y)[q];
-        const P m = reinterpret_cast<const P *>(minv)[q];
-        const P p = reinterpret_cast<const P *>(psi)[q];
-        P o;
-#pragma unroll
-        for (int e = 0; e < W; ++e) o[e] = (V)(s * ((double)m[e] * (double)a[e] - gam * (double)p[e]));
-        reinterpret_cast<P *>(v)[q] = o;
-    }
-    for (int64_t i = nq * W + t0; i < n; i += st)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
         v[i] = (V)(s * ((double)minv[i] * (double)y[i] - gam * (double)psi[i]));
 }
 
-// after t = H v: t <- t - e0 v - c1 r1 (no r1 term when r1 is null); partials
+// after t = H v: t <- t - e0 v - c1 r1 (no r1 term in the first iteration); partials
 // [block][3] of (psi.(t - e0 v), v.t, v.psi).  psi is projected out of (H - e0) v
 // before the r1 term: taking it from the whole of t would also strip r1's own
 // small psi component, which breaks the three-term relation the MINRES update
-// relies on (slower convergence, 1e-11 instead of 1e-13 in a 60-state test).
-template <typename V>
+// relies on (slower convergence, 1e-11 instead of 1e-13 in a 60-state test).  R1: r1
+// is read (every iteration after the first); a template flag, so that no load in the
+// loop is conditional (a conditional load measured 30 % slower in the next pass).
+template <typename V, bool R1>
 __global__ __launch_bounds__(256) void k_minres_a(V *__restrict__ t, const V *__restrict__ v,
                                                   const V *__restrict__ r1,
                                                   const V *__restrict__ psi, double e0,
@@ -326,23 +307,9 @@ __global__ __launch_bounds__(256) void k_minres_a(V *__restrict__ t, const V *__
         s2 += vi * pi;
         return tv;
     };
-    constexpr int W = vpk_w<V>();
-    using P = vpk<V, W>;
-    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t st = (int64_t)gridDim.x * blockDim.x;
-    const int64_t nq = (al16(t) && al16(v) && al16(r1) && al16(psi)) ? n / W : 0;
-    for (int64_t q = t0; q < nq; q += st) {
-        P tv = reinterpret_cast<const P *>(t)[q];
-        const P vv = reinterpret_cast<const P *>(v)[q];
-        const P pv = reinterpret_cast<const P *>(psi)[q];
-        P rv{};
-        if (r1) rv = reinterpret_cast<const P *>(r1)[q];
-#pragma unroll
-        for (int e = 0; e < W; ++e) tv[e] = one((double)tv[e], (double)vv[e], (double)pv[e], (double)rv[e]);
-        reinterpret_cast<P *>(t)[q] = tv;
-    }
-    for (int64_t i = nq * W + t0; i < n; i += st)
-        t[i] = one((double)t[i], (double)v[i], (double)psi[i], r1 ? (double)r1[i] : 0.0);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        t[i] = one((double)t[i], (double)v[i], (double)psi[i], R1 ? (double)r1[i] : 0.0);
     s0 = wave_sum(s0);
     s1 = wave_sum(s1);
     s2 = wave_sum(s2);
@@ -358,8 +325,9 @@ __global__ __launch_bounds__(256) void k_minres_a(V *__restrict__ t, const V *__
 }
 
 // r = t - p psi - c2 r2 (r may not alias t, psi or r2); partials [block][2]: without
-// minv |r|^2 and 0, with it r . (minv r) and psi . (minv r)
-template <typename V>
+// the preconditioner (PC false, minv unused) |r|^2 and 0, with it r . (minv r) and
+// psi . (minv r)
+template <typename V, bool PC>
 __global__ __launch_bounds__(256) void k_minres_b(const V *__restrict__ t,
                                                   const V *__restrict__ psi,
                                                   const V *__restrict__ r2, double p, double c2,
@@ -371,7 +339,7 @@ __global__ __launch_bounds__(256) void k_minres_b(const V *__restrict__ t,
     auto one = [&](double ti, double pi, double r2i, double mi) -> V {
         const V ri = (V)(ti - p * pi - c2 * r2i);
         const double rd = (double)ri;
-        if (minv) {
+        if constexpr (PC) {
             const double mr = mi * rd;
             s0 += rd * mr;
             s1 += pi * mr;
@@ -380,24 +348,9 @@ __global__ __launch_bounds__(256) void k_minres_b(const V *__restrict__ t,
         }
         return ri;
     };
-    constexpr int W = vpk_w<V>();
-    using P = vpk<V, W>;
-    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t st = (int64_t)gridDim.x * blockDim.x;
-    const int64_t nq = (al16(t) && al16(psi) && al16(r2) && al16(minv) && al16(r)) ? n / W : 0;
-    for (int64_t q = t0; q < nq; q += st) {
-        const P tv = reinterpret_cast<const P *>(t)[q];
-        const P pv = reinterpret_cast<const P *>(psi)[q];
-        const P r2v = reinterpret_cast<const P *>(r2)[q];
-        P mv{};
-        if (minv) mv = reinterpret_cast<const P *>(minv)[q];
-        P o;
-#pragma unroll
-        for (int e = 0; e < W; ++e) o[e] = one((double)tv[e], (double)pv[e], (double)r2v[e], (double)mv[e]);
-        reinterpret_cast<P *>(r)[q] = o;
-    }
-    for (int64_t i = nq * W + t0; i < n; i += st)
-        r[i] = one((double)t[i], (double)psi[i], (double)r2[i], minv ? (double)minv[i] : 0.0);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        r[i] = one((double)t[i], (double)psi[i], (double)r2[i], PC ? (double)minv[i] : 0.0);
     s0 = wave_sum(s0);
     s1 = wave_sum(s1);
     if ((threadIdx.x & 63) == 0) {
@@ -416,27 +369,8 @@ __global__ __launch_bounds__(256) void k_minres_c(const V *__restrict__ v, const
                                                   const V *__restrict__ wl, double oe, double de,
                                                   double ig, double phi, V *__restrict__ w,
                                                   V *__restrict__ x, int64_t n) {
-    constexpr int W = vpk_w<V>();
-    using P = vpk<V, W>;
-    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t st = (int64_t)gridDim.x * blockDim.x;
-    const int64_t nq = (al16(v) && al16(wp) && al16(wl) && al16(w) && al16(x)) ? n / W : 0;
-    for (int64_t q = t0; q < nq; q += st) {
-        const P vv = reinterpret_cast<const P *>(v)[q];
-        const P pv = reinterpret_cast<const P *>(wp)[q];
-        const P lv = reinterpret_cast<const P *>(wl)[q];
-        P xv = reinterpret_cast<const P *>(x)[q];
-        P wv;
-#pragma unroll
-        for (int e = 0; e < W; ++e) {
-            const double wi = (((double)vv[e] - oe * (double)pv[e]) - de * (double)lv[e]) * ig;
-            wv[e] = (V)wi;
-            xv[e] = (V)((double)xv[e] + phi * wi);
-        }
-        reinterpret_cast<P *>(w)[q] = wv;
-        reinterpret_cast<P *>(x)[q] = xv;
-    }
-    for (int64_t i = nq * W + t0; i < n; i += st) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
         const double wi = (((double)v[i] - oe * (double)wp[i]) - de * (double)wl[i]) * ig;
         w[i] = (V)wi;
         x[i] = (V)((double)x[i] + phi * wi);
@@ -2230,8 +2164,12 @@ static int minres_core(phub_ctx *c, const phub_op *op, const V *psi, double e0, 
     hipLaunchKernelGGL(k_axpby_to<V>, dim3(g), dim3(256), 0, c->stream, -pb, psi, 1.0, b, r1.p, n);
     PHUB_HIP(c, hipGetLastError());
     // r2 = r1, with r1 . (M^-1 r1) and psi . (M^-1 r1) (or |r1|^2)
-    hipLaunchKernelGGL(k_minres_b<V>, dim3(g), dim3(256), 0, c->stream, (const V *)r1.p, psi,
-                       (const V *)r1.p, 0.0, 0.0, minv, r2.p, n, c->d_red);
+    if (minv)
+        hipLaunchKernelGGL((k_minres_b<V, true>), dim3(g), dim3(256), 0, c->stream, (const V *)r1.p,
+                           psi, (const V *)r1.p, 0.0, 0.0, minv, r2.p, n, c->d_red);
+    else
+        hipLaunchKernelGGL((k_minres_b<V, false>), dim3(g), dim3(256), 0, c->stream, (const V *)r1.p,
+                           psi, (const V *)r1.p, 0.0, 0.0, minv, r2.p, n, c->d_red);
     PHUB_HIP(c, hipGetLastError());
     double d2[2];
     rc = phub_reduce_partials(c, (int)g, 2, d2);
@@ -2255,9 +2193,12 @@ static int minres_core(phub_ctx *c, const phub_op *op, const V *psi, double e0, 
                                1.0 / beta, n);
         rc = phub_hv_launch<V>(c, op, v.p, t.p, 1.0, (const V *)nullptr, 0.0, false, nullptr);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_minres_a<V>, dim3(g), dim3(256), 0, c->stream, t.p, v.p,
-                           it >= 2 ? (const V *)r1.p : (const V *)nullptr, psi, e0,
-                           it >= 2 ? beta / oldb : 0.0, n, c->d_red);
+        if (it >= 2)
+            hipLaunchKernelGGL((k_minres_a<V, true>), dim3(g), dim3(256), 0, c->stream, t.p, v.p,
+                               (const V *)r1.p, psi, e0, beta / oldb, n, c->d_red);
+        else
+            hipLaunchKernelGGL((k_minres_a<V, false>), dim3(g), dim3(256), 0, c->stream, t.p, v.p,
+                               (const V *)nullptr, psi, e0, 0.0, n, c->d_red);
         PHUB_HIP(c, hipGetLastError());
         double d3[3];
         rc = phub_reduce_partials(c, (int)g, 3, d3);
@@ -2265,8 +2206,12 @@ static int minres_core(phub_ctx *c, const phub_op *op, const V *psi, double e0, 
         const double p = d3[0];                    // psi . (H - e0) v
         const double alfa = d3[1] - p * d3[2];     // v . (t - p psi)
         // y_new = t - p psi - (alfa / beta) r2, into r1's slot; then r1 <- r2, r2 <- y_new
-        hipLaunchKernelGGL(k_minres_b<V>, dim3(g), dim3(256), 0, c->stream, (const V *)t.p, psi,
-                           (const V *)r2.p, p, alfa / beta, minv, r1.p, n, c->d_red);
+        if (minv)
+            hipLaunchKernelGGL((k_minres_b<V, true>), dim3(g), dim3(256), 0, c->stream, (const V *)t.p,
+                               psi, (const V *)r2.p, p, alfa / beta, minv, r1.p, n, c->d_red);
+        else
+            hipLaunchKernelGGL((k_minres_b<V, false>), dim3(g), dim3(256), 0, c->stream, (const V *)t.p,
+                               psi, (const V *)r2.p, p, alfa / beta, minv, r1.p, n, c->d_red);
         PHUB_HIP(c, hipGetLastError());
         double yy[2];
         rc = phub_reduce_partials(c, (int)g, 2, yy);
