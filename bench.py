@@ -196,7 +196,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(gbs / 8000.0, 4),
                      "traffic": _profile_traffic("k_hv_hubbard_4x4.md"),
-                     "kernel": "k_hv_up<float,2> + k_hv_dn<float,false> (one H*v)",
+                     "kernel": "k_hv_up<float,3> + k_hv_dn<float,false> (one H*v)",
                      "valu_fp32_tflops": round(flops / (ms_avg * 1e-3) / 1e12, 2)},
     }
 
@@ -694,7 +694,7 @@ def _u4_bench(args, ctx, torch):
         "u4": {"fp64_ms": res["fp64"], "fp32_ms": res["fp32"], "terms_per_state": pairs * pairs},
         "roofline": {"bound": "mfma", "achieved": round(tf, 3), "peak": 78.6, "unit": "TFLOP/s",
                      "frac": round(tf / 78.6, 4), "traffic": None,
-                     "kernel": "k_hv_up<double,1> + k_hv_dn + k_hv_ul<double> (one H*v)"},
+                     "kernel": "k_hv_ul_rows<double> (the 4-index term of one H*v; k_hv_up + k_hv_dn add the one-body part)"},
     }
     print(json.dumps(out), flush=True)
 
@@ -881,7 +881,7 @@ def _grad_bench(args, ctx, torch):
     gbs = 2 * 8 * D / (hv_ms * 1e-3) / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
                        "frac": round(gbs / 8000.0, 4), "traffic": None,
-                       "kernel": "k_hv_up<double,1> + k_hv_dn<double,false> (one FP64 H*v, "
+                       "kernel": "k_hv_up<double,2> + k_hv_dn<double,false> (one FP64 H*v, "
                                  f"D = {D}, {hv_ms * 1e3:.1f} us)"}
     if not args.no_cpu:
         out["cpu_baseline"] = _grad_cpu_baseline(spec, t, eps, U, out["grad"]["mean_minres_iters"])

@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void k_hv_hub(HubArgs<T> a, const T *__restric
 // ---------------------------------------------------------------------------
 constexpr int UP_THREADS = 1024;
 constexpr int UP_CLS_BYTES = 256 * 8;   // coefficient slots ahead of the rows in LDS
-constexpr int UP_LDS_MAX = 150 * 1024;  // coefficient slots + staged rows
+constexpr int UP_LDS_MAX = 159 * 1024;  // coefficient slots + staged rows (of 160 KB per CU)
 constexpr int UP_UNROLL = 8;    // independent hop-entry loads in flight per lane
 
 template <typename T, int R>
@@ -163,7 +163,10 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
                                                       T *__restrict__ out) {
     // LDS: 256 coefficient slots of 8 bytes (UP_CLS_BYTES), then the R rows
     // interleaved: element u of row r at rows[u * R + r], so one gather of column j
-    // fetches all R rows (a single 64-bit LDS read for two FP32 rows)
+    // fetches all R rows (a single 64-bit LDS read for two FP32 rows).  Three FP32
+    // rows: rows 0 and 1 interleaved as for two, row 2 in an array of its own after
+    // them (a 64-bit and a 32-bit read per column); every hop entry then serves three
+    // rows, so the entry table streams through L2 a third less often
     // a static array (not dynamic shared memory): its LDS address is a compile-time
     // constant, so both decoded offsets go straight into the ds_read address
     __shared__ __attribute__((aligned(16))) double smem_up[UP_LDS_MAX / 8];
@@ -176,7 +179,10 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
 #pragma unroll
         for (int r = 0; r < R; ++r)
             v[r] = (d0 + r < a.dim_dn) ? __builtin_nontemporal_load(x + (d0 + r) * (int64_t)nu + u) : (T)0;
-        if constexpr (R == 2 && sizeof(T) == 4) {
+        if constexpr (R == 3) {
+            reinterpret_cast<float2 *>(rows)[u] = make_float2(v[0], v[1]);
+            rows[2 * nu + u] = v[2];
+        } else if constexpr (R == 2 && sizeof(T) == 4) {
             reinterpret_cast<float2 *>(rows)[u] = make_float2(v[0], v[1]);
         } else {
 #pragma unroll
@@ -189,12 +195,19 @@ __global__ __launch_bounds__(UP_THREADS) void k_hv_up(HubArgs<T> a,
     // (two FP64); a packed entry carries both LDS byte offsets for 8-byte columns:
     // e = (j * 8) << 11 | class * 8, so the decode is one shift and one mask (one
     // more shift for 16-byte columns)
-    static_assert(R * sizeof(T) == 8 || R * sizeof(T) == 16, "8- or 16-byte LDS columns");
+    static_assert(R * sizeof(T) == 8 || R * sizeof(T) == 16 || (R == 3 && sizeof(T) == 4),
+                  "8- or 16-byte LDS columns, or three FP32 rows");
     constexpr int CSH = R * sizeof(T) == 16 ? 1 : 0;   // column offset = (j * 8) << CSH
     const char *rowb = lds + UP_CLS_BYTES;
+    const char *rowb2 = rowb + 8 * (size_t)nu;          // row 2 of three FP32 rows: 4 j
 
     auto gather = [&](uint32_t off, T *v) {
-        if constexpr (R == 2 && sizeof(T) == 4) {
+        if constexpr (R == 3) {
+            const float2 p = *reinterpret_cast<const float2 *>(rowb + off);
+            v[0] = p.x;
+            v[1] = p.y;
+            v[2] = *reinterpret_cast<const float *>(rowb2 + (off >> 1));
+        } else if constexpr (R == 2 && sizeof(T) == 4) {
             const float2 p = *reinterpret_cast<const float2 *>(rowb + off);
             v[0] = p.x;
             v[1] = p.y;
@@ -1646,6 +1659,11 @@ static bool phub_force_r1() {
     const char *e = std::getenv("PHUB_UP_ROWS");
     return e && e[0] == '1';
 }
+// PHUB_UP_ROWS=2 keeps FP32 on two rows per workgroup where three fit
+static bool phub_force_r2() {
+    const char *e = std::getenv("PHUB_UP_ROWS");
+    return e && e[0] == '2';
+}
 
 // the one-body (T, per-site U) Hubbard operator and the Heisenberg operator
 template <typename T>
@@ -1654,11 +1672,14 @@ static int hv_launch_main(phub_ctx *c, const phub_op *op, const T *x, T *out, do
     int nblocks = 0;
     const bool epi = want_dots || prev != nullptr || scale != 1.0;
     if (op->kind == 0 && op->up.packed && op->dn.csr) {
-        // two-kernel path: R rows of the state in LDS for the up hops (FP32: 2; FP64: 2
-        // when two rows fit, else 1)
+        // two-kernel path: R rows of the state in LDS for the up hops (FP32: 3 when
+        // three rows fit, else 2; FP64: 2 when two rows fit, else 1)
         const size_t row_bytes = (size_t)op->dim_up * sizeof(T);
         if (sizeof(T) == 8 && UP_CLS_BYTES + 2 * row_bytes <= UP_LDS_MAX && !phub_force_r1())
             return hv_two_kernel<T, 2>(c, op, x, out, scale, prev, beta, epi, nblocks_out);
+        if constexpr (sizeof(T) == 4)
+            if (UP_CLS_BYTES + 3 * row_bytes <= UP_LDS_MAX && !phub_force_r2())
+                return hv_two_kernel<T, 3>(c, op, x, out, scale, prev, beta, epi, nblocks_out);
         constexpr int R = sizeof(T) == 4 ? 2 : 1;
         if (UP_CLS_BYTES + R * row_bytes <= UP_LDS_MAX)
             return hv_two_kernel<T, R>(c, op, x, out, scale, prev, beta, epi, nblocks_out);

@@ -3,9 +3,11 @@ oracle (hprod_hubbard.f90:17-118 restated, bit-exact against the reference):
 
 - lattice hoppings (few signed values) and random hoppings (many); a sector whose
   rows do not fit LDS (the fallback kernel);
-- FP64 with two rows per workgroup where they fit (dim_up <= 9,472), and with one
+- FP64 with two rows per workgroup where they fit (dim_up <= 10,048), and with one
   row: forced by PHUB_UP_ROWS=1 on small sectors, and on a sector whose up
-  dimension (11,440) is past the two-row limit, over every down-hop tile.
+  dimension (11,440) is past the two-row limit, over every down-hop tile;
+- FP32 with three rows per workgroup where they fit (dim_up <= 13,397; every FP32
+  H*v of the other tests), and with two: forced by PHUB_UP_ROWS=2.
 
 Tolerances: FP64 within 1e-13 of the oracle relative to max|w|; FP32 within 2e-5."""
 import numpy as np
@@ -81,6 +83,21 @@ def test_fp64_one_row_forced(ctx, monkeypatch, lattice):
     else:
         T = rng.normal(size=(n, n))
         T = (T + T.T) / 2
+    U = rng.uniform(2, 6, n)
+    op = HubbardOp(ctx, n, nu, nd, T, U)
+    _check(op, O.HubbardModel(n, nu, nd, T, U), rng)
+
+
+@pytest.mark.parametrize("n,nu,nd,seed", [(7, 3, 3, 31), (13, 6, 1, 32)])
+def test_fp32_two_rows_forced(ctx, monkeypatch, n, nu, nd, seed):
+    """PHUB_UP_ROWS=2: FP32 on two rows per up-kernel workgroup where three fit (the
+    path of up sectors between 13,398 and 18,944 states), random hoppings, odd down
+    dimensions (35 and 13) that leave the last workgroup one row."""
+    from pyhub_amd.engine import HubbardOp
+    monkeypatch.setenv("PHUB_UP_ROWS", "2")
+    rng = np.random.default_rng(seed)
+    T = rng.normal(size=(n, n))
+    T = (T + T.T) / 2
     U = rng.uniform(2, 6, n)
     op = HubbardOp(ctx, n, nu, nd, T, U)
     _check(op, O.HubbardModel(n, nu, nd, T, U), rng)
