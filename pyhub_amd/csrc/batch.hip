@@ -33,6 +33,9 @@ constexpr int BT = 1024;          // threads per workgroup
 constexpr int MAXN = 16;          // sites per cluster in the batched path
 constexpr int MAXSTEPS = 256;     // Lanczos steps per task
 constexpr int64_t BIG_GS = 100000;  // ground states of at least this many states: full device
+// dynamic LDS of the persistent workgroup (one per CU) for the H*v's row staging:
+// 19,968 FP64 or 39,936 FP32 elements, at least one row of a 16-site sector (12,870)
+constexpr int BATCH_STG_BYTES = 156 * 1024;
 
 struct SectorDev {
     int64_t nu, nd;
@@ -117,73 +120,89 @@ __device__ inline void wg_sum2(double &a, double &b, double *sh) {
     b = s1;
 }
 
-// out = scale * (H x) - bprev * xp ; returns (<scale x, out>, |out|^2) workgroup-wide
+// out = scale * (H x) - bprev * xp ; returns (<scale x, out>, |out|^2) workgroup-wide.
+// The rows of x go through LDS in blocks of whole rows (stg: cap elements of the
+// workgroup's dynamic LDS): the diagonal, every up hop (its source is in the same
+// row) and the dot product read the staged block, so only the down hops gather from
+// global memory.  A block index fits 32 bits, so the row and column of a state come
+// from a 32-bit division.
 template <typename V>
 __device__ void wg_hv(const ClusterDev &c, const SectorDev &s, const V *__restrict__ x,
                       double scale, const V *__restrict__ xp, double bprev,
-                      V *__restrict__ out, double &d_xo, double &d_oo, double *sh) {
-    const int64_t D = s.nu * s.nd;
+                      V *__restrict__ out, double &d_xo, double &d_oo, double *sh,
+                      V *__restrict__ stg, int cap) {
+    const uint32_t nu = (uint32_t)s.nu;
+    const int64_t nd = s.nd;
+    const int64_t rb = (int64_t)cap / (int64_t)nu;   // rows per block (>= 1: cap >= nu)
     double a = 0.0, b = 0.0;
-    for (int64_t e = threadIdx.x; e < D; e += BT) {
-        const int64_t d = e / s.nu, u = e - d * s.nu;
-        const uint32_t xu = s.mu[u], xd = s.md[d];
-        double dg = s.eu[u] + s.ed[d];
-        const uint32_t both = xu & xd;
-        if (c.u_uniform) {
-            dg += c.u0 * (double)__popc(both);
-        } else {
-            uint32_t bb = both;
-            while (bb) {
-                const int bt = __ffs(bb) - 1;
-                dg += c.U[c.n - 1 - bt];
-                bb &= bb - 1;
+    for (int64_t d0 = 0; d0 < nd; d0 += rb) {
+        const uint32_t nrow = (uint32_t)min<int64_t>(rb, nd - d0);
+        const uint32_t nel = nrow * nu;
+        const V *__restrict__ xb = x + d0 * (int64_t)nu;
+        __syncthreads();                               // the previous block is read
+        for (uint32_t l = threadIdx.x; l < nel; l += BT) stg[l] = xb[l];
+        __syncthreads();
+        for (uint32_t l = threadIdx.x; l < nel; l += BT) {
+            const uint32_t dl = l / nu, u = l - dl * nu;
+            const int64_t d = d0 + dl, e = d * (int64_t)nu + u;
+            const uint32_t xu = s.mu[u], xd = s.md[d];
+            double dg = s.eu[u] + s.ed[d];
+            const uint32_t both = xu & xd;
+            if (c.u_uniform) {
+                dg += c.u0 * (double)__popc(both);
+            } else {
+                uint32_t bb = both;
+                while (bb) {
+                    const int bt = __ffs(bb) - 1;
+                    dg += c.U[c.n - 1 - bt];
+                    bb &= bb - 1;
+                }
             }
-        }
-        V acc = (V)dg * x[e];
-        // hops four at a time: their table loads, then their gathers, are in flight
-        // together (the loop is bound by memory latency, not by issue)
-        const int cu = s.cu[u];
-        const V *xr = x + d * s.nu;
-        int k = 0;
-        for (; k + 4 <= cu; k += 4) {
-            int32_t j[4];
-            V cf[4], g[4];
+            const V *sr = stg + dl * nu;               // the state's row, staged
+            const V xe = sr[u];
+            V acc = (V)dg * xe;
+            // hops four at a time: their table loads, then their gathers, are in flight
+            // together (the down gathers are bound by memory latency, not by issue)
+            const int cu = s.cu[u];
+            int k = 0;
+            for (; k + 4 <= cu; k += 4) {
+                int32_t j[4];
+                V cf[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                j[q] = s.iu[(size_t)(k + q) * s.nu + u];
-                cf[q] = cf_up<V>(s, (size_t)(k + q) * s.nu + u);
+                for (int q = 0; q < 4; ++q) {
+                    j[q] = s.iu[(size_t)(k + q) * nu + u];
+                    cf[q] = cf_up<V>(s, (size_t)(k + q) * nu + u);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc += cf[q] * sr[j[q]];
             }
+            for (; k < cu; ++k)
+                acc += cf_up<V>(s, (size_t)k * nu + u) * sr[s.iu[(size_t)k * nu + u]];
+            const int cd = s.cd[d];
+            const int32_t *idr = s.id + (size_t)d * s.maxh_dn;
+            k = 0;
+            for (; k + 4 <= cd; k += 4) {
+                int32_t j[4];
+                V cf[4], g[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) g[q] = xr[j[q]];
+                for (int q = 0; q < 4; ++q) {
+                    j[q] = idr[k + q];
+                    cf[q] = cf_dn<V>(s, (size_t)d * s.maxh_dn + k + q);
+                }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc += cf[q] * g[q];
-        }
-        for (; k < cu; ++k)
-            acc += cf_up<V>(s, (size_t)k * s.nu + u) * xr[s.iu[(size_t)k * s.nu + u]];
-        const int cd = s.cd[d];
-        const int32_t *idr = s.id + (size_t)d * s.maxh_dn;
-        k = 0;
-        for (; k + 4 <= cd; k += 4) {
-            int32_t j[4];
-            V cf[4], g[4];
+                for (int q = 0; q < 4; ++q) g[q] = x[(int64_t)j[q] * nu + u];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                j[q] = idr[k + q];
-                cf[q] = cf_dn<V>(s, (size_t)d * s.maxh_dn + k + q);
+                for (int q = 0; q < 4; ++q) acc += cf[q] * g[q];
             }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) g[q] = x[(int64_t)j[q] * s.nu + u];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc += cf[q] * g[q];
+            for (; k < cd; ++k)
+                acc += cf_dn<V>(s, (size_t)d * s.maxh_dn + k) * x[(int64_t)idr[k] * nu + u];
+            double v = scale * (double)acc;
+            if (xp) v -= bprev * (double)xp[e];
+            const V vt = (V)v;
+            out[e] = vt;
+            a += scale * (double)xe * (double)vt;
+            b += (double)vt * (double)vt;
         }
-        for (; k < cd; ++k)
-            acc += cf_dn<V>(s, (size_t)d * s.maxh_dn + k) * x[(int64_t)idr[k] * s.nu + u];
-        double v = scale * (double)acc;
-        if (xp) v -= bprev * (double)xp[e];
-        const V vt = (V)v;
-        out[e] = vt;
-        a += scale * (double)x[e] * (double)vt;
-        b += (double)vt * (double)vt;
     }
     wg_sum2(a, b, sh);
     d_xo = a;
@@ -215,8 +234,8 @@ __device__ double tri_lowest(int m, const double *al, const double *be, double *
 template <typename V>
 __device__ int wg_lanczos(const ClusterDev &c, const SectorDev &s, V *v0, V *b1, V *b2,
                           int steps, double *alpha, double *beta, double norm0, const double *y,
-                          V *psi, double *sh, double tol = 0.0, double *ty = nullptr,
-                          double *twk = nullptr) {
+                          V *psi, double *sh, V *stg, int cap, double tol = 0.0,
+                          double *ty = nullptr, double *twk = nullptr) {
     __shared__ int stop_sh;
     const int64_t D = s.nu * s.nd;
     V *xp = b1, *xk = v0, *out = b2;
@@ -228,7 +247,7 @@ __device__ int wg_lanczos(const ClusterDev &c, const SectorDev &s, V *v0, V *b1,
         }
         if (k == steps - 1 && psi) { m = steps; break; }
         double dxo, doo;
-        wg_hv(c, s, xk, sig, k ? xp : nullptr, bk * sigp, out, dxo, doo, sh);
+        wg_hv(c, s, xk, sig, k ? xp : nullptr, bk * sigp, out, dxo, doo, sh, stg, cap);
         const double a = dxo;
         const double nn = wg_update(out, xk, a * sig, D, sh);
         const double b = sqrt(fmax(nn, 0.0));
@@ -350,6 +369,12 @@ template <int KIND>
 __global__ __launch_bounds__(BT) void k_batch(BatchDev B) {
     __shared__ double sh[2 * BT / 64];
     __shared__ int task_sh;
+    // row staging of the H*v: the dynamic LDS (BATCH_STG_BYTES), FP64 for the ground
+    // states, FP32 for the chains
+    extern __shared__ __attribute__((aligned(16))) unsigned char bstg[];
+    double *stg64 = reinterpret_cast<double *>(bstg);
+    float *stg32 = reinterpret_cast<float *>(bstg);
+    constexpr int cap64 = BATCH_STG_BYTES / 8, cap32 = BATCH_STG_BYTES / 4;
     // FP64 view for the ground state, FP32 view (front half) for the chains
     double *d0 = B.scratch + (size_t)blockIdx.x * 3 * B.dmax;
     double *d1 = d0 + B.dmax, *d2 = d1 + B.dmax;
@@ -381,7 +406,7 @@ __global__ __launch_bounds__(BT) void k_batch(BatchDev B) {
             double *al = tri, *be = tri + MAXSTEPS, *y = tri + 2 * MAXSTEPS, *wk = tri + 3 * MAXSTEPS;
             const int steps = (int)min<int64_t>(B.gs_steps, Dg);
             const int m = wg_lanczos<double>(c, g, d0, d1, d2, steps, al, be, nrm, nullptr, nullptr, sh,
-                                             B.gs_tol, y, wk);
+                                             stg64, cap64, B.gs_tol, y, wk);
             __syncthreads();
             if (threadIdx.x == 0) B.e0[tk.cluster] = tri_lowest(m, al, be, y, wk);
             __syncthreads();
@@ -391,7 +416,7 @@ __global__ __launch_bounds__(BT) void k_batch(BatchDev B) {
                 c.psi0[e] = 0.0;
             }
             __syncthreads();
-            wg_lanczos<double>(c, g, d0, d1, d2, m, al, be, nrm, y, c.psi0, sh);
+            wg_lanczos<double>(c, g, d0, d1, d2, m, al, be, nrm, y, c.psi0, sh, stg64, cap64);
             __syncthreads();
             // normalise psi0
             double p2 = 0.0, d2s = 0.0;
@@ -421,7 +446,8 @@ __global__ __launch_bounds__(BT) void k_batch(BatchDev B) {
                 // stopping at the sector dimension would keep an O(1) last beta (and
                 // drop poles not yet converged); the extra steps only add ghost copies
                 // of converged poles, whose weights sum to the pole's own
-                wg_lanczos<float>(c, t, w0, w1, w2, B.gf_steps, ga, gb, sqrt(n2), nullptr, nullptr, sh);
+                wg_lanczos<float>(c, t, w0, w1, w2, B.gf_steps, ga, gb, sqrt(n2), nullptr, nullptr, sh,
+                                  stg32, cap32);
             }
             __syncthreads();
         }
@@ -735,8 +761,23 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
         int rc = bw.prepare(c, big, ops_gs, o);
         if (rc) { cleanup(); return rc; }
     }
+    {
+        static bool attr_set[PHUB_MAX_DEVICES];   // once per device and process
+        if (c->device < 0 || c->device >= PHUB_MAX_DEVICES) {
+            cleanup();
+            return phub_fail(c, PHUB_ERR_UNSUPPORTED, "device index beyond PHUB_MAX_DEVICES");
+        }
+        if (!attr_set[c->device]) {
+            PHUB_HIP(c, hipFuncSetAttribute((const void *)k_batch<0>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, BATCH_STG_BYTES));
+            PHUB_HIP(c, hipFuncSetAttribute((const void *)k_batch<1>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, BATCH_STG_BYTES));
+            attr_set[c->device] = true;
+        }
+    }
     if (ngs > 0)
-        hipLaunchKernelGGL(k_batch<0>, dim3(std::max(1, std::min(grid, ngs))), dim3(BT), 0, c->stream, B);
+        hipLaunchKernelGGL(k_batch<0>, dim3(std::max(1, std::min(grid, ngs))), dim3(BT), BATCH_STG_BYTES,
+                           c->stream, B);
     PHUB_HIP(c, hipGetLastError());
     if (!big.empty()) {
         int rc = bw.run(c, big, ops_gs, hc, o, de0.p);
@@ -751,8 +792,8 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
     B.first = ngs;
     B.last = (int)tasks.size();
     if (B.last > B.first)
-        hipLaunchKernelGGL(k_batch<1>, dim3(std::max(1, std::min(grid, B.last - B.first))), dim3(BT), 0,
-                           c->stream, B);
+        hipLaunchKernelGGL(k_batch<1>, dim3(std::max(1, std::min(grid, B.last - B.first))), dim3(BT),
+                           BATCH_STG_BYTES, c->stream, B);
     PHUB_HIP(c, hipGetLastError());
     PHUB_HIP(c, hipStreamSynchronize(c->stream));
     PHUB_HIP(c, phub_copy(c, e0, de0.p, ncl * sizeof(double), hipMemcpyDeviceToHost));
