@@ -90,10 +90,18 @@ def gii_omega(e0, ga, gb, gn, clusters, omega, eta=0.05, ctx: Context | None = N
     ctx = ctx or Context.default()
     ncl = len(clusters)
     m = ga.shape[-1]
-    a = ga.reshape(-1, m)
-    b = gb.reshape(-1, m)
-    n2 = gn.reshape(-1)
-    shift = np.repeat(np.asarray(e0), 32 * 2 * 2)
-    sign = np.tile(np.array([1.0, -1.0]), ncl * 32 * 2)     # particle +, hole -
-    G = cf_eval(ctx, a, b, n2, shift, sign, np.asarray(omega, float), eta)
-    return G.reshape(ncl, 32, 2, 2, len(omega)).sum(axis=3)
+    nw = len(omega)
+    # only the chains of sites a cluster has (slot [c, site-1, spin, kind]); the other
+    # slots of the (ncl, 32, 2) result stay zero
+    nsite = np.array([cl.n for cl in clusters])
+    site_ok = np.arange(32)[None, :] < nsite[:, None]                    # (ncl, 32)
+    sel = np.broadcast_to(site_ok[:, :, None, None], (ncl, 32, 2, 2)).reshape(-1)
+    a = ga.reshape(-1, m)[sel]
+    b = gb.reshape(-1, m)[sel]
+    n2 = gn.reshape(-1)[sel]
+    shift = np.repeat(np.asarray(e0), 32 * 2 * 2)[sel]
+    sign = np.tile(np.array([1.0, -1.0]), ncl * 32 * 2)[sel]           # particle +, hole -
+    Gc = cf_eval(ctx, a, b, n2, shift, sign, np.asarray(omega, float), eta)
+    G = np.zeros((ncl, 32, 2, nw), dtype=np.complex128)
+    G[np.broadcast_to(site_ok[:, :, None], (ncl, 32, 2))] = Gc[0::2] + Gc[1::2]
+    return G

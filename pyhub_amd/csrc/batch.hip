@@ -33,8 +33,9 @@ constexpr int BT = 1024;          // threads per workgroup
 constexpr int MAXN = 16;          // sites per cluster in the batched path
 constexpr int MAXSTEPS = 256;     // Lanczos steps per task
 constexpr int64_t BIG_GS = 100000;  // ground states of at least this many states: full device
-// dynamic LDS of the persistent workgroup (one per CU) for the H*v's row staging:
-// 19,968 FP64 or 39,936 FP32 elements, at least one row of a 16-site sector (12,870)
+// dynamic LDS of the persistent workgroup (one per CU) for the H*v's row and column
+// staging: 19,968 FP64 or 39,936 FP32 elements, at least one row or column of a
+// 16-site sector (12,870)
 constexpr int BATCH_STG_BYTES = 156 * 1024;
 
 struct SectorDev {
@@ -121,30 +122,33 @@ __device__ inline void wg_sum2(double &a, double &b, double *sh) {
 }
 
 // out = scale * (H x) - bprev * xp ; returns (<scale x, out>, |out|^2) workgroup-wide.
-// The rows of x go through LDS in blocks of whole rows (stg: cap elements of the
-// workgroup's dynamic LDS): the diagonal, every up hop (its source is in the same
-// row) and the dot product read the staged block, so only the down hops gather from
-// global memory.  A block index fits 32 bits, so the row and column of a state come
-// from a 32-bit division.
+// Two passes through the workgroup's dynamic LDS (stg, cap elements), so that every
+// gather reads LDS:
+//  1. blocks of whole rows of x: the diagonal and the up hops (a source is in the
+//     state's own row); out holds their sum;
+//  2. blocks of whole columns (every row's slice of ncol columns, stg[d][c]): the down
+//     hops (a source is in the state's own column), then the epilogue and the dots.
+// Each pass reads x once more than a single pass would (and out once), in place of the
+// down hops' row gathers from global memory.  Indices within a block fit 32 bits.
+// Between the passes, out written by one thread is read by another of the workgroup
+// after a barrier (workgroup-scope visibility, as between the steps of a chain).
 template <typename V>
 __device__ void wg_hv(const ClusterDev &c, const SectorDev &s, const V *__restrict__ x,
                       double scale, const V *__restrict__ xp, double bprev,
                       V *__restrict__ out, double &d_xo, double &d_oo, double *sh,
                       V *__restrict__ stg, int cap) {
-    const uint32_t nu = (uint32_t)s.nu;
-    const int64_t nd = s.nd;
-    const int64_t rb = (int64_t)cap / (int64_t)nu;   // rows per block (>= 1: cap >= nu)
-    double a = 0.0, b = 0.0;
-    for (int64_t d0 = 0; d0 < nd; d0 += rb) {
-        const uint32_t nrow = (uint32_t)min<int64_t>(rb, nd - d0);
+    const uint32_t nu = (uint32_t)s.nu, nd = (uint32_t)s.nd;
+    const uint32_t rb = (uint32_t)cap / nu;    // rows per block (>= 1: cap >= nu)
+    for (uint32_t d0 = 0; d0 < nd; d0 += rb) {
+        const uint32_t nrow = min(rb, nd - d0);
         const uint32_t nel = nrow * nu;
-        const V *__restrict__ xb = x + d0 * (int64_t)nu;
+        const V *__restrict__ xb = x + (int64_t)d0 * nu;
         __syncthreads();                               // the previous block is read
         for (uint32_t l = threadIdx.x; l < nel; l += BT) stg[l] = xb[l];
         __syncthreads();
         for (uint32_t l = threadIdx.x; l < nel; l += BT) {
             const uint32_t dl = l / nu, u = l - dl * nu;
-            const int64_t d = d0 + dl, e = d * (int64_t)nu + u;
+            const uint32_t d = d0 + dl;
             const uint32_t xu = s.mu[u], xd = s.md[d];
             double dg = s.eu[u] + s.ed[d];
             const uint32_t both = xu & xd;
@@ -159,10 +163,8 @@ __device__ void wg_hv(const ClusterDev &c, const SectorDev &s, const V *__restri
                 }
             }
             const V *sr = stg + dl * nu;               // the state's row, staged
-            const V xe = sr[u];
-            V acc = (V)dg * xe;
-            // hops four at a time: their table loads, then their gathers, are in flight
-            // together (the down gathers are bound by memory latency, not by issue)
+            V acc = (V)dg * sr[u];
+            // hops four at a time: their table loads, then their gathers, in flight together
             const int cu = s.cu[u];
             int k = 0;
             for (; k + 4 <= cu; k += 4) {
@@ -178,29 +180,45 @@ __device__ void wg_hv(const ClusterDev &c, const SectorDev &s, const V *__restri
             }
             for (; k < cu; ++k)
                 acc += cf_up<V>(s, (size_t)k * nu + u) * sr[s.iu[(size_t)k * nu + u]];
+            out[(int64_t)d * nu + u] = acc;
+        }
+    }
+    const uint32_t cb = (uint32_t)cap / nd;    // columns per block (>= 1: cap >= nd)
+    double a = 0.0, b = 0.0;
+    for (uint32_t u0 = 0; u0 < nu; u0 += cb) {
+        const uint32_t ncol = min(cb, nu - u0);
+        const uint32_t nel = ncol * nd;
+        __syncthreads();                               // the previous block (and pass 1) done
+        for (uint32_t l = threadIdx.x; l < nel; l += BT) {
+            const uint32_t d = l / ncol, cc = l - d * ncol;
+            stg[l] = x[(int64_t)d * nu + u0 + cc];
+        }
+        __syncthreads();
+        for (uint32_t l = threadIdx.x; l < nel; l += BT) {
+            const uint32_t d = l / ncol, cc = l - d * ncol;
+            const int64_t e = (int64_t)d * nu + u0 + cc;
+            V acc = out[e];
             const int cd = s.cd[d];
             const int32_t *idr = s.id + (size_t)d * s.maxh_dn;
-            k = 0;
+            int k = 0;
             for (; k + 4 <= cd; k += 4) {
                 int32_t j[4];
-                V cf[4], g[4];
+                V cf[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     j[q] = idr[k + q];
                     cf[q] = cf_dn<V>(s, (size_t)d * s.maxh_dn + k + q);
                 }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) g[q] = x[(int64_t)j[q] * nu + u];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc += cf[q] * g[q];
+                for (int q = 0; q < 4; ++q) acc += cf[q] * stg[(uint32_t)j[q] * ncol + cc];
             }
             for (; k < cd; ++k)
-                acc += cf_dn<V>(s, (size_t)d * s.maxh_dn + k) * x[(int64_t)idr[k] * nu + u];
+                acc += cf_dn<V>(s, (size_t)d * s.maxh_dn + k) * stg[(uint32_t)idr[k] * ncol + cc];
             double v = scale * (double)acc;
             if (xp) v -= bprev * (double)xp[e];
             const V vt = (V)v;
             out[e] = vt;
-            a += scale * (double)xe * (double)vt;
+            a += scale * (double)stg[l] * (double)vt;
             b += (double)vt * (double)vt;
         }
     }
