@@ -125,7 +125,8 @@ __device__ inline void wg_sum2(double &a, double &b, double *sh) {
 // Two passes through the workgroup's dynamic LDS (stg, cap elements), so that every
 // gather reads LDS:
 //  1. blocks of whole rows of x: the diagonal and the up hops (a source is in the
-//     state's own row); out holds their sum;
+//     state's own row), one thread per column and group of rows so that each
+//     up-table entry serves the group; out holds their sum;
 //  2. blocks of whole columns (every row's slice of ncol columns, stg[d][c]): the down
 //     hops (a source is in the state's own column), then the epilogue and the dots.
 // Each pass reads x once more than a single pass would (and out once), in place of the
@@ -146,41 +147,47 @@ __device__ void wg_hv(const ClusterDev &c, const SectorDev &s, const V *__restri
         __syncthreads();                               // the previous block is read
         for (uint32_t l = threadIdx.x; l < nel; l += BT) stg[l] = xb[l];
         __syncthreads();
-        for (uint32_t l = threadIdx.x; l < nel; l += BT) {
-            const uint32_t dl = l / nu, u = l - dl * nu;
-            const uint32_t d = d0 + dl;
-            const uint32_t xu = s.mu[u], xd = s.md[d];
-            double dg = s.eu[u] + s.ed[d];
-            const uint32_t both = xu & xd;
-            if (c.u_uniform) {
-                dg += c.u0 * (double)__popc(both);
-            } else {
-                uint32_t bb = both;
-                while (bb) {
-                    const int bt = __ffs(bb) - 1;
-                    dg += c.U[c.n - 1 - bt];
-                    bb &= bb - 1;
+        // one thread per column and group of RG rows: each up-table entry (loaded once)
+        // serves the RG rows (8 in FP32; 4 in FP64, whose path already spills registers)
+        constexpr uint32_t RG = sizeof(V) == 4 ? 8 : 4;
+        const uint32_t ngrp = (nrow + RG - 1) / RG, nitem = ngrp * nu;
+        for (uint32_t l = threadIdx.x; l < nitem; l += BT) {
+            const uint32_t g4 = l / nu, u = l - g4 * nu;
+            const uint32_t r0 = RG * g4;
+            const uint32_t xu = s.mu[u];
+            const double eu = s.eu[u];
+            V acc[RG];
+#pragma unroll
+            for (int r = 0; r < (int)RG; ++r) {
+                const uint32_t rr = min(r0 + r, nrow - 1);   // rows past the block repeat the last
+                const uint32_t d = d0 + rr;
+                double dg = eu + s.ed[d];
+                const uint32_t both = xu & s.md[d];
+                if (c.u_uniform) {
+                    dg += c.u0 * (double)__popc(both);
+                } else {
+                    uint32_t bb = both;
+                    while (bb) {
+                        const int bt = __ffs(bb) - 1;
+                        dg += c.U[c.n - 1 - bt];
+                        bb &= bb - 1;
+                    }
                 }
+                acc[r] = (V)dg * stg[rr * nu + u];
             }
-            const V *sr = stg + dl * nu;               // the state's row, staged
-            V acc = (V)dg * sr[u];
-            // hops four at a time: their table loads, then their gathers, in flight together
+            const V *sr[RG];
+#pragma unroll
+            for (int r = 0; r < (int)RG; ++r) sr[r] = stg + min(r0 + r, nrow - 1) * nu;
             const int cu = s.cu[u];
-            int k = 0;
-            for (; k + 4 <= cu; k += 4) {
-                int32_t j[4];
-                V cf[4];
+            for (int k = 0; k < cu; ++k) {
+                const int32_t j = s.iu[(size_t)k * nu + u];
+                const V cf = cf_up<V>(s, (size_t)k * nu + u);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    j[q] = s.iu[(size_t)(k + q) * nu + u];
-                    cf[q] = cf_up<V>(s, (size_t)(k + q) * nu + u);
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc += cf[q] * sr[j[q]];
+                for (int r = 0; r < (int)RG; ++r) acc[r] += cf * sr[r][j];
             }
-            for (; k < cu; ++k)
-                acc += cf_up<V>(s, (size_t)k * nu + u) * sr[s.iu[(size_t)k * nu + u]];
-            out[(int64_t)d * nu + u] = acc;
+#pragma unroll
+            for (int r = 0; r < (int)RG; ++r)
+                if (r0 + r < nrow) out[(int64_t)(d0 + r0 + r) * nu + u] = acc[r];
         }
     }
     const uint32_t cb = (uint32_t)cap / nd;    // columns per block (>= 1: cap >= nd)
