@@ -342,14 +342,17 @@ int phub_grad_hubbard(phub_ctx *ctx, const phub_op *op, const double *d_bra, con
 int phub_hf_grad_heisenberg(phub_ctx *ctx, const phub_op *op, const double *d_psi,
                             double *grad_j);
 /* Adjoint solve (H - e0) x = b with b orthogonal to psi, by MINRES restricted to
- * the complement of psi (FP64; seven work vectors of the operator dimension, from
- * the workspace or allocated for the call; one iteration is one H*v, four vector
- * passes and two host round trips).  Returns iterations in *iters. */
+ * the complement of psi (FP64; seven work vectors of the operator dimension and, on
+ * one-body Hubbard operators, the diagonal of the Jacobi preconditioner
+ * 1 / (diag(H) - e0), from the workspace or allocated for the call; one iteration is
+ * one H*v, four vector passes and two host round trips).  tol and *rel_res are
+ * relative residuals, in the preconditioner's norm when it applies
+ * (PHUB_JD_PRECOND=0 switches it off).  Returns iterations in *iters. */
 int phub_adjoint_minres(phub_ctx *ctx, const phub_op *op, const double *d_psi, double e0,
                         const double *d_b, double *d_x, int max_iter, double tol,
                         int *iters, double *rel_res, void *workspace /* device or NULL */,
                         size_t workspace_bytes);
-/* Device bytes of the seven MINRES work vectors. */
+/* Device bytes of the MINRES work vectors (eight vectors of the dimension). */
 int phub_adjoint_minres_workspace_bytes(const phub_op *op, size_t *bytes);
 
 /* ---- generic operator strings (operators.x) ---------------------------------------

@@ -1243,8 +1243,9 @@ static HubArgs<T> hub_args(const phub_op *op, const phub_ctx *c = nullptr) {
 // one-body Hubbard operator (the diagonal the H*v kernels add: eloc of both species
 // plus U over the doubly occupied sites), a Jacobi preconditioner for H - e near the
 // ground state (every H_ii >= e0, so the entries are positive)
+template <typename V>
 __global__ __launch_bounds__(256) void k_hub_minv(HubArgs<float> a, double e, double floor_,
-                                                 float *__restrict__ minv) {
+                                                 V *__restrict__ minv) {
     const int64_t n = a.dim_up * a.dim_dn;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -1261,20 +1262,23 @@ __global__ __launch_bounds__(256) void k_hub_minv(HubArgs<float> a, double e, do
                 bb &= bb - 1;
             }
         }
-        minv[i] = (float)(1.0 / fmax(dg - e, floor_));
+        minv[i] = (V)(1.0 / fmax(dg - e, floor_));
     }
 }
 
-int phub_hub_minv(phub_ctx *c, const phub_op *op, double e, float *minv) {
+template <typename V>
+int phub_hub_minv(phub_ctx *c, const phub_op *op, double e, V *minv) {
     if (op->kind != 0 || op->d_ul)
         return phub_fail(c, PHUB_ERR_UNSUPPORTED, "diagonal preconditioner: one-body Hubbard operators only");
     const int64_t n = op->dim;
     const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_hub_minv, dim3(g), dim3(256), 0, c->stream, hub_args<float>(op), e, 1e-3,
+    hipLaunchKernelGGL(k_hub_minv<V>, dim3(g), dim3(256), 0, c->stream, hub_args<float>(op), e, 1e-3,
                        minv);
     PHUB_HIP(c, hipGetLastError());
     return PHUB_OK;
 }
+template int phub_hub_minv<float>(phub_ctx *, const phub_op *, double, float *);
+template int phub_hub_minv<double>(phub_ctx *, const phub_op *, double, double *);
 
 constexpr int HUB_R = 4;
 

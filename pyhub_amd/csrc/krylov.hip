@@ -1406,6 +1406,15 @@ __global__ __launch_bounds__(256) void k_jd_rhs(const double *__restrict__ r, fl
         b[i] = (float)(-r[i]);
 }
 
+// The MINRES solves on (H - e) (the corrections of psi, the adjoint solve) take the
+// Jacobi preconditioner 1 / (diag(H) - e) on one-body Hubbard operators: their diagonal,
+// U times the double occupancy, spreads the spectrum of H - e far more than the hops
+// do.  PHUB_JD_PRECOND=0 solves unpreconditioned.
+bool phub_minres_precond(const phub_op *op) {
+    const char *pe = std::getenv("PHUB_JD_PRECOND");
+    return op->kind == 0 && !op->d_ul && !(pe && pe[0] == '0');
+}
+
 static int jd_refine(phub_ctx *c, const phub_op *op, double *psi, Arena &ar, int rs,
                      double psi_tol, double *e0, double *res, phub_lanczos_result *out) {
     const int64_t n = op->dim;
@@ -1414,14 +1423,11 @@ static int jd_refine(phub_ctx *c, const phub_op *op, double *psi, Arena &ar, int
     float *b32 = (float *)ar.take(n * sizeof(float));
     float *x32 = (float *)ar.take(n * sizeof(float));
     if (!tmp || !b32 || !x32) return phub_fail(c, PHUB_ERR_OOM, "FP64 correction vectors do not fit");
-    // Jacobi preconditioner of the inner solves on one-body Hubbard operators (their
-    // diagonal, U times the double occupancy, spreads the spectrum of H - E far more
-    // than the hops do); PHUB_JD_PRECOND=0 solves unpreconditioned
+    // Jacobi preconditioner of the inner solves on one-body Hubbard operators
     float *psi32 = (float *)ar.take(n * sizeof(float));
     if (!psi32) return phub_fail(c, PHUB_ERR_OOM, "FP64 correction vectors do not fit");
     float *minv = nullptr;
-    const char *pe = std::getenv("PHUB_JD_PRECOND");
-    if (op->kind == 0 && !op->d_ul && !(pe && pe[0] == '0')) {
+    if (phub_minres_precond(op)) {
         minv = (float *)ar.take(n * sizeof(float));
         if (!minv) return phub_fail(c, PHUB_ERR_OOM, "FP64 correction vectors do not fit");
     }
@@ -2125,7 +2131,8 @@ extern "C" int phub_cf_chain(phub_ctx *c, const phub_op *op, const double *d_sta
 // ---------------------------------------------------------------------------
 extern "C" int phub_adjoint_minres_workspace_bytes(const phub_op *op, size_t *bytes) {
     if (!op || !bytes) return PHUB_ERR_ARG;
-    *bytes = 7 * (((size_t)op->dim * sizeof(double) + 255) & ~(size_t)255) + 256;
+    // seven work vectors and the preconditioner's diagonal
+    *bytes = 8 * (((size_t)op->dim * sizeof(double) + 255) & ~(size_t)255) + 256;
     return PHUB_OK;
 }
 
@@ -2251,8 +2258,15 @@ extern "C" int phub_adjoint_minres(phub_ctx *c, const phub_op *op, const double 
                                    int *iters, double *rel_res, void *workspace,
                                    size_t workspace_bytes) {
     if (!c || !op || !psi || !b || !x) return phub_fail(c, PHUB_ERR_ARG, "null argument");
-    // the seven vectors come from the caller's workspace, else are allocated for the call
+    // the work vectors come from the caller's workspace, else are allocated for the call
     Arena ar(c, workspace, workspace_bytes);
+    double *minv = nullptr;
+    if (phub_minres_precond(op)) {
+        minv = (double *)ar.take(op->dim * sizeof(double));
+        if (!minv) return phub_fail(c, PHUB_ERR_OOM, "MINRES preconditioner vector");
+        int rc = phub_hub_minv(c, op, e0, minv);
+        if (rc) return rc;
+    }
     return minres_core<double>(c, op, psi, e0, b, x, max_iter, tol, iters, rel_res, ar,
-                               (const double *)nullptr);
+                               (const double *)minv);
 }

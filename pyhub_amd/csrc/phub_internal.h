@@ -305,6 +305,10 @@ template <typename T>
 int phub_hv_launch(phub_ctx *ctx, const phub_op *op, const T *x, T *out, double scale,
                    const T *prev, double beta, bool want_dots, int *nblocks_out);
 
-// minv = 1 / max(diag(H) - e, 1e-3) of a one-body Hubbard operator, FP32, dim entries
-// (hv.hip); PHUB_ERR_UNSUPPORTED for other operators
-int phub_hub_minv(phub_ctx *ctx, const phub_op *op, double e, float *minv);
+// minv = 1 / max(diag(H) - e, 1e-3) of a one-body Hubbard operator, dim entries of V
+// (float or double; hv.hip); PHUB_ERR_UNSUPPORTED for other operators
+template <typename V>
+int phub_hub_minv(phub_ctx *ctx, const phub_op *op, double e, V *minv);
+// the Jacobi preconditioner of the MINRES solves applies to this operator (one-body
+// Hubbard) and is not switched off (PHUB_JD_PRECOND=0)
+bool phub_minres_precond(const phub_op *op);

@@ -276,7 +276,7 @@ class _Op:
     def minres(self, psi: torch.Tensor, e0: float, b: torch.Tensor, max_iter=500, tol=1e-10,
                workspace: torch.Tensor | None = None, out: torch.Tensor | None = None):
         """Adjoint MINRES (H - e0) x = b on the complement of psi; workspace of
-        minres_workspace() size for the seven work vectors (torch's caching allocator
+        minres_workspace() size for the eight work vectors (torch's caching allocator
         reuses it across calls when none is passed)."""
         x = self.vec(torch.float64) if out is None else out
         ws = workspace if workspace is not None else self.minres_workspace()
