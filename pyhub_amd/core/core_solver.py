@@ -36,7 +36,8 @@ def _genuine_ritz(val, u1, tol):
     Willoughby's spurious values); either would enter the spectrum as an excited state.
     Values within tol (relative) of a cluster's first member form one cluster, kept once
     through its member of largest |u1|; a value alone with |u1| < 1e-10 is dropped.  The
-    same rule as the engine's excited-state selection."""
+    lowest value is always kept as the ground state.  The same rule as the engine's
+    excited-state selection."""
     keep, i, m = [], 0, len(val)
     while i < m:
         j, best = i + 1, i
@@ -44,7 +45,9 @@ def _genuine_ritz(val, u1, tol):
             if abs(u1[j]) > abs(u1[best]):
                 best = j
             j += 1
-        if j - i > 1 or abs(u1[i]) >= 1e-10:
+        if i == 0:
+            best = 0       # the ground state: the lowest value, as the engine reports it
+        if i == 0 or j - i > 1 or abs(u1[i]) >= 1e-10:
             keep.append(best)
         i = j
     return np.asarray(keep, dtype=int)

@@ -800,7 +800,9 @@ static void ref_convergence(LczState &s, double tol, int max_steps, const double
 // of a cluster's first member are one cluster (the copies agree to about 1e-8 with FP64
 // vectors, 1e-5 with FP32 ones), represented by its member of largest |u1|; a value
 // alone with |u1| < 1e-10 is spurious (measured: below 1e-18 for the spurious values,
-// above 1e-6 for the weakest genuine cluster of a 300-state test).  When fewer genuine
+// above 1e-6 for the weakest genuine cluster of a 300-state test).  The lowest value is
+// always the ground state, with its own vector (the energy reported is that value).
+// When fewer genuine
 // values exist than asked for, the rest fill up in ascending order.  tol = 0 (full
 // reorthogonalisation, where a repeated value is a degenerate level): the nst lowest.
 static std::vector<int> genuine_ritz(const double *d, const double *u1, int m, int nst, double tol) {
@@ -816,7 +818,8 @@ static std::vector<int> genuine_ritz(const double *d, const double *u1, int m, i
             if (std::fabs(u1[j]) > std::fabs(u1[best])) best = j;
             ++j;
         }
-        if (j - i > 1 || std::fabs(u1[i]) >= 1e-10) {
+        if (i == 0) best = 0;   // the ground state: the lowest value and its vector, as ever
+        if (i == 0 || j - i > 1 || std::fabs(u1[i]) >= 1e-10) {
             keep.push_back(best);
             used[best] = 1;
         }

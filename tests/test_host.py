@@ -41,16 +41,21 @@ def test_pyhub_import_paths_alias_the_engine():
 
 def test_genuine_ritz_skips_ghosts_and_spurious_values():
     """Ritz values of the three-term recurrence with FP32 vectors (from a 200-step run
-    on 300 states): the copies of a converged level (1e-8 apart) count once, through
-    the member of largest first component; a value whose eigenvector has no weight on
-    the start vector (a copy still forming) is dropped; the FP64 tolerance keeps levels
-    1e-6 apart distinct."""
+    on 300 states, below a ground level added): the copies of a converged level (1e-8
+    apart) count once, through the member of largest first component; a value whose
+    eigenvector has no weight on the start vector (a copy still forming) is dropped;
+    the lowest value is always the ground state; the FP64 tolerance keeps levels 1e-6
+    apart distinct."""
     import numpy as np
     from pyhub_amd.core.core_solver import _GHOST_TOL, _genuine_ritz
-    val = np.array([-7.2696915827, -7.2696915683, -7.2696915654, -6.5907965758, -6.5906695780,
-                    -6.5906695309, -6.3282888371, -6.3269646192])
-    u1 = np.array([3.5e-5, 5.7e-3, 5.8e-3, 8.6e-19, 2.9e-4, 4.9e-2, 1.1e-19, 1.0e-3])
-    assert list(_genuine_ritz(val, u1, _GHOST_TOL["plain32"])) == [2, 5, 7]
+    val = np.array([-7.5, -7.2696915827, -7.2696915683, -7.2696915654, -6.5907965758,
+                    -6.5906695780, -6.5906695309, -6.3282888371, -6.3269646192])
+    u1 = np.array([0.3, 3.5e-5, 5.7e-3, 5.8e-3, 8.6e-19, 2.9e-4, 4.9e-2, 1.1e-19, 1.0e-3])
+    assert list(_genuine_ritz(val, u1, _GHOST_TOL["plain32"])) == [0, 3, 6, 8]
+    # a ghost cluster at the bottom stays the ground state through its lowest member
+    assert list(_genuine_ritz(val[1:], u1[1:], _GHOST_TOL["plain32"])) == [0, 5, 7]
     v64 = np.array([-7.2696915682, -7.2696905682, -6.5906695363])
     assert list(_genuine_ritz(v64, np.array([1e-2, 1e-3, 1e-12]), _GHOST_TOL["plain64"])) == [0, 1]
     assert list(_genuine_ritz(np.array([1.0]), np.array([1.0]), 1e-8)) == [0]
+    # the lowest value stays the ground state, even alone with no start-vector weight
+    assert list(_genuine_ritz(np.array([-2.0, -1.0]), np.array([1e-16, 0.5]), 1e-8)) == [0, 1]
