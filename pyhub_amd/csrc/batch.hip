@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -32,7 +33,9 @@ namespace {
 constexpr int BT = 1024;          // threads per workgroup
 constexpr int MAXN = 16;          // sites per cluster in the batched path
 constexpr int MAXSTEPS = 256;     // Lanczos steps per task
-constexpr int64_t BIG_GS = 100000;  // ground states of at least this many states: full device
+// ground states of at least this many states: full device (a sweep of the 256-cluster
+// workload: 0.905 s at 100,000, 0.880 s at 40,000 and 60,000, 0.91 s at 25,000)
+constexpr int64_t BIG_GS = 50000;
 // dynamic LDS of the persistent workgroup (one per CU) for the H*v's row and column
 // staging: 19,968 FP64 or 39,936 FP32 elements, at least one row or column of a
 // 16-site sector (12,870)
@@ -657,6 +660,9 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
         if (rc) { cleanup(); return rc; }
     }
     int64_t dmax = 1, psi_total = 0;
+    int64_t big_gs = BIG_GS;   // PHUB_BATCH_BIG_GS overrides the threshold
+    if (const char *e = std::getenv("PHUB_BATCH_BIG_GS"))
+        if (std::atoll(e) > 0) big_gs = std::atoll(e);
     for (int i = 0; i < ncl; ++i) {
         const phub_cluster &q = cls[i];
         ClusterDev &h = hc[i];
@@ -696,7 +702,7 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
         }
         h.psi_off = psi_total;
         psi_total += std::max<int64_t>(h.sec[0].nu * h.sec[0].nd, 1);
-        if (h.sec[0].nu * h.sec[0].nd >= BIG_GS) {
+        if (h.sec[0].nu * h.sec[0].nd >= big_gs) {
             phub_hubbard_desc desc{q.n_sites, q.n_up, q.n_dn, q.t, q.u, nullptr};
             phub_op *op = nullptr;
             int rc = phub_op_hubbard(c, &desc, &op);
@@ -714,7 +720,7 @@ extern "C" int phub_batch_hubbard(phub_ctx *c, const phub_cluster *cls, int ncl,
     });
     std::vector<int> big;   // clusters whose ground state runs on the full device
     for (int i : order) {
-        if (hc[i].sec[0].nu * hc[i].sec[0].nd >= BIG_GS) big.push_back(i);
+        if (hc[i].sec[0].nu * hc[i].sec[0].nd >= big_gs) big.push_back(i);
         else tasks.push_back(Task{i, 0, 0, 0, 0, 0});
     }
     const int ngs = (int)tasks.size();
