@@ -305,8 +305,10 @@ int phub_batch_hubbard(phub_ctx *ctx, const phub_cluster *clusters, int n_cluste
                        const phub_batch_opts *opts, double *e0, double *gf_alpha,
                        double *gf_beta, double *gf_norm2);
 
-/* Evaluate G(omega) = norm2 / (z - a0 - b0^2/(z - a1 - ...)) on a grid, on the
- * device, for n_chains chains: z = sign*(omega + i eta) + shift[c].
+/* Evaluate G(omega) = sign * norm2 / (z - a0 - b0^2/(z - a1 - ...)) on a grid, on
+ * the device, for n_chains chains: z = sign*(omega + i eta) + shift[c].  With shift
+ * E0, sign +1 gives a particle chain's sum of w_n / (omega + i eta - (E_n - E0)) and
+ * sign -1 a hole chain's sum of w_n / (omega + i eta + (E_n - E0)).
  * out: host complex128 interleaved, n_chains * n_omega. */
 int phub_cf_eval(phub_ctx *ctx, int n_chains, int nsteps, const double *alpha,
                  const double *beta, const double *norm2, const double *shift,

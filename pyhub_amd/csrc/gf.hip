@@ -71,7 +71,7 @@ extern "C" int phub_excite(phub_ctx *c, const phub_op *src, const phub_op *dst, 
     return PHUB_OK;
 }
 
-// G(z) = norm2 / (z - a0 - b0^2 / (z - a1 - b1^2 / ...)), z = sign*(w + i eta) + shift
+// G = sign * norm2 / (z - a0 - b0^2 / (z - a1 - b1^2 / ...)), z = sign*(w + i eta) + shift
 __global__ void k_cf_eval(int nch, int m, const double *__restrict__ al,
                           const double *__restrict__ be, const double *__restrict__ n2,
                           const double *__restrict__ shift, const double *__restrict__ sgn,
