@@ -7,7 +7,8 @@ oracle (hprod_hubbard.f90:17-118 restated, bit-exact against the reference):
   row: forced by PHUB_UP_ROWS=1 on small sectors, and on a sector whose up
   dimension (11,440) is past the two-row limit, over every down-hop tile;
 - FP32 with three rows per workgroup where they fit (dim_up <= 13,397; every FP32
-  H*v of the other tests), and with two: forced by PHUB_UP_ROWS=2.
+  H*v of the other tests), and with two: forced by PHUB_UP_ROWS=2;
+- 31 and 32 sites (every bit of the 32-bit state masks).
 
 Tolerances: FP64 within 1e-13 of the oracle relative to max|w|; FP32 within 2e-5."""
 import numpy as np
@@ -121,3 +122,37 @@ def test_fp64_one_row_large_up_sector_all_tiles(ctx, monkeypatch, tile, lattice)
     op = HubbardOp(ctx, n, nu, nd, T, U)
     _check(op, O.HubbardModel(n, nu, nd, T, U), rng)
 
+
+
+@pytest.mark.parametrize("n,nu,nd,seed", [(31, 2, 1, 41), (32, 2, 1, 42), (32, 4, 1, 43)])
+def test_32_bit_masks(ctx, n, nu, nd, seed):
+    """Up to 32 sites, every bit of the 32-bit state masks: 2 up and 1 down electron
+    (the two-kernel path) and 4 up and 1 down (35,960 up states, rows too long for LDS
+    in either precision: the one-lane-per-state kernel), random hoppings, against the
+    oracle (pinned at 32 sites by second-quantised spectra)."""
+    from pyhub_amd.engine import HubbardOp
+    rng = np.random.default_rng(seed)
+    T = rng.normal(size=(n, n)) * 0.3
+    T = (T + T.T) / 2
+    U = rng.uniform(2, 6, n)
+    op = HubbardOp(ctx, n, nu, nd, T, U)
+    _check(op, O.HubbardModel(n, nu, nd, T, U), rng)
+
+
+def test_32_sites_ground_state(ctx):
+    """E0 of 1 up and 1 down electron on 32 sites: the lowest eigenvalue of
+    T (x) 1 + 1 (x) T + U_i delta_ij, through the FP64 Lanczos."""
+    from pyhub_amd import _lib as L
+    from pyhub_amd.engine import HubbardOp
+    rng = np.random.default_rng(44)
+    n = 32
+    T = rng.normal(size=(n, n)) * 0.3
+    T = (T + T.T) / 2
+    U = rng.uniform(2, 6, n)
+    Hk = np.kron(T, np.eye(n)) + np.kron(np.eye(n), T) + np.diag(
+        [U[i] if i == j else 0.0 for i in range(n) for j in range(n)])
+    e_ref = np.linalg.eigvalsh(Hk)[0]
+    op = HubbardOp(ctx, n, 1, 1, T, U)
+    r = op.lanczos(mode=L.LCZ_FULL64, max_steps=300, tol=1e-12)
+    print(f"32 sites (1,1): E0 {r.e0:.13f} exact {e_ref:.13f}", flush=True)
+    assert abs(r.e0 - e_ref) < 1e-10 * max(1.0, abs(e_ref))

@@ -38,6 +38,79 @@ def test_hubbard_restatement_bit_exact_vs_reference(n, nu, nd, seed):
 
 
 @pytest.mark.skipif(not O.ref_available(), reason="reference HPROD build (oracle/_ref) absent")
+def test_hubbard_restatement_bit_exact_vs_reference_31_sites():
+    """The largest site count the reference's HPROD handles: 31 sites, 2 up and 1 down
+    (14,415 states), bit-exact.  At 32 sites its spectrum departs from second
+    quantisation (the oracle's does not: test_hubbard_oracle_32_sites_spectrum)."""
+    rng = np.random.default_rng(31)
+    n = 31
+    T = rng.normal(size=(n, n)) * 0.3
+    T = (T + T.T) / 2
+    m = O.HubbardModel(n, 2, 1, T, rng.uniform(2, 6, n))
+    v = rng.normal(size=m.dim)
+    assert np.array_equal(m.hv(v), m.hv_ref(v))
+
+
+def _second_quantised_2up_1dn(T, U):
+    """H of 2 up and 1 down electron on n sites, built directly as a sparse matrix in
+    an order of its own (up pairs i < j, Jordan-Wigner sign of the occupied up site
+    between the two ends of a hop); any consistent fermion convention gives the same
+    spectrum."""
+    import itertools
+    import scipy.sparse as sp
+    n = len(U)
+    ups = list(itertools.combinations(range(n), 2))
+    pos = {u: k for k, u in enumerate(ups)}
+    nu = len(ups)
+    rows, cols, vals = [], [], []
+    for a, (p, q) in enumerate(ups):
+        for d in range(n):
+            s = d * nu + a
+            rows.append(s)
+            cols.append(s)
+            vals.append(T[p, p] + T[q, q] + T[d, d] + (U[p] if d == p else 0.0)
+                        + (U[q] if d == q else 0.0))
+            for j, other in ((p, q), (q, p)):
+                for i in range(n):
+                    if i == p or i == q:
+                        continue
+                    sg = -1.0 if min(i, j) < other < max(i, j) else 1.0
+                    rows.append(d * nu + pos[tuple(sorted((i, other)))])
+                    cols.append(s)
+                    vals.append(sg * T[i, j])
+            for i in range(n):
+                if i != d:
+                    rows.append(i * nu + a)
+                    cols.append(s)
+                    vals.append(T[i, d])
+    return sp.csr_matrix((vals, (rows, cols)), shape=(nu * n, nu * n))
+
+
+def test_hubbard_oracle_32_sites_spectrum():
+    """32 sites (every bit of a 32-bit mask): the oracle's H on 2 up and 1 down electron
+    (15,872 states) has the six lowest eigenvalues of the directly built second-quantised
+    H to 1e-11, and on 1 up and 1 down electron the whole spectrum of
+    T (x) 1 + 1 (x) T + U_i delta_ij."""
+    import scipy.sparse.linalg as sla
+    rng = np.random.default_rng(7)
+    n = 32
+    T = rng.normal(size=(n, n)) * 0.3
+    T = (T + T.T) / 2
+    U = rng.uniform(2, 6, n)
+    m = O.HubbardModel(n, 2, 1, T, U)
+    D = m.dim
+    op = sla.LinearOperator((D, D), dtype=float,
+                            matvec=lambda v: m.hv(np.asarray(v, float).ravel(), fast=True, nthreads=8))
+    ev_or = np.sort(sla.eigsh(op, k=6, which="SA", tol=1e-12)[0])
+    ev_sq = np.sort(sla.eigsh(_second_quantised_2up_1dn(T, U), k=6, which="SA", tol=1e-12)[0])
+    assert np.max(np.abs(ev_or - ev_sq)) < 1e-11, (ev_or, ev_sq)
+    m1 = O.HubbardModel(n, 1, 1, T, U)
+    Hk = np.kron(T, np.eye(n)) + np.kron(np.eye(n), T) + np.diag(
+        [U[i] if i == j else 0.0 for i in range(n) for j in range(n)])
+    assert np.max(np.abs(np.linalg.eigvalsh(m1.dense()) - np.linalg.eigvalsh(Hk))) < 1e-12
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference HPROD build (oracle/_ref) absent")
 @pytest.mark.parametrize("n,seed", [(4, 0), (6, 1), (8, 2), (10, 3)])
 def test_heisenberg_restatement_bit_exact_vs_reference(n, seed):
     rng = np.random.default_rng(seed)
