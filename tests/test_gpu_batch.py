@@ -103,8 +103,9 @@ def test_batched_edge_sectors_and_site_dependent_u(ctx):
     (E0 and the Lehmann sum of G_ii) and closed forms: one site (E0 = 2 eps + U), the
     Hubbard dimer (U/2 - sqrt(U^2/4 + 4 t^2)), no up electrons (free down fermions),
     a full up species (each down electron sees eps_i + U_i: per-site U read by the
-    right site), and site-dependent U on 5 and 7 sites (the per-site branch of the
-    diagonal).  Chains of 120 steps exhaust the sectors up to 120 states; eta = 0.5."""
+    right site), site-dependent U on 5 and 7 sites (the per-site branch of the
+    diagonal), and 16 sites, the batched path's maximum.  Chains of 120 steps exhaust
+    the sectors up to 120 states; eta = 0.5."""
     from oracle import oracle as O
     from pyhub_amd.batch import Cluster, gii_omega, run_batch
     rng = np.random.default_rng(404)
@@ -127,6 +128,9 @@ def test_batched_edge_sectors_and_site_dependent_u(ctx):
     ]
     closed = [2 * -0.3 + 5.0, O.dimer_e0(1.0, 4.0), O.free_fermion_e0(T6, 0, 3),
               float(np.trace(T7)) + O.free_fermion_e0(T7 + np.diag(clusters[3].U), 0, 2), None, None]
+    # the largest cluster the batched path takes (16 sites), one electron of each spin
+    clusters.append(Cluster(16, 1, 1, rand_T(16, 0.5), rng.uniform(1, 9, 16)))
+    closed.append(None)
     e0, ga, gb, gn = run_batch(clusters, gs_steps=256, gf_steps=120, seed=5, gs_tol=1e-10)
     omega = np.linspace(-20, 20, 401)
     eta = 0.5
