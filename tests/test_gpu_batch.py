@@ -162,3 +162,37 @@ def test_batched_edge_sectors_and_site_dependent_u(ctx):
     print("largest G_ii errors (rel. to max|G|, eta=0.5):",
           [(f"{e:.2e}", c, i, s) for e, c, i, s in worst[:5]], flush=True)
     assert worst[0][0] < 1e-4, worst[0]   # 3.8e-6 measured
+
+
+def test_batched_argument_errors(ctx):
+    """The batched entry point rejects, before any launch, what it cannot take, with the
+    reference-style error codes: more than 16 sites (unsupported), particle numbers out
+    of [0, n], a non-symmetric or non-finite t matrix, non-finite U, and step counts
+    outside [2, 256] (argument errors).  A good batch on the same context still runs."""
+    from pyhub_amd import _lib as L
+    from pyhub_amd.batch import Cluster, run_batch
+    from oracle import oracle as O
+    T6 = O.ring_T(6)
+    good = Cluster(6, 3, 3, T6, np.full(6, 4.0))
+    with pytest.raises(L.PhubError) as ei:
+        run_batch([good, Cluster(17, 1, 1, O.ring_T(17), np.full(17, 4.0))], ctx=ctx)
+    assert ei.value.code == L.ERR_UNSUPPORTED
+    for bad in (Cluster(6, 7, 3, T6, np.full(6, 4.0)), Cluster(6, 3, -1, T6, np.full(6, 4.0))):
+        with pytest.raises(ValueError):
+            run_batch([good, bad], ctx=ctx)
+    Tn = T6.copy()
+    Tn[0, 1] += 0.1
+    with pytest.raises(ValueError, match="symmetric"):
+        run_batch([Cluster(6, 3, 3, Tn, np.full(6, 4.0))], ctx=ctx)
+    Tf = T6.copy()
+    Tf[2, 2] = np.nan
+    with pytest.raises(ValueError, match="non-finite"):
+        run_batch([Cluster(6, 3, 3, Tf, np.full(6, 4.0))], ctx=ctx)
+    with pytest.raises(ValueError, match="non-finite"):
+        run_batch([Cluster(6, 3, 3, T6, np.array([4.0, 4.0, np.inf, 4.0, 4.0, 4.0]))], ctx=ctx)
+    for gs, gf in ((1, 120), (256, 0), (300, 120), (256, 257)):
+        with pytest.raises(ValueError, match="steps"):
+            run_batch([good], gs_steps=gs, gf_steps=gf, ctx=ctx)
+    e0, ga, gb, gn = run_batch([good], gs_steps=200, gf_steps=40, ctx=ctx)
+    m = O.HubbardModel(6, 3, 3, T6, 4.0)
+    assert abs(e0[0] - np.linalg.eigvalsh(m.dense())[0]) < 1e-9
