@@ -166,7 +166,7 @@ def test_batched_edge_sectors_and_site_dependent_u(ctx):
 
 def test_batched_argument_errors(ctx):
     """The batched entry point rejects, before any launch, what it cannot take, with the
-    reference-style error codes: more than 16 sites (unsupported), particle numbers out
+    library's error codes: more than 16 sites (unsupported), particle numbers out
     of [0, n], a non-symmetric or non-finite t matrix, non-finite U, and step counts
     outside [2, 256] (argument errors).  A good batch on the same context still runs."""
     from pyhub_amd import _lib as L
